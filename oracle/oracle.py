@@ -1,0 +1,235 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+The product package never imports this module.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+ANGSTROM_TO_BOHR = 1.0 / 0.529177210903
+EV_TO_KCAL = 23.06054783061903
+
+STATUS = {0: "ok", 1: "invalid input", 2: "missing element", 3: "SCF not converged", 4: "unsupported"}
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("max_scf", ctypes.c_int),
+        ("e_tol", ctypes.c_double),
+        ("p_tol", ctypes.c_double),
+        ("use_diis", ctypes.c_int),
+        ("accelerator", ctypes.c_int),
+        ("adiis_switch", ctypes.c_double),
+        ("guess", ctypes.c_int),
+        ("stability", ctypes.c_int),
+        ("scf_memory_mb", ctypes.c_int),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("total_ev", ctypes.c_double),
+        ("electronic_ev", ctypes.c_double),
+        ("core_ev", ctypes.c_double),
+        ("hof_kcal", ctypes.c_double),
+        ("dipole", ctypes.c_double * 3),
+        ("dipole_debye", ctypes.c_double),
+        ("homo_ev", ctypes.c_double),
+        ("lumo_ev", ctypes.c_double),
+        ("last_change", ctypes.c_double),
+        ("scf_improvement_ev", ctypes.c_double),
+        ("iterations", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("scf_paths_tried", ctypes.c_int),
+        ("n_occ", ctypes.c_int),
+        ("nao", ctypes.c_int),
+        ("has_homo", ctypes.c_int),
+        ("has_lumo", ctypes.c_int),
+    ]
+
+
+def options(max_scf=200, e_tol=1e-8, p_tol=1e-7, use_diis=True, accelerator=2, adiis_switch=0.1, guess=0,
+            stability=1, scf_memory_mb=512):
+    """Mirror of the reference's Pm3Options defaults (src/scf.rs:259-287)."""
+    return Options(max_scf, e_tol, p_tol, int(use_diis), accelerator, adiis_switch, guess, stability, scf_memory_mb)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle library not built: run `make -C oracle`")
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_element.argtypes = [ctypes.c_int, dp]
+        L.orc_pair_integrals.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, dp, dp, dp]
+        L.orc_nao.argtypes = [ctypes.c_int, ip]
+        L.orc_hcore.argtypes = [ctypes.c_int, ip, dp, dp]
+        L.orc_fock.argtypes = [ctypes.c_int, ip, dp, dp, dp]
+        L.orc_fock_fourindex.argtypes = [ctypes.c_int, ip, dp, dp, dp]
+        L.orc_core_core.argtypes = [ctypes.c_int, ip, dp]
+        L.orc_core_core.restype = ctypes.c_double
+        L.orc_eigen.argtypes = [ctypes.c_int, dp, dp, dp]
+        L.orc_run.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.POINTER(Options), ctypes.POINTER(Result),
+                              dp, dp, dp, dp]
+        L.orc_fixed_density_gradient.argtypes = [ctypes.c_int, ip, dp, dp, dp]
+        L.orc_gradient.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.POINTER(Options),
+                                   ctypes.POINTER(Result), dp, dp]
+        L.orc_run_batch.argtypes = [ctypes.c_int, ip, ip, dp, dp, ctypes.POINTER(Options), ctypes.c_int, dp, dp, ip,
+                                    ip]
+        L.orc_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _mol(z, xyz_bohr):
+    z = np.ascontiguousarray(z, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+    return z, xyz
+
+
+def element(zz):
+    out = np.zeros(32)
+    st = lib().orc_element(zz, _d(out))
+    if st:
+        raise KeyError(zz)
+    keys = ["n_orb", "dd", "qq", "rho0", "rho1", "rho2", "po1", "po2", "po3", "po7", "po9", "ddp2", "ddp3", "e_isol",
+            "eheat_ev", "tore", "mass", "hsp"]
+    return dict(zip(keys, out[: len(keys)]))
+
+
+def pair_integrals(za, zb, d_bohr):
+    d = np.ascontiguousarray(d_bohr, dtype=np.float64)
+    w = np.zeros(100)
+    e1b = np.zeros(16)
+    e2a = np.zeros(16)
+    s = np.zeros(16)
+    st = lib().orc_pair_integrals(za, zb, _d(d), _d(w), _d(e1b), _d(e2a), _d(s))
+    if st:
+        raise RuntimeError(STATUS[st])
+    return w, e1b.reshape(4, 4), e2a.reshape(4, 4), s.reshape(4, 4)
+
+
+def nao(z):
+    z = np.ascontiguousarray(z, dtype=np.int32)
+    return lib().orc_nao(len(z), _i(z))
+
+
+def hcore(z, xyz_bohr):
+    z, xyz = _mol(z, xyz_bohr)
+    n = nao(z)
+    h = np.zeros((n, n))
+    st = lib().orc_hcore(len(z), _i(z), _d(xyz), _d(h))
+    if st:
+        raise RuntimeError(STATUS[st])
+    return h
+
+
+def fock(z, xyz_bohr, P, four_index=False):
+    z, xyz = _mol(z, xyz_bohr)
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    F = np.zeros_like(P)
+    fn = lib().orc_fock_fourindex if four_index else lib().orc_fock
+    st = fn(len(z), _i(z), _d(xyz), _d(P), _d(F))
+    if st:
+        raise RuntimeError(STATUS[st])
+    return F
+
+
+def core_core(z, xyz_bohr):
+    z, xyz = _mol(z, xyz_bohr)
+    return lib().orc_core_core(len(z), _i(z), _d(xyz))
+
+
+def eigen(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    w = np.zeros(n)
+    V = np.zeros((n, n))
+    lib().orc_eigen(n, _d(A), _d(w), _d(V))
+    return w, V
+
+
+def run(z, xyz_bohr, charge=0.0, opts=None, want_arrays=True):
+    """RHF PM3 single point; returns (status, dict)."""
+    z, xyz = _mol(z, xyz_bohr)
+    opts = opts or options()
+    n = nao(z)
+    res = Result()
+    P = np.zeros((n, n)) if want_arrays else None
+    eps = np.zeros(n) if want_arrays else None
+    C = np.zeros((n, n)) if want_arrays else None
+    q = np.zeros(len(z)) if want_arrays else None
+    nullp = ctypes.POINTER(ctypes.c_double)()
+    st = lib().orc_run(len(z), _i(z), _d(xyz), float(charge), ctypes.byref(opts), ctypes.byref(res),
+                       _d(P) if want_arrays else nullp, _d(eps) if want_arrays else nullp,
+                       _d(C) if want_arrays else nullp, _d(q) if want_arrays else nullp)
+    out = {f: getattr(res, f) for f, _ in Result._fields_}
+    out["dipole"] = np.array(res.dipole[:])
+    if want_arrays:
+        out.update(density=P, mo_energies=eps, mo_coeff=C, charges=q)
+    return st, out
+
+
+def fixed_density_gradient(z, xyz_bohr, P):
+    z, xyz = _mol(z, xyz_bohr)
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    g = np.zeros((len(z), 3))
+    st = lib().orc_fixed_density_gradient(len(z), _i(z), _d(xyz), _d(P), _d(g))
+    if st:
+        raise RuntimeError(STATUS[st])
+    return g
+
+
+def gradient(z, xyz_bohr, charge=0.0, opts=None):
+    z, xyz = _mol(z, xyz_bohr)
+    opts = opts or options()
+    n = nao(z)
+    res = Result()
+    g = np.zeros((len(z), 3))
+    P = np.zeros((n, n))
+    st = lib().orc_gradient(len(z), _i(z), _d(xyz), float(charge), ctypes.byref(opts), ctypes.byref(res), _d(g), _d(P))
+    out = {f: getattr(res, f) for f, _ in Result._fields_}
+    out.update(gradient=g, density=P)
+    return st, out
+
+
+def run_batch(mols, opts=None, nthreads=0):
+    """Independent single points over host threads; mols = list of (z, xyz_bohr)."""
+    opts = opts or options()
+    ptr = np.zeros(len(mols) + 1, dtype=np.int32)
+    for k, (z, _) in enumerate(mols):
+        ptr[k + 1] = ptr[k] + len(z)
+    zz = np.ascontiguousarray(np.concatenate([np.asarray(z, dtype=np.int32) for z, _ in mols]))
+    xyz = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3) for _, x in mols]))
+    k = len(mols)
+    tot = np.zeros(k)
+    hof = np.zeros(k)
+    it = np.zeros(k, dtype=np.int32)
+    st = np.zeros(k, dtype=np.int32)
+    nullp = ctypes.POINTER(ctypes.c_double)()
+    lib().orc_run_batch(k, _i(ptr), _i(zz), _d(xyz), nullp, ctypes.byref(opts), int(nthreads), _d(tot), _d(hof),
+                        _i(it), _i(st))
+    return tot, hof, it, st
+
+
+def num_threads():
+    return lib().orc_num_threads()
