@@ -1,0 +1,133 @@
+/* C ABI of the MI355X PM3 engine (libpm3gpu.so).
+ *
+ * The entry points replace the reference's molecular RHF boundary:
+ *   pm3_run        <- run_pm3(&Molecule, &Pm3Parameters, &Pm3Options) -> Result<Pm3Result>
+ *                     (src/scf.rs:450-454, re-exported at src/lib.rs:108)
+ *   pm3_gradient   <- closed_form_gradient(&Molecule, &Pm3Parameters, &Pm3Options)
+ *                     -> Result<GradientResult> (src/gradient.rs:201-205, src/lib.rs:72)
+ *   pm3_params_standard <- Pm3Parameters::standard() (src/params.rs:135-137)
+ *   pm3_run_batch  -- many independent molecules in one persistent launch (no reference
+ *                     counterpart: the reference solves one molecule per call).
+ *
+ * Units follow the reference (src/system.rs:21-26): positions in Bohr, energies in eV, heat
+ * of formation in kcal/mol, gradient in eV/Bohr, dipole in Debye.  Matrices are n x n
+ * row-major doubles (n = pm3_query_nao), eigenvectors as columns, eigenvalues ascending
+ * (src/linalg.rs:217-263).  All buffers are caller-allocated host memory.
+ *
+ * Errors: every call returns a pm3_status whose values map one-to-one onto the reference's
+ * Pm3Error variants (src/error.rs:12-47); pm3_last_error_message() gives the detail text of the
+ * calling thread's last failure.  Non-convergence is an error (PM3_SCF_NOT_CONVERGED), never a
+ * result with converged = 0 (src/scf.rs:754-760); the result struct still carries the
+ * iteration count and last density change.
+ */
+#ifndef PM3_GPU_H
+#define PM3_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  PM3_OK = 0,
+  PM3_INVALID_INPUT = 1,       /* Pm3Error::InvalidInput */
+  PM3_MISSING_ELEMENT = 2,     /* Pm3Error::MissingElement(z) */
+  PM3_SCF_NOT_CONVERGED = 3,   /* Pm3Error::ScfNotConverged{iterations, error} */
+  PM3_UNSUPPORTED = 4,         /* open shells, d elements, n >= 4 overlaps: not in this engine */
+  PM3_RESOURCE_LIMIT = 5,      /* Pm3Error::ResourceLimit */
+  PM3_DEVICE_ERROR = 6         /* HIP runtime failure (no reference counterpart) */
+} pm3_status;
+
+typedef enum { PM3_ACCEL_NONE = 0, PM3_ACCEL_CDIIS = 1, PM3_ACCEL_ADIIS_CDIIS = 2 } pm3_accelerator;
+/* ScfGuess (src/scf.rs:133-158) plus the learned initial guess of this engine. */
+typedef enum { PM3_GUESS_AUTO = 0, PM3_GUESS_SAD = 1, PM3_GUESS_CORE = 2, PM3_GUESS_SYMMETRY_BROKEN = 3,
+               PM3_GUESS_LEARNED = 4 } pm3_guess;
+typedef enum { PM3_STABILITY_OFF = 0, PM3_STABILITY_AUTO = 1, PM3_STABILITY_ALWAYS = 2 } pm3_stability;
+
+/* Molecule (src/system.rs:28-40): z[i] and xyz_bohr[3*i..3*i+2] per atom. */
+typedef struct {
+  int32_t n_atoms;
+  const int32_t* z;
+  const double* xyz_bohr;
+  double charge;
+  int32_t multiplicity;
+} pm3_molecule;
+
+/* The fields of Pm3Options (src/scf.rs:160-287) that the molecular RHF path reads. */
+typedef struct {
+  int32_t max_scf;        /* default 200 */
+  double e_tol;           /* default 1e-8 eV */
+  double p_tol;           /* default 1e-7 (RMS density change) */
+  int32_t use_diis;       /* default 1 */
+  int32_t accelerator;    /* pm3_accelerator, default ADIIS_CDIIS */
+  double adiis_switch;    /* default 0.1 */
+  int32_t guess;          /* pm3_guess, default AUTO (= SAD) */
+  int32_t stability;      /* pm3_stability, default AUTO */
+  int32_t scf_memory_mb;  /* default 512: sets the DIIS depth (src/scf.rs:309-327) */
+} pm3_options;
+
+/* Scalars of Pm3Result (src/scf.rs:352-396). */
+typedef struct {
+  double total_ev, electronic_ev, core_ev, heat_of_formation_kcal;
+  double dipole_debye[3], dipole_magnitude;
+  double homo_ev, lumo_ev;
+  double last_change;          /* RMS density change of the last iteration */
+  double scf_improvement_ev;
+  int32_t iterations, converged, scf_paths_tried, n_occ, nao, has_homo, has_lumo;
+} pm3_result;
+
+typedef struct pm3_params pm3_params; /* opaque, immutable once built */
+
+/* Pm3Parameters::standard(): the embedded PM3 table for the supported s/p elements
+ * (H, Li, Be, B, C, N, O, F, Na, Mg, Al, Si, P, S, Cl).  Returns NULL on failure. */
+const pm3_params* pm3_params_standard(void);
+
+/* Fills *out with the reference Pm3Options::default() values (src/scf.rs:259-287). */
+void pm3_options_default(pm3_options* out);
+
+/* Number of AOs of the molecule (Basis::build, src/basis.rs:39-62); -1 for an unknown element. */
+int32_t pm3_query_nao(const pm3_molecule* mol, const pm3_params* params);
+
+/* run_pm3 (src/scf.rs:450-848).  Optional outputs may be NULL: density[n*n], mo_energies[n],
+ * mo_coeff[n*n], charges[n_atoms]. */
+pm3_status pm3_run(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts, pm3_result* result,
+                   double* density, double* mo_energies, double* mo_coeff, double* charges);
+
+/* closed_form_gradient (src/gradient.rs:201-276): SCF plus the Hellmann-Feynman gradient at the
+ * converged density.  gradient[3*n_atoms] in eV/Bohr; density may be NULL. */
+pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                        pm3_result* result, double* gradient, double* density);
+
+/* Batched solve of many independent closed-shell molecules in one persistent launch.  Atoms of
+ * molecule k are [atom_ptr[k], atom_ptr[k+1]) of z / xyz_bohr; charge may be NULL (all neutral).
+ * Per molecule: status[k] (pm3_status), results[k].  want_gradient != 0 also fills
+ * gradient[3 * atom_ptr[k] ...] (may be NULL otherwise).  Returns PM3_OK when the launch
+ * itself succeeded; per-molecule failures are reported in status[]. */
+pm3_status pm3_run_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
+                         const double* charge, const pm3_params* params, const pm3_options* opts,
+                         int32_t want_gradient, pm3_status* status, pm3_result* results, double* gradient);
+
+/* Device-resident batch: pm3_batch_create uploads the molecules and allocates the workgroup
+ * scratch once; pm3_batch_solve is the persistent launch alone (inputs already in HBM);
+ * pm3_batch_results copies back per-molecule status, scalars and optionally the Mulliken charges
+ * (charges[atom_ptr[k] + i]) and densities (concatenated nao_k^2 blocks).  No reference
+ * counterpart: this is the batched entry the reference lacks; pm3_run_batch is
+ * create + solve + results + destroy. */
+typedef struct pm3_batch pm3_batch;
+enum { PM3_BATCH_WANT_DENSITY = 1 };
+pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
+                            const double* charge, const pm3_params* params, const pm3_options* opts, int32_t flags,
+                            pm3_batch** out);
+pm3_status pm3_batch_solve(pm3_batch* batch);
+pm3_status pm3_batch_results(pm3_batch* batch, pm3_status* status, pm3_result* results, double* charges,
+                             double* density);
+void pm3_batch_destroy(pm3_batch* batch);
+
+/* Detail text of the calling thread's last error ("" if none). */
+const char* pm3_last_error_message(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PM3_GPU_H */

@@ -1,0 +1,35 @@
+/* Kernel-level entry points of libpm3gpu.so: one stage of the device pipeline at a time, so each
+ * kernel can be checked against the CPU oracle on its own.  Same conventions as pm3_gpu.h
+ * (Bohr, eV, n x n row-major in the caller's AO order unless stated).  Each runs the same device
+ * code the SCF kernel runs, on one workgroup. */
+#ifndef PM3_GPU_STAGES_H
+#define PM3_GPU_STAGES_H
+
+#include "pm3_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* H_core and the core-core energy: replaces build_core_limited's h_core
+ * (src/hamiltonian.rs:191-344) and core_core_energy (src/repulsion.rs:13-27). */
+pm3_status pm3_stage_hcore(const pm3_molecule* mol, const pm3_params* params, double* h_core, double* core_ev);
+
+/* F = H_core + G(P) for a given density: replaces build_fock (src/fock.rs:488-500). */
+pm3_status pm3_stage_fock(const pm3_molecule* mol, const pm3_params* params, const double* density, double* fock);
+
+/* Eigenpairs of a symmetric n x n matrix, ascending, eigenvectors as columns: replaces
+ * symmetric_eigen (src/linalg.rs:221-263).  n <= 190. */
+pm3_status pm3_stage_eigen(int32_t n, const double* a, double* w, double* v);
+
+/* Same, warm-started in an orthonormal basis c (columns): the form the SCF loop uses. */
+pm3_status pm3_stage_eigen_warm(int32_t n, const double* a, const double* c, double* w, double* v);
+
+/* C = A B for n x n matrices on the FP64 matrix cores: replaces Matrix::matmul
+ * (src/linalg.rs:65-122).  n <= 190. */
+pm3_status pm3_stage_gemm(int32_t n, const double* a, const double* b, double* c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PM3_GPU_STAGES_H */

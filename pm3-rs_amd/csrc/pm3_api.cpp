@@ -1,0 +1,364 @@
+// C ABI of libpm3gpu.so (declared in include/pm3_gpu.h).  Thin layer: validation and the error
+// surface of the reference (src/error.rs:12-47), option defaults (src/scf.rs:259-287), heat of
+// formation bookkeeping (src/scf.rs:745-752); all numerical work is on the device.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pm3_gpu.h"
+#include "pm3_host.hpp"
+
+using namespace pm3host;
+
+struct pm3_params {
+  Params p;
+};
+
+struct pm3_batch {
+  Batch* b = nullptr;
+  std::vector<int32_t> z;
+  std::vector<double> xyz;
+  std::vector<MolIn> mols;
+  const pm3_params* params = nullptr;
+  std::vector<MolResult> results;
+  bool fetched = false;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+pm3_status fail(pm3_status s, const std::string& msg) {
+  g_last_error = msg;
+  return s;
+}
+
+const char* status_text(int s) {
+  switch (s) {
+    case PM3_OK: return "ok";
+    case PM3_INVALID_INPUT: return "invalid input";
+    case PM3_MISSING_ELEMENT: return "missing element";
+    case PM3_SCF_NOT_CONVERGED: return "SCF not converged";
+    case PM3_UNSUPPORTED: return "unsupported";
+    case PM3_RESOURCE_LIMIT: return "resource limit";
+    default: return "device error";
+  }
+}
+
+bool engine_options(const pm3_options* o, EngineOptions& e, std::string& why) {
+  pm3_options d;
+  pm3_options_default(&d);
+  if (!o) o = &d;
+  if (o->max_scf < 1) { why = "max_scf must be >= 1"; return false; }
+  if (!(o->e_tol >= 0.0) || !(o->p_tol >= 0.0)) { why = "tolerances must be >= 0"; return false; }
+  if (o->accelerator < 0 || o->accelerator > 2) { why = "unknown accelerator"; return false; }
+  if (o->guess < 0 || o->guess > 4) { why = "unknown guess"; return false; }
+  if (o->stability < 0 || o->stability > 2) { why = "unknown stability mode"; return false; }
+  e.max_scf = o->max_scf;
+  e.e_tol = o->e_tol;
+  e.p_tol = o->p_tol;
+  e.accel = o->use_diis ? o->accelerator : 0;
+  e.adiis_switch = o->adiis_switch;
+  e.guess = (o->guess == PM3_GUESS_AUTO || o->guess == PM3_GUESS_LEARNED) ? 1 : o->guess;
+  e.stability = o->stability;
+  e.scf_memory_mb = o->scf_memory_mb;
+  return true;
+}
+
+void fill_result(const MolResult& r, pm3_result* out) {
+  std::memset(out, 0, sizeof(*out));
+  out->electronic_ev = r.e_elec;
+  out->core_ev = r.e_core;
+  out->total_ev = r.e_elec + r.e_core;
+  out->heat_of_formation_kcal = r.hof_kcal;
+  for (int k = 0; k < 3; ++k) out->dipole_debye[k] = r.dipole[k];
+  out->dipole_magnitude = std::sqrt(r.dipole[0] * r.dipole[0] + r.dipole[1] * r.dipole[1] + r.dipole[2] * r.dipole[2]);
+  out->homo_ev = r.homo;
+  out->lumo_ev = r.lumo;
+  out->has_homo = r.nocc >= 1;
+  out->has_lumo = r.nocc < r.nao;
+  out->last_change = r.last_change;
+  out->scf_improvement_ev = r.improvement;
+  out->iterations = r.iterations;
+  out->converged = r.converged;
+  out->scf_paths_tried = r.paths;
+  out->n_occ = r.nocc;
+  out->nao = r.nao;
+}
+
+pm3_status check_molecule(const pm3_molecule* mol) {
+  if (!mol || mol->n_atoms < 1 || !mol->z || !mol->xyz_bohr) return fail(PM3_INVALID_INPUT, "empty molecule");
+  if (mol->multiplicity < 1) return fail(PM3_INVALID_INPUT, "multiplicity must be >= 1");
+  if (mol->multiplicity != 1) return fail(PM3_UNSUPPORTED, "open-shell (UHF) molecules are not handled by this engine");
+  for (int i = 0; i < 3 * mol->n_atoms; ++i)
+    if (!std::isfinite(mol->xyz_bohr[i])) return fail(PM3_INVALID_INPUT, "non-finite coordinate");
+  return PM3_OK;
+}
+
+pm3_status molecule_status(int st, const MolResult& r) {
+  if (st == PM3_OK) return PM3_OK;
+  if (st == PM3_SCF_NOT_CONVERGED)
+    return fail(PM3_SCF_NOT_CONVERGED, "SCF did not converge in " + std::to_string(r.iterations) +
+                                           " iterations (last RMS density change " + std::to_string(r.last_change) + ")");
+  return fail((pm3_status)st, status_text(st));
+}
+
+}  // namespace
+
+extern "C" {
+
+const pm3_params* pm3_params_standard(void) {
+  static pm3_params* g = nullptr;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    pm3_params* p = new pm3_params();
+    if (build_params(p->p)) g = p;
+    else delete p;
+  });
+  if (!g) g_last_error = "failed to build the PM3 parameter table";
+  return g;
+}
+
+void pm3_options_default(pm3_options* out) {
+  out->max_scf = 200;
+  out->e_tol = 1.0e-8;
+  out->p_tol = 1.0e-7;
+  out->use_diis = 1;
+  out->accelerator = PM3_ACCEL_ADIIS_CDIIS;
+  out->adiis_switch = 0.1;
+  out->guess = PM3_GUESS_AUTO;
+  out->stability = PM3_STABILITY_AUTO;
+  out->scf_memory_mb = 512;
+}
+
+int32_t pm3_query_nao(const pm3_molecule* mol, const pm3_params* params) {
+  if (!mol || !params) return -1;
+  int n = 0;
+  for (int i = 0; i < mol->n_atoms; ++i) {
+    const int z = mol->z[i];
+    if (z < 0 || z >= 128 || params->p.slot_of_z[z] < 0) return -1;
+    n += params->p.table.el[params->p.slot_of_z[z]].norb;
+  }
+  return n;
+}
+
+pm3_status pm3_run(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts, pm3_result* result,
+                   double* density, double* mo_energies, double* mo_coeff, double* charges) {
+  g_last_error.clear();
+  if (!params) return fail(PM3_INVALID_INPUT, "null parameter set");
+  pm3_status s = check_molecule(mol);
+  if (s != PM3_OK) return s;
+  EngineOptions eo;
+  std::string why;
+  if (!engine_options(opts, eo, why)) return fail(PM3_INVALID_INPUT, why);
+  std::vector<MolIn> mols{MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge}};
+  RunRequest req;
+  req.want_density = density != nullptr;
+  req.want_orbitals = mo_energies != nullptr || mo_coeff != nullptr;
+  std::vector<MolResult> out;
+  std::string err;
+  if (!engine_run(params->p, mols, eo, req, out, err)) return fail(PM3_DEVICE_ERROR, err);
+  const MolResult& r = out[0];
+  if (result) fill_result(r, result);
+  if (r.status == PM3_OK || r.status == PM3_SCF_NOT_CONVERGED) {
+    const size_t n2 = (size_t)r.nao * r.nao;
+    if (density && r.density.size() == n2) std::memcpy(density, r.density.data(), n2 * sizeof(double));
+    if (mo_coeff && r.mo_coeff.size() == n2) std::memcpy(mo_coeff, r.mo_coeff.data(), n2 * sizeof(double));
+    if (mo_energies && (int)r.mo_energies.size() == r.nao) std::memcpy(mo_energies, r.mo_energies.data(), r.nao * sizeof(double));
+    if (charges && (int)r.charges.size() == mol->n_atoms) std::memcpy(charges, r.charges.data(), mol->n_atoms * sizeof(double));
+  }
+  return molecule_status(r.status, r);
+}
+
+pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
+                            const double* charge, const pm3_params* params, const pm3_options* opts, int32_t flags,
+                            pm3_batch** out) {
+  g_last_error.clear();
+  if (!out) return fail(PM3_INVALID_INPUT, "null batch handle");
+  *out = nullptr;
+  if (!params || n_mol < 0 || (n_mol > 0 && (!atom_ptr || !z || !xyz_bohr))) return fail(PM3_INVALID_INPUT, "bad batch arguments");
+  EngineOptions eo;
+  std::string why;
+  if (!engine_options(opts, eo, why)) return fail(PM3_INVALID_INPUT, why);
+  pm3_batch* h = new pm3_batch();
+  const int na = n_mol > 0 ? atom_ptr[n_mol] : 0;
+  if (n_mol > 0 && atom_ptr[0] != 0) { delete h; return fail(PM3_INVALID_INPUT, "atom_ptr[0] must be 0"); }
+  for (int k = 0; k < n_mol; ++k)
+    if (atom_ptr[k + 1] < atom_ptr[k]) { delete h; return fail(PM3_INVALID_INPUT, "atom_ptr must be non-decreasing"); }
+  h->z.assign(z, z + na);
+  h->xyz.assign(xyz_bohr, xyz_bohr + 3 * (size_t)na);
+  h->params = params;
+  for (int k = 0; k < n_mol; ++k) {
+    const int a0 = atom_ptr[k];
+    h->mols.push_back(MolIn{atom_ptr[k + 1] - a0, h->z.data() + a0, h->xyz.data() + 3 * (size_t)a0, charge ? charge[k] : 0.0});
+  }
+  RunRequest req;
+  req.want_density = (flags & PM3_BATCH_WANT_DENSITY) != 0;
+  std::string err;
+  if (!batch_create(params->p, h->mols, eo, req, h->b, err)) {
+    batch_destroy(h->b);
+    delete h;
+    return fail(PM3_DEVICE_ERROR, err);
+  }
+  *out = h;
+  return PM3_OK;
+}
+
+pm3_status pm3_batch_solve(pm3_batch* h) {
+  g_last_error.clear();
+  if (!h) return fail(PM3_INVALID_INPUT, "null batch");
+  std::string err;
+  if (!batch_solve(h->b, err)) return fail(PM3_DEVICE_ERROR, err);
+  h->fetched = false;
+  return PM3_OK;
+}
+
+pm3_status pm3_batch_results(pm3_batch* h, pm3_status* status, pm3_result* results, double* charges, double* density) {
+  g_last_error.clear();
+  if (!h) return fail(PM3_INVALID_INPUT, "null batch");
+  std::string err;
+  if (!h->fetched) {
+    if (!batch_fetch(h->b, h->params->p, h->mols, h->results, err)) return fail(PM3_DEVICE_ERROR, err);
+    h->fetched = true;
+  }
+  size_t aoff = 0, moff = 0;
+  for (size_t k = 0; k < h->mols.size(); ++k) {
+    const MolResult& r = h->results[k];
+    if (status) status[k] = (pm3_status)r.status;
+    if (results) fill_result(r, &results[k]);
+    if (charges && (int)r.charges.size() == h->mols[k].n_atoms)
+      std::memcpy(charges + aoff, r.charges.data(), r.charges.size() * sizeof(double));
+    const size_t n2 = (size_t)r.nao * r.nao;
+    if (density && r.density.size() == n2 && n2 > 0) std::memcpy(density + moff, r.density.data(), n2 * sizeof(double));
+    aoff += h->mols[k].n_atoms;
+    moff += n2;
+  }
+  return PM3_OK;
+}
+
+void pm3_batch_destroy(pm3_batch* h) {
+  if (!h) return;
+  batch_destroy(h->b);
+  delete h;
+}
+
+pm3_status pm3_run_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
+                         const double* charge, const pm3_params* params, const pm3_options* opts, int32_t want_gradient,
+                         pm3_status* status, pm3_result* results, double* gradient) {
+  if (want_gradient) return fail(PM3_UNSUPPORTED, "batched gradients are not available yet");
+  (void)gradient;
+  pm3_batch* h = nullptr;
+  pm3_status s = pm3_batch_create(n_mol, atom_ptr, z, xyz_bohr, charge, params, opts, 0, &h);
+  if (s != PM3_OK) return s;
+  s = pm3_batch_solve(h);
+  if (s == PM3_OK) s = pm3_batch_results(h, status, results, nullptr, nullptr);
+  pm3_batch_destroy(h);
+  return s;
+}
+
+pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                        pm3_result* result, double* gradient, double* density) {
+  (void)mol; (void)params; (void)opts; (void)result; (void)gradient; (void)density;
+  return fail(PM3_UNSUPPORTED, "analytic gradient kernel not built yet");
+}
+
+const char* pm3_last_error_message(void) { return g_last_error.c_str(); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Kernel-level stages (include/pm3_gpu_stages.h).
+#include "../../include/pm3_gpu_stages.h"
+
+namespace {
+
+pm3_status run_stage(const pm3_molecule* mol, int mode, const std::vector<double>& in, std::vector<double>& out) {
+  const pm3_params* params = pm3_params_standard();
+  if (!params) return fail(PM3_DEVICE_ERROR, "parameter table unavailable");
+  MolIn m{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge};
+  int st = 0;
+  std::string err;
+  if (!stage_run(params->p, m, mode, in, out, st, err)) return fail(PM3_DEVICE_ERROR, err);
+  if (st != 0) return fail((pm3_status)st, status_text(st));
+  return PM3_OK;
+}
+
+// A chain of n hydrogens 3 Bohr apart: a carrier whose AO count is n for the generic stages.
+pm3_status generic_stage(int n, int mode, const std::vector<double>& in, std::vector<double>& out) {
+  if (n < 1 || n > 190) return fail(PM3_INVALID_INPUT, "n must be in [1, 190]");
+  std::vector<int32_t> z(n, 1);
+  std::vector<double> xyz(3 * (size_t)n, 0.0);
+  for (int i = 0; i < n; ++i) xyz[3 * i] = 3.0 * i;
+  pm3_molecule mol{n, z.data(), xyz.data(), (double)(n % 2), 1};
+  return run_stage(&mol, mode, in, out);
+}
+
+}  // namespace
+
+extern "C" {
+
+pm3_status pm3_stage_hcore(const pm3_molecule* mol, const pm3_params* params, double* h_core, double* core_ev) {
+  g_last_error.clear();
+  if (!params) return fail(PM3_INVALID_INPUT, "null parameter set");
+  pm3_status s = check_molecule(mol);
+  if (s != PM3_OK) return s;
+  std::vector<double> out;
+  s = run_stage(mol, 0, {}, out);
+  if (s != PM3_OK) return s;
+  const size_t n2 = out.size() - 1;
+  if (h_core) std::memcpy(h_core, out.data(), n2 * sizeof(double));
+  if (core_ev) *core_ev = out[n2];
+  return PM3_OK;
+}
+
+pm3_status pm3_stage_fock(const pm3_molecule* mol, const pm3_params* params, const double* density, double* fock) {
+  g_last_error.clear();
+  if (!params || !density || !fock) return fail(PM3_INVALID_INPUT, "null argument");
+  pm3_status s = check_molecule(mol);
+  if (s != PM3_OK) return s;
+  const int n = pm3_query_nao(mol, params);
+  if (n < 0) return fail(PM3_MISSING_ELEMENT, "missing element");
+  std::vector<double> in(density, density + (size_t)n * n), out;
+  s = run_stage(mol, 1, in, out);
+  if (s != PM3_OK) return s;
+  std::memcpy(fock, out.data(), (size_t)n * n * sizeof(double));
+  return PM3_OK;
+}
+
+pm3_status pm3_stage_eigen(int32_t n, const double* a, double* w, double* v) {
+  g_last_error.clear();
+  if (!a || !w || !v) return fail(PM3_INVALID_INPUT, "null argument");
+  std::vector<double> in(a, a + (size_t)n * n), out;
+  pm3_status s = generic_stage(n, 2, in, out);
+  if (s != PM3_OK) return s;
+  std::memcpy(w, out.data(), n * sizeof(double));
+  std::memcpy(v, out.data() + n, (size_t)n * n * sizeof(double));
+  return PM3_OK;
+}
+
+pm3_status pm3_stage_eigen_warm(int32_t n, const double* a, const double* c, double* w, double* v) {
+  g_last_error.clear();
+  if (!a || !c || !w || !v) return fail(PM3_INVALID_INPUT, "null argument");
+  std::vector<double> in(a, a + (size_t)n * n), out;
+  in.insert(in.end(), c, c + (size_t)n * n);
+  pm3_status s = generic_stage(n, 4, in, out);
+  if (s != PM3_OK) return s;
+  std::memcpy(w, out.data(), n * sizeof(double));
+  std::memcpy(v, out.data() + n, (size_t)n * n * sizeof(double));
+  return PM3_OK;
+}
+
+pm3_status pm3_stage_gemm(int32_t n, const double* a, const double* b, double* c) {
+  g_last_error.clear();
+  if (!a || !b || !c) return fail(PM3_INVALID_INPUT, "null argument");
+  std::vector<double> in(a, a + (size_t)n * n), out;
+  in.insert(in.end(), b, b + (size_t)n * n);
+  pm3_status s = generic_stage(n, 3, in, out);
+  if (s != PM3_OK) return s;
+  std::memcpy(c, out.data(), (size_t)n * n * sizeof(double));
+  return PM3_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1462 @@
+// Batched PM3 RHF SCF on CDNA4: one persistent launch, one workgroup per molecule.
+//
+// Workgroups pull molecule indices from an atomic queue (largest first) and run the whole
+// single-point for that molecule without leaving the kernel: pair integrals and H_core, the
+// superposition guess, then the SCF loop -- Fock build G(P), A-DIIS/CDIIS extrapolation,
+// eigensolve, aufbau density -- until the molecule's own convergence test passes, followed by
+// the stability search when its frontier gap is small.  Each molecule therefore carries its own
+// convergence mask (the loop exit) and iteration count, and a finished workgroup immediately
+// takes the next molecule.
+//
+// Reference path restated (file:line in the reference):
+//   H_core and pair cache ....... src/hamiltonian.rs:191-344
+//   Fock build .................. src/fock.rs:93-193 (two-centre), :244-332 (one-centre)
+//   SCF loop .................... src/scf.rs:1179-1316 (rhf_loop)
+//   DIIS history and solvers .... src/scf.rs:1065-1173, :1501-1693
+//   stability search ............ src/scf.rs:599-715
+//   guesses ..................... src/scf.rs:855-935
+//
+// Layout (per workgroup scratch, all doubles, n x n matrices row-major with leading dimension
+// np = nao rounded up to 16, atoms sorted heavy-first inside each molecule so that the pair
+// tables split into homogeneous XX / XH / HH blocks stored structure-of-arrays):
+//   H F P Pn Fu T C Cn V Pb Cb | DIIS F[8] E[8] D[8] | W_xx[100][nxx] W_xh[10][nxh] W_hh[nhh] | vectors
+//
+// Eigensolver: cyclic two-sided Jacobi with the round-robin parallel ordering, the symmetric
+// matrix packed in LDS, warm-started in the previous iteration's orbital basis
+// (A = C^T F C is nearly diagonal once the SCF settles, so few sweeps are needed).  Dense
+// products (C^T F C, C V, C_occ C_occ^T, F P) run on the FP64 matrix cores
+// (v_mfma_f64_16x16x4_f64).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "pm3_host.hpp"
+
+using namespace pm3dev;
+
+namespace {
+
+constexpr int NT = 512;       // threads per workgroup
+constexpr int NW = NT / 64;   // waves per workgroup
+constexpr int MAXD = 8;       // maximum DIIS depth (src/scf.rs:309)
+constexpr int LDS_BYTES_MAX = 160 * 1024;
+constexpr int STATIC_LDS_BYTES = 4096;  // upper bound on the kernel's static __shared__ use
+
+__constant__ TableConst c_tab;
+
+struct MolDesc {
+  int atom0, nat, nheavy, nao, nocc, np;
+  int pad0, pad1;
+  long long out_mat, out_vec;  // offsets into the density/coefficient and vector outputs, -1 if none
+};
+
+struct MolOut {
+  double e_elec, e_core, last_dp, first_e, homo, lumo;
+  double dip[3];
+  int iters, converged, paths, status;
+};
+
+struct KArgs {
+  const MolDesc* mols;
+  const int* order;
+  int nmol;
+  int* queue;
+  const int* aslot;
+  const double* ax;
+  const double* ay;
+  const double* az;
+  const int* aoff;
+  double* scratch;
+  long long stride;  // doubles per workgroup
+  int np_max;
+  long long w_cap;
+  MolOut* out;
+  double* out_P;
+  double* out_C;
+  double* out_eps;
+  double* out_q;  // Mulliken charge per atom (sorted atom order)
+  // options
+  int max_scf;
+  double e_tol, p_tol;
+  int accel;
+  double adiis_switch;
+  int guess, stability, scf_memory_mb;
+};
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------------------------------
+// Workgroup reductions.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <int K> __device__ inline void block_sum(double (&v)[K], double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[wave * K + k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int w = 0; w < NW; ++w) s += red[w * K + k];
+    v[k] = s;
+  }
+}
+
+__device__ inline double block_max(double v, double* red) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = wave_max(v);
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < NW; ++w) s = fmax(s, red[w]);
+  return s;
+}
+
+// Row-major upper-triangle pair index p -> (a, b), a < b < nat.
+__device__ inline void tri_decode(int p, int nat, int& a, int& b) {
+  const double tn = 2.0 * nat - 1.0;
+  int aa = (int)floor((tn - sqrt(fmax(tn * tn - 8.0 * p, 0.0))) * 0.5);
+  aa = max(0, min(aa, nat - 2));
+  while (aa > 0 && aa * (2 * nat - aa - 1) / 2 > p) --aa;
+  while (aa < nat - 2 && (aa + 1) * (2 * nat - aa - 2) / 2 <= p) ++aa;
+  a = aa;
+  b = p - aa * (2 * nat - aa - 1) / 2 + aa + 1;
+}
+__device__ inline int tri_index(int a, int b, int nat) { return a * (2 * nat - a - 1) / 2 + (b - a - 1); }
+
+// Packed upper-triangle index of (i, j) for the LDS copy of a symmetric matrix.
+__device__ inline int sym_idx(int i, int j) { return i <= j ? j * (j + 1) / 2 + i : i * (i + 1) / 2 + j; }
+
+// ------------------------------------------------------------------------------------------
+// Dense product on the FP64 matrix cores: out(i, j) = sum_{k<K} A(i, k) B(k, j) for i < M, j < N.
+// Each wave owns 16 x 16 output tiles; per step the 64 lanes feed one 16x4 slice of A (lane l holds
+// A[l % 16][l / 16]) and one 4x16 slice of B (lane l holds B[l / 16][l % 16]) into
+// v_mfma_f64_16x16x4_f64.  Loaders return 0 outside the logical shape.
+template <class LA, class LB, class ST>
+__device__ inline void wg_gemm(int M, int N, int K, LA la, LB lb, ST st) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int t = wave; t < tm * tn; t += NW) {
+    const int i0 = (t / tn) << 4, j0 = (t % tn) << 4;
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += 4) {
+      const int k = k0 + lk;
+      const double av = (k < K && i0 + li < M) ? la(i0 + li, k) : 0.0;
+      const double bv = (k < K && j0 + li < N) ? lb(k, j0 + li) : 0.0;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+    // result layout: item r of lane l holds row 4 r + l / 16, column l % 16
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + 4 * r + lk, j = j0 + li;
+      if (i < M && j < N) st(i, j, acc[r]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// One-centre two-electron integral (ab|cd) on one atom (src/fock.rs:244-282).
+__device__ inline double oc_int(int a, int b, int c, int d, const ElemConst& e) {
+  if (a == b && c == d) {
+    if (a == 0 && c == 0) return e.gss;
+    if (a == 0 || c == 0) return e.gsp;
+    return a == c ? e.gpp : e.gp2;
+  }
+  const int ba = min(a, b), bb = max(a, b), kc = min(c, d), kd = max(c, d);
+  if (ba == 0 && bb != 0 && kc == 0 && kd != 0 && bb == kd) return e.hsp;
+  if (ba != 0 && bb != 0 && ba != bb && ba == kc && bb == kd) return 0.5 * (e.gpp - e.gp2);
+  return 0.0;
+}
+
+struct Mol {
+  int atom0, nat, nh, nH, n, np, nocc;
+  int nxx, nxh, nhh;
+  double* Wxx;
+  double* Wxh;
+  double* Whh;
+};
+
+// ------------------------------------------------------------------------------------------
+// Pair integrals, core-core energy and H_core (src/hamiltonian.rs:191-344, src/repulsion.rs:13-27).
+__device__ double setup_core(const KArgs& A, const Mol& M, double* H, double* lds, double* red) {
+  const int tid = threadIdx.x;
+  const int np = M.np, nat = M.nat;
+  double* acc = lds;  // [nat][16] electron-core sums per atom block
+  for (int e = tid; e < np * np; e += NT) H[e] = 0.0;
+  for (int e = tid; e < nat * 16; e += NT) acc[e] = 0.0;
+  __syncthreads();
+  double ecc = 0.0;
+  const int npairs = nat * (nat - 1) / 2;
+  for (int p = tid; p < npairs; p += NT) {
+    int a, b;
+    tri_decode(p, nat, a, b);
+    const int ga = M.atom0 + a, gb = M.atom0 + b;
+    const int sa = A.aslot[ga], sb = A.aslot[gb];
+    const ElemConst& ea = c_tab.el[sa];
+    const ElemConst& eb = c_tab.el[sb];
+    const double dx = A.ax[gb] - A.ax[ga], dy = A.ay[gb] - A.ay[ga], dz = A.az[gb] - A.az[ga];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    ecc += core_core_pair<double>(c_tab, sa, sb, r);
+    const double ux = dx / r, uy = dy / r, uz = dz / r;
+    // Two-electron integrals and electron-core attraction (src/integrals.rs:167-230).
+    const double ddx = ux * r, ddy = uy * r, ddz = uz * r;
+    const double rr = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+    const double inv = 1.0 / rr;
+    double R[3][3];
+    rot_to_x(-(ddx * inv), -(ddy * inv), -(ddz * inv), R);
+    double e1b[10], e2a[10];
+    ecore(ea, eb, ddx, ddy, ddz, e1b);
+    ecore(eb, ea, -ddx, -ddy, -ddz, e2a);
+    const int na = ea.norb, nb = eb.norb;
+    for (int mu = 0; mu < na; ++mu)
+      for (int nu = 0; nu < na; ++nu) atomicAdd(&acc[a * 16 + mu * 4 + nu], e1b[pk(mu, nu)]);
+    for (int mu = 0; mu < nb; ++mu)
+      for (int nu = 0; nu < nb; ++nu) atomicAdd(&acc[b * 16 + mu * 4 + nu], e2a[pk(mu, nu)]);
+    if (na == 4 && nb == 4) {
+      double ri[22];
+      local_xx(ea, eb, rr, ri);
+      const int kx = tri_index(a, b, M.nh);
+      double* W = M.Wxx;
+      const int nxx = M.nxx;
+      assemble_xx(ri, R, [&](int idx, double v) { W[(long long)idx * nxx + kx] = v; });
+    } else if (na == 4) {
+      double ri[4], w[10];
+      local_xh(ea.dd, ea.qq, ea.rho0, ea.rho1, ea.rho2, eb.rho0, rr, ri);
+      assemble_xh(ri, R, w);
+      const int kx = a * M.nH + (b - M.nh);
+#pragma unroll
+      for (int k = 0; k < 10; ++k) M.Wxh[(long long)k * M.nxh + kx] = w[k];
+    } else {
+      const double s = ea.rho0 + eb.rho0;
+      M.Whh[tri_index(a - M.nh, b - M.nh, M.nH)] = rsq(rr * rr + s * s) * EV;
+    }
+    // Resonance 1/2 (beta_mu + beta_nu) S_mu,nu (src/hamiltonian.rs:322-331).
+    double S[4][4];
+    overlap_block(ea, eb, ux, uy, uz, r, S);
+    const int oa = A.aoff[ga], ob = A.aoff[gb];
+    for (int i = 0; i < na; ++i) {
+      const double bi = i == 0 ? ea.betas : ea.betap;
+      for (int j = 0; j < nb; ++j) {
+        const double bj = j == 0 ? eb.betas : eb.betap;
+        const double v = 0.5 * (bi + bj) * S[i][j];
+        H[(oa + i) * np + ob + j] = v;
+        H[(ob + j) * np + oa + i] = v;
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < nat * 16; e += NT) {
+    const int a = e >> 4, mu = (e >> 2) & 3, nu = e & 3;
+    const int ga = M.atom0 + a;
+    const ElemConst& ea = c_tab.el[A.aslot[ga]];
+    if (mu < ea.norb && nu < ea.norb) {
+      const int oa = A.aoff[ga];
+      const double u = mu != nu ? 0.0 : (mu == 0 ? ea.uss : ea.upp);
+      H[(oa + mu) * np + oa + nu] = u + acc[e];
+    }
+  }
+  double v[1] = {ecc};
+  block_sum<1>(v, red);
+  __syncthreads();
+  return v[0];
+}
+
+// ------------------------------------------------------------------------------------------
+// F = H + G(P): RHF Fock build (src/fock.rs:285-332, :488-500).  Off-diagonal atom blocks are
+// owned by their pair (exchange K); the Coulomb contributions J to the diagonal blocks are summed
+// per atom in LDS, then each diagonal block adds its one-centre term.
+__device__ void build_fock(const KArgs& A, const Mol& M, const double* H, const double* P, double* F, double* lds) {
+  const int tid = threadIdx.x;
+  const int np = M.np, nat = M.nat;
+  double* J = lds;  // [nat][10]
+  for (int e = tid; e < nat * 10; e += NT) J[e] = 0.0;
+  __syncthreads();
+  // heavy-heavy pairs
+  for (int kx = tid; kx < M.nxx; kx += NT) {
+    int a, b;
+    tri_decode(kx, M.nh, a, b);
+    const int oa = A.aoff[M.atom0 + a], ob = A.aoff[M.atom0 + b];
+    double da[10], db[10], pab[4][4];
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+#pragma unroll
+      for (int nu = 0; nu <= mu; ++nu) {
+        const double f = mu == nu ? 1.0 : 2.0;
+        da[pk(mu, nu)] = f * P[(oa + mu) * np + oa + nu];
+        db[pk(mu, nu)] = f * P[(ob + mu) * np + ob + nu];
+      }
+#pragma unroll
+      for (int la = 0; la < 4; ++la) pab[mu][la] = P[(oa + mu) * np + ob + la];
+    }
+    double ja[10], jb[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, K[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) K[i][j] = 0.0;
+    const double* W = M.Wxx + kx;
+    const long long ld = M.nxx;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+      for (int nu = 0; nu <= mu; ++nu) {
+        const int p = pk(mu, nu);
+        double w[10];
+#pragma unroll
+        for (int q = 0; q < 10; ++q) w[q] = W[(long long)(p * 10 + q) * ld];
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < 10; ++q) {
+          acc += db[q] * w[q];
+          jb[q] += da[p] * w[q];
+        }
+        ja[p] = acc;
+        // exchange: K[mu][la] -= 1/2 sum_si P(nu, si) (mu nu|la si), and the (nu, mu) term
+#pragma unroll
+        for (int la = 0; la < 4; ++la) {
+          double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+          for (int si = 0; si < 4; ++si) {
+            s1 += pab[nu][si] * w[pk(la, si)];
+            s2 += pab[mu][si] * w[pk(la, si)];
+          }
+          K[mu][la] -= 0.5 * s1;
+          if (mu != nu) K[nu][la] -= 0.5 * s2;
+        }
+      }
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+      for (int la = 0; la < 4; ++la) {
+        const double v = H[(oa + mu) * np + ob + la] + K[mu][la];
+        F[(oa + mu) * np + ob + la] = v;
+        F[(ob + la) * np + oa + mu] = v;
+      }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      atomicAdd(&J[a * 10 + q], ja[q]);
+      atomicAdd(&J[b * 10 + q], jb[q]);
+    }
+  }
+  // heavy-hydrogen pairs
+  for (int kx = tid; kx < M.nxh; kx += NT) {
+    const int a = kx / M.nH, b = M.nh + kx % M.nH;
+    const int oa = A.aoff[M.atom0 + a], ob = A.aoff[M.atom0 + b];
+    const double pbb = P[ob * np + ob];
+    double jbv = 0.0, pa[4];
+#pragma unroll
+    for (int nu = 0; nu < 4; ++nu) pa[nu] = P[(oa + nu) * np + ob];
+    double w[10];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) w[q] = M.Wxh[(long long)q * M.nxh + kx];
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+      for (int nu = 0; nu <= mu; ++nu) {
+        const double f = mu == nu ? 1.0 : 2.0;
+        jbv += f * P[(oa + mu) * np + oa + nu] * w[pk(mu, nu)];
+      }
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      double k = 0.0;
+#pragma unroll
+      for (int nu = 0; nu < 4; ++nu) k += pa[nu] * w[pk(mu, nu)];
+      const double v = H[(oa + mu) * np + ob] - 0.5 * k;
+      F[(oa + mu) * np + ob] = v;
+      F[ob * np + oa + mu] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < 10; ++q) atomicAdd(&J[a * 10 + q], w[q] * pbb);
+    atomicAdd(&J[b * 10], jbv);
+  }
+  // hydrogen-hydrogen pairs
+  for (int kx = tid; kx < M.nhh; kx += NT) {
+    int a, b;
+    tri_decode(kx, M.nH, a, b);
+    a += M.nh;
+    b += M.nh;
+    const int oa = A.aoff[M.atom0 + a], ob = A.aoff[M.atom0 + b];
+    const double w = M.Whh[kx];
+    const double v = H[oa * np + ob] - 0.5 * P[oa * np + ob] * w;
+    F[oa * np + ob] = v;
+    F[ob * np + oa] = v;
+    atomicAdd(&J[a * 10], w * P[ob * np + ob]);
+    atomicAdd(&J[b * 10], w * P[oa * np + oa]);
+  }
+  __syncthreads();
+  // diagonal blocks: H + one-centre + J
+  for (int e = tid; e < nat * 16; e += NT) {
+    const int a = e >> 4, mu = (e >> 2) & 3, nu = e & 3;
+    const int ga = M.atom0 + a;
+    const ElemConst& ea = c_tab.el[A.aslot[ga]];
+    const int nn = ea.norb;
+    if (mu < nn && nu < nn) {
+      const int o = A.aoff[ga];
+      double acc = 0.0;
+      for (int la = 0; la < nn; ++la)
+        for (int si = 0; si < nn; ++si) {
+          const double pls = P[(o + la) * np + o + si];
+          acc += pls * oc_int(mu, nu, la, si, ea);
+          acc -= (0.5 * pls) * oc_int(mu, la, nu, si, ea);
+        }
+      F[(o + mu) * np + o + nu] = H[(o + mu) * np + o + nu] + acc + J[a * 10 + pk(mu, nu)];
+    }
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------
+// Symmetric eigensolver: A (n x n, row-major ld np) -> eigenvalues lam[0..n) (unsorted, in the
+// order of V's columns) and V (column-major, ld np) with A = V diag(lam) V^T.
+// Cyclic Jacobi, round-robin (circle) ordering: in round r slot k pairs players
+// ((r + k) mod (nn-1), nn-1) for k = 0 and ((r + k) mod (nn-1), (nn - 1 - k + r) mod (nn-1))
+// otherwise, so every pair meets once per sweep and the m = nn/2 rotations of a round are
+// disjoint.  Golub & Van Loan Alg. 8.5.2 for the rotation angle.
+__device__ inline void rr_pair(int k, int r, int nn, int& p, int& q) {
+  const int m1 = nn - 1;
+  p = (r + k) % m1;
+  q = (k == 0) ? m1 : (m1 - k + r) % m1;
+}
+
+__device__ int jacobi(const double* Ain, int n, int np, double* V, double* lam, double* lds, double* red, int max_sweeps) {
+  const int tid = threadIdx.x;
+  const int nn = n + (n & 1), m = nn >> 1;
+  const int npk = n * (n + 1) / 2;
+  double* Ap = lds;
+  double* cs = lds + npk;  // [m][3]
+  for (int idx = tid; idx < npk; idx += NT) {
+    int j = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+    while (j * (j + 1) / 2 > idx) --j;
+    while ((j + 1) * (j + 2) / 2 <= idx) ++j;
+    const int i = idx - j * (j + 1) / 2;
+    Ap[idx] = Ain[i * np + j];
+  }
+  for (int e = tid; e < n * np; e += NT) {
+    const int c = e / np, rrow = e % np;
+    V[e] = (c == rrow) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  double dmax = 0.0;
+  for (int i = tid; i < n; i += NT) dmax = fmax(dmax, fabs(Ap[i * (i + 1) / 2 + i]));
+  const double scale = fmax(block_max(dmax, red), 1.0);
+  const double tol = 1.0e-14 * scale;
+  int sweep = 0;
+  if (n < 2) return 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    double off = 0.0;
+    for (int idx = tid; idx < npk; idx += NT) {
+      int j = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
+      while (j * (j + 1) / 2 > idx) --j;
+      while ((j + 1) * (j + 2) / 2 <= idx) ++j;
+      const int i = idx - j * (j + 1) / 2;
+      if (i != j) off = fmax(off, fabs(Ap[idx]));
+    }
+    if (block_max(off, red) <= tol) break;
+    for (int r = 0; r < nn - 1; ++r) {
+      for (int k = tid; k < m; k += NT) {
+        int p, q;
+        rr_pair(k, r, nn, p, q);
+        double c = 1.0, s = 0.0, t = 0.0;
+        if (p < n && q < n) {
+          const double apq = Ap[sym_idx(p, q)];
+          const double app = Ap[sym_idx(p, p)], aqq = Ap[sym_idx(q, q)];
+          if (fabs(apq) > 1.0e-300 && fabs(apq) > 1.0e-18 * (fabs(app) + fabs(aqq))) {
+            const double tau = (aqq - app) / (2.0 * apq);
+            t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = t * c;
+          }
+        }
+        cs[3 * k] = c;
+        cs[3 * k + 1] = s;
+        cs[3 * k + 2] = t;
+      }
+      __syncthreads();
+      for (int bidx = tid; bidx < m * m; bidx += NT) {
+        const int K = bidx / m, L = bidx % m;
+        if (K > L) continue;
+        int p, q, u, v;
+        rr_pair(K, r, nn, p, q);
+        rr_pair(L, r, nn, u, v);
+        const double cK = cs[3 * K], sK = cs[3 * K + 1], tK = cs[3 * K + 2];
+        if (K == L) {
+          if (p < n && q < n && tK != 0.0) {
+            const int ipq = sym_idx(p, q);
+            const double apq = Ap[ipq];
+            Ap[sym_idx(p, p)] -= tK * apq;
+            Ap[sym_idx(q, q)] += tK * apq;
+            Ap[ipq] = 0.0;
+          }
+        } else {
+          const double cL = cs[3 * L], sL = cs[3 * L + 1];
+          if (sK == 0.0 && sL == 0.0) continue;
+          const bool vp = p < n, vq = q < n, vu = u < n, vv = v < n;
+          const double apu = (vp && vu) ? Ap[sym_idx(p, u)] : 0.0;
+          const double apv = (vp && vv) ? Ap[sym_idx(p, v)] : 0.0;
+          const double aqu = (vq && vu) ? Ap[sym_idx(q, u)] : 0.0;
+          const double aqv = (vq && vv) ? Ap[sym_idx(q, v)] : 0.0;
+          const double xpu = cK * apu - sK * aqu, xqu = sK * apu + cK * aqu;
+          const double xpv = cK * apv - sK * aqv, xqv = sK * apv + cK * aqv;
+          if (vp && vu) Ap[sym_idx(p, u)] = cL * xpu - sL * xpv;
+          if (vp && vv) Ap[sym_idx(p, v)] = sL * xpu + cL * xpv;
+          if (vq && vu) Ap[sym_idx(q, u)] = cL * xqu - sL * xqv;
+          if (vq && vv) Ap[sym_idx(q, v)] = sL * xqu + cL * xqv;
+        }
+      }
+      for (int e = tid; e < n * m; e += NT) {
+        const int k = e / n, i = e % n;
+        const double s = cs[3 * k + 1];
+        if (s == 0.0) continue;
+        int p, q;
+        rr_pair(k, r, nn, p, q);
+        if (p >= n || q >= n) continue;
+        const double c = cs[3 * k];
+        const double vpv = V[p * np + i], vqv = V[q * np + i];
+        V[p * np + i] = c * vpv - s * vqv;
+        V[q * np + i] = s * vpv + c * vqv;
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < n; i += NT) lam[i] = Ap[i * (i + 1) / 2 + i];
+  __syncthreads();
+  return sweep;
+}
+
+// ------------------------------------------------------------------------------------------
+// DIIS history (src/scf.rs:1065-1173) kept in LDS as logical slots oldest -> newest.
+struct Diis {
+  int len, depth, keep_d, newslot;
+  int slot[MAXD];
+  double b[MAXD][MAXD];
+  double mm[MAXD][MAXD];
+  double coef[MAXD];
+  int ok;
+};
+
+// Pulay CDIIS coefficients from the (scale-normalised) bordered Gram system with a pivot guard
+// (src/scf.rs:1544-1630).  Single thread.
+__device__ bool cdiis_coeffs(const Diis& d, double* c) {
+  const int n = d.len;
+  if (n < 2) return false;
+  double scale = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) scale = fmax(scale, fabs(d.b[i][j]));
+  if (!(scale > 0.0) || !isfinite(scale)) return false;
+  const int dim = n + 1;
+  double m[MAXD + 1][MAXD + 1], rhs[MAXD + 1], x[MAXD + 1];
+  for (int i = 0; i < dim; ++i) {
+    rhs[i] = 0.0;
+    for (int j = 0; j < dim; ++j) m[i][j] = 0.0;
+  }
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) m[i][j] = d.b[i][j] / scale;
+    m[i][n] = -1.0;
+    m[n][i] = -1.0;
+  }
+  rhs[n] = -1.0;
+  for (int col = 0; col < dim; ++col) {
+    int piv = col;
+    double best = fabs(m[col][col]);
+    for (int row = col + 1; row < dim; ++row)
+      if (fabs(m[row][col]) > best) { best = fabs(m[row][col]); piv = row; }
+    if (best < 1.0e-12) return false;
+    if (piv != col) {
+      for (int j = 0; j < dim; ++j) { double t = m[col][j]; m[col][j] = m[piv][j]; m[piv][j] = t; }
+      double t = rhs[col]; rhs[col] = rhs[piv]; rhs[piv] = t;
+    }
+    for (int row = col + 1; row < dim; ++row) {
+      const double f = m[row][col] / m[col][col];
+      if (f == 0.0) continue;
+      for (int j = col; j < dim; ++j) m[row][j] -= f * m[col][j];
+      rhs[row] -= f * rhs[col];
+    }
+  }
+  for (int col = dim - 1; col >= 0; --col) {
+    double s = rhs[col];
+    for (int j = col + 1; j < dim; ++j) s -= m[col][j] * x[j];
+    x[col] = s / m[col][col];
+  }
+  for (int i = 0; i < n; ++i) c[i] = x[i];
+  return true;
+}
+
+// A-DIIS (Hu & Yang 2010): projected gradient on the probability simplex, 400 steps at most
+// (src/scf.rs:1637-1693).  Single thread.
+__device__ void adiis_coeffs(const Diis& D, double* c) {
+  const int n = D.len, last = n - 1;
+  const double mnn = D.mm[last][last];
+  double d[MAXD], s[MAXD][MAXD];
+  for (int i = 0; i < n; ++i) d[i] = D.mm[i][last] - mnn;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) s[i][j] = D.mm[i][j] - D.mm[i][last] - D.mm[last][j] + mnn;
+  double L = 1.0e-12;
+  for (int i = 0; i < n; ++i) {
+    double row = 0.0;
+    for (int j = 0; j < n; ++j) row += fabs(s[i][j] + s[j][i]);
+    L = fmax(L, row);
+  }
+  const double lr = 1.0 / L;
+  for (int i = 0; i < n; ++i) c[i] = 0.0;
+  c[n - 1] = 1.0;
+  for (int it = 0; it < 400; ++it) {
+    double tr[MAXD], u[MAXD];
+    for (int k = 0; k < n; ++k) {
+      double acc = 2.0 * d[k];
+      for (int j = 0; j < n; ++j) acc += (s[k][j] + s[j][k]) * c[j];
+      tr[k] = c[k] - lr * acc;
+      u[k] = tr[k];
+    }
+    // sort u descending (insertion sort, n <= 8) and find the simplex threshold
+    for (int i = 1; i < n; ++i) {
+      const double x = u[i];
+      int j = i - 1;
+      while (j >= 0 && u[j] < x) { u[j + 1] = u[j]; --j; }
+      u[j + 1] = x;
+    }
+    double css = 0.0, theta = 0.0;
+    for (int j = 0; j < n; ++j) {
+      css += u[j];
+      const double t = (css - 1.0) / (double)(j + 1);
+      if (u[j] - t > 0.0) theta = t;
+    }
+    double delta = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double pr = fmax(tr[i] - theta, 0.0);
+      delta += fabs(pr - c[i]);
+      c[i] = pr;
+    }
+    if (delta < 1.0e-12) break;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+struct Bufs {
+  double *H, *F, *P, *Pn, *Fu, *T, *C, *Cn, *V, *Pb, *Cb, *Fh, *Eh, *Dh, *eps, *epsb, *lam;
+  int* perm;
+};
+
+// Eigenvectors of Fu -> Cn (row-major, columns sorted by ascending eigenvalue) and eps.
+// warm != 0: Jacobi on C^T Fu C and Cn = C V; otherwise Jacobi on Fu itself.
+__device__ void eigensolve(const Mol& M, const Bufs& B, const double* Fu, bool warm, double* lds, double* red) {
+  const int n = M.n, np = M.np, tid = threadIdx.x;
+  const double* Amat = Fu;
+  if (warm) {
+    const double* C = B.C;
+    double* T = B.T;
+    // T = Fu C ; A = C^T T  (A written into Cn as scratch)
+    wg_gemm(n, n, n, [&](int i, int k) { return Fu[i * np + k]; }, [&](int k, int j) { return C[k * np + j]; },
+            [&](int i, int j, double v) { T[i * np + j] = v; });
+    __syncthreads();
+    double* Am = B.Cn;
+    wg_gemm(n, n, n, [&](int i, int k) { return C[k * np + i]; }, [&](int k, int j) { return T[k * np + j]; },
+            [&](int i, int j, double v) { Am[i * np + j] = v; });
+    __syncthreads();
+    Amat = Am;
+  }
+  jacobi(Amat, n, np, B.V, B.lam, lds, red, 60);
+  // ascending order, ties by index
+  for (int k = tid; k < n; k += NT) {
+    const double lk = B.lam[k];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double lj = B.lam[j];
+      rank += (lj < lk || (lj == lk && j < k)) ? 1 : 0;
+    }
+    B.perm[rank] = k;
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += NT) B.eps[j] = B.lam[B.perm[j]];
+  const double* V = B.V;
+  const int* perm = B.perm;
+  double* Cn = B.Cn;
+  if (warm) {
+    const double* C = B.C;
+    wg_gemm(n, n, n, [&](int i, int k) { return C[i * np + k]; }, [&](int k, int j) { return V[perm[j] * np + k]; },
+            [&](int i, int j, double v) { Cn[i * np + j] = v; });
+  } else {
+    for (int e = tid; e < n * n; e += NT) {
+      const int i = e / n, j = e % n;
+      Cn[i * np + j] = V[perm[j] * np + i];
+    }
+  }
+  __syncthreads();
+}
+
+// P = 2 C_occ C_occ^T from Cn into Pdst.
+__device__ void density(const Mol& M, const double* Cn, double* Pdst) {
+  const int n = M.n, np = M.np;
+  wg_gemm(n, n, M.nocc, [&](int i, int k) { return Cn[i * np + k]; }, [&](int k, int j) { return Cn[j * np + k]; },
+          [&](int i, int j, double v) { Pdst[i * np + j] = 2.0 * v; });
+  __syncthreads();
+}
+
+// Initial density for a guess mode (src/scf.rs:855-935).  Core guess uses the eigenvectors of H.
+__device__ void initial_guess(const KArgs& A, const Mol& M, const Bufs& B, int guess, double* lds, double* red) {
+  const int np = M.np, tid = threadIdx.x;
+  for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
+  __syncthreads();
+  if (guess == 2) {
+    eigensolve(M, B, B.H, false, lds, red);
+    density(M, B.Cn, B.P);
+    return;
+  }
+  for (int a = tid; a < M.nat; a += NT) {
+    const int ga = M.atom0 + a;
+    const ElemConst& e = c_tab.el[A.aslot[ga]];
+    const int o = A.aoff[ga];
+    B.P[o * np + o] = e.ios;
+    if (e.norb == 4)
+      for (int k = 1; k < 4; ++k) B.P[(o + k) * np + o + k] = e.iop / 3.0;
+  }
+  __syncthreads();
+  if (guess == 3 && tid == 0) {
+    // tilt a fifth of an electron from p_z to p_x on the atom with the largest p population
+    int target = -1;
+    double best = 0.0;
+    for (int a = 0; a < M.nat; ++a) {
+      const ElemConst& e = c_tab.el[A.aslot[M.atom0 + a]];
+      if (e.norb == 4 && e.iop > best) { best = e.iop; target = a; }
+    }
+    if (target >= 0) {
+      const int o = A.aoff[M.atom0 + target];
+      const double tilt = fmin(0.2, B.P[(o + 3) * np + o + 3]);
+      B.P[(o + 3) * np + o + 3] -= tilt;
+      B.P[(o + 1) * np + o + 1] += tilt;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ inline double dot_nn(const double* X, const double* Y, int np, double* red) {
+  double v[1] = {0.0};
+  for (int e = threadIdx.x; e < np * np; e += NT) v[0] += X[e] * Y[e];
+  block_sum<1>(v, red);
+  return v[0];
+}
+
+struct PathResult {
+  double e_elec, last_dp;
+  int iters, converged;
+};
+
+// One restricted SCF path (rhf_loop, src/scf.rs:1179-1316).  Leaves P, eps, Cn (orbitals of the
+// last extrapolated Fock) in B.
+__device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int guess, int accel, double* lds,
+                               double* red, Diis& D) {
+  const int n = M.n, np = M.np, tid = threadIdx.x;
+  initial_guess(A, M, B, guess, lds, red);
+  // DIIS depth from the memory budget (src/scf.rs:309-327, :1205-1228)
+  const long long n2 = (long long)n * n * 8;
+  const long long budget = (long long)A.scf_memory_mb * 1024 * 1024;
+  int keep_d = accel == 2;
+  if (keep_d && !(A.scf_memory_mb == 0 || 2 * 3 * n2 <= budget)) keep_d = 0;
+  if (accel == 2 && !keep_d) accel = 1;
+  const long long per = (keep_d ? 3 : 2) * n2;
+  int depth = MAXD;
+  if (A.scf_memory_mb != 0 && per != 0) depth = (int)max(2LL, min((long long)MAXD, budget / per));
+  if (tid == 0) {
+    D.len = 0;
+    D.depth = depth;
+    D.keep_d = keep_d;
+  }
+  __syncthreads();
+  PathResult R{0.0, 1.0e300, 0, 0};
+  double e_old = 0.0;
+  bool have_c = false;
+  for (int it = 0; it < A.max_scf; ++it) {
+    R.iters = it + 1;
+    build_fock(A, M, B.H, B.P, B.F, lds);
+    double ev[1] = {0.0};
+    for (int e = tid; e < np * np; e += NT) ev[0] += B.P[e] * (B.H[e] + B.F[e]);
+    block_sum<1>(ev, red);
+    const double e_elec = 0.5 * ev[0];
+    const double* Fuse = B.F;
+    if (accel != 0) {
+      // choose the slot of the new entry (drop the oldest when full)
+      if (tid == 0) {
+        if (D.len == D.depth) {
+          const int s0 = D.slot[0];
+          for (int i = 0; i + 1 < D.len; ++i) {
+            D.slot[i] = D.slot[i + 1];
+            for (int j = 0; j + 1 < D.len; ++j) {
+              D.b[i][j] = D.b[i + 1][j + 1];
+              D.mm[i][j] = D.mm[i + 1][j + 1];
+            }
+          }
+          D.len -= 1;
+          D.newslot = s0;
+        } else {
+          D.newslot = D.len;
+        }
+      }
+      __syncthreads();
+      const int ns = D.newslot;
+      const long long ms = (long long)np * np;
+      double* Fn = B.Fh + ns * ms;
+      double* En = B.Eh + ns * ms;
+      double* Dn = B.Dh + ns * ms;
+      // E = F P - P F (src/scf.rs:1039-1054) via T = F P
+      wg_gemm(n, n, n, [&](int i, int k) { return B.F[i * np + k]; }, [&](int k, int j) { return B.P[k * np + j]; },
+              [&](int i, int j, double v) { B.T[i * np + j] = v; });
+      __syncthreads();
+      for (int e = tid; e < np * np; e += NT) {
+        const int i = e / np, j = e % np;
+        En[e] = (i < n && j < n) ? B.T[i * np + j] - B.T[j * np + i] : 0.0;
+        Fn[e] = B.F[e];
+        if (keep_d) Dn[e] = B.P[e];
+      }
+      __syncthreads();
+      // Gram updates: <E_i,E_new>, <D_new,F_j>, <D_i,F_new> for the existing entries, plus
+      // <E_new,E_new> and <D_new,F_new> (index len)
+      const int len = D.len;
+      double v[3 * MAXD + 2];
+#pragma unroll
+      for (int k = 0; k < 3 * MAXD + 2; ++k) v[k] = 0.0;
+      for (int e = tid; e < np * np; e += NT) {
+        const double en = En[e], fn = Fn[e], dn = keep_d ? Dn[e] : 0.0;
+        v[3 * MAXD] += en * en;
+        v[3 * MAXD + 1] += dn * fn;
+#pragma unroll
+        for (int i = 0; i < MAXD; ++i) {
+          if (i < len) {
+            const long long so = (long long)D.slot[i] * ms;
+            v[i] += B.Eh[so + e] * en;
+            if (keep_d) {
+              v[MAXD + i] += dn * B.Fh[so + e];
+              v[2 * MAXD + i] += B.Dh[so + e] * fn;
+            }
+          }
+        }
+      }
+      block_sum<3 * MAXD + 2>(v, red);
+      if (tid == 0) {
+        for (int i = 0; i < len; ++i) {
+          D.b[i][len] = v[i];
+          D.b[len][i] = v[i];
+          D.mm[len][i] = v[MAXD + i];
+          D.mm[i][len] = v[2 * MAXD + i];
+        }
+        D.b[len][len] = v[3 * MAXD];
+        D.mm[len][len] = v[3 * MAXD + 1];
+        D.slot[len] = ns;
+        D.len = len + 1;
+        const double en = sqrt(v[3 * MAXD]);
+        bool ok;
+        if (accel == 2 && en > A.adiis_switch) {
+          ok = D.len >= 2 && D.keep_d;
+          if (ok) adiis_coeffs(D, D.coef);
+        } else {
+          ok = cdiis_coeffs(D, D.coef);
+        }
+        D.ok = ok;
+      }
+      __syncthreads();
+      if (D.ok) {
+        const int L = D.len;
+        for (int e = tid; e < np * np; e += NT) {
+          double s = 0.0;
+          for (int i = 0; i < L; ++i) s += D.coef[i] * B.Fh[(long long)D.slot[i] * ms + e];
+          B.Fu[e] = s;
+        }
+        __syncthreads();
+        Fuse = B.Fu;
+      } else {
+        Fuse = Fn;
+      }
+    }
+    eigensolve(M, B, Fuse, have_c, lds, red);
+    density(M, B.Cn, B.Pn);
+    double dv[1] = {0.0};
+    for (int e = tid; e < np * np; e += NT) {
+      const double d = B.Pn[e] - B.P[e];
+      dv[0] += d * d;
+    }
+    block_sum<1>(dv, red);
+    const double dp = sqrt(dv[0] / (double)max(n * n, 1));
+    const double de = fabs(e_elec - e_old);
+    R.last_dp = dp;
+    for (int e = tid; e < np * np; e += NT) {
+      B.P[e] = B.Pn[e];
+      B.C[e] = B.Cn[e];
+    }
+    __syncthreads();
+    have_c = true;
+    e_old = e_elec;
+    if (it > 0 && de < A.e_tol && dp < A.p_tol) {
+      R.converged = 1;
+      break;
+    }
+  }
+  build_fock(A, M, B.H, B.P, B.F, lds);
+  double ev[1] = {0.0};
+  for (int e = tid; e < np * np; e += NT) ev[0] += B.P[e] * (B.H[e] + B.F[e]);
+  block_sum<1>(ev, red);
+  R.e_elec = 0.5 * ev[0];
+  return R;
+}
+
+__device__ inline Bufs slab(const KArgs& A, double*& Wbase) {
+  double* base = A.scratch + (long long)blockIdx.x * A.stride;
+  const long long m2 = (long long)A.np_max * A.np_max;
+  Bufs B;
+  B.H = base;
+  B.F = B.H + m2;
+  B.P = B.F + m2;
+  B.Pn = B.P + m2;
+  B.Fu = B.Pn + m2;
+  B.T = B.Fu + m2;
+  B.C = B.T + m2;
+  B.Cn = B.C + m2;
+  B.V = B.Cn + m2;
+  B.Pb = B.V + m2;
+  B.Cb = B.Pb + m2;
+  B.Fh = B.Cb + m2;
+  B.Eh = B.Fh + MAXD * m2;
+  B.Dh = B.Eh + MAXD * m2;
+  Wbase = B.Dh + MAXD * m2;
+  B.eps = Wbase + A.w_cap;
+  B.epsb = B.eps + A.np_max;
+  B.lam = B.epsb + A.np_max;
+  B.perm = (int*)(B.lam + A.np_max);
+  return B;
+}
+
+__device__ inline Mol mol_of(const MolDesc& md, double* Wbase) {
+  Mol M;
+  M.atom0 = md.atom0;
+  M.nat = md.nat;
+  M.nh = md.nheavy;
+  M.nH = md.nat - md.nheavy;
+  M.n = md.nao;
+  M.np = md.np;
+  M.nocc = md.nocc;
+  M.nxx = M.nh * (M.nh - 1) / 2;
+  M.nxh = M.nh * M.nH;
+  M.nhh = M.nH * (M.nH - 1) / 2;
+  M.Wxx = Wbase;
+  M.Wxh = M.Wxx + 100LL * M.nxx;
+  M.Whh = M.Wxh + 10LL * M.nxh;
+  return M;
+}
+
+__device__ inline void zero_mats(const Mol& M, const Bufs& B) {
+  const long long mm = (long long)M.np * M.np;
+  for (long long e = threadIdx.x; e < mm; e += NT) {
+    B.F[e] = 0.0; B.P[e] = 0.0; B.Pn[e] = 0.0; B.Fu[e] = 0.0;
+    B.T[e] = 0.0; B.C[e] = 0.0; B.Cn[e] = 0.0;
+  }
+}
+
+__global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
+  extern __shared__ double lds[];
+  __shared__ double red[NW * (3 * MAXD + 2)];
+  __shared__ Diis D;
+  __shared__ int s_mol;
+  const int tid = threadIdx.x;
+  double* Wbase;
+  const Bufs B = slab(A, Wbase);
+
+  for (;;) {
+    if (tid == 0) s_mol = atomicAdd(A.queue, 1);
+    __syncthreads();
+    const int qi = s_mol;
+    __syncthreads();
+    if (qi >= A.nmol) break;
+    const int mi = A.order[qi];
+    const MolDesc md = A.mols[mi];
+    const Mol M = mol_of(md, Wbase);
+    // Matrices use the molecule's own leading dimension np inside the workgroup's slab; the
+    // padding beyond nao stays zero for the whole solve.
+    zero_mats(M, B);
+    const double ecore = setup_core(A, M, B.H, lds, red);
+    PathResult best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D);
+    const int nocc = M.nocc, n = M.n, np = M.np;
+    const double first_e = best.e_elec;
+    int paths = 1;
+    auto gap = [&]() { return (nocc > 0 && nocc < n) ? B.eps[nocc] - B.eps[nocc - 1] : 1.0e300; };
+    const bool search = A.stability == 2 || (A.stability == 1 && gap() < 6.0);
+    if (search) {
+      // keep the incumbent
+      for (int e = tid; e < np * np; e += NT) {
+        B.Pb[e] = B.P[e];
+        B.Cb[e] = B.C[e];
+      }
+      for (int j = tid; j < n; j += NT) B.epsb[j] = B.eps[j];
+      __syncthreads();
+      const int primary = A.guess == 2 ? 2 : 1;
+      const int other = primary == 2 ? 1 : 2;
+      const int rg[5][2] = {{primary, 0}, {other, A.accel}, {other, 0}, {3, A.accel}, {3, 0}};
+      for (int k = 0; k < 5; ++k) {
+        ++paths;
+        PathResult c = rhf_loop(A, M, B, rg[k][0], rg[k][1], lds, red, D);
+        if (!c.converged) continue;
+        if (!best.converged || c.e_elec < best.e_elec - 1.0e-6) {
+          best = c;
+          for (int e = tid; e < np * np; e += NT) {
+            B.Pb[e] = B.P[e];
+            B.Cb[e] = B.C[e];
+          }
+          for (int j = tid; j < n; j += NT) B.epsb[j] = B.eps[j];
+          __syncthreads();
+        }
+      }
+      paths += 2;  // the two unrestricted rungs of the reference ladder are not run here
+      for (int e = tid; e < np * np; e += NT) {
+        B.P[e] = B.Pb[e];
+        B.C[e] = B.Cb[e];
+      }
+      for (int j = tid; j < n; j += NT) B.eps[j] = B.epsb[j];
+      __syncthreads();
+    }
+    // Mulliken charges and the dipole about the centre of mass (src/scf.rs:762-791,
+    // src/dipole.rs:64-140).
+    double com[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int a = tid; a < M.nat; a += NT) {
+      const int ga = M.atom0 + a;
+      const double ms = c_tab.el[A.aslot[ga]].mass;
+      com[0] += ms * A.ax[ga]; com[1] += ms * A.ay[ga]; com[2] += ms * A.az[ga]; com[3] += ms;
+    }
+    block_sum<4>(com, red);
+    const double cx = com[3] > 0.0 ? com[0] / com[3] : 0.0, cy = com[3] > 0.0 ? com[1] / com[3] : 0.0,
+                 cz = com[3] > 0.0 ? com[2] / com[3] : 0.0;
+    double dip[3] = {0.0, 0.0, 0.0};
+    for (int a = tid; a < M.nat; a += NT) {
+      const int ga = M.atom0 + a;
+      const ElemConst& e = c_tab.el[A.aslot[ga]];
+      const int o = A.aoff[ga];
+      double pop = 0.0;
+      for (int mu = 0; mu < e.norb; ++mu) pop += B.P[(o + mu) * np + o + mu];
+      A.out_q[ga] = e.tore - pop;
+      const double disp[3] = {A.ax[ga] - cx, A.ay[ga] - cy, A.az[ga] - cz};
+      for (int ax = 0; ax < 3; ++ax) {
+        double el = pop * disp[ax];
+        if (e.norb == 4) el += (B.P[o * np + o + ax + 1] + B.P[(o + ax + 1) * np + o]) * e.dd;
+        dip[ax] += disp[ax] * e.tore - el;
+      }
+    }
+    block_sum<3>(dip, red);
+    if (tid == 0) {
+      MolOut o;
+      o.dip[0] = dip[0]; o.dip[1] = dip[1]; o.dip[2] = dip[2];
+      o.e_elec = best.e_elec;
+      o.e_core = ecore;
+      o.last_dp = best.last_dp;
+      o.first_e = first_e;
+      o.homo = nocc > 0 ? B.eps[nocc - 1] : 0.0;
+      o.lumo = nocc < n ? B.eps[nocc] : 0.0;
+      o.iters = best.iters;
+      o.converged = best.converged;
+      o.paths = paths;
+      o.status = best.converged ? 0 : 3;
+      A.out[mi] = o;
+    }
+    if (md.out_mat >= 0) {
+      for (int e = tid; e < n * n; e += NT) {
+        const int i = e / n, j = e % n;
+        A.out_P[md.out_mat + e] = B.P[i * np + j];
+        A.out_C[md.out_mat + e] = B.C[i * np + j];
+      }
+      for (int j = tid; j < n; j += NT) A.out_eps[md.out_vec + j] = B.eps[j];
+    }
+    __syncthreads();
+  }
+}
+
+// Single stages of the pipeline on molecule 0 of a batch, for kernel-level parity checks:
+// mode 0 H_core (+ core-core energy at out[n*n]); mode 1 F(P) for P given in `in`;
+// mode 2 eigenpairs of `in` (cold Jacobi): out = eps[n] then C[n*n];
+// mode 3 out = in[0:n*n] . in[n*n:2n*n] on the matrix cores;
+// mode 4 warm eigensolve of in[0:n*n] in the basis in[n*n:2n*n]: out = eps[n] then C[n*n].
+// Matrices in `in`/`out` are dense n x n row-major in the engine's (heavy-first) AO order.
+__global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const double* in, double* out) {
+  extern __shared__ double lds[];
+  __shared__ double red[NW * (3 * MAXD + 2)];
+  const int tid = threadIdx.x;
+  double* Wbase;
+  const Bufs B = slab(A, Wbase);
+  const Mol M = mol_of(A.mols[A.order[0]], Wbase);
+  const int n = M.n, np = M.np;
+  zero_mats(M, B);
+  __syncthreads();
+  if (mode == 0 || mode == 1) {
+    const double ecore = setup_core(A, M, B.H, lds, red);
+    if (mode == 0) {
+      for (int e = tid; e < n * n; e += NT) out[e] = B.H[(e / n) * np + e % n];
+      if (tid == 0) out[n * n] = ecore;
+    } else {
+      for (int e = tid; e < n * n; e += NT) B.P[(e / n) * np + e % n] = in[e];
+      __syncthreads();
+      build_fock(A, M, B.H, B.P, B.F, lds);
+      for (int e = tid; e < n * n; e += NT) out[e] = B.F[(e / n) * np + e % n];
+    }
+  } else if (mode == 2 || mode == 4) {
+    for (int e = tid; e < n * n; e += NT) {
+      B.Fu[(e / n) * np + e % n] = in[e];
+      if (mode == 4) B.C[(e / n) * np + e % n] = in[n * n + e];
+    }
+    __syncthreads();
+    eigensolve(M, B, B.Fu, mode == 4, lds, red);
+    for (int j = tid; j < n; j += NT) out[j] = B.eps[j];
+    for (int e = tid; e < n * n; e += NT) out[n + e] = B.Cn[(e / n) * np + e % n];
+  } else if (mode == 3) {
+    const double* X = in;
+    const double* Y = in + n * n;
+    wg_gemm(n, n, n, [&](int i, int k) { return X[i * n + k]; }, [&](int k, int j) { return Y[k * n + j]; },
+            [&](int i, int j, double v) { out[i * n + j] = v; });
+  }
+}
+
+#define HIP_OK(x)                                                  \
+  do {                                                             \
+    hipError_t e_ = (x);                                           \
+    if (e_ != hipSuccess) {                                        \
+      err = std::string(#x) + ": " + hipGetErrorString(e_);        \
+      return false;                                                \
+    }                                                              \
+  } while (0)
+
+}  // namespace
+
+namespace pm3host {
+
+// Device-resident batch: inputs uploaded and scratch allocated once; solve() is the launch only.
+struct Batch {
+  int nmol = 0, nsolve = 0, nwg = 0, lds_need = 0;
+  std::vector<MolDesc> desc;
+  std::vector<int> solve;
+  std::vector<std::vector<int>> ao_map, atom_map;  // sorted -> caller indices
+  std::vector<int> pre_status;
+  long long mat_total = 0, vec_total = 0, atoms_total = 0;
+  KArgs args{};
+  void* bufs[16] = {nullptr};
+  int nbufs = 0;
+  ~Batch() {
+    for (int i = 0; i < nbufs; ++i)
+      if (bufs[i]) (void)hipFree(bufs[i]);
+  }
+  template <class T> bool alloc(T*& p, size_t count, std::string& err) {
+    hipError_t e = hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(count, 1));
+    if (e != hipSuccess) {
+      err = std::string("hipMalloc: ") + hipGetErrorString(e);
+      return false;
+    }
+    bufs[nbufs++] = p;
+    return true;
+  }
+};
+
+static bool upload_table(const Params& P, std::string& err) {
+  static int uploaded_device = -1;
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (uploaded_device != dev) {
+    HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &P.table, sizeof(TableConst)));
+    uploaded_device = dev;
+  }
+  return true;
+}
+
+bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt, const RunRequest& req,
+                  Batch*& out_batch, std::string& err) {
+  out_batch = nullptr;
+  if (!upload_table(P, err)) return false;
+  Batch* bp = new Batch();
+  Batch& b = *bp;
+  const int nmol = (int)mols.size();
+  b.nmol = nmol;
+  b.desc.resize(nmol);
+  b.ao_map.resize(nmol);
+  b.atom_map.resize(nmol);
+  b.pre_status.assign(nmol, 0);
+  std::vector<int> aslot, aoff;
+  std::vector<double> ax, ay, az;
+  int np_max = 16;
+  long long w_cap = 1;
+  for (int k = 0; k < nmol; ++k) {
+    const MolIn& m = mols[k];
+    MolDesc& d = b.desc[k];
+    d = MolDesc{};
+    d.out_mat = d.out_vec = -1;
+    d.atom0 = (int)aslot.size();
+    int& st = b.pre_status[k];
+    if (m.n_atoms < 1) { st = 1; continue; }
+    double nel = 0.0;
+    for (int i = 0; i < m.n_atoms && !st; ++i) {
+      const int z = m.z[i];
+      if (z < 0 || z >= 128 || P.slot_of_z[z] < 0) st = 2;
+      else nel += P.table.el[P.slot_of_z[z]].tore;
+    }
+    if (st) continue;
+    nel -= m.charge;
+    const long ne = std::lround(nel);
+    if (std::fabs(nel - (double)ne) > 1.0e-6 || ne < 0) { st = 1; continue; }
+    if (ne % 2 != 0) { st = 4; continue; }
+    bool coincident = false;
+    for (int i = 0; i < m.n_atoms && !coincident; ++i)
+      for (int j = 0; j < i; ++j) {
+        const double dx = m.xyz[3 * i] - m.xyz[3 * j], dy = m.xyz[3 * i + 1] - m.xyz[3 * j + 1], dz = m.xyz[3 * i + 2] - m.xyz[3 * j + 2];
+        if (!(dx * dx + dy * dy + dz * dz > 1.0e-16)) { coincident = true; break; }
+      }
+    if (coincident) { st = 1; continue; }
+    // heavy atoms first (stable), so the pair tables split into XX / XH / HH blocks
+    std::vector<int> ord(m.n_atoms);
+    std::iota(ord.begin(), ord.end(), 0);
+    auto heavy = [&](int i) { return P.table.el[P.slot_of_z[m.z[i]]].norb == 4; };
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return heavy(x) && !heavy(y); });
+    std::vector<int> coff(m.n_atoms + 1, 0);
+    for (int i = 0; i < m.n_atoms; ++i) coff[i + 1] = coff[i] + P.table.el[P.slot_of_z[m.z[i]]].norb;
+    int nao = 0, nh = 0;
+    for (int i : ord) {
+      const int s = P.slot_of_z[m.z[i]];
+      const int no = P.table.el[s].norb;
+      nh += no == 4;
+      aslot.push_back(s);
+      aoff.push_back(nao);
+      ax.push_back(m.xyz[3 * i]);
+      ay.push_back(m.xyz[3 * i + 1]);
+      az.push_back(m.xyz[3 * i + 2]);
+      for (int k2 = 0; k2 < no; ++k2) b.ao_map[k].push_back(coff[i] + k2);
+      nao += no;
+    }
+    b.atom_map[k] = ord;
+    d.nat = m.n_atoms;
+    d.nheavy = nh;
+    d.nao = nao;
+    d.nocc = (int)(ne / 2);
+    d.np = (nao + 15) & ~15;
+    const int npk = nao * (nao + 1) / 2 + 3 * ((nao + 2) / 2);
+    const int lneed = std::max(npk, m.n_atoms * 16) * 8;
+    if (lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || d.nocc > nao) {
+      st = d.nocc > nao ? 1 : 5;
+      aslot.resize(d.atom0); aoff.resize(d.atom0); ax.resize(d.atom0); ay.resize(d.atom0); az.resize(d.atom0);
+      d.nat = 0;
+      continue;
+    }
+    b.lds_need = std::max(b.lds_need, lneed);
+    np_max = std::max(np_max, d.np);
+    const long long nH = d.nat - nh;
+    w_cap = std::max(w_cap, 100LL * nh * (nh - 1) / 2 + 10LL * nh * nH + nH * (nH - 1) / 2);
+    if (req.want_density || req.want_orbitals || req.want_gradient) {
+      d.out_mat = b.mat_total;
+      d.out_vec = b.vec_total;
+      b.mat_total += (long long)nao * nao;
+      b.vec_total += nao;
+    }
+    b.solve.push_back(k);
+  }
+  b.atoms_total = (long long)aslot.size();
+  b.nsolve = (int)b.solve.size();
+  // largest molecules first so the queue ends on short work
+  std::stable_sort(b.solve.begin(), b.solve.end(), [&](int x, int y) { return b.desc[x].nao > b.desc[y].nao; });
+  out_batch = bp;
+  if (b.nsolve == 0) return true;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  HIP_OK(hipGetDevice(&dev));
+  HIP_OK(hipGetDeviceProperties(&prop, dev));
+  b.nwg = std::max(1, std::min(b.nsolve, prop.multiProcessorCount));
+  const long long m2 = (long long)np_max * np_max;
+  const long long stride = ((11 + 3 * MAXD) * m2 + w_cap + 4LL * np_max + 64 + 31) & ~31LL;
+  KArgs& a = b.args;
+  MolDesc* d_desc; int *d_order, *d_queue, *d_aslot, *d_aoff;
+  double *d_ax, *d_ay, *d_az, *d_scr, *d_P, *d_C, *d_eps, *d_q;
+  MolOut* d_out;
+  if (!b.alloc(d_desc, nmol, err) || !b.alloc(d_order, b.nsolve, err) || !b.alloc(d_queue, 1, err) ||
+      !b.alloc(d_aslot, b.atoms_total, err) || !b.alloc(d_aoff, b.atoms_total, err) || !b.alloc(d_ax, b.atoms_total, err) ||
+      !b.alloc(d_ay, b.atoms_total, err) || !b.alloc(d_az, b.atoms_total, err) || !b.alloc(d_scr, (size_t)(stride * b.nwg), err) ||
+      !b.alloc(d_out, nmol, err) || !b.alloc(d_P, b.mat_total, err) || !b.alloc(d_C, b.mat_total, err) ||
+      !b.alloc(d_eps, b.vec_total, err) || !b.alloc(d_q, b.atoms_total, err))
+    return false;
+  HIP_OK(hipMemcpy(d_desc, b.desc.data(), sizeof(MolDesc) * nmol, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_order, b.solve.data(), sizeof(int) * b.nsolve, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_aslot, aslot.data(), sizeof(int) * aslot.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_aoff, aoff.data(), sizeof(int) * aoff.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_ax, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_ay, ay.data(), sizeof(double) * ay.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_az, az.data(), sizeof(double) * az.size(), hipMemcpyHostToDevice));
+  a.mols = d_desc;
+  a.order = d_order;
+  a.nmol = b.nsolve;
+  a.queue = d_queue;
+  a.aslot = d_aslot;
+  a.ax = d_ax;
+  a.ay = d_ay;
+  a.az = d_az;
+  a.aoff = d_aoff;
+  a.scratch = d_scr;
+  a.stride = stride;
+  a.np_max = np_max;
+  a.w_cap = w_cap;
+  a.out = d_out;
+  a.out_P = d_P;
+  a.out_C = d_C;
+  a.out_eps = d_eps;
+  a.out_q = d_q;
+  a.max_scf = opt.max_scf;
+  a.e_tol = opt.e_tol;
+  a.p_tol = opt.p_tol;
+  a.accel = opt.accel;
+  a.adiis_switch = opt.adiis_switch;
+  a.guess = opt.guess;
+  a.stability = opt.stability;
+  a.scf_memory_mb = opt.scf_memory_mb;
+  HIP_OK(hipFuncSetAttribute((const void*)scf_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_need));
+  return true;
+}
+
+bool batch_solve(Batch* bp, std::string& err) {
+  Batch& b = *bp;
+  if (b.nsolve == 0) return true;
+  HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), 0));
+  hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipDeviceSynchronize());
+  return true;
+}
+
+bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std::vector<MolResult>& out, std::string& err) {
+  Batch& b = *bp;
+  out.assign(b.nmol, MolResult());
+  for (int k = 0; k < b.nmol; ++k) out[k].status = b.pre_status[k];
+  if (b.nsolve == 0) return true;
+  std::vector<MolOut> ho(b.nmol);
+  std::vector<double> hq(b.atoms_total);
+  HIP_OK(hipMemcpy(ho.data(), b.args.out, sizeof(MolOut) * b.nmol, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(hq.data(), b.args.out_q, sizeof(double) * b.atoms_total, hipMemcpyDeviceToHost));
+  std::vector<double> hP, hC, hE;
+  if (b.mat_total > 0) {
+    hP.resize(b.mat_total);
+    hC.resize(b.mat_total);
+    hE.resize(b.vec_total);
+    HIP_OK(hipMemcpy(hP.data(), b.args.out_P, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(hC.data(), b.args.out_C, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(hE.data(), b.args.out_eps, sizeof(double) * b.vec_total, hipMemcpyDeviceToHost));
+  }
+  for (int k : b.solve) {
+    MolResult& r = out[k];
+    const MolOut& o = ho[k];
+    const MolDesc& d = b.desc[k];
+    r.status = o.status;
+    r.e_elec = o.e_elec;
+    r.e_core = o.e_core;
+    r.last_change = o.last_dp;
+    r.improvement = std::max(o.first_e - o.e_elec, 0.0);
+    r.homo = o.homo;
+    r.lumo = o.lumo;
+    r.iterations = o.iters;
+    r.converged = o.converged;
+    r.paths = o.paths;
+    r.nao = d.nao;
+    r.nocc = d.nocc;
+    for (int x = 0; x < 3; ++x) r.dipole[x] = o.dip[x] * AU_DIPOLE_TO_DEBYE;
+    r.charges.assign(d.nat, 0.0);
+    for (int t = 0; t < d.nat; ++t) r.charges[b.atom_map[k][t]] = hq[d.atom0 + t];
+    double eis = 0.0, ehs = 0.0;
+    for (int i = 0; i < mols[k].n_atoms; ++i) {
+      const int s = P.slot_of_z[mols[k].z[i]];
+      eis += P.host[s].e_isol;
+      ehs += P.host[s].eheat_ev;
+    }
+    r.hof_kcal = (o.e_elec + o.e_core - eis + ehs) * EV_TO_KCAL;
+    if (d.out_mat >= 0) {
+      const int n = d.nao;
+      const std::vector<int>& mp = b.ao_map[k];
+      r.density.assign((size_t)n * n, 0.0);
+      r.mo_coeff.assign((size_t)n * n, 0.0);
+      r.mo_energies.assign(hE.begin() + d.out_vec, hE.begin() + d.out_vec + n);
+      const double* Ps = hP.data() + d.out_mat;
+      const double* Cs = hC.data() + d.out_mat;
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+          r.density[(size_t)mp[i] * n + mp[j]] = Ps[(size_t)i * n + j];
+          r.mo_coeff[(size_t)mp[i] * n + j] = Cs[(size_t)i * n + j];
+        }
+    }
+  }
+  return true;
+}
+
+void batch_destroy(Batch* b) { delete b; }
+
+bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,
+               int& status, std::string& err) {
+  std::vector<MolIn> mols{mol};
+  EngineOptions opt{1, 0.0, 0.0, 0, 0.1, 1, 0, 512};
+  Batch* b = nullptr;
+  status = 0;
+  if (!batch_create(P, mols, opt, RunRequest(), b, err)) {
+    batch_destroy(b);
+    return false;
+  }
+  if (b->nsolve == 0) {
+    status = b->pre_status[0];
+    batch_destroy(b);
+    return true;
+  }
+  const MolDesc& d = b->desc[0];
+  const int n = d.nao;
+  const std::vector<int>& mp = b->ao_map[0];
+  // modes 0 and 1 speak the caller's AO order; the others take generic matrices as given
+  std::vector<double> sin(in.size());
+  if (mode == 1) {
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) sin[(size_t)i * n + j] = in[(size_t)mp[i] * n + mp[j]];
+  } else {
+    sin = in;
+  }
+  const size_t nout = (size_t)n * n + n + 1;
+  double *d_in = nullptr, *d_out = nullptr;
+  bool ok = b->alloc(d_in, std::max<size_t>(sin.size(), 1), err) && b->alloc(d_out, nout, err);
+  if (ok && !sin.empty()) {
+    hipError_t e = hipMemcpy(d_in, sin.data(), sizeof(double) * sin.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { err = hipGetErrorString(e); ok = false; }
+  }
+  if (ok) {
+    hipLaunchKernelGGL(stage_kernel, dim3(1), dim3(NT), b->lds_need, 0, b->args, mode, d_in, d_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    std::vector<double> so(nout);
+    if (e == hipSuccess) e = hipMemcpy(so.data(), d_out, sizeof(double) * nout, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { err = hipGetErrorString(e); ok = false; }
+    if (ok) {
+      if (mode == 0 || mode == 1) {
+        out.assign((size_t)n * n + (mode == 0 ? 1 : 0), 0.0);
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j < n; ++j) out[(size_t)mp[i] * n + mp[j]] = so[(size_t)i * n + j];
+        if (mode == 0) out[(size_t)n * n] = so[(size_t)n * n];
+      } else {
+        out.assign(so.begin(), so.begin() + (size_t)n * n + (mode == 3 ? 0 : n));
+      }
+    }
+  }
+  batch_destroy(b);
+  return ok;
+}
+
+bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt, const RunRequest& req,
+                std::vector<MolResult>& out, std::string& err) {
+  Batch* b = nullptr;
+  bool ok = batch_create(P, mols, opt, req, b, err) && batch_solve(b, err) && batch_fetch(b, P, mols, out, err);
+  batch_destroy(b);
+  return ok;
+}
+
+}  // namespace pm3host

@@ -1,0 +1,80 @@
+// Host-side declarations shared by the C ABI (pm3_api.cpp) and the engine (pm3_engine.hip).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "nddo_device.hpp"
+
+namespace pm3host {
+
+constexpr double EV_TO_KCAL = 23.06054783061903;     // src/constants.rs:27
+constexpr double AU_DIPOLE_TO_DEBYE = 2.541746473;   // src/constants.rs:56
+
+struct ElemHost {
+  double e_isol = 0.0, eheat_ev = 0.0, mass = 0.0;
+};
+
+struct Params {
+  pm3dev::TableConst table;
+  ElemHost host[pm3dev::MAX_SLOTS];
+  int slot_of_z[128];
+  int nslots = 0;
+};
+
+bool build_params(Params& P);
+
+// Options as the engine consumes them (mirrors pm3_options).
+struct EngineOptions {
+  int max_scf;
+  double e_tol, p_tol;
+  int accel;        // 0 none, 1 CDIIS, 2 A-DIIS -> CDIIS (already combined with use_diis)
+  double adiis_switch;
+  int guess;        // 1 SAD, 2 core, 3 symmetry-broken
+  int stability;    // 0 off, 1 auto, 2 always
+  int scf_memory_mb;
+};
+
+// One molecule as handed to the engine, atoms in caller order.
+struct MolIn {
+  int n_atoms;
+  const int32_t* z;
+  const double* xyz;
+  double charge;
+};
+
+// Per-molecule engine output.  Matrices and vectors are in the caller's AO / atom order.
+struct MolResult {
+  int status = 0;           // pm3_status
+  double e_elec = 0, e_core = 0, hof_kcal = 0, last_change = 0, improvement = 0;
+  double homo = 0, lumo = 0;
+  double dipole[3] = {0, 0, 0};  // Debye
+  int iterations = 0, converged = 0, paths = 0, nocc = 0, nao = 0;
+  std::vector<double> charges, density, mo_energies, mo_coeff, gradient;
+};
+
+struct RunRequest {
+  bool want_density = false;
+  bool want_orbitals = false;
+  bool want_gradient = false;
+};
+
+// A batch whose inputs and scratch live on the device; batch_solve is the launch alone.
+struct Batch;
+bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt, const RunRequest& req,
+                  Batch*& out, std::string& err);
+bool batch_solve(Batch* b, std::string& err);
+bool batch_fetch(Batch* b, const Params& P, const std::vector<MolIn>& mols, std::vector<MolResult>& out,
+                 std::string& err);
+void batch_destroy(Batch* b);
+
+// One stage of the pipeline on a single molecule (kernel-level parity hooks); see stage_kernel.
+bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,
+               int& status, std::string& err);
+
+// create + solve + fetch.  Returns false (and sets err) only for a device/launch failure;
+// per-molecule failures are in out[k].status.
+bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt, const RunRequest& req,
+                std::vector<MolResult>& out, std::string& err);
+
+}  // namespace pm3host

@@ -1,0 +1,354 @@
+"""Python mirror of the reference's molecular PM3 interface, backed by libpm3gpu.so (MI355X).
+
+Names, argument meaning and error behaviour follow the reference crate:
+
+* ``Molecule`` (src/system.rs:28-40; positions in Bohr, ``from_xyz_str`` converts from Angstrom),
+* ``Pm3Parameters.standard()`` (src/params.rs:135),
+* ``Pm3Options`` with the ``Pm3Options::default()`` values (src/scf.rs:259-287),
+* ``run_pm3(molecule, params, options) -> Pm3Result`` (src/scf.rs:450-848),
+* ``closed_form_gradient(...) -> GradientResult`` (src/gradient.rs:201-276),
+* errors as ``Pm3Error`` subclasses mirroring ``Pm3Error`` (src/error.rs:12-47); non-convergence raises
+  ``ScfNotConverged`` rather than returning ``converged=False`` (src/scf.rs:754-760).
+
+Every call runs on the GPU through the C ABI in include/pm3_gpu.h; there is no CPU fallback.  A
+missing or unloadable library raises at import of the native handle.
+"""
+
+import ctypes
+import dataclasses
+import os
+from typing import List, Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libpm3gpu.so")
+
+ANGSTROM_TO_BOHR = 1.0 / 0.529177210903   # src/constants.rs:33
+EV_TO_KCAL = 23.06054783061903           # src/constants.rs:27
+
+SYMBOLS = {"H": 1, "Li": 3, "Be": 4, "B": 5, "C": 6, "N": 7, "O": 8, "F": 9, "Na": 11, "Mg": 12, "Al": 13,
+           "Si": 14, "P": 15, "S": 16, "Cl": 17, "He": 2, "Ne": 10, "Ar": 18, "K": 19, "Ca": 20}
+
+
+class Pm3Error(RuntimeError):
+    status = -1
+
+
+class InvalidInput(Pm3Error):
+    status = 1
+
+
+class MissingElement(Pm3Error):
+    status = 2
+
+
+class ScfNotConverged(Pm3Error):
+    status = 3
+
+    def __init__(self, msg, iterations=0, error=float("nan")):
+        super().__init__(msg)
+        self.iterations = iterations
+        self.error = error
+
+
+class Unsupported(Pm3Error):
+    status = 4
+
+
+class ResourceLimit(Pm3Error):
+    status = 5
+
+
+class DeviceError(Pm3Error):
+    status = 6
+
+
+_ERRORS = {1: InvalidInput, 2: MissingElement, 3: ScfNotConverged, 4: Unsupported, 5: ResourceLimit, 6: DeviceError}
+
+ACCEL = {"none": 0, "cdiis": 1, "adiis_cdiis": 2}
+GUESS = {"auto": 0, "sad": 1, "core": 2, "symmetry_broken": 3, "learned": 4}
+STABILITY = {"off": 0, "auto": 1, "always": 2}
+
+
+class _COptions(ctypes.Structure):
+    _fields_ = [("max_scf", ctypes.c_int32), ("e_tol", ctypes.c_double), ("p_tol", ctypes.c_double),
+                ("use_diis", ctypes.c_int32), ("accelerator", ctypes.c_int32), ("adiis_switch", ctypes.c_double),
+                ("guess", ctypes.c_int32), ("stability", ctypes.c_int32), ("scf_memory_mb", ctypes.c_int32)]
+
+
+class _CMolecule(ctypes.Structure):
+    _fields_ = [("n_atoms", ctypes.c_int32), ("z", ctypes.POINTER(ctypes.c_int32)),
+                ("xyz_bohr", ctypes.POINTER(ctypes.c_double)), ("charge", ctypes.c_double),
+                ("multiplicity", ctypes.c_int32)]
+
+
+class _CResult(ctypes.Structure):
+    _fields_ = [("total_ev", ctypes.c_double), ("electronic_ev", ctypes.c_double), ("core_ev", ctypes.c_double),
+                ("heat_of_formation_kcal", ctypes.c_double), ("dipole_debye", ctypes.c_double * 3),
+                ("dipole_magnitude", ctypes.c_double), ("homo_ev", ctypes.c_double), ("lumo_ev", ctypes.c_double),
+                ("last_change", ctypes.c_double), ("scf_improvement_ev", ctypes.c_double),
+                ("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("scf_paths_tried", ctypes.c_int32),
+                ("n_occ", ctypes.c_int32), ("nao", ctypes.c_int32), ("has_homo", ctypes.c_int32),
+                ("has_lumo", ctypes.c_int32)]
+
+
+EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_run", "pm3_gradient",
+            "pm3_run_batch", "pm3_batch_create", "pm3_batch_solve", "pm3_batch_results", "pm3_batch_destroy",
+            "pm3_last_error_message"]
+
+_lib = None
+
+
+def native():
+    """The loaded libpm3gpu.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libpm3gpu.so not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        dp, ip = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
+        L.pm3_params_standard.restype = ctypes.c_void_p
+        L.pm3_options_default.argtypes = [ctypes.POINTER(_COptions)]
+        L.pm3_query_nao.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p]
+        L.pm3_run.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
+                              ctypes.POINTER(_CResult), dp, dp, dp, dp]
+        L.pm3_gradient.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
+                                   ctypes.POINTER(_CResult), dp, dp]
+        L.pm3_run_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ctypes.c_void_p, ctypes.POINTER(_COptions),
+                                    ctypes.c_int32, ip, ctypes.POINTER(_CResult), dp]
+        L.pm3_batch_create.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ctypes.c_void_p, ctypes.POINTER(_COptions),
+                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+        L.pm3_batch_solve.argtypes = [ctypes.c_void_p]
+        L.pm3_batch_results.argtypes = [ctypes.c_void_p, ip, ctypes.POINTER(_CResult), dp, dp]
+        L.pm3_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.pm3_last_error_message.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _raise(status, res=None):
+    if status == 0:
+        return
+    msg = native().pm3_last_error_message().decode()
+    cls = _ERRORS.get(status, Pm3Error)
+    if cls is ScfNotConverged and res is not None:
+        raise ScfNotConverged(msg, res.iterations, res.last_change)
+    raise cls(msg)
+
+
+@dataclasses.dataclass
+class Molecule:
+    """Atoms with positions in Bohr (src/system.rs:21-40)."""
+    z: np.ndarray
+    positions: np.ndarray
+    charge: float = 0.0
+    multiplicity: int = 1
+
+    def __post_init__(self):
+        self.z = np.ascontiguousarray(self.z, dtype=np.int32)
+        self.positions = np.ascontiguousarray(self.positions, dtype=np.float64).reshape(-1, 3)
+
+    @classmethod
+    def from_xyz_str(cls, text, charge=0.0):
+        """Parse an XYZ block in Angstrom (src/system.rs:90-139)."""
+        lines = text.lstrip("﻿").splitlines()
+        try:
+            n = int(lines[0].strip())
+        except (IndexError, ValueError):
+            raise InvalidInput(f"invalid XYZ atom count: {lines[0] if lines else ''}")
+        if len(lines) < n + 2:
+            raise InvalidInput("XYZ ended before all atoms")
+        z, pos = [], []
+        for i in range(n):
+            parts = lines[i + 2].split()
+            if len(parts) < 4:
+                raise InvalidInput(f"XYZ line {i + 3} has fewer than 4 fields")
+            if parts[0] not in SYMBOLS:
+                raise InvalidInput(f"unknown element on line {i + 3}: {parts[0]}")
+            z.append(SYMBOLS[parts[0]])
+            pos.append([float(v) * ANGSTROM_TO_BOHR for v in parts[1:4]])
+        return cls(np.array(z), np.array(pos), charge)
+
+    def _c(self):
+        m = _CMolecule(len(self.z), self.z.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                       self.positions.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(self.charge),
+                       int(self.multiplicity))
+        return m
+
+
+class Pm3Parameters:
+    """Handle on the embedded PM3 table (Pm3Parameters::standard(), src/params.rs:135)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def standard(cls):
+        h = native().pm3_params_standard()
+        if not h:
+            raise Pm3Error(native().pm3_last_error_message().decode())
+        return cls(h)
+
+
+@dataclasses.dataclass
+class Pm3Options:
+    """Pm3Options::default() (src/scf.rs:259-287) for the fields the molecular RHF path reads."""
+    max_scf: int = 200
+    e_tol: float = 1.0e-8
+    p_tol: float = 1.0e-7
+    use_diis: bool = True
+    accelerator: str = "adiis_cdiis"
+    adiis_switch: float = 0.1
+    guess: str = "auto"
+    stability: str = "auto"
+    scf_memory_mb: int = 512
+
+    def _c(self):
+        return _COptions(self.max_scf, self.e_tol, self.p_tol, int(self.use_diis), ACCEL[self.accelerator],
+                         self.adiis_switch, GUESS[self.guess], STABILITY[self.stability], self.scf_memory_mb)
+
+
+@dataclasses.dataclass
+class Pm3Result:
+    """Pm3Result (src/scf.rs:352-396) for a restricted solution."""
+    density: Optional[np.ndarray]
+    mo_energies: Optional[np.ndarray]
+    mo_coeff: Optional[np.ndarray]
+    n_occ: int
+    electronic_ev: float
+    core_ev: float
+    total_ev: float
+    heat_of_formation_kcal: float
+    charges: np.ndarray
+    dipole_debye: np.ndarray
+    dipole_magnitude: float
+    homo_ev: Optional[float]
+    lumo_ev: Optional[float]
+    iterations: int
+    converged: bool
+    scf_paths_tried: int
+    scf_improvement_ev: float
+    unrestricted: bool = False
+
+
+@dataclasses.dataclass
+class GradientResult:
+    """GradientResult (src/gradient.rs:122-134): gradient in eV/Bohr, forces = -gradient."""
+    scf: Pm3Result
+    energy_ev: float
+    gradient: np.ndarray
+    forces: np.ndarray
+    max_gradient: float
+
+
+def _result(r, P, eps, C, q):
+    return Pm3Result(P, eps, C, r.n_occ, r.electronic_ev, r.core_ev, r.total_ev, r.heat_of_formation_kcal, q,
+                     np.array(r.dipole_debye[:]), r.dipole_magnitude, r.homo_ev if r.has_homo else None,
+                     r.lumo_ev if r.has_lumo else None, r.iterations, bool(r.converged), r.scf_paths_tried,
+                     r.scf_improvement_ev)
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def run_pm3(molecule: Molecule, params: Optional[Pm3Parameters] = None, options: Optional[Pm3Options] = None,
+            want_orbitals=True) -> Pm3Result:
+    """RHF PM3 single point on the GPU (run_pm3, src/scf.rs:450)."""
+    params = params or Pm3Parameters.standard()
+    options = options or Pm3Options()
+    cm = molecule._c()
+    nao = native().pm3_query_nao(ctypes.byref(cm), params._h)
+    if nao < 0:
+        bad = [int(z) for z in molecule.z if int(z) not in SYMBOLS.values() or int(z) in (2, 10, 18, 19, 20)]
+        raise MissingElement(f"no PM3 parameters in this engine for Z={bad[0] if bad else '?'}")
+    res = _CResult()
+    P = np.zeros((nao, nao))
+    q = np.zeros(len(molecule.z))
+    eps = np.zeros(nao) if want_orbitals else None
+    C = np.zeros((nao, nao)) if want_orbitals else None
+    null = ctypes.POINTER(ctypes.c_double)()
+    st = native().pm3_run(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(res), _dp(P),
+                          _dp(eps) if want_orbitals else null, _dp(C) if want_orbitals else null, _dp(q))
+    _raise(st, res)
+    return _result(res, P, eps, C, q)
+
+
+def closed_form_gradient(molecule: Molecule, params: Optional[Pm3Parameters] = None,
+                         options: Optional[Pm3Options] = None) -> GradientResult:
+    """SCF plus the analytic Hellmann-Feynman gradient on the GPU (src/gradient.rs:201)."""
+    params = params or Pm3Parameters.standard()
+    options = options or Pm3Options()
+    cm = molecule._c()
+    nao = native().pm3_query_nao(ctypes.byref(cm), params._h)
+    if nao < 0:
+        raise MissingElement("no PM3 parameters in this engine for an element of the molecule")
+    res = _CResult()
+    g = np.zeros((len(molecule.z), 3))
+    P = np.zeros((nao, nao))
+    st = native().pm3_gradient(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(res), _dp(g),
+                               _dp(P))
+    _raise(st, res)
+    scf = _result(res, P, None, None, np.zeros(len(molecule.z)))
+    return GradientResult(scf, res.total_ev, g, -g, float(np.abs(g).max()) if g.size else 0.0)
+
+
+class Batch:
+    """Device-resident batch of independent molecules (pm3_batch_* in include/pm3_gpu.h)."""
+
+    def __init__(self, molecules: List[Molecule], params: Optional[Pm3Parameters] = None,
+                 options: Optional[Pm3Options] = None, want_density=False):
+        self.params = params or Pm3Parameters.standard()
+        self.options = options or Pm3Options()
+        self.molecules = molecules
+        ptr = np.zeros(len(molecules) + 1, dtype=np.int32)
+        for k, m in enumerate(molecules):
+            ptr[k + 1] = ptr[k] + len(m.z)
+        self.atom_ptr = ptr
+        self._z = np.ascontiguousarray(np.concatenate([m.z for m in molecules]).astype(np.int32))
+        self._xyz = np.ascontiguousarray(np.concatenate([m.positions for m in molecules]))
+        self._q = np.ascontiguousarray([float(m.charge) for m in molecules], dtype=np.float64)
+        h = ctypes.c_void_p()
+        ip = ctypes.POINTER(ctypes.c_int32)
+        st = native().pm3_batch_create(len(molecules), ptr.ctypes.data_as(ip), self._z.ctypes.data_as(ip),
+                                       _dp(self._xyz), _dp(self._q), self.params._h, ctypes.byref(self.options._c()),
+                                       1 if want_density else 0, ctypes.byref(h))
+        _raise(st)
+        self._h = h
+        self.want_density = want_density
+
+    def solve(self):
+        _raise(native().pm3_batch_solve(self._h))
+
+    def results(self):
+        """(status array, list of Pm3Result-like objects with density when requested)."""
+        k = len(self.molecules)
+        status = np.zeros(k, dtype=np.int32)
+        res = (_CResult * k)()
+        q = np.zeros(int(self.atom_ptr[-1]))
+        nao = [native().pm3_query_nao(ctypes.byref(m._c()), self.params._h) for m in self.molecules]
+        total = sum(max(n, 0) ** 2 for n in nao)
+        P = np.zeros(max(total, 1)) if self.want_density else None
+        null = ctypes.POINTER(ctypes.c_double)()
+        _raise(native().pm3_batch_results(self._h, status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), res, _dp(q),
+                                          _dp(P) if self.want_density else null))
+        out, moff = [], 0
+        for i in range(k):
+            a0, a1 = self.atom_ptr[i], self.atom_ptr[i + 1]
+            n = max(nao[i], 0)
+            Pi = P[moff:moff + n * n].reshape(n, n) if self.want_density else None
+            moff += n * n
+            out.append(_result(res[i], Pi, None, None, q[a0:a1].copy()))
+        return status, out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            native().pm3_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
