@@ -123,6 +123,8 @@ pm3_status pm3_batch_solve(pm3_batch* batch);
 pm3_status pm3_batch_results(pm3_batch* batch, pm3_status* status, pm3_result* results, double* charges,
                              double* density);
 void pm3_batch_destroy(pm3_batch* batch);
+/* Device time (ms, HIP events on the launch stream) of the last pm3_batch_solve's kernel. */
+double pm3_batch_last_kernel_ms(const pm3_batch* batch);
 
 /* Detail text of the calling thread's last error ("" if none). */
 const char* pm3_last_error_message(void);

@@ -238,6 +238,8 @@ pm3_status pm3_batch_results(pm3_batch* h, pm3_status* status, pm3_result* resul
   return PM3_OK;
 }
 
+double pm3_batch_last_kernel_ms(const pm3_batch* h) { return h ? batch_kernel_ms(h->b) : 0.0; }
+
 void pm3_batch_destroy(pm3_batch* h) {
   if (!h) return;
   batch_destroy(h->b);

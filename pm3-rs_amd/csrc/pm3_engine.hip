@@ -1146,11 +1146,15 @@ struct Batch {
   std::vector<int> pre_status;
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
-  void* bufs[16] = {nullptr};
+  void* bufs[20] = {nullptr};
   int nbufs = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_kernel_ms = 0.0;
   ~Batch() {
     for (int i = 0; i < nbufs; ++i)
       if (bufs[i]) (void)hipFree(bufs[i]);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
   }
   template <class T> bool alloc(T*& p, size_t count, std::string& err) {
     hipError_t e = hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(count, 1));
@@ -1324,13 +1328,25 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
 
 bool batch_solve(Batch* bp, std::string& err) {
   Batch& b = *bp;
+  b.last_kernel_ms = 0.0;
   if (b.nsolve == 0) return true;
+  if (!b.ev0) {
+    HIP_OK(hipEventCreate(&b.ev0));
+    HIP_OK(hipEventCreate(&b.ev1));
+  }
   HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), 0));
+  HIP_OK(hipEventRecord(b.ev0, 0));
   hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipDeviceSynchronize());
+  HIP_OK(hipEventRecord(b.ev1, 0));
+  HIP_OK(hipEventSynchronize(b.ev1));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, b.ev0, b.ev1));
+  b.last_kernel_ms = ms;
   return true;
 }
+
+double batch_kernel_ms(const Batch* b) { return b ? b->last_kernel_ms : 0.0; }
 
 bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std::vector<MolResult>& out, std::string& err) {
   Batch& b = *bp;

@@ -67,6 +67,7 @@ bool batch_solve(Batch* b, std::string& err);
 bool batch_fetch(Batch* b, const Params& P, const std::vector<MolIn>& mols, std::vector<MolResult>& out,
                  std::string& err);
 void batch_destroy(Batch* b);
+double batch_kernel_ms(const Batch* b);  // device time of the last solve's launch (HIP events)
 
 // One stage of the pipeline on a single molecule (kernel-level parity hooks); see stage_kernel.
 bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,

@@ -95,6 +95,8 @@ class _CResult(ctypes.Structure):
 
 EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_run", "pm3_gradient",
             "pm3_run_batch", "pm3_batch_create", "pm3_batch_solve", "pm3_batch_results", "pm3_batch_destroy",
+            "pm3_batch_last_kernel_ms", "pm3_stage_hcore", "pm3_stage_fock", "pm3_stage_eigen", "pm3_stage_eigen_warm",
+            "pm3_stage_gemm",
             "pm3_last_error_message"]
 
 _lib = None
@@ -122,6 +124,8 @@ def native():
         L.pm3_batch_solve.argtypes = [ctypes.c_void_p]
         L.pm3_batch_results.argtypes = [ctypes.c_void_p, ip, ctypes.POINTER(_CResult), dp, dp]
         L.pm3_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.pm3_batch_last_kernel_ms.argtypes = [ctypes.c_void_p]
+        L.pm3_batch_last_kernel_ms.restype = ctypes.c_double
         L.pm3_last_error_message.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -320,6 +324,10 @@ class Batch:
 
     def solve(self):
         _raise(native().pm3_batch_solve(self._h))
+
+    def last_kernel_ms(self):
+        """Device time of the last solve's persistent launch (HIP events on its stream)."""
+        return native().pm3_batch_last_kernel_ms(self._h)
 
     def results(self):
         """(status array, list of Pm3Result-like objects with density when requested)."""

@@ -1,0 +1,199 @@
+"""The HIP engine against the CPU oracle on the same inputs: |dHf| <= 1e-5 kcal/mol and
+max|dP| <= 1e-8 (both sides converged to p_tol 1e-10 for the density check), stage-level
+kernels element-wise, the batched launch, and the reference's error surface."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import ANGSTROM_TO_BOHR, atoms_to_zx
+
+pytestmark = pytest.mark.gpu
+
+HOF_TOL = 1e-5
+P_TOL = 1e-8
+TIGHT = dict(p_tol=1e-10, e_tol=1e-12)
+
+
+@pytest.fixture(scope="module")
+def G():
+    import pm3amd
+    pm3amd.native()
+    return pm3amd
+
+
+def _rows(pins, names):
+    return [r for r in pins["mopac_closed_shell"] if r["name"] in names]
+
+
+PARITY_ROWS = ["water", "methane", "ethane", "benzene", "formaldehyde", "acetic_acid", "methylamine", "nitromethane",
+               "hydroxide", "ammonium", "chloroform", "sulfur_dioxide", "lithium_fluoride", "silicon_tetrafluoride",
+               "aluminium_trichloride", "dimethyl_sulfoxide", "ketene", "butyne"]
+
+
+def test_single_point_parity(G, pins, oracle):
+    for row in _rows(pins, PARITY_ROWS):
+        z, x = atoms_to_zx(row["atoms"])
+        print("case", row["name"], flush=True)
+        m = G.Molecule(z, x, row["charge"])
+        r = G.run_pm3(m, options=G.Pm3Options(max_scf=2000, **TIGHT))
+        st, o = oracle.run(z, x, row["charge"], oracle.options(max_scf=2000, **TIGHT))
+        assert st == 0 and r.converged
+        assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL, row["name"]
+        assert np.abs(r.density - o["density"]).max() <= P_TOL, row["name"]
+        assert np.abs(r.charges - o["charges"]).max() <= 1e-8
+        assert np.abs(r.dipole_debye - o["dipole"]).max() <= 1e-7
+        assert abs(r.homo_ev - o["homo_ev"]) <= 1e-7 and abs(r.lumo_ev - o["lumo_ev"]) <= 1e-7
+        assert r.scf_paths_tried == o["scf_paths_tried"]
+        # and against MOPAC itself with the reference's tolerance
+        assert abs(r.heat_of_formation_kcal - row["hof_kcal"]) < 2e-3
+
+
+def test_default_options_follow_the_oracle_path(G, oracle):
+    # default p_tol 1e-7: same iteration count and HoF as the restatement
+    z, x = atoms_to_zx([["C", 0, 0, 0], ["O", 0, 0, 1.21], ["H", 0.94, 0, -0.54], ["H", -0.94, 0, -0.54]])
+    r = G.run_pm3(G.Molecule(z, x))
+    st, o = oracle.run(z, x)
+    assert r.iterations == o["iterations"]
+    assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL
+    assert abs(r.total_ev - (r.electronic_ev + r.core_ev)) < 1e-12
+
+
+def test_orbitals(G, oracle):
+    z, x = atoms_to_zx([["O", 0, 0, 0], ["H", 0.9584, 0, 0], ["H", -0.24, 0.9278, 0]])
+    r = G.run_pm3(G.Molecule(z, x), options=G.Pm3Options(**TIGHT))
+    st, o = oracle.run(z, x, 0.0, oracle.options(**TIGHT))
+    n = r.mo_coeff.shape[0]
+    assert np.all(np.diff(r.mo_energies) >= 0)
+    assert np.abs(r.mo_coeff.T @ r.mo_coeff - np.eye(n)).max() < 1e-12
+    assert np.abs(r.mo_energies - o["mo_energies"]).max() < 1e-7
+    Cocc = r.mo_coeff[:, : r.n_occ]
+    assert np.abs(2 * Cocc @ Cocc.T - r.density).max() < 1e-12
+
+
+def _lib(G):
+    L = G.native()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.pm3_stage_hcore.argtypes = [ctypes.POINTER(G._CMolecule), ctypes.c_void_p, dp, dp]
+    L.pm3_stage_fock.argtypes = [ctypes.POINTER(G._CMolecule), ctypes.c_void_p, dp, dp]
+    L.pm3_stage_eigen.argtypes = [ctypes.c_int32, dp, dp, dp]
+    L.pm3_stage_eigen_warm.argtypes = [ctypes.c_int32, dp, dp, dp, dp]
+    L.pm3_stage_gemm.argtypes = [ctypes.c_int32, dp, dp, dp]
+    return L, dp
+
+
+def test_stage_hcore_and_fock(G, oracle):
+    from pm3amd import workloads as W
+    L, dp = _lib(G)
+    prm = G.Pm3Parameters.standard()
+    rng = np.random.default_rng(1)
+    mols = [(np.array(z), W.to_bohr(x)) for z, x in W.mixed_batch(6, 4)]
+    mols.append(atoms_to_zx([["Na", 0, 0, 0], ["Cl", 2.4, 0.1, 0], ["S", -1.2, 1.9, 0.4], ["H", -2.3, 2.2, 0.6],
+                             ["P", 1.0, -2.2, 0.3], ["F", 3.6, -0.4, 0.2], ["B", 2.8, 2.3, -0.7]]))
+    for z, x in mols:
+        m = G.Molecule(z, x)
+        n = oracle.nao(z)
+        H = np.zeros((n, n))
+        ec = np.zeros(1)
+        assert L.pm3_stage_hcore(ctypes.byref(m._c()), prm._h, H.ctypes.data_as(dp), ec.ctypes.data_as(dp)) == 0
+        assert np.abs(H - oracle.hcore(z, x)).max() < 1e-11
+        assert abs(ec[0] - oracle.core_core(z, x)) < 1e-9
+        P = rng.standard_normal((n, n))
+        P = P + P.T
+        F = np.zeros((n, n))
+        assert L.pm3_stage_fock(ctypes.byref(m._c()), prm._h, P.ctypes.data_as(dp), F.ctypes.data_as(dp)) == 0
+        assert np.abs(F - oracle.fock(z, x, P)).max() < 1e-10
+        assert np.abs(F - F.T).max() < 1e-12
+
+
+def test_stage_eigen_and_gemm(G, oracle):
+    L, dp = _lib(G)
+    rng = np.random.default_rng(2)
+    for n in (1, 2, 7, 16, 33, 100, 190):
+        A = rng.standard_normal((n, n))
+        B = rng.standard_normal((n, n))
+        C = np.zeros((n, n))
+        assert L.pm3_stage_gemm(n, A.ctypes.data_as(dp), B.ctypes.data_as(dp), C.ctypes.data_as(dp)) == 0
+        assert np.abs(C - A @ B).max() < 1e-12 * n
+        S = A + A.T
+        w = np.zeros(n)
+        V = np.zeros((n, n))
+        assert L.pm3_stage_eigen(n, S.ctypes.data_as(dp), w.ctypes.data_as(dp), V.ctypes.data_as(dp)) == 0
+        wo, _ = oracle.eigen(S)
+        assert np.abs(w - wo).max() < 1e-11 * max(1, n)
+        assert np.abs(V @ np.diag(w) @ V.T - S).max() < 1e-11 * max(1, n)
+        assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12 * max(1, n)
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        Q = np.ascontiguousarray(Q)
+        assert L.pm3_stage_eigen_warm(n, S.ctypes.data_as(dp), Q.ctypes.data_as(dp), w.ctypes.data_as(dp),
+                                      V.ctypes.data_as(dp)) == 0
+        assert np.abs(w - wo).max() < 1e-11 * max(1, n)
+        assert np.abs(V @ np.diag(w) @ V.T - S).max() < 1e-11 * max(1, n)
+    # degenerate spectrum: the projector onto a degenerate block is what must be right
+    w0 = np.array([-3.0, -3.0, -3.0, 1.0, 2.0, 2.0])
+    Q, _ = np.linalg.qr(rng.standard_normal((6, 6)))
+    S = Q @ np.diag(w0) @ Q.T
+    S = np.ascontiguousarray((S + S.T) / 2)
+    w = np.zeros(6)
+    V = np.zeros((6, 6))
+    L.pm3_stage_eigen(6, S.ctypes.data_as(dp), w.ctypes.data_as(dp), V.ctypes.data_as(dp))
+    assert np.abs(w - w0).max() < 1e-12
+    Pv = V[:, :3] @ V[:, :3].T
+    Pq = Q[:, :3] @ Q[:, :3].T
+    assert np.abs(Pv - Pq).max() < 1e-12
+
+
+def test_batch_matches_single_runs_and_oracle(G, oracle):
+    from pm3amd import workloads as W
+    mols = [(np.array(z), W.to_bohr(x)) for z, x in W.mixed_batch(96, 21)]
+    b = G.Batch([G.Molecule(z, x) for z, x in mols], options=G.Pm3Options(**TIGHT), want_density=True)
+    b.solve()
+    st, res = b.results()
+    assert (st == 0).all()
+    tot, hof, it, sto = oracle.run_batch(mols, oracle.options(**TIGHT))
+    dh = np.array([r.heat_of_formation_kcal for r in res]) - hof
+    assert np.abs(dh).max() <= HOF_TOL
+    for k in (0, 1, 50, 95):  # batch order and neighbours do not change a molecule's answer
+        r1 = G.run_pm3(G.Molecule(*mols[k]), options=G.Pm3Options(**TIGHT))
+        assert np.abs(r1.density - res[k].density).max() < 1e-12
+        assert abs(r1.heat_of_formation_kcal - res[k].heat_of_formation_kcal) < 1e-9
+    # repeated solves of the same device-resident batch give the same answers
+    b.solve()
+    st2, res2 = b.results()
+    assert max(abs(a.heat_of_formation_kcal - c.heat_of_formation_kcal) for a, c in zip(res, res2)) == 0.0
+
+
+def test_largest_single_molecules(G, oracle):
+    """The 32-water cluster (192 AOs) and a solvated C12H26 conformer (194 AOs) against the oracle,
+    plus idempotency and trace of the converged density."""
+    from pm3amd import workloads as W
+    for z, x in [W.water_cluster(32, 10.0, 2026), W.solvated_conformers(1, 2026)[0]]:
+        z = np.asarray(z)
+        x = W.to_bohr(x)
+        print("case nat", len(z), flush=True)
+        r = G.run_pm3(G.Molecule(z, x), options=G.Pm3Options(**TIGHT))
+        st, o = oracle.run(z, x, 0.0, oracle.options(**TIGHT))
+        assert st == 0
+        assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL
+        assert np.abs(r.density - o["density"]).max() <= P_TOL
+        P = r.density
+        assert abs(np.trace(P) - 2 * r.n_occ) < 1e-9
+        assert np.abs(P @ P - 2 * P).max() < 1e-9
+
+
+def test_error_surface(G):
+    z, x = atoms_to_zx([["O", 0, 0, 0], ["H", 0.9584, 0, 0], ["H", -0.24, 0.9278, 0]])
+    with pytest.raises(G.ScfNotConverged) as e:
+        G.run_pm3(G.Molecule(z, x), options=G.Pm3Options(max_scf=1))
+    assert e.value.iterations == 1
+    with pytest.raises(G.MissingElement):
+        G.run_pm3(G.Molecule(np.array([18, 1]), np.array([[0, 0, 0], [2.5, 0, 0]])))
+    with pytest.raises(G.InvalidInput):
+        G.run_pm3(G.Molecule(np.array([1, 1]), np.zeros((2, 3))))
+    with pytest.raises(G.Unsupported):  # odd electron count needs the unrestricted path
+        G.run_pm3(G.Molecule(np.array([6, 1, 1, 1]), np.array([[0, 0, 0], [2, 0, 0], [-1, 1.7, 0], [-1, -1.7, 0]])))
+    # a batch reports per-molecule failures without failing the launch
+    b = G.Batch([G.Molecule(z, x), G.Molecule(np.array([1, 1]), np.zeros((2, 3))), G.Molecule(z, x, -1.0)])
+    b.solve()
+    st, res = b.results()
+    assert list(st) == [0, 1, 4]

@@ -1,0 +1,72 @@
+"""The C-ABI library loads and exports every entry point the headers declare (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if f.endswith(".h"):
+            text = open(os.path.join(inc, f)).read()
+            names |= set(re.findall(r"\b(pm3_[a-z_0-9]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    import pm3amd
+    lib = pm3amd.native()
+    names = declared_symbols()
+    assert "pm3_run" in names and "pm3_gradient" in names and "pm3_run_batch" in names
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(pm3amd.EXPORTED) <= names
+
+
+def test_params_and_defaults_without_device():
+    import pm3amd as G
+    p = G.Pm3Parameters.standard()
+    o = G._COptions()
+    G.native().pm3_options_default(ctypes.byref(o))
+    # Pm3Options::default(), src/scf.rs:259-287
+    assert (o.max_scf, o.e_tol, o.p_tol, o.use_diis, o.accelerator, o.adiis_switch, o.scf_memory_mb) == \
+        (200, 1e-8, 1e-7, 1, 2, 0.1, 512)
+    m = G.Molecule(np.array([8, 1, 1]), np.zeros((3, 3)))
+    assert G.native().pm3_query_nao(ctypes.byref(m._c()), p._h) == 6
+    m2 = G.Molecule(np.array([6, 18]), np.zeros((2, 3)))
+    assert G.native().pm3_query_nao(ctypes.byref(m2._c()), p._h) == -1
+
+
+def test_host_side_errors_need_no_device():
+    import pm3amd as G
+    # multiplicity > 1 is the reference's UHF path, not served by this engine
+    m = G.Molecule(np.array([6, 1, 1, 1]), np.eye(4, 3) * 2.0, multiplicity=2)
+    with pytest.raises(G.Unsupported):
+        G.run_pm3(m)
+    with pytest.raises(G.InvalidInput):
+        G.Molecule.from_xyz_str("2\nx\nH 0 0 0\n")
+    with pytest.raises(G.InvalidInput):
+        G.Molecule.from_xyz_str("1\nx\nXx 0 0 0\n")
+
+
+def test_parameter_table_matches_reference_csv():
+    """The embedded rows equal the reference's CSV values (checked where the reference exists)."""
+    ref = "/root/reference/src/data/pm3_parameters.csv"
+    if not os.path.exists(ref):
+        pytest.skip("reference tree not present")
+    import csv
+    rows = {int(r["z"]): r for r in csv.DictReader(l for l in open(ref) if not l.startswith("#"))}
+    inc = open(os.path.join(ROOT, "pm3-rs_amd", "csrc", "pm3_table.inc")).read()
+    keys = ["uss", "upp", "zs", "zp", "betas", "betap", "gss", "gsp", "gpp", "gp2", "hsp", "alp",
+            "g1_k", "g1_l", "g1_m", "g2_k", "g2_l", "g2_m"]
+    for m in re.finditer(r"PM3_ELEMENT\(([^)]*)\)", inc):
+        vals = [float(v) for v in m.group(1).split(",")]
+        z = int(vals[0])
+        for k, v in zip(keys, vals[1:]):
+            assert float(rows[z][k]) == v, (z, k)
