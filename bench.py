@@ -91,11 +91,12 @@ def main():
     batch = G.Batch([G.Molecule(z, x) for z, x in mols], options=opts)
     for _ in range(args.warmup):
         batch.solve()
-    kernel_ms = []
+    kernel_ms, pair_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         batch.solve()
         kernel_ms.append(batch.last_kernel_ms())
+        pair_ms.append(batch.last_pair_ms())
     t1 = time.perf_counter()
     status, res = batch.results()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
@@ -152,9 +153,14 @@ def main():
         "mean_iterations": float(np.mean([r.iterations for r in res])),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "scf_batch_kernel", "kernel_ms": k_ms, "algorithmic_flop": flops},
+                     "kernel": "scf_batch_kernel", "kernel_ms": k_ms, "algorithmic_flop": flops,
+                     "pair_kernel_ms": float(np.mean(pair_ms))},
         "cpu_baseline": cpu,
     }
+    if os.environ.get("PM3_PHASE_TIMERS") == "1":
+        ticks = batch.phase_ticks()
+        tot = sum(ticks.values()) or 1
+        out["phase_share"] = {k: round(v / tot, 4) for k, v in ticks.items()}
     print(json.dumps(out))
 
 

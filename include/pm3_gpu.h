@@ -125,6 +125,12 @@ pm3_status pm3_batch_results(pm3_batch* batch, pm3_status* status, pm3_result* r
 void pm3_batch_destroy(pm3_batch* batch);
 /* Device time (ms, HIP events on the launch stream) of the last pm3_batch_solve's kernel. */
 double pm3_batch_last_kernel_ms(const pm3_batch* batch);
+/* Device time (ms) of the last pm3_batch_solve's pair-integral kernel. */
+double pm3_batch_last_pair_ms(const pm3_batch* batch);
+/* Per-phase wall-clock ticks (10 ns) of the last solve summed over workgroups, collected only when
+ * the environment has PM3_PHASE_TIMERS=1 at pm3_batch_create (zeros otherwise): H_core finish,
+ * Fock, energy+DIIS, warm-start products, Jacobi, sort + C V, density, bookkeeping. */
+pm3_status pm3_batch_phase_ticks(const pm3_batch* batch, uint64_t* out8);
 
 /* Detail text of the calling thread's last error ("" if none). */
 const char* pm3_last_error_message(void);

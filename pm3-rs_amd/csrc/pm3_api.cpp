@@ -240,6 +240,17 @@ pm3_status pm3_batch_results(pm3_batch* h, pm3_status* status, pm3_result* resul
 
 double pm3_batch_last_kernel_ms(const pm3_batch* h) { return h ? batch_kernel_ms(h->b) : 0.0; }
 
+double pm3_batch_last_pair_ms(const pm3_batch* h) { return h ? batch_pair_ms(h->b) : 0.0; }
+
+pm3_status pm3_batch_phase_ticks(const pm3_batch* h, uint64_t* out) {
+  if (!h || !out) return fail(PM3_INVALID_INPUT, "null argument");
+  std::string err;
+  unsigned long long t[8];
+  if (!batch_phase_ticks(h->b, t, err)) return fail(PM3_DEVICE_ERROR, err);
+  for (int k = 0; k < 8; ++k) out[k] = t[k];
+  return PM3_OK;
+}
+
 void pm3_batch_destroy(pm3_batch* h) {
   if (!h) return;
   batch_destroy(h->b);

@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <numeric>
 
 #include "pm3_host.hpp"
@@ -51,6 +52,7 @@ struct MolDesc {
   int atom0, nat, nheavy, nao, nocc, np;
   int pad0, pad1;
   long long out_mat, out_vec;  // offsets into the density/coefficient and vector outputs, -1 if none
+  long long h_off, w_off;      // per-molecule H_core (np x np) and pair tables in the molecule store
 };
 
 struct MolOut {
@@ -78,6 +80,11 @@ struct KArgs {
   double* out_C;
   double* out_eps;
   double* out_q;  // Mulliken charge per atom (sorted atom order)
+  double* store;  // per-molecule H_core and pair tables written by pair_kernel
+  const long long* pair_base;  // first global pair of each queue position (nmol + 1 entries)
+  long long npairs_total;
+  unsigned long long* prof;    // optional per-phase wall-clock sums (10 ns ticks), may be null
+  int lds_doubles;             // dynamic LDS per workgroup, in doubles
   // options
   int max_scf;
   double e_tol, p_tol;
@@ -192,88 +199,101 @@ struct Mol {
   double* Wxx;
   double* Wxh;
   double* Whh;
+  double* H;     // H_core of this molecule (np x np, molecule store)
+  double* S1;    // [10][npairs] attraction of a's AOs to b's core, per pair (molecule store)
+  double* S2;    // [10][npairs] attraction of b's AOs to a's core
+  double* ecc;   // [npairs] core-core energy per pair
+  double* slot;  // per-pair Coulomb partial sums of the Fock build (workgroup scratch)
+  int lds_doubles;  // dynamic LDS of the launch, in doubles
 };
 
 // ------------------------------------------------------------------------------------------
 // Pair integrals, core-core energy and H_core (src/hamiltonian.rs:191-344, src/repulsion.rs:13-27).
-__device__ double setup_core(const KArgs& A, const Mol& M, double* H, double* lds, double* red) {
-  const int tid = threadIdx.x;
+// One work-item per atom pair: W (packed two-electron table), the two electron-core attraction
+// blocks and the core-core energy go to per-pair slots of the molecule store; the resonance block
+// 1/2 (beta_mu + beta_nu) S_mu,nu is written straight into H_core.
+__device__ inline void pair_integrals(const KArgs& A, const Mol& M, int p) {
   const int np = M.np, nat = M.nat;
-  double* acc = lds;  // [nat][16] electron-core sums per atom block
-  for (int e = tid; e < np * np; e += NT) H[e] = 0.0;
-  for (int e = tid; e < nat * 16; e += NT) acc[e] = 0.0;
-  __syncthreads();
-  double ecc = 0.0;
-  const int npairs = nat * (nat - 1) / 2;
-  for (int p = tid; p < npairs; p += NT) {
-    int a, b;
-    tri_decode(p, nat, a, b);
-    const int ga = M.atom0 + a, gb = M.atom0 + b;
-    const int sa = A.aslot[ga], sb = A.aslot[gb];
-    const ElemConst& ea = c_tab.el[sa];
-    const ElemConst& eb = c_tab.el[sb];
-    const double dx = A.ax[gb] - A.ax[ga], dy = A.ay[gb] - A.ay[ga], dz = A.az[gb] - A.az[ga];
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    ecc += core_core_pair<double>(c_tab, sa, sb, r);
-    const double ux = dx / r, uy = dy / r, uz = dz / r;
-    // Two-electron integrals and electron-core attraction (src/integrals.rs:167-230).
-    const double ddx = ux * r, ddy = uy * r, ddz = uz * r;
-    const double rr = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-    const double inv = 1.0 / rr;
-    double R[3][3];
-    rot_to_x(-(ddx * inv), -(ddy * inv), -(ddz * inv), R);
-    double e1b[10], e2a[10];
-    ecore(ea, eb, ddx, ddy, ddz, e1b);
-    ecore(eb, ea, -ddx, -ddy, -ddz, e2a);
-    const int na = ea.norb, nb = eb.norb;
-    for (int mu = 0; mu < na; ++mu)
-      for (int nu = 0; nu < na; ++nu) atomicAdd(&acc[a * 16 + mu * 4 + nu], e1b[pk(mu, nu)]);
-    for (int mu = 0; mu < nb; ++mu)
-      for (int nu = 0; nu < nb; ++nu) atomicAdd(&acc[b * 16 + mu * 4 + nu], e2a[pk(mu, nu)]);
-    if (na == 4 && nb == 4) {
-      double ri[22];
-      local_xx(ea, eb, rr, ri);
-      const int kx = tri_index(a, b, M.nh);
-      double* W = M.Wxx;
-      const int nxx = M.nxx;
-      assemble_xx(ri, R, [&](int idx, double v) { W[(long long)idx * nxx + kx] = v; });
-    } else if (na == 4) {
-      double ri[4], w[10];
-      local_xh(ea.dd, ea.qq, ea.rho0, ea.rho1, ea.rho2, eb.rho0, rr, ri);
-      assemble_xh(ri, R, w);
-      const int kx = a * M.nH + (b - M.nh);
+  const long long npairs = (long long)nat * (nat - 1) / 2;
+  int a, b;
+  tri_decode(p, nat, a, b);
+  const int ga = M.atom0 + a, gb = M.atom0 + b;
+  const int sa = A.aslot[ga], sb = A.aslot[gb];
+  const ElemConst& ea = c_tab.el[sa];
+  const ElemConst& eb = c_tab.el[sb];
+  const double dx = A.ax[gb] - A.ax[ga], dy = A.ay[gb] - A.ay[ga], dz = A.az[gb] - A.az[ga];
+  const double r = sqrt(dx * dx + dy * dy + dz * dz);
+  M.ecc[p] = core_core_pair<double>(c_tab, sa, sb, r);
+  const double ux = dx / r, uy = dy / r, uz = dz / r;
+  // Two-electron integrals and electron-core attraction (src/integrals.rs:167-230).
+  const double ddx = ux * r, ddy = uy * r, ddz = uz * r;
+  const double rr = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+  const double inv = 1.0 / rr;
+  double R[3][3];
+  rot_to_x(-(ddx * inv), -(ddy * inv), -(ddz * inv), R);
+  double e1b[10], e2a[10];
+  ecore(ea, eb, ddx, ddy, ddz, e1b);
+  ecore(eb, ea, -ddx, -ddy, -ddz, e2a);
+  const int na = ea.norb, nb = eb.norb;
+  for (int k = 0; k < (na == 4 ? 10 : 1); ++k) M.S1[k * npairs + p] = e1b[k];
+  for (int k = 0; k < (nb == 4 ? 10 : 1); ++k) M.S2[k * npairs + p] = e2a[k];
+  if (na == 4 && nb == 4) {
+    double ri[22];
+    local_xx(ea, eb, rr, ri);
+    const int kx = tri_index(a, b, M.nh);
+    double* W = M.Wxx;
+    const int nxx = M.nxx;
+    assemble_xx(ri, R, [&](int idx, double v) { W[(long long)idx * nxx + kx] = v; });
+  } else if (na == 4) {
+    double ri[4], w[10];
+    local_xh(ea.dd, ea.qq, ea.rho0, ea.rho1, ea.rho2, eb.rho0, rr, ri);
+    assemble_xh(ri, R, w);
+    const int kx = a * M.nH + (b - M.nh);
 #pragma unroll
-      for (int k = 0; k < 10; ++k) M.Wxh[(long long)k * M.nxh + kx] = w[k];
-    } else {
-      const double s = ea.rho0 + eb.rho0;
-      M.Whh[tri_index(a - M.nh, b - M.nh, M.nH)] = rsq(rr * rr + s * s) * EV;
-    }
-    // Resonance 1/2 (beta_mu + beta_nu) S_mu,nu (src/hamiltonian.rs:322-331).
-    double S[4][4];
-    overlap_block(ea, eb, ux, uy, uz, r, S);
-    const int oa = A.aoff[ga], ob = A.aoff[gb];
-    for (int i = 0; i < na; ++i) {
-      const double bi = i == 0 ? ea.betas : ea.betap;
-      for (int j = 0; j < nb; ++j) {
-        const double bj = j == 0 ? eb.betas : eb.betap;
-        const double v = 0.5 * (bi + bj) * S[i][j];
-        H[(oa + i) * np + ob + j] = v;
-        H[(ob + j) * np + oa + i] = v;
-      }
+    for (int k = 0; k < 10; ++k) M.Wxh[(long long)k * M.nxh + kx] = w[k];
+  } else {
+    const double s = ea.rho0 + eb.rho0;
+    M.Whh[tri_index(a - M.nh, b - M.nh, M.nH)] = rsq(rr * rr + s * s) * EV;
+  }
+  // Resonance 1/2 (beta_mu + beta_nu) S_mu,nu (src/hamiltonian.rs:322-331).
+  double S[4][4];
+  overlap_block(ea, eb, ux, uy, uz, r, S);
+  const int oa = A.aoff[ga], ob = A.aoff[gb];
+  for (int i = 0; i < na; ++i) {
+    const double bi = i == 0 ? ea.betas : ea.betap;
+    for (int j = 0; j < nb; ++j) {
+      const double bj = j == 0 ? eb.betas : eb.betap;
+      const double v = 0.5 * (bi + bj) * S[i][j];
+      M.H[(oa + i) * np + ob + j] = v;
+      M.H[(ob + j) * np + oa + i] = v;
     }
   }
-  __syncthreads();
+}
+
+// H_core diagonal blocks (U plus the attraction to every other core, summed in atom order) and the
+// total core-core energy, from the per-pair slots.  One workgroup per molecule.
+__device__ double finish_core(const KArgs& A, const Mol& M, double* red) {
+  const int tid = threadIdx.x;
+  const int np = M.np, nat = M.nat;
+  const long long npairs = (long long)nat * (nat - 1) / 2;
   for (int e = tid; e < nat * 16; e += NT) {
     const int a = e >> 4, mu = (e >> 2) & 3, nu = e & 3;
     const int ga = M.atom0 + a;
     const ElemConst& ea = c_tab.el[A.aslot[ga]];
     if (mu < ea.norb && nu < ea.norb) {
       const int oa = A.aoff[ga];
+      const int q = pk(mu, nu);
+      double s = 0.0;
+      for (int b = 0; b < nat; ++b) {
+        if (b == a) continue;
+        s += b > a ? M.S1[q * npairs + tri_index(a, b, nat)] : M.S2[q * npairs + tri_index(b, a, nat)];
+      }
       const double u = mu != nu ? 0.0 : (mu == 0 ? ea.uss : ea.upp);
-      H[(oa + mu) * np + oa + nu] = u + acc[e];
+      M.H[(oa + mu) * np + oa + nu] = u + s;
     }
   }
-  double v[1] = {ecc};
+  double v[1] = {0.0};
+  for (long long p = tid; p < npairs; p += NT) v[0] += M.ecc[p];
   block_sum<1>(v, red);
   __syncthreads();
   return v[0];
@@ -286,9 +306,9 @@ __device__ double setup_core(const KArgs& A, const Mol& M, double* H, double* ld
 __device__ void build_fock(const KArgs& A, const Mol& M, const double* H, const double* P, double* F, double* lds) {
   const int tid = threadIdx.x;
   const int np = M.np, nat = M.nat;
-  double* J = lds;  // [nat][10]
-  for (int e = tid; e < nat * 10; e += NT) J[e] = 0.0;
-  __syncthreads();
+  double* J = lds;                              // [nat][10] Coulomb sums per atom
+  double* JX = M.slot;                          // [20][nxx]: J_a (0..9), J_b (10..19) per XX pair
+  double* JH = M.slot + 20LL * M.nxx;           // [11][nxh]: J_a (0..9), J_b (10) per XH pair
   // heavy-heavy pairs
   for (int kx = tid; kx < M.nxx; kx += NT) {
     int a, b;
@@ -351,8 +371,8 @@ __device__ void build_fock(const KArgs& A, const Mol& M, const double* H, const 
       }
 #pragma unroll
     for (int q = 0; q < 10; ++q) {
-      atomicAdd(&J[a * 10 + q], ja[q]);
-      atomicAdd(&J[b * 10 + q], jb[q]);
+      JX[(long long)q * M.nxx + kx] = ja[q];
+      JX[(long long)(10 + q) * M.nxx + kx] = jb[q];
     }
   }
   // heavy-hydrogen pairs
@@ -383,8 +403,8 @@ __device__ void build_fock(const KArgs& A, const Mol& M, const double* H, const 
       F[ob * np + oa + mu] = v;
     }
 #pragma unroll
-    for (int q = 0; q < 10; ++q) atomicAdd(&J[a * 10 + q], w[q] * pbb);
-    atomicAdd(&J[b * 10], jbv);
+    for (int q = 0; q < 10; ++q) JH[(long long)q * M.nxh + kx] = w[q] * pbb;
+    JH[10LL * M.nxh + kx] = jbv;
   }
   // hydrogen-hydrogen pairs
   for (int kx = tid; kx < M.nhh; kx += NT) {
@@ -397,8 +417,29 @@ __device__ void build_fock(const KArgs& A, const Mol& M, const double* H, const 
     const double v = H[oa * np + ob] - 0.5 * P[oa * np + ob] * w;
     F[oa * np + ob] = v;
     F[ob * np + oa] = v;
-    atomicAdd(&J[a * 10], w * P[ob * np + ob]);
-    atomicAdd(&J[b * 10], w * P[oa * np + oa]);
+  }
+  __syncthreads();
+  // Coulomb sums per atom, partners in a fixed order
+  for (int e = tid; e < M.nh * 10; e += NT) {
+    const int a = e / 10, q = e % 10;
+    double s = 0.0;
+    for (int b = 0; b < M.nh; ++b) {
+      if (b == a) continue;
+      s += b > a ? JX[(long long)q * M.nxx + tri_index(a, b, M.nh)] : JX[(long long)(10 + q) * M.nxx + tri_index(b, a, M.nh)];
+    }
+    for (int h = 0; h < M.nH; ++h) s += JH[(long long)q * M.nxh + a * M.nH + h];
+    J[a * 10 + q] = s;
+  }
+  for (int h = tid; h < M.nH; h += NT) {
+    double s = 0.0;
+    for (int a = 0; a < M.nh; ++a) s += JH[10LL * M.nxh + a * M.nH + h];
+    for (int g = 0; g < M.nH; ++g) {
+      if (g == h) continue;
+      const int lo = g < h ? g : h, hi = g < h ? h : g;
+      const int og = A.aoff[M.atom0 + M.nh + g];
+      s += M.Whh[tri_index(lo, hi, M.nH)] * P[og * np + og];
+    }
+    J[(M.nh + h) * 10] = s;
   }
   __syncthreads();
   // diagonal blocks: H + one-centre + J
@@ -435,40 +476,60 @@ __device__ inline void rr_pair(int k, int r, int nn, int& p, int& q) {
   q = (k == 0) ? m1 : (m1 - k + r) % m1;
 }
 
-__device__ int jacobi(const double* Ain, int n, int np, double* V, double* lam, double* lds, double* red, int max_sweeps) {
+// LDS layout of the eigensolver for an n x n problem (in doubles): packed upper triangle of A,
+// then per-slot rotation (c, s, t), then the slot -> (p, q) schedule of the current round (two ints
+// per slot), then, when room allows, the eigenvector matrix V (column-major, ld n).
+__device__ __host__ inline int jac_lds_base(int n) {
+  const int m = (n + 1) >> 1;
+  return n * (n + 1) / 2 + 4 * m;
+}
+__device__ __host__ inline int jac_lds_with_v(int n) { return jac_lds_base(n) + n * n; }
+
+// Largest |off-diagonal| of the packed matrix: each thread walks one contiguous chunk, decoding its
+// starting (i, j) once and stepping the pair incrementally.
+__device__ inline double packed_offmax(const double* Ap, int npk, double* red) {
+  const int tid = threadIdx.x;
+  const int chunk = (npk + NT - 1) / NT;
+  const int i0 = tid * chunk;
+  double off = 0.0;
+  if (i0 < npk) {
+    int j = (int)((sqrtf(8.0f * (float)i0 + 1.0f) - 1.0f) * 0.5f);
+    while (j * (j + 1) / 2 > i0) --j;
+    while ((j + 1) * (j + 2) / 2 <= i0) ++j;
+    int i = i0 - j * (j + 1) / 2;
+    const int iend = min(i0 + chunk, npk);
+    for (int idx = i0; idx < iend; ++idx) {
+      if (i != j) off = fmax(off, fabs(Ap[idx]));
+      if (++i > j) { i = 0; ++j; }
+    }
+  }
+  return block_max(off, red);
+}
+
+__device__ int jacobi(const double* Ain, int n, int np, double* V, int ldv, double* lam, double* lds, double* red,
+                      int max_sweeps) {
   const int tid = threadIdx.x;
   const int nn = n + (n & 1), m = nn >> 1;
   const int npk = n * (n + 1) / 2;
   double* Ap = lds;
-  double* cs = lds + npk;  // [m][3]
-  for (int idx = tid; idx < npk; idx += NT) {
-    int j = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
-    while (j * (j + 1) / 2 > idx) --j;
-    while ((j + 1) * (j + 2) / 2 <= idx) ++j;
-    const int i = idx - j * (j + 1) / 2;
-    Ap[idx] = Ain[i * np + j];
-  }
-  for (int e = tid; e < n * np; e += NT) {
-    const int c = e / np, rrow = e % np;
-    V[e] = (c == rrow) ? 1.0 : 0.0;
+  double* cs = lds + npk;               // [m][3]
+  int* pq = (int*)(lds + npk + 3 * m);  // [m][2]
+  for (int j = 0; j < n; ++j)
+    for (int i = tid; i <= j; i += NT) Ap[j * (j + 1) / 2 + i] = Ain[i * np + j];
+  for (int e = tid; e < n * ldv; e += NT) {
+    const int c = e / ldv, r = e % ldv;
+    V[e] = (c == r) ? 1.0 : 0.0;
   }
   __syncthreads();
   double dmax = 0.0;
   for (int i = tid; i < n; i += NT) dmax = fmax(dmax, fabs(Ap[i * (i + 1) / 2 + i]));
   const double scale = fmax(block_max(dmax, red), 1.0);
   const double tol = 1.0e-14 * scale;
+  const int nblk = m * (m + 1) / 2;
   int sweep = 0;
-  if (n < 2) return 0;
-  for (; sweep < max_sweeps; ++sweep) {
-    double off = 0.0;
-    for (int idx = tid; idx < npk; idx += NT) {
-      int j = (int)((sqrt(8.0 * idx + 1.0) - 1.0) * 0.5);
-      while (j * (j + 1) / 2 > idx) --j;
-      while ((j + 1) * (j + 2) / 2 <= idx) ++j;
-      const int i = idx - j * (j + 1) / 2;
-      if (i != j) off = fmax(off, fabs(Ap[idx]));
-    }
-    if (block_max(off, red) <= tol) break;
+  const int sweeps = n < 2 ? 0 : max_sweeps;
+  for (; sweep < sweeps; ++sweep) {
+    if (packed_offmax(Ap, npk, red) <= tol) break;
     for (int r = 0; r < nn - 1; ++r) {
       for (int k = tid; k < m; k += NT) {
         int p, q;
@@ -487,16 +548,21 @@ __device__ int jacobi(const double* Ain, int n, int np, double* V, double* lam, 
         cs[3 * k] = c;
         cs[3 * k + 1] = s;
         cs[3 * k + 2] = t;
+        pq[2 * k] = p;
+        pq[2 * k + 1] = q;
       }
       __syncthreads();
-      for (int bidx = tid; bidx < m * m; bidx += NT) {
-        const int K = bidx / m, L = bidx % m;
-        if (K > L) continue;
-        int p, q, u, v;
-        rr_pair(K, r, nn, p, q);
-        rr_pair(L, r, nn, u, v);
-        const double cK = cs[3 * K], sK = cs[3 * K + 1], tK = cs[3 * K + 2];
-        if (K == L) {
+      // blocks (K, L) with K <= L of the m x m slot grid, enumerated row by row of the triangle
+      for (int bidx = tid; bidx < nblk; bidx += NT) {
+        int K = (int)((sqrtf(8.0f * (float)bidx + 1.0f) - 1.0f) * 0.5f);  // bidx = K(K+1)/2 + L', L' <= K
+        while (K * (K + 1) / 2 > bidx) --K;
+        while ((K + 1) * (K + 2) / 2 <= bidx) ++K;
+        const int Lp = bidx - K * (K + 1) / 2;
+        const int L = K, Kk = Lp;  // block (Kk, L) with Kk <= L
+        const double sK = cs[3 * Kk + 1], sL = cs[3 * L + 1];
+        const int p = pq[2 * Kk], q = pq[2 * Kk + 1];
+        if (Kk == L) {
+          const double tK = cs[3 * Kk + 2];
           if (p < n && q < n && tK != 0.0) {
             const int ipq = sym_idx(p, q);
             const double apq = Ap[ipq];
@@ -505,32 +571,33 @@ __device__ int jacobi(const double* Ain, int n, int np, double* V, double* lam, 
             Ap[ipq] = 0.0;
           }
         } else {
-          const double cL = cs[3 * L], sL = cs[3 * L + 1];
           if (sK == 0.0 && sL == 0.0) continue;
+          const int u = pq[2 * L], v = pq[2 * L + 1];
+          const double cK = cs[3 * Kk], cL = cs[3 * L];
           const bool vp = p < n, vq = q < n, vu = u < n, vv = v < n;
-          const double apu = (vp && vu) ? Ap[sym_idx(p, u)] : 0.0;
-          const double apv = (vp && vv) ? Ap[sym_idx(p, v)] : 0.0;
-          const double aqu = (vq && vu) ? Ap[sym_idx(q, u)] : 0.0;
-          const double aqv = (vq && vv) ? Ap[sym_idx(q, v)] : 0.0;
+          const int ipu = sym_idx(p, u), ipv = sym_idx(p, v), iqu = sym_idx(q, u), iqv = sym_idx(q, v);
+          const double apu = (vp && vu) ? Ap[ipu] : 0.0;
+          const double apv = (vp && vv) ? Ap[ipv] : 0.0;
+          const double aqu = (vq && vu) ? Ap[iqu] : 0.0;
+          const double aqv = (vq && vv) ? Ap[iqv] : 0.0;
           const double xpu = cK * apu - sK * aqu, xqu = sK * apu + cK * aqu;
           const double xpv = cK * apv - sK * aqv, xqv = sK * apv + cK * aqv;
-          if (vp && vu) Ap[sym_idx(p, u)] = cL * xpu - sL * xpv;
-          if (vp && vv) Ap[sym_idx(p, v)] = sL * xpu + cL * xpv;
-          if (vq && vu) Ap[sym_idx(q, u)] = cL * xqu - sL * xqv;
-          if (vq && vv) Ap[sym_idx(q, v)] = sL * xqu + cL * xqv;
+          if (vp && vu) Ap[ipu] = cL * xpu - sL * xpv;
+          if (vp && vv) Ap[ipv] = sL * xpu + cL * xpv;
+          if (vq && vu) Ap[iqu] = cL * xqu - sL * xqv;
+          if (vq && vv) Ap[iqv] = sL * xqu + cL * xqv;
         }
       }
       for (int e = tid; e < n * m; e += NT) {
-        const int k = e / n, i = e % n;
+        const int k = e / n, i = e - k * n;
         const double s = cs[3 * k + 1];
         if (s == 0.0) continue;
-        int p, q;
-        rr_pair(k, r, nn, p, q);
+        const int p = pq[2 * k], q = pq[2 * k + 1];
         if (p >= n || q >= n) continue;
         const double c = cs[3 * k];
-        const double vpv = V[p * np + i], vqv = V[q * np + i];
-        V[p * np + i] = c * vpv - s * vqv;
-        V[q * np + i] = s * vpv + c * vqv;
+        const double vpv = V[p * ldv + i], vqv = V[q * ldv + i];
+        V[p * ldv + i] = c * vpv - s * vqv;
+        V[q * ldv + i] = s * vpv + c * vqv;
       }
       __syncthreads();
     }
@@ -651,7 +718,17 @@ __device__ void adiis_coeffs(const Diis& D, double* c) {
 struct Bufs {
   double *H, *F, *P, *Pn, *Fu, *T, *C, *Cn, *V, *Pb, *Cb, *Fh, *Eh, *Dh, *eps, *epsb, *lam;
   int* perm;
+  unsigned long long* ph;  // LDS: [0..7] per-phase wall-clock sums, [8] last stamp; null when off
 };
+
+// Phase stamp: adds the wall-clock ticks since the previous stamp to phase k (thread 0 only).
+__device__ inline void stamp(const Bufs& B, int k) {
+  if (B.ph && threadIdx.x == 0) {
+    const unsigned long long now = wall_clock64();
+    B.ph[k] += now - B.ph[8];
+    B.ph[8] = now;
+  }
+}
 
 // Eigenvectors of Fu -> Cn (row-major, columns sorted by ascending eigenvalue) and eps.
 // warm != 0: Jacobi on C^T Fu C and Cn = C V; otherwise Jacobi on Fu itself.
@@ -671,7 +748,12 @@ __device__ void eigensolve(const Mol& M, const Bufs& B, const double* Fu, bool w
     __syncthreads();
     Amat = Am;
   }
-  jacobi(Amat, n, np, B.V, B.lam, lds, red, 60);
+  stamp(B, 3);
+  const bool v_in_lds = jac_lds_with_v(n) <= M.lds_doubles;
+  double* V = v_in_lds ? lds + jac_lds_base(n) : B.V;
+  const int ldv = v_in_lds ? n : np;
+  jacobi(Amat, n, np, V, ldv, B.lam, lds, red, 60);
+  stamp(B, 4);
   // ascending order, ties by index
   for (int k = tid; k < n; k += NT) {
     const double lk = B.lam[k];
@@ -684,20 +766,20 @@ __device__ void eigensolve(const Mol& M, const Bufs& B, const double* Fu, bool w
   }
   __syncthreads();
   for (int j = tid; j < n; j += NT) B.eps[j] = B.lam[B.perm[j]];
-  const double* V = B.V;
   const int* perm = B.perm;
   double* Cn = B.Cn;
   if (warm) {
     const double* C = B.C;
-    wg_gemm(n, n, n, [&](int i, int k) { return C[i * np + k]; }, [&](int k, int j) { return V[perm[j] * np + k]; },
+    wg_gemm(n, n, n, [&](int i, int k) { return C[i * np + k]; }, [&](int k, int j) { return V[perm[j] * ldv + k]; },
             [&](int i, int j, double v) { Cn[i * np + j] = v; });
   } else {
     for (int e = tid; e < n * n; e += NT) {
       const int i = e / n, j = e % n;
-      Cn[i * np + j] = V[perm[j] * np + i];
+      Cn[i * np + j] = V[perm[j] * ldv + i];
     }
   }
   __syncthreads();
+  stamp(B, 5);
 }
 
 // P = 2 C_occ C_occ^T from Cn into Pdst.
@@ -714,7 +796,7 @@ __device__ void initial_guess(const KArgs& A, const Mol& M, const Bufs& B, int g
   for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
   __syncthreads();
   if (guess == 2) {
-    eigensolve(M, B, B.H, false, lds, red);
+    eigensolve(M, B, M.H, false, lds, red);
     density(M, B.Cn, B.P);
     return;
   }
@@ -783,9 +865,11 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
   bool have_c = false;
   for (int it = 0; it < A.max_scf; ++it) {
     R.iters = it + 1;
-    build_fock(A, M, B.H, B.P, B.F, lds);
+    stamp(B, 7);
+    build_fock(A, M, M.H, B.P, B.F, lds);
+    stamp(B, 1);
     double ev[1] = {0.0};
-    for (int e = tid; e < np * np; e += NT) ev[0] += B.P[e] * (B.H[e] + B.F[e]);
+    for (int e = tid; e < np * np; e += NT) ev[0] += B.P[e] * (M.H[e] + B.F[e]);
     block_sum<1>(ev, red);
     const double e_elec = 0.5 * ev[0];
     const double* Fuse = B.F;
@@ -882,6 +966,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
         Fuse = Fn;
       }
     }
+    stamp(B, 2);
     eigensolve(M, B, Fuse, have_c, lds, red);
     density(M, B.Cn, B.Pn);
     double dv[1] = {0.0};
@@ -899,15 +984,16 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
     }
     __syncthreads();
     have_c = true;
+    stamp(B, 6);
     e_old = e_elec;
     if (it > 0 && de < A.e_tol && dp < A.p_tol) {
       R.converged = 1;
       break;
     }
   }
-  build_fock(A, M, B.H, B.P, B.F, lds);
+  build_fock(A, M, M.H, B.P, B.F, lds);
   double ev[1] = {0.0};
-  for (int e = tid; e < np * np; e += NT) ev[0] += B.P[e] * (B.H[e] + B.F[e]);
+  for (int e = tid; e < np * np; e += NT) ev[0] += B.P[e] * (M.H[e] + B.F[e]);
   block_sum<1>(ev, red);
   R.e_elec = 0.5 * ev[0];
   return R;
@@ -939,7 +1025,7 @@ __device__ inline Bufs slab(const KArgs& A, double*& Wbase) {
   return B;
 }
 
-__device__ inline Mol mol_of(const MolDesc& md, double* Wbase) {
+__device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double* Wbase) {
   Mol M;
   M.atom0 = md.atom0;
   M.nat = md.nat;
@@ -951,9 +1037,16 @@ __device__ inline Mol mol_of(const MolDesc& md, double* Wbase) {
   M.nxx = M.nh * (M.nh - 1) / 2;
   M.nxh = M.nh * M.nH;
   M.nhh = M.nH * (M.nH - 1) / 2;
-  M.Wxx = Wbase;
+  const long long npairs = (long long)md.nat * (md.nat - 1) / 2;
+  M.H = A.store + md.h_off;
+  M.Wxx = A.store + md.w_off;
   M.Wxh = M.Wxx + 100LL * M.nxx;
   M.Whh = M.Wxh + 10LL * M.nxh;
+  M.S1 = M.Whh + M.nhh;
+  M.S2 = M.S1 + 10 * npairs;
+  M.ecc = M.S2 + 10 * npairs;
+  M.slot = Wbase;
+  M.lds_doubles = A.lds_doubles;
   return M;
 }
 
@@ -965,28 +1058,53 @@ __device__ inline void zero_mats(const Mol& M, const Bufs& B) {
   }
 }
 
+// All atom pairs of all molecules, one work-item each (grid-stride).
+__global__ __launch_bounds__(256) void pair_kernel(KArgs A) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < A.npairs_total; g += stride) {
+    int lo = 0, hi = A.nmol;  // largest queue position with pair_base <= g
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (A.pair_base[mid] <= g) lo = mid; else hi = mid;
+    }
+    const Mol M = mol_of(A, A.mols[A.order[lo]], nullptr);
+    pair_integrals(A, M, (int)(g - A.pair_base[lo]));
+  }
+}
+
 __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
   extern __shared__ double lds[];
   __shared__ double red[NW * (3 * MAXD + 2)];
   __shared__ Diis D;
   __shared__ int s_mol;
+  __shared__ unsigned long long s_ph[9];
   const int tid = threadIdx.x;
   double* Wbase;
-  const Bufs B = slab(A, Wbase);
+  Bufs B = slab(A, Wbase);
+  B.ph = A.prof ? s_ph : nullptr;
+  if (tid < 9) s_ph[tid] = 0;
+  if (tid == 0) s_ph[8] = wall_clock64();
 
   for (;;) {
     if (tid == 0) s_mol = atomicAdd(A.queue, 1);
+    if (tid == 0 && A.prof) {
+      for (int k = 0; k < 8; ++k) {
+        atomicAdd(&A.prof[k], s_ph[k]);
+        s_ph[k] = 0;
+      }
+    }
     __syncthreads();
     const int qi = s_mol;
     __syncthreads();
     if (qi >= A.nmol) break;
     const int mi = A.order[qi];
     const MolDesc md = A.mols[mi];
-    const Mol M = mol_of(md, Wbase);
+    const Mol M = mol_of(A, md, Wbase);
     // Matrices use the molecule's own leading dimension np inside the workgroup's slab; the
     // padding beyond nao stays zero for the whole solve.
     zero_mats(M, B);
-    const double ecore = setup_core(A, M, B.H, lds, red);
+    const double ecore = finish_core(A, M, red);
+    stamp(B, 0);
     PathResult best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D);
     const int nocc = M.nocc, n = M.n, np = M.np;
     const double first_e = best.e_elec;
@@ -1068,6 +1186,7 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
       o.status = best.converged ? 0 : 3;
       A.out[mi] = o;
     }
+    stamp(B, 7);
     if (md.out_mat >= 0) {
       for (int e = tid; e < n * n; e += NT) {
         const int i = e / n, j = e % n;
@@ -1092,19 +1211,19 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
   const int tid = threadIdx.x;
   double* Wbase;
   const Bufs B = slab(A, Wbase);
-  const Mol M = mol_of(A.mols[A.order[0]], Wbase);
+  const Mol M = mol_of(A, A.mols[A.order[0]], Wbase);
   const int n = M.n, np = M.np;
   zero_mats(M, B);
   __syncthreads();
   if (mode == 0 || mode == 1) {
-    const double ecore = setup_core(A, M, B.H, lds, red);
+    const double ecore = finish_core(A, M, red);
     if (mode == 0) {
-      for (int e = tid; e < n * n; e += NT) out[e] = B.H[(e / n) * np + e % n];
+      for (int e = tid; e < n * n; e += NT) out[e] = M.H[(e / n) * np + e % n];
       if (tid == 0) out[n * n] = ecore;
     } else {
       for (int e = tid; e < n * n; e += NT) B.P[(e / n) * np + e % n] = in[e];
       __syncthreads();
-      build_fock(A, M, B.H, B.P, B.F, lds);
+      build_fock(A, M, M.H, B.P, B.F, lds);
       for (int e = tid; e < n * n; e += NT) out[e] = B.F[(e / n) * np + e % n];
     }
   } else if (mode == 2 || mode == 4) {
@@ -1146,15 +1265,17 @@ struct Batch {
   std::vector<int> pre_status;
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
-  void* bufs[20] = {nullptr};
+  void* bufs[24] = {nullptr};
   int nbufs = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_kernel_ms = 0.0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  double last_kernel_ms = 0.0, last_pair_ms = 0.0;
+  int pair_blocks = 1;
   ~Batch() {
     for (int i = 0; i < nbufs; ++i)
       if (bufs[i]) (void)hipFree(bufs[i]);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev2) (void)hipEventDestroy(ev2);
   }
   template <class T> bool alloc(T*& p, size_t count, std::string& err) {
     hipError_t e = hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(count, 1));
@@ -1246,8 +1367,10 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
     d.nao = nao;
     d.nocc = (int)(ne / 2);
     d.np = (nao + 15) & ~15;
-    const int npk = nao * (nao + 1) / 2 + 3 * ((nao + 2) / 2);
-    const int lneed = std::max(npk, m.n_atoms * 16) * 8;
+    // eigensolver LDS: packed matrix + rotation schedule, plus the eigenvectors when they fit
+    int jl = jac_lds_with_v(nao);
+    if (jl * 8 + STATIC_LDS_BYTES > LDS_BYTES_MAX) jl = jac_lds_base(nao);
+    const int lneed = std::max(jl, m.n_atoms * 16) * 8;
     if (lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || d.nocc > nao) {
       st = d.nocc > nao ? 1 : 5;
       aslot.resize(d.atom0); aoff.resize(d.atom0); ax.resize(d.atom0); ay.resize(d.atom0); az.resize(d.atom0);
@@ -1257,7 +1380,8 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
     b.lds_need = std::max(b.lds_need, lneed);
     np_max = std::max(np_max, d.np);
     const long long nH = d.nat - nh;
-    w_cap = std::max(w_cap, 100LL * nh * (nh - 1) / 2 + 10LL * nh * nH + nH * (nH - 1) / 2);
+    const long long npairs = (long long)d.nat * (d.nat - 1) / 2;
+    w_cap = std::max(w_cap, 20LL * npairs);  // Fock-build pair slots per workgroup
     if (req.want_density || req.want_orbitals || req.want_gradient) {
       d.out_mat = b.mat_total;
       d.out_vec = b.vec_total;
@@ -1270,6 +1394,19 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   b.nsolve = (int)b.solve.size();
   // largest molecules first so the queue ends on short work
   std::stable_sort(b.solve.begin(), b.solve.end(), [&](int x, int y) { return b.desc[x].nao > b.desc[y].nao; });
+  // molecule store: H_core (np x np) then W_xx | W_xh | W_hh | S1 | S2 | e_cc per molecule, and the
+  // first global pair index of every queue position for the pair kernel
+  std::vector<long long> pair_base(b.nsolve + 1, 0);
+  long long store_total = 0;
+  for (int q = 0; q < b.nsolve; ++q) {
+    MolDesc& d = b.desc[b.solve[q]];
+    const long long nh = d.nheavy, nH = d.nat - d.nheavy, npairs = (long long)d.nat * (d.nat - 1) / 2;
+    d.h_off = store_total;
+    store_total += ((long long)d.np * d.np + 7) & ~7LL;
+    d.w_off = store_total;
+    store_total += (100 * nh * (nh - 1) / 2 + 10 * nh * nH + nH * (nH - 1) / 2 + 21 * npairs + 7) & ~7LL;
+    pair_base[q + 1] = pair_base[q] + npairs;
+  }
   out_batch = bp;
   if (b.nsolve == 0) return true;
   int dev = 0;
@@ -1281,14 +1418,18 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   const long long stride = ((11 + 3 * MAXD) * m2 + w_cap + 4LL * np_max + 64 + 31) & ~31LL;
   KArgs& a = b.args;
   MolDesc* d_desc; int *d_order, *d_queue, *d_aslot, *d_aoff;
-  double *d_ax, *d_ay, *d_az, *d_scr, *d_P, *d_C, *d_eps, *d_q;
+  double *d_ax, *d_ay, *d_az, *d_scr, *d_P, *d_C, *d_eps, *d_q, *d_store;
+  long long* d_pbase;
   MolOut* d_out;
   if (!b.alloc(d_desc, nmol, err) || !b.alloc(d_order, b.nsolve, err) || !b.alloc(d_queue, 1, err) ||
       !b.alloc(d_aslot, b.atoms_total, err) || !b.alloc(d_aoff, b.atoms_total, err) || !b.alloc(d_ax, b.atoms_total, err) ||
       !b.alloc(d_ay, b.atoms_total, err) || !b.alloc(d_az, b.atoms_total, err) || !b.alloc(d_scr, (size_t)(stride * b.nwg), err) ||
       !b.alloc(d_out, nmol, err) || !b.alloc(d_P, b.mat_total, err) || !b.alloc(d_C, b.mat_total, err) ||
-      !b.alloc(d_eps, b.vec_total, err) || !b.alloc(d_q, b.atoms_total, err))
+      !b.alloc(d_eps, b.vec_total, err) || !b.alloc(d_q, b.atoms_total, err) ||
+      !b.alloc(d_store, (size_t)store_total, err) || !b.alloc(d_pbase, pair_base.size(), err))
     return false;
+  HIP_OK(hipMemset(d_store, 0, sizeof(double) * std::max<long long>(store_total, 1)));
+  HIP_OK(hipMemcpy(d_pbase, pair_base.data(), sizeof(long long) * pair_base.size(), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_desc, b.desc.data(), sizeof(MolDesc) * nmol, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_order, b.solve.data(), sizeof(int) * b.nsolve, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_aslot, aslot.data(), sizeof(int) * aslot.size(), hipMemcpyHostToDevice));
@@ -1314,6 +1455,19 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   a.out_C = d_C;
   a.out_eps = d_eps;
   a.out_q = d_q;
+  a.store = d_store;
+  a.pair_base = d_pbase;
+  a.npairs_total = pair_base.back();
+  a.prof = nullptr;
+  a.lds_doubles = b.lds_need / 8;
+  if (const char* ev = std::getenv("PM3_PHASE_TIMERS")) {
+    if (ev[0] == '1') {
+      unsigned long long* d_prof;
+      if (!b.alloc(d_prof, 8, err)) return false;
+      a.prof = d_prof;
+    }
+  }
+  b.pair_blocks = (int)std::max<long long>(1, std::min<long long>((a.npairs_total + 255) / 256, 8LL * prop.multiProcessorCount));
   a.max_scf = opt.max_scf;
   a.e_tol = opt.e_tol;
   a.p_tol = opt.p_tol;
@@ -1333,20 +1487,35 @@ bool batch_solve(Batch* bp, std::string& err) {
   if (!b.ev0) {
     HIP_OK(hipEventCreate(&b.ev0));
     HIP_OK(hipEventCreate(&b.ev1));
+    HIP_OK(hipEventCreate(&b.ev2));
   }
   HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), 0));
+  if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, 8 * sizeof(unsigned long long), 0));
   HIP_OK(hipEventRecord(b.ev0, 0));
-  hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
+  hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(256), 0, 0, b.args);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(b.ev1, 0));
-  HIP_OK(hipEventSynchronize(b.ev1));
-  float ms = 0.0f;
-  HIP_OK(hipEventElapsedTime(&ms, b.ev0, b.ev1));
+  hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(b.ev2, 0));
+  HIP_OK(hipEventSynchronize(b.ev2));
+  float ms = 0.0f, pms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&pms, b.ev0, b.ev1));
+  HIP_OK(hipEventElapsedTime(&ms, b.ev1, b.ev2));
+  b.last_pair_ms = pms;
   b.last_kernel_ms = ms;
   return true;
 }
 
 double batch_kernel_ms(const Batch* b) { return b ? b->last_kernel_ms : 0.0; }
+double batch_pair_ms(const Batch* b) { return b ? b->last_pair_ms : 0.0; }
+
+bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err) {
+  for (int k = 0; k < 8; ++k) out[k] = 0;
+  if (!b || !b->args.prof) return true;
+  HIP_OK(hipMemcpy(out, b->args.prof, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  return true;
+}
 
 bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std::vector<MolResult>& out, std::string& err) {
   Batch& b = *bp;
@@ -1446,6 +1615,7 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
     if (e != hipSuccess) { err = hipGetErrorString(e); ok = false; }
   }
   if (ok) {
+    hipLaunchKernelGGL(pair_kernel, dim3(b->pair_blocks), dim3(256), 0, 0, b->args);
     hipLaunchKernelGGL(stage_kernel, dim3(1), dim3(NT), b->lds_need, 0, b->args, mode, d_in, d_out);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();

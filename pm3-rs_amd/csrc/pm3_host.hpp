@@ -67,7 +67,12 @@ bool batch_solve(Batch* b, std::string& err);
 bool batch_fetch(Batch* b, const Params& P, const std::vector<MolIn>& mols, std::vector<MolResult>& out,
                  std::string& err);
 void batch_destroy(Batch* b);
-double batch_kernel_ms(const Batch* b);  // device time of the last solve's launch (HIP events)
+double batch_kernel_ms(const Batch* b);  // device time of the last solve's SCF launch (HIP events)
+double batch_pair_ms(const Batch* b);    // device time of the last solve's pair-integral launch
+// Per-phase wall-clock ticks (10 ns) summed over workgroups, when PM3_PHASE_TIMERS=1 at creation:
+// 0 H_core finish, 1 Fock, 2 energy+DIIS, 3 warm-start products, 4 Jacobi, 5 sort+C V, 6 density,
+// 7 loop bookkeeping and epilogue.
+bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err);
 
 // One stage of the pipeline on a single molecule (kernel-level parity hooks); see stage_kernel.
 bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,

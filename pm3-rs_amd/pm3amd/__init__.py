@@ -95,7 +95,7 @@ class _CResult(ctypes.Structure):
 
 EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_run", "pm3_gradient",
             "pm3_run_batch", "pm3_batch_create", "pm3_batch_solve", "pm3_batch_results", "pm3_batch_destroy",
-            "pm3_batch_last_kernel_ms", "pm3_stage_hcore", "pm3_stage_fock", "pm3_stage_eigen", "pm3_stage_eigen_warm",
+            "pm3_batch_last_kernel_ms", "pm3_batch_last_pair_ms", "pm3_batch_phase_ticks", "pm3_stage_hcore", "pm3_stage_fock", "pm3_stage_eigen", "pm3_stage_eigen_warm",
             "pm3_stage_gemm",
             "pm3_last_error_message"]
 
@@ -126,6 +126,9 @@ def native():
         L.pm3_batch_destroy.argtypes = [ctypes.c_void_p]
         L.pm3_batch_last_kernel_ms.argtypes = [ctypes.c_void_p]
         L.pm3_batch_last_kernel_ms.restype = ctypes.c_double
+        L.pm3_batch_last_pair_ms.argtypes = [ctypes.c_void_p]
+        L.pm3_batch_last_pair_ms.restype = ctypes.c_double
+        L.pm3_batch_phase_ticks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
         L.pm3_last_error_message.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -326,8 +329,20 @@ class Batch:
         _raise(native().pm3_batch_solve(self._h))
 
     def last_kernel_ms(self):
-        """Device time of the last solve's persistent launch (HIP events on its stream)."""
+        """Device time of the last solve's persistent SCF launch (HIP events on its stream)."""
         return native().pm3_batch_last_kernel_ms(self._h)
+
+    def last_pair_ms(self):
+        """Device time of the last solve's pair-integral launch."""
+        return native().pm3_batch_last_pair_ms(self._h)
+
+    PHASES = ("hcore", "fock", "energy_diis", "warm_products", "jacobi", "sort_cv", "density", "other")
+
+    def phase_ticks(self):
+        """Per-phase wall-clock ticks (10 ns) of the last solve (PM3_PHASE_TIMERS=1 at creation)."""
+        out = (ctypes.c_uint64 * 8)()
+        _raise(native().pm3_batch_phase_ticks(self._h, out))
+        return dict(zip(self.PHASES, [int(v) for v in out]))
 
     def results(self):
         """(status array, list of Pm3Result-like objects with density when requested)."""
