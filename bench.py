@@ -159,8 +159,10 @@ def main():
     }
     if os.environ.get("PM3_PHASE_TIMERS") == "1":
         ticks = batch.phase_ticks()
+        counts = ticks.pop("counts")
         tot = sum(ticks.values()) or 1
         out["phase_share"] = {k: round(v / tot, 4) for k, v in ticks.items()}
+        out["phase_counts"] = counts
     print(json.dumps(out))
 
 

@@ -129,8 +129,9 @@ double pm3_batch_last_kernel_ms(const pm3_batch* batch);
 double pm3_batch_last_pair_ms(const pm3_batch* batch);
 /* Per-phase wall-clock ticks (10 ns) of the last solve summed over workgroups, collected only when
  * the environment has PM3_PHASE_TIMERS=1 at pm3_batch_create (zeros otherwise): H_core finish,
- * Fock, energy+DIIS, warm-start products, Jacobi, sort + C V, density, bookkeeping. */
-pm3_status pm3_batch_phase_ticks(const pm3_batch* batch, uint64_t* out8);
+ * Fock, energy+DIIS, warm-start products, Jacobi, sort + C V, density, bookkeeping; then out[8]
+ * unused, out[9] Jacobi sweeps, out[10] eigensolves, out[11] Jacobi rounds. */
+pm3_status pm3_batch_phase_ticks(const pm3_batch* batch, uint64_t* out12);
 
 /* Detail text of the calling thread's last error ("" if none). */
 const char* pm3_last_error_message(void);

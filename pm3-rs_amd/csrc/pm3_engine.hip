@@ -303,7 +303,7 @@ __device__ double finish_core(const KArgs& A, const Mol& M, double* red) {
 // F = H + G(P): RHF Fock build (src/fock.rs:285-332, :488-500).  Off-diagonal atom blocks are
 // owned by their pair (exchange K); the Coulomb contributions J to the diagonal blocks are summed
 // per atom in LDS, then each diagonal block adds its one-centre term.
-__device__ void build_fock(const KArgs& A, const Mol& M, const double* H, const double* P, double* F, double* lds) {
+__device__ __noinline__ void build_fock(const KArgs& A, const Mol& M, const double* H, const double* P, double* F, double* lds) {
   const int tid = threadIdx.x;
   const int np = M.np, nat = M.nat;
   double* J = lds;                              // [nat][10] Coulomb sums per atom
@@ -485,59 +485,80 @@ __device__ __host__ inline int jac_lds_base(int n) {
 }
 __device__ __host__ inline int jac_lds_with_v(int n) { return jac_lds_base(n) + n * n; }
 
-// Largest |off-diagonal| of the packed matrix: each thread walks one contiguous chunk, decoding its
-// starting (i, j) once and stepping the pair incrementally.
-__device__ inline double packed_offmax(const double* Ap, int npk, double* red) {
-  const int tid = threadIdx.x;
-  const int chunk = (npk + NT - 1) / NT;
-  const int i0 = tid * chunk;
-  double off = 0.0;
-  if (i0 < npk) {
-    int j = (int)((sqrtf(8.0f * (float)i0 + 1.0f) - 1.0f) * 0.5f);
-    while (j * (j + 1) / 2 > i0) --j;
-    while ((j + 1) * (j + 2) / 2 <= i0) ++j;
-    int i = i0 - j * (j + 1) / 2;
-    const int iend = min(i0 + chunk, npk);
-    for (int idx = i0; idx < iend; ++idx) {
-      if (i != j) off = fmax(off, fabs(Ap[idx]));
-      if (++i > j) { i = 0; ++j; }
-    }
-  }
-  return block_max(off, red);
-}
+// Dynamic LDS of the SCF kernels.  The eigensolver indexes it directly (not through a pointer
+// parameter) so every access compiles to ds_read / ds_write rather than flat operations.
+extern __shared__ double g_lds[];
 
-__device__ int jacobi(const double* Ain, int n, int np, double* V, int ldv, double* lam, double* lds, double* red,
-                      int max_sweeps) {
+constexpr int JB = 10;  // upper bound on the triangle blocks one thread owns (m(m+1)/2 <= JB * NT)
+
+// V (eigenvectors, column-major, ld ldv) is in LDS at g_lds + vbase when VL, else at Vg.
+template <bool VL>
+__device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg, int ldv, int vbase, double* lam,
+                                   double* red, int max_sweeps) {
   const int tid = threadIdx.x;
   const int nn = n + (n & 1), m = nn >> 1;
   const int npk = n * (n + 1) / 2;
-  double* Ap = lds;
-  double* cs = lds + npk;               // [m][3]
-  int* pq = (int*)(lds + npk + 3 * m);  // [m][2]
+  // g_lds: [0, npk) packed A | [npk, npk+3m) rotation (c, s, t) | [npk+3m, npk+4m) pair schedule
+  const int cso = npk, pqo = npk + 3 * m;
+  int* pq = (int*)(g_lds + pqo);
   for (int j = 0; j < n; ++j)
-    for (int i = tid; i <= j; i += NT) Ap[j * (j + 1) / 2 + i] = Ain[i * np + j];
+    for (int i = tid; i <= j; i += NT) g_lds[j * (j + 1) / 2 + i] = Ain[i * np + j];
   for (int e = tid; e < n * ldv; e += NT) {
-    const int c = e / ldv, r = e % ldv;
-    V[e] = (c == r) ? 1.0 : 0.0;
+    const int c = e / ldv, r = e - c * ldv;
+    const double v = (c == r) ? 1.0 : 0.0;
+    if (VL) g_lds[vbase + e] = v; else Vg[e] = v;
+  }
+  // this thread's blocks (K, L), K <= L, of the m x m slot grid, packed as K | L << 16
+  int kl[JB];
+  const int nblk = m * (m + 1) / 2;
+  {
+    int L = 0, base = 0;  // walk the triangle row by row: row L holds K = 0..L
+#pragma unroll
+    for (int j = 0; j < JB; ++j) {
+      const int bidx = tid + j * NT;
+      kl[j] = -1;
+      if (bidx < nblk) {
+        while (base + L + 1 <= bidx) { base += L + 1; ++L; }
+        kl[j] = (bidx - base) | (L << 16);
+      }
+    }
   }
   __syncthreads();
   double dmax = 0.0;
-  for (int i = tid; i < n; i += NT) dmax = fmax(dmax, fabs(Ap[i * (i + 1) / 2 + i]));
+  for (int i = tid; i < n; i += NT) dmax = fmax(dmax, fabs(g_lds[i * (i + 1) / 2 + i]));
   const double scale = fmax(block_max(dmax, red), 1.0);
   const double tol = 1.0e-14 * scale;
-  const int nblk = m * (m + 1) / 2;
+  // eigenvector update mapping: row i = tid % n, slots k = tid / n, tid / n + rstep, ...
+  const int rstep = NT / max(n, 1);
+  const int vi = tid % max(n, 1), vk0 = tid / max(n, 1);
   int sweep = 0;
   const int sweeps = n < 2 ? 0 : max_sweeps;
   for (; sweep < sweeps; ++sweep) {
-    if (packed_offmax(Ap, npk, red) <= tol) break;
+    {  // largest off-diagonal: each thread walks one contiguous chunk of the packed triangle
+      const int chunk = (npk + NT - 1) / NT;
+      const int i0 = tid * chunk;
+      double off = 0.0;
+      if (i0 < npk) {
+        int j = (int)((sqrtf(8.0f * (float)i0 + 1.0f) - 1.0f) * 0.5f);
+        while (j * (j + 1) / 2 > i0) --j;
+        while ((j + 1) * (j + 2) / 2 <= i0) ++j;
+        int i = i0 - j * (j + 1) / 2;
+        const int iend = min(i0 + chunk, npk);
+        for (int idx = i0; idx < iend; ++idx) {
+          if (i != j) off = fmax(off, fabs(g_lds[idx]));
+          if (++i > j) { i = 0; ++j; }
+        }
+      }
+      if (block_max(off, red) <= tol) break;
+    }
     for (int r = 0; r < nn - 1; ++r) {
       for (int k = tid; k < m; k += NT) {
         int p, q;
         rr_pair(k, r, nn, p, q);
         double c = 1.0, s = 0.0, t = 0.0;
         if (p < n && q < n) {
-          const double apq = Ap[sym_idx(p, q)];
-          const double app = Ap[sym_idx(p, p)], aqq = Ap[sym_idx(q, q)];
+          const double apq = g_lds[sym_idx(p, q)];
+          const double app = g_lds[sym_idx(p, p)], aqq = g_lds[sym_idx(q, q)];
           if (fabs(apq) > 1.0e-300 && fabs(apq) > 1.0e-18 * (fabs(app) + fabs(aqq))) {
             const double tau = (aqq - app) / (2.0 * apq);
             t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
@@ -545,64 +566,97 @@ __device__ int jacobi(const double* Ain, int n, int np, double* V, int ldv, doub
             s = t * c;
           }
         }
-        cs[3 * k] = c;
-        cs[3 * k + 1] = s;
-        cs[3 * k + 2] = t;
+        g_lds[cso + 3 * k] = c;
+        g_lds[cso + 3 * k + 1] = s;
+        g_lds[cso + 3 * k + 2] = t;
         pq[2 * k] = p;
         pq[2 * k + 1] = q;
       }
       __syncthreads();
-      // blocks (K, L) with K <= L of the m x m slot grid, enumerated row by row of the triangle
-      for (int bidx = tid; bidx < nblk; bidx += NT) {
-        int K = (int)((sqrtf(8.0f * (float)bidx + 1.0f) - 1.0f) * 0.5f);  // bidx = K(K+1)/2 + L', L' <= K
-        while (K * (K + 1) / 2 > bidx) --K;
-        while ((K + 1) * (K + 2) / 2 <= bidx) ++K;
-        const int Lp = bidx - K * (K + 1) / 2;
-        const int L = K, Kk = Lp;  // block (Kk, L) with Kk <= L
-        const double sK = cs[3 * Kk + 1], sL = cs[3 * L + 1];
-        const int p = pq[2 * Kk], q = pq[2 * Kk + 1];
-        if (Kk == L) {
-          const double tK = cs[3 * Kk + 2];
-          if (p < n && q < n && tK != 0.0) {
-            const int ipq = sym_idx(p, q);
-            const double apq = Ap[ipq];
-            Ap[sym_idx(p, p)] -= tK * apq;
-            Ap[sym_idx(q, q)] += tK * apq;
-            Ap[ipq] = 0.0;
+      // two-sided update of every block: J_K^T A_KL J_L (the diagonal blocks become diagonal),
+      // four owned blocks at a time so their LDS reads overlap
+#pragma unroll
+      for (int j0 = 0; j0 < JB; j0 += 4) {
+        double x[4][4];
+        int ix[4][4];
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int j = j0 + jj;
+          ix[jj][0] = ix[jj][1] = ix[jj][2] = ix[jj][3] = -1;
+          if (j >= JB || kl[j] < 0) continue;
+          const int K = kl[j] & 0xffff, L = kl[j] >> 16;
+          const int p = pq[2 * K], q = pq[2 * K + 1];
+          if (K == L) {
+            if (p < n && q < n) ix[jj][0] = sym_idx(p, q);
+          } else {
+            const int u = pq[2 * L], v = pq[2 * L + 1];
+            if (p < n && u < n) ix[jj][0] = sym_idx(p, u);
+            if (p < n && v < n) ix[jj][1] = sym_idx(p, v);
+            if (q < n && u < n) ix[jj][2] = sym_idx(q, u);
+            if (q < n && v < n) ix[jj][3] = sym_idx(q, v);
           }
-        } else {
-          if (sK == 0.0 && sL == 0.0) continue;
-          const int u = pq[2 * L], v = pq[2 * L + 1];
-          const double cK = cs[3 * Kk], cL = cs[3 * L];
-          const bool vp = p < n, vq = q < n, vu = u < n, vv = v < n;
-          const int ipu = sym_idx(p, u), ipv = sym_idx(p, v), iqu = sym_idx(q, u), iqv = sym_idx(q, v);
-          const double apu = (vp && vu) ? Ap[ipu] : 0.0;
-          const double apv = (vp && vv) ? Ap[ipv] : 0.0;
-          const double aqu = (vq && vu) ? Ap[iqu] : 0.0;
-          const double aqv = (vq && vv) ? Ap[iqv] : 0.0;
-          const double xpu = cK * apu - sK * aqu, xqu = sK * apu + cK * aqu;
-          const double xpv = cK * apv - sK * aqv, xqv = sK * apv + cK * aqv;
-          if (vp && vu) Ap[ipu] = cL * xpu - sL * xpv;
-          if (vp && vv) Ap[ipv] = sL * xpu + cL * xpv;
-          if (vq && vu) Ap[iqu] = cL * xqu - sL * xqv;
-          if (vq && vv) Ap[iqv] = sL * xqu + cL * xqv;
+#pragma unroll
+          for (int w = 0; w < 4; ++w) x[jj][w] = ix[jj][w] >= 0 ? g_lds[ix[jj][w]] : 0.0;
+        }
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int j = j0 + jj;
+          if (j >= JB || kl[j] < 0) continue;
+          const int K = kl[j] & 0xffff, L = kl[j] >> 16;
+          if (K == L) {
+            const double tK = g_lds[cso + 3 * K + 2];
+            if (ix[jj][0] >= 0 && tK != 0.0) {
+              const int p = pq[2 * K], q = pq[2 * K + 1];
+              g_lds[sym_idx(p, p)] -= tK * x[jj][0];
+              g_lds[sym_idx(q, q)] += tK * x[jj][0];
+              g_lds[ix[jj][0]] = 0.0;
+            }
+          } else {
+            const double cK = g_lds[cso + 3 * K], sK = g_lds[cso + 3 * K + 1];
+            const double cL = g_lds[cso + 3 * L], sL = g_lds[cso + 3 * L + 1];
+            if (sK == 0.0 && sL == 0.0) continue;
+            const double xpu = cK * x[jj][0] - sK * x[jj][2], xqu = sK * x[jj][0] + cK * x[jj][2];
+            const double xpv = cK * x[jj][1] - sK * x[jj][3], xqv = sK * x[jj][1] + cK * x[jj][3];
+            if (ix[jj][0] >= 0) g_lds[ix[jj][0]] = cL * xpu - sL * xpv;
+            if (ix[jj][1] >= 0) g_lds[ix[jj][1]] = sL * xpu + cL * xpv;
+            if (ix[jj][2] >= 0) g_lds[ix[jj][2]] = cL * xqu - sL * xqv;
+            if (ix[jj][3] >= 0) g_lds[ix[jj][3]] = sL * xqu + cL * xqv;
+          }
         }
       }
-      for (int e = tid; e < n * m; e += NT) {
-        const int k = e / n, i = e - k * n;
-        const double s = cs[3 * k + 1];
-        if (s == 0.0) continue;
-        const int p = pq[2 * k], q = pq[2 * k + 1];
-        if (p >= n || q >= n) continue;
-        const double c = cs[3 * k];
-        const double vpv = V[p * ldv + i], vqv = V[q * ldv + i];
-        V[p * ldv + i] = c * vpv - s * vqv;
-        V[q * ldv + i] = s * vpv + c * vqv;
+      // V <- V J: thread owns row vi, slots vk0, vk0 + rstep, ...
+      if (vk0 < rstep) {
+        for (int k0 = vk0; k0 < m; k0 += 8 * rstep) {
+          double vp[8], vq[8], c8[8], s8[8];
+          int op[8], oq[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int k = k0 + u * rstep;
+            op[u] = -1;
+            if (k >= m) continue;
+            const double sv = g_lds[cso + 3 * k + 1];
+            const int p = pq[2 * k], q = pq[2 * k + 1];
+            if (sv == 0.0 || p >= n || q >= n) continue;
+            s8[u] = sv;
+            c8[u] = g_lds[cso + 3 * k];
+            op[u] = p * ldv + vi;
+            oq[u] = q * ldv + vi;
+            vp[u] = VL ? g_lds[vbase + op[u]] : Vg[op[u]];
+            vq[u] = VL ? g_lds[vbase + oq[u]] : Vg[oq[u]];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            if (op[u] < 0) continue;
+            const double a = c8[u] * vp[u] - s8[u] * vq[u], b = s8[u] * vp[u] + c8[u] * vq[u];
+            if (VL) { g_lds[vbase + op[u]] = a; g_lds[vbase + oq[u]] = b; }
+            else { Vg[op[u]] = a; Vg[oq[u]] = b; }
+          }
+        }
       }
       __syncthreads();
     }
   }
-  for (int i = tid; i < n; i += NT) lam[i] = Ap[i * (i + 1) / 2 + i];
+  for (int i = tid; i < n; i += NT) lam[i] = g_lds[i * (i + 1) / 2 + i];
   __syncthreads();
   return sweep;
 }
@@ -620,7 +674,7 @@ struct Diis {
 
 // Pulay CDIIS coefficients from the (scale-normalised) bordered Gram system with a pivot guard
 // (src/scf.rs:1544-1630).  Single thread.
-__device__ bool cdiis_coeffs(const Diis& d, double* c) {
+__device__ __noinline__ bool cdiis_coeffs(const Diis& d, double* c) {
   const int n = d.len;
   if (n < 2) return false;
   double scale = 0.0;
@@ -667,7 +721,7 @@ __device__ bool cdiis_coeffs(const Diis& d, double* c) {
 
 // A-DIIS (Hu & Yang 2010): projected gradient on the probability simplex, 400 steps at most
 // (src/scf.rs:1637-1693).  Single thread.
-__device__ void adiis_coeffs(const Diis& D, double* c) {
+__device__ __noinline__ void adiis_coeffs(const Diis& D, double* c) {
   const int n = D.len, last = n - 1;
   const double mnn = D.mm[last][last];
   double d[MAXD], s[MAXD][MAXD];
@@ -732,7 +786,7 @@ __device__ inline void stamp(const Bufs& B, int k) {
 
 // Eigenvectors of Fu -> Cn (row-major, columns sorted by ascending eigenvalue) and eps.
 // warm != 0: Jacobi on C^T Fu C and Cn = C V; otherwise Jacobi on Fu itself.
-__device__ void eigensolve(const Mol& M, const Bufs& B, const double* Fu, bool warm, double* lds, double* red) {
+__device__ __noinline__ void eigensolve(const Mol& M, const Bufs& B, const double* Fu, bool warm, double* lds, double* red) {
   const int n = M.n, np = M.np, tid = threadIdx.x;
   const double* Amat = Fu;
   if (warm) {
@@ -752,7 +806,13 @@ __device__ void eigensolve(const Mol& M, const Bufs& B, const double* Fu, bool w
   const bool v_in_lds = jac_lds_with_v(n) <= M.lds_doubles;
   double* V = v_in_lds ? lds + jac_lds_base(n) : B.V;
   const int ldv = v_in_lds ? n : np;
-  jacobi(Amat, n, np, V, ldv, B.lam, lds, red, 60);
+  const int nsw = v_in_lds ? jacobi<true>(Amat, n, np, nullptr, ldv, jac_lds_base(n), B.lam, red, 60)
+                           : jacobi<false>(Amat, n, np, B.V, ldv, 0, B.lam, red, 60);
+  if (B.ph && threadIdx.x == 0) {
+    B.ph[9] += (unsigned long long)nsw;
+    B.ph[10] += 1ull;
+    B.ph[11] += (unsigned long long)nsw * (unsigned long long)(n + (n & 1) - 1);
+  }
   stamp(B, 4);
   // ascending order, ties by index
   for (int k = tid; k < n; k += NT) {
@@ -791,7 +851,7 @@ __device__ void density(const Mol& M, const double* Cn, double* Pdst) {
 }
 
 // Initial density for a guess mode (src/scf.rs:855-935).  Core guess uses the eigenvectors of H.
-__device__ void initial_guess(const KArgs& A, const Mol& M, const Bufs& B, int guess, double* lds, double* red) {
+__device__ __noinline__ void initial_guess(const KArgs& A, const Mol& M, const Bufs& B, int guess, double* lds, double* red) {
   const int np = M.np, tid = threadIdx.x;
   for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
   __syncthreads();
@@ -1073,22 +1133,23 @@ __global__ __launch_bounds__(256) void pair_kernel(KArgs A) {
 }
 
 __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
-  extern __shared__ double lds[];
+  double* lds = g_lds;
   __shared__ double red[NW * (3 * MAXD + 2)];
   __shared__ Diis D;
   __shared__ int s_mol;
-  __shared__ unsigned long long s_ph[9];
+  __shared__ unsigned long long s_ph[12];
   const int tid = threadIdx.x;
   double* Wbase;
   Bufs B = slab(A, Wbase);
   B.ph = A.prof ? s_ph : nullptr;
-  if (tid < 9) s_ph[tid] = 0;
+  if (tid < 12) s_ph[tid] = 0;
   if (tid == 0) s_ph[8] = wall_clock64();
 
   for (;;) {
     if (tid == 0) s_mol = atomicAdd(A.queue, 1);
     if (tid == 0 && A.prof) {
-      for (int k = 0; k < 8; ++k) {
+      for (int k = 0; k < 12; ++k) {
+        if (k == 8) continue;
         atomicAdd(&A.prof[k], s_ph[k]);
         s_ph[k] = 0;
       }
@@ -1206,7 +1267,7 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
 // mode 4 warm eigensolve of in[0:n*n] in the basis in[n*n:2n*n]: out = eps[n] then C[n*n].
 // Matrices in `in`/`out` are dense n x n row-major in the engine's (heavy-first) AO order.
 __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const double* in, double* out) {
-  extern __shared__ double lds[];
+  double* lds = g_lds;
   __shared__ double red[NW * (3 * MAXD + 2)];
   const int tid = threadIdx.x;
   double* Wbase;
@@ -1463,7 +1524,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   if (const char* ev = std::getenv("PM3_PHASE_TIMERS")) {
     if (ev[0] == '1') {
       unsigned long long* d_prof;
-      if (!b.alloc(d_prof, 8, err)) return false;
+      if (!b.alloc(d_prof, 12, err)) return false;
       a.prof = d_prof;
     }
   }
@@ -1490,7 +1551,7 @@ bool batch_solve(Batch* bp, std::string& err) {
     HIP_OK(hipEventCreate(&b.ev2));
   }
   HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), 0));
-  if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, 8 * sizeof(unsigned long long), 0));
+  if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, 12 * sizeof(unsigned long long), 0));
   HIP_OK(hipEventRecord(b.ev0, 0));
   hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(256), 0, 0, b.args);
   HIP_OK(hipGetLastError());
@@ -1511,9 +1572,9 @@ double batch_kernel_ms(const Batch* b) { return b ? b->last_kernel_ms : 0.0; }
 double batch_pair_ms(const Batch* b) { return b ? b->last_pair_ms : 0.0; }
 
 bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err) {
-  for (int k = 0; k < 8; ++k) out[k] = 0;
+  for (int k = 0; k < 12; ++k) out[k] = 0;
   if (!b || !b->args.prof) return true;
-  HIP_OK(hipMemcpy(out, b->args.prof, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(out, b->args.prof, 12 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   return true;
 }
 

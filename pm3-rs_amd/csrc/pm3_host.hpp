@@ -71,7 +71,7 @@ double batch_kernel_ms(const Batch* b);  // device time of the last solve's SCF 
 double batch_pair_ms(const Batch* b);    // device time of the last solve's pair-integral launch
 // Per-phase wall-clock ticks (10 ns) summed over workgroups, when PM3_PHASE_TIMERS=1 at creation:
 // 0 H_core finish, 1 Fock, 2 energy+DIIS, 3 warm-start products, 4 Jacobi, 5 sort+C V, 6 density,
-// 7 loop bookkeeping and epilogue.
+// 7 loop bookkeeping and epilogue; then [8] unused, [9] Jacobi sweeps, [10] eigensolves, [11] rounds.
 bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err);
 
 // One stage of the pipeline on a single molecule (kernel-level parity hooks); see stage_kernel.

@@ -340,9 +340,11 @@ class Batch:
 
     def phase_ticks(self):
         """Per-phase wall-clock ticks (10 ns) of the last solve (PM3_PHASE_TIMERS=1 at creation)."""
-        out = (ctypes.c_uint64 * 8)()
+        out = (ctypes.c_uint64 * 12)()
         _raise(native().pm3_batch_phase_ticks(self._h, out))
-        return dict(zip(self.PHASES, [int(v) for v in out]))
+        d = dict(zip(self.PHASES, [int(v) for v in out[:8]]))
+        d["counts"] = {"jacobi_sweeps": int(out[9]), "eigensolves": int(out[10]), "jacobi_rounds": int(out[11])}
+        return d
 
     def results(self):
         """(status array, list of Pm3Result-like objects with density when requested)."""
