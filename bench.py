@@ -66,6 +66,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--cpu-sample", type=int, default=512, help="molecules in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--gradient", action="store_true", help="also compute analytic gradients in every step")
     ap.add_argument("--copy-calibration-mb", type=float, default=0.0,
                     help="profiling helper: run only a plain device copy of this size (for counter scaling)")
     args = ap.parse_args()
@@ -88,7 +89,7 @@ def main():
 
     mols = build_workload(args.workload, args.seed)
     opts = G.Pm3Options(p_tol=1.0e-8, e_tol=1.0e-8)
-    batch = G.Batch([G.Molecule(z, x) for z, x in mols], options=opts)
+    batch = G.Batch([G.Molecule(z, x) for z, x in mols], options=opts, want_gradient=args.gradient)
     for _ in range(args.warmup):
         batch.solve()
     kernel_ms, pair_ms = [], []
@@ -108,6 +109,7 @@ def main():
     achieved = flops / (k_ms * 1e-3) / 1e12
 
     cpu = None
+    grad_check = None
     max_dhf = None
     checked = 0
     hof_ok = True
@@ -121,6 +123,18 @@ def main():
         c0 = time.perf_counter()
         _, hof_cpu, _, st_cpu = O.run_batch(sample, O.options(p_tol=1.0e-8, e_tol=1.0e-8), nthreads=threads)
         c1 = time.perf_counter()
+        if args.gradient:
+            # the CPU path computes the gradient too (closed_form_gradient); time it on a few
+            gsub = sample[: max(1, len(sample) // 8)]
+            g0 = time.perf_counter()
+            gmax = 0.0
+            gpu_g = batch.gradients()
+            for k, (z, x) in enumerate(gsub):
+                so, o = O.gradient(z, x, 0.0, O.options(p_tol=1.0e-8, e_tol=1.0e-8))
+                gmax = max(gmax, float(np.abs(gpu_g[idx[k]] - o["gradient"]).max()))
+            g1 = time.perf_counter()
+            grad_check = {"molecules": len(gsub), "max_abs_dgrad_ev_per_bohr": gmax,
+                          "cpu_gradient_s_per_molecule_1thread": (g1 - g0) / len(gsub)}
         dh = np.array([res[i].heat_of_formation_kcal - hof_cpu[k] for k, i in enumerate(idx)
                        if st_cpu[k] == 0 and status[i] == 0])
         checked = int(dh.size)
@@ -157,6 +171,9 @@ def main():
                      "pair_kernel_ms": float(np.mean(pair_ms))},
         "cpu_baseline": cpu,
     }
+    if args.gradient:
+        out["gradient_check"] = grad_check
+        out["config"]["gradient"] = True
     if os.environ.get("PM3_PHASE_TIMERS") == "1":
         ticks = batch.phase_ticks()
         counts = ticks.pop("counts")

@@ -115,7 +115,7 @@ pm3_status pm3_run_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* 
  * counterpart: this is the batched entry the reference lacks; pm3_run_batch is
  * create + solve + results + destroy. */
 typedef struct pm3_batch pm3_batch;
-enum { PM3_BATCH_WANT_DENSITY = 1 };
+enum { PM3_BATCH_WANT_DENSITY = 1, PM3_BATCH_WANT_GRADIENT = 2 };
 pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
                             const double* charge, const pm3_params* params, const pm3_options* opts, int32_t flags,
                             pm3_batch** out);
@@ -123,6 +123,9 @@ pm3_status pm3_batch_solve(pm3_batch* batch);
 pm3_status pm3_batch_results(pm3_batch* batch, pm3_status* status, pm3_result* results, double* charges,
                              double* density);
 void pm3_batch_destroy(pm3_batch* batch);
+/* Analytic gradients of the last solve (batch created with PM3_BATCH_WANT_GRADIENT), eV/Bohr,
+ * gradient[3 * (atom_ptr[k] + i) + c]; the same per-molecule gradient pm3_gradient returns. */
+pm3_status pm3_batch_gradient(pm3_batch* batch, double* gradient);
 /* Device time (ms, HIP events on the launch stream) of the last pm3_batch_solve's kernel. */
 double pm3_batch_last_kernel_ms(const pm3_batch* batch);
 /* Device time (ms) of the last pm3_batch_solve's pair-integral kernel. */

@@ -196,6 +196,7 @@ pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_
   }
   RunRequest req;
   req.want_density = (flags & PM3_BATCH_WANT_DENSITY) != 0;
+  req.want_gradient = (flags & PM3_BATCH_WANT_GRADIENT) != 0;
   std::string err;
   if (!batch_create(params->p, h->mols, eo, req, h->b, err)) {
     batch_destroy(h->b);
@@ -251,6 +252,22 @@ pm3_status pm3_batch_phase_ticks(const pm3_batch* h, uint64_t* out) {
   return PM3_OK;
 }
 
+pm3_status pm3_batch_gradient(pm3_batch* h, double* gradient) {
+  g_last_error.clear();
+  if (!h || !gradient) return fail(PM3_INVALID_INPUT, "null argument");
+  pm3_status s = pm3_batch_results(h, nullptr, nullptr, nullptr, nullptr);
+  if (s != PM3_OK) return s;
+  size_t off = 0;
+  for (size_t k = 0; k < h->mols.size(); ++k) {
+    const MolResult& r = h->results[k];
+    const size_t na = (size_t)h->mols[k].n_atoms;
+    if (r.gradient.size() == 3 * na) std::memcpy(gradient + 3 * off, r.gradient.data(), 3 * na * sizeof(double));
+    else std::memset(gradient + 3 * off, 0, 3 * na * sizeof(double));
+    off += na;
+  }
+  return PM3_OK;
+}
+
 void pm3_batch_destroy(pm3_batch* h) {
   if (!h) return;
   batch_destroy(h->b);
@@ -260,21 +277,41 @@ void pm3_batch_destroy(pm3_batch* h) {
 pm3_status pm3_run_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
                          const double* charge, const pm3_params* params, const pm3_options* opts, int32_t want_gradient,
                          pm3_status* status, pm3_result* results, double* gradient) {
-  if (want_gradient) return fail(PM3_UNSUPPORTED, "batched gradients are not available yet");
-  (void)gradient;
+  if (want_gradient && !gradient) return fail(PM3_INVALID_INPUT, "gradient buffer required");
   pm3_batch* h = nullptr;
-  pm3_status s = pm3_batch_create(n_mol, atom_ptr, z, xyz_bohr, charge, params, opts, 0, &h);
+  pm3_status s = pm3_batch_create(n_mol, atom_ptr, z, xyz_bohr, charge, params, opts,
+                                  want_gradient ? PM3_BATCH_WANT_GRADIENT : 0, &h);
   if (s != PM3_OK) return s;
   s = pm3_batch_solve(h);
   if (s == PM3_OK) s = pm3_batch_results(h, status, results, nullptr, nullptr);
+  if (s == PM3_OK && want_gradient) s = pm3_batch_gradient(h, gradient);
   pm3_batch_destroy(h);
   return s;
 }
 
 pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
                         pm3_result* result, double* gradient, double* density) {
-  (void)mol; (void)params; (void)opts; (void)result; (void)gradient; (void)density;
-  return fail(PM3_UNSUPPORTED, "analytic gradient kernel not built yet");
+  g_last_error.clear();
+  if (!params || !gradient) return fail(PM3_INVALID_INPUT, "null argument");
+  pm3_status s = check_molecule(mol);
+  if (s != PM3_OK) return s;
+  EngineOptions eo;
+  std::string why;
+  if (!engine_options(opts, eo, why)) return fail(PM3_INVALID_INPUT, why);
+  std::vector<MolIn> mols{MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge}};
+  RunRequest req;
+  req.want_density = true;
+  req.want_gradient = true;
+  std::vector<MolResult> out;
+  std::string err;
+  if (!engine_run(params->p, mols, eo, req, out, err)) return fail(PM3_DEVICE_ERROR, err);
+  const MolResult& r = out[0];
+  if (result) fill_result(r, result);
+  if (r.status == PM3_OK) {
+    std::memcpy(gradient, r.gradient.data(), 3 * (size_t)mol->n_atoms * sizeof(double));
+    if (density) std::memcpy(density, r.density.data(), (size_t)r.nao * r.nao * sizeof(double));
+  }
+  return molecule_status(r.status, r);
 }
 
 const char* pm3_last_error_message(void) { return g_last_error.c_str(); }

@@ -80,6 +80,7 @@ struct KArgs {
   double* out_C;
   double* out_eps;
   double* out_q;  // Mulliken charge per atom (sorted atom order)
+  double* out_g;  // gradient per atom (sorted atom order, eV/Bohr), when requested
   double* store;  // per-molecule H_core and pair tables written by pair_kernel
   const long long* pair_base;  // first global pair of each queue position (nmol + 1 entries)
   long long npairs_total;
@@ -1132,6 +1133,141 @@ __global__ __launch_bounds__(256) void pair_kernel(KArgs A) {
   }
 }
 
+
+// ------------------------------------------------------------------------------------------
+// Analytic gradient at the converged density (closed_form_gradient, src/gradient.rs:201-276;
+// electronic part src/gradient.rs:423-489, core-core src/repulsion.rs:30-49).  One work-item per
+// pair evaluates the pair's integrals in dual numbers seeded on d = R_b - R_a and contracts the
+// derivatives with the density on the fly:
+//   dE/dR_b = sum P_ab (beta_a + beta_b) dS + sum P_aa de1b + sum P_bb de2a
+//           + sum (P_aa P_bb - 1/2 P_ab P_ab) dW + dE_core
+// The per-pair force goes to a slot; grad_gather sums them per atom in a fixed order.
+__device__ inline void pair_gradient(const KArgs& A, const Mol& M, const double* P, int n, int p, double* out) {
+  const int nat = M.nat;
+  int a, b;
+  tri_decode(p, nat, a, b);
+  const int ga = M.atom0 + a, gb = M.atom0 + b;
+  const int sa = A.aslot[ga], sb = A.aslot[gb];
+  const ElemConst& ea = c_tab.el[sa];
+  const ElemConst& eb = c_tab.el[sb];
+  const int oa = A.aoff[ga], ob = A.aoff[gb];
+  const int na = ea.norb, nb = eb.norb;
+  const D3 dx = mk(A.ax[gb] - A.ax[ga], 1.0, 0.0, 0.0);
+  const D3 dy = mk(A.ay[gb] - A.ay[ga], 0.0, 1.0, 0.0);
+  const D3 dz = mk(A.az[gb] - A.az[ga], 0.0, 0.0, 1.0);
+  const D3 r = sqrtx(dx * dx + dy * dy + dz * dz);
+  const D3 inv = recip(r);
+  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+  auto acc = [&](double c, const D3& v) { f0 += c * v.d0; f1 += c * v.d1; f2 += c * v.d2; };
+  // core-core
+  acc(1.0, core_core_pair<D3>(c_tab, sa, sb, r));
+  // density blocks (P_aa, P_bb, P_ab) of the pair
+  double paa[4][4], pbb[4][4], pab[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      paa[i][j] = (i < na && j < na) ? P[(oa + i) * n + oa + j] : 0.0;
+      pbb[i][j] = (i < nb && j < nb) ? P[(ob + i) * n + ob + j] : 0.0;
+      pab[i][j] = (i < na && j < nb) ? P[(oa + i) * n + ob + j] : 0.0;
+    }
+  // electron-core attraction blocks
+  {
+    D3 e1b[10], e2a[10];
+    ecore(ea, eb, dx, dy, dz, e1b);
+    ecore(eb, ea, -dx, -dy, -dz, e2a);
+    for (int mu = 0; mu < na; ++mu)
+      for (int nu = 0; nu <= mu; ++nu) acc(mu == nu ? paa[mu][mu] : paa[mu][nu] + paa[nu][mu], e1b[pk(mu, nu)]);
+    for (int mu = 0; mu < nb; ++mu)
+      for (int nu = 0; nu <= mu; ++nu) acc(mu == nu ? pbb[mu][mu] : pbb[mu][nu] + pbb[nu][mu], e2a[pk(mu, nu)]);
+  }
+  // resonance
+  {
+    D3 S[4][4];
+    overlap_block(ea, eb, dx * inv, dy * inv, dz * inv, r, S);
+    for (int i = 0; i < na; ++i) {
+      const double bi = i == 0 ? ea.betas : ea.betap;
+      for (int j = 0; j < nb; ++j) acc(pab[i][j] * (bi + (j == 0 ? eb.betas : eb.betap)), S[i][j]);
+    }
+  }
+  // two-electron integrals
+  D3 R[3][3];
+  rot_to_x(-(dx * inv), -(dy * inv), -(dz * inv), R);
+  // coefficient of the packed element (mu nu | la si): both orderings of each index pair
+  auto coef = [&](int mu, int nu, int la, int si) {
+    double c = 0.0;
+    const int m2 = mu == nu ? 1 : 2, l2 = la == si ? 1 : 2;
+    for (int x = 0; x < m2; ++x)
+      for (int y = 0; y < l2; ++y) {
+        const int u = x ? nu : mu, v = x ? mu : nu, s = y ? si : la, t = y ? la : si;
+        c += paa[u][v] * pbb[s][t] - 0.5 * pab[u][s] * pab[v][t];
+      }
+    return c;
+  };
+  if (na == 4 && nb == 4) {
+    D3 ri[22];
+    local_xx(ea, eb, r, ri);
+    // idx = pk(kk, ll) * 10 + pk(mm, nn); unpack from the loop order of assemble_xx
+    assemble_xx(ri, R, [&](int idx, const D3& v) {
+      const int pp = idx / 10, qq = idx % 10;
+      const int k1 = pp >= 6 ? 3 : (pp >= 3 ? 2 : (pp >= 1 ? 1 : 0));
+      const int l1 = pp - k1 * (k1 + 1) / 2;
+      const int m1 = qq >= 6 ? 3 : (qq >= 3 ? 2 : (qq >= 1 ? 1 : 0));
+      const int n1 = qq - m1 * (m1 + 1) / 2;
+      acc(coef(k1, l1, m1, n1), v);
+    });
+  } else if (na == 4) {
+    D3 ri[4], w[10];
+    local_xh(ea.dd, ea.qq, ea.rho0, ea.rho1, ea.rho2, eb.rho0, r, ri);
+    assemble_xh(ri, R, w);
+    for (int mu = 0; mu < 4; ++mu)
+      for (int nu = 0; nu <= mu; ++nu) acc(coef(mu, nu, 0, 0), w[pk(mu, nu)]);
+  } else {
+    const double s = ea.rho0 + eb.rho0;
+    acc(coef(0, 0, 0, 0), rsq(r * r + s * s) * EV);
+  }
+  out[0] = f0;
+  out[1] = f1;
+  out[2] = f2;
+}
+
+// Per-pair forces for every pair of every molecule (grid-stride); the densities are the
+// converged ones in the output buffer (sorted AO order, dense n x n).
+__global__ __launch_bounds__(256) void grad_pair_kernel(KArgs A) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < A.npairs_total; g += stride) {
+    int lo = 0, hi = A.nmol;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (A.pair_base[mid] <= g) lo = mid; else hi = mid;
+    }
+    const MolDesc md = A.mols[A.order[lo]];
+    const Mol M = mol_of(A, md, nullptr);
+    const int p = (int)(g - A.pair_base[lo]);
+    double f[3];
+    pair_gradient(A, M, A.out_P + md.out_mat, md.nao, p, f);
+    const long long npairs = (long long)md.nat * (md.nat - 1) / 2;
+    M.S1[p] = f[0];  // the electron-core slots are free once H_core is assembled
+    M.S1[npairs + p] = f[1];
+    M.S1[2 * npairs + p] = f[2];
+  }
+}
+
+// g_x = sum_{b > x} (-f_xb) + sum_{a < x} f_ax, one workgroup per molecule.
+__global__ __launch_bounds__(256) void grad_gather_kernel(KArgs A) {
+  const MolDesc md = A.mols[A.order[blockIdx.x]];
+  const Mol M = mol_of(A, md, nullptr);
+  const int nat = md.nat;
+  const long long npairs = (long long)nat * (nat - 1) / 2;
+  for (int e = threadIdx.x; e < 3 * nat; e += blockDim.x) {
+    const int x = e / 3, c = e % 3;
+    double s = 0.0;
+    for (int y = 0; y < nat; ++y) {
+      if (y == x) continue;
+      s += y > x ? -M.S1[c * npairs + tri_index(x, y, nat)] : M.S1[c * npairs + tri_index(y, x, nat)];
+    }
+    A.out_g[3LL * (md.atom0 + x) + c] = s;
+  }
+}
+
 __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
   double* lds = g_lds;
   __shared__ double red[NW * (3 * MAXD + 2)];
@@ -1324,6 +1460,7 @@ struct Batch {
   std::vector<int> solve;
   std::vector<std::vector<int>> ao_map, atom_map;  // sorted -> caller indices
   std::vector<int> pre_status;
+  bool want_gradient = false;
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
   void* bufs[24] = {nullptr};
@@ -1479,7 +1616,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   const long long stride = ((11 + 3 * MAXD) * m2 + w_cap + 4LL * np_max + 64 + 31) & ~31LL;
   KArgs& a = b.args;
   MolDesc* d_desc; int *d_order, *d_queue, *d_aslot, *d_aoff;
-  double *d_ax, *d_ay, *d_az, *d_scr, *d_P, *d_C, *d_eps, *d_q, *d_store;
+  double *d_ax, *d_ay, *d_az, *d_scr, *d_P, *d_C, *d_eps, *d_q, *d_store, *d_g;
   long long* d_pbase;
   MolOut* d_out;
   if (!b.alloc(d_desc, nmol, err) || !b.alloc(d_order, b.nsolve, err) || !b.alloc(d_queue, 1, err) ||
@@ -1487,8 +1624,10 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
       !b.alloc(d_ay, b.atoms_total, err) || !b.alloc(d_az, b.atoms_total, err) || !b.alloc(d_scr, (size_t)(stride * b.nwg), err) ||
       !b.alloc(d_out, nmol, err) || !b.alloc(d_P, b.mat_total, err) || !b.alloc(d_C, b.mat_total, err) ||
       !b.alloc(d_eps, b.vec_total, err) || !b.alloc(d_q, b.atoms_total, err) ||
-      !b.alloc(d_store, (size_t)store_total, err) || !b.alloc(d_pbase, pair_base.size(), err))
+      !b.alloc(d_store, (size_t)store_total, err) || !b.alloc(d_pbase, pair_base.size(), err) ||
+      !b.alloc(d_g, 3 * (size_t)b.atoms_total, err))
     return false;
+  b.want_gradient = req.want_gradient;
   HIP_OK(hipMemset(d_store, 0, sizeof(double) * std::max<long long>(store_total, 1)));
   HIP_OK(hipMemcpy(d_pbase, pair_base.data(), sizeof(long long) * pair_base.size(), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_desc, b.desc.data(), sizeof(MolDesc) * nmol, hipMemcpyHostToDevice));
@@ -1516,6 +1655,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   a.out_C = d_C;
   a.out_eps = d_eps;
   a.out_q = d_q;
+  a.out_g = d_g;
   a.store = d_store;
   a.pair_base = d_pbase;
   a.npairs_total = pair_base.back();
@@ -1558,6 +1698,11 @@ bool batch_solve(Batch* bp, std::string& err) {
   HIP_OK(hipEventRecord(b.ev1, 0));
   hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
   HIP_OK(hipGetLastError());
+  if (b.want_gradient) {
+    hipLaunchKernelGGL(grad_pair_kernel, dim3(b.pair_blocks), dim3(256), 0, 0, b.args);
+    hipLaunchKernelGGL(grad_gather_kernel, dim3(b.nsolve), dim3(256), 0, 0, b.args);
+    HIP_OK(hipGetLastError());
+  }
   HIP_OK(hipEventRecord(b.ev2, 0));
   HIP_OK(hipEventSynchronize(b.ev2));
   float ms = 0.0f, pms = 0.0f;
@@ -1584,7 +1729,11 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
   for (int k = 0; k < b.nmol; ++k) out[k].status = b.pre_status[k];
   if (b.nsolve == 0) return true;
   std::vector<MolOut> ho(b.nmol);
-  std::vector<double> hq(b.atoms_total);
+  std::vector<double> hq(b.atoms_total), hg;
+  if (b.want_gradient) {
+    hg.resize(3 * b.atoms_total);
+    HIP_OK(hipMemcpy(hg.data(), b.args.out_g, sizeof(double) * hg.size(), hipMemcpyDeviceToHost));
+  }
   HIP_OK(hipMemcpy(ho.data(), b.args.out, sizeof(MolOut) * b.nmol, hipMemcpyDeviceToHost));
   HIP_OK(hipMemcpy(hq.data(), b.args.out_q, sizeof(double) * b.atoms_total, hipMemcpyDeviceToHost));
   std::vector<double> hP, hC, hE;
@@ -1615,6 +1764,11 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
     for (int x = 0; x < 3; ++x) r.dipole[x] = o.dip[x] * AU_DIPOLE_TO_DEBYE;
     r.charges.assign(d.nat, 0.0);
     for (int t = 0; t < d.nat; ++t) r.charges[b.atom_map[k][t]] = hq[d.atom0 + t];
+    if (b.want_gradient) {
+      r.gradient.assign(3 * (size_t)d.nat, 0.0);
+      for (int t = 0; t < d.nat; ++t)
+        for (int c = 0; c < 3; ++c) r.gradient[3 * b.atom_map[k][t] + c] = hg[3 * (d.atom0 + t) + c];
+    }
     double eis = 0.0, ehs = 0.0;
     for (int i = 0; i < mols[k].n_atoms; ++i) {
       const int s = P.slot_of_z[mols[k].z[i]];

@@ -95,7 +95,7 @@ class _CResult(ctypes.Structure):
 
 EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_run", "pm3_gradient",
             "pm3_run_batch", "pm3_batch_create", "pm3_batch_solve", "pm3_batch_results", "pm3_batch_destroy",
-            "pm3_batch_last_kernel_ms", "pm3_batch_last_pair_ms", "pm3_batch_phase_ticks", "pm3_stage_hcore", "pm3_stage_fock", "pm3_stage_eigen", "pm3_stage_eigen_warm",
+            "pm3_batch_last_kernel_ms", "pm3_batch_gradient", "pm3_batch_last_pair_ms", "pm3_batch_phase_ticks", "pm3_stage_hcore", "pm3_stage_fock", "pm3_stage_eigen", "pm3_stage_eigen_warm",
             "pm3_stage_gemm",
             "pm3_last_error_message"]
 
@@ -124,6 +124,7 @@ def native():
         L.pm3_batch_solve.argtypes = [ctypes.c_void_p]
         L.pm3_batch_results.argtypes = [ctypes.c_void_p, ip, ctypes.POINTER(_CResult), dp, dp]
         L.pm3_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.pm3_batch_gradient.argtypes = [ctypes.c_void_p, dp]
         L.pm3_batch_last_kernel_ms.argtypes = [ctypes.c_void_p]
         L.pm3_batch_last_kernel_ms.restype = ctypes.c_double
         L.pm3_batch_last_pair_ms.argtypes = [ctypes.c_void_p]
@@ -305,7 +306,7 @@ class Batch:
     """Device-resident batch of independent molecules (pm3_batch_* in include/pm3_gpu.h)."""
 
     def __init__(self, molecules: List[Molecule], params: Optional[Pm3Parameters] = None,
-                 options: Optional[Pm3Options] = None, want_density=False):
+                 options: Optional[Pm3Options] = None, want_density=False, want_gradient=False):
         self.params = params or Pm3Parameters.standard()
         self.options = options or Pm3Options()
         self.molecules = molecules
@@ -320,7 +321,7 @@ class Batch:
         ip = ctypes.POINTER(ctypes.c_int32)
         st = native().pm3_batch_create(len(molecules), ptr.ctypes.data_as(ip), self._z.ctypes.data_as(ip),
                                        _dp(self._xyz), _dp(self._q), self.params._h, ctypes.byref(self.options._c()),
-                                       1 if want_density else 0, ctypes.byref(h))
+                                       (1 if want_density else 0) | (2 if want_gradient else 0), ctypes.byref(h))
         _raise(st)
         self._h = h
         self.want_density = want_density
@@ -366,6 +367,12 @@ class Batch:
             moff += n * n
             out.append(_result(res[i], Pi, None, None, q[a0:a1].copy()))
         return status, out
+
+    def gradients(self):
+        """Per-molecule analytic gradients (eV/Bohr) of the last solve (needs want_gradient=True)."""
+        g = np.zeros((int(self.atom_ptr[-1]), 3))
+        _raise(native().pm3_batch_gradient(self._h, _dp(g)))
+        return [g[self.atom_ptr[i]:self.atom_ptr[i + 1]].copy() for i in range(len(self.molecules))]
 
     def close(self):
         if getattr(self, "_h", None):

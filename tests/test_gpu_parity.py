@@ -181,6 +181,40 @@ def test_largest_single_molecules(G, oracle):
         assert np.abs(P @ P - 2 * P).max() < 1e-9
 
 
+def test_gradient_matches_oracle(G, pins, oracle):
+    """closed_form_gradient on the GPU vs the dual-number restatement: 1e-5 kcal/mol/A
+    (2.295e-7 eV/Bohr) with both sides converged tightly; translational invariance."""
+    from pm3amd import workloads as W
+    mols = [atoms_to_zx(r["atoms"]) + (r["charge"],) for r in _rows(pins, ["water", "acetic_acid", "methylamine",
+                                                                           "chloroform", "hydroxide", "benzene"])]
+    zc, xc = W.solvated_conformers(1, 7)[0]
+    mols.append((np.asarray(zc), W.to_bohr(xc), 0.0))
+    for z, x, q in mols:
+        print("case grad nat", len(z), flush=True)
+        g = G.closed_form_gradient(G.Molecule(z, x, q), options=G.Pm3Options(**TIGHT))
+        st, o = oracle.gradient(z, x, q, oracle.options(**TIGHT))
+        assert st == 0
+        assert np.abs(g.gradient - o["gradient"]).max() < 2.295e-7
+        assert np.abs(g.gradient.sum(axis=0)).max() < 1e-8
+        assert np.allclose(g.forces, -g.gradient)
+        assert abs(g.energy_ev - o["total_ev"]) < 1e-7
+
+
+def test_batch_gradients(G, oracle):
+    from pm3amd import workloads as W
+    mols = [(np.array(z), W.to_bohr(x)) for z, x in W.mixed_batch(24, 5)]
+    b = G.Batch([G.Molecule(z, x) for z, x in mols], options=G.Pm3Options(**TIGHT), want_gradient=True)
+    b.solve()
+    st, res = b.results()
+    assert (st == 0).all()
+    grads = b.gradients()
+    for k in (0, 3, 17):
+        g1 = G.closed_form_gradient(G.Molecule(*mols[k]), options=G.Pm3Options(**TIGHT)).gradient
+        assert np.abs(grads[k] - g1).max() < 1e-12
+        so, o = oracle.gradient(mols[k][0], mols[k][1], 0.0, oracle.options(**TIGHT))
+        assert np.abs(grads[k] - o["gradient"]).max() < 2.295e-7
+
+
 def test_error_surface(G):
     z, x = atoms_to_zx([["O", 0, 0, 0], ["H", 0.9584, 0, 0], ["H", -0.24, 0.9278, 0]])
     with pytest.raises(G.ScfNotConverged) as e:
