@@ -1,9 +1,14 @@
-"""Throughput benchmark of the MI355X PM3 engine.
+"""Throughput benchmark of the MI355X PM3 engine: batched SCF with the learned initial guess versus
+the plain superposition guess.
 
 A step is one batched solve of the whole workload: every molecule of the batch is taken from its
-superposition guess to a converged RHF PM3 density (density convergence 1e-8, DIIS on) by the
-persistent batched SCF kernel, with the inputs already resident in HBM.  ``value`` is converged
-solves per second over the timed steps.
+initial guess to a converged RHF PM3 density (density convergence 1e-8, DIIS on) -- for the
+learned leg the pair features and the bf16 network run first, then the persistent batched SCF
+kernel -- with the inputs already resident in HBM.  ``value`` is converged, validated solves per
+second of the learned leg over the timed steps; ``plain`` holds the same for the superposition
+guess, timed the same way.  A learned solve counts only if it converged and its heat of
+formation is within 1e-5 kcal/mol of the plain GPU solve of the same molecule (and the CPU sample
+within 1e-5 of the CPU restatement).
 
 Workloads (``--workload``):
   mixed4096      4096 molecules: alkanes CnH2n+2 (n in [1,24]) and water clusters (2-16 waters),
@@ -69,6 +74,8 @@ def main():
     ap.add_argument("--gradient", action="store_true", help="also compute analytic gradients in every step")
     ap.add_argument("--copy-calibration-mb", type=float, default=0.0,
                     help="profiling helper: run only a plain device copy of this size (for counter scaling)")
+    ap.add_argument("--guess-model", default="", help="learned-guess network file (default: the packaged one)")
+    ap.add_argument("--plain-only", action="store_true", help="time only the superposition-guess leg")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -88,24 +95,43 @@ def main():
         return
 
     mols = build_workload(args.workload, args.seed)
-    opts = G.Pm3Options(p_tol=1.0e-8, e_tol=1.0e-8)
-    batch = G.Batch([G.Molecule(z, x) for z, x in mols], options=opts, want_gradient=args.gradient)
-    for _ in range(args.warmup):
-        batch.solve()
-    kernel_ms, pair_ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.solve()
-        kernel_ms.append(batch.last_kernel_ms())
-        pair_ms.append(batch.last_pair_ms())
-    t1 = time.perf_counter()
-    status, res = batch.results()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    molecules = [G.Molecule(z, x) for z, x in mols]
+    model = None
+    if not args.plain_only:
+        import pm3amd.guess as GS
+        path = args.guess_model or GS.DEFAULT_MODEL
+        if os.path.exists(path):
+            model = GS.GuessModel.load(path)
+
+    def leg(opts):
+        batch = G.Batch(molecules, options=opts, want_gradient=args.gradient)
+        for _ in range(args.warmup):
+            batch.solve()
+        kms, pms, gms = [], [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            batch.solve()
+            kms.append(batch.last_kernel_ms())
+            pms.append(batch.last_pair_ms())
+            gms.append(batch.last_guess_ms())
+        t1 = time.perf_counter()
+        st, rs = batch.results()
+        return batch, st, rs, (t1 - t0) * 1e3 / args.steps, float(np.mean(kms)), float(np.mean(pms)), float(np.mean(gms))
+
+    pbatch, pstatus, pres, p_ms, p_kms, p_pms, _ = leg(G.Pm3Options(p_tol=1.0e-8, e_tol=1.0e-8))
+    if model is not None:
+        batch, status, res, ms_per_step, k_ms, pair_ms, guess_ms = leg(
+            G.Pm3Options(p_tol=1.0e-8, e_tol=1.0e-8, guess="learned", guess_model=model))
+    else:
+        batch, status, res, ms_per_step, k_ms, pair_ms, guess_ms = pbatch, pstatus, pres, p_ms, p_kms, p_pms, 0.0
     converged = int(np.sum(status == 0))
+    # every learned solve against the plain GPU solve of the same molecule
+    ok_vs_plain = (status == 0) & (pstatus == 0) & np.array(
+        [abs(a.heat_of_formation_kcal - b.heat_of_formation_kcal) <= HOF_TOL for a, b in zip(res, pres)])
+    max_dhf_plain = float(max(abs(a.heat_of_formation_kcal - b.heat_of_formation_kcal) for a, b in zip(res, pres)))
 
     naos = [int(np.sum(np.where(z <= 2, 1, 4))) for z, _ in mols]
     flops = algorithmic_flops(naos, res)
-    k_ms = float(np.mean(kernel_ms))
     achieved = flops / (k_ms * 1e-3) / 1e12
 
     cpu = None
@@ -136,7 +162,9 @@ def main():
             grad_check = {"molecules": len(gsub), "max_abs_dgrad_ev_per_bohr": gmax,
                           "cpu_gradient_s_per_molecule_1thread": (g1 - g0) / len(gsub)}
         dh = np.array([res[i].heat_of_formation_kcal - hof_cpu[k] for k, i in enumerate(idx)
-                       if st_cpu[k] == 0 and status[i] == 0])
+                       if st_cpu[k] == 0 and status[i] == 0] +
+                      [pres[i].heat_of_formation_kcal - hof_cpu[k] for k, i in enumerate(idx)
+                       if st_cpu[k] == 0 and pstatus[i] == 0])
         checked = int(dh.size)
         max_dhf = float(np.abs(dh).max()) if dh.size else None
         hof_ok = bool(dh.size == 0 or np.abs(dh).max() <= HOF_TOL)
@@ -144,10 +172,13 @@ def main():
                "sample": f"every {stride}th molecule of the same workload ({len(sample)} molecules), "
                          f"C++ restatement of the reference CPU path, one solve per thread"}
 
-    valid = converged if hof_ok else 0
+    valid = int(np.sum(ok_vs_plain)) if hof_ok else 0
     value = valid / (ms_per_step * 1e-3)
+    plain_valid = int(np.sum(pstatus == 0)) if hof_ok else 0
+    plain_value = plain_valid / (p_ms * 1e-3)
+    used = np.array([r.guess_used for r in res])
     out = {
-        "metric": "converged PM3 SCF solves/sec (batched, plain superposition guess) at |dHf|<=1e-5 kcal/mol",
+        "metric": "converged PM3 SCF solves/sec, learned-guess vs plain, at |dHf|<=1e-5 kcal/mol",
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
@@ -160,15 +191,25 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded geometries)",
         "config": {"workload": args.workload, "molecules": len(mols), "seed": args.seed, "p_tol": 1e-8,
-                   "guess": "sad", "parallelism": "one workgroup per molecule, persistent launch"},
+                   "guess": "learned" if model is not None else "sad",
+                   "parallelism": "one workgroup per molecule, persistent launch"},
         "converged": converged,
+        "valid_vs_plain": int(np.sum(ok_vs_plain)),
+        "max_abs_dhf_vs_plain_kcal": max_dhf_plain,
         "hof_checked": checked,
         "max_abs_dhf_kcal": max_dhf,
         "mean_iterations": float(np.mean([r.iterations for r in res])),
+        "plain": {"value": plain_value, "ms_per_step": p_ms, "kernel_ms": p_kms, "pair_kernel_ms": p_pms,
+                  "mean_iterations": float(np.mean([r.iterations for r in pres])), "converged": plain_valid},
+        "speedup_vs_plain": value / plain_value if plain_value > 0 else None,
+        "learned": None if model is None else {
+            "guess_kernels_ms": guess_ms, "kept_fraction": float(np.mean(used == 1)),
+            "fallback_rate": float(np.mean(used == 2)), "not_applicable": float(np.mean(used == 0)),
+            "model_steps": model.steps},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "scf_batch_kernel", "kernel_ms": k_ms, "algorithmic_flop": flops,
-                     "pair_kernel_ms": float(np.mean(pair_ms))},
+                     "pair_kernel_ms": pair_ms},
         "cpu_baseline": cpu,
     }
     if args.gradient:

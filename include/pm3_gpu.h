@@ -54,7 +54,10 @@ typedef struct {
   int32_t multiplicity;
 } pm3_molecule;
 
-/* The fields of Pm3Options (src/scf.rs:160-287) that the molecular RHF path reads. */
+struct pm3_guess_model; /* trained network of the learned guess, see pm3_guess.h */
+
+/* The fields of Pm3Options (src/scf.rs:160-287) that the molecular RHF path reads, plus the
+ * network that PM3_GUESS_LEARNED uses. */
 typedef struct {
   int32_t max_scf;        /* default 200 */
   double e_tol;           /* default 1e-8 eV */
@@ -65,6 +68,7 @@ typedef struct {
   int32_t guess;          /* pm3_guess, default AUTO (= SAD) */
   int32_t stability;      /* pm3_stability, default AUTO */
   int32_t scf_memory_mb;  /* default 512: sets the DIIS depth (src/scf.rs:309-327) */
+  const struct pm3_guess_model* guess_model; /* required by PM3_GUESS_LEARNED, default NULL */
 } pm3_options;
 
 /* Scalars of Pm3Result (src/scf.rs:352-396). */
@@ -75,6 +79,8 @@ typedef struct {
   double last_change;          /* RMS density change of the last iteration */
   double scf_improvement_ev;
   int32_t iterations, converged, scf_paths_tried, n_occ, nao, has_homo, has_lumo;
+  int32_t guess_used;          /* 0 superposition / other rule, 1 learned, 2 learned rejected
+                                  (first-iteration commutator norm above the superposition's) */
 } pm3_result;
 
 typedef struct pm3_params pm3_params; /* opaque, immutable once built */
@@ -130,6 +136,8 @@ pm3_status pm3_batch_gradient(pm3_batch* batch, double* gradient);
 double pm3_batch_last_kernel_ms(const pm3_batch* batch);
 /* Device time (ms) of the last pm3_batch_solve's pair-integral kernel. */
 double pm3_batch_last_pair_ms(const pm3_batch* batch);
+/* Device time (ms) of the last pm3_batch_solve's learned-guess kernels (features + network). */
+double pm3_batch_last_guess_ms(const pm3_batch* batch);
 /* Per-phase wall-clock ticks (10 ns) of the last solve summed over workgroups, collected only when
  * the environment has PM3_PHASE_TIMERS=1 at pm3_batch_create (zeros otherwise): H_core finish,
  * Fock, energy+DIIS, warm-start products, Jacobi, sort + C V, density, bookkeeping; then out[8]

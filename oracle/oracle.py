@@ -82,12 +82,23 @@ def lib():
         L.orc_eigen.argtypes = [ctypes.c_int, dp, dp, dp]
         L.orc_run.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.POINTER(Options), ctypes.POINTER(Result),
                               dp, dp, dp, dp]
+        L.orc_run_from_density.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.POINTER(Options), dp,
+                                           ctypes.POINTER(Result), dp]
         L.orc_fixed_density_gradient.argtypes = [ctypes.c_int, ip, dp, dp, dp]
         L.orc_gradient.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.POINTER(Options),
                                    ctypes.POINTER(Result), dp, dp]
         L.orc_run_batch.argtypes = [ctypes.c_int, ip, ip, dp, dp, ctypes.POINTER(Options), ctypes.c_int, dp, dp, ip,
                                     ip]
         L.orc_num_threads.restype = ctypes.c_int
+        fp = ctypes.POINTER(ctypes.c_float)
+        up = ctypes.POINTER(ctypes.c_uint16)
+        L.orc_guess_pairs.argtypes = [ctypes.c_int, ip, dp, ip, ip]
+        L.orc_guess_features.argtypes = [ctypes.c_int, ip, dp, dp, up]
+        L.orc_guess_forward.argtypes = [fp, ctypes.c_int, up, dp, ctypes.c_int, dp]
+        L.orc_guess_backward.argtypes = [fp, ctypes.c_int, up, dp, dp, ctypes.c_int, dp]
+        L.orc_guess_density.argtypes = [ctypes.c_int, ip, dp, fp, dp]
+        L.orc_guess_loss.argtypes = [ctypes.c_int, ip, dp, fp, dp, dp]
+        L.orc_guess_loss.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -189,6 +200,22 @@ def run(z, xyz_bohr, charge=0.0, opts=None, want_arrays=True):
     return st, out
 
 
+def run_from_density(z, xyz_bohr, P0, charge=0.0, opts=None):
+    """RHF PM3 single point started from the density P0; returns (status, dict with density)."""
+    z, xyz = _mol(z, xyz_bohr)
+    opts = opts or options()
+    n = nao(z)
+    P0 = np.ascontiguousarray(P0, dtype=np.float64).reshape(n, n)
+    res = Result()
+    P = np.zeros((n, n))
+    st = lib().orc_run_from_density(len(z), _i(z), _d(xyz), float(charge), ctypes.byref(opts), _d(P0),
+                                    ctypes.byref(res), _d(P))
+    out = {f: getattr(res, f) for f, _ in Result._fields_}
+    out["dipole"] = np.array(res.dipole[:])
+    out["density"] = P
+    return st, out
+
+
 def fixed_density_gradient(z, xyz_bohr, P):
     z, xyz = _mol(z, xyz_bohr)
     P = np.ascontiguousarray(P, dtype=np.float64)
@@ -233,3 +260,84 @@ def run_batch(mols, opts=None, nthreads=0):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+# ---------------------------------------------------------------------------------------------
+# Learned initial density (guess_oracle.cpp).
+
+GUESS_NF, GUESS_NH, GUESS_NO = 96, 128, 16
+GUESS_NPARAM = GUESS_NF * GUESS_NH + GUESS_NH * GUESS_NH + GUESS_NH + GUESS_NH * GUESS_NO + GUESS_NO
+
+
+def _f(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _u16(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+
+
+def guess_pairs(z, xyz_bohr):
+    """Directed pairs (src, dst) inside 4 Angstrom, ordered by src then dst."""
+    z, xyz = _mol(z, xyz_bohr)
+    n = lib().orc_guess_pairs(len(z), _i(z), _d(xyz), None, None)
+    src = np.zeros(n, dtype=np.int32)
+    dst = np.zeros(n, dtype=np.int32)
+    lib().orc_guess_pairs(len(z), _i(z), _d(xyz), _i(src), _i(dst))
+    return src, dst
+
+
+def guess_features(z, xyz_bohr):
+    """(f double before rounding, x bf16 bits) per directed pair, GUESS_NF each."""
+    z, xyz = _mol(z, xyz_bohr)
+    src, _ = guess_pairs(z, xyz)
+    f = np.zeros((len(src), GUESS_NF))
+    x = np.zeros((len(src), GUESS_NF), dtype=np.uint16)
+    st = lib().orc_guess_features(len(z), _i(z), _d(xyz), _d(f), _u16(x))
+    if st:
+        raise RuntimeError(STATUS[st])
+    return f, x
+
+
+def guess_forward(params, x=None, xd=None, rounded=True):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    n = len(x) if x is not None else len(xd)
+    y = np.zeros((n, GUESS_NO))
+    xa = np.ascontiguousarray(x, dtype=np.uint16) if x is not None else None
+    xda = np.ascontiguousarray(xd, dtype=np.float64) if xd is not None else None
+    lib().orc_guess_forward(_f(params), n, _u16(xa) if xa is not None else None,
+                            _d(xda) if xda is not None else None, int(rounded), _d(y))
+    return y
+
+
+def guess_backward(params, dy, x=None, xd=None, rounded=True):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    n = len(dy)
+    g = np.zeros(GUESS_NPARAM)
+    xa = np.ascontiguousarray(x, dtype=np.uint16) if x is not None else None
+    xda = np.ascontiguousarray(xd, dtype=np.float64) if xd is not None else None
+    lib().orc_guess_backward(_f(params), n, _u16(xa) if xa is not None else None,
+                             _d(xda) if xda is not None else None, _d(dy), int(rounded), _d(g))
+    return g
+
+
+def guess_density(z, xyz_bohr, y):
+    z, xyz = _mol(z, xyz_bohr)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    n = nao(z)
+    P = np.zeros((n, n))
+    st = lib().orc_guess_density(len(z), _i(z), _d(xyz), _f(y), _d(P))
+    if st:
+        raise RuntimeError(STATUS[st])
+    return P
+
+
+def guess_loss(z, xyz_bohr, y, T):
+    """(loss, dloss/dy) of the assembled guess against the target density T."""
+    z, xyz = _mol(z, xyz_bohr)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    dy = np.zeros((len(y), GUESS_NO))
+    loss = lib().orc_guess_loss(len(z), _i(z), _d(xyz), _f(y), _d(T), _d(dy))
+    return loss, dy

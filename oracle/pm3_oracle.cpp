@@ -1202,10 +1202,14 @@ struct State {
   bool converged = false;
 };
 
-static void rhf_loop(const Mol& m, const Core& core, int nocc, const Options& o, int guess, int accel_in, State& st) {
+// P0, when given, is the starting density (n x n, row-major) in place of the guess rule.
+static void rhf_loop(const Mol& m, const Core& core, int nocc, const Options& o, int guess, int accel_in, State& st,
+                     const double* P0 = nullptr) {
   int n = m.nao;
   std::vector<double> P;
-  if (guess == 2) {
+  if (P0) {
+    P.assign(P0, P0 + (size_t)n * n);
+  } else if (guess == 2) {
     std::vector<double> w(n), C((size_t)n * n);
     sym_eigen(n, core.h.data(), w.data(), C.data());
     P.assign((size_t)n * n, 0.0);
@@ -1429,8 +1433,8 @@ int orc_eigen(int n, const double* A, double* w, double* V) {
 // restricted rungs of src/scf.rs:641-673; the two unrestricted rungs (:674-689) are counted in
 // scf_paths_tried but not run (this restatement has no UHF loop).  Open shells return
 // E_UNSUPPORTED.  Optional outputs may be null.
-int orc_run(int nat, const int* z, const double* xyz, double charge, const Options* opt, Result* res, double* density,
-            double* mo_energies, double* mo_coeff, double* charges) {
+static int run_impl(int nat, const int* z, const double* xyz, double charge, const Options* opt, const double* P0,
+                    Result* res, double* density, double* mo_energies, double* mo_coeff, double* charges) {
   Mol m;
   int st = make_mol(nat, z, xyz, m);
   if (st) return st;
@@ -1449,7 +1453,7 @@ int orc_run(int nat, const int* z, const double* xyz, double charge, const Optio
   Options o = *opt;
   State best;
   int accel0 = (o.use_diis && !has_he) ? o.accelerator : 0;
-  rhf_loop(m, core, nocc, o, o.guess, accel0, best);
+  rhf_loop(m, core, nocc, o, o.guess, accel0, best, P0);
   int paths = 1;
   auto gap = [&](const State& s) { return (nocc > 0 && nocc < n) ? s.eps[nocc] - s.eps[nocc - 1] : INFINITY; };
   bool search = o.stability == 2 || (o.stability == 1 && gap(best) < 6.0);
@@ -1523,6 +1527,22 @@ int orc_run(int nat, const int* z, const double* xyz, double charge, const Optio
       charges[i] = m.el[i]->tore - pop;
     }
   return best.converged ? OK : E_NOT_CONVERGED;
+}
+
+int orc_run(int nat, const int* z, const double* xyz, double charge, const Options* opt, Result* res, double* density,
+            double* mo_energies, double* mo_coeff, double* charges) {
+  return run_impl(nat, z, xyz, charge, opt, nullptr, res, density, mo_energies, mo_coeff, charges);
+}
+
+// The same single point started from a caller-provided density P0 (n x n).  The stability rungs,
+// when the gap rule fires, are those of the superposition guess.  This is the restatement of a
+// learned starting density handed to the unchanged SCF.
+int orc_run_from_density(int nat, const int* z, const double* xyz, double charge, const Options* opt, const double* P0,
+                         Result* res, double* density) {
+  if (!P0) return E_INVALID;
+  Options o = *opt;
+  if (o.guess == 2) o.guess = 1;
+  return run_impl(nat, z, xyz, charge, &o, P0, res, density, nullptr, nullptr, nullptr);
 }
 
 // Hellmann-Feynman gradient at a fixed density P (eV/Bohr): src/gradient.rs:306-318 and

@@ -8,12 +8,22 @@
 #include <vector>
 
 #include "../../include/pm3_gpu.h"
+#include "../../include/pm3_guess.h"
+#include "pm3_guess.hpp"
 #include "pm3_host.hpp"
 
 using namespace pm3host;
 
 struct pm3_params {
   Params p;
+};
+
+struct pm3_guess_model {
+  GuessModel* m = nullptr;
+};
+
+struct pm3_guess_data {
+  GuessData* d = nullptr;
 };
 
 struct pm3_batch {
@@ -61,7 +71,12 @@ bool engine_options(const pm3_options* o, EngineOptions& e, std::string& why) {
   e.p_tol = o->p_tol;
   e.accel = o->use_diis ? o->accelerator : 0;
   e.adiis_switch = o->adiis_switch;
-  e.guess = (o->guess == PM3_GUESS_AUTO || o->guess == PM3_GUESS_LEARNED) ? 1 : o->guess;
+  e.guess = o->guess == PM3_GUESS_AUTO ? 1 : o->guess;
+  e.guess_model = nullptr;
+  if (o->guess == PM3_GUESS_LEARNED) {
+    if (!o->guess_model || !o->guess_model->m) { why = "PM3_GUESS_LEARNED needs options.guess_model"; return false; }
+    e.guess_model = o->guess_model->m;
+  }
   e.stability = o->stability;
   e.scf_memory_mb = o->scf_memory_mb;
   return true;
@@ -86,6 +101,7 @@ void fill_result(const MolResult& r, pm3_result* out) {
   out->scf_paths_tried = r.paths;
   out->n_occ = r.nocc;
   out->nao = r.nao;
+  out->guess_used = r.guess_used;
 }
 
 pm3_status check_molecule(const pm3_molecule* mol) {
@@ -131,6 +147,7 @@ void pm3_options_default(pm3_options* out) {
   out->guess = PM3_GUESS_AUTO;
   out->stability = PM3_STABILITY_AUTO;
   out->scf_memory_mb = 512;
+  out->guess_model = nullptr;
 }
 
 int32_t pm3_query_nao(const pm3_molecule* mol, const pm3_params* params) {
@@ -242,6 +259,8 @@ pm3_status pm3_batch_results(pm3_batch* h, pm3_status* status, pm3_result* resul
 double pm3_batch_last_kernel_ms(const pm3_batch* h) { return h ? batch_kernel_ms(h->b) : 0.0; }
 
 double pm3_batch_last_pair_ms(const pm3_batch* h) { return h ? batch_pair_ms(h->b) : 0.0; }
+
+double pm3_batch_last_guess_ms(const pm3_batch* h) { return h ? batch_guess_ms(h->b) : 0.0; }
 
 pm3_status pm3_batch_phase_ticks(const pm3_batch* h, uint64_t* out) {
   if (!h || !out) return fail(PM3_INVALID_INPUT, "null argument");
@@ -408,6 +427,210 @@ pm3_status pm3_stage_gemm(int32_t n, const double* a, const double* b, double* c
   pm3_status s = generic_stage(n, 3, in, out);
   if (s != PM3_OK) return s;
   std::memcpy(c, out.data(), (size_t)n * n * sizeof(double));
+  return PM3_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Learned initial density (include/pm3_guess.h).
+extern "C" {
+
+pm3_status pm3_guess_model_create(uint64_t seed, pm3_guess_model** out) {
+  g_last_error.clear();
+  if (!out) return fail(PM3_INVALID_INPUT, "null handle");
+  *out = nullptr;
+  std::string err;
+  GuessModel* m = nullptr;
+  if (!guess_model_create(seed, m, err)) return fail(PM3_DEVICE_ERROR, err);
+  *out = new pm3_guess_model{m};
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_model_load(const char* path, pm3_guess_model** out) {
+  g_last_error.clear();
+  if (!out || !path) return fail(PM3_INVALID_INPUT, "null argument");
+  *out = nullptr;
+  std::string err;
+  GuessModel* m = nullptr;
+  if (!guess_model_load(path, m, err)) return fail(m ? PM3_DEVICE_ERROR : PM3_INVALID_INPUT, err);
+  *out = new pm3_guess_model{m};
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_model_save(const pm3_guess_model* model, const char* path) {
+  g_last_error.clear();
+  if (!model || !model->m || !path) return fail(PM3_INVALID_INPUT, "null argument");
+  std::string err;
+  if (!guess_model_save(model->m, path, err)) return fail(PM3_INVALID_INPUT, err);
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_model_get_params(const pm3_guess_model* model, float* out) {
+  g_last_error.clear();
+  if (!model || !model->m || !out) return fail(PM3_INVALID_INPUT, "null argument");
+  std::string err;
+  if (!guess_model_params(model->m, out, err)) return fail(PM3_DEVICE_ERROR, err);
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_model_set_params(pm3_guess_model* model, const float* params) {
+  g_last_error.clear();
+  if (!model || !model->m || !params) return fail(PM3_INVALID_INPUT, "null argument");
+  for (int i = 0; i < PM3_GUESS_NPARAM; ++i)
+    if (!std::isfinite(params[i])) return fail(PM3_INVALID_INPUT, "non-finite parameter");
+  std::string err;
+  if (!guess_model_set_params(model->m, params, err)) return fail(PM3_DEVICE_ERROR, err);
+  return PM3_OK;
+}
+
+int32_t pm3_guess_model_steps(const pm3_guess_model* model) { return model ? guess_model_steps(model->m) : 0; }
+
+void pm3_guess_model_destroy(pm3_guess_model* model) {
+  if (!model) return;
+  guess_model_destroy(model->m);
+  delete model;
+}
+
+pm3_status pm3_guess_data_create(pm3_guess_data** out) {
+  g_last_error.clear();
+  if (!out) return fail(PM3_INVALID_INPUT, "null handle");
+  std::string err;
+  GuessData* d = nullptr;
+  if (!guess_data_create(d, err)) return fail(PM3_DEVICE_ERROR, err);
+  *out = new pm3_guess_data{d};
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_data_add(pm3_guess_data* data, pm3_batch* solved, int64_t* n_added) {
+  g_last_error.clear();
+  if (!data || !solved) return fail(PM3_INVALID_INPUT, "null argument");
+  long long added = 0;
+  std::string err;
+  if (!guess_data_add(data->d, solved->b, added, err)) return fail(PM3_INVALID_INPUT, err);
+  if (n_added) *n_added = added;
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_data_stats(const pm3_guess_data* data, int64_t* n_mol, int64_t* n_atoms, int64_t* n_pairs) {
+  if (!data) return fail(PM3_INVALID_INPUT, "null argument");
+  long long a = 0, b = 0, c = 0;
+  guess_data_stats(data->d, a, b, c);
+  if (n_mol) *n_mol = a;
+  if (n_atoms) *n_atoms = b;
+  if (n_pairs) *n_pairs = c;
+  return PM3_OK;
+}
+
+void pm3_guess_data_destroy(pm3_guess_data* data) {
+  if (!data) return;
+  guess_data_destroy(data->d);
+  delete data;
+}
+
+void pm3_guess_train_options_default(pm3_guess_train_options* out) {
+  out->steps = 1000;
+  out->batch_molecules = 1024;
+  out->lr = 1.0e-3f;
+  out->beta1 = 0.9f;
+  out->beta2 = 0.999f;
+  out->eps = 1.0e-8f;
+  out->seed = 1;
+}
+
+pm3_status pm3_guess_train(pm3_guess_model* model, pm3_guess_data* data, const pm3_guess_train_options* opts,
+                           float* losses, double* seconds) {
+  g_last_error.clear();
+  if (!model || !model->m || !data || !opts) return fail(PM3_INVALID_INPUT, "null argument");
+  if (opts->steps < 1 || opts->batch_molecules < 1 || !(opts->lr > 0.0f)) return fail(PM3_INVALID_INPUT, "bad training options");
+  GuessTrainOptions o;
+  o.steps = opts->steps;
+  o.batch_molecules = opts->batch_molecules;
+  o.lr = opts->lr;
+  o.beta1 = opts->beta1;
+  o.beta2 = opts->beta2;
+  o.eps = opts->eps;
+  o.seed = opts->seed;
+  std::string err;
+  if (!guess_train(model->m, data->d, o, losses, seconds, err)) return fail(PM3_DEVICE_ERROR, err);
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_eval(const pm3_guess_model* model, pm3_guess_data* data, double* loss) {
+  g_last_error.clear();
+  if (!model || !model->m || !data || !loss) return fail(PM3_INVALID_INPUT, "null argument");
+  std::string err;
+  if (!guess_eval(model->m, data->d, *loss, err)) return fail(PM3_DEVICE_ERROR, err);
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_grad(const pm3_guess_model* model, pm3_guess_data* data, float* grad, double* loss) {
+  g_last_error.clear();
+  if (!model || !model->m || !data || !grad || !loss) return fail(PM3_INVALID_INPUT, "null argument");
+  std::string err;
+  if (!guess_grad(model->m, data->d, grad, *loss, err)) return fail(PM3_DEVICE_ERROR, err);
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_mlp(const pm3_guess_model* model, int32_t n, const uint16_t* x, float* y, const float* dy,
+                         float* grad) {
+  g_last_error.clear();
+  if (!model || !model->m || !x || !y || n < 1 || (dy && !grad)) return fail(PM3_INVALID_INPUT, "bad arguments");
+  std::string err;
+  if (!guess_mlp(model->m, n, x, y, dy, grad, err)) return fail(PM3_DEVICE_ERROR, err);
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_features(const pm3_molecule* mol, const pm3_params* params, int32_t* n_pairs, int32_t* src,
+                              int32_t* dst, uint16_t* x) {
+  g_last_error.clear();
+  if (!params || !n_pairs) return fail(PM3_INVALID_INPUT, "null argument");
+  pm3_status s = check_molecule(mol);
+  if (s != PM3_OK) return s;
+  for (int i = 0; i < mol->n_atoms; ++i)
+    if (pm3guess::elem_index(mol->z[i]) < 0) return fail(PM3_UNSUPPORTED, "the learned guess covers H, C and O only");
+  std::vector<MolIn> mols{MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge}};
+  EngineOptions eo;
+  std::string why, err;
+  if (!engine_options(nullptr, eo, why)) return fail(PM3_INVALID_INPUT, why);
+  Batch* b = nullptr;
+  std::vector<int> gs, gd, cm;
+  std::vector<uint16_t> gx;
+  const bool ok = batch_create(params->p, mols, eo, RunRequest(), b, err) && guess_batch_features(b, gs, gd, gx, cm, err);
+  batch_destroy(b);
+  if (!ok) return fail(PM3_DEVICE_ERROR, err);
+  const int32_t cap = *n_pairs;
+  if (src || dst || x) {
+    if (!src || !dst || !x || cap < (int32_t)gs.size()) {
+      *n_pairs = (int32_t)gs.size();
+      return fail(PM3_INVALID_INPUT, "pair buffers missing or too small");
+    }
+    for (size_t j = 0; j < gs.size(); ++j) {
+      src[j] = cm[gs[j]];
+      dst[j] = cm[gd[j]];
+    }
+    if (!gx.empty()) std::memcpy(x, gx.data(), gx.size() * sizeof(uint16_t));
+  }
+  *n_pairs = (int32_t)gs.size();
+  return PM3_OK;
+}
+
+pm3_status pm3_guess_density(const pm3_guess_model* model, const pm3_molecule* mol, const pm3_params* params,
+                             double* density, double* c2l, double* c2s) {
+  g_last_error.clear();
+  if (!model || !model->m || !params || !density) return fail(PM3_INVALID_INPUT, "null argument");
+  pm3_status s = check_molecule(mol);
+  if (s != PM3_OK) return s;
+  MolIn m{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge};
+  std::vector<double> out;
+  int st = 0;
+  std::string err;
+  if (!stage_run(params->p, m, 5, {}, out, st, err, model->m)) return fail(PM3_DEVICE_ERROR, err);
+  if (st != 0) return fail((pm3_status)st, st == 4 ? "the learned guess covers H, C and O only" : status_text(st));
+  const size_t n2 = out.size() - 2;
+  std::memcpy(density, out.data(), n2 * sizeof(double));
+  if (c2l) *c2l = out[n2];
+  if (c2s) *c2s = out[n2 + 1];
   return PM3_OK;
 }
 

@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <numeric>
 
+#include "pm3_guess.hpp"
 #include "pm3_host.hpp"
 
 using namespace pm3dev;
@@ -50,15 +51,17 @@ __constant__ TableConst c_tab;
 
 struct MolDesc {
   int atom0, nat, nheavy, nao, nocc, np;
-  int pad0, pad1;
+  int ngp, learn;              // directed pairs of the learned guess; learn = all atoms H, C or O
   long long out_mat, out_vec;  // offsets into the density/coefficient and vector outputs, -1 if none
   long long h_off, w_off;      // per-molecule H_core (np x np) and pair tables in the molecule store
+  long long gp0;               // first directed pair
 };
 
 struct MolOut {
   double e_elec, e_core, last_dp, first_e, homo, lumo;
   double dip[3];
   int iters, converged, paths, status;
+  int guess_used;  // 0 superposition (learned not requested or not applicable), 1 learned, 2 learned rejected
 };
 
 struct KArgs {
@@ -86,6 +89,14 @@ struct KArgs {
   long long npairs_total;
   unsigned long long* prof;    // optional per-phase wall-clock sums (10 ns ticks), may be null
   int lds_doubles;             // dynamic LDS per workgroup, in doubles
+  // learned guess: directed pairs (global indices, see batch_create) and network outputs
+  const int8_t* elem8;
+  const int* gbeg;
+  const int* gcnt;
+  const int* gsrc;
+  const int* gdst;
+  const int* gpart;
+  const float* gY;
   // options
   int max_scf;
   double e_tol, p_tol;
@@ -196,6 +207,8 @@ __device__ inline double oc_int(int a, int b, int c, int d, const ElemConst& e) 
 
 struct Mol {
   int atom0, nat, nh, nH, n, np, nocc;
+  int ngp, learn;
+  long long gp0;
   int nxx, nxh, nhh;
   double* Wxx;
   double* Wxh;
@@ -851,25 +864,117 @@ __device__ void density(const Mol& M, const double* Cn, double* Pdst) {
   __syncthreads();
 }
 
-// Initial density for a guess mode (src/scf.rs:855-935).  Core guess uses the eigenvectors of H.
-__device__ __noinline__ void initial_guess(const KArgs& A, const Mol& M, const Bufs& B, int guess, double* lds, double* red) {
+// Superposition density (src/scf.rs:855-881) into Pdst (assumed zero).
+__device__ inline void sad_into(const KArgs& A, const Mol& M, double* Pdst) {
+  const int np = M.np;
+  for (int a = threadIdx.x; a < M.nat; a += NT) {
+    const int ga = M.atom0 + a;
+    const ElemConst& e = c_tab.el[A.aslot[ga]];
+    const int o = A.aoff[ga];
+    Pdst[o * np + o] = e.ios;
+    if (e.norb == 4)
+      for (int k = 1; k < 4; ++k) Pdst[(o + k) * np + o + k] = e.iop / 3.0;
+  }
+  __syncthreads();
+}
+
+// Learned density (definitions in guess_device.hpp) into Pdst (assumed zero): on-site blocks are
+// the superposition block plus the messages of the atom's directed pairs in their stored order;
+// each pair block is (B(a->b) + B(b->a)^T) / 2, written once from the pair with a < b.
+__device__ void learned_into(const KArgs& A, const Mol& M, double* Pdst) {
+  const int np = M.np;
+  for (int a = threadIdx.x; a < M.nat; a += NT) {
+    const int ga = M.atom0 + a, e = A.elem8[ga], o = A.aoff[ga], na = e == 0 ? 1 : 4;
+    double Pa[4][4];
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) Pa[r][c] = 0.0;
+    Pa[0][0] = pm3guess::sad_s(e);
+    for (int k = 1; k < na; ++k) Pa[k][k] = pm3guess::sad_p(e);
+    const int j0 = A.gbeg[ga], nb = A.gcnt[ga];
+    for (int t = 0; t < nb; ++t) {
+      const int j = j0 + t, c = A.gdst[j];
+      const double dx = A.ax[c] - A.ax[ga], dy = A.ay[c] - A.ay[ga], dz = A.az[c] - A.az[ga];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      double Mm[4][4];
+      pm3guess::onsite_message(A.gY + (size_t)j * pm3guess::NO, dx / r, dy / r, dz / r, Mm);
+      for (int p = 0; p < na; ++p)
+        for (int q = 0; q < na; ++q) Pa[p][q] += Mm[p][q];
+    }
+    for (int p = 0; p < na; ++p)
+      for (int q = 0; q < na; ++q) Pdst[(o + p) * np + o + q] = Pa[p][q];
+  }
+  for (int t = threadIdx.x; t < M.ngp; t += NT) {
+    const long long j = M.gp0 + t;
+    const int a = A.gsrc[j], b = A.gdst[j];
+    if (a > b) continue;
+    const int jp = A.gpart[j];
+    const double dx = A.ax[b] - A.ax[a], dy = A.ay[b] - A.ay[a], dz = A.az[b] - A.az[a];
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double ux = dx / r, uy = dy / r, uz = dz / r;
+    double Bab[4][4], Bba[4][4];
+    pm3guess::pair_block(A.gY + (size_t)j * pm3guess::NO, ux, uy, uz, Bab);
+    pm3guess::pair_block(A.gY + (size_t)jp * pm3guess::NO, -ux, -uy, -uz, Bba);
+    const int na = A.elem8[a] == 0 ? 1 : 4, nb = A.elem8[b] == 0 ? 1 : 4, oa = A.aoff[a], ob = A.aoff[b];
+    for (int p = 0; p < na; ++p)
+      for (int q = 0; q < nb; ++q) {
+        const double v = 0.5 * (Bab[p][q] + Bba[q][p]);
+        Pdst[(oa + p) * np + ob + q] = v;
+        Pdst[(ob + q) * np + oa + p] = v;
+      }
+  }
+  __syncthreads();
+}
+
+// |F(P) P - P F(P)|_F^2 for the density in B.P (F into B.F, F P into B.T): the first-iteration
+// DIIS error norm of an SCF started from B.P.
+__device__ double commutator2(const KArgs& A, const Mol& M, const Bufs& B, double* lds, double* red) {
+  const int n = M.n, np = M.np;
+  build_fock(A, M, M.H, B.P, B.F, lds);
+  wg_gemm(n, n, n, [&](int i, int k) { return B.F[i * np + k]; }, [&](int k, int j) { return B.P[k * np + j]; },
+          [&](int i, int j, double v) { B.T[i * np + j] = v; });
+  __syncthreads();
+  double v[1] = {0.0};
+  for (int e = threadIdx.x; e < n * n; e += NT) {
+    const int i = e / n, j = e % n;
+    const double d = B.T[i * np + j] - B.T[j * np + i];
+    v[0] += d * d;
+  }
+  block_sum<1>(v, red);
+  return v[0];
+}
+
+// Initial density for a guess mode (src/scf.rs:855-935): 1 SAD, 2 core, 3 symmetry-broken,
+// 4 learned with the superposition fallback (the learned density is kept only when its
+// first-iteration commutator norm is not larger than the superposition's).  Returns the guess
+// used: 0 superposition (or not learned), 1 learned, 2 learned rejected.
+__device__ __noinline__ int initial_guess(const KArgs& A, const Mol& M, const Bufs& B, int guess, double* lds,
+                                          double* red) {
   const int np = M.np, tid = threadIdx.x;
   for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
   __syncthreads();
   if (guess == 2) {
     eigensolve(M, B, M.H, false, lds, red);
     density(M, B.Cn, B.P);
-    return;
+    return 0;
   }
-  for (int a = tid; a < M.nat; a += NT) {
-    const int ga = M.atom0 + a;
-    const ElemConst& e = c_tab.el[A.aslot[ga]];
-    const int o = A.aoff[ga];
-    B.P[o * np + o] = e.ios;
-    if (e.norb == 4)
-      for (int k = 1; k < 4; ++k) B.P[(o + k) * np + o + k] = e.iop / 3.0;
+  if (guess == 4 && M.learn && A.gY) {
+    learned_into(A, M, B.P);
+    const double cl = commutator2(A, M, B, lds, red);
+    for (int e = tid; e < np * np; e += NT) {
+      B.Pb[e] = B.P[e];
+      B.P[e] = 0.0;
+    }
+    __syncthreads();
+    sad_into(A, M, B.P);
+    const double cs = commutator2(A, M, B, lds, red);
+    if (cl <= cs) {
+      for (int e = tid; e < np * np; e += NT) B.P[e] = B.Pb[e];
+      __syncthreads();
+      return 1;
+    }
+    return 2;
   }
-  __syncthreads();
+  sad_into(A, M, B.P);
   if (guess == 3 && tid == 0) {
     // tilt a fifth of an electron from p_z to p_x on the atom with the largest p population
     int target = -1;
@@ -886,6 +991,7 @@ __device__ __noinline__ void initial_guess(const KArgs& A, const Mol& M, const B
     }
   }
   __syncthreads();
+  return 0;
 }
 
 __device__ inline double dot_nn(const double* X, const double* Y, int np, double* red) {
@@ -897,7 +1003,7 @@ __device__ inline double dot_nn(const double* X, const double* Y, int np, double
 
 struct PathResult {
   double e_elec, last_dp;
-  int iters, converged;
+  int iters, converged, guess_used;
 };
 
 // One restricted SCF path (rhf_loop, src/scf.rs:1179-1316).  Leaves P, eps, Cn (orbitals of the
@@ -905,7 +1011,7 @@ struct PathResult {
 __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int guess, int accel, double* lds,
                                double* red, Diis& D) {
   const int n = M.n, np = M.np, tid = threadIdx.x;
-  initial_guess(A, M, B, guess, lds, red);
+  const int gused = initial_guess(A, M, B, guess, lds, red);
   // DIIS depth from the memory budget (src/scf.rs:309-327, :1205-1228)
   const long long n2 = (long long)n * n * 8;
   const long long budget = (long long)A.scf_memory_mb * 1024 * 1024;
@@ -921,7 +1027,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
     D.keep_d = keep_d;
   }
   __syncthreads();
-  PathResult R{0.0, 1.0e300, 0, 0};
+  PathResult R{0.0, 1.0e300, 0, 0, gused};
   double e_old = 0.0;
   bool have_c = false;
   for (int it = 0; it < A.max_scf; ++it) {
@@ -1095,6 +1201,9 @@ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double* Wbase) {
   M.n = md.nao;
   M.np = md.np;
   M.nocc = md.nocc;
+  M.ngp = md.ngp;
+  M.learn = md.learn;
+  M.gp0 = md.gp0;
   M.nxx = M.nh * (M.nh - 1) / 2;
   M.nxh = M.nh * M.nH;
   M.nhh = M.nH * (M.nH - 1) / 2;
@@ -1303,6 +1412,7 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
     const double ecore = finish_core(A, M, red);
     stamp(B, 0);
     PathResult best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D);
+    const int guess_used = best.guess_used;
     const int nocc = M.nocc, n = M.n, np = M.np;
     const double first_e = best.e_elec;
     int paths = 1;
@@ -1381,6 +1491,7 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
       o.converged = best.converged;
       o.paths = paths;
       o.status = best.converged ? 0 : 3;
+      o.guess_used = guess_used;
       A.out[mi] = o;
     }
     stamp(B, 7);
@@ -1400,7 +1511,9 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
 // mode 0 H_core (+ core-core energy at out[n*n]); mode 1 F(P) for P given in `in`;
 // mode 2 eigenpairs of `in` (cold Jacobi): out = eps[n] then C[n*n];
 // mode 3 out = in[0:n*n] . in[n*n:2n*n] on the matrix cores;
-// mode 4 warm eigensolve of in[0:n*n] in the basis in[n*n:2n*n]: out = eps[n] then C[n*n].
+// mode 4 warm eigensolve of in[0:n*n] in the basis in[n*n:2n*n]: out = eps[n] then C[n*n];
+// mode 5 learned initial density (network outputs already computed): out = P[n*n], then the
+// squared first-iteration commutator norms of the learned and the superposition densities.
 // Matrices in `in`/`out` are dense n x n row-major in the engine's (heavy-first) AO order.
 __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const double* in, double* out) {
   double* lds = g_lds;
@@ -1437,6 +1550,19 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
     const double* Y = in + n * n;
     wg_gemm(n, n, n, [&](int i, int k) { return X[i * n + k]; }, [&](int k, int j) { return Y[k * n + j]; },
             [&](int i, int j, double v) { out[i * n + j] = v; });
+  } else if (mode == 5 && M.learn && A.gY) {
+    (void)finish_core(A, M, red);
+    learned_into(A, M, B.P);
+    const double cl = commutator2(A, M, B, lds, red);
+    for (int e = tid; e < n * n; e += NT) out[e] = B.P[(e / n) * np + e % n];
+    for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
+    __syncthreads();
+    sad_into(A, M, B.P);
+    const double cs = commutator2(A, M, B, lds, red);
+    if (tid == 0) {
+      out[n * n] = cl;
+      out[n * n + 1] = cs;
+    }
   }
 }
 
@@ -1463,19 +1589,41 @@ struct Batch {
   bool want_gradient = false;
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
-  void* bufs[24] = {nullptr};
+  void* bufs[40] = {nullptr};
   int nbufs = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-  double last_kernel_ms = 0.0, last_pair_ms = 0.0;
+  double last_kernel_ms = 0.0, last_pair_ms = 0.0, last_guess_ms = 0.0;
   int pair_blocks = 1;
+  // learned guess: directed pairs of the eligible molecules and their network inputs/outputs
+  const GuessModel* guess_model = nullptr;
+  bool learned = false, solved_once = false;
+  long long ngp_total = 0;
+  std::vector<int> eligible;
+  uint16_t* d_X = nullptr;
+  float* d_Y = nullptr;
+  hipEvent_t evg = nullptr;
   ~Batch() {
     for (int i = 0; i < nbufs; ++i)
       if (bufs[i]) (void)hipFree(bufs[i]);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev2) (void)hipEventDestroy(ev2);
+    if (evg) (void)hipEventDestroy(evg);
+  }
+  bool ensure_guess_buffers(std::string& err) {
+    if (d_X || ngp_total == 0) return true;
+    return alloc(d_X, (size_t)ngp_total * pm3guess::NF, err) && alloc(d_Y, (size_t)ngp_total * pm3guess::NO, err) &&
+           set_y();
+  }
+  bool set_y() {
+    args.gY = d_Y;
+    return true;
   }
   template <class T> bool alloc(T*& p, size_t count, std::string& err) {
+    if (nbufs >= (int)(sizeof(bufs) / sizeof(bufs[0]))) {
+      err = "internal: too many batch buffers";
+      return false;
+    }
     hipError_t e = hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(count, 1));
     if (e != hipSuccess) {
       err = std::string("hipMalloc: ") + hipGetErrorString(e);
@@ -1590,6 +1738,51 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   }
   b.atoms_total = (long long)aslot.size();
   b.nsolve = (int)b.solve.size();
+  // Directed pairs of the learned guess (molecules made of H, C and O only): for every source
+  // atom in the engine's order, its neighbours inside 4 Angstrom in the caller's atom order.
+  std::vector<int8_t> elem8(b.atoms_total, (int8_t)-1);
+  std::vector<int> gbeg(b.atoms_total, 0), gcnt(b.atoms_total, 0), gsrc, gdst, gpart;
+  b.eligible.assign(nmol, 0);
+  for (long long t = 0; t < b.atoms_total; ++t) elem8[t] = (int8_t)pm3guess::elem_index(P.table.el[aslot[t]].z);
+  for (int k : b.solve) {
+    MolDesc& d = b.desc[k];
+    bool ok = true;
+    for (int t = 0; t < d.nat; ++t) ok = ok && elem8[d.atom0 + t] >= 0;
+    d.learn = ok ? 1 : 0;
+    d.gp0 = (long long)gsrc.size();
+    d.ngp = 0;
+    b.eligible[k] = d.learn;
+    if (!ok) continue;
+    const std::vector<int>& ord = b.atom_map[k];
+    std::vector<int> inv(d.nat);
+    for (int t = 0; t < d.nat; ++t) inv[ord[t]] = t;
+    std::vector<int> pidx((size_t)d.nat * d.nat, -1);
+    for (int sa = 0; sa < d.nat; ++sa) {
+      const int ga = d.atom0 + sa;
+      gbeg[ga] = (int)gsrc.size();
+      for (int j = 0; j < d.nat; ++j) {
+        const int sb = inv[j];
+        if (sb == sa) continue;
+        const int gb = d.atom0 + sb;
+        const double dx = ax[gb] - ax[ga], dy = ay[gb] - ay[ga], dz = az[gb] - az[ga];
+        if (!(std::sqrt(dx * dx + dy * dy + dz * dz) * pm3guess::BOHR_A < pm3guess::CUT_A)) continue;
+        pidx[(size_t)sa * d.nat + sb] = (int)gsrc.size();
+        gsrc.push_back(ga);
+        gdst.push_back(gb);
+      }
+      gcnt[ga] = (int)gsrc.size() - gbeg[ga];
+    }
+    d.ngp = (int)((long long)gsrc.size() - d.gp0);
+    gpart.resize(gsrc.size());
+    for (long long j = d.gp0; j < (long long)gsrc.size(); ++j)
+      gpart[j] = pidx[(size_t)(gdst[j] - d.atom0) * d.nat + (gsrc[j] - d.atom0)];
+  }
+  b.ngp_total = (long long)gsrc.size();
+  if (b.ngp_total >= (1LL << 31)) {
+    err = "too many directed atom pairs in one batch";
+    delete bp;
+    return false;
+  }
   // largest molecules first so the queue ends on short work
   std::stable_sort(b.solve.begin(), b.solve.end(), [&](int x, int y) { return b.desc[x].nao > b.desc[y].nao; });
   // molecule store: H_core (np x np) then W_xx | W_xh | W_hh | S1 | S2 | e_cc per molecule, and the
@@ -1627,6 +1820,20 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
       !b.alloc(d_store, (size_t)store_total, err) || !b.alloc(d_pbase, pair_base.size(), err) ||
       !b.alloc(d_g, 3 * (size_t)b.atoms_total, err))
     return false;
+  int8_t* d_elem8;
+  int *d_gbeg, *d_gcnt, *d_gsrc, *d_gdst, *d_gpart;
+  if (!b.alloc(d_elem8, b.atoms_total, err) || !b.alloc(d_gbeg, b.atoms_total, err) ||
+      !b.alloc(d_gcnt, b.atoms_total, err) || !b.alloc(d_gsrc, gsrc.size(), err) || !b.alloc(d_gdst, gdst.size(), err) ||
+      !b.alloc(d_gpart, gpart.size(), err))
+    return false;
+  HIP_OK(hipMemcpy(d_elem8, elem8.data(), elem8.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_gbeg, gbeg.data(), sizeof(int) * gbeg.size(), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_gcnt, gcnt.data(), sizeof(int) * gcnt.size(), hipMemcpyHostToDevice));
+  if (b.ngp_total > 0) {
+    HIP_OK(hipMemcpy(d_gsrc, gsrc.data(), sizeof(int) * gsrc.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_gdst, gdst.data(), sizeof(int) * gdst.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d_gpart, gpart.data(), sizeof(int) * gpart.size(), hipMemcpyHostToDevice));
+  }
   b.want_gradient = req.want_gradient;
   HIP_OK(hipMemset(d_store, 0, sizeof(double) * std::max<long long>(store_total, 1)));
   HIP_OK(hipMemcpy(d_pbase, pair_base.data(), sizeof(long long) * pair_base.size(), hipMemcpyHostToDevice));
@@ -1661,6 +1868,22 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   a.npairs_total = pair_base.back();
   a.prof = nullptr;
   a.lds_doubles = b.lds_need / 8;
+  a.elem8 = d_elem8;
+  a.gbeg = d_gbeg;
+  a.gcnt = d_gcnt;
+  a.gsrc = d_gsrc;
+  a.gdst = d_gdst;
+  a.gpart = d_gpart;
+  a.gY = nullptr;
+  b.guess_model = opt.guess_model;
+  b.learned = opt.guess == 4;
+  if (b.learned) {
+    if (!b.guess_model) {
+      err = "learned guess requested without a guess model";
+      return false;
+    }
+    if (!b.ensure_guess_buffers(err)) return false;
+  }
   if (const char* ev = std::getenv("PM3_PHASE_TIMERS")) {
     if (ev[0] == '1') {
       unsigned long long* d_prof;
@@ -1681,6 +1904,19 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   return true;
 }
 
+static BatchGuessView infer_view(Batch& b) {
+  BatchGuessView v;
+  v.ngp_total = b.ngp_total;
+  v.atoms_total = b.atoms_total;
+  v.ax = b.args.ax; v.ay = b.args.ay; v.az = b.args.az;
+  v.elem = b.args.elem8;
+  v.aoff = b.args.aoff;
+  v.gbeg = b.args.gbeg; v.gcnt = b.args.gcnt; v.gsrc = b.args.gsrc; v.gdst = b.args.gdst; v.gpart = b.args.gpart;
+  v.X = b.d_X;
+  v.Y = b.d_Y;
+  return v;
+}
+
 bool batch_solve(Batch* bp, std::string& err) {
   Batch& b = *bp;
   b.last_kernel_ms = 0.0;
@@ -1689,12 +1925,15 @@ bool batch_solve(Batch* bp, std::string& err) {
     HIP_OK(hipEventCreate(&b.ev0));
     HIP_OK(hipEventCreate(&b.ev1));
     HIP_OK(hipEventCreate(&b.ev2));
+    HIP_OK(hipEventCreate(&b.evg));
   }
   HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), 0));
   if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, 12 * sizeof(unsigned long long), 0));
   HIP_OK(hipEventRecord(b.ev0, 0));
   hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(256), 0, 0, b.args);
   HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(b.evg, 0));
+  if (b.learned && !guess_infer(b.guess_model, infer_view(b), err)) return false;
   HIP_OK(hipEventRecord(b.ev1, 0));
   hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
   HIP_OK(hipGetLastError());
@@ -1705,13 +1944,43 @@ bool batch_solve(Batch* bp, std::string& err) {
   }
   HIP_OK(hipEventRecord(b.ev2, 0));
   HIP_OK(hipEventSynchronize(b.ev2));
-  float ms = 0.0f, pms = 0.0f;
-  HIP_OK(hipEventElapsedTime(&pms, b.ev0, b.ev1));
+  float ms = 0.0f, pms = 0.0f, gms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&pms, b.ev0, b.evg));
+  HIP_OK(hipEventElapsedTime(&gms, b.evg, b.ev1));
   HIP_OK(hipEventElapsedTime(&ms, b.ev1, b.ev2));
   b.last_pair_ms = pms;
+  b.last_guess_ms = gms;
   b.last_kernel_ms = ms;
+  b.solved_once = true;
   return true;
 }
+
+bool batch_guess_view(Batch* bp, BatchGuessView& v, std::string& err) {
+  Batch& b = *bp;
+  if (!b.ensure_guess_buffers(err)) return false;
+  v = infer_view(b);
+  v.nmol = b.nmol;
+  v.atom0.assign(b.nmol, 0); v.nat.assign(b.nmol, 0); v.nao.assign(b.nmol, 0); v.ngp.assign(b.nmol, 0);
+  v.eligible.assign(b.nmol, 0); v.solved.assign(b.nmol, 0); v.gp0.assign(b.nmol, 0); v.out_mat.assign(b.nmol, -1);
+  std::vector<MolOut> ho;
+  if (b.solved_once && b.nsolve > 0) {
+    ho.resize(b.nmol);
+    HIP_OK(hipMemcpy(ho.data(), b.args.out, sizeof(MolOut) * b.nmol, hipMemcpyDeviceToHost));
+  }
+  v.caller_atom.assign(b.atoms_total, 0);
+  for (int k : b.solve) {
+    const MolDesc& d = b.desc[k];
+    for (int t = 0; t < d.nat; ++t) v.caller_atom[d.atom0 + t] = b.atom_map[k][t];
+    v.atom0[k] = d.atom0; v.nat[k] = d.nat; v.nao[k] = d.nao; v.ngp[k] = d.ngp; v.gp0[k] = d.gp0;
+    v.out_mat[k] = d.out_mat;
+    v.eligible[k] = b.eligible[k];
+    v.solved[k] = !ho.empty() && ho[k].status == 0 ? 1 : 0;
+  }
+  v.out_P = b.mat_total > 0 ? b.args.out_P : nullptr;
+  return true;
+}
+
+double batch_guess_ms(const Batch* b) { return b ? b->last_guess_ms : 0.0; }
 
 double batch_kernel_ms(const Batch* b) { return b ? b->last_kernel_ms : 0.0; }
 double batch_pair_ms(const Batch* b) { return b ? b->last_pair_ms : 0.0; }
@@ -1759,6 +2028,7 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
     r.iterations = o.iters;
     r.converged = o.converged;
     r.paths = o.paths;
+    r.guess_used = o.guess_used;
     r.nao = d.nao;
     r.nocc = d.nocc;
     for (int x = 0; x < 3; ++x) r.dipole[x] = o.dip[x] * AU_DIPOLE_TO_DEBYE;
@@ -1797,9 +2067,17 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
 void batch_destroy(Batch* b) { delete b; }
 
 bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,
-               int& status, std::string& err) {
+               int& status, std::string& err, const GuessModel* model) {
   std::vector<MolIn> mols{mol};
   EngineOptions opt{1, 0.0, 0.0, 0, 0.1, 1, 0, 512};
+  if (mode == 5) {
+    if (!model) {
+      err = "stage 5 needs a guess model";
+      return false;
+    }
+    opt.guess = 4;
+    opt.guess_model = model;
+  }
   Batch* b = nullptr;
   status = 0;
   if (!batch_create(P, mols, opt, RunRequest(), b, err)) {
@@ -1808,6 +2086,11 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
   }
   if (b->nsolve == 0) {
     status = b->pre_status[0];
+    batch_destroy(b);
+    return true;
+  }
+  if (mode == 5 && !b->desc[0].learn) {
+    status = 4;  // the learned guess covers H, C and O only
     batch_destroy(b);
     return true;
   }
@@ -1831,6 +2114,9 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
   }
   if (ok) {
     hipLaunchKernelGGL(pair_kernel, dim3(b->pair_blocks), dim3(256), 0, 0, b->args);
+    if (mode == 5 && !guess_infer(model, infer_view(*b), err)) ok = false;
+  }
+  if (ok) {
     hipLaunchKernelGGL(stage_kernel, dim3(1), dim3(NT), b->lds_need, 0, b->args, mode, d_in, d_out);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -1838,11 +2124,15 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
     if (e == hipSuccess) e = hipMemcpy(so.data(), d_out, sizeof(double) * nout, hipMemcpyDeviceToHost);
     if (e != hipSuccess) { err = hipGetErrorString(e); ok = false; }
     if (ok) {
-      if (mode == 0 || mode == 1) {
-        out.assign((size_t)n * n + (mode == 0 ? 1 : 0), 0.0);
+      if (mode == 0 || mode == 1 || mode == 5) {
+        out.assign((size_t)n * n + (mode == 0 ? 1 : mode == 5 ? 2 : 0), 0.0);
         for (int i = 0; i < n; ++i)
           for (int j = 0; j < n; ++j) out[(size_t)mp[i] * n + mp[j]] = so[(size_t)i * n + j];
         if (mode == 0) out[(size_t)n * n] = so[(size_t)n * n];
+        if (mode == 5) {
+          out[(size_t)n * n] = so[(size_t)n * n];
+          out[(size_t)n * n + 1] = so[(size_t)n * n + 1];
+        }
       } else {
         out.assign(so.begin(), so.begin() + (size_t)n * n + (mode == 3 ? 0 : n));
       }

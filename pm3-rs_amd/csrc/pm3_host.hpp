@@ -30,9 +30,10 @@ struct EngineOptions {
   double e_tol, p_tol;
   int accel;        // 0 none, 1 CDIIS, 2 A-DIIS -> CDIIS (already combined with use_diis)
   double adiis_switch;
-  int guess;        // 1 SAD, 2 core, 3 symmetry-broken
+  int guess;        // 1 SAD, 2 core, 3 symmetry-broken, 4 learned (superposition fallback)
   int stability;    // 0 off, 1 auto, 2 always
   int scf_memory_mb;
+  const struct GuessModel* guess_model = nullptr;  // network of the learned guess (guess 4)
 };
 
 // One molecule as handed to the engine, atoms in caller order.
@@ -50,6 +51,7 @@ struct MolResult {
   double homo = 0, lumo = 0;
   double dipole[3] = {0, 0, 0};  // Debye
   int iterations = 0, converged = 0, paths = 0, nocc = 0, nao = 0;
+  int guess_used = 0;  // 0 superposition, 1 learned, 2 learned rejected by the commutator test
   std::vector<double> charges, density, mo_energies, mo_coeff, gradient;
 };
 
@@ -69,6 +71,7 @@ bool batch_fetch(Batch* b, const Params& P, const std::vector<MolIn>& mols, std:
 void batch_destroy(Batch* b);
 double batch_kernel_ms(const Batch* b);  // device time of the last solve's SCF launch (HIP events)
 double batch_pair_ms(const Batch* b);    // device time of the last solve's pair-integral launch
+double batch_guess_ms(const Batch* b);   // device time of the last solve's learned-guess launches
 // Per-phase wall-clock ticks (10 ns) summed over workgroups, when PM3_PHASE_TIMERS=1 at creation:
 // 0 H_core finish, 1 Fock, 2 energy+DIIS, 3 warm-start products, 4 Jacobi, 5 sort+C V, 6 density,
 // 7 loop bookkeeping and epilogue; then [8] unused, [9] Jacobi sweeps, [10] eigensolves, [11] rounds.
@@ -76,7 +79,7 @@ bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err
 
 // One stage of the pipeline on a single molecule (kernel-level parity hooks); see stage_kernel.
 bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,
-               int& status, std::string& err);
+               int& status, std::string& err, const struct GuessModel* model = nullptr);
 
 // create + solve + fetch.  Returns false (and sets err) only for a device/launch failure;
 // per-molecule failures are in out[k].status.

@@ -74,7 +74,8 @@ STABILITY = {"off": 0, "auto": 1, "always": 2}
 class _COptions(ctypes.Structure):
     _fields_ = [("max_scf", ctypes.c_int32), ("e_tol", ctypes.c_double), ("p_tol", ctypes.c_double),
                 ("use_diis", ctypes.c_int32), ("accelerator", ctypes.c_int32), ("adiis_switch", ctypes.c_double),
-                ("guess", ctypes.c_int32), ("stability", ctypes.c_int32), ("scf_memory_mb", ctypes.c_int32)]
+                ("guess", ctypes.c_int32), ("stability", ctypes.c_int32), ("scf_memory_mb", ctypes.c_int32),
+                ("guess_model", ctypes.c_void_p)]
 
 
 class _CMolecule(ctypes.Structure):
@@ -90,14 +91,18 @@ class _CResult(ctypes.Structure):
                 ("last_change", ctypes.c_double), ("scf_improvement_ev", ctypes.c_double),
                 ("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("scf_paths_tried", ctypes.c_int32),
                 ("n_occ", ctypes.c_int32), ("nao", ctypes.c_int32), ("has_homo", ctypes.c_int32),
-                ("has_lumo", ctypes.c_int32)]
+                ("has_lumo", ctypes.c_int32), ("guess_used", ctypes.c_int32)]
 
 
 EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_run", "pm3_gradient",
             "pm3_run_batch", "pm3_batch_create", "pm3_batch_solve", "pm3_batch_results", "pm3_batch_destroy",
-            "pm3_batch_last_kernel_ms", "pm3_batch_gradient", "pm3_batch_last_pair_ms", "pm3_batch_phase_ticks", "pm3_stage_hcore", "pm3_stage_fock", "pm3_stage_eigen", "pm3_stage_eigen_warm",
-            "pm3_stage_gemm",
-            "pm3_last_error_message"]
+            "pm3_batch_last_kernel_ms", "pm3_batch_gradient", "pm3_batch_last_pair_ms", "pm3_batch_last_guess_ms",
+            "pm3_batch_phase_ticks", "pm3_stage_hcore", "pm3_stage_fock", "pm3_stage_eigen", "pm3_stage_eigen_warm",
+            "pm3_stage_gemm", "pm3_last_error_message",
+            "pm3_guess_model_create", "pm3_guess_model_load", "pm3_guess_model_save", "pm3_guess_model_get_params",
+            "pm3_guess_model_set_params", "pm3_guess_model_steps", "pm3_guess_model_destroy", "pm3_guess_data_create",
+            "pm3_guess_data_add", "pm3_guess_data_stats", "pm3_guess_data_destroy", "pm3_guess_train_options_default",
+            "pm3_guess_train", "pm3_guess_eval", "pm3_guess_grad", "pm3_guess_mlp", "pm3_guess_features", "pm3_guess_density"]
 
 _lib = None
 
@@ -129,8 +134,29 @@ def native():
         L.pm3_batch_last_kernel_ms.restype = ctypes.c_double
         L.pm3_batch_last_pair_ms.argtypes = [ctypes.c_void_p]
         L.pm3_batch_last_pair_ms.restype = ctypes.c_double
+        L.pm3_batch_last_guess_ms.argtypes = [ctypes.c_void_p]
+        L.pm3_batch_last_guess_ms.restype = ctypes.c_double
         L.pm3_batch_phase_ticks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
         L.pm3_last_error_message.restype = ctypes.c_char_p
+        vp, fp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)
+        up, i64p = ctypes.POINTER(ctypes.c_uint16), ctypes.POINTER(ctypes.c_int64)
+        L.pm3_guess_model_create.argtypes = [ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.pm3_guess_model_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.pm3_guess_model_save.argtypes = [vp, ctypes.c_char_p]
+        L.pm3_guess_model_get_params.argtypes = [vp, fp]
+        L.pm3_guess_model_set_params.argtypes = [vp, fp]
+        L.pm3_guess_model_steps.argtypes = [vp]
+        L.pm3_guess_model_destroy.argtypes = [vp]
+        L.pm3_guess_data_create.argtypes = [ctypes.POINTER(vp)]
+        L.pm3_guess_data_add.argtypes = [vp, vp, i64p]
+        L.pm3_guess_data_stats.argtypes = [vp, i64p, i64p, i64p]
+        L.pm3_guess_data_destroy.argtypes = [vp]
+        L.pm3_guess_train.argtypes = [vp, vp, vp, fp, dp]
+        L.pm3_guess_eval.argtypes = [vp, vp, dp]
+        L.pm3_guess_grad.argtypes = [vp, vp, fp, dp]
+        L.pm3_guess_mlp.argtypes = [vp, ctypes.c_int32, up, fp, fp, fp]
+        L.pm3_guess_features.argtypes = [ctypes.POINTER(_CMolecule), vp, ip, ip, ip, up]
+        L.pm3_guess_density.argtypes = [vp, ctypes.POINTER(_CMolecule), vp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -211,10 +237,12 @@ class Pm3Options:
     guess: str = "auto"
     stability: str = "auto"
     scf_memory_mb: int = 512
+    guess_model: Optional[object] = None  # pm3amd.guess.GuessModel, required by guess="learned"
 
     def _c(self):
         return _COptions(self.max_scf, self.e_tol, self.p_tol, int(self.use_diis), ACCEL[self.accelerator],
-                         self.adiis_switch, GUESS[self.guess], STABILITY[self.stability], self.scf_memory_mb)
+                         self.adiis_switch, GUESS[self.guess], STABILITY[self.stability], self.scf_memory_mb,
+                         self.guess_model.handle if self.guess_model is not None else None)
 
 
 @dataclasses.dataclass
@@ -238,6 +266,7 @@ class Pm3Result:
     scf_paths_tried: int
     scf_improvement_ev: float
     unrestricted: bool = False
+    guess_used: int = 0  # 0 superposition / other rule, 1 learned, 2 learned rejected
 
 
 @dataclasses.dataclass
@@ -254,7 +283,7 @@ def _result(r, P, eps, C, q):
     return Pm3Result(P, eps, C, r.n_occ, r.electronic_ev, r.core_ev, r.total_ev, r.heat_of_formation_kcal, q,
                      np.array(r.dipole_debye[:]), r.dipole_magnitude, r.homo_ev if r.has_homo else None,
                      r.lumo_ev if r.has_lumo else None, r.iterations, bool(r.converged), r.scf_paths_tried,
-                     r.scf_improvement_ev)
+                     r.scf_improvement_ev, False, r.guess_used)
 
 
 def _dp(a):
@@ -336,6 +365,10 @@ class Batch:
     def last_pair_ms(self):
         """Device time of the last solve's pair-integral launch."""
         return native().pm3_batch_last_pair_ms(self._h)
+
+    def last_guess_ms(self):
+        """Device time of the last solve's learned-guess launches (features + network)."""
+        return native().pm3_batch_last_guess_ms(self._h)
 
     PHASES = ("hcore", "fock", "energy_diis", "warm_products", "jacobi", "sort_cv", "density", "other")
 
