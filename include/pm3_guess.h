@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-enum { PM3_GUESS_NF = 96, PM3_GUESS_NH = 128, PM3_GUESS_NO = 16, PM3_GUESS_NPARAM = 30992 };
+enum { PM3_GUESS_NF = 96, PM3_GUESS_NH = 128, PM3_GUESS_NO = 16, PM3_GUESS_NPARAM = 30864 };
 
 typedef struct pm3_guess_model pm3_guess_model;
 typedef struct pm3_guess_data pm3_guess_data;

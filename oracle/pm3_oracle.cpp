@@ -1207,8 +1207,16 @@ static void rhf_loop(const Mol& m, const Core& core, int nocc, const Options& o,
                      const double* P0 = nullptr) {
   int n = m.nao;
   std::vector<double> P;
+  int pre = 0;
   if (P0) {
-    P.assign(P0, P0 + (size_t)n * n);
+    // a caller density (the learned guess) only builds the first Fock matrix; the SCF proper
+    // starts from that Fock matrix's aufbau density, counted as one iteration
+    std::vector<double> F0((size_t)n * n), w(n), C((size_t)n * n);
+    build_fock(m, core, P0, F0.data());
+    sym_eigen(n, F0.data(), w.data(), C.data());
+    P.assign((size_t)n * n, 0.0);
+    density_from_coeff(n, C.data(), nocc, 2.0, P.data());
+    pre = 1;
   } else if (guess == 2) {
     std::vector<double> w(n), C((size_t)n * n);
     sym_eigen(n, core.h.data(), w.data(), C.data());
@@ -1282,6 +1290,7 @@ static void rhf_loop(const Mol& m, const Core& core, int nocc, const Options& o,
   build_fock(m, core, P.data(), F.data());
   st.e_elec = 0.5 * (dot(P, core.h) + dot(P, F));
   st.P = P;
+  st.iterations += pre;
 }
 
 }  // namespace orc
@@ -1534,9 +1543,10 @@ int orc_run(int nat, const int* z, const double* xyz, double charge, const Optio
   return run_impl(nat, z, xyz, charge, opt, nullptr, res, density, mo_energies, mo_coeff, charges);
 }
 
-// The same single point started from a caller-provided density P0 (n x n).  The stability rungs,
-// when the gap rule fires, are those of the superposition guess.  This is the restatement of a
-// learned starting density handed to the unchanged SCF.
+// The same single point started from a caller-provided density P0 (n x n): P0 builds the first
+// Fock matrix, whose aufbau density starts the unchanged SCF loop (one extra counted iteration),
+// so only idempotent densities enter the DIIS history.  The stability rungs, when the gap rule
+// fires, are those of the superposition guess.  This restates the learned-guess start.
 int orc_run_from_density(int nat, const int* z, const double* xyz, double charge, const Options* opt, const double* P0,
                          Result* res, double* density) {
   if (!P0) return E_INVALID;

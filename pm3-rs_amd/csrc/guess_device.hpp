@@ -39,7 +39,7 @@ constexpr int NH = 128;  // hidden width
 constexpr int NO = 16;   // outputs (9 used)
 constexpr int NW1 = NF * NH, NW2 = NH * NH, NB2 = NH, NW3 = NH * NO, NB3 = NO;
 constexpr int OFF_W1 = 0, OFF_W2 = OFF_W1 + NW1, OFF_B2 = OFF_W2 + NW2, OFF_W3 = OFF_B2 + NB2, OFF_B3 = OFF_W3 + NW3;
-constexpr int NPARAM = OFF_B3 + NB3;  // 30992
+constexpr int NPARAM = OFF_B3 + NB3;  // 30864
 constexpr double CUT_A = 4.0;
 constexpr double BOHR_A = 0.529177210903;  // src/constants.rs:23
 constexpr double PI = 3.14159265358979323846;

@@ -434,6 +434,10 @@ pm3_status pm3_stage_gemm(int32_t n, const double* a, const double* b, double* c
 
 // ------------------------------------------------------------------------------------------
 // Learned initial density (include/pm3_guess.h).
+static_assert(PM3_GUESS_NPARAM == pm3guess::NPARAM && PM3_GUESS_NF == pm3guess::NF && PM3_GUESS_NH == pm3guess::NH &&
+                  PM3_GUESS_NO == pm3guess::NO,
+              "pm3_guess.h sizes must match the network");
+
 extern "C" {
 
 pm3_status pm3_guess_model_create(uint64_t seed, pm3_guess_model** out) {

@@ -944,9 +944,11 @@ __device__ double commutator2(const KArgs& A, const Mol& M, const Bufs& B, doubl
 }
 
 // Initial density for a guess mode (src/scf.rs:855-935): 1 SAD, 2 core, 3 symmetry-broken,
-// 4 learned with the superposition fallback (the learned density is kept only when its
-// first-iteration commutator norm is not larger than the superposition's).  Returns the guess
-// used: 0 superposition (or not learned), 1 learned, 2 learned rejected.
+// 4 learned with the superposition fallback.  The learned density is kept only when its
+// first-iteration commutator norm is not larger than the superposition's; it then only builds
+// the first Fock matrix, whose aufbau density (orbitals left in B.C for the warm start) starts the
+// SCF, so that only idempotent densities enter the DIIS history.  Returns the guess used:
+// 0 superposition (or not learned), 1 learned (one pre-iteration done), 2 learned rejected.
 __device__ __noinline__ int initial_guess(const KArgs& A, const Mol& M, const Bufs& B, int guess, double* lds,
                                           double* red) {
   const int np = M.np, tid = threadIdx.x;
@@ -958,20 +960,22 @@ __device__ __noinline__ int initial_guess(const KArgs& A, const Mol& M, const Bu
     return 0;
   }
   if (guess == 4 && M.learn && A.gY) {
-    learned_into(A, M, B.P);
-    const double cl = commutator2(A, M, B, lds, red);
-    for (int e = tid; e < np * np; e += NT) {
-      B.Pb[e] = B.P[e];
-      B.P[e] = 0.0;
-    }
-    __syncthreads();
     sad_into(A, M, B.P);
     const double cs = commutator2(A, M, B, lds, red);
+    for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
+    __syncthreads();
+    learned_into(A, M, B.P);
+    const double cl = commutator2(A, M, B, lds, red);  // leaves F(P_learned) in B.F
     if (cl <= cs) {
-      for (int e = tid; e < np * np; e += NT) B.P[e] = B.Pb[e];
+      eigensolve(M, B, B.F, false, lds, red);
+      density(M, B.Cn, B.P);
+      for (int e = tid; e < np * np; e += NT) B.C[e] = B.Cn[e];
       __syncthreads();
       return 1;
     }
+    for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
+    __syncthreads();
+    sad_into(A, M, B.P);
     return 2;
   }
   sad_into(A, M, B.P);
@@ -1029,7 +1033,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
   __syncthreads();
   PathResult R{0.0, 1.0e300, 0, 0, gused};
   double e_old = 0.0;
-  bool have_c = false;
+  bool have_c = gused == 1;
   for (int it = 0; it < A.max_scf; ++it) {
     R.iters = it + 1;
     stamp(B, 7);
@@ -1163,6 +1167,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
   for (int e = tid; e < np * np; e += NT) ev[0] += B.P[e] * (M.H[e] + B.F[e]);
   block_sum<1>(ev, red);
   R.e_elec = 0.5 * ev[0];
+  if (gused == 1) R.iters += 1;  // the learned pre-iteration
   return R;
 }
 
@@ -1555,6 +1560,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
     learned_into(A, M, B.P);
     const double cl = commutator2(A, M, B, lds, red);
     for (int e = tid; e < n * n; e += NT) out[e] = B.P[(e / n) * np + e % n];
+    __syncthreads();
     for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
     __syncthreads();
     sad_into(A, M, B.P);

@@ -195,15 +195,20 @@ def test_learned_scf_keeps_converged_answers(G, oracle):
         assert abs(a.heat_of_formation_kcal - c.heat_of_formation_kcal) <= HOF_TOL
         assert np.abs(a.density - c.density).max() <= 1e-8
         assert c.guess_used in (1, 2)
-    # the oracle started from the device's learned density reaches the same answer
-    for (z, xb), c in list(zip(mols, rl))[:4]:
-        if c.guess_used != 1:
+    # the oracle started from the device's learned density follows the same SCF (tolerances at
+    # which the stopping iteration is not decided by rounding)
+    loose = dict(p_tol=1e-8, e_tol=1e-8)
+    for z, xb in mols[:6]:
+        mol = G.Molecule(z, xb)
+        r = G.run_pm3(mol, options=G.Pm3Options(guess="learned", guess_model=model, **loose))
+        if r.guess_used != 1:
             continue
-        P0, cl, cs = model.density(G.Molecule(z, xb))
+        P0, cl, cs = model.density(mol)
         assert cl <= cs
-        st, o = oracle.run_from_density(z, xb, P0, 0.0, oracle.options(**tight))
-        assert st == 0 and abs(o["hof_kcal"] - c.heat_of_formation_kcal) <= HOF_TOL
-        assert abs(o["iterations"] - c.iterations) <= 1
+        st, o = oracle.run_from_density(z, xb, P0, 0.0, oracle.options(**loose))
+        assert st == 0 and abs(o["hof_kcal"] - r.heat_of_formation_kcal) <= HOF_TOL
+        print("iterations", r.iterations, o["iterations"], flush=True)
+        assert abs(o["iterations"] - r.iterations) <= 1
 
 
 def test_trained_guess_cuts_iterations(G):
