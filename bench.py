@@ -36,6 +36,8 @@ import pm3amd as G  # noqa: E402
 from pm3amd import workloads as W  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
+NET_FLOP_PER_PAIR = 2.0 * (96 * 128 + 128 * 128 + 128 * 16)  # forward pass per directed pair
 HBM_PEAK_GBS = 8000.0
 HOF_TOL = 1.0e-5
 
@@ -107,15 +109,18 @@ def main():
         batch = G.Batch(molecules, options=opts, want_gradient=args.gradient)
         for _ in range(args.warmup):
             batch.solve()
-        kms, pms, gms = [], [], []
+        kms, pms, gms, nms = [], [], [], []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             batch.solve()
             kms.append(batch.last_kernel_ms())
             pms.append(batch.last_pair_ms())
             gms.append(batch.last_guess_ms())
+            nms.append(batch.guess_timing()[1])
         t1 = time.perf_counter()
         st, rs = batch.results()
+        leg.network_ms = float(np.mean(nms))
+        leg.pairs = batch.guess_timing()[2]
         return batch, st, rs, (t1 - t0) * 1e3 / args.steps, float(np.mean(kms)), float(np.mean(pms)), float(np.mean(gms))
 
     pbatch, pstatus, pres, p_ms, p_kms, p_pms, _ = leg(G.Pm3Options(p_tol=1.0e-8, e_tol=1.0e-8))
@@ -203,7 +208,11 @@ def main():
                   "mean_iterations": float(np.mean([r.iterations for r in pres])), "converged": plain_valid},
         "speedup_vs_plain": value / plain_value if plain_value > 0 else None,
         "learned": None if model is None else {
-            "guess_kernels_ms": guess_ms, "kept_fraction": float(np.mean(used == 1)),
+            "guess_kernels_ms": guess_ms, "network_kernel_ms": leg.network_ms, "directed_pairs": leg.pairs,
+            "network_roofline": {"bound": "mfma", "kernel": "mlp_fwd_kernel", "unit": "TFLOP/s", "peak": BF16_PEAK_TFLOPS,
+                                 "achieved": NET_FLOP_PER_PAIR * leg.pairs / (leg.network_ms * 1e-3) / 1e12,
+                                 "frac": NET_FLOP_PER_PAIR * leg.pairs / (leg.network_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS},
+            "kept_fraction": float(np.mean(used == 1)),
             "fallback_rate": float(np.mean(used == 2)), "not_applicable": float(np.mean(used == 0)),
             "model_steps": model.steps},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

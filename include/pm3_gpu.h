@@ -138,6 +138,10 @@ double pm3_batch_last_kernel_ms(const pm3_batch* batch);
 double pm3_batch_last_pair_ms(const pm3_batch* batch);
 /* Device time (ms) of the last pm3_batch_solve's learned-guess kernels (features + network). */
 double pm3_batch_last_guess_ms(const pm3_batch* batch);
+/* The same split into the feature kernel and the network kernel, plus the batch's number of
+ * directed atom pairs inside the 4 Angstrom cutoff (the network's rows). */
+pm3_status pm3_batch_guess_timing(const pm3_batch* batch, double* features_ms, double* network_ms,
+                                  int64_t* directed_pairs);
 /* Per-phase wall-clock ticks (10 ns) of the last solve summed over workgroups, collected only when
  * the environment has PM3_PHASE_TIMERS=1 at pm3_batch_create (zeros otherwise): H_core finish,
  * Fock, energy+DIIS, warm-start products, Jacobi, sort + C V, density, bookkeeping; then out[8]

@@ -48,12 +48,16 @@ void pm3_guess_data_destroy(pm3_guess_data* data);
 
 /* Adam on the per-atom mean squared error of the assembled on-site and pair blocks.  Each step
  * draws batch_molecules molecules without replacement (reshuffled per epoch, seeded).
- * losses[steps] receives the loss before each step (may be NULL); seconds the wall time. */
+ * losses[steps] receives the loss before each step (may be NULL); seconds the wall time.
+ * With decay_steps > 0 the step size follows a cosine from lr to lr_final over the model's
+ * optimiser steps 0..decay_steps (counted across calls), and stays at lr_final after. */
 typedef struct {
   int32_t steps;
   int32_t batch_molecules; /* default 1024 */
   float lr, beta1, beta2, eps; /* defaults 1e-3, 0.9, 0.999, 1e-8 */
   uint64_t seed;
+  float lr_final;          /* default 1e-5 */
+  int32_t decay_steps;     /* default 0: constant lr */
 } pm3_guess_train_options;
 void pm3_guess_train_options_default(pm3_guess_train_options* out);
 pm3_status pm3_guess_train(pm3_guess_model* model, pm3_guess_data* data, const pm3_guess_train_options* opts,

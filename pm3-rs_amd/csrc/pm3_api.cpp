@@ -262,6 +262,17 @@ double pm3_batch_last_pair_ms(const pm3_batch* h) { return h ? batch_pair_ms(h->
 
 double pm3_batch_last_guess_ms(const pm3_batch* h) { return h ? batch_guess_ms(h->b) : 0.0; }
 
+pm3_status pm3_batch_guess_timing(const pm3_batch* h, double* features_ms, double* network_ms, int64_t* directed_pairs) {
+  if (!h) return fail(PM3_INVALID_INPUT, "null batch");
+  double f = 0.0, n = 0.0;
+  long long p = 0;
+  batch_guess_timing(h->b, f, n, p);
+  if (features_ms) *features_ms = f;
+  if (network_ms) *network_ms = n;
+  if (directed_pairs) *directed_pairs = p;
+  return PM3_OK;
+}
+
 pm3_status pm3_batch_phase_ticks(const pm3_batch* h, uint64_t* out) {
   if (!h || !out) return fail(PM3_INVALID_INPUT, "null argument");
   std::string err;
@@ -540,6 +551,8 @@ void pm3_guess_train_options_default(pm3_guess_train_options* out) {
   out->beta2 = 0.999f;
   out->eps = 1.0e-8f;
   out->seed = 1;
+  out->lr_final = 1.0e-5f;
+  out->decay_steps = 0;
 }
 
 pm3_status pm3_guess_train(pm3_guess_model* model, pm3_guess_data* data, const pm3_guess_train_options* opts,
@@ -555,6 +568,9 @@ pm3_status pm3_guess_train(pm3_guess_model* model, pm3_guess_data* data, const p
   o.beta2 = opts->beta2;
   o.eps = opts->eps;
   o.seed = opts->seed;
+  o.lr_final = opts->lr_final;
+  o.decay_steps = opts->decay_steps;
+  if (o.decay_steps < 0 || !(o.lr_final >= 0.0f)) return fail(PM3_INVALID_INPUT, "bad learning-rate schedule");
   std::string err;
   if (!guess_train(model->m, data->d, o, losses, seconds, err)) return fail(PM3_DEVICE_ERROR, err);
   return PM3_OK;

@@ -1607,7 +1607,8 @@ struct Batch {
   std::vector<int> eligible;
   uint16_t* d_X = nullptr;
   float* d_Y = nullptr;
-  hipEvent_t evg = nullptr;
+  hipEvent_t evg = nullptr, evm = nullptr;
+  double last_feature_ms = 0.0, last_network_ms = 0.0;
   ~Batch() {
     for (int i = 0; i < nbufs; ++i)
       if (bufs[i]) (void)hipFree(bufs[i]);
@@ -1615,6 +1616,7 @@ struct Batch {
     if (ev1) (void)hipEventDestroy(ev1);
     if (ev2) (void)hipEventDestroy(ev2);
     if (evg) (void)hipEventDestroy(evg);
+    if (evm) (void)hipEventDestroy(evm);
   }
   bool ensure_guess_buffers(std::string& err) {
     if (d_X || ngp_total == 0) return true;
@@ -1932,6 +1934,7 @@ bool batch_solve(Batch* bp, std::string& err) {
     HIP_OK(hipEventCreate(&b.ev1));
     HIP_OK(hipEventCreate(&b.ev2));
     HIP_OK(hipEventCreate(&b.evg));
+    HIP_OK(hipEventCreate(&b.evm));
   }
   HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), 0));
   if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, 12 * sizeof(unsigned long long), 0));
@@ -1939,7 +1942,8 @@ bool batch_solve(Batch* bp, std::string& err) {
   hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(256), 0, 0, b.args);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(b.evg, 0));
-  if (b.learned && !guess_infer(b.guess_model, infer_view(b), err)) return false;
+  if (b.learned && !guess_infer(b.guess_model, infer_view(b), err, b.evm)) return false;
+  if (!b.learned) HIP_OK(hipEventRecord(b.evm, 0));
   HIP_OK(hipEventRecord(b.ev1, 0));
   hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
   HIP_OK(hipGetLastError());
@@ -1956,6 +1960,10 @@ bool batch_solve(Batch* bp, std::string& err) {
   HIP_OK(hipEventElapsedTime(&ms, b.ev1, b.ev2));
   b.last_pair_ms = pms;
   b.last_guess_ms = gms;
+  float fms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&fms, b.evg, b.evm));
+  b.last_feature_ms = fms;
+  b.last_network_ms = gms - fms;
   b.last_kernel_ms = ms;
   b.solved_once = true;
   return true;
@@ -1987,6 +1995,12 @@ bool batch_guess_view(Batch* bp, BatchGuessView& v, std::string& err) {
 }
 
 double batch_guess_ms(const Batch* b) { return b ? b->last_guess_ms : 0.0; }
+
+void batch_guess_timing(const Batch* b, double& features_ms, double& network_ms, long long& directed_pairs) {
+  features_ms = b ? b->last_feature_ms : 0.0;
+  network_ms = b ? b->last_network_ms : 0.0;
+  directed_pairs = b ? b->ngp_total : 0;
+}
 
 double batch_kernel_ms(const Batch* b) { return b ? b->last_kernel_ms : 0.0; }
 double batch_pair_ms(const Batch* b) { return b ? b->last_pair_ms : 0.0; }

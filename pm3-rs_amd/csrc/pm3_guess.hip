@@ -713,11 +713,15 @@ void guess_model_destroy(GuessModel* M) { delete M; }
 
 static int fwd_blocks(int ntile) { return std::max(1, std::min((ntile + 3) / 4, 2048)); }
 
-bool guess_infer(const GuessModel* M, const BatchGuessView& v, std::string& err) {
-  if (v.ngp_total == 0) return true;
+bool guess_infer(const GuessModel* M, const BatchGuessView& v, std::string& err, hipEvent_t between) {
+  if (v.ngp_total == 0) {
+    if (between) HIP_OK(hipEventRecord(between, 0));
+    return true;
+  }
   PairGeom G{v.ax, v.ay, v.az, v.elem, v.gbeg, v.gcnt, v.gdst};
   const int fb = (int)std::min<long long>((v.ngp_total + 255) / 256, 16384);
   hipLaunchKernelGGL(feature_kernel, dim3(fb), dim3(256), 0, 0, G, v.gsrc, v.ngp_total, v.X);
+  if (between) HIP_OK(hipEventRecord(between, 0));
   const int n = (int)v.ngp_total, ntile = (n + 31) / 32;
   hipLaunchKernelGGL(mlp_fwd_kernel, dim3(fwd_blocks(ntile)), dim3(256), 0, 0, v.X, (const int*)nullptr, n, ntile,
                      (const u32x4*)M->frag, v.Y);
@@ -945,10 +949,15 @@ bool guess_train(GuessModel* M, GuessData* D, const GuessTrainOptions& o, float*
     hipLaunchKernelGGL(dw_kernel, dim3(NTILES_DW, nsplit), dim3(256), 0, 0, act, nks, M->partial);
     hipLaunchKernelGGL(grad_reduce_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, 0, (const float*)M->partial, nsplit,
                        M->grad);
+    float lr = o.lr;
+    if (o.decay_steps > 0) {
+      const double t = std::min(1.0, (double)M->step / (double)o.decay_steps);
+      lr = (float)(o.lr_final + 0.5 * (o.lr - o.lr_final) * (1.0 + std::cos(3.14159265358979323846 * t)));
+    }
     M->step += 1;
     const float c1 = 1.0f - std::pow(o.beta1, (float)M->step), c2 = 1.0f - std::pow(o.beta2, (float)M->step);
     hipLaunchKernelGGL(adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, 0, M->theta, M->m, M->v,
-                       (const float*)M->grad, o.lr, o.beta1, o.beta2, o.eps, c1, c2);
+                       (const float*)M->grad, lr, o.beta1, o.beta2, o.eps, c1, c2);
     if (!model_pack(M, err)) return false;
     HIP_OK(hipGetLastError());
   }

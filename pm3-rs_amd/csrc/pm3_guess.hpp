@@ -41,8 +41,9 @@ bool guess_model_set_params(GuessModel* m, const float* in, std::string& err);
 int guess_model_steps(const GuessModel* m);
 void guess_model_destroy(GuessModel* m);
 
-// Features and network outputs of every directed pair of a batch (into v.X / v.Y).
-bool guess_infer(const GuessModel* m, const BatchGuessView& v, std::string& err);
+// Features and network outputs of every directed pair of a batch (into v.X / v.Y); `between`, when
+// given, is recorded on the stream after the feature kernel.
+bool guess_infer(const GuessModel* m, const BatchGuessView& v, std::string& err, hipEvent_t between = nullptr);
 
 bool guess_data_create(GuessData*& out, std::string& err);
 bool guess_data_add(GuessData* d, Batch* solved, long long& added, std::string& err);
@@ -54,6 +55,8 @@ struct GuessTrainOptions {
   int batch_molecules = 1024;
   float lr = 1.0e-3f, beta1 = 0.9f, beta2 = 0.999f, eps = 1.0e-8f;
   uint64_t seed = 1;
+  float lr_final = 1.0e-5f;
+  int decay_steps = 0;  // cosine from lr to lr_final over the model's steps [0, decay_steps)
 };
 bool guess_train(GuessModel* m, GuessData* d, const GuessTrainOptions& o, float* losses, double* seconds,
                  std::string& err);

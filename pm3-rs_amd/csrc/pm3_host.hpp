@@ -72,6 +72,8 @@ void batch_destroy(Batch* b);
 double batch_kernel_ms(const Batch* b);  // device time of the last solve's SCF launch (HIP events)
 double batch_pair_ms(const Batch* b);    // device time of the last solve's pair-integral launch
 double batch_guess_ms(const Batch* b);   // device time of the last solve's learned-guess launches
+// The same split into the feature kernel and the network kernel, and the batch's directed pairs.
+void batch_guess_timing(const Batch* b, double& features_ms, double& network_ms, long long& directed_pairs);
 // Per-phase wall-clock ticks (10 ns) summed over workgroups, when PM3_PHASE_TIMERS=1 at creation:
 // 0 H_core finish, 1 Fock, 2 energy+DIIS, 3 warm-start products, 4 Jacobi, 5 sort+C V, 6 density,
 // 7 loop bookkeeping and epilogue; then [8] unused, [9] Jacobi sweeps, [10] eigensolves, [11] rounds.

@@ -96,7 +96,7 @@ class _CResult(ctypes.Structure):
 
 EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_run", "pm3_gradient",
             "pm3_run_batch", "pm3_batch_create", "pm3_batch_solve", "pm3_batch_results", "pm3_batch_destroy",
-            "pm3_batch_last_kernel_ms", "pm3_batch_gradient", "pm3_batch_last_pair_ms", "pm3_batch_last_guess_ms",
+            "pm3_batch_last_kernel_ms", "pm3_batch_gradient", "pm3_batch_last_pair_ms", "pm3_batch_last_guess_ms", "pm3_batch_guess_timing",
             "pm3_batch_phase_ticks", "pm3_stage_hcore", "pm3_stage_fock", "pm3_stage_eigen", "pm3_stage_eigen_warm",
             "pm3_stage_gemm", "pm3_last_error_message",
             "pm3_guess_model_create", "pm3_guess_model_load", "pm3_guess_model_save", "pm3_guess_model_get_params",
@@ -136,6 +136,7 @@ def native():
         L.pm3_batch_last_pair_ms.restype = ctypes.c_double
         L.pm3_batch_last_guess_ms.argtypes = [ctypes.c_void_p]
         L.pm3_batch_last_guess_ms.restype = ctypes.c_double
+        L.pm3_batch_guess_timing.argtypes = [ctypes.c_void_p, dp, dp, ctypes.POINTER(ctypes.c_int64)]
         L.pm3_batch_phase_ticks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
         L.pm3_last_error_message.restype = ctypes.c_char_p
         vp, fp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)
@@ -369,6 +370,12 @@ class Batch:
     def last_guess_ms(self):
         """Device time of the last solve's learned-guess launches (features + network)."""
         return native().pm3_batch_last_guess_ms(self._h)
+
+    def guess_timing(self):
+        """(feature kernel ms, network kernel ms, directed pairs) of the last solve."""
+        f, n, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _raise(native().pm3_batch_guess_timing(self._h, ctypes.byref(f), ctypes.byref(n), ctypes.byref(p)))
+        return f.value, n.value, p.value
 
     PHASES = ("hcore", "fock", "energy_diis", "warm_products", "jacobi", "sort_cv", "density", "other")
 

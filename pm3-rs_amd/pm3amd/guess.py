@@ -23,7 +23,7 @@ DEFAULT_MODEL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
 class _CTrainOptions(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int32), ("batch_molecules", ctypes.c_int32), ("lr", ctypes.c_float),
                 ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("lr_final", ctypes.c_float), ("decay_steps", ctypes.c_int32)]
 
 
 def _fp(a):
@@ -161,9 +161,11 @@ class GuessData:
 
 
 def train(model: GuessModel, data: GuessData, steps, batch_molecules=1024, lr=1.0e-3, seed=1, beta1=0.9,
-          beta2=0.999, eps=1.0e-8):
-    """Adam steps on the GPU; returns (loss before each step, seconds)."""
-    o = _CTrainOptions(int(steps), int(batch_molecules), float(lr), float(beta1), float(beta2), float(eps), int(seed))
+          beta2=0.999, eps=1.0e-8, lr_final=1.0e-5, decay_steps=0):
+    """Adam steps on the GPU; returns (loss before each step, seconds).  decay_steps > 0: cosine
+    step size from lr to lr_final over the model's first decay_steps optimiser steps."""
+    o = _CTrainOptions(int(steps), int(batch_molecules), float(lr), float(beta1), float(beta2), float(eps), int(seed),
+                       float(lr_final), int(decay_steps))
     losses = np.zeros(int(steps), dtype=np.float32)
     sec = ctypes.c_double()
     _raise(native().pm3_guess_train(model.handle, data.handle, ctypes.cast(ctypes.pointer(o), ctypes.c_void_p),

@@ -88,10 +88,11 @@ def main(argv=None):
     ap.add_argument("--conf-systems", type=int, default=40)
     ap.add_argument("--conf-per-system", type=int, default=512)
     ap.add_argument("--held-mixed", type=int, default=4096)
-    ap.add_argument("--steps", type=int, default=6000)
-    ap.add_argument("--segment", type=int, default=500, help="steps between held-out evaluations")
+    ap.add_argument("--steps", type=int, default=40000)
+    ap.add_argument("--segment", type=int, default=2000, help="steps between held-out evaluations")
     ap.add_argument("--batch", type=int, default=1024)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr-final", type=float, default=1e-5, help="end of the cosine step-size schedule")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--init", default="", help="start from this model file instead of random weights")
     ap.add_argument("--gain-molecules", type=int, default=2048)
@@ -110,7 +111,8 @@ def main(argv=None):
     done = 0
     while done < args.steps:
         k = min(args.segment, args.steps - done)
-        losses, sec = GS.train(model, data, k, args.batch, args.lr, seed=args.seed * 1000 + done)
+        losses, sec = GS.train(model, data, k, args.batch, args.lr, seed=args.seed * 1000 + done,
+                               lr_final=args.lr_final, decay_steps=args.steps)
         t_train += sec
         done += k
         h = GS.evaluate(model, held)
@@ -122,6 +124,7 @@ def main(argv=None):
     _log(f"gain: {json.dumps(gain)}")
     report = {"train": data.stats(), "held_out": held.stats(), "train_molecules": n_train, "data_seconds": t_data,
               "train_seconds": t_train, "steps": done, "batch_molecules": args.batch, "lr": args.lr,
+              "lr_final": args.lr_final,
               "curve": curve, "gain": gain}
     if args.report:
         with open(args.report, "w") as fh:
