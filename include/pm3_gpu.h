@@ -121,7 +121,10 @@ pm3_status pm3_run_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* 
  * counterpart: this is the batched entry the reference lacks; pm3_run_batch is
  * create + solve + results + destroy. */
 typedef struct pm3_batch pm3_batch;
-enum { PM3_BATCH_WANT_DENSITY = 1, PM3_BATCH_WANT_GRADIENT = 2 };
+/* PM3_BATCH_FORCE_LARGE solves every molecule on the multi-kernel path that molecules too large
+ * for one workgroup (more than about 190 AOs) always take: a host-driven SCF whose steps are
+ * grid-wide kernels, rocBLAS products and a rocSOLVER eigensolve. */
+enum { PM3_BATCH_WANT_DENSITY = 1, PM3_BATCH_WANT_GRADIENT = 2, PM3_BATCH_FORCE_LARGE = 4 };
 pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
                             const double* charge, const pm3_params* params, const pm3_options* opts, int32_t flags,
                             pm3_batch** out);

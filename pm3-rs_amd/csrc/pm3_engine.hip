@@ -34,6 +34,9 @@
 #include <cstdlib>
 #include <numeric>
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include "pm3_guess.hpp"
 #include "pm3_host.hpp"
 
@@ -46,6 +49,7 @@ constexpr int NW = NT / 64;   // waves per workgroup
 constexpr int MAXD = 8;       // maximum DIIS depth (src/scf.rs:309)
 constexpr int LDS_BYTES_MAX = 160 * 1024;
 constexpr int STATIC_LDS_BYTES = 4096;  // upper bound on the kernel's static __shared__ use
+constexpr int LARGE_MAX_NAO = 16384;    // largest molecule of the host-driven path (12 n^2 doubles)
 
 __constant__ TableConst c_tab;
 
@@ -89,6 +93,7 @@ struct KArgs {
   long long npairs_total;
   unsigned long long* prof;    // optional per-phase wall-clock sums (10 ns ticks), may be null
   int lds_doubles;             // dynamic LDS per workgroup, in doubles
+  int nmol_pairs;              // queue positions covered by pair_base: batched, then large molecules
   // learned guess: directed pairs (global indices, see batch_create) and network outputs
   const int8_t* elem8;
   const int* gbeg;
@@ -286,11 +291,12 @@ __device__ inline void pair_integrals(const KArgs& A, const Mol& M, int p) {
 
 // H_core diagonal blocks (U plus the attraction to every other core, summed in atom order) and the
 // total core-core energy, from the per-pair slots.  One workgroup per molecule.
-__device__ double finish_core(const KArgs& A, const Mol& M, double* red) {
-  const int tid = threadIdx.x;
+// Diagonal atom blocks of H_core: U plus the electron-core attractions summed over partners in a
+// fixed order (src/hamiltonian.rs:310-320).  Work-items t0, t0 + st, ...
+__device__ inline void core_diag(const KArgs& A, const Mol& M, int t0, int st) {
   const int np = M.np, nat = M.nat;
   const long long npairs = (long long)nat * (nat - 1) / 2;
-  for (int e = tid; e < nat * 16; e += NT) {
+  for (int e = t0; e < nat * 16; e += st) {
     const int a = e >> 4, mu = (e >> 2) & 3, nu = e & 3;
     const int ga = M.atom0 + a;
     const ElemConst& ea = c_tab.el[A.aslot[ga]];
@@ -306,6 +312,12 @@ __device__ double finish_core(const KArgs& A, const Mol& M, double* red) {
       M.H[(oa + mu) * np + oa + nu] = u + s;
     }
   }
+}
+
+__device__ double finish_core(const KArgs& A, const Mol& M, double* red) {
+  const int tid = threadIdx.x;
+  const long long npairs = (long long)M.nat * (M.nat - 1) / 2;
+  core_diag(A, M, tid, NT);
   double v[1] = {0.0};
   for (long long p = tid; p < npairs; p += NT) v[0] += M.ecc[p];
   block_sum<1>(v, red);
@@ -314,17 +326,20 @@ __device__ double finish_core(const KArgs& A, const Mol& M, double* red) {
 }
 
 // ------------------------------------------------------------------------------------------
-// F = H + G(P): RHF Fock build (src/fock.rs:285-332, :488-500).  Off-diagonal atom blocks are
-// owned by their pair (exchange K); the Coulomb contributions J to the diagonal blocks are summed
-// per atom in LDS, then each diagonal block adds its one-centre term.
-__device__ __noinline__ void build_fock(const KArgs& A, const Mol& M, const double* H, const double* P, double* F, double* lds) {
-  const int tid = threadIdx.x;
-  const int np = M.np, nat = M.nat;
-  double* J = lds;                              // [nat][10] Coulomb sums per atom
+// F = H + G(P), the RHF Fock build (src/fock.rs:285-332, :488-500), in three phases: off-diagonal
+// atom blocks owned by their pair (exchange K) with per-pair Coulomb partials; per-atom Coulomb
+// sums J; then each diagonal block adds its one-centre term and J.  The phases take a work-item
+// range so that one workgroup (batched molecules, J in LDS) or a whole grid (large molecules, one
+// launch per phase) runs the same code.
+// Phase 1 of the Fock build: off-diagonal atom blocks (exchange K, owned by their pair) and the
+// per-pair Coulomb partial sums in M.slot.  Work-items t0, t0 + st, ... (a workgroup or a grid).
+__device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H, const double* P, double* F, int t0,
+                                  int st) {
+  const int np = M.np;
   double* JX = M.slot;                          // [20][nxx]: J_a (0..9), J_b (10..19) per XX pair
   double* JH = M.slot + 20LL * M.nxx;           // [11][nxh]: J_a (0..9), J_b (10) per XH pair
   // heavy-heavy pairs
-  for (int kx = tid; kx < M.nxx; kx += NT) {
+  for (int kx = t0; kx < M.nxx; kx += st) {
     int a, b;
     tri_decode(kx, M.nh, a, b);
     const int oa = A.aoff[M.atom0 + a], ob = A.aoff[M.atom0 + b];
@@ -390,7 +405,7 @@ __device__ __noinline__ void build_fock(const KArgs& A, const Mol& M, const doub
     }
   }
   // heavy-hydrogen pairs
-  for (int kx = tid; kx < M.nxh; kx += NT) {
+  for (int kx = t0; kx < M.nxh; kx += st) {
     const int a = kx / M.nH, b = M.nh + kx % M.nH;
     const int oa = A.aoff[M.atom0 + a], ob = A.aoff[M.atom0 + b];
     const double pbb = P[ob * np + ob];
@@ -421,7 +436,7 @@ __device__ __noinline__ void build_fock(const KArgs& A, const Mol& M, const doub
     JH[10LL * M.nxh + kx] = jbv;
   }
   // hydrogen-hydrogen pairs
-  for (int kx = tid; kx < M.nhh; kx += NT) {
+  for (int kx = t0; kx < M.nhh; kx += st) {
     int a, b;
     tri_decode(kx, M.nH, a, b);
     a += M.nh;
@@ -432,9 +447,14 @@ __device__ __noinline__ void build_fock(const KArgs& A, const Mol& M, const doub
     F[oa * np + ob] = v;
     F[ob * np + oa] = v;
   }
-  __syncthreads();
-  // Coulomb sums per atom, partners in a fixed order
-  for (int e = tid; e < M.nh * 10; e += NT) {
+}
+
+// Phase 2: Coulomb sums per atom (J[nat][10]), partners in a fixed order.
+__device__ inline void fock_jsum(const KArgs& A, const Mol& M, const double* P, double* J, int t0, int st) {
+  const int np = M.np;
+  const double* JX = M.slot;
+  const double* JH = M.slot + 20LL * M.nxx;
+  for (int e = t0; e < M.nh * 10; e += st) {
     const int a = e / 10, q = e % 10;
     double s = 0.0;
     for (int b = 0; b < M.nh; ++b) {
@@ -444,7 +464,7 @@ __device__ __noinline__ void build_fock(const KArgs& A, const Mol& M, const doub
     for (int h = 0; h < M.nH; ++h) s += JH[(long long)q * M.nxh + a * M.nH + h];
     J[a * 10 + q] = s;
   }
-  for (int h = tid; h < M.nH; h += NT) {
+  for (int h = t0; h < M.nH; h += st) {
     double s = 0.0;
     for (int a = 0; a < M.nh; ++a) s += JH[10LL * M.nxh + a * M.nH + h];
     for (int g = 0; g < M.nH; ++g) {
@@ -455,9 +475,13 @@ __device__ __noinline__ void build_fock(const KArgs& A, const Mol& M, const doub
     }
     J[(M.nh + h) * 10] = s;
   }
-  __syncthreads();
-  // diagonal blocks: H + one-centre + J
-  for (int e = tid; e < nat * 16; e += NT) {
+}
+
+// Phase 3: diagonal blocks, H + one-centre terms + J.
+__device__ inline void fock_diag(const KArgs& A, const Mol& M, const double* H, const double* P, double* F,
+                                 const double* J, int t0, int st) {
+  const int np = M.np;
+  for (int e = t0; e < M.nat * 16; e += st) {
     const int a = e >> 4, mu = (e >> 2) & 3, nu = e & 3;
     const int ga = M.atom0 + a;
     const ElemConst& ea = c_tab.el[A.aslot[ga]];
@@ -474,6 +498,16 @@ __device__ __noinline__ void build_fock(const KArgs& A, const Mol& M, const doub
       F[(o + mu) * np + o + nu] = H[(o + mu) * np + o + nu] + acc + J[a * 10 + pk(mu, nu)];
     }
   }
+}
+
+// F = H + G(P): RHF Fock build (src/fock.rs:285-332, :488-500) by one workgroup, J in LDS.
+__device__ __noinline__ void build_fock(const KArgs& A, const Mol& M, const double* H, const double* P, double* F, double* lds) {
+  const int tid = threadIdx.x;
+  fock_pairs(A, M, H, P, F, tid, NT);
+  __syncthreads();
+  fock_jsum(A, M, P, lds, tid, NT);
+  __syncthreads();
+  fock_diag(A, M, H, P, F, lds, tid, NT);
   __syncthreads();
 }
 
@@ -687,8 +721,8 @@ struct Diis {
 };
 
 // Pulay CDIIS coefficients from the (scale-normalised) bordered Gram system with a pivot guard
-// (src/scf.rs:1544-1630).  Single thread.
-__device__ __noinline__ bool cdiis_coeffs(const Diis& d, double* c) {
+// (src/scf.rs:1544-1630).  Single thread (device) or host.
+__host__ __device__ __noinline__ bool cdiis_coeffs(const Diis& d, double* c) {
   const int n = d.len;
   if (n < 2) return false;
   double scale = 0.0;
@@ -734,8 +768,8 @@ __device__ __noinline__ bool cdiis_coeffs(const Diis& d, double* c) {
 }
 
 // A-DIIS (Hu & Yang 2010): projected gradient on the probability simplex, 400 steps at most
-// (src/scf.rs:1637-1693).  Single thread.
-__device__ __noinline__ void adiis_coeffs(const Diis& D, double* c) {
+// (src/scf.rs:1637-1693).  Single thread (device) or host.
+__host__ __device__ __noinline__ void adiis_coeffs(const Diis& D, double* c) {
   const int n = D.len, last = n - 1;
   const double mnn = D.mm[last][last];
   double d[MAXD], s[MAXD][MAXD];
@@ -1197,7 +1231,7 @@ __device__ inline Bufs slab(const KArgs& A, double*& Wbase) {
   return B;
 }
 
-__device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double* Wbase) {
+__host__ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double* Wbase) {
   Mol M;
   M.atom0 = md.atom0;
   M.nat = md.nat;
@@ -1237,7 +1271,7 @@ __device__ inline void zero_mats(const Mol& M, const Bufs& B) {
 __global__ __launch_bounds__(256) void pair_kernel(KArgs A) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < A.npairs_total; g += stride) {
-    int lo = 0, hi = A.nmol;  // largest queue position with pair_base <= g
+    int lo = 0, hi = A.nmol_pairs;  // largest queue position with pair_base <= g
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
       if (A.pair_base[mid] <= g) lo = mid; else hi = mid;
@@ -1348,7 +1382,7 @@ __device__ inline void pair_gradient(const KArgs& A, const Mol& M, const double*
 __global__ __launch_bounds__(256) void grad_pair_kernel(KArgs A) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < A.npairs_total; g += stride) {
-    int lo = 0, hi = A.nmol;
+    int lo = 0, hi = A.nmol_pairs;
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
       if (A.pair_base[mid] <= g) lo = mid; else hi = mid;
@@ -1380,6 +1414,37 @@ __global__ __launch_bounds__(256) void grad_gather_kernel(KArgs A) {
     }
     A.out_g[3LL * (md.atom0 + x) + c] = s;
   }
+}
+
+// Mulliken charges (into out_q) and the dipole about the centre of mass, atomic units
+// (src/scf.rs:762-791, src/dipole.rs:64-140).  One workgroup; P has leading dimension M.np.
+__device__ inline void charges_dipole(const KArgs& A, const Mol& M, const double* P, double* red, double (&dip)[3]) {
+  const int tid = threadIdx.x, np = M.np;
+  double com[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int a = tid; a < M.nat; a += NT) {
+    const int ga = M.atom0 + a;
+    const double ms = c_tab.el[A.aslot[ga]].mass;
+    com[0] += ms * A.ax[ga]; com[1] += ms * A.ay[ga]; com[2] += ms * A.az[ga]; com[3] += ms;
+  }
+  block_sum<4>(com, red);
+  const double cx = com[3] > 0.0 ? com[0] / com[3] : 0.0, cy = com[3] > 0.0 ? com[1] / com[3] : 0.0,
+               cz = com[3] > 0.0 ? com[2] / com[3] : 0.0;
+  dip[0] = dip[1] = dip[2] = 0.0;
+  for (int a = tid; a < M.nat; a += NT) {
+    const int ga = M.atom0 + a;
+    const ElemConst& e = c_tab.el[A.aslot[ga]];
+    const int o = A.aoff[ga];
+    double pop = 0.0;
+    for (int mu = 0; mu < e.norb; ++mu) pop += P[(o + mu) * np + o + mu];
+    A.out_q[ga] = e.tore - pop;
+    const double disp[3] = {A.ax[ga] - cx, A.ay[ga] - cy, A.az[ga] - cz};
+    for (int ax = 0; ax < 3; ++ax) {
+      double el = pop * disp[ax];
+      if (e.norb == 4) el += (P[o * np + o + ax + 1] + P[(o + ax + 1) * np + o]) * e.dd;
+      dip[ax] += disp[ax] * e.tore - el;
+    }
+  }
+  block_sum<3>(dip, red);
 }
 
 __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
@@ -1456,33 +1521,8 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
       for (int j = tid; j < n; j += NT) B.eps[j] = B.epsb[j];
       __syncthreads();
     }
-    // Mulliken charges and the dipole about the centre of mass (src/scf.rs:762-791,
-    // src/dipole.rs:64-140).
-    double com[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int a = tid; a < M.nat; a += NT) {
-      const int ga = M.atom0 + a;
-      const double ms = c_tab.el[A.aslot[ga]].mass;
-      com[0] += ms * A.ax[ga]; com[1] += ms * A.ay[ga]; com[2] += ms * A.az[ga]; com[3] += ms;
-    }
-    block_sum<4>(com, red);
-    const double cx = com[3] > 0.0 ? com[0] / com[3] : 0.0, cy = com[3] > 0.0 ? com[1] / com[3] : 0.0,
-                 cz = com[3] > 0.0 ? com[2] / com[3] : 0.0;
-    double dip[3] = {0.0, 0.0, 0.0};
-    for (int a = tid; a < M.nat; a += NT) {
-      const int ga = M.atom0 + a;
-      const ElemConst& e = c_tab.el[A.aslot[ga]];
-      const int o = A.aoff[ga];
-      double pop = 0.0;
-      for (int mu = 0; mu < e.norb; ++mu) pop += B.P[(o + mu) * np + o + mu];
-      A.out_q[ga] = e.tore - pop;
-      const double disp[3] = {A.ax[ga] - cx, A.ay[ga] - cy, A.az[ga] - cz};
-      for (int ax = 0; ax < 3; ++ax) {
-        double el = pop * disp[ax];
-        if (e.norb == 4) el += (B.P[o * np + o + ax + 1] + B.P[(o + ax + 1) * np + o]) * e.dd;
-        dip[ax] += disp[ax] * e.tore - el;
-      }
-    }
-    block_sum<3>(dip, red);
+    double dip[3];
+    charges_dipole(A, M, B.P, red, dip);
     if (tid == 0) {
       MolOut o;
       o.dip[0] = dip[0]; o.dip[1] = dip[1]; o.dip[2] = dip[2];
@@ -1572,6 +1612,179 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
   }
 }
 
+
+// ------------------------------------------------------------------------------------------
+// Large molecules (more AOs than one workgroup's LDS holds, e.g. 512 waters = 3072 AOs): the SCF
+// loop runs on the host and every step is a grid-wide kernel over the one molecule -- the same
+// integral, Fock-phase and H_core code as the batched kernel -- while the dense products go to
+// rocBLAS and the eigensolve to rocSOLVER.  Matrices keep the molecule's leading dimension np.
+constexpr int LG_BLOCKS = 1024;
+constexpr int LG_THREADS = 256;
+constexpr int LG_MAXDOT = 3 * MAXD + 2;
+
+struct Dots {
+  const double* x[LG_MAXDOT];
+  const double* y[LG_MAXDOT];
+  int k;
+};
+
+__device__ inline int gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ inline int gstride() { return gridDim.x * blockDim.x; }
+
+// Block partials of K dot products over m elements (red[block * LG_MAXDOT + k]); the host adds
+// the blocks in order, so the sums are reproducible.
+__global__ __launch_bounds__(LG_THREADS) void lg_dots_kernel(Dots D, long long m, double* red) {
+  __shared__ double part[LG_MAXDOT][LG_THREADS / 64];
+  double v[LG_MAXDOT];
+#pragma unroll
+  for (int k = 0; k < LG_MAXDOT; ++k) v[k] = 0.0;
+  for (long long e = gtid(); e < m; e += gstride()) {
+#pragma unroll
+    for (int k = 0; k < LG_MAXDOT; ++k)
+      if (k < D.k) v[k] += D.x[k][e] * D.y[k][e];
+  }
+  const int w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < LG_MAXDOT; ++k) {
+    if (k >= D.k) break;
+    const double t = wave_sum(v[k]);
+    if ((threadIdx.x & 63) == 0) part[k][w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < D.k) {
+    double t = 0.0;
+    for (int q = 0; q < LG_THREADS / 64; ++q) t += part[threadIdx.x][q];
+    red[(long long)blockIdx.x * LG_MAXDOT + threadIdx.x] = t;
+  }
+}
+
+// Block partials of sum (X - Y)^2 (red[block]).
+__global__ __launch_bounds__(LG_THREADS) void lg_diff2_kernel(const double* X, const double* Y, long long m, double* red) {
+  __shared__ double part[LG_THREADS / 64];
+  double v = 0.0;
+  for (long long e = gtid(); e < m; e += gstride()) {
+    const double d = X[e] - Y[e];
+    v += d * d;
+  }
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < LG_THREADS / 64; ++q) t += part[q];
+    red[blockIdx.x] = t;
+  }
+}
+
+// Diagonal blocks of H_core and block partials of the core-core energy (red[block]).
+__global__ __launch_bounds__(LG_THREADS) void lg_core_kernel(KArgs A, MolDesc md, double* red) {
+  __shared__ double part[LG_THREADS / 64];
+  const Mol M = mol_of(A, md, nullptr);
+  core_diag(A, M, gtid(), gstride());
+  const long long npairs = (long long)M.nat * (M.nat - 1) / 2;
+  double v = 0.0;
+  for (long long p = gtid(); p < npairs; p += gstride()) v += M.ecc[p];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < LG_THREADS / 64; ++q) t += part[q];
+    red[blockIdx.x] = t;
+  }
+}
+
+// Superposition density on a zeroed P; `tilt_atom` >= 0 moves min(0.2, P_zz) from p_z to p_x on
+// that atom (the symmetry-broken guess, src/scf.rs:894-916).
+__global__ __launch_bounds__(LG_THREADS) void lg_sad_kernel(KArgs A, MolDesc md, double* P, int tilt_atom) {
+  const int np = md.np;
+  for (int a = gtid(); a < md.nat; a += gstride()) {
+    const int ga = md.atom0 + a;
+    const ElemConst& e = c_tab.el[A.aslot[ga]];
+    const int o = A.aoff[ga];
+    P[(long long)o * np + o] = e.ios;
+    if (e.norb == 4) {
+      double pp[3] = {e.iop / 3.0, e.iop / 3.0, e.iop / 3.0};
+      if (a == tilt_atom) {
+        const double t = fmin(0.2, pp[2]);
+        pp[2] -= t;
+        pp[0] += t;
+      }
+      for (int k = 1; k < 4; ++k) P[(long long)(o + k) * np + o + k] = pp[k - 1];
+    }
+  }
+}
+
+__global__ __launch_bounds__(LG_THREADS) void lg_fock_pairs_kernel(KArgs A, MolDesc md, const double* P, double* F,
+                                                                   double* slot) {
+  const Mol M = mol_of(A, md, slot);
+  fock_pairs(A, M, M.H, P, F, gtid(), gstride());
+}
+
+__global__ __launch_bounds__(LG_THREADS) void lg_fock_jsum_kernel(KArgs A, MolDesc md, const double* P, double* J,
+                                                                  double* slot) {
+  const Mol M = mol_of(A, md, slot);
+  fock_jsum(A, M, P, J, gtid(), gstride());
+}
+
+__global__ __launch_bounds__(LG_THREADS) void lg_fock_diag_kernel(KArgs A, MolDesc md, const double* P, double* F,
+                                                                  const double* J) {
+  const Mol M = mol_of(A, md, nullptr);
+  fock_diag(A, M, M.H, P, F, J, gtid(), gstride());
+}
+
+// DIIS entry of one iteration: E = T - T^T with T = F P (commutator, src/scf.rs:1039-1054), and
+// copies of F and (for A-DIIS) P into the history slots.
+__global__ __launch_bounds__(LG_THREADS) void lg_commutator_kernel(const double* T, const double* F, const double* P,
+                                                                   double* En, double* Fn, double* Dn, int n, int np,
+                                                                   int keep_d) {
+  const long long m = (long long)np * np;
+  for (long long e = gtid(); e < m; e += gstride()) {
+    const int i = (int)(e / np), j = (int)(e % np);
+    En[e] = (i < n && j < n) ? T[(long long)i * np + j] - T[(long long)j * np + i] : 0.0;
+    Fn[e] = F[e];
+    if (keep_d) Dn[e] = P[e];
+  }
+}
+
+struct Combine {
+  double c[MAXD];
+  int slot[MAXD];
+  int len;
+};
+
+__global__ __launch_bounds__(LG_THREADS) void lg_combine_kernel(const double* Fh, long long ms, Combine C, double* Fu) {
+  for (long long e = gtid(); e < ms; e += gstride()) {
+    double s = 0.0;
+    for (int i = 0; i < C.len; ++i) s += C.c[i] * Fh[(long long)C.slot[i] * ms + e];
+    Fu[e] = s;
+  }
+}
+
+// Row-major n x n eigenvector matrix (columns = eigenvectors) from the column-major solver output.
+__global__ __launch_bounds__(LG_THREADS) void lg_vectors_kernel(const double* V, double* C, int n, int np) {
+  const long long m = (long long)n * n;
+  for (long long e = gtid(); e < m; e += gstride()) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    C[e] = V[(long long)j * np + i];
+  }
+}
+
+// n x n dense copy out of a leading-dimension-np matrix.
+__global__ __launch_bounds__(LG_THREADS) void lg_pack_kernel(const double* X, double* out, int n, int np) {
+  const long long m = (long long)n * n;
+  for (long long e = gtid(); e < m; e += gstride()) out[e] = X[(e / n) * np + e % n];
+}
+
+__global__ __launch_bounds__(NT) void lg_epilogue_kernel(KArgs A, MolDesc md, const double* P, double* dip_out) {
+  __shared__ double red[NW * 4];
+  const Mol M = mol_of(A, md, nullptr);
+  double dip[3];
+  charges_dipole(A, M, P, red, dip);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) dip_out[k] = dip[k];
+}
+
 #define HIP_OK(x)                                                  \
   do {                                                             \
     hipError_t e_ = (x);                                           \
@@ -1586,16 +1799,32 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
 namespace pm3host {
 
 // Device-resident batch: inputs uploaded and scratch allocated once; solve() is the launch only.
+struct LargeBufs {
+  double *F = nullptr, *P = nullptr, *Pn = nullptr, *Fu = nullptr, *T = nullptr, *V = nullptr, *w = nullptr;
+  double *ework = nullptr, *Pb = nullptr, *Vb = nullptr, *wb = nullptr, *Fh = nullptr, *Eh = nullptr, *Dh = nullptr;
+  double *J = nullptr, *slot = nullptr, *red = nullptr, *dip = nullptr;
+  int* info = nullptr;
+  long long m2 = 0;  // elements per matrix (np_max^2)
+  int depth = 0;     // DIIS history slots allocated
+};
+
 struct Batch {
   int nmol = 0, nsolve = 0, nwg = 0, lds_need = 0;
   std::vector<MolDesc> desc;
   std::vector<int> solve;
+  std::vector<int> large;      // molecules solved by the host-driven multi-kernel path
+  std::vector<int> order_all;  // queue order of the pair kernels: solve (largest first), then large
+  LargeBufs lg;
+  rocblas_handle blas = nullptr;
+  EngineOptions opt{};
+  TableConst table{};
+  std::vector<double> h_red;
   std::vector<std::vector<int>> ao_map, atom_map;  // sorted -> caller indices
   std::vector<int> pre_status;
   bool want_gradient = false;
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
-  void* bufs[40] = {nullptr};
+  void* bufs[64] = {nullptr};
   int nbufs = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   double last_kernel_ms = 0.0, last_pair_ms = 0.0, last_guess_ms = 0.0;
@@ -1617,6 +1846,7 @@ struct Batch {
     if (ev2) (void)hipEventDestroy(ev2);
     if (evg) (void)hipEventDestroy(evg);
     if (evm) (void)hipEventDestroy(evm);
+    if (blas) (void)rocblas_destroy_handle(blas);
   }
   bool ensure_guess_buffers(std::string& err) {
     if (d_X || ngp_total == 0) return true;
@@ -1649,6 +1879,338 @@ static bool upload_table(const Params& P, std::string& err) {
   if (uploaded_device != dev) {
     HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &P.table, sizeof(TableConst)));
     uploaded_device = dev;
+  }
+  return true;
+}
+
+
+// ------------------------------------------------------------------------------------------
+// Host-driven SCF for large molecules (see the lg_* kernels).  The loop mirrors rhf_loop step for
+// step: Fock build, energy, DIIS entry and coefficients (the same solver code, on the host),
+// eigensolve, aufbau density, convergence test, then the stability search of scf_batch_kernel.
+#define BLAS_OK(x)                                                    \
+  do {                                                                \
+    rocblas_status s_ = (x);                                          \
+    if (s_ != rocblas_status_success) {                               \
+      err = std::string(#x) + ": " + rocblas_status_to_string(s_);    \
+      return false;                                                   \
+    }                                                                 \
+  } while (0)
+
+static int diis_depth_for(int n, int keep_d, int scf_memory_mb) {
+  const long long n2 = (long long)n * n * 8, per = (keep_d ? 3 : 2) * n2;
+  const long long budget = (long long)scf_memory_mb * 1024 * 1024;
+  if (scf_memory_mb == 0 || per == 0) return MAXD;
+  return (int)std::max(2LL, std::min((long long)MAXD, budget / per));
+}
+
+static bool large_setup(Batch& b, std::string& err) {
+  int np = 16, nat = 1, depth = 2;
+  long long slot = 1;
+  for (int k : b.large) {
+    const MolDesc& d = b.desc[k];
+    np = std::max(np, d.np);
+    nat = std::max(nat, d.nat);
+    const long long nh = d.nheavy, nH = d.nat - d.nheavy;
+    slot = std::max(slot, 20 * (nh * (nh - 1) / 2) + 11 * nh * nH);
+    const int keep_d = b.opt.accel == 2 &&
+                       (b.opt.scf_memory_mb == 0 || 2LL * 3 * d.nao * d.nao * 8 <= (long long)b.opt.scf_memory_mb * 1024 * 1024);
+    depth = std::max(depth, diis_depth_for(d.nao, keep_d ? 1 : 0, b.opt.scf_memory_mb));
+  }
+  LargeBufs& L = b.lg;
+  L.m2 = (long long)np * np;
+  L.depth = depth;
+  if (!b.alloc(L.F, L.m2, err) || !b.alloc(L.P, L.m2, err) || !b.alloc(L.Pn, L.m2, err) || !b.alloc(L.Fu, L.m2, err) ||
+      !b.alloc(L.T, L.m2, err) || !b.alloc(L.V, L.m2, err) || !b.alloc(L.w, np, err) || !b.alloc(L.ework, np, err) ||
+      !b.alloc(L.Pb, L.m2, err) || !b.alloc(L.Vb, L.m2, err) || !b.alloc(L.wb, np, err) ||
+      !b.alloc(L.Fh, L.m2 * depth, err) || !b.alloc(L.Eh, L.m2 * depth, err) || !b.alloc(L.Dh, L.m2 * depth, err) ||
+      !b.alloc(L.J, 10LL * nat, err) || !b.alloc(L.slot, slot, err) ||
+      !b.alloc(L.red, (size_t)LG_BLOCKS * LG_MAXDOT, err) || !b.alloc(L.dip, 3, err) || !b.alloc(L.info, 1, err))
+    return false;
+  b.h_red.resize((size_t)LG_BLOCKS * LG_MAXDOT);
+  BLAS_OK(rocblas_create_handle(&b.blas));
+  BLAS_OK(rocblas_set_stream(b.blas, 0));
+  return true;
+}
+
+// Sum of the first k columns of the per-block partials, blocks in order.
+static bool large_reduce(Batch& b, int k, double* out, std::string& err) {
+  HIP_OK(hipMemcpy(b.h_red.data(), b.lg.red, sizeof(double) * (size_t)LG_BLOCKS * LG_MAXDOT, hipMemcpyDeviceToHost));
+  for (int q = 0; q < k; ++q) out[q] = 0.0;
+  for (int blk = 0; blk < LG_BLOCKS; ++blk)
+    for (int q = 0; q < k; ++q) out[q] += b.h_red[(size_t)blk * LG_MAXDOT + q];
+  return true;
+}
+
+static bool large_dots(Batch& b, const Dots& D, long long m, double* out, std::string& err) {
+  hipLaunchKernelGGL(lg_dots_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, D, m, b.lg.red);
+  HIP_OK(hipGetLastError());
+  return large_reduce(b, D.k, out, err);
+}
+
+// Block partials written as red[block] (one column): sum them.
+static bool large_sum1(Batch& b, double& out, std::string& err) {
+  std::vector<double> h(LG_BLOCKS);
+  HIP_OK(hipMemcpy(h.data(), b.lg.red, sizeof(double) * LG_BLOCKS, hipMemcpyDeviceToHost));
+  out = 0.0;
+  for (double v : h) out += v;
+  return true;
+}
+
+static bool large_fock(Batch& b, const MolDesc& d, const double* P, double* F, std::string& err) {
+  hipLaunchKernelGGL(lg_fock_pairs_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, F, b.lg.slot);
+  hipLaunchKernelGGL(lg_fock_jsum_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, b.lg.J, b.lg.slot);
+  hipLaunchKernelGGL(lg_fock_diag_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, F,
+                     (const double*)b.lg.J);
+  HIP_OK(hipGetLastError());
+  return true;
+}
+
+static bool large_energy(Batch& b, const MolDesc& d, const double* P, const double* F, double& e, std::string& err) {
+  const Mol M = mol_of(b.args, d, nullptr);
+  Dots D{};
+  D.k = 2;
+  D.x[0] = P; D.y[0] = M.H;
+  D.x[1] = P; D.y[1] = F;
+  double v[2];
+  if (!large_dots(b, D, (long long)d.np * d.np, v, err)) return false;
+  e = 0.5 * (v[0] + v[1]);
+  return true;
+}
+
+// Eigenpairs of the symmetric matrix A (np leading dimension) into L.V (column-major, ascending
+// eigenvalues in L.w), then P = 2 C_occ C_occ^T into Pdst.
+static bool large_eigen_density(Batch& b, const MolDesc& d, const double* A, double* Pdst, std::string& err) {
+  LargeBufs& L = b.lg;
+  const int n = d.nao, np = d.np;
+  HIP_OK(hipMemcpyAsync(L.V, A, sizeof(double) * (size_t)np * np, hipMemcpyDeviceToDevice, 0));
+  BLAS_OK(rocsolver_dsyevd(b.blas, rocblas_evect_original, rocblas_fill_upper, n, L.V, np, L.w, L.ework, L.info));
+  int info = 0;
+  HIP_OK(hipMemcpy(&info, L.info, sizeof(int), hipMemcpyDeviceToHost));
+  if (info != 0) {
+    err = "rocsolver_dsyevd did not converge (info " + std::to_string(info) + ")";
+    return false;
+  }
+  const double two = 2.0, zero = 0.0;
+  BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_transpose, n, n, d.nocc, &two, L.V, np, L.V,
+                        np, &zero, Pdst, np));
+  return true;
+}
+
+struct LargePath {
+  double e_elec = 0.0, last_dp = 1.0e300;
+  int iters = 0, converged = 0;
+};
+
+static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePath& R, std::string& err) {
+  LargeBufs& L = b.lg;
+  const int n = d.nao, np = d.np;
+  const long long ms = (long long)np * np;
+  const Mol M = mol_of(b.args, d, nullptr);
+  const EngineOptions& o = b.opt;
+  HIP_OK(hipMemsetAsync(L.P, 0, sizeof(double) * ms, 0));
+  HIP_OK(hipMemsetAsync(L.Pn, 0, sizeof(double) * ms, 0));
+  if (guess == 2) {
+    if (!large_eigen_density(b, d, M.H, L.P, err)) return false;
+  } else {
+    int tilt = -1;
+    if (guess == 3) {
+      double best = 0.0;
+      for (int a = 0; a < d.nat; ++a) {
+        int slot = 0;
+        HIP_OK(hipMemcpy(&slot, b.args.aslot + d.atom0 + a, sizeof(int), hipMemcpyDeviceToHost));
+        const ElemConst& e = b.table.el[slot];
+        if (e.norb == 4 && e.iop > best) { best = e.iop; tilt = a; }
+      }
+    }
+    hipLaunchKernelGGL(lg_sad_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, L.P, tilt);
+    HIP_OK(hipGetLastError());
+  }
+  int keep_d = accel == 2;
+  if (keep_d && !(o.scf_memory_mb == 0 || 2LL * 3 * n * n * 8 <= (long long)o.scf_memory_mb * 1024 * 1024)) keep_d = 0;
+  if (accel == 2 && !keep_d) accel = 1;
+  Diis D{};
+  D.len = 0;
+  D.depth = std::min(diis_depth_for(n, keep_d, o.scf_memory_mb), L.depth);
+  D.keep_d = keep_d;
+  R = LargePath();
+  double e_old = 0.0;
+  double* P = L.P;
+  double* Pn = L.Pn;
+  for (int it = 0; it < o.max_scf; ++it) {
+    R.iters = it + 1;
+    if (!large_fock(b, d, P, L.F, err)) return false;
+    double e_elec = 0.0;
+    if (!large_energy(b, d, P, L.F, e_elec, err)) return false;
+    const double* Fuse = L.F;
+    if (accel != 0) {
+      if (D.len == D.depth) {
+        const int s0 = D.slot[0];
+        for (int i = 0; i + 1 < D.len; ++i) {
+          D.slot[i] = D.slot[i + 1];
+          for (int j = 0; j + 1 < D.len; ++j) {
+            D.b[i][j] = D.b[i + 1][j + 1];
+            D.mm[i][j] = D.mm[i + 1][j + 1];
+          }
+        }
+        D.len -= 1;
+        D.newslot = s0;
+      } else {
+        D.newslot = D.len;
+      }
+      const int ns = D.newslot;
+      double* Fn = L.Fh + ns * ms;
+      double* En = L.Eh + ns * ms;
+      double* Dn = L.Dh + ns * ms;
+      const double one = 1.0, zero = 0.0;
+      // T = F P (row-major; rocBLAS is column-major and both factors are symmetric)
+      BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, P, np, L.F, np, &zero,
+                            L.T, np));
+      hipLaunchKernelGGL(lg_commutator_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.T,
+                         (const double*)L.F, (const double*)P, En, Fn, Dn, n, np, keep_d);
+      HIP_OK(hipGetLastError());
+      const int len = D.len;
+      Dots G{};
+      int k = 0;
+      int ie[MAXD], idf[MAXD], ifd[MAXD];
+      for (int i = 0; i < len; ++i) {
+        const long long so = (long long)D.slot[i] * ms;
+        ie[i] = k; G.x[k] = L.Eh + so; G.y[k] = En; ++k;
+        if (keep_d) {
+          idf[i] = k; G.x[k] = Dn; G.y[k] = L.Fh + so; ++k;
+          ifd[i] = k; G.x[k] = L.Dh + so; G.y[k] = Fn; ++k;
+        }
+      }
+      const int iee = k;
+      G.x[k] = En; G.y[k] = En; ++k;
+      const int idd = k;
+      if (keep_d) { G.x[k] = Dn; G.y[k] = Fn; ++k; }
+      G.k = k;
+      double v[LG_MAXDOT];
+      if (!large_dots(b, G, ms, v, err)) return false;
+      for (int i = 0; i < len; ++i) {
+        D.b[i][len] = v[ie[i]];
+        D.b[len][i] = v[ie[i]];
+        D.mm[len][i] = keep_d ? v[idf[i]] : 0.0;
+        D.mm[i][len] = keep_d ? v[ifd[i]] : 0.0;
+      }
+      D.b[len][len] = v[iee];
+      D.mm[len][len] = keep_d ? v[idd] : 0.0;
+      D.slot[len] = ns;
+      D.len = len + 1;
+      const double en = std::sqrt(v[iee]);
+      bool ok;
+      if (accel == 2 && en > o.adiis_switch) {
+        ok = D.len >= 2 && D.keep_d;
+        if (ok) adiis_coeffs(D, D.coef);
+      } else {
+        ok = cdiis_coeffs(D, D.coef);
+      }
+      if (ok) {
+        Combine C{};
+        C.len = D.len;
+        for (int i = 0; i < D.len; ++i) {
+          C.c[i] = D.coef[i];
+          C.slot[i] = D.slot[i];
+        }
+        hipLaunchKernelGGL(lg_combine_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.Fh, ms, C, L.Fu);
+        HIP_OK(hipGetLastError());
+        Fuse = L.Fu;
+      } else {
+        Fuse = Fn;
+      }
+    }
+    if (!large_eigen_density(b, d, Fuse, Pn, err)) return false;
+    hipLaunchKernelGGL(lg_diff2_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)Pn, (const double*)P, ms,
+                       L.red);
+    double s2 = 0.0;
+    if (!large_sum1(b, s2, err)) return false;
+    const double dp = std::sqrt(s2 / (double)std::max(n * n, 1));
+    const double de = std::fabs(e_elec - e_old);
+    R.last_dp = dp;
+    std::swap(P, Pn);
+    e_old = e_elec;
+    if (it > 0 && de < o.e_tol && dp < o.p_tol) {
+      R.converged = 1;
+      break;
+    }
+  }
+  if (P != L.P) {  // keep the final density in L.P
+    HIP_OK(hipMemcpyAsync(L.P, P, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
+  }
+  if (!large_fock(b, d, L.P, L.F, err)) return false;
+  return large_energy(b, d, L.P, L.F, R.e_elec, err);
+}
+
+static bool large_solve(Batch& b, int k, std::string& err) {
+  const MolDesc& d = b.desc[k];
+  LargeBufs& L = b.lg;
+  const int n = d.nao, np = d.np, nocc = d.nocc;
+  const long long ms = (long long)np * np;
+  HIP_OK(hipMemsetAsync(L.F, 0, sizeof(double) * ms, 0));
+  hipLaunchKernelGGL(lg_core_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, L.red);
+  HIP_OK(hipGetLastError());
+  double ecore = 0.0;
+  if (!large_sum1(b, ecore, err)) return false;
+  const int guess = b.opt.guess == 4 ? 1 : b.opt.guess;  // the learned guess covers batched molecules
+  LargePath best;
+  if (!large_rhf(b, d, guess, b.opt.accel, best, err)) return false;
+  std::vector<double> w(n);
+  HIP_OK(hipMemcpy(w.data(), L.w, sizeof(double) * n, hipMemcpyDeviceToHost));
+  const double first_e = best.e_elec;
+  int paths = 1;
+  const double gap = (nocc > 0 && nocc < n) ? w[nocc] - w[nocc - 1] : 1.0e300;
+  const bool search = b.opt.stability == 2 || (b.opt.stability == 1 && gap < 6.0);
+  if (search) {
+    HIP_OK(hipMemcpyAsync(L.Pb, L.P, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
+    HIP_OK(hipMemcpyAsync(L.Vb, L.V, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
+    HIP_OK(hipMemcpyAsync(L.wb, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+    const int primary = guess == 2 ? 2 : 1;
+    const int other = primary == 2 ? 1 : 2;
+    const int rg[5][2] = {{primary, 0}, {other, b.opt.accel}, {other, 0}, {3, b.opt.accel}, {3, 0}};
+    for (int r = 0; r < 5; ++r) {
+      ++paths;
+      LargePath c;
+      if (!large_rhf(b, d, rg[r][0], rg[r][1], c, err)) return false;
+      if (!c.converged) continue;
+      if (!best.converged || c.e_elec < best.e_elec - 1.0e-6) {
+        best = c;
+        HIP_OK(hipMemcpyAsync(L.Pb, L.P, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
+        HIP_OK(hipMemcpyAsync(L.Vb, L.V, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
+        HIP_OK(hipMemcpyAsync(L.wb, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+      }
+    }
+    paths += 2;  // the two unrestricted rungs of the reference ladder are not run here
+    HIP_OK(hipMemcpyAsync(L.P, L.Pb, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
+    HIP_OK(hipMemcpyAsync(L.V, L.Vb, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
+    HIP_OK(hipMemcpyAsync(L.w, L.wb, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+    HIP_OK(hipMemcpy(w.data(), L.w, sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
+  hipLaunchKernelGGL(lg_epilogue_kernel, dim3(1), dim3(NT), 0, 0, b.args, d, (const double*)L.P, L.dip);
+  HIP_OK(hipGetLastError());
+  double dip[3];
+  HIP_OK(hipMemcpy(dip, L.dip, sizeof(dip), hipMemcpyDeviceToHost));
+  MolOut mo{};
+  mo.e_elec = best.e_elec;
+  mo.e_core = ecore;
+  mo.last_dp = best.last_dp;
+  mo.first_e = first_e;
+  mo.homo = nocc > 0 ? w[nocc - 1] : 0.0;
+  mo.lumo = nocc < n ? w[nocc] : 0.0;
+  for (int x = 0; x < 3; ++x) mo.dip[x] = dip[x];
+  mo.iters = best.iters;
+  mo.converged = best.converged;
+  mo.paths = paths;
+  mo.status = best.converged ? 0 : 3;
+  mo.guess_used = 0;
+  HIP_OK(hipMemcpy(b.args.out + k, &mo, sizeof(MolOut), hipMemcpyHostToDevice));
+  if (d.out_mat >= 0) {
+    hipLaunchKernelGGL(lg_pack_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.P,
+                       b.args.out_P + d.out_mat, n, np);
+    hipLaunchKernelGGL(lg_vectors_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.V,
+                       b.args.out_C + d.out_mat, n, np);
+    HIP_OK(hipMemcpyAsync(b.args.out_eps + d.out_vec, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+    HIP_OK(hipGetLastError());
   }
   return true;
 }
@@ -1724,25 +2286,28 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
     // eigensolver LDS: packed matrix + rotation schedule, plus the eigenvectors when they fit
     int jl = jac_lds_with_v(nao);
     if (jl * 8 + STATIC_LDS_BYTES > LDS_BYTES_MAX) jl = jac_lds_base(nao);
-    const int lneed = std::max(jl, m.n_atoms * 16) * 8;
-    if (lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || d.nocc > nao) {
+    const long long lneed = (long long)std::max(jl, m.n_atoms * 16) * 8;
+    // molecules that do not fit one workgroup's LDS take the host-driven multi-kernel path
+    const bool large = lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || req.force_large;
+    if (d.nocc > nao || (large && nao > LARGE_MAX_NAO)) {
       st = d.nocc > nao ? 1 : 5;
       aslot.resize(d.atom0); aoff.resize(d.atom0); ax.resize(d.atom0); ay.resize(d.atom0); az.resize(d.atom0);
       d.nat = 0;
       continue;
     }
-    b.lds_need = std::max(b.lds_need, lneed);
-    np_max = std::max(np_max, d.np);
-    const long long nH = d.nat - nh;
-    const long long npairs = (long long)d.nat * (d.nat - 1) / 2;
-    w_cap = std::max(w_cap, 20LL * npairs);  // Fock-build pair slots per workgroup
+    if (!large) {
+      b.lds_need = std::max(b.lds_need, (int)lneed);
+      np_max = std::max(np_max, d.np);
+      const long long npairs = (long long)d.nat * (d.nat - 1) / 2;
+      w_cap = std::max(w_cap, 20LL * npairs);  // Fock-build pair slots per workgroup
+    }
     if (req.want_density || req.want_orbitals || req.want_gradient) {
       d.out_mat = b.mat_total;
       d.out_vec = b.vec_total;
       b.mat_total += (long long)nao * nao;
       b.vec_total += nao;
     }
-    b.solve.push_back(k);
+    (large ? b.large : b.solve).push_back(k);
   }
   b.atoms_total = (long long)aslot.size();
   b.nsolve = (int)b.solve.size();
@@ -1793,12 +2358,17 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   }
   // largest molecules first so the queue ends on short work
   std::stable_sort(b.solve.begin(), b.solve.end(), [&](int x, int y) { return b.desc[x].nao > b.desc[y].nao; });
+  b.order_all = b.solve;
+  b.order_all.insert(b.order_all.end(), b.large.begin(), b.large.end());
+  const int nq = (int)b.order_all.size();
+  b.opt = opt;
+  b.table = P.table;
   // molecule store: H_core (np x np) then W_xx | W_xh | W_hh | S1 | S2 | e_cc per molecule, and the
   // first global pair index of every queue position for the pair kernel
-  std::vector<long long> pair_base(b.nsolve + 1, 0);
+  std::vector<long long> pair_base(nq + 1, 0);
   long long store_total = 0;
-  for (int q = 0; q < b.nsolve; ++q) {
-    MolDesc& d = b.desc[b.solve[q]];
+  for (int q = 0; q < nq; ++q) {
+    MolDesc& d = b.desc[b.order_all[q]];
     const long long nh = d.nheavy, nH = d.nat - d.nheavy, npairs = (long long)d.nat * (d.nat - 1) / 2;
     d.h_off = store_total;
     store_total += ((long long)d.np * d.np + 7) & ~7LL;
@@ -1807,12 +2377,12 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
     pair_base[q + 1] = pair_base[q] + npairs;
   }
   out_batch = bp;
-  if (b.nsolve == 0) return true;
+  if (nq == 0) return true;
   int dev = 0;
   hipDeviceProp_t prop;
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipGetDeviceProperties(&prop, dev));
-  b.nwg = std::max(1, std::min(b.nsolve, prop.multiProcessorCount));
+  b.nwg = std::max(1, std::min(std::max(b.nsolve, 1), prop.multiProcessorCount));
   const long long m2 = (long long)np_max * np_max;
   const long long stride = ((11 + 3 * MAXD) * m2 + w_cap + 4LL * np_max + 64 + 31) & ~31LL;
   KArgs& a = b.args;
@@ -1820,7 +2390,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   double *d_ax, *d_ay, *d_az, *d_scr, *d_P, *d_C, *d_eps, *d_q, *d_store, *d_g;
   long long* d_pbase;
   MolOut* d_out;
-  if (!b.alloc(d_desc, nmol, err) || !b.alloc(d_order, b.nsolve, err) || !b.alloc(d_queue, 1, err) ||
+  if (!b.alloc(d_desc, nmol, err) || !b.alloc(d_order, nq, err) || !b.alloc(d_queue, 1, err) ||
       !b.alloc(d_aslot, b.atoms_total, err) || !b.alloc(d_aoff, b.atoms_total, err) || !b.alloc(d_ax, b.atoms_total, err) ||
       !b.alloc(d_ay, b.atoms_total, err) || !b.alloc(d_az, b.atoms_total, err) || !b.alloc(d_scr, (size_t)(stride * b.nwg), err) ||
       !b.alloc(d_out, nmol, err) || !b.alloc(d_P, b.mat_total, err) || !b.alloc(d_C, b.mat_total, err) ||
@@ -1846,7 +2416,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   HIP_OK(hipMemset(d_store, 0, sizeof(double) * std::max<long long>(store_total, 1)));
   HIP_OK(hipMemcpy(d_pbase, pair_base.data(), sizeof(long long) * pair_base.size(), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_desc, b.desc.data(), sizeof(MolDesc) * nmol, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_order, b.solve.data(), sizeof(int) * b.nsolve, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(d_order, b.order_all.data(), sizeof(int) * nq, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_aslot, aslot.data(), sizeof(int) * aslot.size(), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_aoff, aoff.data(), sizeof(int) * aoff.size(), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_ax, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
@@ -1855,6 +2425,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   a.mols = d_desc;
   a.order = d_order;
   a.nmol = b.nsolve;
+  a.nmol_pairs = nq;
   a.queue = d_queue;
   a.aslot = d_aslot;
   a.ax = d_ax;
@@ -1909,6 +2480,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   a.stability = opt.stability;
   a.scf_memory_mb = opt.scf_memory_mb;
   HIP_OK(hipFuncSetAttribute((const void*)scf_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_need));
+  if (!b.large.empty() && !large_setup(b, err)) return false;
   return true;
 }
 
@@ -1928,7 +2500,7 @@ static BatchGuessView infer_view(Batch& b) {
 bool batch_solve(Batch* bp, std::string& err) {
   Batch& b = *bp;
   b.last_kernel_ms = 0.0;
-  if (b.nsolve == 0) return true;
+  if (b.order_all.empty()) return true;
   if (!b.ev0) {
     HIP_OK(hipEventCreate(&b.ev0));
     HIP_OK(hipEventCreate(&b.ev1));
@@ -1945,11 +2517,13 @@ bool batch_solve(Batch* bp, std::string& err) {
   if (b.learned && !guess_infer(b.guess_model, infer_view(b), err, b.evm)) return false;
   if (!b.learned) HIP_OK(hipEventRecord(b.evm, 0));
   HIP_OK(hipEventRecord(b.ev1, 0));
-  hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
+  if (b.nsolve > 0) hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
   HIP_OK(hipGetLastError());
+  for (int k : b.large)
+    if (!large_solve(b, k, err)) return false;
   if (b.want_gradient) {
     hipLaunchKernelGGL(grad_pair_kernel, dim3(b.pair_blocks), dim3(256), 0, 0, b.args);
-    hipLaunchKernelGGL(grad_gather_kernel, dim3(b.nsolve), dim3(256), 0, 0, b.args);
+    hipLaunchKernelGGL(grad_gather_kernel, dim3((unsigned)b.order_all.size()), dim3(256), 0, 0, b.args);
     HIP_OK(hipGetLastError());
   }
   HIP_OK(hipEventRecord(b.ev2, 0));
@@ -1977,12 +2551,12 @@ bool batch_guess_view(Batch* bp, BatchGuessView& v, std::string& err) {
   v.atom0.assign(b.nmol, 0); v.nat.assign(b.nmol, 0); v.nao.assign(b.nmol, 0); v.ngp.assign(b.nmol, 0);
   v.eligible.assign(b.nmol, 0); v.solved.assign(b.nmol, 0); v.gp0.assign(b.nmol, 0); v.out_mat.assign(b.nmol, -1);
   std::vector<MolOut> ho;
-  if (b.solved_once && b.nsolve > 0) {
+  if (b.solved_once && !b.order_all.empty()) {
     ho.resize(b.nmol);
     HIP_OK(hipMemcpy(ho.data(), b.args.out, sizeof(MolOut) * b.nmol, hipMemcpyDeviceToHost));
   }
   v.caller_atom.assign(b.atoms_total, 0);
-  for (int k : b.solve) {
+  for (int k : b.order_all) {
     const MolDesc& d = b.desc[k];
     for (int t = 0; t < d.nat; ++t) v.caller_atom[d.atom0 + t] = b.atom_map[k][t];
     v.atom0[k] = d.atom0; v.nat[k] = d.nat; v.nao[k] = d.nao; v.ngp[k] = d.ngp; v.gp0[k] = d.gp0;
@@ -2016,7 +2590,7 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
   Batch& b = *bp;
   out.assign(b.nmol, MolResult());
   for (int k = 0; k < b.nmol; ++k) out[k].status = b.pre_status[k];
-  if (b.nsolve == 0) return true;
+  if (b.order_all.empty()) return true;
   std::vector<MolOut> ho(b.nmol);
   std::vector<double> hq(b.atoms_total), hg;
   if (b.want_gradient) {
@@ -2034,7 +2608,7 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
     HIP_OK(hipMemcpy(hC.data(), b.args.out_C, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy(hE.data(), b.args.out_eps, sizeof(double) * b.vec_total, hipMemcpyDeviceToHost));
   }
-  for (int k : b.solve) {
+  for (int k : b.order_all) {
     MolResult& r = out[k];
     const MolOut& o = ho[k];
     const MolDesc& d = b.desc[k];
@@ -2105,7 +2679,7 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
     return false;
   }
   if (b->nsolve == 0) {
-    status = b->pre_status[0];
+    status = b->pre_status[0] ? b->pre_status[0] : 5;  // stage hooks run on one workgroup only
     batch_destroy(b);
     return true;
   }

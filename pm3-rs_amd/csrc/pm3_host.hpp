@@ -59,6 +59,7 @@ struct RunRequest {
   bool want_density = false;
   bool want_orbitals = false;
   bool want_gradient = false;
+  bool force_large = false;  // solve every molecule on the host-driven multi-kernel path
 };
 
 // A batch whose inputs and scratch live on the device; batch_solve is the launch alone.

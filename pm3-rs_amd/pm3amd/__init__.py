@@ -336,7 +336,7 @@ class Batch:
     """Device-resident batch of independent molecules (pm3_batch_* in include/pm3_gpu.h)."""
 
     def __init__(self, molecules: List[Molecule], params: Optional[Pm3Parameters] = None,
-                 options: Optional[Pm3Options] = None, want_density=False, want_gradient=False):
+                 options: Optional[Pm3Options] = None, want_density=False, want_gradient=False, force_large=False):
         self.params = params or Pm3Parameters.standard()
         self.options = options or Pm3Options()
         self.molecules = molecules
@@ -351,7 +351,8 @@ class Batch:
         ip = ctypes.POINTER(ctypes.c_int32)
         st = native().pm3_batch_create(len(molecules), ptr.ctypes.data_as(ip), self._z.ctypes.data_as(ip),
                                        _dp(self._xyz), _dp(self._q), self.params._h, ctypes.byref(self.options._c()),
-                                       (1 if want_density else 0) | (2 if want_gradient else 0), ctypes.byref(h))
+                                       (1 if want_density else 0) | (2 if want_gradient else 0) |
+                                       (4 if force_large else 0), ctypes.byref(h))
         _raise(st)
         self._h = h
         self.want_density = want_density

@@ -1,0 +1,89 @@
+"""The host-driven multi-kernel path for molecules too large for one workgroup (grid-wide integral,
+Fock and reduction kernels, rocBLAS products, rocSOLVER eigensolve): the same molecule solved on
+both paths agrees, a cluster beyond the batched kernel's size limit agrees with the CPU oracle,
+and the 512-water cluster (3072 AOs) satisfies the size-independent invariants."""
+
+import numpy as np
+import pytest
+
+from pm3amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+HOF_TOL = 1e-5
+TIGHT = dict(p_tol=1e-10, e_tol=1e-12, max_scf=500)
+
+
+@pytest.fixture(scope="module")
+def G():
+    import pm3amd
+    pm3amd.native()
+    return pm3amd
+
+
+def _mols(count, seed):
+    return [(np.asarray(z, np.int32), W.to_bohr(x)) for z, x in W.mixed_batch(count, seed)]
+
+
+def test_forced_large_path_matches_batched(G):
+    mols = _mols(6, 99)
+    ms = [G.Molecule(z, x) for z, x in mols]
+    for opts in (G.Pm3Options(**TIGHT), G.Pm3Options(p_tol=1e-8, e_tol=1e-8)):
+        a = G.Batch(ms, options=opts, want_density=True, want_gradient=True)
+        b = G.Batch(ms, options=opts, want_density=True, want_gradient=True, force_large=True)
+        a.solve()
+        b.solve()
+        sa, ra = a.results()
+        sb, rb = b.results()
+        ga, gb = a.gradients(), b.gradients()
+        assert np.all(sa == 0) and np.all(sb == 0)
+        for x, y, g1, g2 in zip(ra, rb, ga, gb):
+            print("nao", x.density.shape[0], "iterations", x.iterations, y.iterations, flush=True)
+            assert abs(x.heat_of_formation_kcal - y.heat_of_formation_kcal) <= HOF_TOL
+            assert np.abs(x.density - y.density).max() <= (1e-8 if opts.p_tol < 1e-9 else 1e-6)
+            assert np.abs(x.charges - y.charges).max() <= 1e-6
+            assert np.abs(x.dipole_debye - y.dipole_debye).max() <= 1e-5
+            assert abs(x.homo_ev - y.homo_ev) <= 1e-6 and abs(x.lumo_ev - y.lumo_ev) <= 1e-6
+            assert np.abs(g1 - g2).max() <= 1e-6
+            if opts.p_tol >= 1e-9:
+                assert abs(x.iterations - y.iterations) <= 1
+
+
+def test_cluster_beyond_workgroup_limit_matches_oracle(G, oracle):
+    z, x = W.water_cluster(70, W.water_box_for(70), 70)
+    z = np.asarray(z, np.int32)
+    xb = W.to_bohr(x)
+    mol = G.Molecule(z, xb)
+    print("atoms", len(z), flush=True)
+    r = G.run_pm3(mol, options=G.Pm3Options(**TIGHT))
+    assert r.density.shape[0] == 420  # beyond the one-workgroup limit of about 190 AOs
+    st, o = oracle.run(z, xb, 0.0, oracle.options(**TIGHT))
+    assert st == 0 and r.converged
+    assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL
+    assert np.abs(r.density - o["density"]).max() <= 1e-8
+    assert np.abs(r.charges - o["charges"]).max() <= 1e-8
+    assert abs(r.homo_ev - o["homo_ev"]) <= 1e-7
+    n = r.mo_coeff.shape[0]
+    C = r.mo_coeff
+    assert np.abs(C.T @ C - np.eye(n)).max() < 1e-10
+    assert np.abs(2 * C[:, :r.n_occ] @ C[:, :r.n_occ].T - r.density).max() < 1e-10
+
+
+def test_water512_invariants(G):
+    z, x = W.water_cluster(512, 25.0, 2026)
+    z = np.asarray(z, np.int32)
+    xb = W.to_bohr(x)
+    mol = G.Molecule(z, xb)
+    g = G.closed_form_gradient(mol, options=G.Pm3Options(p_tol=1e-8, e_tol=1e-8))
+    r = g.scf
+    n = r.density.shape[0]
+    print("nao", n, "iterations", r.iterations, "HoF", r.heat_of_formation_kcal, flush=True)
+    assert n == 3072 and r.converged and r.n_occ == 2048
+    P = r.density
+    assert np.abs(P - P.T).max() < 1e-12
+    # idempotency of the closed-shell density (P/2)^2 = P/2 and the electron count
+    assert np.abs((P / 2) @ (P / 2) - P / 2).max() < 1e-9
+    assert abs(np.trace(P) - 4096.0) < 1e-8
+    # translational invariance of the analytic gradient
+    assert np.abs(g.gradient.sum(axis=0)).max() < 1e-8
+    assert abs(r.total_ev - (r.electronic_ev + r.core_ev)) < 1e-9
