@@ -16,6 +16,8 @@ Workloads (``--workload``):
                  solves/sec metric is quoted on)
   water32        one 32-water cluster (96 atoms, 192 AOs)
   conformers2048 2048 jittered conformers (sigma 0.1 A) of C12H26 + 20 waters (98 atoms, 194 AOs)
+  water512       one 512-water cluster in a 25 A cube (1536 atoms, 3072 AOs): SCF plus analytic
+                 gradient on the multi-kernel large-molecule path; a step is one solve
 
 The CPU baseline is the C++ restatement of the reference CPU path (oracle/) on a bounded sample of
 the same workload, one solve per host thread.  The same sample checks |dHf| <= 1e-5 kcal/mol.
@@ -64,6 +66,49 @@ def algorithmic_flops(naos, results):
     return tot
 
 
+def bench_large(args):
+    """One large molecule (SCF + analytic gradient per step) on the host-driven multi-kernel path."""
+    z, x = W.water_cluster(512, 25.0, args.seed)
+    mol = G.Molecule(np.asarray(z, np.int32), W.to_bohr(x))
+    opts = G.Pm3Options(p_tol=1.0e-8, e_tol=1.0e-8)
+    batch = G.Batch([mol], options=opts, want_gradient=True, want_density=True)
+    for _ in range(args.warmup):
+        batch.solve()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.solve()
+        kms.append(batch.last_kernel_ms())
+    t1 = time.perf_counter()
+    st, res = batch.results()
+    r = res[0]
+    g = batch.gradients()[0]
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    P = r.density
+    n = P.shape[0]
+    idem = float(np.abs((P / 2) @ (P / 2) - P / 2).max())
+    out = {
+        "metric": "converged PM3 SCF + analytic gradient solves/sec (one 512-water cluster, 3072 AOs)",
+        "value": (1.0 if st[0] == 0 else 0.0) / (ms_per_step * 1e-3),
+        "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded geometry)",
+        "config": {"workload": "water512", "molecules": 1, "atoms": len(z), "nao": n, "seed": args.seed,
+                   "p_tol": 1e-8, "guess": "sad", "gradient": True,
+                   "parallelism": "grid-wide kernels per SCF step, rocBLAS dgemm, rocSOLVER dsyevd"},
+        "iterations": r.iterations, "converged": int(st[0] == 0), "heat_of_formation_kcal": r.heat_of_formation_kcal,
+        "max_abs_gradient_ev_per_bohr": float(np.abs(g).max()),
+        "checks": {"idempotency_max_abs": idem, "trace_minus_electrons": float(np.trace(P) - 2 * r.n_occ),
+                   "gradient_sum_max_abs": float(np.abs(g.sum(axis=0)).max())},
+        "device_ms_per_step": float(np.mean(kms)),
+        "roofline": None,
+        "cpu_baseline": None,
+        "cpu_baseline_note": "the CPU restatement's dense eigensolver is not bounded at 3072 AOs "
+                             "(one eigensolve takes minutes); the CPU leg is measured on the batched workloads",
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +129,9 @@ def main():
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is described in DESIGN.md "
                                    "and not launched", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.workload == "water512":
+        return bench_large(args)
 
     if args.copy_calibration_mb > 0:
         import torch

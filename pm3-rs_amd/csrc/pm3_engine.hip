@@ -50,6 +50,11 @@ constexpr int MAXD = 8;       // maximum DIIS depth (src/scf.rs:309)
 constexpr int LDS_BYTES_MAX = 160 * 1024;
 constexpr int STATIC_LDS_BYTES = 4096;  // upper bound on the kernel's static __shared__ use
 constexpr int LARGE_MAX_NAO = 16384;    // largest molecule of the host-driven path (12 n^2 doubles)
+// Early SCF iterations need eigenvectors only to a fraction of the current DIIS error: Jacobi
+// stops at max(1e-14 |A|, EIG_TOL_FACTOR * |F P - P F|), which reaches full precision as the
+// SCF converges, so the converged state is unchanged.
+constexpr double EIG_TOL_FACTOR = 1.0e-2;
+constexpr double EIG_REL = 1.0e-2;  // cap of the loose tolerance relative to the starting off-diagonal
 
 __constant__ TableConst c_tab;
 
@@ -94,6 +99,8 @@ struct KArgs {
   unsigned long long* prof;    // optional per-phase wall-clock sums (10 ns ticks), may be null
   int lds_doubles;             // dynamic LDS per workgroup, in doubles
   int nmol_pairs;              // queue positions covered by pair_base: batched, then large molecules
+  double eig_tol_factor;       // eigensolver stops at max(1e-14 |A|, min(factor * |[F,P]|, eig_rel * off0))
+  double eig_rel;
   // learned guess: directed pairs (global indices, see batch_create) and network outputs
   const int8_t* elem8;
   const int* gbeg;
@@ -224,6 +231,7 @@ struct Mol {
   double* ecc;   // [npairs] core-core energy per pair
   double* slot;  // per-pair Coulomb partial sums of the Fock build (workgroup scratch)
   int lds_doubles;  // dynamic LDS of the launch, in doubles
+  double eig_rel;   // loose-tolerance cap relative to the starting off-diagonal
 };
 
 // ------------------------------------------------------------------------------------------
@@ -542,7 +550,7 @@ constexpr int JB = 10;  // upper bound on the triangle blocks one thread owns (m
 // V (eigenvectors, column-major, ld ldv) is in LDS at g_lds + vbase when VL, else at Vg.
 template <bool VL>
 __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg, int ldv, int vbase, double* lam,
-                                   double* red, int max_sweeps) {
+                                   double* red, int max_sweeps, double tol_abs, double rel) {
   const int tid = threadIdx.x;
   const int nn = n + (n & 1), m = nn >> 1;
   const int npk = n * (n + 1) / 2;
@@ -575,7 +583,8 @@ __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg,
   double dmax = 0.0;
   for (int i = tid; i < n; i += NT) dmax = fmax(dmax, fabs(g_lds[i * (i + 1) / 2 + i]));
   const double scale = fmax(block_max(dmax, red), 1.0);
-  const double tol = 1.0e-14 * scale;
+  const double tol_floor = 1.0e-14 * scale;
+  double tol = tol_floor;
   // eigenvector update mapping: row i = tid % n, slots k = tid / n, tid / n + rstep, ...
   const int rstep = NT / max(n, 1);
   const int vi = tid % max(n, 1), vk0 = tid / max(n, 1);
@@ -597,7 +606,11 @@ __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg,
           if (++i > j) { i = 0; ++j; }
         }
       }
-      if (block_max(off, red) <= tol) break;
+      const double offmax = block_max(off, red);
+      // a loose tolerance never exceeds a fixed fraction of the starting off-diagonal, so every
+      // call that has anything to rotate does rotate (no stalled, falsely converged SCF)
+      if (sweep == 0) tol = fmax(tol_floor, fmin(tol_abs, rel * offmax));
+      if (offmax <= tol) break;
     }
     for (int r = 0; r < nn - 1; ++r) {
       for (int k = tid; k < m; k += NT) {
@@ -713,6 +726,7 @@ __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg,
 // DIIS history (src/scf.rs:1065-1173) kept in LDS as logical slots oldest -> newest.
 struct Diis {
   int len, depth, keep_d, newslot;
+  double en;  // |E| of the newest entry
   int slot[MAXD];
   double b[MAXD][MAXD];
   double mm[MAXD][MAXD];
@@ -833,8 +847,10 @@ __device__ inline void stamp(const Bufs& B, int k) {
 }
 
 // Eigenvectors of Fu -> Cn (row-major, columns sorted by ascending eigenvalue) and eps.
-// warm != 0: Jacobi on C^T Fu C and Cn = C V; otherwise Jacobi on Fu itself.
-__device__ __noinline__ void eigensolve(const Mol& M, const Bufs& B, const double* Fu, bool warm, double* lds, double* red) {
+// warm != 0: Jacobi on C^T Fu C and Cn = C V; otherwise Jacobi on Fu itself.  Sweeps stop once
+// the largest off-diagonal element is below max(1e-14 |A|, tol_abs).
+__device__ __noinline__ void eigensolve(const Mol& M, const Bufs& B, const double* Fu, bool warm, double* lds, double* red,
+                                        double tol_abs = 0.0) {
   const int n = M.n, np = M.np, tid = threadIdx.x;
   const double* Amat = Fu;
   if (warm) {
@@ -854,8 +870,8 @@ __device__ __noinline__ void eigensolve(const Mol& M, const Bufs& B, const doubl
   const bool v_in_lds = jac_lds_with_v(n) <= M.lds_doubles;
   double* V = v_in_lds ? lds + jac_lds_base(n) : B.V;
   const int ldv = v_in_lds ? n : np;
-  const int nsw = v_in_lds ? jacobi<true>(Amat, n, np, nullptr, ldv, jac_lds_base(n), B.lam, red, 60)
-                           : jacobi<false>(Amat, n, np, B.V, ldv, 0, B.lam, red, 60);
+  const int nsw = v_in_lds ? jacobi<true>(Amat, n, np, nullptr, ldv, jac_lds_base(n), B.lam, red, 60, tol_abs, M.eig_rel)
+                           : jacobi<false>(Amat, n, np, B.V, ldv, 0, B.lam, red, 60, tol_abs, M.eig_rel);
   if (B.ph && threadIdx.x == 0) {
     B.ph[9] += (unsigned long long)nsw;
     B.ph[10] += 1ull;
@@ -1001,7 +1017,7 @@ __device__ __noinline__ int initial_guess(const KArgs& A, const Mol& M, const Bu
     learned_into(A, M, B.P);
     const double cl = commutator2(A, M, B, lds, red);  // leaves F(P_learned) in B.F
     if (cl <= cs) {
-      eigensolve(M, B, B.F, false, lds, red);
+      eigensolve(M, B, B.F, false, lds, red, A.eig_tol_factor * sqrt(cl));
       density(M, B.Cn, B.P);
       for (int e = tid; e < np * np; e += NT) B.C[e] = B.Cn[e];
       __syncthreads();
@@ -1068,6 +1084,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
   PathResult R{0.0, 1.0e300, 0, 0, gused};
   double e_old = 0.0;
   bool have_c = gused == 1;
+  double eig_tol = 0.0;  // per-iteration eigensolver tolerance (DIIS error norm scaled)
   for (int it = 0; it < A.max_scf; ++it) {
     R.iters = it + 1;
     stamp(B, 7);
@@ -1148,6 +1165,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
         D.slot[len] = ns;
         D.len = len + 1;
         const double en = sqrt(v[3 * MAXD]);
+        D.en = en;
         bool ok;
         if (accel == 2 && en > A.adiis_switch) {
           ok = D.len >= 2 && D.keep_d;
@@ -1172,7 +1190,8 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       }
     }
     stamp(B, 2);
-    eigensolve(M, B, Fuse, have_c, lds, red);
+    eig_tol = accel != 0 ? A.eig_tol_factor * D.en : 0.0;
+    eigensolve(M, B, Fuse, have_c, lds, red, eig_tol);
     density(M, B.Cn, B.Pn);
     double dv[1] = {0.0};
     for (int e = tid; e < np * np; e += NT) {
@@ -1256,6 +1275,7 @@ __host__ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double*
   M.ecc = M.S2 + 10 * npairs;
   M.slot = Wbase;
   M.lds_doubles = A.lds_doubles;
+  M.eig_rel = A.eig_rel;
   return M;
 }
 
@@ -2479,6 +2499,10 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   a.guess = opt.guess;
   a.stability = opt.stability;
   a.scf_memory_mb = opt.scf_memory_mb;
+  a.eig_tol_factor = EIG_TOL_FACTOR;
+  if (const char* ev = std::getenv("PM3_EIG_TOL_FACTOR")) a.eig_tol_factor = std::atof(ev);
+  a.eig_rel = EIG_REL;
+  if (const char* ev = std::getenv("PM3_EIG_REL")) a.eig_rel = std::atof(ev);
   HIP_OK(hipFuncSetAttribute((const void*)scf_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_need));
   if (!b.large.empty() && !large_setup(b, err)) return false;
   return true;

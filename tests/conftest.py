@@ -30,6 +30,13 @@ def atoms_to_zx(atoms):
     return z, x
 
 
+@pytest.fixture
+def exact_eigensolves(monkeypatch):
+    """Full-precision eigensolves in every SCF iteration (the engine's default loosens early
+    iterations), for tests that follow the reference's iteration trajectory step for step."""
+    monkeypatch.setenv("PM3_EIG_TOL_FACTOR", "0")
+
+
 @pytest.fixture(scope="session")
 def oracle():
     import subprocess

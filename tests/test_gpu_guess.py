@@ -182,7 +182,7 @@ def _model(G):
     return m, False
 
 
-def test_learned_scf_keeps_converged_answers(G, oracle):
+def test_learned_scf_keeps_converged_answers(G, oracle, exact_eigensolves):
     model, _ = _model(G)
     mols = _mols(48, 88)
     tight = dict(p_tol=1e-10, e_tol=1e-12, max_scf=500)

@@ -25,7 +25,7 @@ def _mols(count, seed):
     return [(np.asarray(z, np.int32), W.to_bohr(x)) for z, x in W.mixed_batch(count, seed)]
 
 
-def test_forced_large_path_matches_batched(G):
+def test_forced_large_path_matches_batched(G, exact_eigensolves):
     mols = _mols(6, 99)
     ms = [G.Molecule(z, x) for z, x in mols]
     for opts in (G.Pm3Options(**TIGHT), G.Pm3Options(p_tol=1e-8, e_tol=1e-8)):

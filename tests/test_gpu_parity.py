@@ -31,7 +31,9 @@ PARITY_ROWS = ["water", "methane", "ethane", "benzene", "formaldehyde", "acetic_
                "aluminium_trichloride", "dimethyl_sulfoxide", "ketene", "butyne"]
 
 
-def test_single_point_parity(G, pins, oracle):
+def test_single_point_parity(G, pins, oracle, exact_eigensolves):
+    """Full-precision eigensolves: the SCF follows the restated reference path step for step,
+    including which stability rungs run."""
     for row in _rows(pins, PARITY_ROWS):
         z, x = atoms_to_zx(row["atoms"])
         print("case", row["name"], flush=True)
@@ -49,7 +51,21 @@ def test_single_point_parity(G, pins, oracle):
         assert abs(r.heat_of_formation_kcal - row["hof_kcal"]) < 2e-3
 
 
-def test_default_options_follow_the_oracle_path(G, oracle):
+def test_default_engine_settings_reach_the_same_answers(G, pins, oracle):
+    """The engine's default loosens the eigensolves of early iterations; trajectories (iteration
+    counts, stability rungs) may differ, the converged answers do not."""
+    for row in _rows(pins, PARITY_ROWS):
+        z, x = atoms_to_zx(row["atoms"])
+        print("case", row["name"], flush=True)
+        r = G.run_pm3(G.Molecule(z, x, row["charge"]), options=G.Pm3Options(max_scf=2000, **TIGHT))
+        st, o = oracle.run(z, x, row["charge"], oracle.options(max_scf=2000, **TIGHT))
+        assert st == 0 and r.converged
+        assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL, row["name"]
+        assert np.abs(r.density - o["density"]).max() <= P_TOL, row["name"]
+        assert abs(r.homo_ev - o["homo_ev"]) <= 1e-7 and abs(r.lumo_ev - o["lumo_ev"]) <= 1e-7
+
+
+def test_default_options_follow_the_oracle_path(G, oracle, exact_eigensolves):
     # default p_tol 1e-7: same iteration count and HoF as the restatement
     z, x = atoms_to_zx([["C", 0, 0, 0], ["O", 0, 0, 1.21], ["H", 0.94, 0, -0.54], ["H", -0.94, 0, -0.54]])
     r = G.run_pm3(G.Molecule(z, x))
