@@ -540,6 +540,13 @@ __device__ __host__ inline int jac_lds_base(int n) {
   return n * (n + 1) / 2 + 4 * m;
 }
 __device__ __host__ inline int jac_lds_with_v(int n) { return jac_lds_base(n) + n * n; }
+// Eigenvectors in global memory with deferred updates: the rotations of JR rounds are buffered
+// (c, s, p | q << 16) and applied to V in chunks of JCH rows staged through LDS.
+constexpr int JR = 16, JCH = 32;  // JCH is a multiple of the 8 waves of a workgroup
+__device__ __host__ inline int jac_lds_deferred(int n) {
+  const int m = (n + (n & 1)) >> 1;
+  return jac_lds_base(n) + 3 * m * JR + JCH * n;
+}
 
 // Dynamic LDS of the SCF kernels.  The eigensolver indexes it directly (not through a pointer
 // parameter) so every access compiles to ds_read / ds_write rather than flat operations.
@@ -550,10 +557,59 @@ constexpr int JB = 10;  // upper bound on the triangle blocks one thread owns (m
 // V (eigenvectors, column-major, ld ldv) is in LDS at g_lds + vbase when VL, else at Vg.
 template <bool VL>
 __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg, int ldv, int vbase, double* lam,
-                                   double* red, int max_sweeps, double tol_abs, double rel) {
+                                   double* red, int max_sweeps, double tol_abs, double rel, bool defer) {
   const int tid = threadIdx.x;
   const int nn = n + (n & 1), m = nn >> 1;
   const int npk = n * (n + 1) / 2;
+  // deferred eigenvector updates (VL == false): ring of JR rounds after the schedule, then the
+  // staging chunk
+  const bool dv = !VL && defer;
+  double* ring_c = g_lds + jac_lds_base(n);
+  double* ring_s = ring_c + JR * m;
+  int* ring_pq = (int*)(ring_s + JR * m);
+  double* chunk = ring_c + 3 * m * JR;
+  int nbuf = 0;
+  auto flush = [&](int nb) {
+    for (int r0 = 0; r0 < n; r0 += JCH) {
+      const int rows = min(JCH, n - r0);
+      for (int e = tid; e < rows * n; e += NT) {
+        const int i = e % rows, k = e / rows;
+        chunk[i * n + k] = Vg[k * ldv + r0 + i];
+      }
+      __syncthreads();
+      // wave w owns rows w, w + NW, ...; each lane applies its rotations of a round to all of the
+      // wave's rows (independent chains), then the wave moves to the next round
+      const int lane = tid & 63, w = tid >> 6;
+      for (int j = 0; j < nb; ++j) {
+        for (int kk = lane; kk < m; kk += 64) {
+          const double sv = ring_s[j * m + kk];
+          const int pqv = ring_pq[j * m + kk];
+          const int p = pqv & 0xffff, q = pqv >> 16;
+          if (sv == 0.0 || p >= n || q >= n) continue;
+          const double cv = ring_c[j * m + kk];
+#pragma unroll
+          for (int u = 0; u < JCH / NW; ++u) {
+            const int row = w + u * NW;
+            if (row < rows) {
+              double* v = chunk + row * n;
+              const double a = v[p], b = v[q];
+              v[p] = cv * a - sv * b;
+              v[q] = sv * a + cv * b;
+            }
+          }
+        }
+        // the next round may pair columns this round's other lanes wrote
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      __syncthreads();
+      for (int e = tid; e < rows * n; e += NT) {
+        const int i = e % rows, k = e / rows;
+        Vg[k * ldv + r0 + i] = chunk[i * n + k];
+      }
+      __syncthreads();
+    }
+  };
   // g_lds: [0, npk) packed A | [npk, npk+3m) rotation (c, s, t) | [npk+3m, npk+4m) pair schedule
   const int cso = npk, pqo = npk + 3 * m;
   int* pq = (int*)(g_lds + pqo);
@@ -632,6 +688,11 @@ __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg,
         g_lds[cso + 3 * k + 2] = t;
         pq[2 * k] = p;
         pq[2 * k + 1] = q;
+        if (dv) {
+          ring_c[nbuf * m + k] = c;
+          ring_s[nbuf * m + k] = s;
+          ring_pq[nbuf * m + k] = p | (q << 16);
+        }
       }
       __syncthreads();
       // two-sided update of every block: J_K^T A_KL J_L (the diagonal blocks become diagonal),
@@ -686,7 +747,7 @@ __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg,
         }
       }
       // V <- V J: thread owns row vi, slots vk0, vk0 + rstep, ...
-      if (vk0 < rstep) {
+      if (vk0 < rstep && !dv) {
         for (int k0 = vk0; k0 < m; k0 += 8 * rstep) {
           double vp[8], vq[8], c8[8], s8[8];
           int op[8], oq[8];
@@ -715,8 +776,13 @@ __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg,
         }
       }
       __syncthreads();
+      if (dv && ++nbuf == JR) {
+        flush(nbuf);
+        nbuf = 0;
+      }
     }
   }
+  if (dv && nbuf > 0) flush(nbuf);
   for (int i = tid; i < n; i += NT) lam[i] = g_lds[i * (i + 1) / 2 + i];
   __syncthreads();
   return sweep;
@@ -870,8 +936,9 @@ __device__ __noinline__ void eigensolve(const Mol& M, const Bufs& B, const doubl
   const bool v_in_lds = jac_lds_with_v(n) <= M.lds_doubles;
   double* V = v_in_lds ? lds + jac_lds_base(n) : B.V;
   const int ldv = v_in_lds ? n : np;
-  const int nsw = v_in_lds ? jacobi<true>(Amat, n, np, nullptr, ldv, jac_lds_base(n), B.lam, red, 60, tol_abs, M.eig_rel)
-                           : jacobi<false>(Amat, n, np, B.V, ldv, 0, B.lam, red, 60, tol_abs, M.eig_rel);
+  const int nsw = v_in_lds ? jacobi<true>(Amat, n, np, nullptr, ldv, jac_lds_base(n), B.lam, red, 60, tol_abs, M.eig_rel, false)
+                           : jacobi<false>(Amat, n, np, B.V, ldv, 0, B.lam, red, 60, tol_abs, M.eig_rel,
+                                           jac_lds_deferred(n) <= M.lds_doubles);
   if (B.ph && threadIdx.x == 0) {
     B.ph[9] += (unsigned long long)nsw;
     B.ph[10] += 1ull;
@@ -2305,6 +2372,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
     d.np = (nao + 15) & ~15;
     // eigensolver LDS: packed matrix + rotation schedule, plus the eigenvectors when they fit
     int jl = jac_lds_with_v(nao);
+    if (jl * 8 + STATIC_LDS_BYTES > LDS_BYTES_MAX) jl = jac_lds_deferred(nao);
     if (jl * 8 + STATIC_LDS_BYTES > LDS_BYTES_MAX) jl = jac_lds_base(nao);
     const long long lneed = (long long)std::max(jl, m.n_atoms * 16) * 8;
     // molecules that do not fit one workgroup's LDS take the host-driven multi-kernel path

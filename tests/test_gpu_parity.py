@@ -125,7 +125,7 @@ def test_stage_hcore_and_fock(G, oracle):
 def test_stage_eigen_and_gemm(G, oracle):
     L, dp = _lib(G)
     rng = np.random.default_rng(2)
-    for n in (1, 2, 7, 16, 33, 100, 190):
+    for n in (1, 2, 7, 16, 33, 100, 113, 131, 146, 149, 150, 190):  # LDS, deferred and global eigenvectors
         A = rng.standard_normal((n, n))
         B = rng.standard_normal((n, n))
         C = np.zeros((n, n))
