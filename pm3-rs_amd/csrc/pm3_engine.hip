@@ -175,6 +175,21 @@ __device__ inline int tri_index(int a, int b, int nat) { return a * (2 * nat - a
 // Packed upper-triangle index of (i, j) for the LDS copy of a symmetric matrix.
 __device__ inline int sym_idx(int i, int j) { return i <= j ? j * (j + 1) / 2 + i : i * (i + 1) / 2 + j; }
 
+// Packed triangle of an n x n matrix, diagonal first: e < n is (e, e); then the strict upper
+// triangle column by column, e = n + j (j - 1) / 2 + i for i < j.
+__device__ inline void tri_unpack(int e, int n, int& i, int& j) {
+  if (e < n) {
+    i = j = e;
+    return;
+  }
+  const int f = e - n;
+  int c = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)f)) * 0.5f);
+  while (c * (c - 1) / 2 > f) --c;
+  while ((c + 1) * c / 2 <= f) ++c;
+  j = c;
+  i = f - c * (c - 1) / 2;
+}
+
 // ------------------------------------------------------------------------------------------
 // Dense product on the FP64 matrix cores: out(i, j) = sum_{k<K} A(i, k) B(k, j) for i < M, j < N.
 // Each wave owns 16 x 16 output tiles; per step the 64 lanes feed one 16x4 slice of A (lane l holds
@@ -1190,11 +1205,15 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       wg_gemm(n, n, n, [&](int i, int k) { return B.F[i * np + k]; }, [&](int k, int j) { return B.P[k * np + j]; },
               [&](int i, int j, double v) { B.T[i * np + j] = v; });
       __syncthreads();
-      for (int e = tid; e < np * np; e += NT) {
-        const int i = e / np, j = e % np;
-        En[e] = (i < n && j < n) ? B.T[i * np + j] - B.T[j * np + i] : 0.0;
-        Fn[e] = B.F[e];
-        if (keep_d) Dn[e] = B.P[e];
+      // history entries are stored as packed triangles: F and D symmetric (diagonal weight 1,
+      // off-diagonal weight 2 in the inner products), E antisymmetric (zero diagonal)
+      const int npk = n * (n + 1) / 2;
+      for (int e = tid; e < npk; e += NT) {
+        int i, j;
+        tri_unpack(e, n, i, j);
+        En[e] = (i == j) ? 0.0 : B.T[i * np + j] - B.T[j * np + i];
+        Fn[e] = B.F[i * np + j];
+        if (keep_d) Dn[e] = B.P[i * np + j];
       }
       __syncthreads();
       // Gram updates: <E_i,E_new>, <D_new,F_j>, <D_i,F_new> for the existing entries, plus
@@ -1203,18 +1222,20 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       double v[3 * MAXD + 2];
 #pragma unroll
       for (int k = 0; k < 3 * MAXD + 2; ++k) v[k] = 0.0;
-      for (int e = tid; e < np * np; e += NT) {
+      for (int e = tid; e < npk; e += NT) {
+        const double wgt = e < n ? 1.0 : 2.0;
         const double en = En[e], fn = Fn[e], dn = keep_d ? Dn[e] : 0.0;
-        v[3 * MAXD] += en * en;
-        v[3 * MAXD + 1] += dn * fn;
+        const double enw = wgt * en, fnw = wgt * fn, dnw = wgt * dn;
+        v[3 * MAXD] += enw * en;
+        v[3 * MAXD + 1] += dnw * fn;
 #pragma unroll
         for (int i = 0; i < MAXD; ++i) {
           if (i < len) {
             const long long so = (long long)D.slot[i] * ms;
-            v[i] += B.Eh[so + e] * en;
+            v[i] += B.Eh[so + e] * enw;
             if (keep_d) {
-              v[MAXD + i] += dn * B.Fh[so + e];
-              v[2 * MAXD + i] += B.Dh[so + e] * fn;
+              v[MAXD + i] += dnw * B.Fh[so + e];
+              v[2 * MAXD + i] += B.Dh[so + e] * fnw;
             }
           }
         }
@@ -1245,15 +1266,18 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       __syncthreads();
       if (D.ok) {
         const int L = D.len;
-        for (int e = tid; e < np * np; e += NT) {
+        for (int e = tid; e < npk; e += NT) {
           double s = 0.0;
           for (int i = 0; i < L; ++i) s += D.coef[i] * B.Fh[(long long)D.slot[i] * ms + e];
-          B.Fu[e] = s;
+          int i, j;
+          tri_unpack(e, n, i, j);
+          B.Fu[i * np + j] = s;
+          B.Fu[j * np + i] = s;
         }
         __syncthreads();
         Fuse = B.Fu;
       } else {
-        Fuse = Fn;
+        Fuse = B.F;  // the newest entry's Fock matrix
       }
     }
     stamp(B, 2);
