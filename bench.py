@@ -40,6 +40,10 @@ from pm3amd import workloads as W  # noqa: E402
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
 NET_FLOP_PER_PAIR = 2.0 * (96 * 128 + 128 * 128 + 128 * 16)  # forward pass per directed pair
+# HBM bytes per scf_batch_kernel launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a
+# 512 MB copy: x2.0 / x1.0) for this build on the default workload, per leg
+# (profiles/scf_batch_kernel_traffic_mixed4096.md); not measured for other workloads.
+MEASURED_TRAFFIC = {("mixed4096", "learned"): 1.766e11, ("mixed4096", "plain"): 2.336e11}
 HBM_PEAK_GBS = 8000.0
 HOF_TOL = 1.0e-5
 
@@ -264,7 +268,9 @@ def main():
             "fallback_rate": float(np.mean(used == 2)), "not_applicable": float(np.mean(used == 0)),
             "model_steps": model.steps},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": MEASURED_TRAFFIC.get((args.workload, "learned" if model is not None else "plain"))
+                     if args.seed == 2026 else None,
                      "kernel": "scf_batch_kernel", "kernel_ms": k_ms, "algorithmic_flop": flops,
                      "pair_kernel_ms": pair_ms},
         "cpu_baseline": cpu,
