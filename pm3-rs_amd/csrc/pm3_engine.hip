@@ -55,6 +55,7 @@ constexpr int LARGE_MAX_NAO = 16384;    // largest molecule of the host-driven p
 // SCF converges, so the converged state is unchanged.
 constexpr double EIG_TOL_FACTOR = 1.0e-2;
 constexpr double EIG_REL = 1.0e-2;  // cap of the loose tolerance relative to the starting off-diagonal
+constexpr double EIG_SKIP = 0.0;    // threshold Jacobi: skip rotations of |a_pq| <= EIG_SKIP x tolerance
 
 __constant__ TableConst c_tab;
 
@@ -101,6 +102,7 @@ struct KArgs {
   int nmol_pairs;              // queue positions covered by pair_base: batched, then large molecules
   double eig_tol_factor;       // eigensolver stops at max(1e-14 |A|, min(factor * |[F,P]|, eig_rel * off0))
   double eig_rel;
+  double eig_skip;
   // learned guess: directed pairs (global indices, see batch_create) and network outputs
   const int8_t* elem8;
   const int* gbeg;
@@ -195,25 +197,58 @@ __device__ inline void tri_unpack(int e, int n, int& i, int& j) {
 // Each wave owns 16 x 16 output tiles; per step the 64 lanes feed one 16x4 slice of A (lane l holds
 // A[l % 16][l / 16]) and one 4x16 slice of B (lane l holds B[l / 16][l % 16]) into
 // v_mfma_f64_16x16x4_f64.  Loaders return 0 outside the logical shape.
-template <class LA, class LB, class ST>
+// C = A B on the FP64 matrix cores by one workgroup: each wave owns 32 x 32 output blocks (four
+// 16 x 16 accumulators, so consecutive MFMAs are independent) and loads 16 k-values ahead.
+// SYM: the product is known to be symmetric (M == N); only blocks on or above the diagonal are
+// computed and each value is stored at (i, j) and (j, i).
+template <bool SYM = false, class LA, class LB, class ST>
 __device__ inline void wg_gemm(int M, int N, int K, LA la, LB lb, ST st) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int tm = (M + 15) >> 4, tn = (N + 15) >> 4;
+  const int bm = (M + 31) >> 5, bn = (N + 31) >> 5;
   const int li = lane & 15, lk = lane >> 4;
-  for (int t = wave; t < tm * tn; t += NW) {
-    const int i0 = (t / tn) << 4, j0 = (t % tn) << 4;
-    double4_t acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < K; k0 += 4) {
-      const int k = k0 + lk;
-      const double av = (k < K && i0 + li < M) ? la(i0 + li, k) : 0.0;
-      const double bv = (k < K && j0 + li < N) ? lb(k, j0 + li) : 0.0;
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  const int nblk = SYM ? bm * (bm + 1) / 2 : bm * bn;
+  for (int t = wave; t < nblk; t += NW) {
+    int bi, bj;
+    if (SYM) {  // upper-triangle block t -> (bi, bj), bi <= bj, row by row
+      bi = 0;
+      int rest = t;
+      while (rest >= bm - bi) { rest -= bm - bi; ++bi; }
+      bj = bi + rest;
+    } else {
+      bi = t / bn;
+      bj = t % bn;
     }
-    // result layout: item r of lane l holds row 4 r + l / 16, column l % 16
+    const int i0 = bi << 5, j0 = bj << 5;
+    const int ia = i0 + li, ib = ia + 16, ja = j0 + li, jb = ja + 16;
+    const bool va = ia < M, vb = ib < M, wa = ja < N, wb = jb < N;
+    double4_t c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      double a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u + lk;
+        const bool kk = k < K;
+        a0[u] = (kk && va) ? la(ia, k) : 0.0;
+        a1[u] = (kk && vb) ? la(ib, k) : 0.0;
+        b0[u] = (kk && wa) ? lb(k, ja) : 0.0;
+        b1[u] = (kk && wb) ? lb(k, jb) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], c00, 0, 0, 0);
+        c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], c01, 0, 0, 0);
+        c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], c10, 0, 0, 0);
+        c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], c11, 0, 0, 0);
+      }
+    }
+    // result layout: item r of lane l holds row 4 r + l / 16, column l % 16 of its tile
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int i = i0 + 4 * r + lk, j = j0 + li;
-      if (i < M && j < N) st(i, j, acc[r]);
+      const int ra = i0 + 4 * r + lk, rb = ra + 16;
+      if (ra < M && ja < N && (!SYM || bi < bj || ra <= ja)) { st(ra, ja, c00[r]); if (SYM && ra != ja) st(ja, ra, c00[r]); }
+      if (ra < M && jb < N && (!SYM || bi < bj || ra <= jb)) { st(ra, jb, c01[r]); if (SYM && ra != jb) st(jb, ra, c01[r]); }
+      if (rb < M && ja < N && (!SYM || bi < bj || rb <= ja)) { st(rb, ja, c10[r]); if (SYM && rb != ja) st(ja, rb, c10[r]); }
+      if (rb < M && jb < N && (!SYM || bi < bj || rb <= jb)) { st(rb, jb, c11[r]); if (SYM && rb != jb) st(jb, rb, c11[r]); }
     }
   }
 }
@@ -247,6 +282,7 @@ struct Mol {
   double* slot;  // per-pair Coulomb partial sums of the Fock build (workgroup scratch)
   int lds_doubles;  // dynamic LDS of the launch, in doubles
   double eig_rel;   // loose-tolerance cap relative to the starting off-diagonal
+  double eig_skip;  // rotations of elements below eig_skip x tolerance are skipped
 };
 
 // ------------------------------------------------------------------------------------------
@@ -572,7 +608,8 @@ constexpr int JB = 10;  // upper bound on the triangle blocks one thread owns (m
 // V (eigenvectors, column-major, ld ldv) is in LDS at g_lds + vbase when VL, else at Vg.
 template <bool VL>
 __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg, int ldv, int vbase, double* lam,
-                                   double* red, int max_sweeps, double tol_abs, double rel, bool defer) {
+                                   double* red, int max_sweeps, double tol_abs, double rel, bool defer,
+                                   double skip_frac) {
   const int tid = threadIdx.x;
   const int nn = n + (n & 1), m = nn >> 1;
   const int npk = n * (n + 1) / 2;
@@ -655,7 +692,7 @@ __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg,
   for (int i = tid; i < n; i += NT) dmax = fmax(dmax, fabs(g_lds[i * (i + 1) / 2 + i]));
   const double scale = fmax(block_max(dmax, red), 1.0);
   const double tol_floor = 1.0e-14 * scale;
-  double tol = tol_floor;
+  double tol = tol_floor, skip = 1.0e-300;
   // eigenvector update mapping: row i = tid % n, slots k = tid / n, tid / n + rstep, ...
   const int rstep = NT / max(n, 1);
   const int vi = tid % max(n, 1), vk0 = tid / max(n, 1);
@@ -680,7 +717,10 @@ __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg,
       const double offmax = block_max(off, red);
       // a loose tolerance never exceeds a fixed fraction of the starting off-diagonal, so every
       // call that has anything to rotate does rotate (no stalled, falsely converged SCF)
-      if (sweep == 0) tol = fmax(tol_floor, fmin(tol_abs, rel * offmax));
+      if (sweep == 0) {
+        tol = fmax(tol_floor, fmin(tol_abs, rel * offmax));
+        skip = fmax(1.0e-300, skip_frac * tol);
+      }
       if (offmax <= tol) break;
     }
     for (int r = 0; r < nn - 1; ++r) {
@@ -691,7 +731,8 @@ __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg,
         if (p < n && q < n) {
           const double apq = g_lds[sym_idx(p, q)];
           const double app = g_lds[sym_idx(p, p)], aqq = g_lds[sym_idx(q, q)];
-          if (fabs(apq) > 1.0e-300 && fabs(apq) > 1.0e-18 * (fabs(app) + fabs(aqq))) {
+          // threshold Jacobi: elements already below the stopping tolerance are not rotated
+          if (fabs(apq) > skip && fabs(apq) > 1.0e-18 * (fabs(app) + fabs(aqq))) {
             const double tau = (aqq - app) / (2.0 * apq);
             t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
             c = 1.0 / sqrt(1.0 + t * t);
@@ -722,6 +763,8 @@ __device__ __noinline__ int jacobi(const double* Ain, int n, int np, double* Vg,
           ix[jj][0] = ix[jj][1] = ix[jj][2] = ix[jj][3] = -1;
           if (j >= JB || kl[j] < 0) continue;
           const int K = kl[j] & 0xffff, L = kl[j] >> 16;
+          // blocks whose two rotations are both identities keep their values
+          if (g_lds[cso + 3 * K + 1] == 0.0 && g_lds[cso + 3 * L + 1] == 0.0) continue;
           const int p = pq[2 * K], q = pq[2 * K + 1];
           if (K == L) {
             if (p < n && q < n) ix[jj][0] = sym_idx(p, q);
@@ -951,9 +994,10 @@ __device__ __noinline__ void eigensolve(const Mol& M, const Bufs& B, const doubl
   const bool v_in_lds = jac_lds_with_v(n) <= M.lds_doubles;
   double* V = v_in_lds ? lds + jac_lds_base(n) : B.V;
   const int ldv = v_in_lds ? n : np;
-  const int nsw = v_in_lds ? jacobi<true>(Amat, n, np, nullptr, ldv, jac_lds_base(n), B.lam, red, 60, tol_abs, M.eig_rel, false)
+  const int nsw = v_in_lds ? jacobi<true>(Amat, n, np, nullptr, ldv, jac_lds_base(n), B.lam, red, 60, tol_abs, M.eig_rel, false,
+                                          M.eig_skip)
                            : jacobi<false>(Amat, n, np, B.V, ldv, 0, B.lam, red, 60, tol_abs, M.eig_rel,
-                                           jac_lds_deferred(n) <= M.lds_doubles);
+                                           jac_lds_deferred(n) <= M.lds_doubles, M.eig_skip);
   if (B.ph && threadIdx.x == 0) {
     B.ph[9] += (unsigned long long)nsw;
     B.ph[10] += 1ull;
@@ -991,8 +1035,8 @@ __device__ __noinline__ void eigensolve(const Mol& M, const Bufs& B, const doubl
 // P = 2 C_occ C_occ^T from Cn into Pdst.
 __device__ void density(const Mol& M, const double* Cn, double* Pdst) {
   const int n = M.n, np = M.np;
-  wg_gemm(n, n, M.nocc, [&](int i, int k) { return Cn[i * np + k]; }, [&](int k, int j) { return Cn[j * np + k]; },
-          [&](int i, int j, double v) { Pdst[i * np + j] = 2.0 * v; });
+  wg_gemm<true>(n, n, M.nocc, [&](int i, int k) { return Cn[i * np + k]; }, [&](int k, int j) { return Cn[j * np + k]; },
+                [&](int i, int j, double v) { Pdst[i * np + j] = 2.0 * v; });
   __syncthreads();
 }
 
@@ -1367,6 +1411,7 @@ __host__ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double*
   M.slot = Wbase;
   M.lds_doubles = A.lds_doubles;
   M.eig_rel = A.eig_rel;
+  M.eig_skip = A.eig_skip;
   return M;
 }
 
@@ -2595,6 +2640,8 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   if (const char* ev = std::getenv("PM3_EIG_TOL_FACTOR")) a.eig_tol_factor = std::atof(ev);
   a.eig_rel = EIG_REL;
   if (const char* ev = std::getenv("PM3_EIG_REL")) a.eig_rel = std::atof(ev);
+  a.eig_skip = EIG_SKIP;
+  if (const char* ev = std::getenv("PM3_EIG_SKIP")) a.eig_skip = std::atof(ev);
   HIP_OK(hipFuncSetAttribute((const void*)scf_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_need));
   if (!b.large.empty() && !large_setup(b, err)) return false;
   return true;
