@@ -29,6 +29,15 @@ pm3_status pm3_stage_eigen_warm(int32_t n, const double* a, const double* c, dou
  * (src/linalg.rs:65-122).  n <= 190. */
 pm3_status pm3_stage_gemm(int32_t n, const double* a, const double* b, double* c);
 
+/* Aufbau density 2 C_occ C_occ^T of a symmetric Fock matrix (caller's AO order) without its
+ * eigenvectors, by trace-correcting purification, n_occ from the molecule's electron count, plus
+ * the HOMO and LUMO by Lanczos: what the batched SCF does in place of the eigensolve and
+ * density_from_coeff when no orbitals are requested; replaces the pair symmetric_eigen
+ * (src/scf.rs:1264) + density_from_coeff (src/scf.rs:982-994).  products = number of
+ * purification products, -1 when the frontier gap is closed (no density returned then). */
+pm3_status pm3_stage_purify(const pm3_molecule* mol, const pm3_params* params, const double* fock, double* density,
+                            int32_t* products, double* homo, double* lumo);
+
 #ifdef __cplusplus
 }
 #endif

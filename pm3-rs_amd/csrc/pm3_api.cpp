@@ -431,6 +431,24 @@ pm3_status pm3_stage_eigen_warm(int32_t n, const double* a, const double* c, dou
   return PM3_OK;
 }
 
+pm3_status pm3_stage_purify(const pm3_molecule* mol, const pm3_params* params, const double* fock, double* density,
+                            int32_t* products, double* homo, double* lumo) {
+  g_last_error.clear();
+  if (!params || !fock || !density) return fail(PM3_INVALID_INPUT, "null argument");
+  pm3_status s = check_molecule(mol);
+  if (s != PM3_OK) return s;
+  const int n = pm3_query_nao(mol, params);
+  if (n < 0) return fail(PM3_MISSING_ELEMENT, "missing element");
+  std::vector<double> in(fock, fock + (size_t)n * n), out;
+  s = run_stage(mol, 6, in, out);
+  if (s != PM3_OK) return s;
+  std::memcpy(density, out.data(), (size_t)n * n * sizeof(double));
+  if (products) *products = (int32_t)out[(size_t)n * n];
+  if (homo) *homo = out[(size_t)n * n + 1];
+  if (lumo) *lumo = out[(size_t)n * n + 2];
+  return PM3_OK;
+}
+
 pm3_status pm3_stage_gemm(int32_t n, const double* a, const double* b, double* c) {
   g_last_error.clear();
   if (!a || !b || !c) return fail(PM3_INVALID_INPUT, "null argument");

@@ -103,6 +103,7 @@ struct KArgs {
   double eig_tol_factor;       // eigensolver stops at max(1e-14 |A|, min(factor * |[F,P]|, eig_rel * off0))
   double eig_rel;
   double eig_skip;
+  int purify;                  // 1: densities by purification, frontier levels by Lanczos (no orbitals)
   // learned guess: directed pairs (global indices, see batch_create) and network outputs
   const int8_t* elem8;
   const int* gbeg;
@@ -1040,6 +1041,363 @@ __device__ void density(const Mol& M, const double* Cn, double* Pdst) {
   __syncthreads();
 }
 
+// Lowest and highest Gershgorin bound of the symmetric matrix F (columns read coalesced).
+__device__ inline void gershgorin(const double* F, int n, int np, double* red, double& lo, double& hi) {
+  double l = 1.0e300, h = -1.0e300;
+  for (int i = threadIdx.x; i < n; i += NT) {
+    double r = 0.0;
+    for (int j = 0; j < n; ++j) r += j == i ? 0.0 : fabs(F[j * np + i]);
+    const double c = F[i * np + i];
+    l = fmin(l, c - r);
+    h = fmax(h, c + r);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    l = fmin(l, __shfl_xor(l, o, 64));
+    h = fmax(h, __shfl_xor(h, o, 64));
+  }
+  __syncthreads();
+  if (lane == 0) {
+    red[2 * wave] = l;
+    red[2 * wave + 1] = h;
+  }
+  __syncthreads();
+  lo = red[0];
+  hi = red[1];
+  for (int w = 1; w < NW; ++w) {
+    lo = fmin(lo, red[2 * w]);
+    hi = fmax(hi, red[2 * w + 1]);
+  }
+  __syncthreads();
+}
+
+// Aufbau density of the symmetric matrix Fu by purification instead of eigenvectors: the
+// trace-correcting second-order spectral projection of Niklasson (Phys. Rev. B 66, 155115, 2002).
+// X0 = (e_max I - Fu) / (e_max - e_min) with Gershgorin bounds maps the spectrum into [0, 1] in
+// reverse order; every step replaces X by X^2 or by 2X - X^2, whichever brings tr(X) nearer to
+// n_occ, which drives the n_occ lowest eigenvalues of Fu to 1 and the others to 0.  P = 2 X is then
+// the projector 2 C_occ C_occ^T of the eigenvector route whenever the HOMO-LUMO gap is open.
+// tr(X^2) = |X|_F^2 for symmetric X, so the trace and the norm of each new X are summed in the
+// product's store.  The iteration stops SP2_EXTRA products after the idempotency error
+// tr(X) - tr(X^2) has fallen below SP2_TAIL (each further product squares the distance of the
+// eigenvalues on one side from 0 or 1).  X ping-pongs between B.Cn and B.T; P goes to Pdst.
+// Returns the number of products, or -1 when the error has not fallen within SP2_MAX products
+// (frontier orbitals degenerate or nearly so); the caller then diagonalises.
+constexpr int SP2_MAX = 90;
+constexpr int SP2_EXTRA = 2;
+constexpr double SP2_TAIL = 1.0e-7;
+__device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red) {
+  const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
+  if (nocc == 0 || nocc == n) {
+    for (int e = tid; e < n * n; e += NT) {
+      const int i = e / n, j = e % n;
+      Pdst[i * np + j] = (nocc == n && i == j) ? 2.0 : 0.0;
+    }
+    __syncthreads();
+    return 0;
+  }
+  double lo, hi;
+  gershgorin(Fu, n, np, red, lo, hi);
+  if (!(hi - lo > 1.0e-12 * fmax(1.0, fabs(hi)))) return -1;
+  const double s = 1.0 / (hi - lo);
+  double* X = B.Cn;
+  double* Y = B.T;
+  double v[2] = {0.0, 0.0};
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e % n;
+    const double x = ((i == j ? hi : 0.0) - Fu[i * np + j]) * s;
+    X[i * np + j] = x;
+    v[1] += x * x;
+    if (i == j) v[0] += x;
+  }
+  block_sum<2>(v, red);
+  double tr = v[0], t2 = v[1];
+  const double occ = (double)nocc;
+  int tail = -1, k = 0;
+  for (; k < SP2_MAX; ++k) {
+    if (tail < 0 && tr - t2 < SP2_TAIL) tail = SP2_EXTRA;
+    if (tail == 0) break;
+    const bool sq = fabs(t2 - occ) < fabs(2.0 * tr - t2 - occ);
+    const double* Xc = X;
+    double* Yc = Y;
+    double w[2] = {0.0, 0.0};
+    wg_gemm<true>(n, n, n, [&](int i, int q) { return Xc[q * np + i]; }, [&](int q, int j) { return Xc[q * np + j]; },
+                  [&](int i, int j, double x2) {
+                    const double y = sq ? x2 : 2.0 * Xc[i * np + j] - x2;
+                    Yc[i * np + j] = y;
+                    w[1] += y * y;
+                    if (i == j) w[0] += y;
+                  });
+    block_sum<2>(w, red);
+    tr = w[0];
+    t2 = w[1];
+    double* t = X;
+    X = Y;
+    Y = t;
+    if (tail > 0) --tail;
+  }
+  if (tail != 0) return -1;
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e % n;
+    Pdst[i * np + j] = 2.0 * X[i * np + j];
+  }
+  __syncthreads();
+  return k;
+}
+
+// Number of eigenvalues below x of the symmetric tridiagonal matrix (diagonal a, off-diagonal b)
+// by the Sturm sequence of its LDL^T pivots.
+__device__ inline int sturm_below(const double* a, const double* b, int m, double x, double pivmin) {
+  int cnt = 0;
+  double d = a[0] - x;
+  if (fabs(d) < pivmin) d = -pivmin;
+  cnt += d < 0.0;
+  for (int i = 1; i < m; ++i) {
+    d = a[i] - x - b[i - 1] * b[i - 1] / d;
+    if (fabs(d) < pivmin) d = -pivmin;
+    cnt += d < 0.0;
+  }
+  return cnt;
+}
+
+// Largest eigenvalue of a symmetric tridiagonal matrix by multisection: the 64 lanes of the
+// calling wave test 64 points of the bracketing interval per round.
+__device__ inline double tridiag_max(const double* a, const double* b, int m) {
+  const int lane = threadIdx.x & 63;
+  double lo = 1.0e300, hi = -1.0e300, bmax = 0.0;
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i + 1 < m ? fabs(b[i]) : 0.0);
+    lo = fmin(lo, a[i] - r);
+    hi = fmax(hi, a[i] + r);
+    if (i + 1 < m) bmax = fmax(bmax, fabs(b[i]));
+  }
+  const double scale = fmax(fabs(lo), fabs(hi));
+  const double pivmin = fmax(1.0e-300, 1.0e-300 * bmax * bmax);
+  lo -= 1.0e-14 * scale + 1.0e-300;
+  hi += 1.0e-14 * scale + 1.0e-300;
+  for (int round = 0; round < 16 && hi - lo > 2.0e-16 * scale; ++round) {
+    const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+    const bool above = sturm_below(a, b, m, x, pivmin) == m;  // x above the largest eigenvalue
+    const unsigned long long mask = __ballot(above);
+    const int first = mask ? __ffsll((long long)mask) - 1 : 64;
+    const double nhi = first < 64 ? lo + (hi - lo) * (double)(first + 1) / 65.0 : hi;
+    const double nlo = first > 0 ? lo + (hi - lo) * (double)first / 65.0 : lo;
+    lo = nlo;
+    hi = nhi;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// Residual |T s - theta s| = b[m-1] |s[m-1]| of the Lanczos Ritz pair of the tridiagonal T (diagonal a,
+// off-diagonal b, size m) for its eigenvalue theta: s by two steps of inverse iteration with the
+// partially pivoted LU of T - theta I (the dgttrf / dgtts2 scheme), one thread, work = 6 (m + 2)
+// doubles.
+__device__ inline double ritz_residual(const double* a, const double* b, int m, double theta, double* work) {
+  double* d = work;
+  double* dl = d + (m + 2);
+  double* du = dl + (m + 2);
+  double* du2 = du + (m + 2);
+  double* x = du2 + (m + 2);
+  double* piv = x + (m + 2);
+  double tmax = 0.0;
+  for (int i = 0; i < m; ++i) tmax = fmax(tmax, fabs(a[i] - theta) + (i + 1 < m ? fabs(b[i]) : 0.0));
+  const double pivmin = fmax(1.0e-30 * tmax, 1.0e-290);
+  for (int i = 0; i < m; ++i) {
+    d[i] = a[i] - theta;
+    if (i + 1 < m) dl[i] = du[i] = b[i];
+    du2[i] = 0.0;
+  }
+  for (int i = 0; i + 1 < m; ++i) {
+    if (fabs(d[i]) >= fabs(dl[i])) {
+      if (fabs(d[i]) < pivmin) d[i] = d[i] < 0.0 ? -pivmin : pivmin;
+      const double f = dl[i] / d[i];
+      dl[i] = f;
+      d[i + 1] -= f * du[i];
+      piv[i] = 0.0;
+    } else {
+      const double f = d[i] / dl[i];
+      d[i] = dl[i];
+      dl[i] = f;
+      const double t = du[i];
+      du[i] = d[i + 1];
+      d[i + 1] = t - f * d[i + 1];
+      if (i + 2 < m) {
+        du2[i] = du[i + 1];
+        du[i + 1] = -f * du[i + 1];
+      }
+      piv[i] = 1.0;
+    }
+  }
+  if (fabs(d[m - 1]) < pivmin) d[m - 1] = d[m - 1] < 0.0 ? -pivmin : pivmin;
+  for (int i = 0; i < m; ++i) x[i] = 1.0;
+  for (int it = 0; it < 2; ++it) {
+    for (int i = 0; i + 1 < m; ++i) {
+      if (piv[i] == 0.0) {
+        x[i + 1] -= dl[i] * x[i];
+      } else {
+        const double t = x[i];
+        x[i] = x[i + 1];
+        x[i + 1] = t - dl[i] * x[i];
+      }
+    }
+    x[m - 1] /= d[m - 1];
+    if (m > 1) x[m - 2] = (x[m - 2] - du[m - 2] * x[m - 1]) / d[m - 2];
+    for (int i = m - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / d[i];
+    double mx = 0.0;
+    for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(x[i]));
+    if (!(mx > 0.0) || !(mx < 1.0e300)) return 0.0;  // exactly singular: theta is an eigenvalue of T
+    double nn = 0.0;
+    for (int i = 0; i < m; ++i) {
+      x[i] /= mx;
+      nn += x[i] * x[i];
+    }
+    nn = 1.0 / sqrt(nn);
+    for (int i = 0; i < m; ++i) x[i] *= nn;
+  }
+  return fabs(b[m - 1] * x[m - 1]);
+}
+
+// LDS doubles of frontier_levels: per half a Lanczos vector, the residual, the projection
+// coefficients, the tridiagonal (diagonal, off-diagonal) and the work of ritz_residual; then the
+// per-half Ritz values and stop flags.
+__host__ __device__ constexpr int lanczos_half(int n) { return 2 * n + 9 * (n + 2); }
+__host__ __device__ constexpr int lanczos_lds(int n) { return 2 * lanczos_half(n) + 16; }
+constexpr double LZ_TOL = 1.0e-10;  // Ritz residual (eV) at which a frontier level is taken
+
+// HOMO and LUMO of Fu given the projector X = P / 2 onto its n_occ lowest eigenvectors, without
+// diagonalising Fu: the HOMO is the largest eigenvalue of A_o = X Fu X + c (I - X) and the LUMO the
+// smallest of A_v = (I - X) Fu (I - X) + c' X, with c below and c' above the spectrum, so that the
+// complement of each subspace sits at the far end.  Each comes from a Lanczos run with two passes
+// of full reorthogonalisation (so the basis stays orthonormal and the run is exact at dimension
+// n); every fourth step from the eighth on, the run stops once the residual of its top Ritz pair
+// is below LZ_TOL (the Ritz value is then within LZ_TOL of the level).  The two runs proceed side
+// by side on the two halves of the workgroup.  Scratch: B.T (Fu X), B.Cn (A_o), B.V (-A_v),
+// Lanczos vectors in B.Pn and B.C; LDS as lanczos_lds.  Writes eps[nocc - 1] and eps[nocc].
+__device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const double* Fu, const double* P, double* lds,
+                                             double* red) {
+  const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
+  if (n == 0) return;
+  if (lanczos_lds(n) > M.lds_doubles) {  // the host sizes the launch's LDS for this; never taken
+    if (tid == 0) {
+      if (nocc > 0) B.eps[nocc - 1] = __builtin_nan("");
+      if (nocc < n) B.eps[nocc] = __builtin_nan("");
+    }
+    __syncthreads();
+    return;
+  }
+  // (with n_occ = 0 or n_occ = n one side is empty and the other operator is Fu itself)
+  double lo, hi;
+  gershgorin(Fu, n, np, red, lo, hi);
+  const double c_lo = lo - 1.0 - 0.1 * (hi - lo), c_hi = hi + 1.0 + 0.1 * (hi - lo);
+  double* G = B.T;
+  double* Ao = B.Cn;
+  double* Av = B.V;
+  wg_gemm(n, n, n, [&](int i, int k) { return Fu[k * np + i]; }, [&](int k, int j) { return 0.5 * P[k * np + j]; },
+          [&](int i, int j, double v) { G[i * np + j] = v; });
+  __syncthreads();
+  wg_gemm<true>(n, n, n, [&](int i, int k) { return 0.5 * P[k * np + i]; }, [&](int k, int j) { return G[k * np + j]; },
+                [&](int i, int j, double sv) {
+                  const double x = 0.5 * P[i * np + j];
+                  const double id = i == j ? 1.0 : 0.0;
+                  Ao[i * np + j] = sv + c_lo * (id - x);
+                  Av[i * np + j] = -(Fu[i * np + j] - G[i * np + j] - G[j * np + i] + sv + c_hi * x);
+                });
+  __syncthreads();
+  // half h = 0: occupied side (A_o), h = 1: virtual side (-A_v); the largest eigenvalue of each
+  const int h = tid >> 8, t = tid & 255, lane = tid & 63, wv = (tid >> 6) & 3;
+  const double* A = h == 0 ? Ao : Av;
+  double* Q = h == 0 ? B.Pn : B.C;
+  const bool active_half = h == 0 ? nocc > 0 : nocc < n;
+  double* q = lds + h * lanczos_half(n);  // current vector
+  double* w = q + n;                      // residual
+  double* hc = w + n;                     // projection coefficients
+  double* al = hc + (n + 2);              // tridiagonal diagonal
+  double* be = al + (n + 2);              // off-diagonal
+  double* work = be + (n + 2);            // ritz_residual
+  double* top = lds + 2 * lanczos_half(n);  // [h] Ritz value, [2 + h] stop flag
+  const double scale = fmax(fabs(c_lo), fabs(c_hi));
+  // start vector: a fixed spread over all basis directions, normalised
+  for (int i = t; i < n; i += 256) q[i] = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);
+  if (tid == 0) top[2] = top[3] = 0.0;
+  __syncthreads();
+  double nn = 0.0;
+  for (int i = lane; i < n; i += 64) nn += q[i] * q[i];
+  nn = sqrt(wave_sum(nn));  // every wave of the half sums the same vector
+  __syncthreads();
+  for (int i = t; i < n; i += 256) {
+    q[i] /= nn;
+    Q[i] = q[i];
+  }
+  __syncthreads();
+  bool done = !active_half;
+  for (int k = 0; k < n; ++k) {
+    const bool run = !done;
+    // w = A q (A symmetric: column reads are coalesced across rows)
+    if (run)
+      for (int i = t; i < n; i += 256) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += A[j * np + i] * q[j];
+        w[i] = s;
+      }
+    __syncthreads();
+    double alpha = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+      // hc_j = Q_j . w for j <= k: wave wv of the half takes j = wv, wv + 4, ...
+      if (run)
+        for (int j = wv; j <= k; j += 4) {
+          double s = 0.0;
+          for (int i = lane; i < n; i += 64) s += Q[j * np + i] * w[i];
+          s = wave_sum(s);
+          if (lane == 0) hc[j] = s;
+        }
+      __syncthreads();
+      if (run)
+        for (int i = t; i < n; i += 256) {
+          double s = w[i];
+          for (int j = 0; j <= k; ++j) s -= hc[j] * Q[j * np + i];
+          w[i] = s;
+        }
+      alpha += run ? hc[k] : 0.0;
+      __syncthreads();
+    }
+    double bb = 0.0;
+    if (run)
+      for (int i = lane; i < n; i += 64) bb += w[i] * w[i];
+    bb = sqrt(wave_sum(bb));
+    const int mk = k + 1;
+    const bool breakdown = bb <= 1.0e-13 * scale;
+    if (run) {
+      if (t == 0) {
+        al[k] = alpha;
+        be[k] = bb;
+      }
+      if (mk < n && !breakdown)
+        for (int i = t; i < n; i += 256) {
+          const double x = w[i] / bb;
+          q[i] = x;
+          Q[mk * np + i] = x;
+        }
+    }
+    __syncthreads();
+    const bool check = run && (mk == n || breakdown || (mk >= 8 && (mk & 3) == 0));
+    if (check && wv == 0) {
+      const double th = tridiag_max(al, be, mk);
+      if (lane == 0) {
+        const bool stop = mk == n || breakdown || ritz_residual(al, be, mk, th, work) <= LZ_TOL;
+        top[h] = th;
+        top[2 + h] = stop ? 1.0 : 0.0;
+      }
+    }
+    __syncthreads();
+    if (check && top[2 + h] != 0.0) done = true;
+    if ((nocc == 0 || top[2] != 0.0) && (nocc == n || top[3] != 0.0)) break;  // both runs stopped
+  }
+  if (tid == 0 && nocc > 0) B.eps[nocc - 1] = top[0];
+  if (tid == 0 && nocc < n) B.eps[nocc] = -top[1];
+  __syncthreads();
+}
+
 // Superposition density (src/scf.rs:855-881) into Pdst (assumed zero).
 __device__ inline void sad_into(const KArgs& A, const Mol& M, double* Pdst) {
   const int np = M.np;
@@ -1131,6 +1489,7 @@ __device__ __noinline__ int initial_guess(const KArgs& A, const Mol& M, const Bu
   for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
   __syncthreads();
   if (guess == 2) {
+    if (A.purify && sp2_density(M, B, M.H, B.P, red) >= 0) return 0;
     eigensolve(M, B, M.H, false, lds, red);
     density(M, B.Cn, B.P);
     return 0;
@@ -1143,6 +1502,13 @@ __device__ __noinline__ int initial_guess(const KArgs& A, const Mol& M, const Bu
     learned_into(A, M, B.P);
     const double cl = commutator2(A, M, B, lds, red);  // leaves F(P_learned) in B.F
     if (cl <= cs) {
+      if (A.purify) {
+        if (sp2_density(M, B, B.F, B.P, red) < 0) {
+          eigensolve(M, B, B.F, false, lds, red);
+          density(M, B.Cn, B.P);
+        }
+        return 1;
+      }
       eigensolve(M, B, B.F, false, lds, red, A.eig_tol_factor * sqrt(cl));
       density(M, B.Cn, B.P);
       for (int e = tid; e < np * np; e += NT) B.C[e] = B.Cn[e];
@@ -1187,7 +1553,8 @@ struct PathResult {
 };
 
 // One restricted SCF path (rhf_loop, src/scf.rs:1179-1316).  Leaves P, eps, Cn (orbitals of the
-// last extrapolated Fock) in B.
+// last extrapolated Fock) in B; with A.purify each density is the purified aufbau projector of
+// the extrapolated Fock matrix and only eps[nocc - 1], eps[nocc] (its frontier levels) are set.
 __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int guess, int accel, double* lds,
                                double* red, Diis& D) {
   const int n = M.n, np = M.np, tid = threadIdx.x;
@@ -1209,8 +1576,9 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
   __syncthreads();
   PathResult R{0.0, 1.0e300, 0, 0, gused};
   double e_old = 0.0;
-  bool have_c = gused == 1;
+  bool have_c = gused == 1 && !A.purify;
   double eig_tol = 0.0;  // per-iteration eigensolver tolerance (DIIS error norm scaled)
+  const double* Flast = B.F;  // the matrix of the last aufbau step
   for (int it = 0; it < A.max_scf; ++it) {
     R.iters = it + 1;
     stamp(B, 7);
@@ -1325,9 +1693,24 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       }
     }
     stamp(B, 2);
-    eig_tol = accel != 0 ? A.eig_tol_factor * D.en : 0.0;
-    eigensolve(M, B, Fuse, have_c, lds, red, eig_tol);
-    density(M, B.Cn, B.Pn);
+    Flast = Fuse;
+    if (A.purify) {
+      const int ns = sp2_density(M, B, Fuse, B.Pn, red);
+      if (ns < 0) {  // no open gap: diagonalise
+        eigensolve(M, B, Fuse, false, lds, red);
+        density(M, B.Cn, B.Pn);
+      }
+      if (B.ph && tid == 0) {
+        B.ph[9] += (unsigned long long)max(ns, 0);
+        B.ph[10] += 1ull;
+        B.ph[11] += ns < 0 ? 1ull : 0ull;
+      }
+      stamp(B, 4);
+    } else {
+      eig_tol = accel != 0 ? A.eig_tol_factor * D.en : 0.0;
+      eigensolve(M, B, Fuse, have_c, lds, red, eig_tol);
+      density(M, B.Cn, B.Pn);
+    }
     double dv[1] = {0.0};
     for (int e = tid; e < np * np; e += NT) {
       const double d = B.Pn[e] - B.P[e];
@@ -1339,16 +1722,20 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
     R.last_dp = dp;
     for (int e = tid; e < np * np; e += NT) {
       B.P[e] = B.Pn[e];
-      B.C[e] = B.Cn[e];
+      if (!A.purify) B.C[e] = B.Cn[e];
     }
     __syncthreads();
-    have_c = true;
+    have_c = !A.purify;
     stamp(B, 6);
     e_old = e_elec;
     if (it > 0 && de < A.e_tol && dp < A.p_tol) {
       R.converged = 1;
       break;
     }
+  }
+  if (A.purify) {
+    frontier_levels(M, B, Flast, B.P, lds, red);
+    stamp(B, 5);
   }
   build_fock(A, M, M.H, B.P, B.F, lds);
   double ev[1] = {0.0};
@@ -1700,9 +2087,10 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
       for (int e = tid; e < n * n; e += NT) {
         const int i = e / n, j = e % n;
         A.out_P[md.out_mat + e] = B.P[i * np + j];
-        A.out_C[md.out_mat + e] = B.C[i * np + j];
+        if (!A.purify) A.out_C[md.out_mat + e] = B.C[i * np + j];
       }
-      for (int j = tid; j < n; j += NT) A.out_eps[md.out_vec + j] = B.eps[j];
+      if (!A.purify)
+        for (int j = tid; j < n; j += NT) A.out_eps[md.out_vec + j] = B.eps[j];
     }
     __syncthreads();
   }
@@ -1714,7 +2102,9 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
 // mode 3 out = in[0:n*n] . in[n*n:2n*n] on the matrix cores;
 // mode 4 warm eigensolve of in[0:n*n] in the basis in[n*n:2n*n]: out = eps[n] then C[n*n];
 // mode 5 learned initial density (network outputs already computed): out = P[n*n], then the
-// squared first-iteration commutator norms of the learned and the superposition densities.
+// squared first-iteration commutator norms of the learned and the superposition densities;
+// mode 6 purified aufbau density of `in` (n_occ of molecule 0): out = P[n*n], then the number of
+// products (-1: not converged), HOMO and LUMO from the Lanczos runs.
 // Matrices in `in`/`out` are dense n x n row-major in the engine's (heavy-first) AO order.
 __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const double* in, double* out) {
   double* lds = g_lds;
@@ -1764,6 +2154,17 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
     if (tid == 0) {
       out[n * n] = cl;
       out[n * n + 1] = cs;
+    }
+  } else if (mode == 6) {
+    for (int e = tid; e < n * n; e += NT) B.Fu[(e / n) * np + e % n] = in[e];
+    __syncthreads();
+    const int ns = sp2_density(M, B, B.Fu, B.P, red);
+    if (ns >= 0) frontier_levels(M, B, B.Fu, B.P, lds, red);
+    for (int e = tid; e < n * n; e += NT) out[e] = B.P[(e / n) * np + e % n];
+    if (tid == 0) {
+      out[n * n] = (double)ns;
+      out[n * n + 1] = (ns >= 0 && M.nocc > 0) ? B.eps[M.nocc - 1] : 0.0;
+      out[n * n + 2] = (ns >= 0 && M.nocc < n) ? B.eps[M.nocc] : 0.0;
     }
   }
 }
@@ -1978,6 +2379,7 @@ struct Batch {
   std::vector<std::vector<int>> ao_map, atom_map;  // sorted -> caller indices
   std::vector<int> pre_status;
   bool want_gradient = false;
+  bool purify = false;  // densities by purification; orbitals are then not produced
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
   void* bufs[64] = {nullptr};
@@ -2383,6 +2785,10 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   b.ao_map.resize(nmol);
   b.atom_map.resize(nmol);
   b.pre_status.assign(nmol, 0);
+  // Without orbitals to return, each SCF density is purified from the Fock matrix instead of
+  // built from its eigenvectors (PM3_PURIFY=0 keeps the eigensolver).
+  b.purify = !req.want_orbitals;
+  if (const char* ev = std::getenv("PM3_PURIFY")) b.purify = b.purify && ev[0] != '0';
   std::vector<int> aslot, aoff;
   std::vector<double> ax, ay, az;
   int np_max = 16;
@@ -2440,9 +2846,12 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
     d.nocc = (int)(ne / 2);
     d.np = (nao + 15) & ~15;
     // eigensolver LDS: packed matrix + rotation schedule, plus the eigenvectors when they fit
-    int jl = jac_lds_with_v(nao);
-    if (jl * 8 + STATIC_LDS_BYTES > LDS_BYTES_MAX) jl = jac_lds_deferred(nao);
-    if (jl * 8 + STATIC_LDS_BYTES > LDS_BYTES_MAX) jl = jac_lds_base(nao);
+    // (with purification the eigensolver only runs when a gap closes: its vectors stay in HBM)
+    auto fits = [](int doubles) { return doubles * 8 + STATIC_LDS_BYTES <= LDS_BYTES_MAX; };
+    int jl = jac_lds_base(nao);
+    if (!b.purify && fits(jac_lds_with_v(nao))) jl = jac_lds_with_v(nao);
+    else if (!b.purify && fits(jac_lds_deferred(nao))) jl = jac_lds_deferred(nao);
+    if (b.purify) jl = std::max(jl, lanczos_lds(nao));
     const long long lneed = (long long)std::max(jl, m.n_atoms * 16) * 8;
     // molecules that do not fit one workgroup's LDS take the host-driven multi-kernel path
     const bool large = lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || req.force_large;
@@ -2641,6 +3050,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   a.eig_rel = EIG_REL;
   if (const char* ev = std::getenv("PM3_EIG_REL")) a.eig_rel = std::atof(ev);
   a.eig_skip = EIG_SKIP;
+  a.purify = b.purify ? 1 : 0;
   if (const char* ev = std::getenv("PM3_EIG_SKIP")) a.eig_skip = std::atof(ev);
   HIP_OK(hipFuncSetAttribute((const void*)scf_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_need));
   if (!b.large.empty() && !large_setup(b, err)) return false;
@@ -2765,11 +3175,13 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
   std::vector<double> hP, hC, hE;
   if (b.mat_total > 0) {
     hP.resize(b.mat_total);
-    hC.resize(b.mat_total);
-    hE.resize(b.vec_total);
     HIP_OK(hipMemcpy(hP.data(), b.args.out_P, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(hC.data(), b.args.out_C, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemcpy(hE.data(), b.args.out_eps, sizeof(double) * b.vec_total, hipMemcpyDeviceToHost));
+    if (!b.purify) {
+      hC.resize(b.mat_total);
+      hE.resize(b.vec_total);
+      HIP_OK(hipMemcpy(hC.data(), b.args.out_C, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(hE.data(), b.args.out_eps, sizeof(double) * b.vec_total, hipMemcpyDeviceToHost));
+    }
   }
   for (int k : b.order_all) {
     MolResult& r = out[k];
@@ -2807,15 +3219,16 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
       const int n = d.nao;
       const std::vector<int>& mp = b.ao_map[k];
       r.density.assign((size_t)n * n, 0.0);
-      r.mo_coeff.assign((size_t)n * n, 0.0);
-      r.mo_energies.assign(hE.begin() + d.out_vec, hE.begin() + d.out_vec + n);
       const double* Ps = hP.data() + d.out_mat;
-      const double* Cs = hC.data() + d.out_mat;
       for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
-          r.density[(size_t)mp[i] * n + mp[j]] = Ps[(size_t)i * n + j];
-          r.mo_coeff[(size_t)mp[i] * n + j] = Cs[(size_t)i * n + j];
-        }
+        for (int j = 0; j < n; ++j) r.density[(size_t)mp[i] * n + mp[j]] = Ps[(size_t)i * n + j];
+      if (!b.purify) {
+        r.mo_coeff.assign((size_t)n * n, 0.0);
+        r.mo_energies.assign(hE.begin() + d.out_vec, hE.begin() + d.out_vec + n);
+        const double* Cs = hC.data() + d.out_mat;
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j < n; ++j) r.mo_coeff[(size_t)mp[i] * n + j] = Cs[(size_t)i * n + j];
+      }
     }
   }
   return true;
@@ -2854,15 +3267,15 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
   const MolDesc& d = b->desc[0];
   const int n = d.nao;
   const std::vector<int>& mp = b->ao_map[0];
-  // modes 0 and 1 speak the caller's AO order; the others take generic matrices as given
+  // modes 0, 1, 5 and 6 speak the caller's AO order; the others take generic matrices as given
   std::vector<double> sin(in.size());
-  if (mode == 1) {
+  if (mode == 1 || mode == 6) {
     for (int i = 0; i < n; ++i)
       for (int j = 0; j < n; ++j) sin[(size_t)i * n + j] = in[(size_t)mp[i] * n + mp[j]];
   } else {
     sin = in;
   }
-  const size_t nout = (size_t)n * n + n + 1;
+  const size_t nout = (size_t)n * n + n + 3;
   double *d_in = nullptr, *d_out = nullptr;
   bool ok = b->alloc(d_in, std::max<size_t>(sin.size(), 1), err) && b->alloc(d_out, nout, err);
   if (ok && !sin.empty()) {
@@ -2881,15 +3294,12 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
     if (e == hipSuccess) e = hipMemcpy(so.data(), d_out, sizeof(double) * nout, hipMemcpyDeviceToHost);
     if (e != hipSuccess) { err = hipGetErrorString(e); ok = false; }
     if (ok) {
-      if (mode == 0 || mode == 1 || mode == 5) {
-        out.assign((size_t)n * n + (mode == 0 ? 1 : mode == 5 ? 2 : 0), 0.0);
+      if (mode == 0 || mode == 1 || mode == 5 || mode == 6) {
+        const int extra = mode == 0 ? 1 : mode == 5 ? 2 : mode == 6 ? 3 : 0;
+        out.assign((size_t)n * n + extra, 0.0);
         for (int i = 0; i < n; ++i)
           for (int j = 0; j < n; ++j) out[(size_t)mp[i] * n + mp[j]] = so[(size_t)i * n + j];
-        if (mode == 0) out[(size_t)n * n] = so[(size_t)n * n];
-        if (mode == 5) {
-          out[(size_t)n * n] = so[(size_t)n * n];
-          out[(size_t)n * n + 1] = so[(size_t)n * n + 1];
-        }
+        for (int k = 0; k < extra; ++k) out[(size_t)n * n + k] = so[(size_t)n * n + k];
       } else {
         out.assign(so.begin(), so.begin() + (size_t)n * n + (mode == 3 ? 0 : n));
       }

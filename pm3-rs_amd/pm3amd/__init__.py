@@ -98,7 +98,7 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_run_batch", "pm3_batch_create", "pm3_batch_solve", "pm3_batch_results", "pm3_batch_destroy",
             "pm3_batch_last_kernel_ms", "pm3_batch_gradient", "pm3_batch_last_pair_ms", "pm3_batch_last_guess_ms", "pm3_batch_guess_timing",
             "pm3_batch_phase_ticks", "pm3_stage_hcore", "pm3_stage_fock", "pm3_stage_eigen", "pm3_stage_eigen_warm",
-            "pm3_stage_gemm", "pm3_last_error_message",
+            "pm3_stage_gemm", "pm3_stage_purify", "pm3_last_error_message",
             "pm3_guess_model_create", "pm3_guess_model_load", "pm3_guess_model_save", "pm3_guess_model_get_params",
             "pm3_guess_model_set_params", "pm3_guess_model_steps", "pm3_guess_model_destroy", "pm3_guess_data_create",
             "pm3_guess_data_add", "pm3_guess_data_stats", "pm3_guess_data_destroy", "pm3_guess_train_options_default",

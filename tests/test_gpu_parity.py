@@ -170,9 +170,14 @@ def test_batch_matches_single_runs_and_oracle(G, oracle):
     dh = np.array([r.heat_of_formation_kcal for r in res]) - hof
     assert np.abs(dh).max() <= HOF_TOL
     for k in (0, 1, 50, 95):  # batch order and neighbours do not change a molecule's answer
-        r1 = G.run_pm3(G.Molecule(*mols[k]), options=G.Pm3Options(**TIGHT))
+        # (batches purify their densities; a single run without orbitals takes the same route)
+        r1 = G.run_pm3(G.Molecule(*mols[k]), options=G.Pm3Options(**TIGHT), want_orbitals=False)
         assert np.abs(r1.density - res[k].density).max() < 1e-12
         assert abs(r1.heat_of_formation_kcal - res[k].heat_of_formation_kcal) < 1e-9
+        # the eigenvector route (orbitals requested) reaches the same converged state
+        r2 = G.run_pm3(G.Molecule(*mols[k]), options=G.Pm3Options(**TIGHT))
+        assert np.abs(r2.density - res[k].density).max() <= P_TOL
+        assert abs(r2.heat_of_formation_kcal - res[k].heat_of_formation_kcal) <= HOF_TOL
     # repeated solves of the same device-resident batch give the same answers
     b.solve()
     st2, res2 = b.results()
@@ -186,15 +191,17 @@ def test_largest_single_molecules(G, oracle):
     for z, x in [W.water_cluster(32, 10.0, 2026), W.solvated_conformers(1, 2026)[0]]:
         z = np.asarray(z)
         x = W.to_bohr(x)
-        print("case nat", len(z), flush=True)
-        r = G.run_pm3(G.Molecule(z, x), options=G.Pm3Options(**TIGHT))
         st, o = oracle.run(z, x, 0.0, oracle.options(**TIGHT))
         assert st == 0
-        assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL
-        assert np.abs(r.density - o["density"]).max() <= P_TOL
-        P = r.density
-        assert abs(np.trace(P) - 2 * r.n_occ) < 1e-9
-        assert np.abs(P @ P - 2 * P).max() < 1e-9
+        for orbitals in (True, False):  # eigenvector route, purified route
+            print("case nat", len(z), "orbitals", orbitals, flush=True)
+            r = G.run_pm3(G.Molecule(z, x), options=G.Pm3Options(**TIGHT), want_orbitals=orbitals)
+            assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL
+            assert np.abs(r.density - o["density"]).max() <= P_TOL
+            assert abs(r.homo_ev - o["homo_ev"]) <= 1e-7 and abs(r.lumo_ev - o["lumo_ev"]) <= 1e-7
+            P = r.density
+            assert abs(np.trace(P) - 2 * r.n_occ) < 1e-9
+            assert np.abs(P @ P - 2 * P).max() < 1e-9
 
 
 def test_gradient_matches_oracle(G, pins, oracle):
