@@ -254,6 +254,79 @@ __device__ inline void wg_gemm(int M, int N, int K, LA la, LB lb, ST st) {
   }
 }
 
+// C = S B for n x n operands held np-strided with zero padding up to np rows and columns, S
+// symmetric (read as S[k][i], so that both operands are read along rows: 16 consecutive doubles
+// per lane group).  No bounds tests on the loads (padding supplies the zeros; rows beyond np are
+// clamped and their results dropped), and the next 16-wide k chunk is loaded while the MFMAs of
+// the current one run.  Same arithmetic order as wg_gemm, so the results are bitwise equal.
+template <bool SYM = false, class ST>
+__device__ inline void wg_gemm_pad(int n, int np, const double* S, const double* Bm, ST st) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int bm = (n + 31) >> 5;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nblk = SYM ? bm * (bm + 1) / 2 : bm * bm;
+  const int kp = (n + 15) & ~15;  // <= np
+  for (int t = wave; t < nblk; t += NW) {
+    int bi, bj;
+    if (SYM) {
+      bi = 0;
+      int rest = t;
+      while (rest >= bm - bi) { rest -= bm - bi; ++bi; }
+      bj = bi + rest;
+    } else {
+      bi = t / bm;
+      bj = t % bm;
+    }
+    const int i0 = bi << 5, j0 = bj << 5;
+    const int ia = min(i0 + li, np - 1), ib = min(i0 + li + 16, np - 1);
+    const int ja = min(j0 + li, np - 1), jb = min(j0 + li + 16, np - 1);
+    double4_t c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
+    double a0[4], a1[4], b0[4], b1[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t r = (size_t)(4 * u + lk) * np;
+      a0[u] = S[r + ia];
+      a1[u] = S[r + ib];
+      b0[u] = Bm[r + ja];
+      b1[u] = Bm[r + jb];
+    }
+    for (int k0 = 0; k0 < kp; k0 += 16) {
+      const int kn = k0 + 16 < kp ? k0 + 16 : k0;  // after the last chunk: a harmless reload
+      double na0[4], na1[4], nb0[4], nb1[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const size_t r = (size_t)(kn + 4 * u + lk) * np;
+        na0[u] = S[r + ia];
+        na1[u] = S[r + ib];
+        nb0[u] = Bm[r + ja];
+        nb1[u] = Bm[r + jb];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        c00 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b0[u], c00, 0, 0, 0);
+        c01 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], b1[u], c01, 0, 0, 0);
+        c10 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b0[u], c10, 0, 0, 0);
+        c11 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], b1[u], c11, 0, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a0[u] = na0[u];
+        a1[u] = na1[u];
+        b0[u] = nb0[u];
+        b1[u] = nb1[u];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ra = i0 + 4 * r + lk, rb = ra + 16, ca = j0 + li, cb = ca + 16;
+      if (ra < n && ca < n && (!SYM || bi < bj || ra <= ca)) { st(ra, ca, c00[r]); if (SYM && ra != ca) st(ca, ra, c00[r]); }
+      if (ra < n && cb < n && (!SYM || bi < bj || ra <= cb)) { st(ra, cb, c01[r]); if (SYM && ra != cb) st(cb, ra, c01[r]); }
+      if (rb < n && ca < n && (!SYM || bi < bj || rb <= ca)) { st(rb, ca, c10[r]); if (SYM && rb != ca) st(ca, rb, c10[r]); }
+      if (rb < n && cb < n && (!SYM || bi < bj || rb <= cb)) { st(rb, cb, c11[r]); if (SYM && rb != cb) st(cb, rb, c11[r]); }
+    }
+  }
+}
+
 // ------------------------------------------------------------------------------------------
 // One-centre two-electron integral (ab|cd) on one atom (src/fock.rs:244-282).
 __device__ inline double oc_int(int a, int b, int c, int d, const ElemConst& e) {
@@ -1122,8 +1195,7 @@ __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const doubl
     const double* Xc = X;
     double* Yc = Y;
     double w[2] = {0.0, 0.0};
-    wg_gemm<true>(n, n, n, [&](int i, int q) { return Xc[q * np + i]; }, [&](int q, int j) { return Xc[q * np + j]; },
-                  [&](int i, int j, double x2) {
+    wg_gemm_pad<true>(n, np, Xc, Xc, [&](int i, int j, double x2) {
                     const double y = sq ? x2 : 2.0 * Xc[i * np + j] - x2;
                     Yc[i * np + j] = y;
                     w[1] += y * y;
@@ -1146,39 +1218,39 @@ __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const doubl
   return k;
 }
 
-// Number of eigenvalues below x of the symmetric tridiagonal matrix (diagonal a, off-diagonal b)
-// by the Sturm sequence of its LDL^T pivots.
-__device__ inline int sturm_below(const double* a, const double* b, int m, double x, double pivmin) {
-  int cnt = 0;
-  double d = a[0] - x;
-  if (fabs(d) < pivmin) d = -pivmin;
-  cnt += d < 0.0;
+// Number of eigenvalues below x of the symmetric tridiagonal matrix (diagonal a, off-diagonal b):
+// sign changes of the Sturm sequence q_0 = 1, q_i = (a_i - x) q_{i-1} - b_{i-1}^2 q_{i-2}
+// (q_i = det(T_i - x I)), rescaled as it goes; only multiply-adds on the dependent chain.
+__device__ inline int sturm_below(const double* a, const double* b, int m, double x) {
+  double q0 = 1.0, q1 = a[0] - x;
+  if (q1 == 0.0) q1 = -1.0e-300;
+  int cnt = q1 < 0.0;
   for (int i = 1; i < m; ++i) {
-    d = a[i] - x - b[i - 1] * b[i - 1] / d;
-    if (fabs(d) < pivmin) d = -pivmin;
-    cnt += d < 0.0;
+    double q2 = (a[i] - x) * q1 - (b[i - 1] * b[i - 1]) * q0;
+    if (q2 == 0.0) q2 = q1 < 0.0 ? 1.0e-300 : -1.0e-300;
+    cnt += (q2 < 0.0) != (q1 < 0.0);
+    q0 = q1;
+    q1 = q2;
+    const double aq = fabs(q1);
+    if (aq > 1.0e100) {
+      q0 *= 1.0e-100;
+      q1 *= 1.0e-100;
+    } else if (aq < 1.0e-100 && fabs(q0) < 1.0e-100) {
+      q0 *= 1.0e100;
+      q1 *= 1.0e100;
+    }
   }
   return cnt;
 }
 
-// Largest eigenvalue of a symmetric tridiagonal matrix by multisection: the 64 lanes of the
-// calling wave test 64 points of the bracketing interval per round.
-__device__ inline double tridiag_max(const double* a, const double* b, int m) {
+// Largest eigenvalue of a symmetric tridiagonal matrix inside the bracket [lo, hi] by
+// multisection: the 64 lanes of the calling wave test 64 points of the bracket per round.
+__device__ inline double tridiag_max(const double* a, const double* b, int m, double lo, double hi) {
   const int lane = threadIdx.x & 63;
-  double lo = 1.0e300, hi = -1.0e300, bmax = 0.0;
-  for (int i = 0; i < m; ++i) {
-    const double r = (i > 0 ? fabs(b[i - 1]) : 0.0) + (i + 1 < m ? fabs(b[i]) : 0.0);
-    lo = fmin(lo, a[i] - r);
-    hi = fmax(hi, a[i] + r);
-    if (i + 1 < m) bmax = fmax(bmax, fabs(b[i]));
-  }
   const double scale = fmax(fabs(lo), fabs(hi));
-  const double pivmin = fmax(1.0e-300, 1.0e-300 * bmax * bmax);
-  lo -= 1.0e-14 * scale + 1.0e-300;
-  hi += 1.0e-14 * scale + 1.0e-300;
-  for (int round = 0; round < 16 && hi - lo > 2.0e-16 * scale; ++round) {
+  for (int round = 0; round < 24 && hi - lo > 2.0e-16 * scale; ++round) {
     const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
-    const bool above = sturm_below(a, b, m, x, pivmin) == m;  // x above the largest eigenvalue
+    const bool above = sturm_below(a, b, m, x) == m;  // x lies above the largest eigenvalue
     const unsigned long long mask = __ballot(above);
     const int first = mask ? __ffsll((long long)mask) - 1 : 64;
     const double nhi = first < 64 ? lo + (hi - lo) * (double)(first + 1) / 65.0 : hi;
@@ -1189,96 +1261,163 @@ __device__ inline double tridiag_max(const double* a, const double* b, int m) {
   return 0.5 * (lo + hi);
 }
 
-// Residual |T s - theta s| = b[m-1] |s[m-1]| of the Lanczos Ritz pair of the tridiagonal T (diagonal a,
-// off-diagonal b, size m) for its eigenvalue theta: s by two steps of inverse iteration with the
-// partially pivoted LU of T - theta I (the dgttrf / dgtts2 scheme), one thread, work = 6 (m + 2)
-// doubles.
-__device__ inline double ritz_residual(const double* a, const double* b, int m, double theta, double* work) {
-  double* d = work;
-  double* dl = d + (m + 2);
-  double* du = dl + (m + 2);
-  double* du2 = du + (m + 2);
-  double* x = du2 + (m + 2);
-  double* piv = x + (m + 2);
-  double tmax = 0.0;
-  for (int i = 0; i < m; ++i) tmax = fmax(tmax, fabs(a[i] - theta) + (i + 1 < m ? fabs(b[i]) : 0.0));
-  const double pivmin = fmax(1.0e-30 * tmax, 1.0e-290);
-  for (int i = 0; i < m; ++i) {
-    d[i] = a[i] - theta;
-    if (i + 1 < m) dl[i] = du[i] = b[i];
-    du2[i] = 0.0;
-  }
+// Residual b[m-1] |s[m-1]| of the top Ritz pair (theta, s), |s| = 1, of the Lanczos tridiagonal:
+// s by the forward three-term recurrence from s_0 = 1, which for the largest eigenvalue (above every
+// eigenvalue of the leading submatrices) follows the dominant, sign-stable solution.
+__device__ inline double ritz_residual(const double* a, const double* b, int m, double theta) {
+  double s0 = 0.0, s1 = 1.0, ss = 1.0;
   for (int i = 0; i + 1 < m; ++i) {
-    if (fabs(d[i]) >= fabs(dl[i])) {
-      if (fabs(d[i]) < pivmin) d[i] = d[i] < 0.0 ? -pivmin : pivmin;
-      const double f = dl[i] / d[i];
-      dl[i] = f;
-      d[i + 1] -= f * du[i];
-      piv[i] = 0.0;
-    } else {
-      const double f = d[i] / dl[i];
-      d[i] = dl[i];
-      dl[i] = f;
-      const double t = du[i];
-      du[i] = d[i + 1];
-      d[i + 1] = t - f * d[i + 1];
-      if (i + 2 < m) {
-        du2[i] = du[i + 1];
-        du[i + 1] = -f * du[i + 1];
-      }
-      piv[i] = 1.0;
+    const double s2 = ((theta - a[i]) * s1 - (i > 0 ? b[i - 1] * s0 : 0.0)) / b[i];
+    ss += s2 * s2;
+    s0 = s1;
+    s1 = s2;
+    if (fabs(s1) > 1.0e100) {
+      s0 *= 1.0e-100;
+      s1 *= 1.0e-100;
+      ss *= 1.0e-200;
     }
   }
-  if (fabs(d[m - 1]) < pivmin) d[m - 1] = d[m - 1] < 0.0 ? -pivmin : pivmin;
-  for (int i = 0; i < m; ++i) x[i] = 1.0;
-  for (int it = 0; it < 2; ++it) {
-    for (int i = 0; i + 1 < m; ++i) {
-      if (piv[i] == 0.0) {
-        x[i + 1] -= dl[i] * x[i];
-      } else {
-        const double t = x[i];
-        x[i] = x[i + 1];
-        x[i + 1] = t - dl[i] * x[i];
-      }
-    }
-    x[m - 1] /= d[m - 1];
-    if (m > 1) x[m - 2] = (x[m - 2] - du[m - 2] * x[m - 1]) / d[m - 2];
-    for (int i = m - 3; i >= 0; --i) x[i] = (x[i] - du[i] * x[i + 1] - du2[i] * x[i + 2]) / d[i];
-    double mx = 0.0;
-    for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(x[i]));
-    if (!(mx > 0.0) || !(mx < 1.0e300)) return 0.0;  // exactly singular: theta is an eigenvalue of T
-    double nn = 0.0;
-    for (int i = 0; i < m; ++i) {
-      x[i] /= mx;
-      nn += x[i] * x[i];
-    }
-    nn = 1.0 / sqrt(nn);
-    for (int i = 0; i < m; ++i) x[i] *= nn;
-  }
-  return fabs(b[m - 1] * x[m - 1]);
+  return fabs(b[m - 1] * s1) / sqrt(ss);
 }
 
-// LDS doubles of frontier_levels: per half a Lanczos vector, the residual, the projection
-// coefficients, the tridiagonal (diagonal, off-diagonal) and the work of ritz_residual; then the
-// per-half Ritz values and stop flags.
-__host__ __device__ constexpr int lanczos_half(int n) { return 2 * n + 9 * (n + 2); }
-__host__ __device__ constexpr int lanczos_lds(int n) { return 2 * lanczos_half(n) + 16; }
+// LDS doubles of one Lanczos run of frontier_levels: q, q_prev, w, the tridiagonal (diagonal,
+// off-diagonal) and reduction slots; the run's matrix, packed lower triangle, comes first when it
+// fits (lanczos_lds_packed), else it stays in HBM (lanczos_lds).
+__host__ __device__ constexpr int lanczos_cap(int n) { return 2 * n + 16; }
+__host__ __device__ constexpr int lanczos_lds(int n) { return 3 * n + 2 * (lanczos_cap(n) + 2) + 32; }
+__host__ __device__ constexpr int lanczos_lds_packed(int n) { return n * (n + 1) / 2 + lanczos_lds(n); }
 constexpr double LZ_TOL = 1.0e-10;  // Ritz residual (eV) at which a frontier level is taken
+
+// Largest eigenvalue of the symmetric matrix A (n x n) by a plain Lanczos run (three-term
+// recurrence, no reorthogonalisation: extreme Ritz values stay accurate, C. Paige 1976).  A is either
+// the packed lower triangle Lp in LDS (row i at i (i + 1) / 2) or, when Lp is null, the np-strided
+// square Ag in HBM.  One wave per row of each product, lanes along the row.  Every eighth step from
+// the sixteenth, wave 0 takes the top eigenvalue of the tridiagonal by multisection (the previous
+// value is a lower bound: by interlacing it never decreases) and the run stops when the Ritz residual
+// is below LZ_TOL or the value has stalled over 8 and 16 steps (a converged value gains ghost
+// copies, whose near-degeneracy spoils the residual estimate).
+__device__ double lanczos_top(const double* Lp, const double* Ag, int n, int np, double scale, double* v) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int cap = lanczos_cap(n);
+  double* q = v;
+  double* qp = q + n;
+  double* w = qp + n;
+  double* al = w + n;
+  double* be = al + (cap + 2);
+  double* sh = be + (cap + 2);  // [0, 8) alpha partials, [8, 16) beta partials, [16] value, [17] stop
+  for (int i = tid; i < n; i += NT) {
+    q[i] = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);  // a fixed spread over all basis directions
+    qp[i] = 0.0;
+  }
+  __syncthreads();
+  {
+    double nn = 0.0;
+    for (int i = lane; i < n; i += 64) nn += q[i] * q[i];
+    nn = sqrt(wave_sum(nn));  // every wave sums the same vector
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) q[i] /= nn;
+  }
+  __syncthreads();
+  double bprev = 0.0, th1 = -1.0e300, th2 = -1.0e300, top = 0.0;
+  for (int k = 0; k < cap; ++k) {
+    // w = A q - beta_prev q_prev, one row per wave
+    for (int i = wave; i < n; i += NW) {
+      double s = 0.0;
+      if (Lp) {
+        const double* row = Lp + i * (i + 1) / 2;
+        for (int j = lane; j <= i; j += 64) s += row[j] * q[j];
+        for (int j = i + 1 + lane; j < n; j += 64) s += Lp[j * (j + 1) / 2 + i] * q[j];
+      } else {
+        const double* row = Ag + (size_t)i * np;
+        for (int j = lane; j < n; j += 64) s += row[j] * q[j];
+      }
+      s = wave_sum(s);
+      if (lane == 0) w[i] = s - bprev * qp[i];
+    }
+    __syncthreads();
+    double av = 0.0;
+    for (int i = tid; i < n; i += NT) av += q[i] * w[i];
+    av = wave_sum(av);
+    if (lane == 0) sh[wave] = av;
+    __syncthreads();
+    double alpha = 0.0;
+    for (int x = 0; x < NW; ++x) alpha += sh[x];
+    double bv = 0.0;
+    for (int i = tid; i < n; i += NT) {
+      const double x = w[i] - alpha * q[i];
+      w[i] = x;
+      bv += x * x;
+    }
+    bv = wave_sum(bv);
+    if (lane == 0) sh[8 + wave] = bv;
+    __syncthreads();
+    double b2 = 0.0;
+    for (int x = 0; x < NW; ++x) b2 += sh[8 + x];
+    const double beta = sqrt(b2);
+    const int mk = k + 1;
+    const bool breakdown = beta <= 1.0e-13 * scale;
+    if (tid == 0) {
+      al[k] = alpha;
+      be[k] = beta;
+    }
+    if (!breakdown)
+      for (int i = tid; i < n; i += NT) {
+        qp[i] = q[i];
+        q[i] = w[i] / beta;
+      }
+    bprev = beta;
+    __syncthreads();
+    const bool check = mk == cap || breakdown || (mk >= 16 && (mk & 7) == 0);
+    if (!check) continue;
+    if (wave == 0) {
+      double ghi = -1.0e300, glo = 1.0e300;
+      for (int i = lane; i < mk; i += 64) {
+        const double r = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i + 1 < mk ? fabs(be[i]) : 0.0);
+        ghi = fmax(ghi, al[i] + r);
+        glo = fmin(glo, al[i] - r);
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        ghi = fmax(ghi, __shfl_xor(ghi, o, 64));
+        glo = fmin(glo, __shfl_xor(glo, o, 64));
+      }
+      const double sc = fmax(fabs(ghi), fabs(glo));
+      double blo = fmax(glo, th1) - 1.0e-14 * sc, bhi = ghi + 1.0e-14 * sc;
+      if (th2 > -1.0e299) {
+        // the values rise by shrinking steps: try a narrow bracket above the last one first
+        const double x = th1 + fmax(4.0 * (th1 - th2), 1.0e-9 * sc);
+        if (x < bhi && sturm_below(al, be, mk, x) == mk) bhi = x;
+      }
+      const double th = tridiag_max(al, be, mk, blo, bhi);
+      const bool stall = th - th1 <= 1.0e-13 * scale && th - th2 <= 1.0e-12 * scale;
+      const bool stop = mk == cap || breakdown || stall || ritz_residual(al, be, mk, th) <= LZ_TOL;
+      th2 = th1;
+      th1 = th;
+      if (lane == 0) {
+        sh[16] = th;
+        sh[17] = stop ? 1.0 : 0.0;
+      }
+    }
+    __syncthreads();
+    top = sh[16];
+    const bool stop = sh[17] != 0.0;
+    __syncthreads();
+    if (stop) break;
+  }
+  return top;
+}
 
 // HOMO and LUMO of Fu given the projector X = P / 2 onto its n_occ lowest eigenvectors, without
 // diagonalising Fu: the HOMO is the largest eigenvalue of A_o = X Fu X + c (I - X) and the LUMO the
 // smallest of A_v = (I - X) Fu (I - X) + c' X, with c below and c' above the spectrum, so that the
-// complement of each subspace sits at the far end.  Each comes from a Lanczos run with two passes
-// of full reorthogonalisation (so the basis stays orthonormal and the run is exact at dimension
-// n); every fourth step from the eighth on, the run stops once the residual of its top Ritz pair
-// is below LZ_TOL (the Ritz value is then within LZ_TOL of the level).  The two runs proceed side
-// by side on the two halves of the workgroup.  Scratch: B.T (Fu X), B.Cn (A_o), B.V (-A_v),
-// Lanczos vectors in B.Pn and B.C; LDS as lanczos_lds.  Writes eps[nocc - 1] and eps[nocc].
+// complement of each subspace sits at the far end; one Lanczos run each (lanczos_top), A_o written
+// straight into LDS (packed) and A_v to HBM, then staged into LDS for its run when it fits.
+// Scratch: B.T (Fu X), B.Cn (A_o when it does not fit), B.V (-A_v).  Writes eps[nocc - 1], eps[nocc].
 __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const double* Fu, const double* P, double* lds,
                                              double* red) {
   const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
   if (n == 0) return;
-  if (lanczos_lds(n) > M.lds_doubles) {  // the host sizes the launch's LDS for this; never taken
+  if (lanczos_lds(n) > M.lds_doubles) {  // the host sizes the launch for this; never taken
     if (tid == 0) {
       if (nocc > 0) B.eps[nocc - 1] = __builtin_nan("");
       if (nocc < n) B.eps[nocc] = __builtin_nan("");
@@ -1286,115 +1425,50 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
     __syncthreads();
     return;
   }
+  const bool packed = lanczos_lds_packed(n) <= M.lds_doubles;
+  double* Lp = packed ? lds : nullptr;
+  double* vec = packed ? lds + n * (n + 1) / 2 : lds;
   // (with n_occ = 0 or n_occ = n one side is empty and the other operator is Fu itself)
   double lo, hi;
   gershgorin(Fu, n, np, red, lo, hi);
   const double c_lo = lo - 1.0 - 0.1 * (hi - lo), c_hi = hi + 1.0 + 0.1 * (hi - lo);
+  const double scale = fmax(fabs(c_lo), fabs(c_hi));
   double* G = B.T;
   double* Ao = B.Cn;
   double* Av = B.V;
-  wg_gemm(n, n, n, [&](int i, int k) { return Fu[k * np + i]; }, [&](int k, int j) { return 0.5 * P[k * np + j]; },
-          [&](int i, int j, double v) { G[i * np + j] = v; });
+  wg_gemm_pad(n, np, Fu, P, [&](int i, int j, double v) { G[i * np + j] = 0.5 * v; });
   __syncthreads();
-  wg_gemm<true>(n, n, n, [&](int i, int k) { return 0.5 * P[k * np + i]; }, [&](int k, int j) { return G[k * np + j]; },
-                [&](int i, int j, double sv) {
+  wg_gemm_pad<true>(n, np, P, G, [&](int i, int j, double sv2) {
+                  const double sv = 0.5 * sv2;
                   const double x = 0.5 * P[i * np + j];
                   const double id = i == j ? 1.0 : 0.0;
-                  Ao[i * np + j] = sv + c_lo * (id - x);
+                  const double ao = sv + c_lo * (id - x);
+                  if (packed) {
+                    if (j <= i) Lp[i * (i + 1) / 2 + j] = ao;
+                  } else {
+                    Ao[i * np + j] = ao;
+                  }
                   Av[i * np + j] = -(Fu[i * np + j] - G[i * np + j] - G[j * np + i] + sv + c_hi * x);
                 });
   __syncthreads();
-  // half h = 0: occupied side (A_o), h = 1: virtual side (-A_v); the largest eigenvalue of each
-  const int h = tid >> 8, t = tid & 255, lane = tid & 63, wv = (tid >> 6) & 3;
-  const double* A = h == 0 ? Ao : Av;
-  double* Q = h == 0 ? B.Pn : B.C;
-  const bool active_half = h == 0 ? nocc > 0 : nocc < n;
-  double* q = lds + h * lanczos_half(n);  // current vector
-  double* w = q + n;                      // residual
-  double* hc = w + n;                     // projection coefficients
-  double* al = hc + (n + 2);              // tridiagonal diagonal
-  double* be = al + (n + 2);              // off-diagonal
-  double* work = be + (n + 2);            // ritz_residual
-  double* top = lds + 2 * lanczos_half(n);  // [h] Ritz value, [2 + h] stop flag
-  const double scale = fmax(fabs(c_lo), fabs(c_hi));
-  // start vector: a fixed spread over all basis directions, normalised
-  for (int i = t; i < n; i += 256) q[i] = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);
-  if (tid == 0) top[2] = top[3] = 0.0;
-  __syncthreads();
-  double nn = 0.0;
-  for (int i = lane; i < n; i += 64) nn += q[i] * q[i];
-  nn = sqrt(wave_sum(nn));  // every wave of the half sums the same vector
-  __syncthreads();
-  for (int i = t; i < n; i += 256) {
-    q[i] /= nn;
-    Q[i] = q[i];
-  }
-  __syncthreads();
-  bool done = !active_half;
-  for (int k = 0; k < n; ++k) {
-    const bool run = !done;
-    // w = A q (A symmetric: column reads are coalesced across rows)
-    if (run)
-      for (int i = t; i < n; i += 256) {
-        double s = 0.0;
-        for (int j = 0; j < n; ++j) s += A[j * np + i] * q[j];
-        w[i] = s;
+  double homo = 0.0, lumo = 0.0;
+  if (nocc > 0) homo = lanczos_top(Lp, Ao, n, np, scale, vec);
+  if (nocc < n) {
+    if (packed) {
+      for (int e = tid; e < n * (n + 1) / 2; e += NT) {
+        int i = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
+        while (i * (i + 1) / 2 > e) --i;
+        while ((i + 1) * (i + 2) / 2 <= e) ++i;
+        Lp[e] = Av[i * np + (e - i * (i + 1) / 2)];
       }
-    __syncthreads();
-    double alpha = 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-      // hc_j = Q_j . w for j <= k: wave wv of the half takes j = wv, wv + 4, ...
-      if (run)
-        for (int j = wv; j <= k; j += 4) {
-          double s = 0.0;
-          for (int i = lane; i < n; i += 64) s += Q[j * np + i] * w[i];
-          s = wave_sum(s);
-          if (lane == 0) hc[j] = s;
-        }
-      __syncthreads();
-      if (run)
-        for (int i = t; i < n; i += 256) {
-          double s = w[i];
-          for (int j = 0; j <= k; ++j) s -= hc[j] * Q[j * np + i];
-          w[i] = s;
-        }
-      alpha += run ? hc[k] : 0.0;
       __syncthreads();
     }
-    double bb = 0.0;
-    if (run)
-      for (int i = lane; i < n; i += 64) bb += w[i] * w[i];
-    bb = sqrt(wave_sum(bb));
-    const int mk = k + 1;
-    const bool breakdown = bb <= 1.0e-13 * scale;
-    if (run) {
-      if (t == 0) {
-        al[k] = alpha;
-        be[k] = bb;
-      }
-      if (mk < n && !breakdown)
-        for (int i = t; i < n; i += 256) {
-          const double x = w[i] / bb;
-          q[i] = x;
-          Q[mk * np + i] = x;
-        }
-    }
-    __syncthreads();
-    const bool check = run && (mk == n || breakdown || (mk >= 8 && (mk & 3) == 0));
-    if (check && wv == 0) {
-      const double th = tridiag_max(al, be, mk);
-      if (lane == 0) {
-        const bool stop = mk == n || breakdown || ritz_residual(al, be, mk, th, work) <= LZ_TOL;
-        top[h] = th;
-        top[2 + h] = stop ? 1.0 : 0.0;
-      }
-    }
-    __syncthreads();
-    if (check && top[2 + h] != 0.0) done = true;
-    if ((nocc == 0 || top[2] != 0.0) && (nocc == n || top[3] != 0.0)) break;  // both runs stopped
+    lumo = -lanczos_top(Lp, Av, n, np, scale, vec);
   }
-  if (tid == 0 && nocc > 0) B.eps[nocc - 1] = top[0];
-  if (tid == 0 && nocc < n) B.eps[nocc] = -top[1];
+  if (tid == 0) {
+    if (nocc > 0) B.eps[nocc - 1] = homo;
+    if (nocc < n) B.eps[nocc] = lumo;
+  }
   __syncthreads();
 }
 
@@ -1464,8 +1538,7 @@ __device__ void learned_into(const KArgs& A, const Mol& M, double* Pdst) {
 __device__ double commutator2(const KArgs& A, const Mol& M, const Bufs& B, double* lds, double* red) {
   const int n = M.n, np = M.np;
   build_fock(A, M, M.H, B.P, B.F, lds);
-  wg_gemm(n, n, n, [&](int i, int k) { return B.F[i * np + k]; }, [&](int k, int j) { return B.P[k * np + j]; },
-          [&](int i, int j, double v) { B.T[i * np + j] = v; });
+  wg_gemm_pad(n, np, B.F, B.P, [&](int i, int j, double v) { B.T[i * np + j] = v; });
   __syncthreads();
   double v[1] = {0.0};
   for (int e = threadIdx.x; e < n * n; e += NT) {
@@ -1614,8 +1687,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       double* En = B.Eh + ns * ms;
       double* Dn = B.Dh + ns * ms;
       // E = F P - P F (src/scf.rs:1039-1054) via T = F P
-      wg_gemm(n, n, n, [&](int i, int k) { return B.F[i * np + k]; }, [&](int k, int j) { return B.P[k * np + j]; },
-              [&](int i, int j, double v) { B.T[i * np + j] = v; });
+      wg_gemm_pad(n, np, B.F, B.P, [&](int i, int j, double v) { B.T[i * np + j] = v; });
       __syncthreads();
       // history entries are stored as packed triangles: F and D symmetric (diagonal weight 1,
       // off-diagonal weight 2 in the inner products), E antisymmetric (zero diagonal)
@@ -2851,7 +2923,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
     int jl = jac_lds_base(nao);
     if (!b.purify && fits(jac_lds_with_v(nao))) jl = jac_lds_with_v(nao);
     else if (!b.purify && fits(jac_lds_deferred(nao))) jl = jac_lds_deferred(nao);
-    if (b.purify) jl = std::max(jl, lanczos_lds(nao));
+    if (b.purify) jl = std::max(jl, fits(lanczos_lds_packed(nao)) ? lanczos_lds_packed(nao) : lanczos_lds(nao));
     const long long lneed = (long long)std::max(jl, m.n_atoms * 16) * 8;
     // molecules that do not fit one workgroup's LDS take the host-driven multi-kernel path
     const bool large = lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || req.force_large;
