@@ -146,10 +146,13 @@ double pm3_batch_last_guess_ms(const pm3_batch* batch);
 pm3_status pm3_batch_guess_timing(const pm3_batch* batch, double* features_ms, double* network_ms,
                                   int64_t* directed_pairs);
 /* Per-phase wall-clock ticks (10 ns) of the last solve summed over workgroups, collected only when
- * the environment has PM3_PHASE_TIMERS=1 at pm3_batch_create (zeros otherwise): H_core finish,
- * Fock, energy+DIIS, warm-start products, Jacobi, sort + C V, density, bookkeeping; then out[8]
- * unused, out[9] Jacobi sweeps, out[10] eigensolves, out[11] Jacobi rounds. */
-pm3_status pm3_batch_phase_ticks(const pm3_batch* batch, uint64_t* out12);
+ * the environment has PM3_PHASE_TIMERS=1 at pm3_batch_create (zeros otherwise): out[0..9] H_core
+ * finish, Fock, energy + DIIS history and Gram sums, dense products (F P, warm-start transform),
+ * eigensolve or purification, sort + C V or frontier levels, density change, bookkeeping, DIIS
+ * coefficient solve, DIIS extrapolation; out[10..12] unused; out[13] Jacobi sweeps or
+ * purification products, out[14] eigensolves or purifications, out[15] Jacobi rounds or
+ * purification fallbacks. */
+pm3_status pm3_batch_phase_ticks(const pm3_batch* batch, uint64_t* out16);
 
 /* Detail text of the calling thread's last error ("" if none). */
 const char* pm3_last_error_message(void);

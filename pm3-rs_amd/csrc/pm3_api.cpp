@@ -277,9 +277,9 @@ pm3_status pm3_batch_guess_timing(const pm3_batch* h, double* features_ms, doubl
 pm3_status pm3_batch_phase_ticks(const pm3_batch* h, uint64_t* out) {
   if (!h || !out) return fail(PM3_INVALID_INPUT, "null argument");
   std::string err;
-  unsigned long long t[12];
+  unsigned long long t[16];
   if (!batch_phase_ticks(h->b, t, err)) return fail(PM3_DEVICE_ERROR, err);
-  for (int k = 0; k < 12; ++k) out[k] = t[k];
+  for (int k = 0; k < 16; ++k) out[k] = t[k];
   return PM3_OK;
 }
 

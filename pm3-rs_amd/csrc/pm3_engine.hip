@@ -55,7 +55,9 @@ constexpr int LARGE_MAX_NAO = 16384;    // largest molecule of the host-driven p
 // SCF converges, so the converged state is unchanged.
 constexpr double EIG_TOL_FACTOR = 1.0e-2;
 constexpr double EIG_REL = 1.0e-2;  // cap of the loose tolerance relative to the starting off-diagonal
-constexpr double EIG_SKIP = 0.0;    // threshold Jacobi: skip rotations of |a_pq| <= EIG_SKIP x tolerance
+constexpr double EIG_SKIP = 0.0;
+constexpr int PH_LAST = 12;   // phase timers: slots [0, 12) phases, [12] last stamp, [13, 16) counters
+constexpr int PH_SLOTS = 16;    // threshold Jacobi: skip rotations of |a_pq| <= EIG_SKIP x tolerance
 
 __constant__ TableConst c_tab;
 
@@ -930,6 +932,7 @@ struct Diis {
   double mm[MAXD][MAXD];
   double coef[MAXD];
   int ok;
+  int mode;  // coefficient rule of the newest step: 0 none, 1 CDIIS, 2 A-DIIS
 };
 
 // Pulay CDIIS coefficients from the (scale-normalised) bordered Gram system with a pivot guard
@@ -1028,19 +1031,173 @@ __host__ __device__ __noinline__ void adiis_coeffs(const Diis& D, double* c) {
   }
 }
 
+// cdiis_coeffs on one wave: lane r holds row r of the bordered system in registers (static
+// indices only, so nothing goes to scratch).  Pivot search (first largest), row exchange,
+// elimination (row r updated with the same operations in the same order) and back substitution
+// follow the serial routine, so the coefficients are bitwise those of cdiis_coeffs.
+// Returns the same success flag on every lane.
+__device__ bool cdiis_wave(const Diis& D, double* c) {
+  constexpr int DM = MAXD + 1;
+  const int lane = threadIdx.x & 63;
+  const int n = D.len;
+  if (n < 2) return false;
+  double sc = 0.0;
+  if (lane < n)
+    for (int j = 0; j < n; ++j) sc = fmax(sc, fabs(D.b[lane][j]));
+  double scale = 0.0;
+  for (int i = 0; i < n; ++i) scale = fmax(scale, __shfl(sc, i, 64));
+  if (!(scale > 0.0) || !isfinite(scale)) return false;
+  const int dim = n + 1;
+  double mr[DM];
+  double rhs = 0.0;
+#pragma unroll
+  for (int j = 0; j < DM; ++j) {
+    double v = 0.0;
+    if (lane < n) {
+      if (j < n) v = D.b[lane][j] / scale;
+      else if (j == n) v = -1.0;
+    } else if (lane == n && j < n) {
+      v = -1.0;
+    }
+    mr[j] = v;
+  }
+  if (lane == n) rhs = -1.0;
+  for (int col = 0; col < dim; ++col) {
+    double mc = 0.0;
+#pragma unroll
+    for (int j = 0; j < DM; ++j)
+      if (j == col) mc = mr[j];
+    const double val = (lane >= col && lane < dim) ? fabs(mc) : -1.0;
+    double best = -1.0;
+    for (int r = col; r < dim; ++r) best = fmax(best, __shfl(val, r, 64));
+    if (best < 1.0e-12) return false;
+    const unsigned long long hit = __ballot(lane >= col && lane < dim && val == best);
+    const int piv = __ffsll((long long)hit) - 1;
+    if (piv != col) {
+      const int partner = lane == col ? piv : lane == piv ? col : lane;
+#pragma unroll
+      for (int j = 0; j < DM; ++j) mr[j] = __shfl(mr[j], partner, 64);
+      rhs = __shfl(rhs, partner, 64);
+    }
+    double prow[DM];
+#pragma unroll
+    for (int j = 0; j < DM; ++j) prow[j] = __shfl(mr[j], col, 64);
+    const double prhs = __shfl(rhs, col, 64);
+    double pc = 0.0, rc = 0.0;
+#pragma unroll
+    for (int j = 0; j < DM; ++j)
+      if (j == col) {
+        pc = prow[j];
+        rc = mr[j];
+      }
+    if (lane > col && lane < dim) {
+      const double f = rc / pc;
+      if (f != 0.0) {
+#pragma unroll
+        for (int j = 0; j < DM; ++j)
+          if (j >= col && j < dim) mr[j] -= f * prow[j];
+        rhs -= f * prhs;
+      }
+    }
+  }
+  double x[DM];
+#pragma unroll
+  for (int j = 0; j < DM; ++j) x[j] = 0.0;
+  for (int col = dim - 1; col >= 0; --col) {
+    double sv = rhs, dg = 1.0;
+#pragma unroll
+    for (int j = 0; j < DM; ++j) {
+      if (j > col && j < dim) sv -= mr[j] * x[j];
+      if (j == col) dg = mr[j];
+    }
+    const double xc = __shfl(sv / dg, col, 64);
+#pragma unroll
+    for (int j = 0; j < DM; ++j)
+      if (j == col) x[j] = xc;
+  }
+  if (lane < n) {
+#pragma unroll
+    for (int j = 0; j < DM; ++j)
+      if (j == lane) c[lane] = x[j];
+  }
+  return true;
+}
+
+// adiis_coeffs on one wave: lane k < len owns c[k].  Every lane keeps the serial routine's order
+// of operations (its own row of the gradient summed in j order; the stable descending sort as a
+// rank, then prefix sums over the sorted values in order; the change summed in index order), so
+// the coefficients are bitwise those of adiis_coeffs.  srt: 8 doubles of LDS.
+__device__ void adiis_wave(Diis& D, double* srt) {
+  const int lane = threadIdx.x & 63;
+  const int n = D.len, last = n - 1;
+  const int k = lane < n ? lane : 0;
+  const double mnn = D.mm[last][last];
+  const double dk = D.mm[k][last] - mnn;
+  double srow[MAXD];
+  double row = 0.0;
+#pragma unroll
+  for (int j = 0; j < MAXD; ++j) {
+    srow[j] = 0.0;
+    if (j < n) {
+      const double skj = D.mm[k][j] - D.mm[k][last] - D.mm[last][j] + mnn;
+      const double sjk = D.mm[j][k] - D.mm[j][last] - D.mm[last][k] + mnn;
+      srow[j] = skj + sjk;
+      row += fabs(srow[j]);
+    }
+  }
+  double L = 1.0e-12;
+  for (int i = 0; i < n; ++i) L = fmax(L, __shfl(row, i, 64));
+  const double lr = 1.0 / L;
+  double c = (lane == n - 1) ? 1.0 : 0.0;
+  for (int it = 0; it < 400; ++it) {
+    double acc = 2.0 * dk;
+#pragma unroll
+    for (int j = 0; j < MAXD; ++j)
+      if (j < n) acc += srow[j] * __shfl(c, j, 64);
+    const double tr = c - lr * acc;
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double uj = __shfl(tr, j, 64);
+      rank += (uj > tr || (uj == tr && j < lane)) ? 1 : 0;
+    }
+    if (lane < n) srt[rank] = tr;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    double css = 0.0;
+    for (int i = 0; i <= k; ++i) css += srt[i];
+    const double t = (css - 1.0) / (double)(k + 1);
+    const bool cond = lane < n && srt[k] - t > 0.0;
+    const unsigned long long m = __ballot(cond);
+    const double theta = m ? __shfl(t, 63 - __clzll((long long)m), 64) : 0.0;
+    const double pr = fmax(tr - theta, 0.0);
+    const double dlt = fabs(pr - c);
+    if (lane < n) c = pr;
+    double delta = 0.0;
+    for (int i = 0; i < n; ++i) delta += __shfl(dlt, i, 64);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (delta < 1.0e-12) break;
+  }
+  if (lane < n) D.coef[lane] = c;
+}
+
 // ------------------------------------------------------------------------------------------
 struct Bufs {
   double *H, *F, *P, *Pn, *Fu, *T, *C, *Cn, *V, *Pb, *Cb, *Fh, *Eh, *Dh, *eps, *epsb, *lam;
   int* perm;
-  unsigned long long* ph;  // LDS: [0..7] per-phase wall-clock sums, [8] last stamp; null when off
+  unsigned long long* ph;  // LDS: [0..11] per-phase wall-clock sums, [12] last stamp, [13..15] counters; null when off
 };
 
 // Phase stamp: adds the wall-clock ticks since the previous stamp to phase k (thread 0 only).
+// Phases: 0 H_core finish, 1 Fock build, 2 energy + DIIS history and Gram sums, 3 dense products
+// (F P and the eigensolver's warm-start transform), 4 eigensolve or purification, 5 eigenvector
+// sort and back-transform or frontier levels, 6 density change and convergence test, 7 other
+// (guess, stability search, epilogue), 8 DIIS coefficient solve, 9 DIIS extrapolation.
 __device__ inline void stamp(const Bufs& B, int k) {
   if (B.ph && threadIdx.x == 0) {
     const unsigned long long now = wall_clock64();
-    B.ph[k] += now - B.ph[8];
-    B.ph[8] = now;
+    B.ph[k] += now - B.ph[PH_LAST];
+    B.ph[PH_LAST] = now;
   }
 }
 
@@ -1073,9 +1230,9 @@ __device__ __noinline__ void eigensolve(const Mol& M, const Bufs& B, const doubl
                            : jacobi<false>(Amat, n, np, B.V, ldv, 0, B.lam, red, 60, tol_abs, M.eig_rel,
                                            jac_lds_deferred(n) <= M.lds_doubles, M.eig_skip);
   if (B.ph && threadIdx.x == 0) {
-    B.ph[9] += (unsigned long long)nsw;
-    B.ph[10] += 1ull;
-    B.ph[11] += (unsigned long long)nsw * (unsigned long long)(n + (n & 1) - 1);
+    B.ph[PH_LAST + 1] += (unsigned long long)nsw;
+    B.ph[PH_LAST + 2] += 1ull;
+    B.ph[PH_LAST + 3] += (unsigned long long)nsw * (unsigned long long)(n + (n & 1) - 1);
   }
   stamp(B, 4);
   // ascending order, ties by index
@@ -1280,29 +1437,34 @@ __device__ inline double ritz_residual(const double* a, const double* b, int m, 
   return fabs(b[m - 1] * s1) / sqrt(ss);
 }
 
-// LDS doubles of one Lanczos run of frontier_levels: q, q_prev, w, the tridiagonal (diagonal,
-// off-diagonal) and reduction slots; the run's matrix, packed lower triangle, comes first when it
-// fits (lanczos_lds_packed), else it stays in HBM (lanczos_lds).
+// LDS doubles of one Lanczos run of frontier_levels: q, q_prev, w, the product's partial sums
+// (8 slices), the tridiagonal (diagonal, off-diagonal) and reduction slots; the run's matrix comes
+// first, as a full square (lanczos_lds_square) or a packed lower triangle (lanczos_lds_packed)
+// when it fits, else it stays in HBM (lanczos_lds).
 __host__ __device__ constexpr int lanczos_cap(int n) { return 2 * n + 16; }
-__host__ __device__ constexpr int lanczos_lds(int n) { return 3 * n + 2 * (lanczos_cap(n) + 2) + 32; }
+__host__ __device__ constexpr int lanczos_lds(int n) { return 11 * n + 2 * (lanczos_cap(n) + 2) + 32; }
 __host__ __device__ constexpr int lanczos_lds_packed(int n) { return n * (n + 1) / 2 + lanczos_lds(n); }
+__host__ __device__ constexpr int lanczos_lds_square(int n) { return n * n + lanczos_lds(n); }
 constexpr double LZ_TOL = 1.0e-10;  // Ritz residual (eV) at which a frontier level is taken
 
-// Largest eigenvalue of the symmetric matrix A (n x n) by a plain Lanczos run (three-term
-// recurrence, no reorthogonalisation: extreme Ritz values stay accurate, C. Paige 1976).  A is either
-// the packed lower triangle Lp in LDS (row i at i (i + 1) / 2) or, when Lp is null, the np-strided
-// square Ag in HBM.  One wave per row of each product, lanes along the row.  Every eighth step from
+// Largest eigenvalue of the symmetric matrix A (n x n, n <= 256) by a plain Lanczos run (three-term
+// recurrence, no reorthogonalisation: extreme Ritz values stay accurate, C. Paige 1976).  A is the
+// LDS matrix La -- full square (ld n) when sq, else packed lower triangle (row i at i (i + 1) / 2) --
+// or, when La is null, the np-strided square Ag in HBM.  Products run lanes along rows (64 rows per
+// wave) and each wave over one slice of the columns, the slices' partial sums meeting in LDS: no
+// cross-lane reductions inside the product.  Every eighth step from
 // the sixteenth, wave 0 takes the top eigenvalue of the tridiagonal by multisection (the previous
 // value is a lower bound: by interlacing it never decreases) and the run stops when the Ritz residual
 // is below LZ_TOL or the value has stalled over 8 and 16 steps (a converged value gains ghost
 // copies, whose near-degeneracy spoils the residual estimate).
-__device__ double lanczos_top(const double* Lp, const double* Ag, int n, int np, double scale, double* v) {
+__device__ double lanczos_top(const double* La, bool sq, const double* Ag, int n, int np, double scale, double* v) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cap = lanczos_cap(n);
   double* q = v;
   double* qp = q + n;
   double* w = qp + n;
-  double* al = w + n;
+  double* part = w + n;  // [8][n]
+  double* al = part + 8 * n;
   double* be = al + (cap + 2);
   double* sh = be + (cap + 2);  // [0, 8) alpha partials, [8, 16) beta partials, [16] value, [17] stop
   for (int i = tid; i < n; i += NT) {
@@ -1318,25 +1480,42 @@ __device__ double lanczos_top(const double* Lp, const double* Ag, int n, int np,
     for (int i = tid; i < n; i += NT) q[i] /= nn;
   }
   __syncthreads();
+  // product layout: wave -> (row block rb of 64 rows, column slice c)
+  const int nrb = (n + 63) >> 6, nsl = NW / nrb, rb = wave % nrb, sl = wave / nrb;
+  const int row = rb * 64 + lane, slw = (n + nsl - 1) / nsl, c0 = sl * slw, c1 = min(n, c0 + slw);
+  const bool prow = sl < nsl && row < n;
   double bprev = 0.0, th1 = -1.0e300, th2 = -1.0e300, top = 0.0;
   for (int k = 0; k < cap; ++k) {
-    // w = A q - beta_prev q_prev, one row per wave
-    for (int i = wave; i < n; i += NW) {
-      double s = 0.0;
-      if (Lp) {
-        const double* row = Lp + i * (i + 1) / 2;
-        for (int j = lane; j <= i; j += 64) s += row[j] * q[j];
-        for (int j = i + 1 + lane; j < n; j += 64) s += Lp[j * (j + 1) / 2 + i] * q[j];
+    // partial (A q)_row over columns [c0, c1) (A symmetric: A[row][j] read as A[j][row])
+    if (prow) {
+      double s0 = 0.0, s1 = 0.0;
+      if (La && sq) {
+        int j = c0;
+        for (; j + 1 < c1; j += 2) {
+          s0 += La[j * n + row] * q[j];
+          s1 += La[(j + 1) * n + row] * q[j + 1];
+        }
+        if (j < c1) s0 += La[j * n + row] * q[j];
+      } else if (La) {
+        for (int j = c0; j < c1; ++j) s0 += La[j <= row ? row * (row + 1) / 2 + j : j * (j + 1) / 2 + row] * q[j];
       } else {
-        const double* row = Ag + (size_t)i * np;
-        for (int j = lane; j < n; j += 64) s += row[j] * q[j];
+        int j = c0;
+        for (; j + 1 < c1; j += 2) {
+          s0 += Ag[(size_t)j * np + row] * q[j];
+          s1 += Ag[(size_t)(j + 1) * np + row] * q[j + 1];
+        }
+        if (j < c1) s0 += Ag[(size_t)j * np + row] * q[j];
       }
-      s = wave_sum(s);
-      if (lane == 0) w[i] = s - bprev * qp[i];
+      part[sl * n + row] = s0 + s1;
     }
     __syncthreads();
     double av = 0.0;
-    for (int i = tid; i < n; i += NT) av += q[i] * w[i];
+    for (int i = tid; i < n; i += NT) {
+      double x = -bprev * qp[i];
+      for (int c = 0; c < nsl; ++c) x += part[c * n + i];
+      w[i] = x;
+      av += q[i] * x;
+    }
     av = wave_sum(av);
     if (lane == 0) sh[wave] = av;
     __syncthreads();
@@ -1425,9 +1604,10 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
     __syncthreads();
     return;
   }
-  const bool packed = lanczos_lds_packed(n) <= M.lds_doubles;
+  const bool square = lanczos_lds_square(n) <= M.lds_doubles;
+  const bool packed = square || lanczos_lds_packed(n) <= M.lds_doubles;
   double* Lp = packed ? lds : nullptr;
-  double* vec = packed ? lds + n * (n + 1) / 2 : lds;
+  double* vec = square ? lds + n * n : packed ? lds + n * (n + 1) / 2 : lds;
   // (with n_occ = 0 or n_occ = n one side is empty and the other operator is Fu itself)
   double lo, hi;
   gershgorin(Fu, n, np, red, lo, hi);
@@ -1443,7 +1623,9 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
                   const double x = 0.5 * P[i * np + j];
                   const double id = i == j ? 1.0 : 0.0;
                   const double ao = sv + c_lo * (id - x);
-                  if (packed) {
+                  if (square) {
+                    Lp[i * n + j] = ao;
+                  } else if (packed) {
                     if (j <= i) Lp[i * (i + 1) / 2 + j] = ao;
                   } else {
                     Ao[i * np + j] = ao;
@@ -1452,9 +1634,12 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
                 });
   __syncthreads();
   double homo = 0.0, lumo = 0.0;
-  if (nocc > 0) homo = lanczos_top(Lp, Ao, n, np, scale, vec);
+  if (nocc > 0) homo = lanczos_top(Lp, square, Ao, n, np, scale, vec);
   if (nocc < n) {
-    if (packed) {
+    if (square) {
+      for (int e = tid; e < n * n; e += NT) Lp[e] = Av[(e / n) * np + e % n];
+      __syncthreads();
+    } else if (packed) {
       for (int e = tid; e < n * (n + 1) / 2; e += NT) {
         int i = (int)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
         while (i * (i + 1) / 2 > e) --i;
@@ -1463,7 +1648,7 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
       }
       __syncthreads();
     }
-    lumo = -lanczos_top(Lp, Av, n, np, scale, vec);
+    lumo = -lanczos_top(Lp, square, Av, n, np, scale, vec);
   }
   if (tid == 0) {
     if (nocc > 0) B.eps[nocc - 1] = homo;
@@ -1687,8 +1872,10 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       double* En = B.Eh + ns * ms;
       double* Dn = B.Dh + ns * ms;
       // E = F P - P F (src/scf.rs:1039-1054) via T = F P
+      stamp(B, 2);
       wg_gemm_pad(n, np, B.F, B.P, [&](int i, int j, double v) { B.T[i * np + j] = v; });
       __syncthreads();
+      stamp(B, 3);
       // history entries are stored as packed triangles: F and D symmetric (diagonal weight 1,
       // off-diagonal weight 2 in the inner products), E antisymmetric (zero diagonal)
       const int npk = n * (n + 1) / 2;
@@ -1725,6 +1912,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
         }
       }
       block_sum<3 * MAXD + 2>(v, red);
+      stamp(B, 2);
       if (tid == 0) {
         for (int i = 0; i < len; ++i) {
           D.b[i][len] = v[i];
@@ -1738,16 +1926,26 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
         D.len = len + 1;
         const double en = sqrt(v[3 * MAXD]);
         D.en = en;
-        bool ok;
         if (accel == 2 && en > A.adiis_switch) {
-          ok = D.len >= 2 && D.keep_d;
-          if (ok) adiis_coeffs(D, D.coef);
+          D.mode = (D.len >= 2 && D.keep_d) ? 2 : 0;
+          D.ok = D.mode == 2;
         } else {
-          ok = cdiis_coeffs(D, D.coef);
+          D.mode = 1;
         }
-        D.ok = ok;
       }
       __syncthreads();
+      if (D.mode != 0) {  // coefficients on wave 0
+        if (tid < 64) {
+          if (D.mode == 2) {
+            adiis_wave(D, red);
+          } else {
+            const bool ok = cdiis_wave(D, D.coef);
+            if (tid == 0) D.ok = ok;
+          }
+        }
+        __syncthreads();
+      }
+      stamp(B, 8);
       if (D.ok) {
         const int L = D.len;
         for (int e = tid; e < npk; e += NT) {
@@ -1763,6 +1961,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       } else {
         Fuse = B.F;  // the newest entry's Fock matrix
       }
+      stamp(B, 9);
     }
     stamp(B, 2);
     Flast = Fuse;
@@ -1773,9 +1972,9 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
         density(M, B.Cn, B.Pn);
       }
       if (B.ph && tid == 0) {
-        B.ph[9] += (unsigned long long)max(ns, 0);
-        B.ph[10] += 1ull;
-        B.ph[11] += ns < 0 ? 1ull : 0ull;
+        B.ph[PH_LAST + 1] += (unsigned long long)max(ns, 0);
+        B.ph[PH_LAST + 2] += 1ull;
+        B.ph[PH_LAST + 3] += ns < 0 ? 1ull : 0ull;
       }
       stamp(B, 4);
     } else {
@@ -2067,19 +2266,19 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
   __shared__ double red[NW * (3 * MAXD + 2)];
   __shared__ Diis D;
   __shared__ int s_mol;
-  __shared__ unsigned long long s_ph[12];
+  __shared__ unsigned long long s_ph[PH_SLOTS];
   const int tid = threadIdx.x;
   double* Wbase;
   Bufs B = slab(A, Wbase);
   B.ph = A.prof ? s_ph : nullptr;
-  if (tid < 12) s_ph[tid] = 0;
-  if (tid == 0) s_ph[8] = wall_clock64();
+  if (tid < PH_SLOTS) s_ph[tid] = 0;
+  if (tid == 0) s_ph[PH_LAST] = wall_clock64();
 
   for (;;) {
     if (tid == 0) s_mol = atomicAdd(A.queue, 1);
     if (tid == 0 && A.prof) {
-      for (int k = 0; k < 12; ++k) {
-        if (k == 8) continue;
+      for (int k = 0; k < PH_SLOTS; ++k) {
+        if (k == PH_LAST) continue;
         atomicAdd(&A.prof[k], s_ph[k]);
         s_ph[k] = 0;
       }
@@ -2923,7 +3122,10 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
     int jl = jac_lds_base(nao);
     if (!b.purify && fits(jac_lds_with_v(nao))) jl = jac_lds_with_v(nao);
     else if (!b.purify && fits(jac_lds_deferred(nao))) jl = jac_lds_deferred(nao);
-    if (b.purify) jl = std::max(jl, fits(lanczos_lds_packed(nao)) ? lanczos_lds_packed(nao) : lanczos_lds(nao));
+    if (b.purify)
+      jl = std::max(jl, fits(lanczos_lds_square(nao))   ? lanczos_lds_square(nao)
+                        : fits(lanczos_lds_packed(nao)) ? lanczos_lds_packed(nao)
+                                                        : lanczos_lds(nao));
     const long long lneed = (long long)std::max(jl, m.n_atoms * 16) * 8;
     // molecules that do not fit one workgroup's LDS take the host-driven multi-kernel path
     const bool large = lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || req.force_large;
@@ -3104,7 +3306,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   if (const char* ev = std::getenv("PM3_PHASE_TIMERS")) {
     if (ev[0] == '1') {
       unsigned long long* d_prof;
-      if (!b.alloc(d_prof, 12, err)) return false;
+      if (!b.alloc(d_prof, PH_SLOTS, err)) return false;
       a.prof = d_prof;
     }
   }
@@ -3225,9 +3427,9 @@ double batch_kernel_ms(const Batch* b) { return b ? b->last_kernel_ms : 0.0; }
 double batch_pair_ms(const Batch* b) { return b ? b->last_pair_ms : 0.0; }
 
 bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err) {
-  for (int k = 0; k < 12; ++k) out[k] = 0;
+  for (int k = 0; k < PH_SLOTS; ++k) out[k] = 0;
   if (!b || !b->args.prof) return true;
-  HIP_OK(hipMemcpy(out, b->args.prof, 12 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(out, b->args.prof, PH_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   return true;
 }
 

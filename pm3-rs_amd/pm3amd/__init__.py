@@ -378,14 +378,16 @@ class Batch:
         _raise(native().pm3_batch_guess_timing(self._h, ctypes.byref(f), ctypes.byref(n), ctypes.byref(p)))
         return f.value, n.value, p.value
 
-    PHASES = ("hcore", "fock", "energy_diis", "warm_products", "jacobi", "sort_cv", "density", "other")
+    PHASES = ("hcore", "fock", "energy_diis_gram", "products", "eigen_or_purify", "sort_or_frontier", "density",
+              "other", "diis_solve", "diis_combine")
 
     def phase_ticks(self):
         """Per-phase wall-clock ticks (10 ns) of the last solve (PM3_PHASE_TIMERS=1 at creation)."""
-        out = (ctypes.c_uint64 * 12)()
+        out = (ctypes.c_uint64 * 16)()
         _raise(native().pm3_batch_phase_ticks(self._h, out))
-        d = dict(zip(self.PHASES, [int(v) for v in out[:8]]))
-        d["counts"] = {"jacobi_sweeps": int(out[9]), "eigensolves": int(out[10]), "jacobi_rounds": int(out[11])}
+        d = dict(zip(self.PHASES, [int(v) for v in out[:10]]))
+        d["counts"] = {"sweeps_or_products": int(out[13]), "eigensolves_or_purifications": int(out[14]),
+                       "rounds_or_fallbacks": int(out[15])}
         return d
 
     def results(self):
