@@ -47,6 +47,7 @@ namespace {
 constexpr int NT = 512;       // threads per workgroup
 constexpr int NW = NT / 64;   // waves per workgroup
 constexpr int MAXD = 8;       // maximum DIIS depth (src/scf.rs:309)
+constexpr int RED_K = 3 * MAXD + 3;  // most values one block_sum reduces (the DIIS history pass)
 constexpr int LDS_BYTES_MAX = 160 * 1024;
 constexpr int STATIC_LDS_BYTES = 4096;  // upper bound on the kernel's static __shared__ use
 constexpr int LARGE_MAX_NAO = 16384;    // largest molecule of the host-driven path (12 n^2 doubles)
@@ -137,7 +138,9 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
+// red holds NW * K doubles (the SCF kernels declare NW * RED_K).
 template <int K> __device__ inline void block_sum(double (&v)[K], double* red) {
+  static_assert(K <= RED_K, "reduction buffer holds RED_K values per wave");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
@@ -932,7 +935,8 @@ struct Diis {
   double mm[MAXD][MAXD];
   double coef[MAXD];
   int ok;
-  int mode;  // coefficient rule of the newest step: 0 none, 1 CDIIS, 2 A-DIIS
+  int mode;      // coefficient rule of the newest step: 0 none, 1 CDIIS, 2 A-DIIS
+  int mm_stale;  // some <D_i, F_j> entries were skipped (only A-DIIS reads them)
 };
 
 // Pulay CDIIS coefficients from the (scale-normalised) bordered Gram system with a pivot guard
@@ -1830,6 +1834,8 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
     D.len = 0;
     D.depth = depth;
     D.keep_d = keep_d;
+    D.mm_stale = 0;
+    D.en = 1.0e300;
   }
   __syncthreads();
   PathResult R{0.0, 1.0e300, 0, 0, gused};
@@ -1842,12 +1848,14 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
     stamp(B, 7);
     build_fock(A, M, M.H, B.P, B.F, lds);
     stamp(B, 1);
-    double ev[1] = {0.0};
-    for (int e = tid; e < np * np; e += NT) ev[0] += B.P[e] * (M.H[e] + B.F[e]);
-    block_sum<1>(ev, red);
-    const double e_elec = 0.5 * ev[0];
+    double e_elec = 0.0;
     const double* Fuse = B.F;
-    if (accel != 0) {
+    if (accel == 0) {
+      double ev[1] = {0.0};
+      for (int e = tid; e < np * np; e += NT) ev[0] += B.P[e] * (M.H[e] + B.F[e]);
+      block_sum<1>(ev, red);
+      e_elec = 0.5 * ev[0];
+    } else {
       // choose the slot of the new entry (drop the oldest when full)
       if (tid == 0) {
         if (D.len == D.depth) {
@@ -1876,42 +1884,47 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       wg_gemm_pad(n, np, B.F, B.P, [&](int i, int j, double v) { B.T[i * np + j] = v; });
       __syncthreads();
       stamp(B, 3);
-      // history entries are stored as packed triangles: F and D symmetric (diagonal weight 1,
-      // off-diagonal weight 2 in the inner products), E antisymmetric (zero diagonal)
+      // One pass over the packed triangle: the new history entry (F and D symmetric, E
+      // antisymmetric with zero diagonal; diagonal weight 1, off-diagonal weight 2 in the inner
+      // products), the energy P.(H + F) / 2, and the Gram updates <E_i,E_new>, <D_new,F_j>,
+      // <D_i,F_new> for the existing entries plus <E_new,E_new>, <D_new,F_new> (index len).  The
+      // <D,F> entries feed only A-DIIS: they are skipped while the previous error is well below
+      // the A-DIIS switch, and recomputed in full should A-DIIS be needed later.
       const int npk = n * (n + 1) / 2;
+      const int len = D.len;
+      const bool want_mm = keep_d && accel == 2 && !D.mm_stale && D.en > 0.25 * A.adiis_switch;
+      double v[3 * MAXD + 3];
+#pragma unroll
+      for (int k = 0; k < 3 * MAXD + 3; ++k) v[k] = 0.0;
       for (int e = tid; e < npk; e += NT) {
         int i, j;
         tri_unpack(e, n, i, j);
-        En[e] = (i == j) ? 0.0 : B.T[i * np + j] - B.T[j * np + i];
-        Fn[e] = B.F[i * np + j];
-        if (keep_d) Dn[e] = B.P[i * np + j];
-      }
-      __syncthreads();
-      // Gram updates: <E_i,E_new>, <D_new,F_j>, <D_i,F_new> for the existing entries, plus
-      // <E_new,E_new> and <D_new,F_new> (index len)
-      const int len = D.len;
-      double v[3 * MAXD + 2];
-#pragma unroll
-      for (int k = 0; k < 3 * MAXD + 2; ++k) v[k] = 0.0;
-      for (int e = tid; e < npk; e += NT) {
+        const double en = (i == j) ? 0.0 : B.T[i * np + j] - B.T[j * np + i];
+        const double fn = B.F[i * np + j];
+        const double pn = B.P[i * np + j];
+        En[e] = en;
+        Fn[e] = fn;
+        if (keep_d) Dn[e] = pn;
         const double wgt = e < n ? 1.0 : 2.0;
-        const double en = En[e], fn = Fn[e], dn = keep_d ? Dn[e] : 0.0;
+        const double dn = keep_d ? pn : 0.0;
         const double enw = wgt * en, fnw = wgt * fn, dnw = wgt * dn;
+        v[3 * MAXD + 2] += wgt * pn * (M.H[i * np + j] + fn);
         v[3 * MAXD] += enw * en;
-        v[3 * MAXD + 1] += dnw * fn;
+        if (want_mm) v[3 * MAXD + 1] += dnw * fn;
 #pragma unroll
-        for (int i = 0; i < MAXD; ++i) {
-          if (i < len) {
-            const long long so = (long long)D.slot[i] * ms;
-            v[i] += B.Eh[so + e] * enw;
-            if (keep_d) {
-              v[MAXD + i] += dnw * B.Fh[so + e];
-              v[2 * MAXD + i] += B.Dh[so + e] * fnw;
+        for (int q = 0; q < MAXD; ++q) {
+          if (q < len) {
+            const long long so = (long long)D.slot[q] * ms;
+            v[q] += B.Eh[so + e] * enw;
+            if (want_mm) {
+              v[MAXD + q] += dnw * B.Fh[so + e];
+              v[2 * MAXD + q] += B.Dh[so + e] * fnw;
             }
           }
         }
       }
-      block_sum<3 * MAXD + 2>(v, red);
+      block_sum<3 * MAXD + 3>(v, red);
+      e_elec = 0.5 * v[3 * MAXD + 2];
       stamp(B, 2);
       if (tid == 0) {
         for (int i = 0; i < len; ++i) {
@@ -1924,6 +1937,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
         D.mm[len][len] = v[3 * MAXD + 1];
         D.slot[len] = ns;
         D.len = len + 1;
+        if (keep_d && accel == 2 && !want_mm) D.mm_stale = 1;
         const double en = sqrt(v[3 * MAXD]);
         D.en = en;
         if (accel == 2 && en > A.adiis_switch) {
@@ -1932,6 +1946,28 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
         } else {
           D.mode = 1;
         }
+      }
+      __syncthreads();
+      if (D.mode == 2 && D.mm_stale) {
+        // A-DIIS after skipped steps: every <D_i, F_j> again, summed as in the pass above
+        const int L = D.len;
+        for (int a = 0; a < L; ++a) {
+          double w[MAXD];
+#pragma unroll
+          for (int q = 0; q < MAXD; ++q) w[q] = 0.0;
+          const double* Da = B.Dh + (long long)D.slot[a] * ms;
+          for (int e = tid; e < npk; e += NT) {
+            const double dw = (e < n ? 1.0 : 2.0) * Da[e];
+#pragma unroll
+            for (int q = 0; q < MAXD; ++q)
+              if (q < L) w[q] += dw * B.Fh[(long long)D.slot[q] * ms + e];
+          }
+          block_sum<MAXD>(w, red);
+          if (tid == 0)
+            for (int q = 0; q < L; ++q) D.mm[a][q] = w[q];
+          __syncthreads();
+        }
+        if (tid == 0) D.mm_stale = 0;
       }
       __syncthreads();
       if (D.mode != 0) {  // coefficients on wave 0
@@ -2263,7 +2299,7 @@ __device__ inline void charges_dipole(const KArgs& A, const Mol& M, const double
 
 __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
   double* lds = g_lds;
-  __shared__ double red[NW * (3 * MAXD + 2)];
+  __shared__ double red[NW * RED_K];
   __shared__ Diis D;
   __shared__ int s_mol;
   __shared__ unsigned long long s_ph[PH_SLOTS];
@@ -2379,7 +2415,7 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
 // Matrices in `in`/`out` are dense n x n row-major in the engine's (heavy-first) AO order.
 __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const double* in, double* out) {
   double* lds = g_lds;
-  __shared__ double red[NW * (3 * MAXD + 2)];
+  __shared__ double red[NW * RED_K];
   const int tid = threadIdx.x;
   double* Wbase;
   const Bufs B = slab(A, Wbase);
