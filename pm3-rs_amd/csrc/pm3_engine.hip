@@ -1306,6 +1306,91 @@ __device__ inline void gershgorin(const double* F, int n, int np, double* red, d
   __syncthreads();
 }
 
+constexpr int SP2_MAX = 90;
+constexpr int SP2_EXTRA = 2;
+constexpr double SP2_TAIL = 1.0e-7;
+
+// LDS image of an n x n operand for the products: kp = n rounded up to 16 rows (zero beyond n) of
+// sp2_ld(n) doubles, a stride of 16 mod 32 so that the four k rows one load touches fall in
+// different banks.
+__host__ __device__ constexpr int sp2_kp(int n) { return (n + 15) & ~15; }
+__host__ __device__ constexpr int sp2_ld(int n) { return (sp2_kp(n) & 31) == 0 ? sp2_kp(n) + 16 : sp2_kp(n); }
+__host__ __device__ constexpr int sp2_img(int n) { return sp2_kp(n) * sp2_ld(n); }
+
+// The purification loop for one placement of the operands: LM = 2 both X and the product in LDS
+// (ping-pong of two images), LM = 1 X in LDS and the product to HBM (B.T, copied back into the
+// image after each step), LM = 0 both in HBM (B.Cn / B.T).
+template <int LM>
+__device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo, double hi) {
+  const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
+  const int img = sp2_img(n), ldl = sp2_ld(n);
+  const double s = 1.0 / (hi - lo);
+  double* X = LM ? g_lds : B.Cn;
+  double* Y = LM == 2 ? g_lds + img : B.T;
+  const int ldx = LM ? ldl : np, ldy = LM == 2 ? ldl : np;
+  double v[2] = {0.0, 0.0};
+  if (LM) {
+    for (int e = tid; e < (LM == 2 ? 2 * img : img); e += NT) {
+      const int r = (e % img) / ldl, c = e % ldl;
+      double x = 0.0;
+      if (e < img && r < n && c < n) {
+        x = ((r == c ? hi : 0.0) - Fu[r * np + c]) * s;
+        v[1] += x * x;
+        if (r == c) v[0] += x;
+      }
+      g_lds[e] = x;
+    }
+  } else {
+    for (int e = tid; e < n * n; e += NT) {
+      const int i = e / n, j = e % n;
+      const double x = ((i == j ? hi : 0.0) - Fu[i * np + j]) * s;
+      X[i * np + j] = x;
+      v[1] += x * x;
+      if (i == j) v[0] += x;
+    }
+  }
+  block_sum<2>(v, red);
+  double tr = v[0], t2 = v[1];
+  const double occ = (double)nocc;
+  int tail = -1, k = 0;
+  for (; k < SP2_MAX; ++k) {
+    if (tail < 0 && tr - t2 < SP2_TAIL) tail = SP2_EXTRA;
+    if (tail == 0) break;
+    const bool sq = fabs(t2 - occ) < fabs(2.0 * tr - t2 - occ);
+    const double* Xc = X;
+    double* Yc = Y;
+    double w[2] = {0.0, 0.0};
+    wg_gemm_pad<true>(n, ldx, Xc, Xc, [&](int i, int j, double x2) {
+      const double y = sq ? x2 : 2.0 * Xc[i * ldx + j] - x2;
+      Yc[i * ldy + j] = y;
+      w[1] += y * y;
+      if (i == j) w[0] += y;
+    });
+    block_sum<2>(w, red);
+    tr = w[0];
+    t2 = w[1];
+    if (LM == 1) {
+      for (int e = tid; e < n * n; e += NT) {
+        const int i = e / n, j = e % n;
+        X[i * ldl + j] = Y[i * np + j];
+      }
+      __syncthreads();
+    } else {
+      double* t = X;
+      X = Y;
+      Y = t;
+    }
+    if (tail > 0) --tail;
+  }
+  if (tail != 0) return -1;
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e % n;
+    Pdst[i * np + j] = 2.0 * X[i * ldx + j];
+  }
+  __syncthreads();
+  return k;
+}
+
 // Aufbau density of the symmetric matrix Fu by purification instead of eigenvectors: the
 // trace-correcting second-order spectral projection of Niklasson (Phys. Rev. B 66, 155115, 2002).
 // X0 = (e_max I - Fu) / (e_max - e_min) with Gershgorin bounds maps the spectrum into [0, 1] in
@@ -1315,12 +1400,9 @@ __device__ inline void gershgorin(const double* F, int n, int np, double* red, d
 // tr(X^2) = |X|_F^2 for symmetric X, so the trace and the norm of each new X are summed in the
 // product's store.  The iteration stops SP2_EXTRA products after the idempotency error
 // tr(X) - tr(X^2) has fallen below SP2_TAIL (each further product squares the distance of the
-// eigenvalues on one side from 0 or 1).  X ping-pongs between B.Cn and B.T; P goes to Pdst.
-// Returns the number of products, or -1 when the error has not fallen within SP2_MAX products
-// (frontier orbitals degenerate or nearly so); the caller then diagonalises.
-constexpr int SP2_MAX = 90;
-constexpr int SP2_EXTRA = 2;
-constexpr double SP2_TAIL = 1.0e-7;
+// eigenvalues on one side from 0 or 1).  The operands live in LDS when they fit (sp2_run); P goes
+// to Pdst.  Returns the number of products, or -1 when the error has not fallen within SP2_MAX
+// products (frontier orbitals degenerate or nearly so); the caller then diagonalises.
 __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red) {
   const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
   if (nocc == 0 || nocc == n) {
@@ -1334,49 +1416,10 @@ __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const doubl
   double lo, hi;
   gershgorin(Fu, n, np, red, lo, hi);
   if (!(hi - lo > 1.0e-12 * fmax(1.0, fabs(hi)))) return -1;
-  const double s = 1.0 / (hi - lo);
-  double* X = B.Cn;
-  double* Y = B.T;
-  double v[2] = {0.0, 0.0};
-  for (int e = tid; e < n * n; e += NT) {
-    const int i = e / n, j = e % n;
-    const double x = ((i == j ? hi : 0.0) - Fu[i * np + j]) * s;
-    X[i * np + j] = x;
-    v[1] += x * x;
-    if (i == j) v[0] += x;
-  }
-  block_sum<2>(v, red);
-  double tr = v[0], t2 = v[1];
-  const double occ = (double)nocc;
-  int tail = -1, k = 0;
-  for (; k < SP2_MAX; ++k) {
-    if (tail < 0 && tr - t2 < SP2_TAIL) tail = SP2_EXTRA;
-    if (tail == 0) break;
-    const bool sq = fabs(t2 - occ) < fabs(2.0 * tr - t2 - occ);
-    const double* Xc = X;
-    double* Yc = Y;
-    double w[2] = {0.0, 0.0};
-    wg_gemm_pad<true>(n, np, Xc, Xc, [&](int i, int j, double x2) {
-                    const double y = sq ? x2 : 2.0 * Xc[i * np + j] - x2;
-                    Yc[i * np + j] = y;
-                    w[1] += y * y;
-                    if (i == j) w[0] += y;
-                  });
-    block_sum<2>(w, red);
-    tr = w[0];
-    t2 = w[1];
-    double* t = X;
-    X = Y;
-    Y = t;
-    if (tail > 0) --tail;
-  }
-  if (tail != 0) return -1;
-  for (int e = tid; e < n * n; e += NT) {
-    const int i = e / n, j = e % n;
-    Pdst[i * np + j] = 2.0 * X[i * np + j];
-  }
-  __syncthreads();
-  return k;
+  const int img = sp2_img(n);
+  if (2 * img <= M.lds_doubles) return sp2_run<2>(M, B, Fu, Pdst, red, lo, hi);
+  if (img <= M.lds_doubles) return sp2_run<1>(M, B, Fu, Pdst, red, lo, hi);
+  return sp2_run<0>(M, B, Fu, Pdst, red, lo, hi);
 }
 
 // Number of eigenvalues below x of the symmetric tridiagonal matrix (diagonal a, off-diagonal b):
@@ -3187,6 +3230,10 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   }
   b.atoms_total = (long long)aslot.size();
   b.nsolve = (int)b.solve.size();
+  // One 512-thread workgroup per CU is resident whatever its LDS (its registers decide), so the
+  // launch takes all of it: eigenvectors, purification operands and Lanczos matrices then stay in
+  // LDS for as many molecules as they fit.
+  if (b.nsolve > 0) b.lds_need = LDS_BYTES_MAX - STATIC_LDS_BYTES;
   // Directed pairs of the learned guess (molecules made of H, C and O only): for every source
   // atom in the engine's order, its neighbours inside 4 Angstrom in the caller's atom order.
   std::vector<int8_t> elem8(b.atoms_total, (int8_t)-1);
