@@ -1504,64 +1504,66 @@ constexpr double LZ_TOL = 1.0e-10;  // Ritz residual (eV) at which a frontier le
 // value is a lower bound: by interlacing it never decreases) and the run stops when the Ritz residual
 // is below LZ_TOL or the value has stalled over 8 and 16 steps (a converged value gains ghost
 // copies, whose near-degeneracy spoils the residual estimate).
-__device__ double lanczos_top(const double* La, bool sq, const double* Ag, int n, int np, double scale, double* v) {
+__device__ __forceinline__ double lanczos_top(const double* La, bool sq, const double* Ag, int n, int np, double scale,
+                                              double* v) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cap = lanczos_cap(n);
-  double* q = v;
-  double* qp = q + n;
-  double* w = qp + n;
-  double* part = w + n;  // [8][n]
+  double* q = v;         // current basis vector q_k
+  double* w = q + n;     // next basis vector before its normalisation by wn
+  double* part = w + 2 * n;  // [8][n]
   double* al = part + 8 * n;
   double* be = al + (cap + 2);
   double* sh = be + (cap + 2);  // [0, 8) alpha partials, [8, 16) beta partials, [16] value, [17] stop
   for (int i = tid; i < n; i += NT) {
-    q[i] = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);  // a fixed spread over all basis directions
-    qp[i] = 0.0;
+    w[i] = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);  // a fixed spread over all basis directions
+    q[i] = 0.0;
   }
   __syncthreads();
+  double wn;
   {
     double nn = 0.0;
-    for (int i = lane; i < n; i += 64) nn += q[i] * q[i];
-    nn = sqrt(wave_sum(nn));  // every wave sums the same vector
-    __syncthreads();
-    for (int i = tid; i < n; i += NT) q[i] /= nn;
+    for (int i = lane; i < n; i += 64) nn += w[i] * w[i];
+    wn = sqrt(wave_sum(nn));  // every wave sums the same vector
   }
-  __syncthreads();
   // product layout: wave -> (row block rb of 64 rows, column slice c)
   const int nrb = (n + 63) >> 6, nsl = NW / nrb, rb = wave % nrb, sl = wave / nrb;
   const int row = rb * 64 + lane, slw = (n + nsl - 1) / nsl, c0 = sl * slw, c1 = min(n, c0 + slw);
   const bool prow = sl < nsl && row < n;
-  double bprev = 0.0, th1 = -1.0e300, th2 = -1.0e300, top = 0.0;
+  double bsub = 0.0, th1 = -1.0e300, th2 = -1.0e300, top = 0.0;
   for (int k = 0; k < cap; ++k) {
-    // partial (A q)_row over columns [c0, c1) (A symmetric: A[row][j] read as A[j][row])
+    // partial (A w)_row over columns [c0, c1) (A symmetric: A[row][j] read as A[j][row])
     if (prow) {
       double s0 = 0.0, s1 = 0.0;
       if (La && sq) {
         int j = c0;
         for (; j + 1 < c1; j += 2) {
-          s0 += La[j * n + row] * q[j];
-          s1 += La[(j + 1) * n + row] * q[j + 1];
+          s0 += La[j * n + row] * w[j];
+          s1 += La[(j + 1) * n + row] * w[j + 1];
         }
-        if (j < c1) s0 += La[j * n + row] * q[j];
+        if (j < c1) s0 += La[j * n + row] * w[j];
       } else if (La) {
-        for (int j = c0; j < c1; ++j) s0 += La[j <= row ? row * (row + 1) / 2 + j : j * (j + 1) / 2 + row] * q[j];
+        for (int j = c0; j < c1; ++j) s0 += La[j <= row ? row * (row + 1) / 2 + j : j * (j + 1) / 2 + row] * w[j];
       } else {
         int j = c0;
         for (; j + 1 < c1; j += 2) {
-          s0 += Ag[(size_t)j * np + row] * q[j];
-          s1 += Ag[(size_t)(j + 1) * np + row] * q[j + 1];
+          s0 += Ag[(size_t)j * np + row] * w[j];
+          s1 += Ag[(size_t)(j + 1) * np + row] * w[j + 1];
         }
-        if (j < c1) s0 += Ag[(size_t)j * np + row] * q[j];
+        if (j < c1) s0 += Ag[(size_t)j * np + row] * w[j];
       }
       part[sl * n + row] = s0 + s1;
     }
     __syncthreads();
+    // q_k = w / wn; w = A q_k - beta_{k-1} q_{k-1}; alpha_k = q_k . w
     double av = 0.0;
     for (int i = tid; i < n; i += NT) {
-      double x = -bprev * qp[i];
-      for (int c = 0; c < nsl; ++c) x += part[c * n + i];
+      double pa = 0.0;
+      for (int c = 0; c < nsl; ++c) pa += part[c * n + i];
+      const double qn = w[i] / wn;
+      const double x = pa / wn - bsub * q[i];
+      q[i] = qn;
       w[i] = x;
-      av += q[i] * x;
+      av += qn * x;
     }
     av = wave_sum(av);
     if (lane == 0) sh[wave] = av;
@@ -1582,19 +1584,15 @@ __device__ double lanczos_top(const double* La, bool sq, const double* Ag, int n
     const double beta = sqrt(b2);
     const int mk = k + 1;
     const bool breakdown = beta <= 1.0e-13 * scale;
+    wn = beta;
+    bsub = beta;
+    const bool check = mk == cap || breakdown || (mk >= 16 && (mk & 7) == 0);
     if (tid == 0) {
       al[k] = alpha;
       be[k] = beta;
     }
-    if (!breakdown)
-      for (int i = tid; i < n; i += NT) {
-        qp[i] = q[i];
-        q[i] = w[i] / beta;
-      }
-    bprev = beta;
-    __syncthreads();
-    const bool check = mk == cap || breakdown || (mk >= 16 && (mk & 7) == 0);
     if (!check) continue;
+    __syncthreads();
     if (wave == 0) {
       double ghi = -1.0e300, glo = 1.0e300;
       for (int i = lane; i < mk; i += 64) {
@@ -1653,8 +1651,8 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
   }
   const bool square = lanczos_lds_square(n) <= M.lds_doubles;
   const bool packed = square || lanczos_lds_packed(n) <= M.lds_doubles;
-  double* Lp = packed ? lds : nullptr;
-  double* vec = square ? lds + n * n : packed ? lds + n * (n + 1) / 2 : lds;
+  double* Lp = packed ? g_lds : nullptr;
+  double* vec = square ? g_lds + n * n : packed ? g_lds + n * (n + 1) / 2 : g_lds;
   // (with n_occ = 0 or n_occ = n one side is empty and the other operator is Fu itself)
   double lo, hi;
   gershgorin(Fu, n, np, red, lo, hi);
