@@ -595,16 +595,22 @@ __device__ inline void fock_jsum(const KArgs& A, const Mol& M, const double* P, 
   for (int e = t0; e < M.nh * 10; e += st) {
     const int a = e / 10, q = e % 10;
     double s = 0.0;
+#pragma unroll 8
     for (int b = 0; b < M.nh; ++b) {
-      if (b == a) continue;
-      s += b > a ? JX[(long long)q * M.nxx + tri_index(a, b, M.nh)] : JX[(long long)(10 + q) * M.nxx + tri_index(b, a, M.nh)];
+      const double x = b > a ? JX[(long long)q * M.nxx + tri_index(a, b, M.nh)]
+                             : b < a ? JX[(long long)(10 + q) * M.nxx + tri_index(b, a, M.nh)] : 0.0;
+      if (b != a) s += x;
     }
-    for (int h = 0; h < M.nH; ++h) s += JH[(long long)q * M.nxh + a * M.nH + h];
+    const double* jh = JH + (long long)q * M.nxh + a * M.nH;
+#pragma unroll 8
+    for (int h = 0; h < M.nH; ++h) s += jh[h];
     J[a * 10 + q] = s;
   }
   for (int h = t0; h < M.nH; h += st) {
     double s = 0.0;
+#pragma unroll 8
     for (int a = 0; a < M.nh; ++a) s += JH[10LL * M.nxh + a * M.nH + h];
+#pragma unroll 8
     for (int g = 0; g < M.nH; ++g) {
       if (g == h) continue;
       const int lo = g < h ? g : h, hi = g < h ? h : g;
