@@ -57,6 +57,7 @@ constexpr int LARGE_MAX_NAO = 16384;    // largest molecule of the host-driven p
 constexpr double EIG_TOL_FACTOR = 1.0e-2;
 constexpr double EIG_REL = 1.0e-2;  // cap of the loose tolerance relative to the starting off-diagonal
 constexpr double EIG_SKIP = 0.0;
+constexpr int SP2_LDS = 2;     // purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS
 constexpr int PH_LAST = 12;   // phase timers: slots [0, 12) phases, [12] last stamp, [13, 16) counters
 constexpr int PH_SLOTS = 16;    // threshold Jacobi: skip rotations of |a_pq| <= EIG_SKIP x tolerance
 
@@ -107,6 +108,7 @@ struct KArgs {
   double eig_rel;
   double eig_skip;
   int purify;                  // 1: densities by purification, frontier levels by Lanczos (no orbitals)
+  int sp2_lds;                 // purification operand placements allowed: 0 HBM only, 1 + LDS pair, 2 + LDS X
   // learned guess: directed pairs (global indices, see batch_create) and network outputs
   const int8_t* elem8;
   const int* gbeg;
@@ -362,6 +364,7 @@ struct Mol {
   int lds_doubles;  // dynamic LDS of the launch, in doubles
   double eig_rel;   // loose-tolerance cap relative to the starting off-diagonal
   double eig_skip;  // rotations of elements below eig_skip x tolerance are skipped
+  int sp2_lds;      // purification operand placements allowed (KArgs::sp2_lds)
 };
 
 // ------------------------------------------------------------------------------------------
@@ -1423,8 +1426,8 @@ __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const doubl
   gershgorin(Fu, n, np, red, lo, hi);
   if (!(hi - lo > 1.0e-12 * fmax(1.0, fabs(hi)))) return -1;
   const int img = sp2_img(n);
-  if (2 * img <= M.lds_doubles) return sp2_run<2>(M, B, Fu, Pdst, red, lo, hi);
-  if (img <= M.lds_doubles) return sp2_run<1>(M, B, Fu, Pdst, red, lo, hi);
+  if (M.sp2_lds >= 1 && 2 * img <= M.lds_doubles) return sp2_run<2>(M, B, Fu, Pdst, red, lo, hi);
+  if (M.sp2_lds >= 2 && img <= M.lds_doubles) return sp2_run<1>(M, B, Fu, Pdst, red, lo, hi);
   return sp2_run<0>(M, B, Fu, Pdst, red, lo, hi);
 }
 
@@ -2153,6 +2156,7 @@ __host__ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double*
   M.lds_doubles = A.lds_doubles;
   M.eig_rel = A.eig_rel;
   M.eig_skip = A.eig_skip;
+  M.sp2_lds = A.sp2_lds;
   return M;
 }
 
@@ -3412,6 +3416,8 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   if (const char* ev = std::getenv("PM3_EIG_REL")) a.eig_rel = std::atof(ev);
   a.eig_skip = EIG_SKIP;
   a.purify = b.purify ? 1 : 0;
+  a.sp2_lds = SP2_LDS;
+  if (const char* ev = std::getenv("PM3_SP2_LDS")) a.sp2_lds = std::atoi(ev);
   if (const char* ev = std::getenv("PM3_EIG_SKIP")) a.eig_skip = std::atof(ev);
   HIP_OK(hipFuncSetAttribute((const void*)scf_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_need));
   if (!b.large.empty() && !large_setup(b, err)) return false;
