@@ -1197,6 +1197,7 @@ __device__ void adiis_wave(Diis& D, double* srt) {
 // ------------------------------------------------------------------------------------------
 struct Bufs {
   double *H, *F, *P, *Pn, *Fu, *T, *C, *Cn, *V, *Pb, *Cb, *Fh, *Eh, *Dh, *eps, *epsb, *lam;
+  double* bnd;  // LDS: cached spectral-bound estimates of the purification ([0] lo, [1] hi, [2] age)
   int* perm;
   unsigned long long* ph;  // LDS: [0..11] per-phase wall-clock sums, [12] last stamp, [13..15] counters; null when off
 };
@@ -1363,6 +1364,8 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
   const double occ = (double)nocc;
   int tail = -1, k = 0;
   for (; k < SP2_MAX; ++k) {
+    // every eigenvalue of X in [0, 1] keeps tr X^2 <= tr X; more means a bound was violated
+    if (t2 > tr + 1.0e-9 * fmax(1.0, tr)) return -2;
     if (tail < 0 && tr - t2 < SP2_TAIL) tail = SP2_EXTRA;
     if (tail == 0) break;
     const bool sq = fabs(t2 - occ) < fabs(2.0 * tr - t2 - occ);
@@ -1392,6 +1395,7 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
     if (tail > 0) --tail;
   }
   if (tail != 0) return -1;
+  if (fabs(tr - occ) > 1.0e-6) return -2;
   for (int e = tid; e < n * n; e += NT) {
     const int i = e / n, j = e % n;
     Pdst[i * np + j] = 2.0 * X[i * ldx + j];
@@ -1400,19 +1404,40 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
   return k;
 }
 
+__host__ __device__ constexpr int lanczos_lds(int n);
+__device__ __noinline__ void lanczos_extremes(const double* Ag, int n, int np, double* v, double& lo_est, double& hi_est);
+constexpr int SP2_TIGHT_N = 48;        // from this size on, spectral bounds from Lanczos pay for themselves
+constexpr double SP2_MARGIN = 0.05;    // margin of those bounds, relative to the estimated width
+constexpr double SP2_MARGIN_AGED = 0.08;  // margin when the estimate is from an earlier SCF step
+constexpr int SP2_REFRESH = 4;         // estimate again every this many steps (and the first two)
+
+__device__ inline int sp2_dispatch(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo,
+                                   double hi) {
+  const int img = sp2_img(M.n);
+  if (M.sp2_lds >= 1 && 2 * img <= M.lds_doubles) return sp2_run<2>(M, B, Fu, Pdst, red, lo, hi);
+  if (M.sp2_lds >= 2 && img <= M.lds_doubles) return sp2_run<1>(M, B, Fu, Pdst, red, lo, hi);
+  return sp2_run<0>(M, B, Fu, Pdst, red, lo, hi);
+}
+
 // Aufbau density of the symmetric matrix Fu by purification instead of eigenvectors: the
 // trace-correcting second-order spectral projection of Niklasson (Phys. Rev. B 66, 155115, 2002).
-// X0 = (e_max I - Fu) / (e_max - e_min) with Gershgorin bounds maps the spectrum into [0, 1] in
-// reverse order; every step replaces X by X^2 or by 2X - X^2, whichever brings tr(X) nearer to
-// n_occ, which drives the n_occ lowest eigenvalues of Fu to 1 and the others to 0.  P = 2 X is then
-// the projector 2 C_occ C_occ^T of the eigenvector route whenever the HOMO-LUMO gap is open.
-// tr(X^2) = |X|_F^2 for symmetric X, so the trace and the norm of each new X are summed in the
-// product's store.  The iteration stops SP2_EXTRA products after the idempotency error
-// tr(X) - tr(X^2) has fallen below SP2_TAIL (each further product squares the distance of the
-// eigenvalues on one side from 0 or 1).  The operands live in LDS when they fit (sp2_run); P goes
-// to Pdst.  Returns the number of products, or -1 when the error has not fallen within SP2_MAX
-// products (frontier orbitals degenerate or nearly so); the caller then diagonalises.
-__device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red) {
+// X0 = (e_max I - Fu) / (e_max - e_min) maps the spectrum into [0, 1] in reverse order; every
+// step replaces X by X^2 or by 2X - X^2, whichever brings tr(X) nearer to n_occ, which drives the
+// n_occ lowest eigenvalues of Fu to 1 and the others to 0.  P = 2 X is then the projector
+// 2 C_occ C_occ^T of the eigenvector route whenever the HOMO-LUMO gap is open.  tr(X^2) = |X|_F^2
+// for symmetric X, so the trace and the norm of each new X are summed in the product's store.  The
+// iteration stops SP2_EXTRA products after the idempotency error tr(X) - tr(X^2) has fallen below
+// SP2_TAIL (each further product squares the distance of the eigenvalues on one side from 0 or 1).
+// Bounds: Gershgorin, or with `tight` (n >= SP2_TIGHT_N) the Ritz values of a short Lanczos run
+// widened by a margin and clipped to Gershgorin -- a narrower map costs fewer products (about 14
+// instead of 20).  The estimate is kept in B.bnd across the SCF steps of one path (the extremes of
+// the spectrum move little) and taken again at the first two steps, every SP2_REFRESH steps and
+// after a miss.  A bound that misses an eigenvalue shows as tr X^2 > tr X or a wrong final trace;
+// the step is then redone with Gershgorin bounds.  The operands live in LDS when they fit
+// (sp2_run); P goes to Pdst.  Returns the number of products, or -1 when the error has not fallen
+// within SP2_MAX products (frontier orbitals degenerate or nearly so); the caller then diagonalises.
+__device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red,
+                                        bool tight) {
   const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
   if (nocc == 0 || nocc == n) {
     for (int e = tid; e < n * n; e += NT) {
@@ -1422,13 +1447,32 @@ __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const doubl
     __syncthreads();
     return 0;
   }
-  double lo, hi;
-  gershgorin(Fu, n, np, red, lo, hi);
-  if (!(hi - lo > 1.0e-12 * fmax(1.0, fabs(hi)))) return -1;
-  const int img = sp2_img(n);
-  if (M.sp2_lds >= 1 && 2 * img <= M.lds_doubles) return sp2_run<2>(M, B, Fu, Pdst, red, lo, hi);
-  if (M.sp2_lds >= 2 && img <= M.lds_doubles) return sp2_run<1>(M, B, Fu, Pdst, red, lo, hi);
-  return sp2_run<0>(M, B, Fu, Pdst, red, lo, hi);
+  double glo, ghi;
+  gershgorin(Fu, n, np, red, glo, ghi);
+  if (!(ghi - glo > 1.0e-12 * fmax(1.0, fabs(ghi)))) return -1;
+  if (tight && n >= SP2_TIGHT_N && lanczos_lds(n) <= M.lds_doubles) {
+    const double age = B.bnd[2];
+    double le = B.bnd[0], he = B.bnd[1], mf = SP2_MARGIN_AGED;
+    __syncthreads();  // every thread has read the cache before it changes
+    if (age < 2.0 || fmod(age, (double)SP2_REFRESH) == 0.0) {
+      lanczos_extremes(Fu, n, np, g_lds, le, he);
+      mf = SP2_MARGIN;
+      if (tid == 0) {
+        B.bnd[0] = le;
+        B.bnd[1] = he;
+      }
+    }
+    if (tid == 0) B.bnd[2] = age < 0.0 ? 1.0 : age + 1.0;
+    const double m = mf * (he - le);
+    const double lo = fmax(glo, le - m), hi = fmin(ghi, he + m);
+    int r = -1;
+    if (hi - lo < ghi - glo && hi - lo > 1.0e-12 * fmax(1.0, fabs(hi))) r = sp2_dispatch(M, B, Fu, Pdst, red, lo, hi);
+    if (r < 0 && tid == 0) B.bnd[2] = 0.0;  // a miss: estimate afresh next time
+    __syncthreads();
+    if (r >= 0) return r;
+  }
+  const int r = sp2_dispatch(M, B, Fu, Pdst, red, glo, ghi);
+  return r >= 0 ? r : -1;
 }
 
 // Number of eigenvalues below x of the symmetric tridiagonal matrix (diagonal a, off-diagonal b):
@@ -1503,94 +1547,109 @@ __host__ __device__ constexpr int lanczos_lds_packed(int n) { return n * (n + 1)
 __host__ __device__ constexpr int lanczos_lds_square(int n) { return n * n + lanczos_lds(n); }
 constexpr double LZ_TOL = 1.0e-10;  // Ritz residual (eV) at which a frontier level is taken
 
-// Largest eigenvalue of the symmetric matrix A (n x n, n <= 256) by a plain Lanczos run (three-term
-// recurrence, no reorthogonalisation: extreme Ritz values stay accurate, C. Paige 1976).  A is the
-// LDS matrix La -- full square (ld n) when sq, else packed lower triangle (row i at i (i + 1) / 2) --
-// or, when La is null, the np-strided square Ag in HBM.  Products run lanes along rows (64 rows per
-// wave) and each wave over one slice of the columns, the slices' partial sums meeting in LDS: no
-// cross-lane reductions inside the product.  Every eighth step from
-// the sixteenth, wave 0 takes the top eigenvalue of the tridiagonal by multisection (the previous
-// value is a lower bound: by interlacing it never decreases) and the run stops when the Ritz residual
-// is below LZ_TOL or the value has stalled over 8 and 16 steps (a converged value gains ghost
-// copies, whose near-degeneracy spoils the residual estimate).
+// One Lanczos step (three-term recurrence, no reorthogonalisation) on the symmetric n x n matrix A:
+// the LDS matrix La -- full square (ld n) when sq, else packed lower triangle (row i at
+// i (i + 1) / 2) -- or, when La is null, the np-strided square Ag in HBM.  On entry w holds the next
+// basis vector before its normalisation by wn and q the previous one; on exit q = w / wn, w the
+// new residual, (alpha, beta) the new tridiagonal entries.  The product runs lanes along rows (64
+// rows per wave), each wave over one slice of the columns, the slices' partial sums meeting in LDS:
+// no cross-lane reductions inside the product.  v: q, w, part[8][n] at offsets 0, n, 3n; sh: 16
+// reduction slots.  Three barriers.
+__device__ __forceinline__ void lanczos_step(const double* La, bool sq, const double* Ag, int n, int np, double* v,
+                                             double* sh, double wn, double bsub, double& alpha, double& beta) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double* q = v;
+  double* w = q + n;
+  double* part = w + 2 * n;
+  const int nrb = (n + 63) >> 6, nsl = NW / nrb, rb = wave % nrb, sl = wave / nrb;
+  const int row = rb * 64 + lane, slw = (n + nsl - 1) / nsl, c0 = sl * slw, c1 = min(n, c0 + slw);
+  if (sl < nsl && row < n) {
+    // partial (A w)_row over columns [c0, c1) (A symmetric: A[row][j] read as A[j][row])
+    double s0 = 0.0, s1 = 0.0;
+    if (La && sq) {
+      int j = c0;
+      for (; j + 1 < c1; j += 2) {
+        s0 += La[j * n + row] * w[j];
+        s1 += La[(j + 1) * n + row] * w[j + 1];
+      }
+      if (j < c1) s0 += La[j * n + row] * w[j];
+    } else if (La) {
+      for (int j = c0; j < c1; ++j) s0 += La[j <= row ? row * (row + 1) / 2 + j : j * (j + 1) / 2 + row] * w[j];
+    } else {
+      int j = c0;
+      for (; j + 1 < c1; j += 2) {
+        s0 += Ag[(size_t)j * np + row] * w[j];
+        s1 += Ag[(size_t)(j + 1) * np + row] * w[j + 1];
+      }
+      if (j < c1) s0 += Ag[(size_t)j * np + row] * w[j];
+    }
+    part[sl * n + row] = s0 + s1;
+  }
+  __syncthreads();
+  // q_k = w / wn; w = A q_k - beta_{k-1} q_{k-1}; alpha_k = q_k . w
+  double av = 0.0;
+  for (int i = tid; i < n; i += NT) {
+    double pa = 0.0;
+    for (int c = 0; c < nsl; ++c) pa += part[c * n + i];
+    const double qn = w[i] / wn;
+    const double x = pa / wn - bsub * q[i];
+    q[i] = qn;
+    w[i] = x;
+    av += qn * x;
+  }
+  av = wave_sum(av);
+  if (lane == 0) sh[wave] = av;
+  __syncthreads();
+  alpha = 0.0;
+  for (int x = 0; x < NW; ++x) alpha += sh[x];
+  double bv = 0.0;
+  for (int i = tid; i < n; i += NT) {
+    const double x = w[i] - alpha * q[i];
+    w[i] = x;
+    bv += x * x;
+  }
+  bv = wave_sum(bv);
+  if (lane == 0) sh[8 + wave] = bv;
+  __syncthreads();
+  double b2 = 0.0;
+  for (int x = 0; x < NW; ++x) b2 += sh[8 + x];
+  beta = sqrt(b2);
+}
+
+// Start vector of the Lanczos runs: a fixed spread over all basis directions in w (its norm
+// returned), q zeroed.
+__device__ inline double lanczos_start(int n, double* v) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  double* q = v;
+  double* w = q + n;
+  for (int i = tid; i < n; i += NT) {
+    w[i] = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);
+    q[i] = 0.0;
+  }
+  __syncthreads();
+  double nn = 0.0;
+  for (int i = lane; i < n; i += 64) nn += w[i] * w[i];
+  return sqrt(wave_sum(nn));  // every wave sums the same vector
+}
+
+// Largest eigenvalue of the symmetric matrix A (n x n, n <= 256) by a plain Lanczos run (extreme
+// Ritz values stay accurate without reorthogonalisation, C. Paige 1976), steps as lanczos_step.
+// Every eighth step from the sixteenth, wave 0 takes the top eigenvalue of the tridiagonal by
+// multisection (the previous value is a lower bound: by interlacing it never decreases) and the
+// run stops when the Ritz residual is below LZ_TOL or the value has stalled over 8 and 16 steps (a
+// converged value gains ghost copies, whose near-degeneracy spoils the residual estimate).
 __device__ __forceinline__ double lanczos_top(const double* La, bool sq, const double* Ag, int n, int np, double scale,
                                               double* v) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cap = lanczos_cap(n);
-  double* q = v;         // current basis vector q_k
-  double* w = q + n;     // next basis vector before its normalisation by wn
-  double* part = w + 2 * n;  // [8][n]
-  double* al = part + 8 * n;
+  double* al = v + 11 * n;
   double* be = al + (cap + 2);
-  double* sh = be + (cap + 2);  // [0, 8) alpha partials, [8, 16) beta partials, [16] value, [17] stop
-  for (int i = tid; i < n; i += NT) {
-    w[i] = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);  // a fixed spread over all basis directions
-    q[i] = 0.0;
-  }
-  __syncthreads();
-  double wn;
-  {
-    double nn = 0.0;
-    for (int i = lane; i < n; i += 64) nn += w[i] * w[i];
-    wn = sqrt(wave_sum(nn));  // every wave sums the same vector
-  }
-  // product layout: wave -> (row block rb of 64 rows, column slice c)
-  const int nrb = (n + 63) >> 6, nsl = NW / nrb, rb = wave % nrb, sl = wave / nrb;
-  const int row = rb * 64 + lane, slw = (n + nsl - 1) / nsl, c0 = sl * slw, c1 = min(n, c0 + slw);
-  const bool prow = sl < nsl && row < n;
+  double* sh = be + (cap + 2);  // [0, 16) reductions, [16] value, [17] stop
+  double wn = lanczos_start(n, v);
   double bsub = 0.0, th1 = -1.0e300, th2 = -1.0e300, top = 0.0;
   for (int k = 0; k < cap; ++k) {
-    // partial (A w)_row over columns [c0, c1) (A symmetric: A[row][j] read as A[j][row])
-    if (prow) {
-      double s0 = 0.0, s1 = 0.0;
-      if (La && sq) {
-        int j = c0;
-        for (; j + 1 < c1; j += 2) {
-          s0 += La[j * n + row] * w[j];
-          s1 += La[(j + 1) * n + row] * w[j + 1];
-        }
-        if (j < c1) s0 += La[j * n + row] * w[j];
-      } else if (La) {
-        for (int j = c0; j < c1; ++j) s0 += La[j <= row ? row * (row + 1) / 2 + j : j * (j + 1) / 2 + row] * w[j];
-      } else {
-        int j = c0;
-        for (; j + 1 < c1; j += 2) {
-          s0 += Ag[(size_t)j * np + row] * w[j];
-          s1 += Ag[(size_t)(j + 1) * np + row] * w[j + 1];
-        }
-        if (j < c1) s0 += Ag[(size_t)j * np + row] * w[j];
-      }
-      part[sl * n + row] = s0 + s1;
-    }
-    __syncthreads();
-    // q_k = w / wn; w = A q_k - beta_{k-1} q_{k-1}; alpha_k = q_k . w
-    double av = 0.0;
-    for (int i = tid; i < n; i += NT) {
-      double pa = 0.0;
-      for (int c = 0; c < nsl; ++c) pa += part[c * n + i];
-      const double qn = w[i] / wn;
-      const double x = pa / wn - bsub * q[i];
-      q[i] = qn;
-      w[i] = x;
-      av += qn * x;
-    }
-    av = wave_sum(av);
-    if (lane == 0) sh[wave] = av;
-    __syncthreads();
-    double alpha = 0.0;
-    for (int x = 0; x < NW; ++x) alpha += sh[x];
-    double bv = 0.0;
-    for (int i = tid; i < n; i += NT) {
-      const double x = w[i] - alpha * q[i];
-      w[i] = x;
-      bv += x * x;
-    }
-    bv = wave_sum(bv);
-    if (lane == 0) sh[8 + wave] = bv;
-    __syncthreads();
-    double b2 = 0.0;
-    for (int x = 0; x < NW; ++x) b2 += sh[8 + x];
-    const double beta = sqrt(b2);
+    double alpha, beta;
+    lanczos_step(La, sq, Ag, n, np, v, sh, wn, bsub, alpha, beta);
     const int mk = k + 1;
     const bool breakdown = beta <= 1.0e-13 * scale;
     wn = beta;
@@ -1638,6 +1697,61 @@ __device__ __forceinline__ double lanczos_top(const double* La, bool sq, const d
     if (stop) break;
   }
   return top;
+}
+
+// Estimates of the lowest and highest eigenvalue of the symmetric np-strided matrix Ag from a
+// fixed number of Lanczos steps (Ritz values: lo_est >= lambda_min, hi_est <= lambda_max); LDS
+// v as lanczos_lds(n).
+constexpr int SP2_LZ_STEPS = 12;
+__device__ __noinline__ void lanczos_extremes(const double* Ag, int n, int np, double* v, double& lo_est, double& hi_est) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int steps = min(n, SP2_LZ_STEPS);
+  double* al = v + 11 * n;
+  double* be = al + (steps + 2);
+  double* nal = be + (steps + 2);
+  double* sh = nal + (steps + 2);
+  double wn = lanczos_start(n, v);
+  double bsub = 0.0, scale = 0.0;
+  int m = 0;
+  for (int k = 0; k < steps; ++k) {
+    double alpha, beta;
+    lanczos_step(nullptr, false, Ag, n, np, v, sh, wn, bsub, alpha, beta);
+    if (tid == 0) {
+      al[k] = alpha;
+      nal[k] = -alpha;
+      be[k] = beta;
+    }
+    m = k + 1;
+    scale = fmax(scale, fabs(alpha) + beta);
+    if (beta <= 1.0e-13 * scale) break;
+    wn = beta;
+    bsub = beta;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    double ghi = -1.0e300, glo = 1.0e300;
+    for (int i = lane; i < m; i += 64) {
+      const double r = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i + 1 < m ? fabs(be[i]) : 0.0);
+      ghi = fmax(ghi, al[i] + r);
+      glo = fmin(glo, al[i] - r);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ghi = fmax(ghi, __shfl_xor(ghi, o, 64));
+      glo = fmin(glo, __shfl_xor(glo, o, 64));
+    }
+    const double sc = fmax(fabs(ghi), fabs(glo)) * 1.0e-14 + 1.0e-300;
+    const double hi = tridiag_max(al, be, m, glo - sc, ghi + sc);
+    const double lo = -tridiag_max(nal, be, m, -ghi - sc, -glo + sc);
+    if (lane == 0) {
+      sh[16] = lo;
+      sh[17] = hi;
+    }
+  }
+  __syncthreads();
+  lo_est = sh[16];
+  hi_est = sh[17];
+  __syncthreads();
 }
 
 // HOMO and LUMO of Fu given the projector X = P / 2 onto its n_occ lowest eigenvectors, without
@@ -1796,12 +1910,12 @@ __device__ double commutator2(const KArgs& A, const Mol& M, const Bufs& B, doubl
 // SCF, so that only idempotent densities enter the DIIS history.  Returns the guess used:
 // 0 superposition (or not learned), 1 learned (one pre-iteration done), 2 learned rejected.
 __device__ __noinline__ int initial_guess(const KArgs& A, const Mol& M, const Bufs& B, int guess, double* lds,
-                                          double* red) {
+                                          double* red, bool tight) {
   const int np = M.np, tid = threadIdx.x;
   for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
   __syncthreads();
   if (guess == 2) {
-    if (A.purify && sp2_density(M, B, M.H, B.P, red) >= 0) return 0;
+    if (A.purify && sp2_density(M, B, M.H, B.P, red, tight) >= 0) return 0;
     eigensolve(M, B, M.H, false, lds, red);
     density(M, B.Cn, B.P);
     return 0;
@@ -1815,7 +1929,7 @@ __device__ __noinline__ int initial_guess(const KArgs& A, const Mol& M, const Bu
     const double cl = commutator2(A, M, B, lds, red);  // leaves F(P_learned) in B.F
     if (cl <= cs) {
       if (A.purify) {
-        if (sp2_density(M, B, B.F, B.P, red) < 0) {
+        if (sp2_density(M, B, B.F, B.P, red, tight) < 0) {
           eigensolve(M, B, B.F, false, lds, red);
           density(M, B.Cn, B.P);
         }
@@ -1868,9 +1982,11 @@ struct PathResult {
 // last extrapolated Fock) in B; with A.purify each density is the purified aufbau projector of
 // the extrapolated Fock matrix and only eps[nocc - 1], eps[nocc] (its frontier levels) are set.
 __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int guess, int accel, double* lds,
-                               double* red, Diis& D) {
+                               double* red, Diis& D, bool tight = true) {
   const int n = M.n, np = M.np, tid = threadIdx.x;
-  const int gused = initial_guess(A, M, B, guess, lds, red);
+  if (tid == 0) B.bnd[2] = -1.0;  // no spectral-bound estimate for this path yet
+  __syncthreads();
+  const int gused = initial_guess(A, M, B, guess, lds, red, tight);
   // DIIS depth from the memory budget (src/scf.rs:309-327, :1205-1228)
   const long long n2 = (long long)n * n * 8;
   const long long budget = (long long)A.scf_memory_mb * 1024 * 1024;
@@ -2052,7 +2168,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
     stamp(B, 2);
     Flast = Fuse;
     if (A.purify) {
-      const int ns = sp2_density(M, B, Fuse, B.Pn, red);
+      const int ns = sp2_density(M, B, Fuse, B.Pn, red, tight);
       if (ns < 0) {  // no open gap: diagonalise
         eigensolve(M, B, Fuse, false, lds, red);
         density(M, B.Cn, B.Pn);
@@ -2103,10 +2219,22 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
   return R;
 }
 
+// With purified densities: a final HOMO above the LUMO means some eigenvalue ended on the wrong
+// side of the purification (a spectral bound that missed it); the path is then run again with
+// Gershgorin bounds throughout.
+__device__ inline bool aufbau_broken(const KArgs& A, const Mol& M, const Bufs& B) {
+  const int n = M.n, nocc = M.nocc;
+  if (!A.purify || nocc <= 0 || nocc >= n) return false;
+  const double h = B.eps[nocc - 1], l = B.eps[nocc];
+  return h > l + 1.0e-9 * fmax(1.0, fabs(l));
+}
+
 __device__ inline Bufs slab(const KArgs& A, double*& Wbase) {
   double* base = A.scratch + (long long)blockIdx.x * A.stride;
   const long long m2 = (long long)A.np_max * A.np_max;
   Bufs B;
+  B.ph = nullptr;   // phase timers off unless the kernel sets them
+  B.bnd = nullptr;  // set by the kernel to its LDS slots
   B.H = base;
   B.F = B.H + m2;
   B.P = B.F + m2;
@@ -2354,10 +2482,12 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
   __shared__ Diis D;
   __shared__ int s_mol;
   __shared__ unsigned long long s_ph[PH_SLOTS];
+  __shared__ double s_bnd[4];
   const int tid = threadIdx.x;
   double* Wbase;
   Bufs B = slab(A, Wbase);
   B.ph = A.prof ? s_ph : nullptr;
+  B.bnd = s_bnd;
   if (tid < PH_SLOTS) s_ph[tid] = 0;
   if (tid == 0) s_ph[PH_LAST] = wall_clock64();
 
@@ -2383,6 +2513,7 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
     const double ecore = finish_core(A, M, red);
     stamp(B, 0);
     PathResult best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D);
+    if (aufbau_broken(A, M, B)) best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D, false);
     const int guess_used = best.guess_used;
     const int nocc = M.nocc, n = M.n, np = M.np;
     const double first_e = best.e_elec;
@@ -2403,6 +2534,7 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
       for (int k = 0; k < 5; ++k) {
         ++paths;
         PathResult c = rhf_loop(A, M, B, rg[k][0], rg[k][1], lds, red, D);
+        if (aufbau_broken(A, M, B)) c = rhf_loop(A, M, B, rg[k][0], rg[k][1], lds, red, D, false);
         if (!c.converged) continue;
         if (!best.converged || c.e_elec < best.e_elec - 1.0e-6) {
           best = c;
@@ -2469,7 +2601,10 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
   __shared__ double red[NW * RED_K];
   const int tid = threadIdx.x;
   double* Wbase;
-  const Bufs B = slab(A, Wbase);
+  __shared__ double s_bnd[4];
+  Bufs B = slab(A, Wbase);
+  B.bnd = s_bnd;
+  if (tid == 0) s_bnd[2] = -1.0;
   const Mol M = mol_of(A, A.mols[A.order[0]], Wbase);
   const int n = M.n, np = M.np;
   zero_mats(M, B);
@@ -2516,7 +2651,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
   } else if (mode == 6) {
     for (int e = tid; e < n * n; e += NT) B.Fu[(e / n) * np + e % n] = in[e];
     __syncthreads();
-    const int ns = sp2_density(M, B, B.Fu, B.P, red);
+    const int ns = sp2_density(M, B, B.Fu, B.P, red, true);
     if (ns >= 0) frontier_levels(M, B, B.Fu, B.P, lds, red);
     for (int e = tid; e < n * n; e += NT) out[e] = B.P[(e / n) * np + e % n];
     if (tid == 0) {
