@@ -58,6 +58,7 @@ constexpr double EIG_TOL_FACTOR = 1.0e-2;
 constexpr double EIG_REL = 1.0e-2;  // cap of the loose tolerance relative to the starting off-diagonal
 constexpr double EIG_SKIP = 0.0;
 constexpr int SP2_LDS = 2;     // purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS
+constexpr int SP2_TIGHT_N = 48;  // from this size on, purification bounds from Lanczos (PM3_SP2_TIGHT_N)
 constexpr int PH_LAST = 12;   // phase timers: slots [0, 12) phases, [12] last stamp, [13, 16) counters
 constexpr int PH_SLOTS = 16;    // threshold Jacobi: skip rotations of |a_pq| <= EIG_SKIP x tolerance
 
@@ -109,6 +110,7 @@ struct KArgs {
   double eig_skip;
   int purify;                  // 1: densities by purification, frontier levels by Lanczos (no orbitals)
   int sp2_lds;                 // purification operand placements allowed: 0 HBM only, 1 + LDS pair, 2 + LDS X
+  int sp2_tight_n;             // smallest molecule (AOs) whose purification uses Lanczos spectral bounds
   // learned guess: directed pairs (global indices, see batch_create) and network outputs
   const int8_t* elem8;
   const int* gbeg;
@@ -365,6 +367,7 @@ struct Mol {
   double eig_rel;   // loose-tolerance cap relative to the starting off-diagonal
   double eig_skip;  // rotations of elements below eig_skip x tolerance are skipped
   int sp2_lds;      // purification operand placements allowed (KArgs::sp2_lds)
+  int sp2_tight_n;  // KArgs::sp2_tight_n
 };
 
 // ------------------------------------------------------------------------------------------
@@ -1406,7 +1409,6 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
 
 __host__ __device__ constexpr int lanczos_lds(int n);
 __device__ __noinline__ void lanczos_extremes(const double* Ag, int n, int np, double* v, double& lo_est, double& hi_est);
-constexpr int SP2_TIGHT_N = 48;        // from this size on, spectral bounds from Lanczos pay for themselves
 constexpr double SP2_MARGIN = 0.05;    // margin of those bounds, relative to the estimated width
 constexpr double SP2_MARGIN_AGED = 0.08;  // margin when the estimate is from an earlier SCF step
 constexpr int SP2_REFRESH = 4;         // estimate again every this many steps (and the first two)
@@ -1428,7 +1430,7 @@ __device__ inline int sp2_dispatch(const Mol& M, const Bufs& B, const double* Fu
 // for symmetric X, so the trace and the norm of each new X are summed in the product's store.  The
 // iteration stops SP2_EXTRA products after the idempotency error tr(X) - tr(X^2) has fallen below
 // SP2_TAIL (each further product squares the distance of the eigenvalues on one side from 0 or 1).
-// Bounds: Gershgorin, or with `tight` (n >= SP2_TIGHT_N) the Ritz values of a short Lanczos run
+// Bounds: Gershgorin, or with `tight` (n >= M.sp2_tight_n) the Ritz values of a short Lanczos run
 // widened by a margin and clipped to Gershgorin -- a narrower map costs fewer products (about 14
 // instead of 20).  The estimate is kept in B.bnd across the SCF steps of one path (the extremes of
 // the spectrum move little) and taken again at the first two steps, every SP2_REFRESH steps and
@@ -1450,7 +1452,7 @@ __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const doubl
   double glo, ghi;
   gershgorin(Fu, n, np, red, glo, ghi);
   if (!(ghi - glo > 1.0e-12 * fmax(1.0, fabs(ghi)))) return -1;
-  if (tight && n >= SP2_TIGHT_N && lanczos_lds(n) <= M.lds_doubles) {
+  if (tight && n >= M.sp2_tight_n && lanczos_lds(n) <= M.lds_doubles) {
     const double age = B.bnd[2];
     double le = B.bnd[0], he = B.bnd[1], mf = SP2_MARGIN_AGED;
     __syncthreads();  // every thread has read the cache before it changes
@@ -2285,6 +2287,7 @@ __host__ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double*
   M.eig_rel = A.eig_rel;
   M.eig_skip = A.eig_skip;
   M.sp2_lds = A.sp2_lds;
+  M.sp2_tight_n = A.sp2_tight_n;
   return M;
 }
 
@@ -3552,6 +3555,8 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   a.eig_skip = EIG_SKIP;
   a.purify = b.purify ? 1 : 0;
   a.sp2_lds = SP2_LDS;
+  a.sp2_tight_n = SP2_TIGHT_N;
+  if (const char* ev = std::getenv("PM3_SP2_TIGHT_N")) a.sp2_tight_n = std::atoi(ev);
   if (const char* ev = std::getenv("PM3_SP2_LDS")) a.sp2_lds = std::atoi(ev);
   if (const char* ev = std::getenv("PM3_EIG_SKIP")) a.eig_skip = std::atof(ev);
   HIP_OK(hipFuncSetAttribute((const void*)scf_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_need));
