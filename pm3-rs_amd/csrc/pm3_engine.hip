@@ -59,6 +59,7 @@ constexpr double EIG_REL = 1.0e-2;  // cap of the loose tolerance relative to th
 constexpr double EIG_SKIP = 0.0;
 constexpr int SP2_LDS = 2;     // purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS
 constexpr int SP2_TIGHT_N = 48;  // from this size on, purification bounds from Lanczos (PM3_SP2_TIGHT_N)
+constexpr double SP2_MARGIN = 0.05;  // their margin, relative to the estimated width (PM3_SP2_MARGIN)
 constexpr int PH_LAST = 12;   // phase timers: slots [0, 12) phases, [12] last stamp, [13, 16) counters
 constexpr int PH_SLOTS = 16;    // threshold Jacobi: skip rotations of |a_pq| <= EIG_SKIP x tolerance
 
@@ -111,6 +112,7 @@ struct KArgs {
   int purify;                  // 1: densities by purification, frontier levels by Lanczos (no orbitals)
   int sp2_lds;                 // purification operand placements allowed: 0 HBM only, 1 + LDS pair, 2 + LDS X
   int sp2_tight_n;             // smallest molecule (AOs) whose purification uses Lanczos spectral bounds
+  double sp2_margin;           // margin of those bounds relative to the estimated width (fresh estimate)
   // learned guess: directed pairs (global indices, see batch_create) and network outputs
   const int8_t* elem8;
   const int* gbeg;
@@ -368,6 +370,7 @@ struct Mol {
   double eig_skip;  // rotations of elements below eig_skip x tolerance are skipped
   int sp2_lds;      // purification operand placements allowed (KArgs::sp2_lds)
   int sp2_tight_n;  // KArgs::sp2_tight_n
+  double sp2_margin;  // KArgs::sp2_margin
 };
 
 // ------------------------------------------------------------------------------------------
@@ -627,7 +630,9 @@ __device__ inline void fock_jsum(const KArgs& A, const Mol& M, const double* P, 
   }
 }
 
-// Phase 3: diagonal blocks, H + one-centre terms + J.
+// Phase 3: diagonal blocks, H + one-centre terms + J.  Each element with mu <= nu is computed once
+// and stored on both sides, so that F is exactly symmetric (the (nu, mu) sum visits the same terms
+// in another order).
 __device__ inline void fock_diag(const KArgs& A, const Mol& M, const double* H, const double* P, double* F,
                                  const double* J, int t0, int st) {
   const int np = M.np;
@@ -636,7 +641,7 @@ __device__ inline void fock_diag(const KArgs& A, const Mol& M, const double* H, 
     const int ga = M.atom0 + a;
     const ElemConst& ea = c_tab.el[A.aslot[ga]];
     const int nn = ea.norb;
-    if (mu < nn && nu < nn) {
+    if (mu < nn && nu < nn && mu <= nu) {
       const int o = A.aoff[ga];
       double acc = 0.0;
       for (int la = 0; la < nn; ++la)
@@ -645,7 +650,9 @@ __device__ inline void fock_diag(const KArgs& A, const Mol& M, const double* H, 
           acc += pls * oc_int(mu, nu, la, si, ea);
           acc -= (0.5 * pls) * oc_int(mu, la, nu, si, ea);
         }
-      F[(o + mu) * np + o + nu] = H[(o + mu) * np + o + nu] + acc + J[a * 10 + pk(mu, nu)];
+      const double v = H[(o + mu) * np + o + nu] + acc + J[a * 10 + pk(mu, nu)];
+      F[(o + mu) * np + o + nu] = v;
+      F[(o + nu) * np + o + mu] = v;
     }
   }
 }
@@ -1409,8 +1416,7 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
 
 __host__ __device__ constexpr int lanczos_lds(int n);
 __device__ __noinline__ void lanczos_extremes(const double* Ag, int n, int np, double* v, double& lo_est, double& hi_est);
-constexpr double SP2_MARGIN = 0.05;    // margin of those bounds, relative to the estimated width
-constexpr double SP2_MARGIN_AGED = 0.08;  // margin when the estimate is from an earlier SCF step
+constexpr double SP2_MARGIN_AGED = 0.03;  // extra margin when the estimate is from an earlier SCF step
 constexpr int SP2_REFRESH = 4;         // estimate again every this many steps (and the first two)
 
 __device__ inline int sp2_dispatch(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo,
@@ -1454,11 +1460,11 @@ __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const doubl
   if (!(ghi - glo > 1.0e-12 * fmax(1.0, fabs(ghi)))) return -1;
   if (tight && n >= M.sp2_tight_n && lanczos_lds(n) <= M.lds_doubles) {
     const double age = B.bnd[2];
-    double le = B.bnd[0], he = B.bnd[1], mf = SP2_MARGIN_AGED;
+    double le = B.bnd[0], he = B.bnd[1], mf = M.sp2_margin + SP2_MARGIN_AGED;
     __syncthreads();  // every thread has read the cache before it changes
     if (age < 2.0 || fmod(age, (double)SP2_REFRESH) == 0.0) {
       lanczos_extremes(Fu, n, np, g_lds, le, he);
-      mf = SP2_MARGIN;
+      mf = M.sp2_margin;
       if (tid == 0) {
         B.bnd[0] = le;
         B.bnd[1] = he;
@@ -2066,17 +2072,17 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       for (int k = 0; k < 3 * MAXD + 3; ++k) v[k] = 0.0;
       for (int e = tid; e < npk; e += NT) {
         int i, j;
-        tri_unpack(e, n, i, j);
+        tri_unpack(e, n, i, j);  // i <= j; consecutive e run down column j: read (j, i) of symmetric F, P, H
         const double en = (i == j) ? 0.0 : B.T[i * np + j] - B.T[j * np + i];
-        const double fn = B.F[i * np + j];
-        const double pn = B.P[i * np + j];
+        const double fn = B.F[j * np + i];
+        const double pn = B.P[j * np + i];
         En[e] = en;
         Fn[e] = fn;
         if (keep_d) Dn[e] = pn;
         const double wgt = e < n ? 1.0 : 2.0;
         const double dn = keep_d ? pn : 0.0;
         const double enw = wgt * en, fnw = wgt * fn, dnw = wgt * dn;
-        v[3 * MAXD + 2] += wgt * pn * (M.H[i * np + j] + fn);
+        v[3 * MAXD + 2] += wgt * pn * (M.H[j * np + i] + fn);
         v[3 * MAXD] += enw * en;
         if (want_mm) v[3 * MAXD + 1] += dnw * fn;
 #pragma unroll
@@ -2288,6 +2294,7 @@ __host__ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double*
   M.eig_skip = A.eig_skip;
   M.sp2_lds = A.sp2_lds;
   M.sp2_tight_n = A.sp2_tight_n;
+  M.sp2_margin = A.sp2_margin;
   return M;
 }
 
@@ -3556,6 +3563,8 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   a.purify = b.purify ? 1 : 0;
   a.sp2_lds = SP2_LDS;
   a.sp2_tight_n = SP2_TIGHT_N;
+  a.sp2_margin = SP2_MARGIN;
+  if (const char* ev = std::getenv("PM3_SP2_MARGIN")) a.sp2_margin = std::atof(ev);
   if (const char* ev = std::getenv("PM3_SP2_TIGHT_N")) a.sp2_tight_n = std::atoi(ev);
   if (const char* ev = std::getenv("PM3_SP2_LDS")) a.sp2_lds = std::atoi(ev);
   if (const char* ev = std::getenv("PM3_EIG_SKIP")) a.eig_skip = std::atof(ev);

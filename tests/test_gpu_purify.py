@@ -144,3 +144,38 @@ def test_purified_and_eigensolver_routes_agree(G, monkeypatch):
         same += a.iterations == c.iterations
     print("equal iteration counts", same, "of", len(r1), flush=True)
     assert same >= 0.95 * len(r1)
+
+
+def test_missed_spectral_bounds_are_caught(G, pins, oracle, monkeypatch):
+    """Spectral bounds deliberately set inside the spectrum (negative margin): the checks redo each
+    purification with Gershgorin bounds, and a path whose final HOMO lies above its LUMO is run
+    again, so densities and answers are those of the default settings."""
+    monkeypatch.setenv("PM3_SP2_MARGIN", "-0.3")
+    monkeypatch.setenv("PM3_SP2_TIGHT_N", "8")
+    run = _stage(G)
+    for name, z, x, q in _cases(pins)[-4:]:
+        st, o = oracle.run(z, x, q, oracle.options(**TIGHT))
+        F = oracle.fock(z, x, o["density"])
+        nocc = o["n_occ"]
+        w, V = np.linalg.eigh(F)
+        P, k, homo, lumo = run(G.Molecule(z, x, q), F)
+        print("case", name, "products", k, flush=True)
+        assert k > 0 and np.abs(P - 2.0 * V[:, :nocc] @ V[:, :nocc].T).max() <= 1e-11
+        assert abs(homo - w[nocc - 1]) <= 1e-9 and abs(lumo - w[nocc]) <= 1e-9
+    mols = [G.Molecule(np.asarray(z, np.int32), W.to_bohr(x)) for z, x in W.mixed_batch(48, 616)]
+    opts = G.Pm3Options(p_tol=1e-8, e_tol=1e-8)
+    b = G.Batch(mols, options=opts, want_density=True)
+    b.solve()
+    s1, r1 = b.results()
+    b.close()
+    monkeypatch.delenv("PM3_SP2_MARGIN")
+    monkeypatch.delenv("PM3_SP2_TIGHT_N")
+    b = G.Batch(mols, options=opts, want_density=True)
+    b.solve()
+    s2, r2 = b.results()
+    b.close()
+    assert np.all(s1 == 0) and np.all(s2 == 0)
+    for a, c in zip(r1, r2):
+        assert abs(a.heat_of_formation_kcal - c.heat_of_formation_kcal) <= HOF_TOL
+        assert np.abs(a.density - c.density).max() <= 1e-7
+        assert abs(a.homo_ev - c.homo_ev) <= 1e-6 and abs(a.lumo_ev - c.lumo_ev) <= 1e-6
