@@ -2718,6 +2718,106 @@ __global__ __launch_bounds__(LG_THREADS) void lg_dots_kernel(Dots D, long long m
   }
 }
 
+// Purification of the large path (the batched kernel's sp2_density, with rocBLAS products).
+// Gershgorin bounds of the symmetric np-strided A, one wave per row: per-block lowest disc bottom
+// and highest disc top in red[block * LG_MAXDOT + 0 / 1].
+__global__ __launch_bounds__(LG_THREADS) void lg_gershgorin_kernel(const double* A, int n, int np, double* red) {
+  __shared__ double part[2][LG_THREADS / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double lo = 1.0e300, hi = -1.0e300;
+  for (int i = blockIdx.x * (LG_THREADS / 64) + w; i < n; i += gridDim.x * (LG_THREADS / 64)) {
+    const double* row = A + (size_t)i * np;
+    double r = 0.0;
+    for (int j = lane; j < n; j += 64) r += j == i ? 0.0 : fabs(row[j]);
+    r = wave_sum(r);
+    lo = fmin(lo, row[i] - r);
+    hi = fmax(hi, row[i] + r);
+  }
+  if (lane == 0) {
+    part[0][w] = lo;
+    part[1][w] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < LG_THREADS / 64; ++q) {
+      lo = fmin(lo, part[0][q]);
+      hi = fmax(hi, part[1][q]);
+    }
+    red[(size_t)blockIdx.x * LG_MAXDOT] = fmin(lo, part[0][0]);
+    red[(size_t)blockIdx.x * LG_MAXDOT + 1] = fmax(hi, part[1][0]);
+  }
+}
+
+// Per-block partials of tr X and |X|_F^2 of the n x n block-stride loop (red columns 0 and 1).
+__device__ inline void lg_trace_partials(double tr, double fr, double* red) {
+  __shared__ double part[2][LG_THREADS / 64];
+  tr = wave_sum(tr);
+  fr = wave_sum(fr);
+  if ((threadIdx.x & 63) == 0) {
+    part[0][threadIdx.x >> 6] = tr;
+    part[1][threadIdx.x >> 6] = fr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0, c = 0.0;
+    for (int q = 0; q < LG_THREADS / 64; ++q) {
+      a += part[0][q];
+      c += part[1][q];
+    }
+    red[(size_t)blockIdx.x * LG_MAXDOT] = a;
+    red[(size_t)blockIdx.x * LG_MAXDOT + 1] = c;
+  }
+}
+
+// X0 = (hi I - A) / (hi - lo), with its trace and norm partials.
+__global__ __launch_bounds__(LG_THREADS) void lg_sp2_init_kernel(const double* A, double* X, int n, int np, double lo,
+                                                                 double hi, double* red) {
+  const double s = 1.0 / (hi - lo);
+  double tr = 0.0, fr = 0.0;
+  for (long long e = gtid(); e < (long long)n * n; e += gstride()) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double x = ((i == j ? hi : 0.0) - A[(size_t)i * np + j]) * s;
+    X[(size_t)i * np + j] = x;
+    fr += x * x;
+    if (i == j) tr += x;
+  }
+  lg_trace_partials(tr, fr, red);
+}
+
+// X <- Y or 2X - Y, with Y = X X from rocblas_dsyrk (upper triangle in column-major order, i.e.
+// the row-major lower triangle), then the trace and norm partials of the new X.
+__global__ __launch_bounds__(LG_THREADS) void lg_sp2_update_kernel(const double* Y, double* X, int n, int np, int sq,
+                                                                   double* red) {
+  double tr = 0.0, fr = 0.0;
+  for (long long e = gtid(); e < (long long)n * n; e += gstride()) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double y = i >= j ? Y[(size_t)i * np + j] : Y[(size_t)j * np + i];
+    const double x = sq ? y : 2.0 * X[(size_t)i * np + j] - y;
+    X[(size_t)i * np + j] = x;
+    fr += x * x;
+    if (i == j) tr += x;
+  }
+  lg_trace_partials(tr, fr, red);
+}
+
+// Pdst = 2 X on the n x n block.
+__global__ __launch_bounds__(LG_THREADS) void lg_twice_kernel(const double* X, double* Pdst, int n, int np) {
+  for (long long e = gtid(); e < (long long)n * n; e += gstride()) {
+    const size_t o = (size_t)(e / n) * np + (size_t)(e % n);
+    Pdst[o] = 2.0 * X[o];
+  }
+}
+
+// Sums of the first k columns of the block partials, blocks in order, into out[0..k) (one thread
+// per column), so that the host reads k values instead of every block's.
+__global__ __launch_bounds__(64) void lg_finish_kernel(const double* red, int k, double* out) {
+  if (threadIdx.x < k) {
+    double t = 0.0;
+    for (int b = 0; b < LG_BLOCKS; ++b) t += red[(size_t)b * LG_MAXDOT + threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
 // Block partials of sum (X - Y)^2 (red[block]).
 __global__ __launch_bounds__(LG_THREADS) void lg_diff2_kernel(const double* X, const double* Y, long long m, double* red) {
   __shared__ double part[LG_THREADS / 64];
@@ -2862,7 +2962,7 @@ namespace pm3host {
 struct LargeBufs {
   double *F = nullptr, *P = nullptr, *Pn = nullptr, *Fu = nullptr, *T = nullptr, *V = nullptr, *w = nullptr;
   double *ework = nullptr, *Pb = nullptr, *Vb = nullptr, *wb = nullptr, *Fh = nullptr, *Eh = nullptr, *Dh = nullptr;
-  double *J = nullptr, *slot = nullptr, *red = nullptr, *dip = nullptr;
+  double *J = nullptr, *slot = nullptr, *red = nullptr, *dip = nullptr, *fin = nullptr;
   int* info = nullptr;
   long long m2 = 0;  // elements per matrix (np_max^2)
   int depth = 0;     // DIIS history slots allocated
@@ -2986,7 +3086,8 @@ static bool large_setup(Batch& b, std::string& err) {
       !b.alloc(L.Pb, L.m2, err) || !b.alloc(L.Vb, L.m2, err) || !b.alloc(L.wb, np, err) ||
       !b.alloc(L.Fh, L.m2 * depth, err) || !b.alloc(L.Eh, L.m2 * depth, err) || !b.alloc(L.Dh, L.m2 * depth, err) ||
       !b.alloc(L.J, 10LL * nat, err) || !b.alloc(L.slot, slot, err) ||
-      !b.alloc(L.red, (size_t)LG_BLOCKS * LG_MAXDOT, err) || !b.alloc(L.dip, 3, err) || !b.alloc(L.info, 1, err))
+      !b.alloc(L.red, (size_t)LG_BLOCKS * LG_MAXDOT, err) || !b.alloc(L.dip, 3, err) || !b.alloc(L.info, 1, err) ||
+      !b.alloc(L.fin, LG_MAXDOT, err))
     return false;
   b.h_red.resize((size_t)LG_BLOCKS * LG_MAXDOT);
   BLAS_OK(rocblas_create_handle(&b.blas));
@@ -3058,6 +3159,83 @@ static bool large_eigen_density(Batch& b, const MolDesc& d, const double* A, dou
   return true;
 }
 
+// Aufbau density of the symmetric A into Pdst by purification (the steps, stopping rule and
+// checks of sp2_density, Gershgorin bounds), X in L.V and the product in L.T.  False (nothing
+// written) when the frontier gap is closed or a check fails: the caller then diagonalises.
+static bool large_purify(Batch& b, const MolDesc& d, const double* A, double* Pdst, bool& ok, std::string& err) {
+  LargeBufs& L = b.lg;
+  const int n = d.nao, np = d.np, nocc = d.nocc;
+  ok = false;
+  if (nocc <= 0 || nocc >= n) return true;
+  hipLaunchKernelGGL(lg_gershgorin_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, A, n, np, L.red);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(b.h_red.data(), L.red, sizeof(double) * (size_t)LG_BLOCKS * LG_MAXDOT, hipMemcpyDeviceToHost));
+  double lo = 1.0e300, hi = -1.0e300;
+  for (int blk = 0; blk < LG_BLOCKS; ++blk) {
+    lo = std::min(lo, b.h_red[(size_t)blk * LG_MAXDOT]);
+    hi = std::max(hi, b.h_red[(size_t)blk * LG_MAXDOT + 1]);
+  }
+  if (!(hi - lo > 1.0e-12 * std::max(1.0, std::fabs(hi)))) return true;
+  auto sums = [&](double& tr, double& t2) -> bool {
+    hipLaunchKernelGGL(lg_finish_kernel, dim3(1), dim3(64), 0, 0, (const double*)L.red, 2, L.fin);
+    double h[2];
+    HIP_OK(hipMemcpy(h, L.fin, sizeof(h), hipMemcpyDeviceToHost));
+    tr = h[0];
+    t2 = h[1];
+    return true;
+  };
+  double tr, t2;
+  hipLaunchKernelGGL(lg_sp2_init_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, A, L.V, n, np, lo, hi, L.red);
+  HIP_OK(hipGetLastError());
+  if (!sums(tr, t2)) return false;
+  const double occ = (double)nocc, one = 1.0, zero = 0.0;
+  int tail = -1, k = 0;
+  for (; k < SP2_MAX; ++k) {
+    if (t2 > tr + 1.0e-9 * std::max(1.0, tr)) return true;
+    if (tail < 0 && tr - t2 < SP2_TAIL) tail = SP2_EXTRA;
+    if (tail == 0) break;
+    const int sq = std::fabs(t2 - occ) < std::fabs(2.0 * tr - t2 - occ) ? 1 : 0;
+    BLAS_OK(rocblas_dsyrk(b.blas, rocblas_fill_upper, rocblas_operation_none, n, n, &one, L.V, np, &zero, L.T, np));
+    hipLaunchKernelGGL(lg_sp2_update_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.T, L.V, n, np, sq,
+                       L.red);
+    HIP_OK(hipGetLastError());
+    if (!sums(tr, t2)) return false;
+    if (tail > 0) --tail;
+  }
+  if (tail != 0 || std::fabs(tr - occ) > 1.0e-6) return true;
+  hipLaunchKernelGGL(lg_twice_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.V, Pdst, n, np);
+  HIP_OK(hipGetLastError());
+  ok = true;
+  return true;
+}
+
+// Eigenvalues only of the symmetric A (ascending, into L.w), for the frontier levels of a
+// purified path.
+static bool large_levels(Batch& b, const MolDesc& d, const double* A, std::string& err) {
+  LargeBufs& L = b.lg;
+  const int n = d.nao, np = d.np;
+  HIP_OK(hipMemcpyAsync(L.V, A, sizeof(double) * (size_t)np * np, hipMemcpyDeviceToDevice, 0));
+  BLAS_OK(rocsolver_dsyevd(b.blas, rocblas_evect_none, rocblas_fill_upper, n, L.V, np, L.w, L.ework, L.info));
+  int info = 0;
+  HIP_OK(hipMemcpy(&info, L.info, sizeof(int), hipMemcpyDeviceToHost));
+  if (info != 0) {
+    err = "rocsolver_dsyevd did not converge (info " + std::to_string(info) + ")";
+    return false;
+  }
+  return true;
+}
+
+// The aufbau density of A into Pdst: purified when the batch returns no orbitals, else (or when
+// purification declines) from the eigenvectors.
+static bool large_density(Batch& b, const MolDesc& d, const double* A, double* Pdst, std::string& err) {
+  if (b.purify) {
+    bool ok = false;
+    if (!large_purify(b, d, A, Pdst, ok, err)) return false;
+    if (ok) return true;
+  }
+  return large_eigen_density(b, d, A, Pdst, err);
+}
+
 struct LargePath {
   double e_elec = 0.0, last_dp = 1.0e300;
   int iters = 0, converged = 0;
@@ -3072,7 +3250,7 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
   HIP_OK(hipMemsetAsync(L.P, 0, sizeof(double) * ms, 0));
   HIP_OK(hipMemsetAsync(L.Pn, 0, sizeof(double) * ms, 0));
   if (guess == 2) {
-    if (!large_eigen_density(b, d, M.H, L.P, err)) return false;
+    if (!large_density(b, d, M.H, L.P, err)) return false;
   } else {
     int tilt = -1;
     if (guess == 3) {
@@ -3095,6 +3273,7 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
   D.depth = std::min(diis_depth_for(n, keep_d, o.scf_memory_mb), L.depth);
   D.keep_d = keep_d;
   R = LargePath();
+  const double* Flast = nullptr;  // the matrix of the last aufbau step
   double e_old = 0.0;
   double* P = L.P;
   double* Pn = L.Pn;
@@ -3181,7 +3360,8 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
         Fuse = Fn;
       }
     }
-    if (!large_eigen_density(b, d, Fuse, Pn, err)) return false;
+    if (!large_density(b, d, Fuse, Pn, err)) return false;
+    Flast = Fuse;
     hipLaunchKernelGGL(lg_diff2_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)Pn, (const double*)P, ms,
                        L.red);
     double s2 = 0.0;
@@ -3199,6 +3379,7 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
   if (P != L.P) {  // keep the final density in L.P
     HIP_OK(hipMemcpyAsync(L.P, P, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
   }
+  if (b.purify && Flast && !large_levels(b, d, Flast, err)) return false;  // frontier levels of the last step
   if (!large_fock(b, d, L.P, L.F, err)) return false;
   return large_energy(b, d, L.P, L.F, R.e_elec, err);
 }
@@ -3268,9 +3449,11 @@ static bool large_solve(Batch& b, int k, std::string& err) {
   if (d.out_mat >= 0) {
     hipLaunchKernelGGL(lg_pack_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.P,
                        b.args.out_P + d.out_mat, n, np);
-    hipLaunchKernelGGL(lg_vectors_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.V,
-                       b.args.out_C + d.out_mat, n, np);
-    HIP_OK(hipMemcpyAsync(b.args.out_eps + d.out_vec, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+    if (!b.purify) {  // orbitals only on the eigenvector route
+      hipLaunchKernelGGL(lg_vectors_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.V,
+                         b.args.out_C + d.out_mat, n, np);
+      HIP_OK(hipMemcpyAsync(b.args.out_eps + d.out_vec, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+    }
     HIP_OK(hipGetLastError());
   }
   return true;
