@@ -2681,6 +2681,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
 constexpr int LG_BLOCKS = 1024;
 constexpr int LG_THREADS = 256;
 constexpr int LG_MAXDOT = 3 * MAXD + 2;
+constexpr double LARGE_SP2_MARGIN = 0.15;  // least relative margin of the large path's Lanczos bounds
 
 struct Dots {
   const double* x[LG_MAXDOT];
@@ -2963,6 +2964,10 @@ struct LargeBufs {
   double *F = nullptr, *P = nullptr, *Pn = nullptr, *Fu = nullptr, *T = nullptr, *V = nullptr, *w = nullptr;
   double *ework = nullptr, *Pb = nullptr, *Vb = nullptr, *wb = nullptr, *Fh = nullptr, *Eh = nullptr, *Dh = nullptr;
   double *J = nullptr, *slot = nullptr, *red = nullptr, *dip = nullptr, *fin = nullptr;
+  double* lz = nullptr;                          // Lanczos vectors (3 np)
+  double bnd_lo = 0.0, bnd_hi = 0.0;            // cached spectral-bound estimate of the purification
+  int bnd_age = -1;
+  bool no_tight = false;                         // Gershgorin bounds only (a path being rerun)
   int* info = nullptr;
   long long m2 = 0;  // elements per matrix (np_max^2)
   int depth = 0;     // DIIS history slots allocated
@@ -3087,7 +3092,7 @@ static bool large_setup(Batch& b, std::string& err) {
       !b.alloc(L.Fh, L.m2 * depth, err) || !b.alloc(L.Eh, L.m2 * depth, err) || !b.alloc(L.Dh, L.m2 * depth, err) ||
       !b.alloc(L.J, 10LL * nat, err) || !b.alloc(L.slot, slot, err) ||
       !b.alloc(L.red, (size_t)LG_BLOCKS * LG_MAXDOT, err) || !b.alloc(L.dip, 3, err) || !b.alloc(L.info, 1, err) ||
-      !b.alloc(L.fin, LG_MAXDOT, err))
+      !b.alloc(L.fin, LG_MAXDOT, err) || !b.alloc(L.lz, 3LL * np, err))
     return false;
   b.h_red.resize((size_t)LG_BLOCKS * LG_MAXDOT);
   BLAS_OK(rocblas_create_handle(&b.blas));
@@ -3159,9 +3164,79 @@ static bool large_eigen_density(Batch& b, const MolDesc& d, const double* A, dou
   return true;
 }
 
+// Largest eigenvalue of the symmetric tridiagonal (a, b) of size m, by bisection on the Sturm count.
+static double host_tridiag_max(const std::vector<double>& a, const std::vector<double>& b, int m) {
+  double lo = 1.0e300, hi = -1.0e300;
+  for (int i = 0; i < m; ++i) {
+    const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  for (int it = 0; it < 200 && hi - lo > 1.0e-15 * std::max(std::fabs(lo), std::fabs(hi)); ++it) {
+    const double x = 0.5 * (lo + hi);
+    int below = 0;
+    double d = 1.0;
+    for (int i = 0; i < m; ++i) {
+      d = a[i] - x - (i > 0 ? b[i - 1] * b[i - 1] / d : 0.0);
+      if (d == 0.0) d = -1.0e-300;
+      below += d < 0.0;
+    }
+    if (below == m) hi = x; else lo = x;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// Ritz estimates of the extreme eigenvalues of the symmetric np-strided A: SP2_LZ_STEPS plain
+// Lanczos steps with rocBLAS level-1/2 calls (start vector as lanczos_start).
+static bool large_extremes(Batch& b, const MolDesc& d, const double* A, double& lo_est, double& hi_est, std::string& err) {
+  LargeBufs& L = b.lg;
+  const int n = d.nao, np = d.np;
+  double* q = L.lz;
+  double* qp = L.lz + np;
+  double* w = L.lz + 2 * np;
+  std::vector<double> h(n);
+  for (int i = 0; i < n; ++i) h[i] = 1.0 + 0.5 * std::sin(1.7 * (double)i + 0.3);
+  double nn = 0.0;
+  for (double x : h) nn += x * x;
+  nn = std::sqrt(nn);
+  for (double& x : h) x /= nn;
+  HIP_OK(hipMemcpy(q, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIP_OK(hipMemsetAsync(qp, 0, sizeof(double) * n, 0));
+  std::vector<double> al, be, nal;
+  double beta = 0.0;
+  const double one = 1.0, zero = 0.0;
+  const int steps = std::min(n, SP2_LZ_STEPS);
+  for (int k = 0; k < steps; ++k) {
+    BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_none, n, n, &one, A, np, q, 1, &zero, w, 1));  // A symmetric
+    const double mb = -beta;
+    BLAS_OK(rocblas_daxpy(b.blas, n, &mb, qp, 1, w, 1));
+    double alpha = 0.0;
+    BLAS_OK(rocblas_ddot(b.blas, n, q, 1, w, 1, &alpha));
+    const double ma = -alpha;
+    BLAS_OK(rocblas_daxpy(b.blas, n, &ma, q, 1, w, 1));
+    BLAS_OK(rocblas_dnrm2(b.blas, n, w, 1, &beta));
+    al.push_back(alpha);
+    nal.push_back(-alpha);
+    be.push_back(beta);
+    if (beta <= 1.0e-13 * (std::fabs(alpha) + beta)) break;
+    HIP_OK(hipMemcpyAsync(qp, q, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+    const double ib = 1.0 / beta;
+    BLAS_OK(rocblas_dscal(b.blas, n, &ib, w, 1));
+    HIP_OK(hipMemcpyAsync(q, w, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+  }
+  const int m = (int)al.size();
+  hi_est = host_tridiag_max(al, be, m);
+  lo_est = -host_tridiag_max(nal, be, m);
+  return true;
+}
+
 // Aufbau density of the symmetric A into Pdst by purification (the steps, stopping rule and
-// checks of sp2_density, Gershgorin bounds), X in L.V and the product in L.T.  False (nothing
-// written) when the frontier gap is closed or a check fails: the caller then diagonalises.
+// checks of sp2_density), X in L.V and the product in L.T.  Bounds from Gershgorin, or from a
+// cached Lanczos estimate as in sp2_density (refreshed at the first two steps of a path, every
+// SP2_REFRESH steps and after a miss).  False in `ok` (nothing written) when the frontier gap is
+// closed or a check fails with Gershgorin bounds: the caller then diagonalises.
+static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double* Pdst, double lo, double hi, bool& ok,
+                             std::string& err);
 static bool large_purify(Batch& b, const MolDesc& d, const double* A, double* Pdst, bool& ok, std::string& err) {
   LargeBufs& L = b.lg;
   const int n = d.nao, np = d.np, nocc = d.nocc;
@@ -3170,12 +3245,38 @@ static bool large_purify(Batch& b, const MolDesc& d, const double* A, double* Pd
   hipLaunchKernelGGL(lg_gershgorin_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, A, n, np, L.red);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpy(b.h_red.data(), L.red, sizeof(double) * (size_t)LG_BLOCKS * LG_MAXDOT, hipMemcpyDeviceToHost));
-  double lo = 1.0e300, hi = -1.0e300;
+  double glo = 1.0e300, ghi = -1.0e300;
   for (int blk = 0; blk < LG_BLOCKS; ++blk) {
-    lo = std::min(lo, b.h_red[(size_t)blk * LG_MAXDOT]);
-    hi = std::max(hi, b.h_red[(size_t)blk * LG_MAXDOT + 1]);
+    glo = std::min(glo, b.h_red[(size_t)blk * LG_MAXDOT]);
+    ghi = std::max(ghi, b.h_red[(size_t)blk * LG_MAXDOT + 1]);
   }
-  if (!(hi - lo > 1.0e-12 * std::max(1.0, std::fabs(hi)))) return true;
+  if (!(ghi - glo > 1.0e-12 * std::max(1.0, std::fabs(ghi)))) return true;
+  if (n >= b.args.sp2_tight_n && !L.no_tight) {
+    const int age = L.bnd_age;
+    // (a few Lanczos steps see less of a large spectrum: a wider margin here)
+    const double m0 = b.args.sp2_margin < 0.0 ? b.args.sp2_margin : std::max(b.args.sp2_margin, LARGE_SP2_MARGIN);
+    double mf = m0 + SP2_MARGIN_AGED;
+    if (age < 2 || age % SP2_REFRESH == 0) {
+      if (!large_extremes(b, d, A, L.bnd_lo, L.bnd_hi, err)) return false;
+      mf = m0;
+    }
+    L.bnd_age = age < 0 ? 1 : age + 1;
+    const double m = mf * (L.bnd_hi - L.bnd_lo);
+    const double lo = std::max(glo, L.bnd_lo - m), hi = std::min(ghi, L.bnd_hi + m);
+    if (hi - lo < ghi - glo && hi - lo > 1.0e-12 * std::max(1.0, std::fabs(hi))) {
+      if (!large_purify_run(b, d, A, Pdst, lo, hi, ok, err)) return false;
+      if (ok) return true;
+      L.bnd_age = 0;  // a miss: estimate afresh next time
+    }
+  }
+  return large_purify_run(b, d, A, Pdst, glo, ghi, ok, err);
+}
+
+static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double* Pdst, double lo, double hi, bool& ok,
+                             std::string& err) {
+  LargeBufs& L = b.lg;
+  const int n = d.nao, np = d.np, nocc = d.nocc;
+  ok = false;
   auto sums = [&](double& tr, double& t2) -> bool {
     hipLaunchKernelGGL(lg_finish_kernel, dim3(1), dim3(64), 0, 0, (const double*)L.red, 2, L.fin);
     double h[2];
@@ -3239,6 +3340,7 @@ static bool large_density(Batch& b, const MolDesc& d, const double* A, double* P
 struct LargePath {
   double e_elec = 0.0, last_dp = 1.0e300;
   int iters = 0, converged = 0;
+  bool aufbau = true;  // purified path: the final density is the aufbau projector of its Fock matrix
 };
 
 static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePath& R, std::string& err) {
@@ -3249,6 +3351,7 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
   const EngineOptions& o = b.opt;
   HIP_OK(hipMemsetAsync(L.P, 0, sizeof(double) * ms, 0));
   HIP_OK(hipMemsetAsync(L.Pn, 0, sizeof(double) * ms, 0));
+  L.bnd_age = -1;  // no spectral-bound estimate for this path yet
   if (guess == 2) {
     if (!large_density(b, d, M.H, L.P, err)) return false;
   } else {
@@ -3379,7 +3482,23 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
   if (P != L.P) {  // keep the final density in L.P
     HIP_OK(hipMemcpyAsync(L.P, P, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
   }
-  if (b.purify && Flast && !large_levels(b, d, Flast, err)) return false;  // frontier levels of the last step
+  if (b.purify && Flast) {
+    // frontier levels of the last step, and the aufbau test: tr(P F) / 2 is the sum of the n_occ
+    // lowest eigenvalues only for the aufbau projector (another set of n_occ levels would exceed
+    // it by at least one HOMO-LUMO gap)
+    if (!large_levels(b, d, Flast, err)) return false;
+    Dots D{};
+    D.k = 1;
+    D.x[0] = L.P;
+    D.y[0] = Flast;
+    double pf = 0.0;
+    if (!large_dots(b, D, ms, &pf, err)) return false;
+    std::vector<double> w(n);
+    HIP_OK(hipMemcpy(w.data(), L.w, sizeof(double) * n, hipMemcpyDeviceToHost));
+    double so = 0.0;
+    for (int i = 0; i < d.nocc; ++i) so += w[i];
+    R.aufbau = std::fabs(0.5 * pf - so) <= 1.0e-9 * std::max(1.0, std::fabs(so));
+  }
   if (!large_fock(b, d, L.P, L.F, err)) return false;
   return large_energy(b, d, L.P, L.F, R.e_elec, err);
 }
@@ -3396,7 +3515,13 @@ static bool large_solve(Batch& b, int k, std::string& err) {
   if (!large_sum1(b, ecore, err)) return false;
   const int guess = b.opt.guess == 4 ? 1 : b.opt.guess;  // the learned guess covers batched molecules
   LargePath best;
+  L.no_tight = false;
   if (!large_rhf(b, d, guess, b.opt.accel, best, err)) return false;
+  if (!best.aufbau) {  // an eigenvalue ended on the wrong side: the path again with Gershgorin bounds
+    L.no_tight = true;
+    if (!large_rhf(b, d, guess, b.opt.accel, best, err)) return false;
+    L.no_tight = false;
+  }
   std::vector<double> w(n);
   HIP_OK(hipMemcpy(w.data(), L.w, sizeof(double) * n, hipMemcpyDeviceToHost));
   const double first_e = best.e_elec;
@@ -3414,6 +3539,11 @@ static bool large_solve(Batch& b, int k, std::string& err) {
       ++paths;
       LargePath c;
       if (!large_rhf(b, d, rg[r][0], rg[r][1], c, err)) return false;
+      if (!c.aufbau) {
+        L.no_tight = true;
+        if (!large_rhf(b, d, rg[r][0], rg[r][1], c, err)) return false;
+        L.no_tight = false;
+      }
       if (!c.converged) continue;
       if (!best.converged || c.e_elec < best.e_elec - 1.0e-6) {
         best = c;

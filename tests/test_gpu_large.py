@@ -1,7 +1,8 @@
 """The host-driven multi-kernel path for molecules too large for one workgroup (grid-wide integral,
-Fock and reduction kernels, rocBLAS products, rocSOLVER eigensolve): the same molecule solved on
-both paths agrees, a cluster beyond the batched kernel's size limit agrees with the CPU oracle,
-and the 512-water cluster (3072 AOs) satisfies the size-independent invariants."""
+Fock and reduction kernels, rocBLAS products, purified densities or the rocSOLVER eigensolve):
+the same molecule solved on both paths agrees, a cluster beyond the batched kernel's size limit
+agrees with the CPU oracle on both routes, spectral bounds set inside the spectrum on purpose are
+caught, and the 512-water cluster (3072 AOs) satisfies the size-independent invariants."""
 
 import numpy as np
 import pytest
@@ -67,6 +68,42 @@ def test_cluster_beyond_workgroup_limit_matches_oracle(G, oracle):
     C = r.mo_coeff
     assert np.abs(C.T @ C - np.eye(n)).max() < 1e-10
     assert np.abs(2 * C[:, :r.n_occ] @ C[:, :r.n_occ].T - r.density).max() < 1e-10
+    # the purified route (no orbitals requested)
+    b = G.Batch([mol], options=G.Pm3Options(**TIGHT), want_density=True)
+    b.solve()
+    sp, rp = b.results()
+    b.close()
+    q = rp[0]
+    print("purified route iterations", q.iterations, r.iterations, flush=True)
+    assert sp[0] == 0 and abs(q.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL
+    assert np.abs(q.density - o["density"]).max() <= 1e-8
+    assert abs(q.homo_ev - o["homo_ev"]) <= 1e-7 and abs(q.lumo_ev - o["lumo_ev"]) <= 1e-7
+
+
+def test_large_path_missed_spectral_bounds_are_caught(G, monkeypatch):
+    """Purification bounds deliberately set inside the spectrum (negative margin) on a 240-AO
+    cluster: each miss is caught and redone, and the answers are those of the default settings."""
+    z, x = W.water_cluster(40, W.water_box_for(40), 41)
+    mol = G.Molecule(np.asarray(z, np.int32), W.to_bohr(x))
+    opts = G.Pm3Options(p_tol=1e-8, e_tol=1e-8)
+    out = []
+    for env in ({"PM3_SP2_MARGIN": "-0.3", "PM3_SP2_TIGHT_N": "8"}, {}):
+        for k in ("PM3_SP2_MARGIN", "PM3_SP2_TIGHT_N"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        b = G.Batch([mol], options=opts, want_density=True)
+        b.solve()
+        st, res = b.results()
+        b.close()
+        assert st[0] == 0
+        out.append(res[0])
+        print("env", env, "nao", res[0].density.shape[0], "iterations", res[0].iterations, flush=True)
+    a, c = out
+    assert a.density.shape[0] == 240
+    assert abs(a.heat_of_formation_kcal - c.heat_of_formation_kcal) <= HOF_TOL
+    assert np.abs(a.density - c.density).max() <= 1e-7
+    assert abs(a.homo_ev - c.homo_ev) <= 1e-6 and abs(a.lumo_ev - c.lumo_ev) <= 1e-6
 
 
 def test_water512_invariants(G):
