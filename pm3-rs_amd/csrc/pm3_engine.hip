@@ -2785,8 +2785,8 @@ __global__ __launch_bounds__(LG_THREADS) void lg_sp2_init_kernel(const double* A
   lg_trace_partials(tr, fr, red);
 }
 
-// X <- Y or 2X - Y, with Y = X X from rocblas_dsyrk (upper triangle in column-major order, i.e.
-// the row-major lower triangle), then the trace and norm partials of the new X.
+// X <- Y or 2X - Y, with Y = X X (only its row-major lower triangle is read, so X stays exactly
+// symmetric), then the trace and norm partials of the new X.
 __global__ __launch_bounds__(LG_THREADS) void lg_sp2_update_kernel(const double* Y, double* X, int n, int np, int sq,
                                                                    double* red) {
   double tr = 0.0, fr = 0.0;
@@ -3296,7 +3296,10 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     if (tail < 0 && tr - t2 < SP2_TAIL) tail = SP2_EXTRA;
     if (tail == 0) break;
     const int sq = std::fabs(t2 - occ) < std::fabs(2.0 * tr - t2 - occ) ? 1 : 0;
-    BLAS_OK(rocblas_dsyrk(b.blas, rocblas_fill_upper, rocblas_operation_none, n, n, &one, L.V, np, &zero, L.T, np));
+    // the whole square (rocBLAS's dgemm runs faster than its dsyrk at these sizes); the update
+    // reads one triangle of it, so X stays exactly symmetric
+    BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, L.V, np, L.V, np, &zero,
+                          L.T, np));
     hipLaunchKernelGGL(lg_sp2_update_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.T, L.V, n, np, sq,
                        L.red);
     HIP_OK(hipGetLastError());
