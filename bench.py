@@ -43,7 +43,7 @@ NET_FLOP_PER_PAIR = 2.0 * (96 * 128 + 128 * 128 + 128 * 16)  # forward pass per 
 # HBM bytes per scf_batch_kernel launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a
 # 512 MB copy: x2.0 / x1.0) for this build on the default workload, per leg
 # (profiles/scf_batch_kernel_traffic_mixed4096.md); not measured for other workloads.
-MEASURED_TRAFFIC = {("mixed4096", "learned"): 1.478e11, ("mixed4096", "plain"): 2.052e11}
+MEASURED_TRAFFIC = {("mixed4096", "learned"): 1.489e11, ("mixed4096", "plain"): 2.045e11}
 HBM_PEAK_GBS = 8000.0
 HOF_TOL = 1.0e-5
 
@@ -75,6 +75,7 @@ def bench_large(args):
     z, x = W.water_cluster(512, 25.0, args.seed)
     mol = G.Molecule(np.asarray(z, np.int32), W.to_bohr(x))
     opts = G.Pm3Options(p_tol=1.0e-8, e_tol=1.0e-8)
+    tuning = apply_tuning(args, [mol], opts, True, reps=1)
     batch = G.Batch([mol], options=opts, want_gradient=True, want_density=True)
     for _ in range(args.warmup):
         batch.solve()
@@ -109,8 +110,23 @@ def bench_large(args):
         "cpu_baseline": None,
         "cpu_baseline_note": "the CPU restatement's dense eigensolver is not bounded at 3072 AOs "
                              "(one eigensolve takes minutes); the CPU leg is measured on the batched workloads",
+        "tuning": tuning,
     }
     print(json.dumps(out))
+
+
+def apply_tuning(args, molecules, options, want_gradient, reps=2):
+    """Set the process-wide tuning table as --tune asks; returns its record for the JSON line."""
+    if args.tune == "off":
+        G.tuning_set(G.tuning_default())
+        return {"source": "built-in", "table": G.tuning_get()}
+    if args.tune != "auto":
+        G.tuning_load(args.tune)
+        return {"source": args.tune, "table": G.tuning_get()}
+    t0 = time.perf_counter()
+    table, rep = G.autotune(molecules, options=options, want_gradient=want_gradient, reps=reps)
+    return {"source": "pm3_autotune on this workload (untimed, before the legs)", "table": table, "report": rep,
+            "seconds": time.perf_counter() - t0}
 
 
 def main():
@@ -129,6 +145,9 @@ def main():
                     help="profiling helper: run only a plain device copy of this size (for counter scaling)")
     ap.add_argument("--guess-model", default="", help="learned-guess network file (default: the packaged one)")
     ap.add_argument("--plain-only", action="store_true", help="time only the superposition-guess leg")
+    ap.add_argument("--tune", default="auto",
+                    help="launch-shape / tile table: 'auto' runs pm3_autotune on the workload first (untimed), "
+                         "'off' keeps the built-in table, anything else is a table file to load")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -162,6 +181,10 @@ def main():
         path = args.guess_model or GS.DEFAULT_MODEL
         if os.path.exists(path):
             model = GS.GuessModel.load(path)
+
+    tune_opts = (G.Pm3Options(p_tol=p_tol, e_tol=e_tol, guess="learned", guess_model=model) if model is not None
+                 else G.Pm3Options(p_tol=p_tol, e_tol=e_tol))
+    tuning = apply_tuning(args, molecules, tune_opts, args.gradient)
 
     def leg(opts):
         batch = G.Batch(molecules, options=opts, want_gradient=args.gradient)
@@ -280,6 +303,7 @@ def main():
                      "kernel": "scf_batch_kernel", "kernel_ms": k_ms, "algorithmic_flop": flops,
                      "pair_kernel_ms": pair_ms},
         "cpu_baseline": cpu,
+        "tuning": tuning,
     }
     if args.gradient:
         out["gradient_check"] = grad_check

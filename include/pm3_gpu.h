@@ -154,6 +154,39 @@ pm3_status pm3_batch_guess_timing(const pm3_batch* batch, double* features_ms, d
  * purification fallbacks. */
 pm3_status pm3_batch_phase_ticks(const pm3_batch* batch, uint64_t* out16);
 
+/* Launch shapes and tile choices of the kernels, and an autotuner for them.  No reference
+ * counterpart (the reference has no device kernels).  The entries change speed only: results
+ * agree to rounding whatever the values, and pm3_autotune checks that on the workload it tunes.
+ * A process-wide table is read by every pm3_batch_create / pm3_run / pm3_gradient; it starts at
+ * the defaults, or at the file the environment variable PM3_TUNING_FILE names.  Setting or
+ * loading it is not thread-safe against concurrent batch creation. */
+typedef struct {
+  int32_t pair_block;         /* threads per block of the pair-integral and gradient-pair kernels: 64, 128, 256 */
+  int32_t pair_blocks_per_cu; /* their grid (grid-stride loops), blocks per CU: 1..64 */
+  int32_t sp2_tile16_n;       /* purification products on 16x16 wave tiles up to this many AOs, 32x32 above */
+  int32_t sp2_lds;            /* purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS */
+  int32_t sp2_tight_n;        /* purification spectral bounds from Lanczos from this many AOs on */
+  int32_t large_syrk;         /* multi-kernel path: purification products by dsyrk (1) or dgemm (0) */
+} pm3_tuning;
+void pm3_tuning_default(pm3_tuning* t);
+pm3_status pm3_tuning_get(pm3_tuning* t);
+pm3_status pm3_tuning_set(const pm3_tuning* t); /* PM3_INVALID_INPUT for an entry out of range */
+/* Text file, one "key value" line per entry (keys as the field names), '#' comments. */
+pm3_status pm3_tuning_load(const char* path);
+pm3_status pm3_tuning_save(const char* path);
+typedef struct {
+  double default_ms;  /* device time per solve with the starting table (median of reps) */
+  double best_ms;     /* the same with the returned table */
+  int32_t candidates; /* tables timed, the starting one included */
+  int32_t rejected;   /* candidates whose statuses or heats of formation (1e-5 kcal/mol) differed */
+} pm3_tune_report;
+/* Coordinate descent over the entries on this workload (arguments as pm3_batch_create; one
+ * batch per candidate, one untimed and `reps` timed solves each).  On success the process-wide
+ * table is the best one found and is returned in *best; on failure it is left as it was. */
+pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
+                        const double* charge, const pm3_params* params, const pm3_options* opts, int32_t flags,
+                        int32_t reps, pm3_tuning* best, pm3_tune_report* report);
+
 /* Detail text of the calling thread's last error ("" if none). */
 const char* pm3_last_error_message(void);
 

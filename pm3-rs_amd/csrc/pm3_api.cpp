@@ -1,7 +1,10 @@
 // C ABI of libpm3gpu.so (declared in include/pm3_gpu.h).  Thin layer: validation and the error
 // surface of the reference (src/error.rs:12-47), option defaults (src/scf.rs:259-287), heat of
 // formation bookkeeping (src/scf.rs:745-752); all numerical work is on the device.
+#include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -343,6 +346,194 @@ pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const
     if (density) std::memcpy(density, r.density.data(), (size_t)r.nao * r.nao * sizeof(double));
   }
   return molecule_status(r.status, r);
+}
+
+// ------------------------------------------------------------------------------------------
+// Tuning table and autotuner (no reference counterpart: the reference has no device kernels).
+
+static void to_c(const Tuning& t, pm3_tuning* c) {
+  c->pair_block = t.pair_block;
+  c->pair_blocks_per_cu = t.pair_blocks_per_cu;
+  c->sp2_tile16_n = t.sp2_tile16_n;
+  c->sp2_lds = t.sp2_lds;
+  c->sp2_tight_n = t.sp2_tight_n;
+  c->large_syrk = t.large_syrk;
+}
+
+static Tuning from_c(const pm3_tuning* c) {
+  Tuning t;
+  t.pair_block = c->pair_block;
+  t.pair_blocks_per_cu = c->pair_blocks_per_cu;
+  t.sp2_tile16_n = c->sp2_tile16_n;
+  t.sp2_lds = c->sp2_lds;
+  t.sp2_tight_n = c->sp2_tight_n;
+  t.large_syrk = c->large_syrk;
+  return t;
+}
+
+void pm3_tuning_default(pm3_tuning* t) {
+  if (t) to_c(Tuning(), t);
+}
+
+pm3_status pm3_tuning_get(pm3_tuning* t) {
+  g_last_error.clear();
+  if (!t) return fail(PM3_INVALID_INPUT, "null tuning");
+  to_c(tuning_table(), t);
+  return PM3_OK;
+}
+
+pm3_status pm3_tuning_set(const pm3_tuning* t) {
+  g_last_error.clear();
+  if (!t) return fail(PM3_INVALID_INPUT, "null tuning");
+  const Tuning v = from_c(t);
+  std::string why;
+  if (!tuning_valid(v, why)) return fail(PM3_INVALID_INPUT, why);
+  tuning_table() = v;
+  return PM3_OK;
+}
+
+pm3_status pm3_tuning_load(const char* path) {
+  g_last_error.clear();
+  if (!path) return fail(PM3_INVALID_INPUT, "null path");
+  Tuning t;
+  std::string err;
+  if (!tuning_load(path, t, err)) return fail(PM3_INVALID_INPUT, err);
+  tuning_table() = t;
+  return PM3_OK;
+}
+
+pm3_status pm3_tuning_save(const char* path) {
+  g_last_error.clear();
+  if (!path) return fail(PM3_INVALID_INPUT, "null path");
+  std::string err;
+  if (!tuning_save(path, tuning_table(), err)) return fail(PM3_INVALID_INPUT, err);
+  return PM3_OK;
+}
+
+// Coordinate descent over the table's entries on the caller's workload: each candidate table is
+// used for a fresh batch (the table is read at creation), solved once untimed and `reps` times
+// timed (device time of pair integrals + learned guess + SCF/gradient, HIP events; the median
+// counts).  A candidate is kept when it is faster by more than max(0.2 %, 0.02 ms) and every
+// molecule's status, and heat of formation to 1e-5 kcal/mol, equal those of the starting table.
+// PM3_TUNE_VERBOSE=1 prints each candidate's time to stderr.
+pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
+                        const double* charge, const pm3_params* params, const pm3_options* opts, int32_t flags,
+                        int32_t reps, pm3_tuning* best_out, pm3_tune_report* report) {
+  g_last_error.clear();
+  if (!params || n_mol <= 0 || !atom_ptr || !z || !xyz_bohr) return fail(PM3_INVALID_INPUT, "bad autotune arguments");
+  if (atom_ptr[0] != 0) return fail(PM3_INVALID_INPUT, "atom_ptr[0] must be 0");
+  for (int k = 0; k < n_mol; ++k)
+    if (atom_ptr[k + 1] < atom_ptr[k]) return fail(PM3_INVALID_INPUT, "atom_ptr must be non-decreasing");
+  EngineOptions eo;
+  std::string why;
+  if (!engine_options(opts, eo, why)) return fail(PM3_INVALID_INPUT, why);
+  std::vector<MolIn> mols;
+  for (int k = 0; k < n_mol; ++k)
+    mols.push_back(MolIn{atom_ptr[k + 1] - atom_ptr[k], z + atom_ptr[k], xyz_bohr + 3 * (size_t)atom_ptr[k],
+                         charge ? charge[k] : 0.0});
+  RunRequest req;
+  req.want_density = (flags & PM3_BATCH_WANT_DENSITY) != 0;
+  req.want_gradient = (flags & PM3_BATCH_WANT_GRADIENT) != 0;
+  req.force_large = (flags & PM3_BATCH_FORCE_LARGE) != 0;
+  reps = reps < 1 ? 1 : reps;
+  const Tuning start = tuning_table();
+  int nlarge = 0, nsmall = 0;
+  auto measure = [&](const Tuning& t, double& ms, std::vector<MolResult>& res, std::string& err) -> bool {
+    tuning_table() = t;
+    Batch* b = nullptr;
+    bool ok = batch_create(params->p, mols, eo, req, b, err) && batch_solve(b, err);
+    std::vector<double> ts;
+    for (int r = 0; ok && r < reps; ++r) {
+      ok = batch_solve(b, err);
+      ts.push_back(batch_pair_ms(b) + batch_guess_ms(b) + batch_kernel_ms(b));
+    }
+    if (ok) ok = batch_fetch(b, params->p, mols, res, err);
+    if (ok) {
+      nlarge = batch_large_count(b);
+      nsmall = batch_small_count(b);
+    }
+    batch_destroy(b);
+    if (!ok) return false;
+    std::sort(ts.begin(), ts.end());
+    ms = ts[ts.size() / 2];
+    return true;
+  };
+  std::string err;
+  std::vector<MolResult> ref;
+  double t0 = 0.0;
+  if (!measure(start, t0, ref, err)) {
+    tuning_table() = start;
+    return fail(PM3_DEVICE_ERROR, err);
+  }
+  // entries of the persistent kernel (small) and of the host-driven path (large) are tried only
+  // when the batch has molecules there
+  struct Axis {
+    int Tuning::*m;
+    std::vector<int> values;
+    int needs;  // 0 any batch, 1 molecules on the persistent kernel, 2 on the host-driven path
+  };
+  const Axis axes[] = {
+      {&Tuning::pair_block, {64, 128, 256}, 0},
+      {&Tuning::pair_blocks_per_cu, {2, 4, 8, 16, 32}, 0},
+      {&Tuning::sp2_tile16_n, {0, 32, 48, 64, 96, 128, 160, 200}, 1},
+      {&Tuning::sp2_lds, {0, 1, 2}, 1},
+      {&Tuning::sp2_tight_n, {24, 32, 48, 64, 96, 1 << 20}, 1},
+      {&Tuning::large_syrk, {0, 1}, 2},
+  };
+  Tuning best = start;
+  double tbest = t0;
+  int candidates = 1, rejected = 0;
+  const char* vb = std::getenv("PM3_TUNE_VERBOSE");
+  const bool verbose = vb && vb[0] == '1';
+  if (verbose) std::fprintf(stderr, "pm3_autotune: starting table %.3f ms\n", t0);
+  for (const Axis& ax : axes) {
+    if ((ax.needs == 1 && nsmall == 0) || (ax.needs == 2 && nlarge == 0)) continue;
+    const int base = best.*(ax.m);
+    for (int v : ax.values) {
+      if (v == base) continue;
+      Tuning c = best;
+      c.*(ax.m) = v;
+      double tc = 0.0;
+      std::vector<MolResult> res;
+      if (!measure(c, tc, res, err)) {
+        tuning_table() = start;
+        return fail(PM3_DEVICE_ERROR, err);
+      }
+      ++candidates;
+      bool same = res.size() == ref.size();
+      for (size_t k = 0; same && k < res.size(); ++k) {
+        if (res[k].status != ref[k].status) same = false;
+        else if (res[k].status == 0 && std::fabs(res[k].hof_kcal - ref[k].hof_kcal) > 1.0e-5) same = false;
+      }
+      if (verbose) {
+        std::string name = "?";
+        if (ax.m == &Tuning::pair_block) name = "pair_block";
+        else if (ax.m == &Tuning::pair_blocks_per_cu) name = "pair_blocks_per_cu";
+        else if (ax.m == &Tuning::sp2_tile16_n) name = "sp2_tile16_n";
+        else if (ax.m == &Tuning::sp2_lds) name = "sp2_lds";
+        else if (ax.m == &Tuning::sp2_tight_n) name = "sp2_tight_n";
+        else if (ax.m == &Tuning::large_syrk) name = "large_syrk";
+        std::fprintf(stderr, "pm3_autotune: %s %d: %.3f ms%s\n", name.c_str(), v, tc, same ? "" : " (answers differ)");
+      }
+      if (!same) {
+        ++rejected;
+        continue;
+      }
+      if (tc < tbest - std::max(0.002 * tbest, 0.02)) {
+        best = c;
+        tbest = tc;
+      }
+    }
+  }
+  tuning_table() = best;
+  if (best_out) to_c(best, best_out);
+  if (report) {
+    report->default_ms = t0;
+    report->best_ms = tbest;
+    report->candidates = candidates;
+    report->rejected = rejected;
+  }
+  return PM3_OK;
 }
 
 const char* pm3_last_error_message(void) { return g_last_error.c_str(); }

@@ -56,12 +56,10 @@ constexpr int LARGE_MAX_NAO = 16384;    // largest molecule of the host-driven p
 // SCF converges, so the converged state is unchanged.
 constexpr double EIG_TOL_FACTOR = 1.0e-2;
 constexpr double EIG_REL = 1.0e-2;  // cap of the loose tolerance relative to the starting off-diagonal
-constexpr double EIG_SKIP = 0.0;
-constexpr int SP2_LDS = 2;     // purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS
-constexpr int SP2_TIGHT_N = 48;  // from this size on, purification bounds from Lanczos (PM3_SP2_TIGHT_N)
-constexpr double SP2_MARGIN = 0.05;  // their margin, relative to the estimated width (PM3_SP2_MARGIN)
+constexpr double EIG_SKIP = 0.0;  // threshold Jacobi: skip rotations of |a_pq| <= EIG_SKIP x tolerance
+constexpr double SP2_MARGIN = 0.05;  // margin of the Lanczos purification bounds, relative to the width
 constexpr int PH_LAST = 12;   // phase timers: slots [0, 12) phases, [12] last stamp, [13, 16) counters
-constexpr int PH_SLOTS = 16;    // threshold Jacobi: skip rotations of |a_pq| <= EIG_SKIP x tolerance
+constexpr int PH_SLOTS = 16;
 
 __constant__ TableConst c_tab;
 
@@ -112,6 +110,7 @@ struct KArgs {
   int purify;                  // 1: densities by purification, frontier levels by Lanczos (no orbitals)
   int sp2_lds;                 // purification operand placements allowed: 0 HBM only, 1 + LDS pair, 2 + LDS X
   int sp2_tight_n;             // smallest molecule (AOs) whose purification uses Lanczos spectral bounds
+  int sp2_tile16_n;            // purification products on 16 x 16 wave tiles up to this size (AOs), 32 x 32 above
   double sp2_margin;           // margin of those bounds relative to the estimated width (fresh estimate)
   // learned guess: directed pairs (global indices, see batch_create) and network outputs
   const int8_t* elem8;
@@ -270,9 +269,63 @@ __device__ inline void wg_gemm(int M, int N, int K, LA la, LB lb, ST st) {
 // per lane group).  No bounds tests on the loads (padding supplies the zeros; rows beyond np are
 // clamped and their results dropped), and the next 16-wide k chunk is loaded while the MFMAs of
 // the current one run.  Same arithmetic order as wg_gemm, so the results are bitwise equal.
-template <bool SYM = false, class ST>
+// T is the output tile of one wave: 32 x 32 (four 16 x 16 MFMA chains sharing their loads) or
+// 16 x 16 (one chain; four times the tiles, so small n keeps more of the eight waves busy).  Each
+// element is the same chain of MFMAs in the same k order either way, so both give the same bits.
+template <bool SYM = false, int T = 32, class ST>
 __device__ inline void wg_gemm_pad(int n, int np, const double* S, const double* Bm, ST st) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if constexpr (T == 16) {
+    const int bm = (n + 15) >> 4;
+    const int li = lane & 15, lk = lane >> 4;
+    const int nblk = SYM ? bm * (bm + 1) / 2 : bm * bm;
+    const int kp = (n + 15) & ~15;  // <= np
+    for (int t = wave; t < nblk; t += NW) {
+      int bi, bj;
+      if (SYM) {
+        bi = 0;
+        int rest = t;
+        while (rest >= bm - bi) { rest -= bm - bi; ++bi; }
+        bj = bi + rest;
+      } else {
+        bi = t / bm;
+        bj = t % bm;
+      }
+      const int i0 = bi << 4, j0 = bj << 4;
+      const int ia = min(i0 + li, np - 1), ja = min(j0 + li, np - 1);
+      double4_t c = {0.0, 0.0, 0.0, 0.0};
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const size_t r = (size_t)(4 * u + lk) * np;
+        a[u] = S[r + ia];
+        b[u] = Bm[r + ja];
+      }
+      for (int k0 = 0; k0 < kp; k0 += 16) {
+        const int kn = k0 + 16 < kp ? k0 + 16 : k0;  // after the last chunk: a harmless reload
+        double na[4], nb[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const size_t r = (size_t)(kn + 4 * u + lk) * np;
+          na[u] = S[r + ia];
+          nb[u] = Bm[r + ja];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], c, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          a[u] = na[u];
+          b[u] = nb[u];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ra = i0 + 4 * r + lk, ca = j0 + li;
+        if (ra < n && ca < n && (!SYM || bi < bj || ra <= ca)) { st(ra, ca, c[r]); if (SYM && ra != ca) st(ca, ra, c[r]); }
+      }
+    }
+    return;
+  }
   const int bm = (n + 31) >> 5;
   const int li = lane & 15, lk = lane >> 4;
   const int nblk = SYM ? bm * (bm + 1) / 2 : bm * bm;
@@ -370,6 +423,7 @@ struct Mol {
   double eig_skip;  // rotations of elements below eig_skip x tolerance are skipped
   int sp2_lds;      // purification operand placements allowed (KArgs::sp2_lds)
   int sp2_tight_n;  // KArgs::sp2_tight_n
+  int sp2_tile16_n;  // KArgs::sp2_tile16_n
   double sp2_margin;  // KArgs::sp2_margin
 };
 
@@ -1382,12 +1436,14 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
     const double* Xc = X;
     double* Yc = Y;
     double w[2] = {0.0, 0.0};
-    wg_gemm_pad<true>(n, ldx, Xc, Xc, [&](int i, int j, double x2) {
+    auto store = [&](int i, int j, double x2) {
       const double y = sq ? x2 : 2.0 * Xc[i * ldx + j] - x2;
       Yc[i * ldy + j] = y;
       w[1] += y * y;
       if (i == j) w[0] += y;
-    });
+    };
+    if (n <= M.sp2_tile16_n) wg_gemm_pad<true, 16>(n, ldx, Xc, Xc, store);
+    else wg_gemm_pad<true, 32>(n, ldx, Xc, Xc, store);
     block_sum<2>(w, red);
     tr = w[0];
     t2 = w[1];
@@ -2294,6 +2350,7 @@ __host__ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double*
   M.eig_skip = A.eig_skip;
   M.sp2_lds = A.sp2_lds;
   M.sp2_tight_n = A.sp2_tight_n;
+  M.sp2_tile16_n = A.sp2_tile16_n;
   M.sp2_margin = A.sp2_margin;
   return M;
 }
@@ -2994,7 +3051,8 @@ struct Batch {
   int nbufs = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   double last_kernel_ms = 0.0, last_pair_ms = 0.0, last_guess_ms = 0.0;
-  int pair_blocks = 1;
+  int pair_blocks = 1, pair_block = 256;
+  bool large_syrk = false;
   // learned guess: directed pairs of the eligible molecules and their network inputs/outputs
   const GuessModel* guess_model = nullptr;
   bool learned = false, solved_once = false;
@@ -3296,10 +3354,13 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     if (tail < 0 && tr - t2 < SP2_TAIL) tail = SP2_EXTRA;
     if (tail == 0) break;
     const int sq = std::fabs(t2 - occ) < std::fabs(2.0 * tr - t2 - occ) ? 1 : 0;
-    // the whole square (rocBLAS's dgemm runs faster than its dsyrk at these sizes); the update
-    // reads one triangle of it, so X stays exactly symmetric
-    BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, L.V, np, L.V, np, &zero,
-                          L.T, np));
+    // the whole square by dgemm, or one triangle by dsyrk (upper in column-major order = the
+    // row-major lower triangle): the update reads only that triangle, so X stays exactly symmetric
+    if (b.large_syrk)
+      BLAS_OK(rocblas_dsyrk(b.blas, rocblas_fill_upper, rocblas_operation_none, n, n, &one, L.V, np, &zero, L.T, np));
+    else
+      BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, L.V, np, L.V, np,
+                            &zero, L.T, np));
     hipLaunchKernelGGL(lg_sp2_update_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.T, L.V, n, np, sq,
                        L.red);
     HIP_OK(hipGetLastError());
@@ -3862,7 +3923,12 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
       a.prof = d_prof;
     }
   }
-  b.pair_blocks = (int)std::max<long long>(1, std::min<long long>((a.npairs_total + 255) / 256, 8LL * prop.multiProcessorCount));
+  const Tuning tune = tuning_table();
+  b.pair_block = tune.pair_block;
+  b.pair_blocks = (int)std::max<long long>(
+      1, std::min<long long>((a.npairs_total + b.pair_block - 1) / b.pair_block,
+                             (long long)tune.pair_blocks_per_cu * prop.multiProcessorCount));
+  b.large_syrk = tune.large_syrk != 0;
   a.max_scf = opt.max_scf;
   a.e_tol = opt.e_tol;
   a.p_tol = opt.p_tol;
@@ -3877,8 +3943,9 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   if (const char* ev = std::getenv("PM3_EIG_REL")) a.eig_rel = std::atof(ev);
   a.eig_skip = EIG_SKIP;
   a.purify = b.purify ? 1 : 0;
-  a.sp2_lds = SP2_LDS;
-  a.sp2_tight_n = SP2_TIGHT_N;
+  a.sp2_lds = tune.sp2_lds;
+  a.sp2_tight_n = tune.sp2_tight_n;
+  a.sp2_tile16_n = tune.sp2_tile16_n;
   a.sp2_margin = SP2_MARGIN;
   if (const char* ev = std::getenv("PM3_SP2_MARGIN")) a.sp2_margin = std::atof(ev);
   if (const char* ev = std::getenv("PM3_SP2_TIGHT_N")) a.sp2_tight_n = std::atoi(ev);
@@ -3916,7 +3983,7 @@ bool batch_solve(Batch* bp, std::string& err) {
   HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), 0));
   if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, 12 * sizeof(unsigned long long), 0));
   HIP_OK(hipEventRecord(b.ev0, 0));
-  hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(256), 0, 0, b.args);
+  hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, 0, b.args);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(b.evg, 0));
   if (b.learned && !guess_infer(b.guess_model, infer_view(b), err, b.evm)) return false;
@@ -3927,7 +3994,7 @@ bool batch_solve(Batch* bp, std::string& err) {
   for (int k : b.large)
     if (!large_solve(b, k, err)) return false;
   if (b.want_gradient) {
-    hipLaunchKernelGGL(grad_pair_kernel, dim3(b.pair_blocks), dim3(256), 0, 0, b.args);
+    hipLaunchKernelGGL(grad_pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, 0, b.args);
     hipLaunchKernelGGL(grad_gather_kernel, dim3((unsigned)b.order_all.size()), dim3(256), 0, 0, b.args);
     HIP_OK(hipGetLastError());
   }
@@ -3974,6 +4041,8 @@ bool batch_guess_view(Batch* bp, BatchGuessView& v, std::string& err) {
 }
 
 double batch_guess_ms(const Batch* b) { return b ? b->last_guess_ms : 0.0; }
+int batch_large_count(const Batch* b) { return b ? (int)b->large.size() : 0; }
+int batch_small_count(const Batch* b) { return b ? b->nsolve : 0; }
 
 void batch_guess_timing(const Batch* b, double& features_ms, double& network_ms, long long& directed_pairs) {
   features_ms = b ? b->last_feature_ms : 0.0;
@@ -4115,7 +4184,7 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
     if (e != hipSuccess) { err = hipGetErrorString(e); ok = false; }
   }
   if (ok) {
-    hipLaunchKernelGGL(pair_kernel, dim3(b->pair_blocks), dim3(256), 0, 0, b->args);
+    hipLaunchKernelGGL(pair_kernel, dim3(b->pair_blocks), dim3(b->pair_block), 0, 0, b->args);
     if (mode == 5 && !guess_infer(model, infer_view(*b), err)) ok = false;
   }
   if (ok) {

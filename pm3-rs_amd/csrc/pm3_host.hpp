@@ -62,6 +62,25 @@ struct RunRequest {
   bool force_large = false;  // solve every molecule on the host-driven multi-kernel path
 };
 
+// Launch shapes and tile choices.  They change speed only: results agree to rounding whatever
+// the values (pm3_autotune checks that on the workload it tunes).  The process-wide table is read
+// by batch_create; environment knobs (PM3_SP2_LDS, PM3_SP2_TIGHT_N) still override its entries.
+struct Tuning {
+  int pair_block = 256;         // threads per block of the pair and gradient-pair kernels: 64, 128 or 256
+  int pair_blocks_per_cu = 8;   // grid of those grid-stride kernels, blocks per CU: 1 to 64
+  int sp2_tile16_n = 128;       // purification products on 16 x 16 wave tiles up to this size (AOs)
+  int sp2_lds = 2;              // purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS
+  int sp2_tight_n = 48;         // purification bounds from Lanczos from this size (AOs) on
+  int large_syrk = 0;           // host-driven path: purification products by dsyrk (1) or dgemm (0)
+};
+// The table batch_create reads; on first use it is loaded from the file PM3_TUNING_FILE names,
+// if set.  Set or loaded tables apply to batches created afterwards (not thread-safe against a
+// concurrent batch_create).
+Tuning& tuning_table();
+bool tuning_valid(const Tuning& t, std::string& why);
+bool tuning_load(const char* path, Tuning& t, std::string& err);
+bool tuning_save(const char* path, const Tuning& t, std::string& err);
+
 // A batch whose inputs and scratch live on the device; batch_solve is the launch alone.
 struct Batch;
 bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt, const RunRequest& req,
@@ -73,6 +92,8 @@ void batch_destroy(Batch* b);
 double batch_kernel_ms(const Batch* b);  // device time of the last solve's SCF launch (HIP events)
 double batch_pair_ms(const Batch* b);    // device time of the last solve's pair-integral launch
 double batch_guess_ms(const Batch* b);   // device time of the last solve's learned-guess launches
+int batch_large_count(const Batch* b);  // molecules of the batch on the host-driven multi-kernel path
+int batch_small_count(const Batch* b);  // molecules of the batch on the persistent one-workgroup kernel
 // The same split into the feature kernel and the network kernel, and the batch's directed pairs.
 void batch_guess_timing(const Batch* b, double& features_ms, double& network_ms, long long& directed_pairs);
 // Per-phase wall-clock ticks (10 ns) summed over workgroups, when PM3_PHASE_TIMERS=1 at creation:

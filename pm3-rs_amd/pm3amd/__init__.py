@@ -102,7 +102,8 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_guess_model_create", "pm3_guess_model_load", "pm3_guess_model_save", "pm3_guess_model_get_params",
             "pm3_guess_model_set_params", "pm3_guess_model_steps", "pm3_guess_model_destroy", "pm3_guess_data_create",
             "pm3_guess_data_add", "pm3_guess_data_stats", "pm3_guess_data_destroy", "pm3_guess_train_options_default",
-            "pm3_guess_train", "pm3_guess_eval", "pm3_guess_grad", "pm3_guess_mlp", "pm3_guess_features", "pm3_guess_density"]
+            "pm3_guess_train", "pm3_guess_eval", "pm3_guess_grad", "pm3_guess_mlp", "pm3_guess_features", "pm3_guess_density",
+            "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune"]
 
 _lib = None
 
@@ -158,8 +159,83 @@ def native():
         L.pm3_guess_mlp.argtypes = [vp, ctypes.c_int32, up, fp, fp, fp]
         L.pm3_guess_features.argtypes = [ctypes.POINTER(_CMolecule), vp, ip, ip, ip, up]
         L.pm3_guess_density.argtypes = [vp, ctypes.POINTER(_CMolecule), vp, dp, dp, dp]
+        L.pm3_tuning_default.argtypes = [ctypes.POINTER(_CTuning)]
+        L.pm3_tuning_default.restype = None
+        L.pm3_tuning_get.argtypes = [ctypes.POINTER(_CTuning)]
+        L.pm3_tuning_set.argtypes = [ctypes.POINTER(_CTuning)]
+        L.pm3_tuning_load.argtypes = [ctypes.c_char_p]
+        L.pm3_tuning_save.argtypes = [ctypes.c_char_p]
+        L.pm3_autotune.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ctypes.c_void_p, ctypes.POINTER(_COptions),
+                                   ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_CTuning),
+                                   ctypes.POINTER(_CTuneReport)]
         _lib = L
     return _lib
+
+
+class _CTuning(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in ("pair_block", "pair_blocks_per_cu", "sp2_tile16_n", "sp2_lds",
+                                              "sp2_tight_n", "large_syrk")]
+
+
+class _CTuneReport(ctypes.Structure):
+    _fields_ = [("default_ms", ctypes.c_double), ("best_ms", ctypes.c_double), ("candidates", ctypes.c_int32),
+                ("rejected", ctypes.c_int32)]
+
+
+TUNING_KEYS = tuple(k for k, _ in _CTuning._fields_)
+
+
+def tuning_default():
+    """The built-in tuning table (launch shapes and tile choices) as a dict."""
+    t = _CTuning()
+    native().pm3_tuning_default(ctypes.byref(t))
+    return {k: getattr(t, k) for k in TUNING_KEYS}
+
+
+def tuning_get():
+    """The process-wide tuning table that batch creation reads, as a dict."""
+    t = _CTuning()
+    _raise(native().pm3_tuning_get(ctypes.byref(t)))
+    return {k: getattr(t, k) for k in TUNING_KEYS}
+
+
+def tuning_set(table):
+    """Set the process-wide table from a dict (missing keys keep their current values)."""
+    cur = tuning_get()
+    cur.update(table)
+    t = _CTuning(**{k: int(cur[k]) for k in TUNING_KEYS})
+    _raise(native().pm3_tuning_set(ctypes.byref(t)))
+
+
+def tuning_load(path):
+    _raise(native().pm3_tuning_load(os.fsencode(path)))
+
+
+def tuning_save(path):
+    _raise(native().pm3_tuning_save(os.fsencode(path)))
+
+
+def autotune(molecules, params=None, options=None, want_density=False, want_gradient=False, force_large=False,
+             reps=3):
+    """Tune the launch shapes and tile choices on this workload (pm3_autotune): returns
+    (table dict, report dict); the process-wide table is left at the best table found."""
+    params = params or Pm3Parameters.standard()
+    options = options or Pm3Options()
+    ptr = np.zeros(len(molecules) + 1, dtype=np.int32)
+    for k, m in enumerate(molecules):
+        ptr[k + 1] = ptr[k] + len(m.z)
+    z = np.ascontiguousarray(np.concatenate([m.z for m in molecules]).astype(np.int32))
+    xyz = np.ascontiguousarray(np.concatenate([m.positions for m in molecules]))
+    q = np.ascontiguousarray([float(m.charge) for m in molecules], dtype=np.float64)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    t, rep = _CTuning(), _CTuneReport()
+    flags = (1 if want_density else 0) | (2 if want_gradient else 0) | (4 if force_large else 0)
+    _raise(native().pm3_autotune(len(molecules), ptr.ctypes.data_as(ip), z.ctypes.data_as(ip), _dp(xyz), _dp(q),
+                                 params._h, ctypes.byref(options._c()), flags, int(reps), ctypes.byref(t),
+                                 ctypes.byref(rep)))
+    return ({k: getattr(t, k) for k in TUNING_KEYS},
+            {"default_ms": rep.default_ms, "best_ms": rep.best_ms, "candidates": rep.candidates,
+             "rejected": rep.rejected})
 
 
 def _raise(status, res=None):

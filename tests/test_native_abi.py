@@ -70,3 +70,43 @@ def test_parameter_table_matches_reference_csv():
         z = int(vals[0])
         for k, v in zip(keys, vals[1:]):
             assert float(rows[z][k]) == v, (z, k)
+
+
+def test_tuning_table_round_trip_and_validation(tmp_path):
+    """The tuning table is host state: set/get/save/load and range checks need no device."""
+    import pm3amd as G
+    d = G.tuning_default()
+    assert d == {"pair_block": 256, "pair_blocks_per_cu": 8, "sp2_tile16_n": 128, "sp2_lds": 2, "sp2_tight_n": 48,
+                 "large_syrk": 0}
+    start = G.tuning_get()
+    try:
+        G.tuning_set({"pair_block": 128, "sp2_tile16_n": 96})
+        assert G.tuning_get()["pair_block"] == 128 and G.tuning_get()["sp2_tile16_n"] == 96
+        path = str(tmp_path / "t.txt")
+        G.tuning_save(path)
+        G.tuning_set(d)
+        G.tuning_load(path)
+        assert G.tuning_get() == dict(d, pair_block=128, sp2_tile16_n=96)
+        for bad in ({"pair_block": 96}, {"pair_block": 512}, {"sp2_lds": 3}, {"pair_blocks_per_cu": 0}):
+            with pytest.raises(G.InvalidInput):
+                G.tuning_set(bad)
+        assert G.tuning_get() == dict(d, pair_block=128, sp2_tile16_n=96)  # a rejected table changes nothing
+        (tmp_path / "u.txt").write_text("pair_block 64\nno_such_key 1\n")
+        with pytest.raises(G.InvalidInput):
+            G.tuning_load(str(tmp_path / "u.txt"))
+        (tmp_path / "v.txt").write_text("# comment\n  sp2_tight_n 96  # trailing\n\n")
+        G.tuning_load(str(tmp_path / "v.txt"))
+        assert G.tuning_get() == dict(d, sp2_tight_n=96)  # keys not listed take their defaults
+    finally:
+        G.tuning_set(start)
+
+
+def test_tuning_file_from_environment(tmp_path):
+    import subprocess
+    import sys
+    (tmp_path / "t.txt").write_text("pair_blocks_per_cu 4\nsp2_tile16_n 64\n")
+    code = ("import sys; sys.path.insert(0, %r); import pm3amd as G; t = G.tuning_get(); "
+            "print(t['pair_blocks_per_cu'], t['sp2_tile16_n'], t['pair_block'])" % os.path.join(ROOT, "pm3-rs_amd"))
+    env = dict(os.environ, PM3_TUNING_FILE=str(tmp_path / "t.txt"))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert out.stdout.split() == ["4", "64", "256"], out.stderr
