@@ -153,6 +153,11 @@ pm3_status pm3_batch_guess_timing(const pm3_batch* batch, double* features_ms, d
  * purification products, out[14] eigensolves or purifications, out[15] Jacobi rounds or
  * purification fallbacks. */
 pm3_status pm3_batch_phase_ticks(const pm3_batch* batch, uint64_t* out16);
+/* Molecules on the multi-kernel path, last solve: device time (ms, HIP events around each rocBLAS
+ * call), count and FLOP (2 n^3 per dgemm, n^3 per dsyrk) of the purification products, and
+ * device time and count of the Fock builds.  Zeros when the batch has no such molecule. */
+pm3_status pm3_batch_large_timing(const pm3_batch* batch, double* product_ms, int64_t* products,
+                                  int64_t* product_flop, double* fock_ms, int64_t* fock_builds);
 
 /* Launch shapes and tile choices of the kernels, and an autotuner for them.  No reference
  * counterpart (the reference has no device kernels).  The entries change speed only: results

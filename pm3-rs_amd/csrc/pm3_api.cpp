@@ -262,6 +262,21 @@ pm3_status pm3_batch_results(pm3_batch* h, pm3_status* status, pm3_result* resul
 
 double pm3_batch_last_kernel_ms(const pm3_batch* h) { return h ? batch_kernel_ms(h->b) : 0.0; }
 
+pm3_status pm3_batch_large_timing(const pm3_batch* h, double* product_ms, int64_t* products, int64_t* product_flop,
+                                  double* fock_ms, int64_t* fock_builds) {
+  g_last_error.clear();
+  if (!h) return fail(PM3_INVALID_INPUT, "null batch");
+  double pms = 0.0, fms = 0.0;
+  long long np = 0, nf = 0, fl = 0;
+  batch_large_timing(h->b, pms, np, fl, fms, nf);
+  if (product_ms) *product_ms = pms;
+  if (products) *products = np;
+  if (product_flop) *product_flop = fl;
+  if (fock_ms) *fock_ms = fms;
+  if (fock_builds) *fock_builds = nf;
+  return PM3_OK;
+}
+
 double pm3_batch_last_pair_ms(const pm3_batch* h) { return h ? batch_pair_ms(h->b) : 0.0; }
 
 double pm3_batch_last_guess_ms(const pm3_batch* h) { return h ? batch_guess_ms(h->b) : 0.0; }

@@ -2512,6 +2512,37 @@ __global__ __launch_bounds__(256) void grad_gather_kernel(KArgs A) {
   }
 }
 
+// The same sums for molecules of the host-driven path (queue positions first + blockIdx.x): one
+// wave per atom x, lanes over the partner atoms, then a fixed shuffle tree; blockIdx.y strides the
+// atoms in groups of four.
+__global__ __launch_bounds__(256) void grad_gather_wide_kernel(KArgs A, int first) {
+  const MolDesc md = A.mols[A.order[first + blockIdx.x]];
+  const Mol M = mol_of(A, md, nullptr);
+  const int nat = md.nat, lane = threadIdx.x & 63;
+  const long long npairs = (long long)nat * (nat - 1) / 2;
+  for (int x = blockIdx.y * 4 + (threadIdx.x >> 6); x < nat; x += gridDim.y * 4) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int y = lane; y < nat; y += 64) {
+      if (y == x) continue;
+      const long long t = y > x ? tri_index(x, y, nat) : tri_index(y, x, nat);
+      const double sg = y > x ? -1.0 : 1.0;
+      s0 += sg * M.S1[t];
+      s1 += sg * M.S1[npairs + t];
+      s2 += sg * M.S1[2 * npairs + t];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      s0 += __shfl_down(s0, off, 64);
+      s1 += __shfl_down(s1, off, 64);
+      s2 += __shfl_down(s2, off, 64);
+    }
+    if (lane == 0) {
+      A.out_g[3LL * (md.atom0 + x)] = s0;
+      A.out_g[3LL * (md.atom0 + x) + 1] = s1;
+      A.out_g[3LL * (md.atom0 + x) + 2] = s2;
+    }
+  }
+}
+
 // Mulliken charges (into out_q) and the dipole about the centre of mass, atomic units
 // (src/scf.rs:762-791, src/dipole.rs:64-140).  One workgroup; P has leading dimension M.np.
 __device__ inline void charges_dipole(const KArgs& A, const Mol& M, const double* P, double* red, double (&dip)[3]) {
@@ -2866,13 +2897,19 @@ __global__ __launch_bounds__(LG_THREADS) void lg_twice_kernel(const double* X, d
   }
 }
 
-// Sums of the first k columns of the block partials, blocks in order, into out[0..k) (one thread
-// per column), so that the host reads k values instead of every block's.
-__global__ __launch_bounds__(64) void lg_finish_kernel(const double* red, int k, double* out) {
-  if (threadIdx.x < k) {
-    double t = 0.0;
-    for (int b = 0; b < LG_BLOCKS; ++b) t += red[(size_t)b * LG_MAXDOT + threadIdx.x];
-    out[threadIdx.x] = t;
+// Sums of the first k columns of the block partials into out[0..k), so that the host reads k
+// values instead of every block's: one thread per block partial and a fixed halving tree.
+__global__ __launch_bounds__(LG_BLOCKS) void lg_finish_kernel(const double* red, int k, double* out) {
+  __shared__ double t[LG_BLOCKS];
+  for (int q = 0; q < k; ++q) {
+    t[threadIdx.x] = red[(size_t)threadIdx.x * LG_MAXDOT + q];
+    __syncthreads();
+    for (int h = LG_BLOCKS / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) t[threadIdx.x] += t[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = t[0];
+    __syncthreads();
   }
 }
 
@@ -3028,6 +3065,10 @@ struct LargeBufs {
   int* info = nullptr;
   long long m2 = 0;  // elements per matrix (np_max^2)
   int depth = 0;     // DIIS history slots allocated
+  // device time of the purification products and of the Fock builds in the last solve
+  hipEvent_t evp0 = nullptr, evp1 = nullptr, evf0 = nullptr, evf1 = nullptr;
+  double product_ms = 0.0, fock_ms = 0.0;
+  long long products = 0, fock_builds = 0, product_flop = 0;
 };
 
 struct Batch {
@@ -3071,6 +3112,8 @@ struct Batch {
     if (evg) (void)hipEventDestroy(evg);
     if (evm) (void)hipEventDestroy(evm);
     if (blas) (void)rocblas_destroy_handle(blas);
+    for (hipEvent_t e : {lg.evp0, lg.evp1, lg.evf0, lg.evf1})
+      if (e) (void)hipEventDestroy(e);
   }
   bool ensure_guess_buffers(std::string& err) {
     if (d_X || ngp_total == 0) return true;
@@ -3155,6 +3198,10 @@ static bool large_setup(Batch& b, std::string& err) {
   b.h_red.resize((size_t)LG_BLOCKS * LG_MAXDOT);
   BLAS_OK(rocblas_create_handle(&b.blas));
   BLAS_OK(rocblas_set_stream(b.blas, 0));
+  HIP_OK(hipEventCreate(&L.evp0));
+  HIP_OK(hipEventCreate(&L.evp1));
+  HIP_OK(hipEventCreate(&L.evf0));
+  HIP_OK(hipEventCreate(&L.evf1));
   return true;
 }
 
@@ -3183,11 +3230,19 @@ static bool large_sum1(Batch& b, double& out, std::string& err) {
 }
 
 static bool large_fock(Batch& b, const MolDesc& d, const double* P, double* F, std::string& err) {
+  LargeBufs& L = b.lg;
+  HIP_OK(hipEventRecord(L.evf0, 0));
   hipLaunchKernelGGL(lg_fock_pairs_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, F, b.lg.slot);
   hipLaunchKernelGGL(lg_fock_jsum_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, b.lg.J, b.lg.slot);
   hipLaunchKernelGGL(lg_fock_diag_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, F,
                      (const double*)b.lg.J);
   HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(L.evf1, 0));
+  HIP_OK(hipEventSynchronize(L.evf1));
+  float ms = 0.0f;
+  HIP_OK(hipEventElapsedTime(&ms, L.evf0, L.evf1));
+  L.fock_ms += ms;
+  ++L.fock_builds;
   return true;
 }
 
@@ -3265,7 +3320,8 @@ static bool large_extremes(Batch& b, const MolDesc& d, const double* A, double& 
   const double one = 1.0, zero = 0.0;
   const int steps = std::min(n, SP2_LZ_STEPS);
   for (int k = 0; k < steps; ++k) {
-    BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_none, n, n, &one, A, np, q, 1, &zero, w, 1));  // A symmetric
+    // A symmetric: A^T q, whose kernel reads the column-major columns along rows of A (coalesced)
+    BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_transpose, n, n, &one, A, np, q, 1, &zero, w, 1));
     const double mb = -beta;
     BLAS_OK(rocblas_daxpy(b.blas, n, &mb, qp, 1, w, 1));
     double alpha = 0.0;
@@ -3336,7 +3392,7 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
   const int n = d.nao, np = d.np, nocc = d.nocc;
   ok = false;
   auto sums = [&](double& tr, double& t2) -> bool {
-    hipLaunchKernelGGL(lg_finish_kernel, dim3(1), dim3(64), 0, 0, (const double*)L.red, 2, L.fin);
+    hipLaunchKernelGGL(lg_finish_kernel, dim3(1), dim3(LG_BLOCKS), 0, 0, (const double*)L.red, 2, L.fin);
     double h[2];
     HIP_OK(hipMemcpy(h, L.fin, sizeof(h), hipMemcpyDeviceToHost));
     tr = h[0];
@@ -3354,6 +3410,7 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     if (tail < 0 && tr - t2 < SP2_TAIL) tail = SP2_EXTRA;
     if (tail == 0) break;
     const int sq = std::fabs(t2 - occ) < std::fabs(2.0 * tr - t2 - occ) ? 1 : 0;
+    HIP_OK(hipEventRecord(L.evp0, 0));
     // the whole square by dgemm, or one triangle by dsyrk (upper in column-major order = the
     // row-major lower triangle): the update reads only that triangle, so X stays exactly symmetric
     if (b.large_syrk)
@@ -3361,10 +3418,16 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     else
       BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, L.V, np, L.V, np,
                             &zero, L.T, np));
+    HIP_OK(hipEventRecord(L.evp1, 0));
     hipLaunchKernelGGL(lg_sp2_update_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.T, L.V, n, np, sq,
                        L.red);
     HIP_OK(hipGetLastError());
-    if (!sums(tr, t2)) return false;
+    if (!sums(tr, t2)) return false;  // (synchronises, so the product's events have completed)
+    float pms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&pms, L.evp0, L.evp1));
+    L.product_ms += pms;
+    ++L.products;
+    L.product_flop += (b.large_syrk ? 1LL : 2LL) * n * (long long)n * n;
     if (tail > 0) --tail;
   }
   if (tail != 0 || std::fabs(tr - occ) > 1.0e-6) return true;
@@ -3972,6 +4035,8 @@ static BatchGuessView infer_view(Batch& b) {
 bool batch_solve(Batch* bp, std::string& err) {
   Batch& b = *bp;
   b.last_kernel_ms = 0.0;
+  b.lg.product_ms = b.lg.fock_ms = 0.0;
+  b.lg.products = b.lg.fock_builds = b.lg.product_flop = 0;
   if (b.order_all.empty()) return true;
   if (!b.ev0) {
     HIP_OK(hipEventCreate(&b.ev0));
@@ -3995,7 +4060,13 @@ bool batch_solve(Batch* bp, std::string& err) {
     if (!large_solve(b, k, err)) return false;
   if (b.want_gradient) {
     hipLaunchKernelGGL(grad_pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, 0, b.args);
-    hipLaunchKernelGGL(grad_gather_kernel, dim3((unsigned)b.order_all.size()), dim3(256), 0, 0, b.args);
+    if (b.nsolve > 0) hipLaunchKernelGGL(grad_gather_kernel, dim3((unsigned)b.nsolve), dim3(256), 0, 0, b.args);
+    if (!b.large.empty()) {
+      int nat = 1;
+      for (int k : b.large) nat = std::max(nat, b.desc[k].nat);
+      hipLaunchKernelGGL(grad_gather_wide_kernel, dim3((unsigned)b.large.size(), (unsigned)std::min(256, (nat + 3) / 4)),
+                         dim3(256), 0, 0, b.args, b.nsolve);
+    }
     HIP_OK(hipGetLastError());
   }
   HIP_OK(hipEventRecord(b.ev2, 0));
@@ -4043,6 +4114,14 @@ bool batch_guess_view(Batch* bp, BatchGuessView& v, std::string& err) {
 double batch_guess_ms(const Batch* b) { return b ? b->last_guess_ms : 0.0; }
 int batch_large_count(const Batch* b) { return b ? (int)b->large.size() : 0; }
 int batch_small_count(const Batch* b) { return b ? b->nsolve : 0; }
+void batch_large_timing(const Batch* b, double& product_ms, long long& products, long long& product_flop,
+                        double& fock_ms, long long& fock_builds) {
+  product_ms = b ? b->lg.product_ms : 0.0;
+  products = b ? b->lg.products : 0;
+  product_flop = b ? b->lg.product_flop : 0;
+  fock_ms = b ? b->lg.fock_ms : 0.0;
+  fock_builds = b ? b->lg.fock_builds : 0;
+}
 
 void batch_guess_timing(const Batch* b, double& features_ms, double& network_ms, long long& directed_pairs) {
   features_ms = b ? b->last_feature_ms : 0.0;

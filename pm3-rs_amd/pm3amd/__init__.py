@@ -103,7 +103,8 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_guess_model_set_params", "pm3_guess_model_steps", "pm3_guess_model_destroy", "pm3_guess_data_create",
             "pm3_guess_data_add", "pm3_guess_data_stats", "pm3_guess_data_destroy", "pm3_guess_train_options_default",
             "pm3_guess_train", "pm3_guess_eval", "pm3_guess_grad", "pm3_guess_mlp", "pm3_guess_features", "pm3_guess_density",
-            "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune"]
+            "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune",
+            "pm3_batch_large_timing"]
 
 _lib = None
 
@@ -139,6 +140,8 @@ def native():
         L.pm3_batch_last_guess_ms.restype = ctypes.c_double
         L.pm3_batch_guess_timing.argtypes = [ctypes.c_void_p, dp, dp, ctypes.POINTER(ctypes.c_int64)]
         L.pm3_batch_phase_ticks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
+        L.pm3_batch_large_timing.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(ctypes.c_int64),
+                                             ctypes.POINTER(ctypes.c_int64), dp, ctypes.POINTER(ctypes.c_int64)]
         L.pm3_last_error_message.restype = ctypes.c_char_p
         vp, fp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)
         up, i64p = ctypes.POINTER(ctypes.c_uint16), ctypes.POINTER(ctypes.c_int64)
@@ -453,6 +456,16 @@ class Batch:
         f, n, p = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
         _raise(native().pm3_batch_guess_timing(self._h, ctypes.byref(f), ctypes.byref(n), ctypes.byref(p)))
         return f.value, n.value, p.value
+
+    def large_timing(self):
+        """Molecules on the multi-kernel path, last solve: purification products (ms, count, FLOP)
+        and Fock builds (ms, count), from HIP events around each step."""
+        pm, fm = ctypes.c_double(), ctypes.c_double()
+        np_, fl, nf = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _raise(native().pm3_batch_large_timing(self._h, ctypes.byref(pm), ctypes.byref(np_), ctypes.byref(fl),
+                                               ctypes.byref(fm), ctypes.byref(nf)))
+        return {"product_ms": pm.value, "products": np_.value, "product_flop": fl.value, "fock_ms": fm.value,
+                "fock_builds": nf.value}
 
     PHASES = ("hcore", "fock", "energy_diis_gram", "products", "eigen_or_purify", "sort_or_frontier", "density",
               "other", "diis_solve", "diis_combine")

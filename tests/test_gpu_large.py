@@ -124,3 +124,24 @@ def test_water512_invariants(G):
     # translational invariance of the analytic gradient
     assert np.abs(g.gradient.sum(axis=0)).max() < 1e-8
     assert abs(r.total_ev - (r.electronic_ev + r.core_ev)) < 1e-9
+
+
+def test_large_path_timing_counters(G):
+    """The per-step timers of the multi-kernel path count what ran: one product of 2 n^3 FLOP per
+    purification step, at least one Fock build per SCF iteration; zeros for batched molecules."""
+    z, x = W.water_cluster(40, W.water_box_for(40), 41)
+    b = G.Batch([G.Molecule(np.asarray(z, np.int32), W.to_bohr(x))], options=G.Pm3Options(p_tol=1e-8, e_tol=1e-8),
+                want_density=True)
+    b.solve()
+    st, res = b.results()
+    t = b.large_timing()
+    b.close()
+    print("timing", t, "iterations", res[0].iterations, flush=True)
+    assert st[0] == 0
+    assert t["products"] >= 10 * res[0].iterations and t["product_ms"] > 0.0
+    assert t["product_flop"] == 2 * 240 ** 3 * t["products"]
+    assert t["fock_builds"] >= res[0].iterations and t["fock_ms"] > 0.0
+    s = G.Batch([G.Molecule(z2, x2) for z2, x2 in _mols(4, 5)])
+    s.solve()
+    assert s.large_timing() == {"product_ms": 0.0, "products": 0, "product_flop": 0, "fock_ms": 0.0, "fock_builds": 0}
+    s.close()
