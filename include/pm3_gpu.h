@@ -149,7 +149,8 @@ pm3_status pm3_batch_guess_timing(const pm3_batch* batch, double* features_ms, d
  * the environment has PM3_PHASE_TIMERS=1 at pm3_batch_create (zeros otherwise): out[0..9] H_core
  * finish, Fock, energy + DIIS history and Gram sums, dense products (F P, warm-start transform),
  * eigensolve or purification, sort + C V or frontier levels, density change, bookkeeping, DIIS
- * coefficient solve, DIIS extrapolation; out[10..12] unused; out[13] Jacobi sweeps or
+ * coefficient solve, DIIS extrapolation; out[10] frontier-level Lanczos steps, out[11] frontier-level
+ * Lanczos runs (purified route); out[12] unused; out[13] Jacobi sweeps or
  * purification products, out[14] eigensolves or purifications, out[15] Jacobi rounds or
  * purification fallbacks. */
 pm3_status pm3_batch_phase_ticks(const pm3_batch* batch, uint64_t* out16);

@@ -1261,7 +1261,9 @@ __device__ void adiis_wave(Diis& D, double* srt) {
 // ------------------------------------------------------------------------------------------
 struct Bufs {
   double *H, *F, *P, *Pn, *Fu, *T, *C, *Cn, *V, *Pb, *Cb, *Fh, *Eh, *Dh, *eps, *epsb, *lam;
-  double* bnd;  // LDS: cached spectral-bound estimates of the purification ([0] lo, [1] hi, [2] age)
+  double *fs, *ft;  // frontier-level seed s of the last purification, and X s
+  double* bnd;  // LDS: cached spectral-bound estimates of the purification ([0] lo, [1] hi, [2] age),
+                // [3] 1 once a purification of this path has written the frontier seed
   int* perm;
   unsigned long long* ph;  // LDS: [0..11] per-phase wall-clock sums, [12] last stamp, [13..15] counters; null when off
 };
@@ -1391,11 +1393,39 @@ __host__ __device__ constexpr int sp2_kp(int n) { return (n + 15) & ~15; }
 __host__ __device__ constexpr int sp2_ld(int n) { return (sp2_kp(n) & 31) == 0 ? sp2_kp(n) + 16 : sp2_kp(n); }
 __host__ __device__ constexpr int sp2_img(int n) { return sp2_kp(n) * sp2_ld(n); }
 
+// Fixed generic vector of the frontier seed (cheap to evaluate: one per row and thread).
+__device__ inline double seed_r(int j) {
+  const double x = 0.6180339887498949 * (double)j + 0.1;
+  return 1.0 + 0.5 * (x - floor(x));
+}
+
+// Frontier seed s = (X - X^2) r from the purification iterate X at the start of its stopping tail.
+// X = p(Fu) for a polynomial p that is increasing on [0, 1] and splits the spectrum at the gap, so
+// x (1 - x) is largest at the levels next to the gap (the HOMO among the occupied levels, the LUMO
+// among the virtual ones): s is dominated by the frontier directions.  One thread per row; X is
+// symmetric, so row i is read as column i (consecutive threads, consecutive addresses).
+__device__ void frontier_seed(const double* X, int ld, int n, const Bufs& B) {
+  const int tid = threadIdx.x;
+  double t = 0.0;
+  if (tid < n)
+    for (int j = 0; j < n; ++j) t += X[j * ld + tid] * seed_r(j);
+  if (tid < n) B.ft[tid] = t;
+  __syncthreads();
+  double u = 0.0;
+  if (tid < n)
+    for (int j = 0; j < n; ++j) u += X[j * ld + tid] * B.ft[j];
+  __syncthreads();
+  if (tid < n) B.fs[tid] = t - u;
+  if (tid == 0 && B.bnd) B.bnd[3] = 1.0;
+  __syncthreads();
+}
+
 // The purification loop for one placement of the operands: LM = 2 both X and the product in LDS
 // (ping-pong of two images), LM = 1 X in LDS and the product to HBM (B.T, copied back into the
 // image after each step), LM = 0 both in HBM (B.Cn / B.T).
 template <int LM>
-__device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo, double hi) {
+__device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo, double hi,
+                       bool seed) {
   const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
   const int img = sp2_img(n), ldl = sp2_ld(n);
   const double s = 1.0 / (hi - lo);
@@ -1430,7 +1460,10 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
   for (; k < SP2_MAX; ++k) {
     // every eigenvalue of X in [0, 1] keeps tr X^2 <= tr X; more means a bound was violated
     if (t2 > tr + 1.0e-9 * fmax(1.0, tr)) return -2;
-    if (tail < 0 && tr - t2 < SP2_TAIL) tail = SP2_EXTRA;
+    if (tail < 0 && tr - t2 < SP2_TAIL) {
+      tail = SP2_EXTRA;
+      if (seed) frontier_seed(X, ldx, n, B);
+    }
     if (tail == 0) break;
     const bool sq = fabs(t2 - occ) < fabs(2.0 * tr - t2 - occ);
     const double* Xc = X;
@@ -1474,13 +1507,14 @@ __host__ __device__ constexpr int lanczos_lds(int n);
 __device__ __noinline__ void lanczos_extremes(const double* Ag, int n, int np, double* v, double& lo_est, double& hi_est);
 constexpr double SP2_MARGIN_AGED = 0.03;  // extra margin when the estimate is from an earlier SCF step
 constexpr int SP2_REFRESH = 4;         // estimate again every this many steps (and the first two)
+constexpr double FRONTIER_SEED_DP = 1.0e-4;  // RMS density change below which purifications seed the frontier runs
 
 __device__ inline int sp2_dispatch(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo,
-                                   double hi) {
+                                   double hi, bool seed) {
   const int img = sp2_img(M.n);
-  if (M.sp2_lds >= 1 && 2 * img <= M.lds_doubles) return sp2_run<2>(M, B, Fu, Pdst, red, lo, hi);
-  if (M.sp2_lds >= 2 && img <= M.lds_doubles) return sp2_run<1>(M, B, Fu, Pdst, red, lo, hi);
-  return sp2_run<0>(M, B, Fu, Pdst, red, lo, hi);
+  if (M.sp2_lds >= 1 && 2 * img <= M.lds_doubles) return sp2_run<2>(M, B, Fu, Pdst, red, lo, hi, seed);
+  if (M.sp2_lds >= 2 && img <= M.lds_doubles) return sp2_run<1>(M, B, Fu, Pdst, red, lo, hi, seed);
+  return sp2_run<0>(M, B, Fu, Pdst, red, lo, hi, seed);
 }
 
 // Aufbau density of the symmetric matrix Fu by purification instead of eigenvectors: the
@@ -1501,7 +1535,7 @@ __device__ inline int sp2_dispatch(const Mol& M, const Bufs& B, const double* Fu
 // (sp2_run); P goes to Pdst.  Returns the number of products, or -1 when the error has not fallen
 // within SP2_MAX products (frontier orbitals degenerate or nearly so); the caller then diagonalises.
 __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red,
-                                        bool tight) {
+                                        bool tight, bool seed = false) {
   const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
   if (nocc == 0 || nocc == n) {
     for (int e = tid; e < n * n; e += NT) {
@@ -1530,12 +1564,12 @@ __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const doubl
     const double m = mf * (he - le);
     const double lo = fmax(glo, le - m), hi = fmin(ghi, he + m);
     int r = -1;
-    if (hi - lo < ghi - glo && hi - lo > 1.0e-12 * fmax(1.0, fabs(hi))) r = sp2_dispatch(M, B, Fu, Pdst, red, lo, hi);
+    if (hi - lo < ghi - glo && hi - lo > 1.0e-12 * fmax(1.0, fabs(hi))) r = sp2_dispatch(M, B, Fu, Pdst, red, lo, hi, seed);
     if (r < 0 && tid == 0) B.bnd[2] = 0.0;  // a miss: estimate afresh next time
     __syncthreads();
     if (r >= 0) return r;
   }
-  const int r = sp2_dispatch(M, B, Fu, Pdst, red, glo, ghi);
+  const int r = sp2_dispatch(M, B, Fu, Pdst, red, glo, ghi, seed);
   return r >= 0 ? r : -1;
 }
 
@@ -1680,20 +1714,40 @@ __device__ __forceinline__ void lanczos_step(const double* La, bool sq, const do
   beta = sqrt(b2);
 }
 
-// Start vector of the Lanczos runs: a fixed spread over all basis directions in w (its norm
-// returned), q zeroed.
-__device__ inline double lanczos_start(int n, double* v) {
+// Start vector of the Lanczos runs in w (its norm returned), q zeroed: a fixed spread over all
+// basis directions, or, with a seed (side 1: t, side 2: s - t), the normalised seed plus 1e-3 of
+// the normalised spread, so that no direction is missing from the start (a seed that is zero or
+// not finite falls back to the spread alone).
+__device__ inline double lanczos_start(int n, double* v, const double* s = nullptr, const double* t = nullptr,
+                                       int side = 0) {
   const int tid = threadIdx.x, lane = tid & 63;
   double* q = v;
   double* w = q + n;
+  double ca = 0.0, cg = 1.0;
+  if (side != 0) {
+    double a2 = 0.0, g2 = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      const double x = side == 1 ? t[i] : s[i] - t[i];
+      const double g = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);
+      a2 += x * x;
+      g2 += g * g;
+    }
+    a2 = wave_sum(a2);  // every wave sums the same vectors
+    g2 = wave_sum(g2);
+    if (a2 > 1.0e-280 && a2 < 1.0e300) {
+      ca = 1.0 / sqrt(a2);
+      cg = 1.0e-3 / sqrt(g2);
+    }
+  }
   for (int i = tid; i < n; i += NT) {
-    w[i] = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);
+    const double g = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);
+    w[i] = ca != 0.0 ? ca * (side == 1 ? t[i] : s[i] - t[i]) + cg * g : g;
     q[i] = 0.0;
   }
   __syncthreads();
   double nn = 0.0;
   for (int i = lane; i < n; i += 64) nn += w[i] * w[i];
-  return sqrt(wave_sum(nn));  // every wave sums the same vector
+  return sqrt(wave_sum(nn));
 }
 
 // Largest eigenvalue of the symmetric matrix A (n x n, n <= 256) by a plain Lanczos run (extreme
@@ -1702,14 +1756,17 @@ __device__ inline double lanczos_start(int n, double* v) {
 // multisection (the previous value is a lower bound: by interlacing it never decreases) and the
 // run stops when the Ritz residual is below LZ_TOL or the value has stalled over 8 and 16 steps (a
 // converged value gains ghost copies, whose near-degeneracy spoils the residual estimate).
+// A seeded start (side != 0, lanczos_start) is checked from the eighth step on, every fourth.
 __device__ __forceinline__ double lanczos_top(const double* La, bool sq, const double* Ag, int n, int np, double scale,
-                                              double* v) {
+                                              double* v, const double* s = nullptr, const double* t = nullptr,
+                                              int side = 0, unsigned long long* steps = nullptr) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cap = lanczos_cap(n);
   double* al = v + 11 * n;
   double* be = al + (cap + 2);
   double* sh = be + (cap + 2);  // [0, 16) reductions, [16] value, [17] stop
-  double wn = lanczos_start(n, v);
+  double wn = lanczos_start(n, v, s, t, side);
+  const int first = side != 0 ? 8 : 16, every = side != 0 ? 3 : 7;
   double bsub = 0.0, th1 = -1.0e300, th2 = -1.0e300, top = 0.0;
   for (int k = 0; k < cap; ++k) {
     double alpha, beta;
@@ -1718,7 +1775,7 @@ __device__ __forceinline__ double lanczos_top(const double* La, bool sq, const d
     const bool breakdown = beta <= 1.0e-13 * scale;
     wn = beta;
     bsub = beta;
-    const bool check = mk == cap || breakdown || (mk >= 16 && (mk & 7) == 0);
+    const bool check = mk == cap || breakdown || (mk >= first && (mk & every) == 0);
     if (tid == 0) {
       al[k] = alpha;
       be[k] = beta;
@@ -1758,7 +1815,10 @@ __device__ __forceinline__ double lanczos_top(const double* La, bool sq, const d
     top = sh[16];
     const bool stop = sh[17] != 0.0;
     __syncthreads();
-    if (stop) break;
+    if (stop) {
+      if (steps && tid == 0) *steps += (unsigned long long)mk;
+      break;
+    }
   }
   return top;
 }
@@ -1865,8 +1925,20 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
                   Av[i * np + j] = -(Fu[i * np + j] - G[i * np + j] - G[j * np + i] + sv + c_hi * x);
                 });
   __syncthreads();
+  // seeds from the last purification: t = X s, HOMO start t, LUMO start s - t (X = P / 2)
+  const int seeded = (B.bnd && B.bnd[3] != 0.0) ? 1 : 0;
+  if (seeded) {
+    if (tid < n) {
+      double t = 0.0;
+      for (int j = 0; j < n; ++j) t += 0.5 * P[j * np + tid] * B.fs[j];
+      B.ft[tid] = t;
+    }
+    __syncthreads();
+  }
+  unsigned long long* lsteps = B.ph ? &B.ph[10] : nullptr;
+  if (B.ph && tid == 0) B.ph[11] += (unsigned long long)((nocc > 0) + (nocc < n));
   double homo = 0.0, lumo = 0.0;
-  if (nocc > 0) homo = lanczos_top(Lp, square, Ao, n, np, scale, vec);
+  if (nocc > 0) homo = lanczos_top(Lp, square, Ao, n, np, scale, vec, B.fs, B.ft, seeded, lsteps);
   if (nocc < n) {
     if (square) {
       for (int e = tid; e < n * n; e += NT) Lp[e] = Av[(e / n) * np + e % n];
@@ -1880,7 +1952,7 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
       }
       __syncthreads();
     }
-    lumo = -lanczos_top(Lp, square, Av, n, np, scale, vec);
+    lumo = -lanczos_top(Lp, square, Av, n, np, scale, vec, B.fs, B.ft, seeded ? 2 : 0, lsteps);
   }
   if (tid == 0) {
     if (nocc > 0) B.eps[nocc - 1] = homo;
@@ -2048,7 +2120,10 @@ struct PathResult {
 __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int guess, int accel, double* lds,
                                double* red, Diis& D, bool tight = true) {
   const int n = M.n, np = M.np, tid = threadIdx.x;
-  if (tid == 0) B.bnd[2] = -1.0;  // no spectral-bound estimate for this path yet
+  if (tid == 0) {
+    B.bnd[2] = -1.0;  // no spectral-bound estimate for this path yet
+    B.bnd[3] = 0.0;   // nor a frontier seed
+  }
   __syncthreads();
   const int gused = initial_guess(A, M, B, guess, lds, red, tight);
   // DIIS depth from the memory budget (src/scf.rs:309-327, :1205-1228)
@@ -2232,7 +2307,9 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
     stamp(B, 2);
     Flast = Fuse;
     if (A.purify) {
-      const int ns = sp2_density(M, B, Fuse, B.Pn, red, tight);
+      // the frontier seed is only needed from the path's last purification: take it once the SCF is
+      // close to converged
+      const int ns = sp2_density(M, B, Fuse, B.Pn, red, tight, R.last_dp < FRONTIER_SEED_DP);
       if (ns < 0) {  // no open gap: diagonalise
         eigensolve(M, B, Fuse, false, lds, red);
         density(M, B.Cn, B.Pn);
@@ -2318,6 +2395,8 @@ __device__ inline Bufs slab(const KArgs& A, double*& Wbase) {
   B.epsb = B.eps + A.np_max;
   B.lam = B.epsb + A.np_max;
   B.perm = (int*)(B.lam + A.np_max);
+  B.fs = B.lam + 2 * A.np_max;
+  B.ft = B.fs + A.np_max;
   return B;
 }
 
@@ -2702,7 +2781,10 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
   __shared__ double s_bnd[4];
   Bufs B = slab(A, Wbase);
   B.bnd = s_bnd;
-  if (tid == 0) s_bnd[2] = -1.0;
+  if (tid == 0) {
+    s_bnd[2] = -1.0;
+    s_bnd[3] = 0.0;
+  }
   const Mol M = mol_of(A, A.mols[A.order[0]], Wbase);
   const int n = M.n, np = M.np;
   zero_mats(M, B);
@@ -2749,7 +2831,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(KArgs A, int mode, const doub
   } else if (mode == 6) {
     for (int e = tid; e < n * n; e += NT) B.Fu[(e / n) * np + e % n] = in[e];
     __syncthreads();
-    const int ns = sp2_density(M, B, B.Fu, B.P, red, true);
+    const int ns = sp2_density(M, B, B.Fu, B.P, red, true, true);
     if (ns >= 0) frontier_levels(M, B, B.Fu, B.P, lds, red);
     for (int e = tid; e < n * n; e += NT) out[e] = B.P[(e / n) * np + e % n];
     if (tid == 0) {
@@ -3900,7 +3982,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   HIP_OK(hipGetDeviceProperties(&prop, dev));
   b.nwg = std::max(1, std::min(std::max(b.nsolve, 1), prop.multiProcessorCount));
   const long long m2 = (long long)np_max * np_max;
-  const long long stride = ((11 + 3 * MAXD) * m2 + w_cap + 4LL * np_max + 64 + 31) & ~31LL;
+  const long long stride = ((11 + 3 * MAXD) * m2 + w_cap + 6LL * np_max + 64 + 31) & ~31LL;
   KArgs& a = b.args;
   MolDesc* d_desc; int *d_order, *d_queue, *d_aslot, *d_aoff;
   double *d_ax, *d_ay, *d_az, *d_scr, *d_P, *d_C, *d_eps, *d_q, *d_store, *d_g;
@@ -4046,7 +4128,7 @@ bool batch_solve(Batch* bp, std::string& err) {
     HIP_OK(hipEventCreate(&b.evm));
   }
   HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), 0));
-  if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, 12 * sizeof(unsigned long long), 0));
+  if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, PH_SLOTS * sizeof(unsigned long long), 0));
   HIP_OK(hipEventRecord(b.ev0, 0));
   hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, 0, b.args);
   HIP_OK(hipGetLastError());

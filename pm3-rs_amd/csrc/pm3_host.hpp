@@ -102,7 +102,9 @@ void batch_large_timing(const Batch* b, double& product_ms, long long& products,
 void batch_guess_timing(const Batch* b, double& features_ms, double& network_ms, long long& directed_pairs);
 // Per-phase wall-clock ticks (10 ns) summed over workgroups, when PM3_PHASE_TIMERS=1 at creation:
 // 0 H_core finish, 1 Fock, 2 energy+DIIS, 3 warm-start products, 4 Jacobi, 5 sort+C V, 6 density,
-// 7 loop bookkeeping and epilogue; then [8] unused, [9] Jacobi sweeps, [10] eigensolves, [11] rounds.
+// 7 other, 8 DIIS coefficient solve, 9 DIIS extrapolation; [10] frontier Lanczos steps, [11] frontier
+// Lanczos runs, [13] Jacobi sweeps or purification products, [14] eigensolves or purifications,
+// [15] Jacobi rounds or purification fallbacks (16 slots).
 bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err);
 
 // One stage of the pipeline on a single molecule (kernel-level parity hooks); see stage_kernel.

@@ -476,7 +476,8 @@ class Batch:
         _raise(native().pm3_batch_phase_ticks(self._h, out))
         d = dict(zip(self.PHASES, [int(v) for v in out[:10]]))
         d["counts"] = {"sweeps_or_products": int(out[13]), "eigensolves_or_purifications": int(out[14]),
-                       "rounds_or_fallbacks": int(out[15])}
+                       "rounds_or_fallbacks": int(out[15]), "frontier_lanczos_steps": int(out[10]),
+                       "frontier_lanczos_runs": int(out[11])}
         return d
 
     def results(self):
