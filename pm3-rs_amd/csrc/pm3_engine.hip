@@ -108,7 +108,8 @@ struct KArgs {
   double eig_rel;
   double eig_skip;
   int purify;                  // 1: densities by purification, frontier levels by Lanczos (no orbitals)
-  int sp2_lds;                 // purification operand placements allowed: 0 HBM only, 1 + LDS pair, 2 + LDS X
+  int sp2_lds;                 // purification operand placements allowed: 0 HBM only, 1 + LDS pair, 2 + LDS X,
+                               // 3 + LDS X overwritten in place
   int sp2_tight_n;             // smallest molecule (AOs) whose purification uses Lanczos spectral bounds
   int sp2_tile16_n;            // purification products on 16 x 16 wave tiles up to this size (AOs), 32 x 32 above
   double sp2_margin;           // margin of those bounds relative to the estimated width (fresh estimate)
@@ -264,84 +265,39 @@ __device__ inline void wg_gemm(int M, int N, int K, LA la, LB lb, ST st) {
   }
 }
 
-// C = S B for n x n operands held np-strided with zero padding up to np rows and columns, S
-// symmetric (read as S[k][i], so that both operands are read along rows: 16 consecutive doubles
-// per lane group).  No bounds tests on the loads (padding supplies the zeros; rows beyond np are
-// clamped and their results dropped), and the next 16-wide k chunk is loaded while the MFMAs of
-// the current one run.  Same arithmetic order as wg_gemm, so the results are bitwise equal.
-// T is the output tile of one wave: 32 x 32 (four 16 x 16 MFMA chains sharing their loads) or
-// 16 x 16 (one chain; four times the tiles, so small n keeps more of the eight waves busy).  Each
-// element is the same chain of MFMAs in the same k order either way, so both give the same bits.
-template <bool SYM = false, int T = 32, class ST>
-__device__ inline void wg_gemm_pad(int n, int np, const double* S, const double* Bm, ST st) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// One wave's output tile of wg_gemm_pad at (i0, j0): T = 32 fills c[0..3] (c00, c01, c10, c11),
+// T = 16 fills c[0].
+template <int T>
+__device__ inline void pad_tile(int np, int kp, const double* S, const double* Bm, int i0, int j0, int li, int lk,
+                                double4_t (&c)[T == 32 ? 4 : 1]) {
   if constexpr (T == 16) {
-    const int bm = (n + 15) >> 4;
-    const int li = lane & 15, lk = lane >> 4;
-    const int nblk = SYM ? bm * (bm + 1) / 2 : bm * bm;
-    const int kp = (n + 15) & ~15;  // <= np
-    for (int t = wave; t < nblk; t += NW) {
-      int bi, bj;
-      if (SYM) {
-        bi = 0;
-        int rest = t;
-        while (rest >= bm - bi) { rest -= bm - bi; ++bi; }
-        bj = bi + rest;
-      } else {
-        bi = t / bm;
-        bj = t % bm;
-      }
-      const int i0 = bi << 4, j0 = bj << 4;
-      const int ia = min(i0 + li, np - 1), ja = min(j0 + li, np - 1);
-      double4_t c = {0.0, 0.0, 0.0, 0.0};
-      double a[4], b[4];
+    const int ia = min(i0 + li, np - 1), ja = min(j0 + li, np - 1);
+    c[0] = double4_t{0.0, 0.0, 0.0, 0.0};
+    double a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t r = (size_t)(4 * u + lk) * np;
+      a[u] = S[r + ia];
+      b[u] = Bm[r + ja];
+    }
+    for (int k0 = 0; k0 < kp; k0 += 16) {
+      const int kn = k0 + 16 < kp ? k0 + 16 : k0;  // after the last chunk: a harmless reload
+      double na[4], nb[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const size_t r = (size_t)(4 * u + lk) * np;
-        a[u] = S[r + ia];
-        b[u] = Bm[r + ja];
-      }
-      for (int k0 = 0; k0 < kp; k0 += 16) {
-        const int kn = k0 + 16 < kp ? k0 + 16 : k0;  // after the last chunk: a harmless reload
-        double na[4], nb[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const size_t r = (size_t)(kn + 4 * u + lk) * np;
-          na[u] = S[r + ia];
-          nb[u] = Bm[r + ja];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], c, 0, 0, 0);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          a[u] = na[u];
-          b[u] = nb[u];
-        }
+        const size_t r = (size_t)(kn + 4 * u + lk) * np;
+        na[u] = S[r + ia];
+        nb[u] = Bm[r + ja];
       }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int ra = i0 + 4 * r + lk, ca = j0 + li;
-        if (ra < n && ca < n && (!SYM || bi < bj || ra <= ca)) { st(ra, ca, c[r]); if (SYM && ra != ca) st(ca, ra, c[r]); }
+      for (int u = 0; u < 4; ++u) c[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], c[0], 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = na[u];
+        b[u] = nb[u];
       }
     }
-    return;
-  }
-  const int bm = (n + 31) >> 5;
-  const int li = lane & 15, lk = lane >> 4;
-  const int nblk = SYM ? bm * (bm + 1) / 2 : bm * bm;
-  const int kp = (n + 15) & ~15;  // <= np
-  for (int t = wave; t < nblk; t += NW) {
-    int bi, bj;
-    if (SYM) {
-      bi = 0;
-      int rest = t;
-      while (rest >= bm - bi) { rest -= bm - bi; ++bi; }
-      bj = bi + rest;
-    } else {
-      bi = t / bm;
-      bj = t % bm;
-    }
-    const int i0 = bi << 5, j0 = bj << 5;
+  } else {
     const int ia = min(i0 + li, np - 1), ib = min(i0 + li + 16, np - 1);
     const int ja = min(j0 + li, np - 1), jb = min(j0 + li + 16, np - 1);
     double4_t c00 = {0.0, 0.0, 0.0, 0.0}, c01 = c00, c10 = c00, c11 = c00;
@@ -380,13 +336,106 @@ __device__ inline void wg_gemm_pad(int n, int np, const double* S, const double*
         b1[u] = nb1[u];
       }
     }
+    c[0] = c00;
+    c[1] = c01;
+    c[2] = c10;
+    c[3] = c11;
+  }
+}
+
+// Block (bi, bj) of tile t: row-major over the bm x bm grid, or over its upper triangle (SYM).
+template <bool SYM>
+__device__ inline void pad_block(int t, int bm, int& bi, int& bj) {
+  if (SYM) {
+    bi = 0;
+    int rest = t;
+    while (rest >= bm - bi) { rest -= bm - bi; ++bi; }
+    bj = bi + rest;
+  } else {
+    bi = t / bm;
+    bj = t % bm;
+  }
+}
+
+// Stores one tile's results: item r of lane l holds row 4 r + l / 16, column l % 16 of each 16 x 16
+// quarter; SYM stores the upper part of diagonal blocks and mirrors every element.
+template <bool SYM, int T, class ST>
+__device__ inline void pad_store(int n, int bi, int bj, int li, int lk, const double4_t (&c)[T == 32 ? 4 : 1], ST& st) {
+  const int i0 = bi * T, j0 = bj * T;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int ra = i0 + 4 * r + lk, rb = ra + 16, ca = j0 + li, cb = ca + 16;
-      if (ra < n && ca < n && (!SYM || bi < bj || ra <= ca)) { st(ra, ca, c00[r]); if (SYM && ra != ca) st(ca, ra, c00[r]); }
-      if (ra < n && cb < n && (!SYM || bi < bj || ra <= cb)) { st(ra, cb, c01[r]); if (SYM && ra != cb) st(cb, ra, c01[r]); }
-      if (rb < n && ca < n && (!SYM || bi < bj || rb <= ca)) { st(rb, ca, c10[r]); if (SYM && rb != ca) st(ca, rb, c10[r]); }
-      if (rb < n && cb < n && (!SYM || bi < bj || rb <= cb)) { st(rb, cb, c11[r]); if (SYM && rb != cb) st(cb, rb, c11[r]); }
+  for (int r = 0; r < 4; ++r) {
+#pragma unroll
+    for (int q = 0; q < (T == 32 ? 4 : 1); ++q) {
+      const int di = (q >> 1) * 16, dj = (q & 1) * 16;
+      const int ra = i0 + di + 4 * r + lk, ca = j0 + dj + li;
+      if (ra < n && ca < n && (!SYM || bi < bj || ra <= ca)) {
+        st(ra, ca, c[q][r]);
+        if (SYM && ra != ca) st(ca, ra, c[q][r]);
+      }
+    }
+  }
+}
+
+// C = S B for n x n operands held np-strided with zero padding up to np rows and columns, S
+// symmetric (read as S[k][i], so that both operands are read along rows: 16 consecutive doubles
+// per lane group).  No bounds tests on the loads (padding supplies the zeros; rows beyond np are
+// clamped and their results dropped), and the next 16-wide k chunk is loaded while the MFMAs of
+// the current one run.  Same arithmetic order as wg_gemm, so the results are bitwise equal.
+// T is the output tile of one wave: 32 x 32 (four 16 x 16 MFMA chains sharing their loads) or
+// 16 x 16 (one chain; four times the tiles, so small n keeps more of the eight waves busy).  Each
+// element is the same chain of MFMAs in the same k order either way, so both give the same bits.
+template <bool SYM = false, int T = 32, class ST>
+__device__ inline void wg_gemm_pad(int n, int np, const double* S, const double* Bm, ST st) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int bm = (n + T - 1) / T;
+  const int nblk = SYM ? bm * (bm + 1) / 2 : bm * bm;
+  const int kp = (n + 15) & ~15;  // <= np
+  for (int t = wave; t < nblk; t += NW) {
+    int bi, bj;
+    pad_block<SYM>(t, bm, bi, bj);
+    double4_t c[T == 32 ? 4 : 1];
+    pad_tile<T>(np, kp, S, Bm, bi * T, bj * T, li, lk, c);
+    pad_store<SYM, T>(n, bi, bj, li, lk, c, st);
+  }
+}
+
+// Tiles a wave holds at most in wg_gemm_sym_hold, and whether a size fits that budget.
+constexpr int HOLD_T16 = 5, HOLD_T32 = 2;
+__host__ __device__ constexpr bool hold_fits(int n, int T) {
+  return ((n + T - 1) / T) * ((n + T - 1) / T + 1) / 2 <= (T == 16 ? HOLD_T16 : HOLD_T32) * NW;
+}
+
+// The symmetric product X X of wg_gemm_pad<true, T>, with every wave's tiles held in registers
+// until all waves have finished reading X, so that the store may overwrite X itself (one LDS image
+// instead of two).  Each element and its mirror belong to one wave, so the stores do not race.
+// Requires hold_fits(n, T).  Same MFMA chains, so the same bits as wg_gemm_pad.
+template <int T, class ST>
+__device__ inline void wg_gemm_sym_hold(int n, int np, const double* X, ST st) {
+  constexpr int H = T == 16 ? HOLD_T16 : HOLD_T32;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int bm = (n + T - 1) / T;
+  const int nblk = bm * (bm + 1) / 2;
+  const int kp = (n + 15) & ~15;
+  double4_t c[H][T == 32 ? 4 : 1];
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    const int t = wave + h * NW;
+    if (t < nblk) {
+      int bi, bj;
+      pad_block<true>(t, bm, bi, bj);
+      pad_tile<T>(np, kp, X, X, bi * T, bj * T, li, lk, c[h]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    const int t = wave + h * NW;
+    if (t < nblk) {
+      int bi, bj;
+      pad_block<true>(t, bm, bi, bj);
+      pad_store<true, T>(n, bi, bj, li, lk, c[h], st);
     }
   }
 }
@@ -1421,8 +1470,9 @@ __device__ void frontier_seed(const double* X, int ld, int n, const Bufs& B) {
 }
 
 // The purification loop for one placement of the operands: LM = 2 both X and the product in LDS
-// (ping-pong of two images), LM = 1 X in LDS and the product to HBM (B.T, copied back into the
-// image after each step), LM = 0 both in HBM (B.Cn / B.T).
+// (ping-pong of two images), LM = 3 one LDS image that each product overwrites in place (its tiles
+// held in registers until every wave has read X, wg_gemm_sym_hold), LM = 1 X in LDS and the
+// product to HBM (B.T, copied back into the image after each step), LM = 0 both in HBM (B.Cn / B.T).
 template <int LM>
 __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo, double hi,
                        bool seed) {
@@ -1430,8 +1480,8 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
   const int img = sp2_img(n), ldl = sp2_ld(n);
   const double s = 1.0 / (hi - lo);
   double* X = LM ? g_lds : B.Cn;
-  double* Y = LM == 2 ? g_lds + img : B.T;
-  const int ldx = LM ? ldl : np, ldy = LM == 2 ? ldl : np;
+  double* Y = LM == 2 ? g_lds + img : LM == 3 ? g_lds : B.T;
+  const int ldx = LM ? ldl : np, ldy = LM >= 2 ? ldl : np;
   double v[2] = {0.0, 0.0};
   if (LM) {
     for (int e = tid; e < (LM == 2 ? 2 * img : img); e += NT) {
@@ -1475,8 +1525,13 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
       w[1] += y * y;
       if (i == j) w[0] += y;
     };
-    if (n <= M.sp2_tile16_n) wg_gemm_pad<true, 16>(n, ldx, Xc, Xc, store);
-    else wg_gemm_pad<true, 32>(n, ldx, Xc, Xc, store);
+    if constexpr (LM == 3) {
+      if (n <= M.sp2_tile16_n) wg_gemm_sym_hold<16>(n, ldx, Xc, store);
+      else wg_gemm_sym_hold<32>(n, ldx, Xc, store);
+    } else {
+      if (n <= M.sp2_tile16_n) wg_gemm_pad<true, 16>(n, ldx, Xc, Xc, store);
+      else wg_gemm_pad<true, 32>(n, ldx, Xc, Xc, store);
+    }
     block_sum<2>(w, red);
     tr = w[0];
     t2 = w[1];
@@ -1486,7 +1541,7 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
         X[i * ldl + j] = Y[i * np + j];
       }
       __syncthreads();
-    } else {
+    } else if (LM != 3) {
       double* t = X;
       X = Y;
       Y = t;
@@ -1513,6 +1568,8 @@ __device__ inline int sp2_dispatch(const Mol& M, const Bufs& B, const double* Fu
                                    double hi, bool seed) {
   const int img = sp2_img(M.n);
   if (M.sp2_lds >= 1 && 2 * img <= M.lds_doubles) return sp2_run<2>(M, B, Fu, Pdst, red, lo, hi, seed);
+  if (M.sp2_lds >= 3 && img <= M.lds_doubles && hold_fits(M.n, M.n <= M.sp2_tile16_n ? 16 : 32))
+    return sp2_run<3>(M, B, Fu, Pdst, red, lo, hi, seed);
   if (M.sp2_lds >= 2 && img <= M.lds_doubles) return sp2_run<1>(M, B, Fu, Pdst, red, lo, hi, seed);
   return sp2_run<0>(M, B, Fu, Pdst, red, lo, hi, seed);
 }

@@ -1,5 +1,6 @@
 """Tuning entries change speed only: the 16 x 16 wave tiles of the purification products give the
-same products bit for bit as the 32 x 32 tiles, whole batches agree with the default table under
+same products bit for bit as the 32 x 32 tiles, the in-place LDS placement the same densities bit
+for bit as the two-image and HBM-product placements, whole batches agree with the default table under
 every entry's extreme values, and the autotuner returns a table that keeps every answer while
 leaving it as the process-wide table."""
 import numpy as np
@@ -52,6 +53,12 @@ def test_tile16_purification_products_are_bitwise_equal(G, oracle):
         P32, k32, h32, l32 = run(m, F)
         G.tuning_set(dict(d, sp2_tile16_n=1 << 20))
         P16, k16, h16, l16 = run(m, F)
+        # the in-place placement (one LDS image) against the two-image / HBM-product ones
+        G.tuning_set(dict(d, sp2_lds=3))
+        Pin, kin, hin, lin = run(m, F)
+        G.tuning_set(dict(d, sp2_lds=2))
+        Pim, kim, him, lim = run(m, F)
+        assert kin == kim and np.array_equal(Pin, Pim) and hin == him and lin == lim, F.shape[0]
         print("nao", F.shape[0], "products", k32, k16, "max|dP|", np.abs(P16 - P32).max(), flush=True)
         assert k32 > 0 and k16 > 0
         assert np.abs(P16 - P32).max() <= 1e-13
@@ -68,6 +75,7 @@ def test_every_entry_changes_speed_only(G):
     assert np.all(s0 == 0)
     tables = [dict(d, sp2_tile16_n=1 << 20), dict(d, sp2_tile16_n=0), dict(d, pair_block=64, pair_blocks_per_cu=1),
               dict(d, pair_block=128, pair_blocks_per_cu=64), dict(d, sp2_lds=0), dict(d, sp2_lds=1),
+              dict(d, sp2_lds=3), dict(d, sp2_lds=3, sp2_tile16_n=0),
               dict(d, sp2_tight_n=1 << 20), dict(d, sp2_tight_n=8)]
     for t in tables:
         s, r, ms = _solve(G, mols, t)
