@@ -171,7 +171,8 @@ typedef struct {
   int32_t pair_blocks_per_cu; /* their grid (grid-stride loops), blocks per CU: 1..64 */
   int32_t sp2_tile16_n;       /* purification products on 16x16 wave tiles up to this many AOs, 32x32 above */
   int32_t sp2_lds;            /* purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS,
-                                 3 + X in LDS overwritten in place by each product */
+                                 3 + X in LDS overwritten in place by each product,
+                                 4 + the leading rows of X in LDS where the whole of X does not fit */
   int32_t sp2_tight_n;        /* purification spectral bounds from Lanczos from this many AOs on */
   int32_t large_syrk;         /* multi-kernel path: purification products by dsyrk (1) or dgemm (0) */
 } pm3_tuning;

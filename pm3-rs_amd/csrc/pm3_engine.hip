@@ -109,7 +109,7 @@ struct KArgs {
   double eig_skip;
   int purify;                  // 1: densities by purification, frontier levels by Lanczos (no orbitals)
   int sp2_lds;                 // purification operand placements allowed: 0 HBM only, 1 + LDS pair, 2 + LDS X,
-                               // 3 + LDS X overwritten in place
+                               // 3 + LDS X overwritten in place, 4 + leading rows of X in LDS
   int sp2_tight_n;             // smallest molecule (AOs) whose purification uses Lanczos spectral bounds
   int sp2_tile16_n;            // purification products on 16 x 16 wave tiles up to this size (AOs), 32 x 32 above
   double sp2_margin;           // margin of those bounds relative to the estimated width (fresh estimate)
@@ -437,6 +437,83 @@ __device__ inline void wg_gemm_sym_hold(int n, int np, const double* X, ST st) {
       pad_block<true>(t, bm, bi, bj);
       pad_store<true, T>(n, bi, bj, li, lk, c[h], st);
     }
+  }
+}
+
+// pad_tile with the k rows [0, kL) of the operand read from an LDS copy (Sl, leading dimension
+// ldl) and the rows from kL on from HBM (Sh, ldh); kL is a multiple of 16, so each 16-row chunk
+// has one source.  The same MFMA chains, so the same bits as pad_tile on either copy.
+template <int T>
+__device__ inline void pad_tile_split(int kL, const double* Sl, int ldl, const double* Sh, int ldh, int kp, int i0,
+                                      int j0, int li, int lk, double4_t (&c)[T == 32 ? 4 : 1]) {
+  constexpr int NC = T == 32 ? 2 : 1;  // 16-wide column groups per operand
+  int il[NC], ih[NC], jl[NC], jh[NC];
+#pragma unroll
+  for (int g = 0; g < NC; ++g) {
+    il[g] = min(i0 + li + 16 * g, ldl - 1);
+    ih[g] = min(i0 + li + 16 * g, ldh - 1);
+    jl[g] = min(j0 + li + 16 * g, ldl - 1);
+    jh[g] = min(j0 + li + 16 * g, ldh - 1);
+  }
+  double4_t acc[NC][NC];
+#pragma unroll
+  for (int x = 0; x < NC; ++x)
+#pragma unroll
+    for (int y = 0; y < NC; ++y) acc[x][y] = double4_t{0.0, 0.0, 0.0, 0.0};
+  double a[NC][4], b[NC][4];
+  auto load = [&](int k0, double (&ra)[NC][4], double (&rb)[NC][4]) {
+    const bool in_lds = k0 < kL;
+    const double* S = in_lds ? Sl : Sh;
+    const int ld = in_lds ? ldl : ldh;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const size_t r = (size_t)(k0 + 4 * u + lk) * ld;
+#pragma unroll
+      for (int g = 0; g < NC; ++g) {
+        ra[g][u] = S[r + (in_lds ? il[g] : ih[g])];
+        rb[g][u] = S[r + (in_lds ? jl[g] : jh[g])];
+      }
+    }
+  };
+  load(0, a, b);
+  for (int k0 = 0; k0 < kp; k0 += 16) {
+    const int kn = k0 + 16 < kp ? k0 + 16 : k0;  // after the last chunk: a harmless reload
+    double na[NC][4], nb[NC][4];
+    load(kn, na, nb);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int x = 0; x < NC; ++x)
+#pragma unroll
+        for (int y = 0; y < NC; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x][u], b[y][u], acc[x][y], 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int g = 0; g < NC; ++g) {
+        a[g][u] = na[g][u];
+        b[g][u] = nb[g][u];
+      }
+  }
+#pragma unroll
+  for (int x = 0; x < NC; ++x)
+#pragma unroll
+    for (int y = 0; y < NC; ++y) c[x * NC + y] = acc[x][y];
+}
+
+// The symmetric product X X of wg_gemm_pad<true, T> with X's first kL rows read from LDS.
+template <int T, class ST>
+__device__ inline void wg_gemm_sym_split(int n, int kL, const double* Xl, int ldl, const double* Xh, int ldh, ST st) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int bm = (n + T - 1) / T;
+  const int nblk = bm * (bm + 1) / 2;
+  const int kp = (n + 15) & ~15;
+  for (int t = wave; t < nblk; t += NW) {
+    int bi, bj;
+    pad_block<true>(t, bm, bi, bj);
+    double4_t c[T == 32 ? 4 : 1];
+    pad_tile_split<T>(kL, Xl, ldl, Xh, ldh, kp, bi * T, bj * T, li, lk, c);
+    pad_store<true, T>(n, bi, bj, li, lk, c, st);
   }
 }
 
@@ -1441,6 +1518,10 @@ constexpr double SP2_TAIL = 1.0e-7;
 __host__ __device__ constexpr int sp2_kp(int n) { return (n + 15) & ~15; }
 __host__ __device__ constexpr int sp2_ld(int n) { return (sp2_kp(n) & 31) == 0 ? sp2_kp(n) + 16 : sp2_kp(n); }
 __host__ __device__ constexpr int sp2_img(int n) { return sp2_kp(n) * sp2_ld(n); }
+// Rows of X that the LM = 4 placement keeps in LDS: as many 16-row chunks as lds doubles allow.
+__host__ __device__ constexpr int sp2_rows(int n, int lds) {
+  return ((lds / sp2_ld(n)) & ~15) < sp2_kp(n) ? ((lds / sp2_ld(n)) & ~15) : sp2_kp(n);
+}
 
 // Fixed generic vector of the frontier seed (cheap to evaluate: one per row and thread).
 __device__ inline double seed_r(int j) {
@@ -1472,18 +1553,30 @@ __device__ void frontier_seed(const double* X, int ld, int n, const Bufs& B) {
 // The purification loop for one placement of the operands: LM = 2 both X and the product in LDS
 // (ping-pong of two images), LM = 3 one LDS image that each product overwrites in place (its tiles
 // held in registers until every wave has read X, wg_gemm_sym_hold), LM = 1 X in LDS and the
-// product to HBM (B.T, copied back into the image after each step), LM = 0 both in HBM (B.Cn / B.T).
+// product to HBM (B.T, copied back into the image after each step), LM = 0 both in HBM (B.Cn / B.T),
+// LM = 4 both in HBM with X's first sp2_rows(n) rows also kept in LDS (refreshed after each
+// product), for sizes whose image does not fit.
 template <int LM>
 __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo, double hi,
                        bool seed) {
   const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
   const int img = sp2_img(n), ldl = sp2_ld(n);
   const double s = 1.0 / (hi - lo);
-  double* X = LM ? g_lds : B.Cn;
+  constexpr bool HB = LM == 0 || LM == 4;  // X and its product in HBM
+  double* X = HB ? B.Cn : g_lds;
   double* Y = LM == 2 ? g_lds + img : LM == 3 ? g_lds : B.T;
-  const int ldx = LM ? ldl : np, ldy = LM >= 2 ? ldl : np;
+  const int ldx = HB ? np : ldl, ldy = (LM == 2 || LM == 3) ? ldl : np;
+  const int kL = LM == 4 ? sp2_rows(n, M.lds_doubles) : 0;
+  // LM = 4: rows [0, kL) of the HBM iterate copied into the LDS image (zeros beyond n)
+  auto refresh = [&](const double* Xh) {
+    for (int e = tid; e < kL * ldl; e += NT) {
+      const int r = e / ldl, c = e % ldl;
+      g_lds[e] = (r < n && c < n) ? Xh[r * np + c] : 0.0;
+    }
+    __syncthreads();
+  };
   double v[2] = {0.0, 0.0};
-  if (LM) {
+  if (!HB) {
     for (int e = tid; e < (LM == 2 ? 2 * img : img); e += NT) {
       const int r = (e % img) / ldl, c = e % ldl;
       double x = 0.0;
@@ -1504,6 +1597,7 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
     }
   }
   block_sum<2>(v, red);
+  if (LM == 4) refresh(X);
   double tr = v[0], t2 = v[1];
   const double occ = (double)nocc;
   int tail = -1, k = 0;
@@ -1528,6 +1622,9 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
     if constexpr (LM == 3) {
       if (n <= M.sp2_tile16_n) wg_gemm_sym_hold<16>(n, ldx, Xc, store);
       else wg_gemm_sym_hold<32>(n, ldx, Xc, store);
+    } else if constexpr (LM == 4) {
+      if (n <= M.sp2_tile16_n) wg_gemm_sym_split<16>(n, kL, g_lds, ldl, Xc, np, store);
+      else wg_gemm_sym_split<32>(n, kL, g_lds, ldl, Xc, np, store);
     } else {
       if (n <= M.sp2_tile16_n) wg_gemm_pad<true, 16>(n, ldx, Xc, Xc, store);
       else wg_gemm_pad<true, 32>(n, ldx, Xc, Xc, store);
@@ -1545,6 +1642,7 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
       double* t = X;
       X = Y;
       Y = t;
+      if (LM == 4) refresh(X);
     }
     if (tail > 0) --tail;
   }
@@ -1571,6 +1669,7 @@ __device__ inline int sp2_dispatch(const Mol& M, const Bufs& B, const double* Fu
   if (M.sp2_lds >= 3 && img <= M.lds_doubles && hold_fits(M.n, M.n <= M.sp2_tile16_n ? 16 : 32))
     return sp2_run<3>(M, B, Fu, Pdst, red, lo, hi, seed);
   if (M.sp2_lds >= 2 && img <= M.lds_doubles) return sp2_run<1>(M, B, Fu, Pdst, red, lo, hi, seed);
+  if (M.sp2_lds >= 4 && sp2_rows(M.n, M.lds_doubles) >= 16) return sp2_run<4>(M, B, Fu, Pdst, red, lo, hi, seed);
   return sp2_run<0>(M, B, Fu, Pdst, red, lo, hi, seed);
 }
 

@@ -87,7 +87,7 @@ def test_tuning_table_round_trip_and_validation(tmp_path):
         G.tuning_set(d)
         G.tuning_load(path)
         assert G.tuning_get() == dict(d, pair_block=128, sp2_tile16_n=96)
-        for bad in ({"pair_block": 96}, {"pair_block": 512}, {"sp2_lds": 4}, {"pair_blocks_per_cu": 0}):
+        for bad in ({"pair_block": 96}, {"pair_block": 512}, {"sp2_lds": 5}, {"pair_blocks_per_cu": 0}):
             with pytest.raises(G.InvalidInput):
                 G.tuning_set(bad)
         assert G.tuning_get() == dict(d, pair_block=128, sp2_tile16_n=96)  # a rejected table changes nothing
