@@ -113,7 +113,7 @@ def bench_large(args):
         "data": "synthetic (seeded geometry)",
         "config": {"workload": "water512", "molecules": 1, "atoms": len(z), "nao": n, "seed": args.seed,
                    "p_tol": 1e-8, "guess": "sad", "gradient": True,
-                   "parallelism": "grid-wide kernels per SCF step; purified densities on rocBLAS dgemm; frontier levels by rocSOLVER dsyevd (eigenvalues only)"},
+                   "parallelism": "grid-wide kernels per SCF step; purified densities on rocBLAS dgemm; frontier levels by seeded Lanczos (rocBLAS gemv)"},
         "iterations": r.iterations, "converged": int(st[0] == 0), "heat_of_formation_kcal": r.heat_of_formation_kcal,
         "max_abs_gradient_ev_per_bohr": float(np.abs(g).max()),
         "checks": {"idempotency_max_abs": idem, "trace_minus_electrons": float(np.trace(P) - 2 * r.n_occ),

@@ -155,10 +155,12 @@ pm3_status pm3_batch_guess_timing(const pm3_batch* batch, double* features_ms, d
  * purification fallbacks. */
 pm3_status pm3_batch_phase_ticks(const pm3_batch* batch, uint64_t* out16);
 /* Molecules on the multi-kernel path, last solve: device time (ms, HIP events around each rocBLAS
- * call), count and FLOP (2 n^3 per dgemm, n^3 per dsyrk) of the purification products, and
- * device time and count of the Fock builds.  Zeros when the batch has no such molecule. */
+ * call), count and FLOP (2 n^3 per dgemm, n^3 per dsyrk) of the purification products, device
+ * time and count of the Fock builds, and the Lanczos steps and runs of the frontier levels.
+ * Zeros when the batch has no such molecule; any output pointer may be null. */
 pm3_status pm3_batch_large_timing(const pm3_batch* batch, double* product_ms, int64_t* products,
-                                  int64_t* product_flop, double* fock_ms, int64_t* fock_builds);
+                                  int64_t* product_flop, double* fock_ms, int64_t* fock_builds,
+                                  int64_t* lanczos_steps, int64_t* lanczos_runs);
 
 /* Launch shapes and tile choices of the kernels, and an autotuner for them.  No reference
  * counterpart (the reference has no device kernels).  The entries change speed only: results

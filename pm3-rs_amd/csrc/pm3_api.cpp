@@ -263,12 +263,14 @@ pm3_status pm3_batch_results(pm3_batch* h, pm3_status* status, pm3_result* resul
 double pm3_batch_last_kernel_ms(const pm3_batch* h) { return h ? batch_kernel_ms(h->b) : 0.0; }
 
 pm3_status pm3_batch_large_timing(const pm3_batch* h, double* product_ms, int64_t* products, int64_t* product_flop,
-                                  double* fock_ms, int64_t* fock_builds) {
+                                  double* fock_ms, int64_t* fock_builds, int64_t* lanczos_steps, int64_t* lanczos_runs) {
   g_last_error.clear();
   if (!h) return fail(PM3_INVALID_INPUT, "null batch");
   double pms = 0.0, fms = 0.0;
-  long long np = 0, nf = 0, fl = 0;
-  batch_large_timing(h->b, pms, np, fl, fms, nf);
+  long long np = 0, nf = 0, fl = 0, ls = 0, lr = 0;
+  batch_large_timing(h->b, pms, np, fl, fms, nf, ls, lr);
+  if (lanczos_steps) *lanczos_steps = ls;
+  if (lanczos_runs) *lanczos_runs = lr;
   if (product_ms) *product_ms = pms;
   if (products) *products = np;
   if (product_flop) *product_flop = fl;

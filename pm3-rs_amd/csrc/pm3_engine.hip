@@ -1524,7 +1524,7 @@ __host__ __device__ constexpr int sp2_rows(int n, int lds) {
 }
 
 // Fixed generic vector of the frontier seed (cheap to evaluate: one per row and thread).
-__device__ inline double seed_r(int j) {
+__host__ __device__ inline double seed_r(int j) {
   const double x = 0.6180339887498949 * (double)j + 0.1;
   return 1.0 + 0.5 * (x - floor(x));
 }
@@ -1775,7 +1775,7 @@ __device__ inline double tridiag_max(const double* a, const double* b, int m, do
 // Residual b[m-1] |s[m-1]| of the top Ritz pair (theta, s), |s| = 1, of the Lanczos tridiagonal:
 // s by the forward three-term recurrence from s_0 = 1, which for the largest eigenvalue (above every
 // eigenvalue of the leading submatrices) follows the dominant, sign-stable solution.
-__device__ inline double ritz_residual(const double* a, const double* b, int m, double theta) {
+__host__ __device__ inline double ritz_residual(const double* a, const double* b, int m, double theta) {
   double s0 = 0.0, s1 = 1.0, ss = 1.0;
   for (int i = 0; i + 1 < m; ++i) {
     const double s2 = ((theta - a[i]) * s1 - (i > 0 ? b[i - 1] * s0 : 0.0)) / b[i];
@@ -3111,6 +3111,20 @@ __global__ __launch_bounds__(LG_THREADS) void lg_sp2_init_kernel(const double* A
   lg_trace_partials(tr, fr, red);
 }
 
+// The two frontier operators from F, P (X = P / 2), G = X F and H = X F X (row-major):
+// A_o = (H + H^T) / 2 + c_lo (I - X) and A_v = -(F - G - G^T + (H + H^T) / 2 + c_hi X).
+__global__ __launch_bounds__(LG_THREADS) void lg_frontier_ops_kernel(const double* F, const double* P, const double* G,
+                                                                     const double* H, int n, int np, double c_lo,
+                                                                     double c_hi, double* Ao, double* Av) {
+  for (long long e = gtid(); e < (long long)n * n; e += gstride()) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const size_t ij = (size_t)i * np + j, ji = (size_t)j * np + i;
+    const double x = 0.5 * P[ij], h = 0.5 * (H[ij] + H[ji]), id = i == j ? 1.0 : 0.0;
+    Ao[ij] = h + c_lo * (id - x);
+    Av[ij] = -(F[ij] - G[ij] - G[ji] + h + c_hi * x);
+  }
+}
+
 // X <- Y or 2X - Y, with Y = X X (only its row-major lower triangle is read, so X stays exactly
 // symmetric), then the trace and norm partials of the new X.
 __global__ __launch_bounds__(LG_THREADS) void lg_sp2_update_kernel(const double* Y, double* X, int n, int np, int sq,
@@ -3297,6 +3311,10 @@ struct LargeBufs {
   double *ework = nullptr, *Pb = nullptr, *Vb = nullptr, *wb = nullptr, *Fh = nullptr, *Eh = nullptr, *Dh = nullptr;
   double *J = nullptr, *slot = nullptr, *red = nullptr, *dip = nullptr, *fin = nullptr;
   double* lz = nullptr;                          // Lanczos vectors (3 np)
+  double* fs = nullptr;                          // frontier seed s and X s (np each), then ...
+  double* fr = nullptr;                          // ... the generic start vector r (np_max)
+  bool seeded = false;                           // a purification of this path has written the seed
+  bool last_eig = false;                         // the path's last density came from the eigensolver
   double bnd_lo = 0.0, bnd_hi = 0.0;            // cached spectral-bound estimate of the purification
   int bnd_age = -1;
   bool no_tight = false;                         // Gershgorin bounds only (a path being rerun)
@@ -3307,6 +3325,7 @@ struct LargeBufs {
   hipEvent_t evp0 = nullptr, evp1 = nullptr, evf0 = nullptr, evf1 = nullptr;
   double product_ms = 0.0, fock_ms = 0.0;
   long long products = 0, fock_builds = 0, product_flop = 0;
+  long long lanczos_steps = 0, lanczos_runs = 0;  // frontier-level Lanczos of the last solve
 };
 
 struct Batch {
@@ -3431,8 +3450,14 @@ static bool large_setup(Batch& b, std::string& err) {
       !b.alloc(L.Fh, L.m2 * depth, err) || !b.alloc(L.Eh, L.m2 * depth, err) || !b.alloc(L.Dh, L.m2 * depth, err) ||
       !b.alloc(L.J, 10LL * nat, err) || !b.alloc(L.slot, slot, err) ||
       !b.alloc(L.red, (size_t)LG_BLOCKS * LG_MAXDOT, err) || !b.alloc(L.dip, 3, err) || !b.alloc(L.info, 1, err) ||
-      !b.alloc(L.fin, LG_MAXDOT, err) || !b.alloc(L.lz, 3LL * np, err))
+      !b.alloc(L.fin, LG_MAXDOT, err) || !b.alloc(L.lz, 3LL * np, err) || !b.alloc(L.fs, 3LL * np, err))
     return false;
+  {
+    std::vector<double> r(np);
+    for (int j = 0; j < np; ++j) r[j] = seed_r(j);
+    L.fr = L.fs + 2 * np;
+    HIP_OK(hipMemcpy(L.fr, r.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+  }
   b.h_red.resize((size_t)LG_BLOCKS * LG_MAXDOT);
   BLAS_OK(rocblas_create_handle(&b.blas));
   BLAS_OK(rocblas_set_stream(b.blas, 0));
@@ -3587,9 +3612,10 @@ static bool large_extremes(Batch& b, const MolDesc& d, const double* A, double& 
 // cached Lanczos estimate as in sp2_density (refreshed at the first two steps of a path, every
 // SP2_REFRESH steps and after a miss).  False in `ok` (nothing written) when the frontier gap is
 // closed or a check fails with Gershgorin bounds: the caller then diagonalises.
-static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double* Pdst, double lo, double hi, bool& ok,
-                             std::string& err);
-static bool large_purify(Batch& b, const MolDesc& d, const double* A, double* Pdst, bool& ok, std::string& err) {
+static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double* Pdst, double lo, double hi, bool seed,
+                             bool& ok, std::string& err);
+static bool large_purify(Batch& b, const MolDesc& d, const double* A, double* Pdst, bool seed, bool& ok,
+                         std::string& err) {
   LargeBufs& L = b.lg;
   const int n = d.nao, np = d.np, nocc = d.nocc;
   ok = false;
@@ -3616,16 +3642,33 @@ static bool large_purify(Batch& b, const MolDesc& d, const double* A, double* Pd
     const double m = mf * (L.bnd_hi - L.bnd_lo);
     const double lo = std::max(glo, L.bnd_lo - m), hi = std::min(ghi, L.bnd_hi + m);
     if (hi - lo < ghi - glo && hi - lo > 1.0e-12 * std::max(1.0, std::fabs(hi))) {
-      if (!large_purify_run(b, d, A, Pdst, lo, hi, ok, err)) return false;
+      if (!large_purify_run(b, d, A, Pdst, lo, hi, seed, ok, err)) return false;
       if (ok) return true;
       L.bnd_age = 0;  // a miss: estimate afresh next time
     }
   }
-  return large_purify_run(b, d, A, Pdst, glo, ghi, ok, err);
+  return large_purify_run(b, d, A, Pdst, glo, ghi, seed, ok, err);
 }
 
-static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double* Pdst, double lo, double hi, bool& ok,
-                             std::string& err) {
+// Frontier seed s = (X - X^2) r of the iterate X in L.V (as frontier_seed in the batched kernel).
+static bool large_seed(Batch& b, const MolDesc& d, std::string& err) {
+  LargeBufs& L = b.lg;
+  const int n = d.nao, np = d.np;
+  double* sv = L.fs;
+  double* t = L.fs + np;
+  const double* r = L.fr;
+  double* u = L.lz;
+  const double one = 1.0, zero = 0.0, mone = -1.0;
+  BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_transpose, n, n, &one, L.V, np, r, 1, &zero, t, 1));
+  BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_transpose, n, n, &one, L.V, np, t, 1, &zero, u, 1));
+  BLAS_OK(rocblas_dcopy(b.blas, n, t, 1, sv, 1));
+  BLAS_OK(rocblas_daxpy(b.blas, n, &mone, u, 1, sv, 1));
+  L.seeded = true;
+  return true;
+}
+
+static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double* Pdst, double lo, double hi, bool seed,
+                             bool& ok, std::string& err) {
   LargeBufs& L = b.lg;
   const int n = d.nao, np = d.np, nocc = d.nocc;
   ok = false;
@@ -3645,7 +3688,10 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
   int tail = -1, k = 0;
   for (; k < SP2_MAX; ++k) {
     if (t2 > tr + 1.0e-9 * std::max(1.0, tr)) return true;
-    if (tail < 0 && tr - t2 < SP2_TAIL) tail = SP2_EXTRA;
+    if (tail < 0 && tr - t2 < SP2_TAIL) {
+      tail = SP2_EXTRA;
+      if (seed && !large_seed(b, d, err)) return false;
+    }
     if (tail == 0) break;
     const int sq = std::fabs(t2 - occ) < std::fabs(2.0 * tr - t2 - occ) ? 1 : 0;
     HIP_OK(hipEventRecord(L.evp0, 0));
@@ -3693,13 +3739,143 @@ static bool large_levels(Batch& b, const MolDesc& d, const double* A, std::strin
 
 // The aufbau density of A into Pdst: purified when the batch returns no orbitals, else (or when
 // purification declines) from the eigenvectors.
-static bool large_density(Batch& b, const MolDesc& d, const double* A, double* Pdst, std::string& err) {
+static bool large_density(Batch& b, const MolDesc& d, const double* A, double* Pdst, std::string& err,
+                          bool seed = false) {
+  b.lg.last_eig = false;
   if (b.purify) {
     bool ok = false;
-    if (!large_purify(b, d, A, Pdst, ok, err)) return false;
+    if (!large_purify(b, d, A, Pdst, seed, ok, err)) return false;
     if (ok) return true;
   }
+  b.lg.last_eig = true;
   return large_eigen_density(b, d, A, Pdst, err);
+}
+
+// Largest eigenvalue of the symmetric np-strided A by plain Lanczos from the normalised device
+// vector st (rocBLAS level-1/2 calls on L.lz), with frontier_levels' stopping tests: a Ritz
+// residual below LZ_TOL, a value stalled over two checks, breakdown or the step cap; checks from
+// step 8 every fourth for a seeded start, else from 16 every eighth.
+static bool large_lanczos_top(Batch& b, const MolDesc& d, const double* A, const double* st, bool seeded, double scale,
+                              double& top, std::string& err) {
+  LargeBufs& L = b.lg;
+  const int n = d.nao, np = d.np;
+  double* q = L.lz;
+  double* qp = L.lz + np;
+  double* w = L.lz + 2 * np;
+  HIP_OK(hipMemcpyAsync(q, st, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+  HIP_OK(hipMemsetAsync(qp, 0, sizeof(double) * n, 0));
+  std::vector<double> al, be;
+  double beta = 0.0, th1 = -1.0e300, th2 = -1.0e300;
+  const double one = 1.0, zero = 0.0;
+  const int cap = 2 * n + 16, first = seeded ? 8 : 16, every = seeded ? 4 : 8;
+  top = 0.0;
+  for (int k = 0; k < cap; ++k) {
+    BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_transpose, n, n, &one, A, np, q, 1, &zero, w, 1));
+    const double mb = -beta;
+    BLAS_OK(rocblas_daxpy(b.blas, n, &mb, qp, 1, w, 1));
+    double alpha = 0.0;
+    BLAS_OK(rocblas_ddot(b.blas, n, q, 1, w, 1, &alpha));
+    const double ma = -alpha;
+    BLAS_OK(rocblas_daxpy(b.blas, n, &ma, q, 1, w, 1));
+    BLAS_OK(rocblas_dnrm2(b.blas, n, w, 1, &beta));
+    al.push_back(alpha);
+    be.push_back(beta);
+    const int mk = k + 1;
+    const bool breakdown = beta <= 1.0e-13 * scale;
+    if (mk == cap || breakdown || (mk >= first && mk % every == 0)) {
+      const double th = host_tridiag_max(al, be, mk);
+      const bool stall = th - th1 <= 1.0e-13 * scale && th - th2 <= 1.0e-12 * scale;
+      top = th;
+      if (mk == cap || breakdown || stall || ritz_residual(al.data(), be.data(), mk, th) <= LZ_TOL) {
+        L.lanczos_steps += mk;
+        ++L.lanczos_runs;
+        return true;
+      }
+      th2 = th1;
+      th1 = th;
+    }
+    BLAS_OK(rocblas_dcopy(b.blas, n, q, 1, qp, 1));
+    const double ib = 1.0 / beta;
+    BLAS_OK(rocblas_dscal(b.blas, n, &ib, w, 1));
+    BLAS_OK(rocblas_dcopy(b.blas, n, w, 1, q, 1));
+  }
+  return true;
+}
+
+// v <- v / |v| + 1e-3 r / |r| with the generic vector r of L.fr (so that no direction is missing),
+// or r / |r| when v is not to be used, is zero or is not finite (lanczos_start).
+static bool large_start(Batch& b, const MolDesc& d, double* v, bool use, std::string& err) {
+  LargeBufs& L = b.lg;
+  const int n = d.nao;
+  const double* r = L.fr;
+  double rn = 0.0, vn = 0.0;
+  BLAS_OK(rocblas_dnrm2(b.blas, n, r, 1, &rn));
+  if (use) BLAS_OK(rocblas_dnrm2(b.blas, n, v, 1, &vn));
+  if (use && vn > 1.0e-140 && vn < 1.0e140) {
+    const double iv = 1.0 / vn, cr = 1.0e-3 / rn;
+    BLAS_OK(rocblas_dscal(b.blas, n, &iv, v, 1));
+    BLAS_OK(rocblas_daxpy(b.blas, n, &cr, r, 1, v, 1));
+  } else {
+    const double ir = 1.0 / rn;
+    BLAS_OK(rocblas_dcopy(b.blas, n, r, 1, v, 1));
+    BLAS_OK(rocblas_dscal(b.blas, n, &ir, v, 1));
+  }
+  double nn = 0.0;
+  BLAS_OK(rocblas_dnrm2(b.blas, n, v, 1, &nn));
+  const double in = 1.0 / nn;
+  BLAS_OK(rocblas_dscal(b.blas, n, &in, v, 1));
+  return true;
+}
+
+// Frontier levels of the last step without eigenvalues: the HOMO as the largest eigenvalue of
+// A_o = X F X + c (I - X) and the LUMO as minus the largest of A_v = -((I - X) F (I - X) + c' X),
+// X = P / 2, c and c' beyond the Gershgorin interval of F (frontier_levels); A_o and A_v from two
+// dgemm products and a grid-wide kernel, then one Lanczos run each, seeded from the path's last
+// purification when it left a seed.  HOMO and LUMO go to L.w[nocc - 1], L.w[nocc]; `aufbau` is
+// false when HOMO > LUMO (an eigenvalue on the wrong side of the projector).
+static bool large_levels_lanczos(Batch& b, const MolDesc& d, const double* Fu, bool& aufbau, std::string& err) {
+  LargeBufs& L = b.lg;
+  const int n = d.nao, np = d.np, nocc = d.nocc;
+  aufbau = true;
+  hipLaunchKernelGGL(lg_gershgorin_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, Fu, n, np, L.red);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(b.h_red.data(), L.red, sizeof(double) * (size_t)LG_BLOCKS * LG_MAXDOT, hipMemcpyDeviceToHost));
+  double glo = 1.0e300, ghi = -1.0e300;
+  for (int blk = 0; blk < LG_BLOCKS; ++blk) {
+    glo = std::min(glo, b.h_red[(size_t)blk * LG_MAXDOT]);
+    ghi = std::max(ghi, b.h_red[(size_t)blk * LG_MAXDOT + 1]);
+  }
+  const double c_lo = glo - 1.0 - 0.1 * (ghi - glo), c_hi = ghi + 1.0 + 0.1 * (ghi - glo);
+  const double scale = std::max(std::fabs(c_lo), std::fabs(c_hi));
+  // row-major products through column-major dgemm: dgemm(A = a, B = c) leaves c a (row-major)
+  const double half = 0.5, zero = 0.0;
+  BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &half, Fu, np, L.P, np, &zero,
+                        L.T, np));  // G = X F
+  BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &half, L.P, np, L.T, np, &zero,
+                        L.V, np));  // X F X
+  double* Ao = L.Pn;
+  double* Av = L.Eh;
+  hipLaunchKernelGGL(lg_frontier_ops_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, Fu, (const double*)L.P,
+                     (const double*)L.T, (const double*)L.V, n, np, c_lo, c_hi, Ao, Av);
+  HIP_OK(hipGetLastError());
+  // starts: X s for the HOMO, s - X s for the LUMO (s the seed), or the generic vector
+  double* sl = L.fs;       // s, then the LUMO start
+  double* sh = L.fs + np;  // X s, then the HOMO start
+  const bool seeded = L.seeded;
+  if (seeded) {
+    const double mone = -1.0;
+    BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_transpose, n, n, &half, L.P, np, sl, 1, &zero, sh, 1));
+    BLAS_OK(rocblas_daxpy(b.blas, n, &mone, sh, 1, sl, 1));
+  }
+  if (!large_start(b, d, sh, seeded, err) || !large_start(b, d, sl, seeded, err)) return false;
+  double homo = 0.0, lumo = 0.0;
+  if (!large_lanczos_top(b, d, Ao, sh, seeded, scale, homo, err)) return false;
+  if (!large_lanczos_top(b, d, Av, sl, seeded, scale, lumo, err)) return false;
+  lumo = -lumo;
+  aufbau = homo <= lumo;
+  const double lv[2] = {homo, lumo};
+  HIP_OK(hipMemcpy(L.w + nocc - 1, lv, sizeof(lv), hipMemcpyHostToDevice));
+  return true;
 }
 
 struct LargePath {
@@ -3717,6 +3893,7 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
   HIP_OK(hipMemsetAsync(L.P, 0, sizeof(double) * ms, 0));
   HIP_OK(hipMemsetAsync(L.Pn, 0, sizeof(double) * ms, 0));
   L.bnd_age = -1;  // no spectral-bound estimate for this path yet
+  L.seeded = false;  // nor a frontier seed
   if (guess == 2) {
     if (!large_density(b, d, M.H, L.P, err)) return false;
   } else {
@@ -3828,7 +4005,7 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
         Fuse = Fn;
       }
     }
-    if (!large_density(b, d, Fuse, Pn, err)) return false;
+    if (!large_density(b, d, Fuse, Pn, err, R.last_dp < FRONTIER_SEED_DP)) return false;
     Flast = Fuse;
     hipLaunchKernelGGL(lg_diff2_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)Pn, (const double*)P, ms,
                        L.red);
@@ -3847,22 +4024,17 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
   if (P != L.P) {  // keep the final density in L.P
     HIP_OK(hipMemcpyAsync(L.P, P, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
   }
-  if (b.purify && Flast) {
-    // frontier levels of the last step, and the aufbau test: tr(P F) / 2 is the sum of the n_occ
-    // lowest eigenvalues only for the aufbau projector (another set of n_occ levels would exceed
-    // it by at least one HOMO-LUMO gap)
-    if (!large_levels(b, d, Flast, err)) return false;
-    Dots D{};
-    D.k = 1;
-    D.x[0] = L.P;
-    D.y[0] = Flast;
-    double pf = 0.0;
-    if (!large_dots(b, D, ms, &pf, err)) return false;
-    std::vector<double> w(n);
-    HIP_OK(hipMemcpy(w.data(), L.w, sizeof(double) * n, hipMemcpyDeviceToHost));
-    double so = 0.0;
-    for (int i = 0; i < d.nocc; ++i) so += w[i];
-    R.aufbau = std::fabs(0.5 * pf - so) <= 1.0e-9 * std::max(1.0, std::fabs(so));
+  if (b.purify && Flast && !L.last_eig) {
+    // frontier levels of the last step by Lanczos, and the aufbau test HOMO <= LUMO (a last step
+    // taken by the eigensolver left every eigenvalue in L.w already); with an empty side, the
+    // eigenvalues of the last Fock matrix
+    bool ok = true;
+    if (d.nocc > 0 && d.nocc < n) {
+      if (!large_levels_lanczos(b, d, Flast, ok, err)) return false;
+    } else if (!large_levels(b, d, Flast, err)) {
+      return false;
+    }
+    R.aufbau = ok;
   }
   if (!large_fock(b, d, L.P, L.F, err)) return false;
   return large_energy(b, d, L.P, L.F, R.e_elec, err);
@@ -4275,6 +4447,7 @@ bool batch_solve(Batch* bp, std::string& err) {
   b.last_kernel_ms = 0.0;
   b.lg.product_ms = b.lg.fock_ms = 0.0;
   b.lg.products = b.lg.fock_builds = b.lg.product_flop = 0;
+  b.lg.lanczos_steps = b.lg.lanczos_runs = 0;
   if (b.order_all.empty()) return true;
   if (!b.ev0) {
     HIP_OK(hipEventCreate(&b.ev0));
@@ -4353,7 +4526,9 @@ double batch_guess_ms(const Batch* b) { return b ? b->last_guess_ms : 0.0; }
 int batch_large_count(const Batch* b) { return b ? (int)b->large.size() : 0; }
 int batch_small_count(const Batch* b) { return b ? b->nsolve : 0; }
 void batch_large_timing(const Batch* b, double& product_ms, long long& products, long long& product_flop,
-                        double& fock_ms, long long& fock_builds) {
+                        double& fock_ms, long long& fock_builds, long long& lanczos_steps, long long& lanczos_runs) {
+  lanczos_steps = b ? b->lg.lanczos_steps : 0;
+  lanczos_runs = b ? b->lg.lanczos_runs : 0;
   product_ms = b ? b->lg.product_ms : 0.0;
   products = b ? b->lg.products : 0;
   product_flop = b ? b->lg.product_flop : 0;

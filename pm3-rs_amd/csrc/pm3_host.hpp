@@ -97,9 +97,10 @@ double batch_guess_ms(const Batch* b);   // device time of the last solve's lear
 int batch_large_count(const Batch* b);  // molecules of the batch on the host-driven multi-kernel path
 int batch_small_count(const Batch* b);  // molecules of the batch on the persistent one-workgroup kernel
 // Host-driven path, last solve: device time, count and FLOP of the purification products
-// (HIP events around each rocBLAS call), and device time and count of the Fock builds.
+// (HIP events around each rocBLAS call), device time and count of the Fock builds, and the
+// frontier-level Lanczos steps and runs.
 void batch_large_timing(const Batch* b, double& product_ms, long long& products, long long& product_flop,
-                        double& fock_ms, long long& fock_builds);
+                        double& fock_ms, long long& fock_builds, long long& lanczos_steps, long long& lanczos_runs);
 // The same split into the feature kernel and the network kernel, and the batch's directed pairs.
 void batch_guess_timing(const Batch* b, double& features_ms, double& network_ms, long long& directed_pairs);
 // Per-phase wall-clock ticks (10 ns) summed over workgroups, when PM3_PHASE_TIMERS=1 at creation:

@@ -141,7 +141,8 @@ def native():
         L.pm3_batch_guess_timing.argtypes = [ctypes.c_void_p, dp, dp, ctypes.POINTER(ctypes.c_int64)]
         L.pm3_batch_phase_ticks.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]
         L.pm3_batch_large_timing.argtypes = [ctypes.c_void_p, dp, ctypes.POINTER(ctypes.c_int64),
-                                             ctypes.POINTER(ctypes.c_int64), dp, ctypes.POINTER(ctypes.c_int64)]
+                                             ctypes.POINTER(ctypes.c_int64), dp, ctypes.POINTER(ctypes.c_int64),
+                                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
         L.pm3_last_error_message.restype = ctypes.c_char_p
         vp, fp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)
         up, i64p = ctypes.POINTER(ctypes.c_uint16), ctypes.POINTER(ctypes.c_int64)
@@ -458,14 +459,15 @@ class Batch:
         return f.value, n.value, p.value
 
     def large_timing(self):
-        """Molecules on the multi-kernel path, last solve: purification products (ms, count, FLOP)
-        and Fock builds (ms, count), from HIP events around each step."""
+        """Molecules on the multi-kernel path, last solve: purification products (ms, count, FLOP),
+        Fock builds (ms, count), from HIP events around each step, and frontier Lanczos steps/runs."""
         pm, fm = ctypes.c_double(), ctypes.c_double()
         np_, fl, nf = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        ls, lr = ctypes.c_int64(), ctypes.c_int64()
         _raise(native().pm3_batch_large_timing(self._h, ctypes.byref(pm), ctypes.byref(np_), ctypes.byref(fl),
-                                               ctypes.byref(fm), ctypes.byref(nf)))
+                                               ctypes.byref(fm), ctypes.byref(nf), ctypes.byref(ls), ctypes.byref(lr)))
         return {"product_ms": pm.value, "products": np_.value, "product_flop": fl.value, "fock_ms": fm.value,
-                "fock_builds": nf.value}
+                "fock_builds": nf.value, "lanczos_steps": ls.value, "lanczos_runs": lr.value}
 
     PHASES = ("hcore", "fock", "energy_diis_gram", "products", "eigen_or_purify", "sort_or_frontier", "density",
               "other", "diis_solve", "diis_combine")

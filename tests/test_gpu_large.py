@@ -141,7 +141,9 @@ def test_large_path_timing_counters(G):
     assert t["products"] >= 10 * res[0].iterations and t["product_ms"] > 0.0
     assert t["product_flop"] == 2 * 240 ** 3 * t["products"]
     assert t["fock_builds"] >= res[0].iterations and t["fock_ms"] > 0.0
+    assert t["lanczos_runs"] == 2 and 8 <= t["lanczos_steps"] <= 2 * (2 * 240 + 16)
     s = G.Batch([G.Molecule(z2, x2) for z2, x2 in _mols(4, 5)])
     s.solve()
-    assert s.large_timing() == {"product_ms": 0.0, "products": 0, "product_flop": 0, "fock_ms": 0.0, "fock_builds": 0}
+    assert s.large_timing() == {"product_ms": 0.0, "products": 0, "product_flop": 0, "fock_ms": 0.0, "fock_builds": 0,
+                                "lanczos_steps": 0, "lanczos_runs": 0}
     s.close()
