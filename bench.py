@@ -119,7 +119,9 @@ def bench_large(args):
         "checks": {"idempotency_max_abs": idem, "trace_minus_electrons": float(np.trace(P) - 2 * r.n_occ),
                    "gradient_sum_max_abs": float(np.abs(g.sum(axis=0)).max())},
         "device_ms_per_step": float(np.mean(kms)),
-        "roofline": {"bound": "mfma", "kernel": "purification product (rocBLAS dgemm, n = %d)" % n,
+        "roofline": {"bound": "mfma", "kernel": "purification product (rocBLAS dgemm: the %s of an n = %d square)"
+                     % ("2 x 2 block-lower part" if G.tuning_get()["large_split"] == 2 and n % 2 == 0 else
+                        "block-lower part" if G.tuning_get()["large_split"] > 2 else "whole", n),
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "launch_ms": prod_ms / max(nprod, 1), "launches_per_step": nprod / args.steps,

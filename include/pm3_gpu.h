@@ -177,6 +177,8 @@ typedef struct {
                                  4 + the leading rows of X in LDS where the whole of X does not fit */
   int32_t sp2_tight_n;        /* purification spectral bounds from Lanczos from this many AOs on */
   int32_t large_syrk;         /* multi-kernel path: purification products by dsyrk (1) or dgemm (0) */
+  int32_t large_split;        /* multi-kernel path: q in 2..4 computes only the q x q block-lower part of
+                                 each product (q batched dgemm rounds) when q divides the AO count */
 } pm3_tuning;
 void pm3_tuning_default(pm3_tuning* t);
 pm3_status pm3_tuning_get(pm3_tuning* t);

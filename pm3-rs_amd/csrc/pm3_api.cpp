@@ -375,6 +375,7 @@ static void to_c(const Tuning& t, pm3_tuning* c) {
   c->sp2_lds = t.sp2_lds;
   c->sp2_tight_n = t.sp2_tight_n;
   c->large_syrk = t.large_syrk;
+  c->large_split = t.large_split;
 }
 
 static Tuning from_c(const pm3_tuning* c) {
@@ -385,6 +386,7 @@ static Tuning from_c(const pm3_tuning* c) {
   t.sp2_lds = c->sp2_lds;
   t.sp2_tight_n = c->sp2_tight_n;
   t.large_syrk = c->large_syrk;
+  t.large_split = c->large_split;
   return t;
 }
 
@@ -496,6 +498,7 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
       {&Tuning::sp2_lds, {0, 1, 2, 3, 4}, 1},
       {&Tuning::sp2_tight_n, {24, 32, 48, 64, 96, 1 << 20}, 1},
       {&Tuning::large_syrk, {0, 1}, 2},
+      {&Tuning::large_split, {0, 2, 3, 4}, 2},
   };
   Tuning best = start;
   double tbest = t0;
@@ -530,6 +533,7 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
         else if (ax.m == &Tuning::sp2_lds) name = "sp2_lds";
         else if (ax.m == &Tuning::sp2_tight_n) name = "sp2_tight_n";
         else if (ax.m == &Tuning::large_syrk) name = "large_syrk";
+        else if (ax.m == &Tuning::large_split) name = "large_split";
         std::fprintf(stderr, "pm3_autotune: %s %d: %.3f ms%s\n", name.c_str(), v, tc, same ? "" : " (answers differ)");
       }
       if (!same) {

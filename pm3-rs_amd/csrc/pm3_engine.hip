@@ -3008,6 +3008,7 @@ constexpr int LG_BLOCKS = 1024;
 constexpr int LG_THREADS = 256;
 constexpr int LG_MAXDOT = 3 * MAXD + 2;
 constexpr double LARGE_SP2_MARGIN = 0.15;  // least relative margin of the large path's Lanczos bounds
+constexpr int LG_SPLIT_MAX = 4;            // largest q of the q x q block-lower purification products
 
 struct Dots {
   const double* x[LG_MAXDOT];
@@ -3313,6 +3314,7 @@ struct LargeBufs {
   double* lz = nullptr;                          // Lanczos vectors (3 np)
   double* fs = nullptr;                          // frontier seed s and X s (np each), then ...
   double* fr = nullptr;                          // ... the generic start vector r (np_max)
+  double** bp = nullptr;                         // device pointer arrays of the block-lower products
   bool seeded = false;                           // a purification of this path has written the seed
   bool last_eig = false;                         // the path's last density came from the eigensolver
   double bnd_lo = 0.0, bnd_hi = 0.0;            // cached spectral-bound estimate of the purification
@@ -3351,6 +3353,7 @@ struct Batch {
   double last_kernel_ms = 0.0, last_pair_ms = 0.0, last_guess_ms = 0.0;
   int pair_blocks = 1, pair_block = 256;
   bool large_syrk = false;
+  int large_split = 0;
   // learned guess: directed pairs of the eligible molecules and their network inputs/outputs
   const GuessModel* guess_model = nullptr;
   bool learned = false, solved_once = false;
@@ -3450,7 +3453,8 @@ static bool large_setup(Batch& b, std::string& err) {
       !b.alloc(L.Fh, L.m2 * depth, err) || !b.alloc(L.Eh, L.m2 * depth, err) || !b.alloc(L.Dh, L.m2 * depth, err) ||
       !b.alloc(L.J, 10LL * nat, err) || !b.alloc(L.slot, slot, err) ||
       !b.alloc(L.red, (size_t)LG_BLOCKS * LG_MAXDOT, err) || !b.alloc(L.dip, 3, err) || !b.alloc(L.info, 1, err) ||
-      !b.alloc(L.fin, LG_MAXDOT, err) || !b.alloc(L.lz, 3LL * np, err) || !b.alloc(L.fs, 3LL * np, err))
+      !b.alloc(L.fin, LG_MAXDOT, err) || !b.alloc(L.lz, 3LL * np, err) || !b.alloc(L.fs, 3LL * np, err) ||
+      !b.alloc(L.bp, (size_t)LG_SPLIT_MAX * 3 * (LG_SPLIT_MAX * (LG_SPLIT_MAX + 1) / 2), err))
     return false;
   {
     std::vector<double> r(np);
@@ -3680,6 +3684,26 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     t2 = h[1];
     return true;
   };
+  // q x q block-lower products: X X restricted to the blocks (I, J), I >= J, of size bs = n / q, as
+  // q rounds of one batched dgemm each (round K: Y_IJ += X_IK X_KJ); the update reads only the lower
+  // triangle, which these blocks cover.  (q + 1) / (2 q) of the full product's flops.
+  const int q = (!b.large_syrk && b.large_split >= 2 && b.large_split <= LG_SPLIT_MAX && n % b.large_split == 0 &&
+                 n / b.large_split >= 64) ? b.large_split : 0;
+  const int bs = q ? n / q : 0, nb = q * (q + 1) / 2;
+  if (q) {
+    std::vector<double*> h((size_t)q * 3 * nb);
+    for (int K = 0; K < q; ++K) {
+      int e = 0;
+      for (int I = 0; I < q; ++I)
+        for (int J = 0; J <= I; ++J, ++e) {
+          // row-major Y_IJ = X_IK X_KJ through column-major dgemm: A = X_KJ, B = X_IK
+          h[((size_t)K * 3 + 0) * nb + e] = L.V + (size_t)K * bs * np + (size_t)J * bs;
+          h[((size_t)K * 3 + 1) * nb + e] = L.V + (size_t)I * bs * np + (size_t)K * bs;
+          h[((size_t)K * 3 + 2) * nb + e] = L.T + (size_t)I * bs * np + (size_t)J * bs;
+        }
+    }
+    HIP_OK(hipMemcpy(L.bp, h.data(), sizeof(double*) * h.size(), hipMemcpyHostToDevice));
+  }
   double tr, t2;
   hipLaunchKernelGGL(lg_sp2_init_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, A, L.V, n, np, lo, hi, L.red);
   HIP_OK(hipGetLastError());
@@ -3695,9 +3719,18 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     if (tail == 0) break;
     const int sq = std::fabs(t2 - occ) < std::fabs(2.0 * tr - t2 - occ) ? 1 : 0;
     HIP_OK(hipEventRecord(L.evp0, 0));
-    // the whole square by dgemm, or one triangle by dsyrk (upper in column-major order = the
-    // row-major lower triangle): the update reads only that triangle, so X stays exactly symmetric
-    if (b.large_syrk)
+    // the block-lower products, or the whole square by dgemm, or one triangle by dsyrk (upper in
+    // column-major order = the row-major lower triangle): the update reads only the row-major lower
+    // triangle, so X stays exactly symmetric
+    if (q) {
+      for (int K = 0; K < q; ++K) {
+        const double beta = K == 0 ? 0.0 : 1.0;
+        BLAS_OK(rocblas_dgemm_batched(b.blas, rocblas_operation_none, rocblas_operation_none, bs, bs, bs, &one,
+                                      (const double* const*)(L.bp + ((size_t)K * 3 + 0) * nb), np,
+                                      (const double* const*)(L.bp + ((size_t)K * 3 + 1) * nb), np, &beta,
+                                      L.bp + ((size_t)K * 3 + 2) * nb, np, nb));
+      }
+    } else if (b.large_syrk)
       BLAS_OK(rocblas_dsyrk(b.blas, rocblas_fill_upper, rocblas_operation_none, n, n, &one, L.V, np, &zero, L.T, np));
     else
       BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, L.V, np, L.V, np,
@@ -3711,7 +3744,7 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     HIP_OK(hipEventElapsedTime(&pms, L.evp0, L.evp1));
     L.product_ms += pms;
     ++L.products;
-    L.product_flop += (b.large_syrk ? 1LL : 2LL) * n * (long long)n * n;
+    L.product_flop += q ? 2LL * bs * (long long)bs * bs * nb * q : (b.large_syrk ? 1LL : 2LL) * n * (long long)n * n;
     if (tail > 0) --tail;
   }
   if (tail != 0 || std::fabs(tr - occ) > 1.0e-6) return true;
@@ -4402,6 +4435,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
       1, std::min<long long>((a.npairs_total + b.pair_block - 1) / b.pair_block,
                              (long long)tune.pair_blocks_per_cu * prop.multiProcessorCount));
   b.large_syrk = tune.large_syrk != 0;
+  b.large_split = tune.large_split;
   a.max_scf = opt.max_scf;
   a.e_tol = opt.e_tol;
   a.p_tol = opt.p_tol;

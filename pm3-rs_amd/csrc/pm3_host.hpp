@@ -74,6 +74,9 @@ struct Tuning {
                                 // 4 + the leading rows of X in LDS where the whole of X does not fit
   int sp2_tight_n = 48;         // purification bounds from Lanczos from this size (AOs) on
   int large_syrk = 0;           // host-driven path: purification products by dsyrk (1) or dgemm (0)
+  int large_split = 2;          // host-driven path: q >= 2 computes only the q x q block-lower part of each
+                                // product (q batched dgemm rounds) when q divides n; 0 one dgemm
+                                // (2 measured fastest at 3072 AOs: 0.70 ms per product against 0.87)
 };
 // The table batch_create reads; on first use it is loaded from the file PM3_TUNING_FILE names,
 // if set.  Set or loaded tables apply to batches created afterwards (not thread-safe against a

@@ -27,6 +27,7 @@ const Field kFields[] = {
     {"sp2_lds", &Tuning::sp2_lds, 0, 4},
     {"sp2_tight_n", &Tuning::sp2_tight_n, 0, 1 << 20},
     {"large_syrk", &Tuning::large_syrk, 0, 1},
+    {"large_split", &Tuning::large_split, 0, 4},
 };
 
 Tuning initial_table() {

@@ -109,12 +109,16 @@ def test_every_entry_changes_speed_only(G):
         assert dh <= HOF_TOL and dp <= 1e-7
     # the host-driven path's product routine
     big = [G.Molecule(np.asarray(z, np.int32), W.to_bohr(x)) for z, x in [W.water_cluster(40, W.water_box_for(40), 40)]]
-    sa, ra, _ = _solve(G, big, dict(d, large_syrk=0))
-    sb, rb, _ = _solve(G, big, dict(d, large_syrk=1))
+    sa, ra, _ = _solve(G, big, dict(d, large_split=0))
+    for t in (dict(d, large_syrk=1), dict(d, large_split=2), dict(d, large_split=3)):
+        sb, rb, _ = _solve(G, big, t)
+        print("large table", {k: v for k, v in t.items() if v != d[k]}, "iterations", ra[0].iterations,
+              rb[0].iterations, flush=True)
+        assert sa[0] == 0 and sb[0] == 0
+        assert abs(ra[0].heat_of_formation_kcal - rb[0].heat_of_formation_kcal) <= HOF_TOL
+        assert np.abs(ra[0].density - rb[0].density).max() <= 1e-7
+        assert abs(ra[0].homo_ev - rb[0].homo_ev) <= 1e-6 and abs(ra[0].lumo_ev - rb[0].lumo_ev) <= 1e-6
     G.tuning_set(d)
-    assert sa[0] == 0 and sb[0] == 0
-    assert abs(ra[0].heat_of_formation_kcal - rb[0].heat_of_formation_kcal) <= HOF_TOL
-    assert np.abs(ra[0].density - rb[0].density).max() <= 1e-7
 
 
 def test_autotune_keeps_answers_and_sets_the_table(G):
