@@ -155,7 +155,8 @@ pm3_status pm3_batch_guess_timing(const pm3_batch* batch, double* features_ms, d
  * purification fallbacks. */
 pm3_status pm3_batch_phase_ticks(const pm3_batch* batch, uint64_t* out16);
 /* Molecules on the multi-kernel path, last solve: device time (ms, HIP events around each rocBLAS
- * call), count and FLOP (2 n^3 per dgemm, n^3 per dsyrk) of the purification products, device
+ * call), count and FLOP issued (2 n^3 per dgemm, n^3 per dsyrk, 2 b^3 per b x b block product of
+ * the block-lower form: 1.5 n^3 at q = 2) of the purification products, device
  * time and count of the Fock builds, and the Lanczos steps and runs of the frontier levels.
  * Zeros when the batch has no such molecule; any output pointer may be null. */
 pm3_status pm3_batch_large_timing(const pm3_batch* batch, double* product_ms, int64_t* products,
