@@ -3229,10 +3229,41 @@ __global__ __launch_bounds__(LG_THREADS) void lg_fock_pairs_kernel(KArgs A, MolD
   fock_pairs(A, M, M.H, P, F, gtid(), gstride());
 }
 
-__global__ __launch_bounds__(LG_THREADS) void lg_fock_jsum_kernel(KArgs A, MolDesc md, const double* P, double* J,
-                                                                  double* slot) {
+// The sums of fock_jsum with one wave per output (heavy atom and component, or hydrogen): lanes
+// over the partner atoms, then a fixed shuffle tree, so the whole grid works on a single molecule's
+// few thousand outputs and the sums stay reproducible.
+__global__ __launch_bounds__(LG_THREADS) void lg_fock_jsum_wide_kernel(KArgs A, MolDesc md, const double* P, double* J,
+                                                                       double* slot) {
   const Mol M = mol_of(A, md, slot);
-  fock_jsum(A, M, P, J, gtid(), gstride());
+  const int np = M.np, lane = threadIdx.x & 63;
+  const double* JX = M.slot;
+  const double* JH = M.slot + 20LL * M.nxx;
+  const int nitems = M.nh * 10 + M.nH;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (int e = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); e < nitems; e += nwaves) {
+    double s = 0.0;
+    if (e < M.nh * 10) {
+      const int a = e / 10, q = e % 10;
+      for (int b = lane; b < M.nh; b += 64) {
+        if (b == a) continue;
+        s += b > a ? JX[(long long)q * M.nxx + tri_index(a, b, M.nh)]
+                   : JX[(long long)(10 + q) * M.nxx + tri_index(b, a, M.nh)];
+      }
+      const double* jh = JH + (long long)q * M.nxh + (long long)a * M.nH;
+      for (int h = lane; h < M.nH; h += 64) s += jh[h];
+    } else {
+      const int h = e - M.nh * 10;
+      for (int a = lane; a < M.nh; a += 64) s += JH[10LL * M.nxh + (long long)a * M.nH + h];
+      for (int g = lane; g < M.nH; g += 64) {
+        if (g == h) continue;
+        const int lo = g < h ? g : h, hi = g < h ? h : g;
+        const int og = A.aoff[M.atom0 + M.nh + g];
+        s += M.Whh[tri_index(lo, hi, M.nH)] * P[(long long)og * np + og];
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) J[e < M.nh * 10 ? e : (M.nh + (e - M.nh * 10)) * 10] = s;
+  }
 }
 
 __global__ __launch_bounds__(LG_THREADS) void lg_fock_diag_kernel(KArgs A, MolDesc md, const double* P, double* F,
@@ -3500,7 +3531,8 @@ static bool large_fock(Batch& b, const MolDesc& d, const double* P, double* F, s
   LargeBufs& L = b.lg;
   HIP_OK(hipEventRecord(L.evf0, 0));
   hipLaunchKernelGGL(lg_fock_pairs_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, F, b.lg.slot);
-  hipLaunchKernelGGL(lg_fock_jsum_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, b.lg.J, b.lg.slot);
+  hipLaunchKernelGGL(lg_fock_jsum_wide_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, b.lg.J,
+                     b.lg.slot);
   hipLaunchKernelGGL(lg_fock_diag_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, F,
                      (const double*)b.lg.J);
   HIP_OK(hipGetLastError());
