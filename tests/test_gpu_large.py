@@ -152,3 +152,28 @@ def test_large_path_timing_counters(G):
     assert s.large_timing() == {"product_ms": 0.0, "products": 0, "product_flop": 0, "fock_ms": 0.0, "fock_builds": 0,
                                 "lanczos_steps": 0, "lanczos_runs": 0}
     s.close()
+
+
+def test_mixed_batch_of_small_and_large_molecules(G):
+    """One batch holding molecules for the persistent kernel and one for the host-driven path, with
+    gradients: every molecule's answers equal those of its own single-molecule gradient run (the
+    queue puts the large molecule last, and its gradient gather starts after the small ones)."""
+    small = _mols(3, 7)
+    z, x = W.water_cluster(40, W.water_box_for(40), 41)
+    mols = small + [(np.asarray(z, np.int32), W.to_bohr(x))]
+    ms = [G.Molecule(zz, xx) for zz, xx in mols]
+    opts = G.Pm3Options(p_tol=1e-10, e_tol=1e-12, max_scf=500)
+    b = G.Batch(ms, options=opts, want_density=True, want_gradient=True)
+    b.solve()
+    st, res = b.results()
+    grads = b.gradients()
+    t = b.large_timing()
+    b.close()
+    assert np.all(st == 0) and t["products"] > 0
+    for m, r, g in zip(ms, res, grads):
+        one = G.closed_form_gradient(m, options=opts)
+        print("nao", r.density.shape[0], "dHf", abs(r.heat_of_formation_kcal - one.scf.heat_of_formation_kcal),
+              "dg", np.abs(g - one.gradient).max(), flush=True)
+        assert abs(r.heat_of_formation_kcal - one.scf.heat_of_formation_kcal) <= HOF_TOL
+        assert np.abs(g - one.gradient).max() <= 1e-6
+        assert np.abs(g.sum(axis=0)).max() < 1e-8
