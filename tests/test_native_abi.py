@@ -110,3 +110,22 @@ def test_tuning_file_from_environment(tmp_path):
     env = dict(os.environ, PM3_TUNING_FILE=str(tmp_path / "t.txt"))
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert out.stdout.split() == ["4", "64", "256"], out.stderr
+
+
+def test_autotune_rejects_bad_arguments_without_device():
+    """The tuner validates its workload on the host first (no batch is created) and leaves the
+    process-wide table as it was."""
+    import pm3amd as G
+    start = G.tuning_get()
+    ip, dp = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)
+    z = np.array([8, 1, 1], np.int32)
+    xyz = np.zeros((3, 3))
+    t, rep = G._CTuning(), G._CTuneReport()
+    o = G._COptions()
+    G.native().pm3_options_default(ctypes.byref(o))
+    for n_mol, ptr in ((0, np.array([0], np.int32)), (1, np.array([1, 3], np.int32)), (2, np.array([0, 3, 2], np.int32))):
+        st = G.native().pm3_autotune(n_mol, ptr.ctypes.data_as(ip), z.ctypes.data_as(ip), xyz.ctypes.data_as(dp), None,
+                                      G.Pm3Parameters.standard()._h, ctypes.byref(o), 0, 1, ctypes.byref(t),
+                                      ctypes.byref(rep))
+        assert st == 1, (n_mol, ptr)  # PM3_INVALID_INPUT
+    assert G.tuning_get() == start
