@@ -89,6 +89,8 @@ def lib():
                                    ctypes.POINTER(Result), dp, dp]
         L.orc_run_batch.argtypes = [ctypes.c_int, ip, ip, dp, dp, ctypes.POINTER(Options), ctypes.c_int, dp, dp, ip,
                                     ip]
+        L.orc_gradient_batch.argtypes = [ctypes.c_int, ip, ip, dp, dp, ctypes.POINTER(Options), ctypes.c_int, dp,
+                                         dp, ip, ip, dp]
         L.orc_num_threads.restype = ctypes.c_int
         fp = ctypes.POINTER(ctypes.c_float)
         up = ctypes.POINTER(ctypes.c_uint16)
@@ -256,6 +258,26 @@ def run_batch(mols, opts=None, nthreads=0):
     lib().orc_run_batch(k, _i(ptr), _i(zz), _d(xyz), nullp, ctypes.byref(opts), int(nthreads), _d(tot), _d(hof),
                         _i(it), _i(st))
     return tot, hof, it, st
+
+
+def gradient_batch(mols, opts=None, nthreads=0):
+    """run_batch with the gradient of each molecule too; returns (tot, hof, it, st, [grad (nat, 3)])."""
+    opts = opts or options()
+    ptr = np.zeros(len(mols) + 1, dtype=np.int32)
+    for k, (z, _) in enumerate(mols):
+        ptr[k + 1] = ptr[k] + len(z)
+    zz = np.ascontiguousarray(np.concatenate([np.asarray(z, dtype=np.int32) for z, _ in mols]))
+    xyz = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1, 3) for _, x in mols]))
+    k = len(mols)
+    tot = np.zeros(k)
+    hof = np.zeros(k)
+    it = np.zeros(k, dtype=np.int32)
+    st = np.zeros(k, dtype=np.int32)
+    g = np.zeros((int(ptr[-1]), 3))
+    nullp = ctypes.POINTER(ctypes.c_double)()
+    lib().orc_gradient_batch(k, _i(ptr), _i(zz), _d(xyz), nullp, ctypes.byref(opts), int(nthreads), _d(tot), _d(hof),
+                             _i(it), _i(st), _d(g))
+    return tot, hof, it, st, [g[ptr[m]:ptr[m + 1]] for m in range(k)]
 
 
 def num_threads():

@@ -1661,6 +1661,27 @@ int orc_run_batch(int nmol, const int* atom_ptr, const int* z, const double* xyz
   return OK;
 }
 
+// The same harness for SCF + closed_form_gradient (src/gradient.rs:201-276) per molecule.  grad
+// holds 3 doubles per atom (eV/Bohr) in the CSR order of z.
+int orc_gradient_batch(int nmol, const int* atom_ptr, const int* z, const double* xyz, const double* charge,
+                       const Options* opt, int nthreads, double* total_ev, double* hof, int* iters, int* status,
+                       double* grad) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int k = 0; k < nmol; ++k) {
+    int a0 = atom_ptr[k], nat = atom_ptr[k + 1] - a0;
+    Result r{};
+    int s = orc_gradient(nat, z + a0, xyz + 3 * a0, charge ? charge[k] : 0.0, opt, &r, grad + 3 * a0, nullptr);
+    total_ev[k] = r.total_ev;
+    hof[k] = r.hof_kcal;
+    iters[k] = r.iterations;
+    status[k] = s;
+  }
+  return OK;
+}
+
 int orc_num_threads() {
 #ifdef _OPENMP
   return omp_get_max_threads();

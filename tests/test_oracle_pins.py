@@ -172,6 +172,24 @@ def test_gradient_matches_finite_difference(oracle):
             assert abs((ep - em) / (2 * h) - G[a, k]) < 5e-5, (a, k)
 
 
+def test_threaded_gradient_batch_equals_single_runs(oracle):
+    # the CPU-baseline harness for SCF + gradient: molecules of different sizes packed CSR and
+    # solved over threads give each molecule's own run bit for bit, in the molecule's atom order
+    mols = [atoms_to_zx([["O", 0, 0, 0], ["H", 0.9584, 0, 0], ["H", -0.24, 0.9278, 0]]),
+            atoms_to_zx([["C", 0, 0, 0], ["O", 0.05, 0, 1.21], ["H", 0.94, 0.1, -0.54], ["H", -0.94, 0, -0.5]]),
+            atoms_to_zx([["H", 0, 0, 0], ["H", 0.74, 0, 0]]),
+            atoms_to_zx([["C", 0, 0, 0], ["H", 0.63, 0.63, 0.63], ["H", -0.63, -0.63, 0.63], ["H", -0.63, 0.63, -0.6],
+                         ["H", 0.63, -0.63, -0.63]])]
+    opts = oracle.options(p_tol=1e-10, e_tol=1e-12)
+    tot, hof, it, st, g = oracle.gradient_batch(mols, opts, nthreads=3)
+    assert np.all(st == 0)
+    for k, (z, x) in enumerate(mols):
+        s1, o = oracle.gradient(z, x, 0.0, opts)
+        assert s1 == 0 and g[k].shape == (len(z), 3)
+        assert hof[k] == o["hof_kcal"] and tot[k] == o["total_ev"] and it[k] == o["iterations"]
+        assert np.array_equal(g[k], o["gradient"]), k
+
+
 def test_accelerators_agree(oracle):
     # src/scf.rs:1938-1963
     z, x = atoms_to_zx([["C", 0, 0, 0], ["O", 0, 0, 1.21], ["H", 0.94, 0, -0.54], ["H", -0.94, 0, -0.54]])
