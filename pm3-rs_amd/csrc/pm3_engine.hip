@@ -1979,6 +1979,107 @@ __device__ __forceinline__ double lanczos_top(const double* La, bool sq, const d
   return top;
 }
 
+// Lane j's value of v, given to every lane of the wave (j wave-uniform).
+__device__ __forceinline__ double lane_value(double v, int j) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+  return __hiloint2double(hi, lo);
+}
+
+// lanczos_top run by one wave for n <= 64, with no workgroup barrier: lane i holds element i of
+// the basis vector q and the residual w in registers; the product reads the square LDS matrix L
+// (ld n) down its columns, w_j given to the wave by lane j.  The tridiagonal al / be (lanczos_cap(n)
+// + 2 doubles each) is the wave's own LDS; lane 0 writes it and the whole wave reads it at the
+// checks.  Start vector, recurrence, checks and stopping rules are lanczos_top's; only the
+// product's summation order differs (two running sums over the columns instead of per-slice sums).
+__device__ __noinline__ double lanczos_top_wave(const double* L, int n, double scale, double* al, double* be,
+                                                const double* s, const double* t, int side,
+                                                unsigned long long* steps) {
+  const int lane = threadIdx.x & 63;
+  const int cap = lanczos_cap(n);
+  const bool live = lane < n;
+  const int r = live ? lane : 0;  // lanes past n read row 0 and keep zeros
+  double w, q = 0.0;
+  {
+    const double g = 1.0 + 0.5 * sin(1.7 * (double)lane + 0.3);
+    const double x = side == 0 || !live ? 0.0 : side == 1 ? t[lane] : s[lane] - t[lane];
+    double ca = 0.0, cg = 1.0;
+    if (side != 0) {
+      const double a2 = wave_sum(x * x);
+      const double g2 = wave_sum(live ? g * g : 0.0);
+      if (a2 > 1.0e-280 && a2 < 1.0e300) {
+        ca = 1.0 / sqrt(a2);
+        cg = 1.0e-3 / sqrt(g2);
+      }
+    }
+    w = !live ? 0.0 : ca != 0.0 ? ca * x + cg * g : g;
+  }
+  double wn = sqrt(wave_sum(w * w));
+  const int first = side != 0 ? 8 : 16, every = side != 0 ? 3 : 7;
+  double bsub = 0.0, th1 = -1.0e300, th2 = -1.0e300, top = 0.0;
+  for (int k = 0; k < cap; ++k) {
+    double s0 = 0.0, s1 = 0.0;
+    int j = 0;
+    for (; j + 1 < n; j += 2) {
+      s0 += L[j * n + r] * lane_value(w, j);
+      s1 += L[(j + 1) * n + r] * lane_value(w, j + 1);
+    }
+    if (j < n) s0 += L[j * n + r] * lane_value(w, j);
+    // q_k = w / wn; w = A q_k - beta_{k-1} q_{k-1}; alpha_k = q_k . w; w -= alpha_k q_k
+    const double qn = w / wn;
+    const double x = live ? (s0 + s1) / wn - bsub * q : 0.0;
+    q = qn;
+    const double alpha = wave_sum(qn * x);
+    w = x - alpha * q;
+    const double beta = sqrt(wave_sum(w * w));
+    const int mk = k + 1;
+    const bool breakdown = beta <= 1.0e-13 * scale;
+    wn = beta;
+    bsub = beta;
+    const bool check = mk == cap || breakdown || (mk >= first && (mk & every) == 0);
+    if (lane == 0) {
+      al[k] = alpha;
+      be[k] = beta;
+    }
+    if (!check) continue;
+    // lane 0's LDS stores come before the wave's loads (a wave's LDS operations run in order)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double ghi = -1.0e300, glo = 1.0e300;
+    for (int i = lane; i < mk; i += 64) {
+      const double rr = (i > 0 ? fabs(be[i - 1]) : 0.0) + (i + 1 < mk ? fabs(be[i]) : 0.0);
+      ghi = fmax(ghi, al[i] + rr);
+      glo = fmin(glo, al[i] - rr);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      ghi = fmax(ghi, __shfl_xor(ghi, o, 64));
+      glo = fmin(glo, __shfl_xor(glo, o, 64));
+    }
+    const double sc = fmax(fabs(ghi), fabs(glo));
+    double blo = fmax(glo, th1) - 1.0e-14 * sc, bhi = ghi + 1.0e-14 * sc;
+    if (th2 > -1.0e299) {
+      const double xb = th1 + fmax(4.0 * (th1 - th2), 1.0e-9 * sc);
+      if (xb < bhi && sturm_below(al, be, mk, xb) == mk) bhi = xb;
+    }
+    const double th = tridiag_max(al, be, mk, blo, bhi);
+    const bool stall = th - th1 <= 1.0e-13 * scale && th - th2 <= 1.0e-12 * scale;
+    const bool stop = mk == cap || breakdown || stall || ritz_residual(al, be, mk, th) <= LZ_TOL;
+    th2 = th1;
+    th1 = th;
+    top = th;
+    if (stop) {
+      if (steps && lane == 0) atomicAdd(steps, (unsigned long long)mk);
+      break;
+    }
+  }
+  return top;
+}
+// LDS doubles of frontier_levels' one-wave runs: both operators as full squares and each run's
+// tridiagonal.
+__host__ __device__ constexpr int lanczos_wave_lds(int n) { return 2 * n * n + 4 * (lanczos_cap(n) + 2); }
+
 // Estimates of the lowest and highest eigenvalue of the symmetric np-strided matrix Ag from a
 // fixed number of Lanczos steps (Ritz values: lo_est >= lambda_min, hi_est <= lambda_max); LDS
 // v as lanczos_lds(n).
@@ -2052,9 +2153,12 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
     __syncthreads();
     return;
   }
-  const bool square = lanczos_lds_square(n) <= M.lds_doubles;
+  // up to 64 AOs both runs go at once, one wave each, both operators square in LDS
+  const bool waves = n <= 64 && lanczos_wave_lds(n) <= M.lds_doubles;
+  const bool square = waves || lanczos_lds_square(n) <= M.lds_doubles;
   const bool packed = square || lanczos_lds_packed(n) <= M.lds_doubles;
   double* Lp = packed ? g_lds : nullptr;
+  double* Lv = g_lds + n * n;  // -A_v of the one-wave runs
   double* vec = square ? g_lds + n * n : packed ? g_lds + n * (n + 1) / 2 : g_lds;
   // (with n_occ = 0 or n_occ = n one side is empty and the other operator is Fu itself)
   double lo, hi;
@@ -2078,7 +2182,9 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
                   } else {
                     Ao[i * np + j] = ao;
                   }
-                  Av[i * np + j] = -(Fu[i * np + j] - G[i * np + j] - G[j * np + i] + sv + c_hi * x);
+                  const double av = -(Fu[i * np + j] - G[i * np + j] - G[j * np + i] + sv + c_hi * x);
+                  if (waves) Lv[i * n + j] = av;
+                  else Av[i * np + j] = av;
                 });
   __syncthreads();
   // seeds from the last purification: t = X s, HOMO start t, LUMO start s - t (X = P / 2)
@@ -2093,6 +2199,19 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
   }
   unsigned long long* lsteps = B.ph ? &B.ph[10] : nullptr;
   if (B.ph && tid == 0) B.ph[11] += (unsigned long long)((nocc > 0) + (nocc < n));
+  if (waves) {
+    const int wave = tid >> 6, cap2 = lanczos_cap(n) + 2;
+    double* tri = g_lds + 2 * n * n + 2 * cap2 * wave;
+    if (wave == 0 && nocc > 0) {
+      const double homo = lanczos_top_wave(Lp, n, scale, tri, tri + cap2, B.fs, B.ft, seeded, lsteps);
+      if ((tid & 63) == 0) B.eps[nocc - 1] = homo;
+    } else if (wave == 1 && nocc < n) {
+      const double lumo = -lanczos_top_wave(Lv, n, scale, tri, tri + cap2, B.fs, B.ft, seeded ? 2 : 0, lsteps);
+      if ((tid & 63) == 0) B.eps[nocc] = lumo;
+    }
+    __syncthreads();
+    return;
+  }
   double homo = 0.0, lumo = 0.0;
   if (nocc > 0) homo = lanczos_top(Lp, square, Ao, n, np, scale, vec, B.fs, B.ft, seeded, lsteps);
   if (nocc < n) {
