@@ -1731,24 +1731,24 @@ __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const doubl
 
 // Number of eigenvalues below x of the symmetric tridiagonal matrix (diagonal a, off-diagonal b):
 // sign changes of the Sturm sequence q_0 = 1, q_i = (a_i - x) q_{i-1} - b_{i-1}^2 q_{i-2}
-// (q_i = det(T_i - x I)), rescaled as it goes; only multiply-adds on the dependent chain.
+// (q_i = det(T_i - x I)); only one multiply-add on the dependent chain per element.  The pair
+// (q_{i-1}, q_i) is rescaled by a select every fourth element: over four elements |q| changes by
+// far less than the 1e100 headroom on either side, for matrix entries up to about 1e20.
 __device__ inline int sturm_below(const double* a, const double* b, int m, double x) {
   double q0 = 1.0, q1 = a[0] - x;
-  if (q1 == 0.0) q1 = -1.0e-300;
+  q1 = q1 == 0.0 ? -1.0e-300 : q1;
   int cnt = q1 < 0.0;
   for (int i = 1; i < m; ++i) {
-    double q2 = (a[i] - x) * q1 - (b[i - 1] * b[i - 1]) * q0;
-    if (q2 == 0.0) q2 = q1 < 0.0 ? 1.0e-300 : -1.0e-300;
+    double q2 = fma(a[i] - x, q1, -(b[i - 1] * b[i - 1]) * q0);
+    q2 = q2 == 0.0 ? (q1 < 0.0 ? 1.0e-300 : -1.0e-300) : q2;
     cnt += (q2 < 0.0) != (q1 < 0.0);
     q0 = q1;
     q1 = q2;
-    const double aq = fabs(q1);
-    if (aq > 1.0e100) {
-      q0 *= 1.0e-100;
-      q1 *= 1.0e-100;
-    } else if (aq < 1.0e-100 && fabs(q0) < 1.0e-100) {
-      q0 *= 1.0e100;
-      q1 *= 1.0e100;
+    if ((i & 3) == 0) {
+      const double aq = fabs(q1);
+      const double f = aq > 1.0e100 ? 1.0e-100 : aq < 1.0e-100 && fabs(q0) < 1.0e-100 ? 1.0e100 : 1.0;
+      q0 *= f;
+      q1 *= f;
     }
   }
   return cnt;
@@ -1775,18 +1775,19 @@ __device__ inline double tridiag_max(const double* a, const double* b, int m, do
 // Residual b[m-1] |s[m-1]| of the top Ritz pair (theta, s), |s| = 1, of the Lanczos tridiagonal:
 // s by the forward three-term recurrence from s_0 = 1, which for the largest eigenvalue (above every
 // eigenvalue of the leading submatrices) follows the dominant, sign-stable solution.
+// The reciprocals 1 / b[i] do not depend on the recurrence, so its chain is multiply-adds only.
 __host__ __device__ inline double ritz_residual(const double* a, const double* b, int m, double theta) {
   double s0 = 0.0, s1 = 1.0, ss = 1.0;
   for (int i = 0; i + 1 < m; ++i) {
-    const double s2 = ((theta - a[i]) * s1 - (i > 0 ? b[i - 1] * s0 : 0.0)) / b[i];
+    const double rb = 1.0 / b[i];
+    const double s2 = ((theta - a[i]) * s1 - (i > 0 ? b[i - 1] * s0 : 0.0)) * rb;
     ss += s2 * s2;
     s0 = s1;
     s1 = s2;
-    if (fabs(s1) > 1.0e100) {
-      s0 *= 1.0e-100;
-      s1 *= 1.0e-100;
-      ss *= 1.0e-200;
-    }
+    const double f = fabs(s1) > 1.0e100 ? 1.0e-100 : 1.0;
+    s0 *= f;
+    s1 *= f;
+    ss *= f * f;
   }
   return fabs(b[m - 1] * s1) / sqrt(ss);
 }
