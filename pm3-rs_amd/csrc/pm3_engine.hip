@@ -138,6 +138,13 @@ __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// Lane j's value of v, given to every lane of the wave (j wave-uniform): v_readlane, a register
+// read, where __shfl would go through the LDS crossbar (ds_bpermute).
+__device__ __forceinline__ double lane_value(double v, int j) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
+  return __hiloint2double(hi, lo);
+}
 __device__ inline double wave_max(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -1248,7 +1255,7 @@ __device__ bool cdiis_wave(const Diis& D, double* c) {
   if (lane < n)
     for (int j = 0; j < n; ++j) sc = fmax(sc, fabs(D.b[lane][j]));
   double scale = 0.0;
-  for (int i = 0; i < n; ++i) scale = fmax(scale, __shfl(sc, i, 64));
+  for (int i = 0; i < n; ++i) scale = fmax(scale, lane_value(sc, i));
   if (!(scale > 0.0) || !isfinite(scale)) return false;
   const int dim = n + 1;
   double mr[DM];
@@ -1272,7 +1279,7 @@ __device__ bool cdiis_wave(const Diis& D, double* c) {
       if (j == col) mc = mr[j];
     const double val = (lane >= col && lane < dim) ? fabs(mc) : -1.0;
     double best = -1.0;
-    for (int r = col; r < dim; ++r) best = fmax(best, __shfl(val, r, 64));
+    for (int r = col; r < dim; ++r) best = fmax(best, lane_value(val, r));
     if (best < 1.0e-12) return false;
     const unsigned long long hit = __ballot(lane >= col && lane < dim && val == best);
     const int piv = __ffsll((long long)hit) - 1;
@@ -1284,8 +1291,8 @@ __device__ bool cdiis_wave(const Diis& D, double* c) {
     }
     double prow[DM];
 #pragma unroll
-    for (int j = 0; j < DM; ++j) prow[j] = __shfl(mr[j], col, 64);
-    const double prhs = __shfl(rhs, col, 64);
+    for (int j = 0; j < DM; ++j) prow[j] = lane_value(mr[j], col);
+    const double prhs = lane_value(rhs, col);
     double pc = 0.0, rc = 0.0;
 #pragma unroll
     for (int j = 0; j < DM; ++j)
@@ -1313,7 +1320,7 @@ __device__ bool cdiis_wave(const Diis& D, double* c) {
       if (j > col && j < dim) sv -= mr[j] * x[j];
       if (j == col) dg = mr[j];
     }
-    const double xc = __shfl(sv / dg, col, 64);
+    const double xc = lane_value(sv / dg, col);
 #pragma unroll
     for (int j = 0; j < DM; ++j)
       if (j == col) x[j] = xc;
@@ -1349,18 +1356,18 @@ __device__ void adiis_wave(Diis& D, double* srt) {
     }
   }
   double L = 1.0e-12;
-  for (int i = 0; i < n; ++i) L = fmax(L, __shfl(row, i, 64));
+  for (int i = 0; i < n; ++i) L = fmax(L, lane_value(row, i));
   const double lr = 1.0 / L;
   double c = (lane == n - 1) ? 1.0 : 0.0;
   for (int it = 0; it < 400; ++it) {
     double acc = 2.0 * dk;
 #pragma unroll
     for (int j = 0; j < MAXD; ++j)
-      if (j < n) acc += srow[j] * __shfl(c, j, 64);
+      if (j < n) acc += srow[j] * lane_value(c, j);
     const double tr = c - lr * acc;
     int rank = 0;
     for (int j = 0; j < n; ++j) {
-      const double uj = __shfl(tr, j, 64);
+      const double uj = lane_value(tr, j);
       rank += (uj > tr || (uj == tr && j < lane)) ? 1 : 0;
     }
     if (lane < n) srt[rank] = tr;
@@ -1371,12 +1378,12 @@ __device__ void adiis_wave(Diis& D, double* srt) {
     const double t = (css - 1.0) / (double)(k + 1);
     const bool cond = lane < n && srt[k] - t > 0.0;
     const unsigned long long m = __ballot(cond);
-    const double theta = m ? __shfl(t, 63 - __clzll((long long)m), 64) : 0.0;
+    const double theta = m ? lane_value(t, 63 - __clzll((long long)m)) : 0.0;
     const double pr = fmax(tr - theta, 0.0);
     const double dlt = fabs(pr - c);
     if (lane < n) c = pr;
     double delta = 0.0;
-    for (int i = 0; i < n; ++i) delta += __shfl(dlt, i, 64);
+    for (int i = 0; i < n; ++i) delta += lane_value(dlt, i);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (delta < 1.0e-12) break;
@@ -1978,13 +1985,6 @@ __device__ __forceinline__ double lanczos_top(const double* La, bool sq, const d
     }
   }
   return top;
-}
-
-// Lane j's value of v, given to every lane of the wave (j wave-uniform).
-__device__ __forceinline__ double lane_value(double v, int j) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), j);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), j);
-  return __hiloint2double(hi, lo);
 }
 
 // lanczos_top run by one wave for n <= 64, with no workgroup barrier: lane i holds element i of
