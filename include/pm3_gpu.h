@@ -8,6 +8,9 @@
  *   pm3_params_standard <- Pm3Parameters::standard() (src/params.rs:135-137)
  *   pm3_run_batch  -- many independent molecules in one persistent launch (no reference
  *                     counterpart: the reference solves one molecule per call).
+ *   pm3_optimize   <- optimize(&Molecule, &Pm3Parameters, &Pm3Options, &OptOptions)
+ *                     -> Result<OptResult> (src/optimizer.rs:233-262)
+ *   pm3_optimize_batch -- the same L-BFGS for many molecules, one batched solve per round.
  *
  * Units follow the reference (src/system.rs:21-26): positions in Bohr, energies in eV, heat
  * of formation in kcal/mol, gradient in eV/Bohr, dipole in Debye.  Matrices are n x n
@@ -132,6 +135,13 @@ pm3_status pm3_batch_solve(pm3_batch* batch);
 pm3_status pm3_batch_results(pm3_batch* batch, pm3_status* status, pm3_result* results, double* charges,
                              double* density);
 void pm3_batch_destroy(pm3_batch* batch);
+/* New positions (Bohr, the creation layout xyz_bohr[3 * (atom_ptr[k] + i) + c], the same atoms)
+ * for every molecule of the batch; the next pm3_batch_solve computes integrals, SCF and gradients
+ * there.  PM3_INVALID_INPUT for positions that are not finite or put two atoms of a molecule
+ * together, PM3_UNSUPPORTED for a batch using PM3_GUESS_LEARNED (its neighbour lists are fixed at
+ * creation); the batch is unchanged on failure.  No reference counterpart: a geometry change in
+ * the reference is a new run_pm3 call (src/optimizer.rs:344-347). */
+pm3_status pm3_batch_set_positions(pm3_batch* batch, const double* xyz_bohr);
 /* Analytic gradients of the last solve (batch created with PM3_BATCH_WANT_GRADIENT), eV/Bohr,
  * gradient[3 * (atom_ptr[k] + i) + c]; the same per-molecule gradient pm3_gradient returns. */
 pm3_status pm3_batch_gradient(pm3_batch* batch, double* gradient);
@@ -202,6 +212,39 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
 
 /* Detail text of the calling thread's last error ("" if none). */
 const char* pm3_last_error_message(void);
+
+/* OptOptions (src/optimizer.rs:15-35).  grad_step is the reference's finite-difference step; the
+ * gradient here is analytic, so it is accepted and not used. */
+typedef struct {
+  int32_t max_iter; /* default 200 */
+  double gtol;      /* default 1e-3 eV/Bohr, on the largest gradient component */
+  double grad_step; /* default 5e-4 Bohr */
+  int32_t history;  /* default 8 L-BFGS pairs */
+} pm3_opt_options;
+void pm3_opt_options_default(pm3_opt_options* out);
+
+/* OptResult (src/optimizer.rs:45-52): its flags and counts, and the scalars of the first and last
+ * trajectory steps (OptStep, :37-43).  evaluations counts the SCF + gradient solves of the
+ * molecule: the first point and every line-search trial (an accepted trial is not solved again). */
+typedef struct {
+  int32_t status; /* PM3_OK, or the failure of the first evaluation */
+  int32_t converged, iterations, evaluations;
+  double energy_ev, heat_of_formation_kcal, max_gradient; /* at the final geometry */
+  double first_energy_ev;                                 /* at the starting geometry */
+} pm3_opt_result;
+
+/* optimize (src/optimizer.rs:233-262) for many closed-shell molecules at once: L-BFGS with the
+ * backtracking Armijo search of drive (:77-221) per molecule, every round one batched SCF +
+ * gradient solve over all running molecules.  Molecules in the atom_ptr / z / xyz_bohr / charge
+ * layout of pm3_run_batch; results[k] per molecule, final positions xyz_out[3 * atom_ptr[n_mol]]
+ * (Bohr).  Returns the first device or launch failure, else PM3_OK (per-molecule failures are in
+ * results[k].status).  PM3_UNSUPPORTED with PM3_GUESS_LEARNED. */
+pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
+                              const double* charge, const pm3_params* params, const pm3_options* opts,
+                              const pm3_opt_options* opt, pm3_opt_result* results, double* xyz_out);
+/* optimize for one molecule (pm3_optimize_batch of one); xyz_out[3 * n_atoms]. */
+pm3_status pm3_optimize(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                        const pm3_opt_options* opt, pm3_opt_result* result, double* xyz_out);
 
 #ifdef __cplusplus
 }

@@ -363,3 +363,95 @@ def guess_loss(z, xyz_bohr, y, T):
     dy = np.zeros((len(y), GUESS_NO))
     loss = lib().orc_guess_loss(len(z), _i(z), _d(xyz), _f(y), _d(T), _d(dy))
     return loss, dy
+
+
+def _dot(a, b):
+    # the reference's iterator sum: products added in index order
+    s = 0.0
+    for u, v in zip(a.tolist(), b.tolist()):
+        s += u * v
+    return s
+
+
+def optimize(z, xyz_bohr, charge=0.0, opts=None, max_iter=200, gtol=1.0e-3, history=8):
+    """optimize / drive (src/optimizer.rs:77-262): L-BFGS on the PM3 surface with a backtracking
+    Armijo search.  Trials ask for the energy alone (run_pm3's total_ev, src/optimizer.rs:254),
+    accepted points are evaluated again for the gradient (closed_form_gradient, :244-252).
+    Returns a dict: converged, iterations, evaluations (gradient calls), energy_ev, hof_kcal,
+    max_gradient, xyz (final, Bohr), trajectory (energies of the accepted points, first included)."""
+    z = np.asarray(z, dtype=np.int32)
+    opts = opts or options()
+    x = np.array(xyz_bohr, dtype=np.float64).reshape(-1)
+
+    def evaluate(xv):
+        st, o = gradient(z, xv.reshape(-1, 3), charge, opts)
+        if st != 0:
+            raise RuntimeError(STATUS[st])
+        g = o["gradient"].reshape(-1).copy()
+        return o["total_ev"], g, o["hof_kcal"], float(np.abs(g).max()) if g.size else 0.0
+
+    def energy_only(xv):
+        st, o = run(z, xv.reshape(-1, 3), charge, opts, want_arrays=False)
+        return o["total_ev"] if st == 0 else None
+
+    energy, g, heat, maxg = evaluate(x)
+    evaluations = 1
+    s_hist, y_hist, rho_hist = [], [], []
+    trajectory = [energy]
+    converged = maxg < gtol
+    iterations = 0
+    for it in range(max_iter):
+        iterations = it + 1
+        if converged:
+            break
+        q = g.copy()
+        m = len(s_hist)
+        alpha = [0.0] * m
+        for i in range(m - 1, -1, -1):
+            a = rho_hist[i] * _dot(s_hist[i], q)
+            alpha[i] = a
+            q += (-a) * y_hist[i]
+        if m > 0:
+            sy = _dot(s_hist[-1], y_hist[-1])
+            yy = _dot(y_hist[-1], y_hist[-1])
+            gamma = sy / yy if yy > 0.0 else 1.0
+        else:
+            gamma = 0.1 / max(maxg, 1.0e-6)
+        q *= gamma
+        for i in range(m):
+            beta = rho_hist[i] * _dot(y_hist[i], q)
+            q += (alpha[i] - beta) * s_hist[i]
+        d = -q
+        if _dot(d, g) > 0.0:
+            d = -g
+        gdd = _dot(g, d)
+        step, ok = 1.0, False
+        while True:
+            x_new = x + step * d
+            trial = energy_only(x_new)
+            if trial is not None and trial <= energy + 1.0e-4 * step * gdd:
+                ok = True
+                break
+            step *= 0.5
+            if step < 1.0e-8:
+                break
+        if not ok:
+            break
+        e_new, g_new, h_new, m_new = evaluate(x_new)
+        evaluations += 1
+        s = x_new - x
+        y = g_new - g
+        sy = _dot(s, y)
+        if sy > 1.0e-10:
+            s_hist.append(s)
+            y_hist.append(y)
+            rho_hist.append(1.0 / sy)
+            if len(s_hist) > history:
+                s_hist.pop(0)
+                y_hist.pop(0)
+                rho_hist.pop(0)
+        x, g, energy, heat, maxg = x_new, g_new, e_new, h_new, m_new
+        converged = maxg < gtol
+        trajectory.append(energy)
+    return {"converged": converged, "iterations": iterations, "evaluations": evaluations, "energy_ev": energy,
+            "hof_kcal": heat, "max_gradient": maxg, "xyz": x.reshape(-1, 3), "trajectory": trajectory}

@@ -319,6 +319,23 @@ pm3_status pm3_batch_gradient(pm3_batch* h, double* gradient) {
   return PM3_OK;
 }
 
+pm3_status pm3_batch_set_positions(pm3_batch* h, const double* xyz_bohr) {
+  g_last_error.clear();
+  if (!h || !h->b) return fail(PM3_INVALID_INPUT, "null batch");
+  if (!xyz_bohr && !h->xyz.empty()) return fail(PM3_INVALID_INPUT, "null positions");
+  // the host copy (read by pm3_batch_results) changes only once the device has the new values
+  std::vector<double> keep(h->xyz);
+  std::copy(xyz_bohr, xyz_bohr + h->xyz.size(), h->xyz.begin());
+  std::string why;
+  const int s = batch_set_positions(h->b, h->mols, why);
+  if (s != 0) {
+    std::copy(keep.begin(), keep.end(), h->xyz.begin());
+    return fail((pm3_status)s, why);
+  }
+  h->fetched = false;
+  return PM3_OK;
+}
+
 void pm3_batch_destroy(pm3_batch* h) {
   if (!h) return;
   batch_destroy(h->b);

@@ -4815,6 +4815,60 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
 
 void batch_destroy(Batch* b) { delete b; }
 
+int batch_set_positions(Batch* bp, const std::vector<MolIn>& mols, std::string& err) {
+  Batch& b = *bp;
+  if (b.learned) {
+    err = "new positions for a batch with the learned guess: its neighbour lists come from the creation geometry";
+    return 4;
+  }
+  if ((int)mols.size() != b.nmol) {
+    err = "molecule count differs from the batch's";
+    return 1;
+  }
+  if (b.atoms_total == 0) return 0;
+  std::vector<double> ax(b.atoms_total), ay(b.atoms_total), az(b.atoms_total);
+  for (int k = 0; k < b.nmol; ++k) {
+    const MolDesc& d = b.desc[k];
+    if (d.nat == 0) continue;  // refused at creation: no atoms on the device
+    const MolIn& m = mols[k];
+    if (m.n_atoms != d.nat) {
+      err = "molecule " + std::to_string(k) + ": atom count differs from the batch's";
+      return 1;
+    }
+    for (int i = 0; i < m.n_atoms; ++i) {
+      for (int c = 0; c < 3; ++c)
+        if (!std::isfinite(m.xyz[3 * i + c])) {
+          err = "molecule " + std::to_string(k) + ": positions must be finite";
+          return 1;
+        }
+      for (int j = 0; j < i; ++j) {
+        const double dx = m.xyz[3 * i] - m.xyz[3 * j], dy = m.xyz[3 * i + 1] - m.xyz[3 * j + 1],
+                     dz = m.xyz[3 * i + 2] - m.xyz[3 * j + 2];
+        if (!(dx * dx + dy * dy + dz * dz > 1.0e-16)) {
+          err = "molecule " + std::to_string(k) + ": atoms " + std::to_string(j) + " and " + std::to_string(i) +
+                " coincide";
+          return 1;
+        }
+      }
+    }
+    const std::vector<int>& ord = b.atom_map[k];
+    for (int t = 0; t < d.nat; ++t) {
+      const int i = ord[t];
+      ax[d.atom0 + t] = m.xyz[3 * i];
+      ay[d.atom0 + t] = m.xyz[3 * i + 1];
+      az[d.atom0 + t] = m.xyz[3 * i + 2];
+    }
+  }
+  const size_t bytes = sizeof(double) * (size_t)b.atoms_total;
+  auto upload = [&]() -> bool {
+    HIP_OK(hipMemcpy(const_cast<double*>(b.args.ax), ax.data(), bytes, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(const_cast<double*>(b.args.ay), ay.data(), bytes, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(const_cast<double*>(b.args.az), az.data(), bytes, hipMemcpyHostToDevice));
+    return true;
+  };
+  return upload() ? 0 : 6;
+}
+
 bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,
                int& status, std::string& err, const GuessModel* model) {
   std::vector<MolIn> mols{mol};

@@ -94,6 +94,11 @@ bool batch_solve(Batch* b, std::string& err);
 bool batch_fetch(Batch* b, const Params& P, const std::vector<MolIn>& mols, std::vector<MolResult>& out,
                  std::string& err);
 void batch_destroy(Batch* b);
+// New positions for the batch's molecules (the same atoms, in the caller's order, as mols at
+// creation); the next solve's kernels read them.  Returns 0, or a pm3_status code with err set:
+// 1 bad positions (not finite, coincident atoms, wrong counts), 4 a learned-guess batch, 6 a
+// device failure.
+int batch_set_positions(Batch* b, const std::vector<MolIn>& mols, std::string& err);
 double batch_kernel_ms(const Batch* b);  // device time of the last solve's SCF launch (HIP events)
 double batch_pair_ms(const Batch* b);    // device time of the last solve's pair-integral launch
 double batch_guess_ms(const Batch* b);   // device time of the last solve's learned-guess launches

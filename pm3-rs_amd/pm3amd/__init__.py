@@ -94,6 +94,18 @@ class _CResult(ctypes.Structure):
                 ("has_lumo", ctypes.c_int32), ("guess_used", ctypes.c_int32)]
 
 
+class _COptOptions(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int32), ("gtol", ctypes.c_double), ("grad_step", ctypes.c_double),
+                ("history", ctypes.c_int32)]
+
+
+class _COptResult(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("converged", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("evaluations", ctypes.c_int32), ("energy_ev", ctypes.c_double),
+                ("heat_of_formation_kcal", ctypes.c_double), ("max_gradient", ctypes.c_double),
+                ("first_energy_ev", ctypes.c_double)]
+
+
 EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_run", "pm3_gradient",
             "pm3_run_batch", "pm3_batch_create", "pm3_batch_solve", "pm3_batch_results", "pm3_batch_destroy",
             "pm3_batch_last_kernel_ms", "pm3_batch_gradient", "pm3_batch_last_pair_ms", "pm3_batch_last_guess_ms", "pm3_batch_guess_timing",
@@ -104,7 +116,8 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_guess_data_add", "pm3_guess_data_stats", "pm3_guess_data_destroy", "pm3_guess_train_options_default",
             "pm3_guess_train", "pm3_guess_eval", "pm3_guess_grad", "pm3_guess_mlp", "pm3_guess_features", "pm3_guess_density",
             "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune",
-            "pm3_batch_large_timing"]
+            "pm3_batch_large_timing", "pm3_batch_set_positions", "pm3_opt_options_default", "pm3_optimize_batch",
+            "pm3_optimize"]
 
 _lib = None
 
@@ -172,6 +185,13 @@ def native():
         L.pm3_autotune.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ctypes.c_void_p, ctypes.POINTER(_COptions),
                                    ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_CTuning),
                                    ctypes.POINTER(_CTuneReport)]
+        L.pm3_batch_set_positions.argtypes = [ctypes.c_void_p, dp]
+        L.pm3_opt_options_default.argtypes = [ctypes.POINTER(_COptOptions)]
+        L.pm3_opt_options_default.restype = None
+        L.pm3_optimize_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ctypes.c_void_p, ctypes.POINTER(_COptions),
+                                         ctypes.POINTER(_COptOptions), ctypes.POINTER(_COptResult), dp]
+        L.pm3_optimize.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
+                                   ctypes.POINTER(_COptOptions), ctypes.POINTER(_COptResult), dp]
         _lib = L
     return _lib
 
@@ -503,6 +523,17 @@ class Batch:
             out.append(_result(res[i], Pi, None, None, q[a0:a1].copy()))
         return status, out
 
+    def set_positions(self, positions):
+        """New positions (Bohr), one (n_atoms, 3) array per molecule, for the next solve
+        (pm3_batch_set_positions); the batch is unchanged if they are refused."""
+        xyz = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1, 3) for p in positions]))
+        if xyz.shape != self._xyz.shape:
+            raise InvalidInput("positions do not match the batch's atoms")
+        _raise(native().pm3_batch_set_positions(self._h, _dp(xyz)))
+        self._xyz = xyz
+        self.molecules = [Molecule(m.z, xyz[self.atom_ptr[i]:self.atom_ptr[i + 1]].copy(), m.charge, m.multiplicity)
+                          for i, m in enumerate(self.molecules)]
+
     def gradients(self):
         """Per-molecule analytic gradients (eV/Bohr) of the last solve (needs want_gradient=True)."""
         g = np.zeros((int(self.atom_ptr[-1]), 3))
@@ -519,3 +550,73 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+@dataclasses.dataclass
+class OptOptions:
+    """OptOptions (src/optimizer.rs:15-35); grad_step is accepted and unused (analytic gradient)."""
+    max_iter: int = 200
+    gtol: float = 1.0e-3
+    grad_step: float = 5.0e-4
+    history: int = 8
+
+    def _c(self):
+        return _COptOptions(int(self.max_iter), float(self.gtol), float(self.grad_step), int(self.history))
+
+
+@dataclasses.dataclass
+class OptResult:
+    """OptResult (src/optimizer.rs:45-52) without the per-step trajectory: the final molecule,
+    the flags and counts, and the energies of the first and last steps."""
+    molecule: Molecule
+    converged: bool
+    iterations: int
+    evaluations: int
+    energy_ev: float
+    heat_of_formation_kcal: float
+    max_gradient: float
+    first_energy_ev: float
+
+
+def _opt_result(m, r, xyz):
+    return OptResult(Molecule(m.z, xyz, m.charge, m.multiplicity), bool(r.converged), int(r.iterations),
+                     int(r.evaluations), r.energy_ev, r.heat_of_formation_kcal, r.max_gradient, r.first_energy_ev)
+
+
+def optimize_batch(molecules: List[Molecule], params: Optional[Pm3Parameters] = None,
+                   options: Optional[Pm3Options] = None, opt: Optional[OptOptions] = None):
+    """L-BFGS geometry optimisation of many molecules at once (pm3_optimize_batch): the optimiser
+    of src/optimizer.rs:77-262 per molecule, one batched SCF + gradient solve per round.
+    Returns (status array, list of OptResult)."""
+    params = params or Pm3Parameters.standard()
+    options = options or Pm3Options()
+    opt = opt or OptOptions()
+    k = len(molecules)
+    ptr = np.zeros(k + 1, dtype=np.int32)
+    for i, m in enumerate(molecules):
+        ptr[i + 1] = ptr[i] + len(m.z)
+    z = np.ascontiguousarray(np.concatenate([m.z for m in molecules]).astype(np.int32)) if k else np.zeros(0, np.int32)
+    xyz = np.ascontiguousarray(np.concatenate([m.positions for m in molecules])) if k else np.zeros((0, 3))
+    q = np.ascontiguousarray([float(m.charge) for m in molecules], dtype=np.float64)
+    out = np.zeros_like(xyz)
+    res = (_COptResult * max(k, 1))()
+    ip = ctypes.POINTER(ctypes.c_int32)
+    _raise(native().pm3_optimize_batch(k, ptr.ctypes.data_as(ip), z.ctypes.data_as(ip), _dp(xyz), _dp(q), params._h,
+                                       ctypes.byref(options._c()), ctypes.byref(opt._c()), res, _dp(out)))
+    status = np.array([res[i].status for i in range(k)], dtype=np.int32)
+    return status, [_opt_result(m, res[i], out[ptr[i]:ptr[i + 1]].copy()) for i, m in enumerate(molecules)]
+
+
+def optimize(molecule: Molecule, params: Optional[Pm3Parameters] = None, scf_options: Optional[Pm3Options] = None,
+             opt: Optional[OptOptions] = None) -> OptResult:
+    """optimize (src/optimizer.rs:233-262) on the GPU: L-BFGS to max |g| < gtol; raises like the
+    reference when the first evaluation fails."""
+    params = params or Pm3Parameters.standard()
+    scf_options = scf_options or Pm3Options()
+    opt = opt or OptOptions()
+    cm = molecule._c()
+    res = _COptResult()
+    out = np.zeros((len(molecule.z), 3))
+    _raise(native().pm3_optimize(ctypes.byref(cm), params._h, ctypes.byref(scf_options._c()), ctypes.byref(opt._c()),
+                                 ctypes.byref(res), _dp(out)))
+    return _opt_result(molecule, res, out)

@@ -1,0 +1,129 @@
+"""Geometry optimisation on the GPU (pm3_optimize / pm3_optimize_batch): the reference's L-BFGS
+(src/optimizer.rs:77-262) over batched SCF + analytic gradient solves, against the oracle's
+restatement of the same optimiser on the CPU path; and the position upload it is built on
+(pm3_batch_set_positions) against freshly created batches."""
+import numpy as np
+import pytest
+
+from pm3amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+A = 1.0 / 0.529177210903
+TIGHT = dict(p_tol=1e-10, e_tol=1e-12)
+
+
+@pytest.fixture(scope="module")
+def G():
+    import pm3amd
+    pm3amd.native()
+    return pm3amd
+
+
+def _cases():
+    rng = np.random.default_rng(31)
+    out = [("water", [8, 1, 1], np.array([[0.0, 0.0, 0.0], [1.05, 0.0, 0.0], [-0.30, 1.02, 0.0]])),
+           ("formaldehyde", [6, 8, 1, 1], np.array([[0, 0, 0], [0.05, 0, 1.25], [0.94, 0.1, -0.54], [-0.94, 0, -0.5]])),
+           ("methane", [6, 1, 1, 1, 1], np.array([[0, 0, 0], [0.7, 0.6, 0.62], [-0.6, -0.66, 0.6], [-0.62, 0.6, -0.7],
+                                                  [0.6, -0.6, -0.6]]))]
+    z, x = W.alkane(3)
+    out.append(("propane", list(z), W.jitter(np.asarray(x), 0.08, rng)))
+    out.append(("methanol", [6, 8, 1, 1, 1, 1], np.array([[0, 0, 0], [1.42, 0.02, 0], [1.75, 0.93, 0.05],
+                                                           [-0.37, 1.03, 0.02], [-0.38, -0.5, 0.9],
+                                                           [-0.36, -0.52, -0.88]])))
+    return [(name, np.asarray(z, np.int32), np.asarray(x, np.float64) * A) for name, z, x in out]
+
+
+def _dimer():
+    z, x = W.water_cluster(2, 6.0, 7)
+    return np.asarray(z, np.int32), np.asarray(x, np.float64) * A
+
+
+def test_optimisations_follow_the_oracle(G, oracle):
+    for name, z, x in _cases():
+        r = G.optimize(G.Molecule(z, x), scf_options=G.Pm3Options(**TIGHT))
+        o = oracle.optimize(z, x, 0.0, oracle.options(**TIGHT))
+        print(f"case {name} iterations {r.iterations} / {o['iterations']} evaluations {r.evaluations} "
+              f"dHf {r.heat_of_formation_kcal - o['hof_kcal']:.1e} max|dx| "
+              f"{np.abs(r.molecule.positions - o['xyz']).max():.1e}", flush=True)
+        assert r.converged and o["converged"] and r.max_gradient < 1e-3
+        assert r.iterations == o["iterations"], name
+        assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= 1e-5, name
+        assert np.abs(r.molecule.positions - o["xyz"]).max() <= 1e-5, name
+        assert abs(r.first_energy_ev - o["trajectory"][0]) <= 1e-6 and r.energy_ev <= r.first_energy_ev
+        # one solve per accepted point less than the reference's energy-then-gradient evaluations
+        assert r.evaluations >= r.iterations
+
+
+def test_flat_surface_reaches_the_oracles_basin(G, oracle):
+    """A water dimer: its intermolecular modes are nearly flat, so gradient differences of 1e-9
+    eV/Bohr grow over the hundred-odd steps until an Armijo decision differs and the two
+    trajectories part; both still end in the same minimum."""
+    z, x = _dimer()
+    r = G.optimize(G.Molecule(z, x), scf_options=G.Pm3Options(**TIGHT))
+    o = oracle.optimize(z, x, 0.0, oracle.options(**TIGHT))
+    print("dimer", r.iterations, o["iterations"], r.heat_of_formation_kcal - o["hof_kcal"], flush=True)
+    assert r.converged and o["converged"]
+    assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= 1e-3
+
+
+def test_reference_water_run_on_the_gpu(G):
+    # src/optimizer.rs:362-386 with the default SCF options
+    z = np.array([8, 1, 1], np.int32)
+    x = np.array([[0.0, 0.0, 0.0], [1.05, 0.0, 0.0], [-0.30, 1.02, 0.0]]) * A
+    r = G.optimize(G.Molecule(z, x))
+    print("water", r.iterations, r.heat_of_formation_kcal, r.max_gradient, flush=True)
+    assert r.converged and abs(r.heat_of_formation_kcal - (-53.4330121104622)) < 1e-4
+
+
+def test_batch_gives_each_molecule_its_single_run(G):
+    """Molecules of different sizes finish after different numbers of rounds, so the batch is
+    rebuilt from the running ones several times; each result is still the molecule's own run."""
+    rng = np.random.default_rng(5)
+    mols = [G.Molecule(z, x) for _, z, x in _cases()] + [G.Molecule(*_dimer())]
+    for z, x in W.mixed_batch(10, 44, sigma=0.08):
+        mols.append(G.Molecule(np.asarray(z, np.int32), W.to_bohr(W.jitter(np.asarray(x), 0.03, rng))))
+    opts = G.Pm3Options(**TIGHT)
+    st, rs = G.optimize_batch(mols, options=opts)
+    assert np.all(st == 0)
+    for m, r in zip(mols, rs):
+        # (a floppy water cluster may reach max_iter unconverged, as the reference's would)
+        one = G.optimize(m, scf_options=opts)
+        assert r.converged == one.converged
+        assert r.iterations == one.iterations and r.evaluations == one.evaluations
+        assert abs(r.energy_ev - one.energy_ev) <= 1e-10
+        assert np.abs(r.molecule.positions - one.molecule.positions).max() <= 1e-10
+    print("iterations", [r.iterations for r in rs], "converged", sum(r.converged for r in rs), flush=True)
+    assert all(r.converged for r in rs[:5])  # the rigid molecules
+    assert len({r.iterations for r in rs}) > 1
+
+
+def test_new_positions_equal_a_fresh_batch(G):
+    rng = np.random.default_rng(9)
+    base = [(np.asarray(z, np.int32), W.to_bohr(x)) for z, x in W.mixed_batch(12, 3)]
+    moved = [W.jitter(x, 0.05, rng) for _, x in base]
+    opts = G.Pm3Options(**TIGHT)
+    b = G.Batch([G.Molecule(z, x) for z, x in base], options=opts, want_gradient=True)
+    b.solve()
+    b.set_positions(moved)
+    b.solve()
+    s1, r1 = b.results()
+    g1 = b.gradients()
+    f = G.Batch([G.Molecule(z, x) for (z, _), x in zip(base, moved)], options=opts, want_gradient=True)
+    f.solve()
+    s2, r2 = f.results()
+    g2 = f.gradients()
+    f.close()
+    assert np.all(s1 == 0) and np.all(s2 == 0)
+    for a, c, ga, gc in zip(r1, r2, g1, g2):
+        assert a.heat_of_formation_kcal == c.heat_of_formation_kcal and a.iterations == c.iterations
+        assert np.array_equal(ga, gc)
+    # refused positions leave the batch as it was
+    bad = [x.copy() for x in moved]
+    bad[4][1] = bad[4][0]
+    with pytest.raises(G.InvalidInput):
+        b.set_positions(bad)
+    b.solve()
+    s3, r3 = b.results()
+    assert all(a.heat_of_formation_kcal == c.heat_of_formation_kcal for a, c in zip(r3, r2))
+    b.close()

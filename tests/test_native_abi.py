@@ -129,3 +129,40 @@ def test_autotune_rejects_bad_arguments_without_device():
                                       ctypes.byref(rep))
         assert st == 1, (n_mol, ptr)  # PM3_INVALID_INPUT
     assert G.tuning_get() == start
+
+
+def test_optimizer_defaults_and_host_checks_without_device():
+    """OptOptions::default (src/optimizer.rs:26-35), and the optimiser's argument checks, which
+    return before any batch is created."""
+    import pm3amd as G
+    L = G.native()
+    o = G._COptOptions()
+    L.pm3_opt_options_default(ctypes.byref(o))
+    assert (o.max_iter, o.gtol, o.grad_step, o.history) == (200, 1.0e-3, 5.0e-4, 8)
+    ip, dp = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)
+    z = np.array([8, 1, 1], np.int32)
+    xyz = np.zeros((3, 3))
+    out = np.zeros((3, 3))
+    res = (G._COptResult * 1)()
+    params = G.Pm3Parameters.standard()._h
+    so = G._COptions()
+    L.pm3_options_default(ctypes.byref(so))
+
+    def call(n_mol, ptr, opts=so, opt=o):
+        return L.pm3_optimize_batch(n_mol, ptr.ctypes.data_as(ip), z.ctypes.data_as(ip), xyz.ctypes.data_as(dp), None,
+                                    params, ctypes.byref(opts), ctypes.byref(opt), res, out.ctypes.data_as(dp))
+
+    assert call(0, np.array([0], np.int32)) == 0  # nothing to do, no device work
+    assert call(-1, np.array([0], np.int32)) == 1
+    assert call(1, np.array([1, 3], np.int32)) == 1  # atom_ptr[0] != 0
+    bad = G._COptOptions(-1, 1e-3, 5e-4, 8)
+    assert call(1, np.array([0, 3], np.int32), opt=bad) == 1
+    learned = G._COptions()
+    L.pm3_options_default(ctypes.byref(learned))
+    learned.guess = 4  # PM3_GUESS_LEARNED: neighbour lists are fixed at batch creation
+    assert call(1, np.array([0, 3], np.int32), opts=learned) == 4
+    m = G.Molecule(z, np.eye(3), 0.0, 2)
+    r = G._COptResult()
+    assert L.pm3_optimize(ctypes.byref(m._c()), params, ctypes.byref(so), ctypes.byref(o), ctypes.byref(r),
+                          out.ctypes.data_as(dp)) == 4  # open shell: not in this engine
+    assert L.pm3_batch_set_positions(None, xyz.ctypes.data_as(dp)) == 1

@@ -190,6 +190,47 @@ def test_threaded_gradient_batch_equals_single_runs(oracle):
         assert np.array_equal(g[k], o["gradient"]), k
 
 
+def test_optimizer_reproduces_the_reference_water_run(oracle):
+    # src/optimizer.rs:362-386: the distorted water reaches dHf = -53.4330121104622 kcal/mol
+    # (the reference asserts 5e-3; the restated L-BFGS follows the same trajectory)
+    A = oracle.ANGSTROM_TO_BOHR
+    z = np.array([8, 1, 1], np.int32)
+    x = np.array([[0.0, 0.0, 0.0], [1.05, 0.0, 0.0], [-0.30, 1.02, 0.0]]) * A
+    r = oracle.optimize(z, x)
+    assert r["converged"] and r["max_gradient"] < 1e-3
+    assert abs(r["hof_kcal"] - (-53.4330121104622)) < 1e-5
+    assert r["trajectory"][-1] < r["trajectory"][0]
+    assert all(b <= a for a, b in zip(r["trajectory"], r["trajectory"][1:]))  # Armijo: never uphill
+
+
+def test_optimizer_finds_the_minimum_of_a_bond_scan(oracle):
+    # H2 from 0.9 A: the optimised bond length against a golden-section minimisation of the
+    # energy along the bond; stopping at max |g| < gtol leaves |d - d_min| <= gtol / k
+    opts = oracle.options(p_tol=1e-11, e_tol=1e-12)
+    z = np.array([1, 1], np.int32)
+
+    def energy(d):
+        return oracle.run(z, np.array([[0.0, 0.0, 0.0], [d, 0.0, 0.0]]), 0.0, opts, want_arrays=False)[1]["total_ev"]
+
+    a, b = 1.0, 2.0
+    gr = (5 ** 0.5 - 1) / 2
+    for _ in range(80):
+        c, d = b - gr * (b - a), a + gr * (b - a)
+        if energy(c) < energy(d):
+            b = d
+        else:
+            a = c
+    d_min = 0.5 * (a + b)
+    e_min = energy(d_min)
+    h = 1e-3
+    k = (energy(d_min + h) - 2 * e_min + energy(d_min - h)) / h ** 2
+    r = oracle.optimize(z, np.array([[0.0, 0.0, 0.0], [0.9 * oracle.ANGSTROM_TO_BOHR, 0.1, 0.0]]), 0.0, opts)
+    assert r["converged"]
+    d_opt = float(np.linalg.norm(r["xyz"][1] - r["xyz"][0]))
+    assert abs(d_opt - d_min) <= 1.5e-3 / k
+    assert -1e-9 <= r["energy_ev"] - e_min <= 1e-6
+
+
 def test_accelerators_agree(oracle):
     # src/scf.rs:1938-1963
     z, x = atoms_to_zx([["C", 0, 0, 0], ["O", 0, 0, 1.21], ["H", 0.94, 0, -0.54], ["H", -0.94, 0, -0.54]])
