@@ -46,7 +46,7 @@ NET_FLOP_PER_PAIR = 2.0 * (96 * 128 + 128 * 128 + 128 * 16)  # forward pass per 
 # HBM bytes per scf_batch_kernel launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a
 # 512 MB copy: x2.0 / x1.0) for this build on the default workload, per leg
 # (profiles/scf_batch_kernel_traffic_mixed4096.md); not measured for other workloads.
-MEASURED_TRAFFIC = {("mixed4096", "learned"): 1.296e11, ("mixed4096", "plain"): 1.773e11}
+MEASURED_TRAFFIC = {("mixed4096", "learned"): 1.295e11, ("mixed4096", "plain"): 1.771e11}
 HBM_PEAK_GBS = 8000.0
 HOF_TOL = 1.0e-5
 GRAD_TOL = 2.295e-7  # eV/Bohr, max |dg| against the CPU path
