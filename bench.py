@@ -156,7 +156,12 @@ def bench_optimize(args):
         z, x = W.alkane(1 + k % 24)
         zx.append((np.asarray(z, np.int32), W.to_bohr(W.jitter(np.asarray(x), 0.08, rng))))
     mols = [G.Molecule(z, x) for z, x in zx]
-    opts = G.Pm3Options(p_tol=1.0e-10, e_tol=1.0e-12)
+    # SCF to p_tol 1e-10 by default, so that final geometries are checkable against the CPU path
+    # (--p-tol 1e-7 is the reference's Pm3Options::default(), whose SCF noise parts trajectories
+    # along soft torsions by more than the check's 1e-3 kcal/mol)
+    p_tol = args.p_tol or 1.0e-10
+    e_tol = 1.0e-12 if p_tol < 1.0e-8 else 1.0e-8
+    opts = G.Pm3Options(p_tol=p_tol, e_tol=e_tol)
     opt = G.OptOptions()
     tuning = apply_tuning(args, mols, opts, True, reps=1)
     for _ in range(args.warmup):
@@ -176,7 +181,7 @@ def bench_optimize(args):
         stride = max(1, len(mols) // n)
         idx = list(range(0, len(mols), stride))[:n]
         threads = min(args.cpu_threads, os.cpu_count() or 1)
-        oo = O.options(p_tol=1.0e-10, e_tol=1.0e-12)
+        oo = O.options(p_tol=p_tol, e_tol=e_tol)
         c0 = time.perf_counter()
         with ThreadPoolExecutor(threads) as ex:
             cres = list(ex.map(lambda i: O.optimize(zx[i][0], zx[i][1], 0.0, oo), idx))
@@ -199,8 +204,8 @@ def bench_optimize(args):
         "value": valid / (ms_per_step * 1e-3), "unit": "optimisations/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded geometries, jittered by 0.08 A)",
-        "config": {"workload": "optimize512", "molecules": len(mols), "seed": args.seed, "p_tol": 1e-10,
-                   "e_tol": 1e-12, "gtol": opt.gtol, "history": opt.history,
+        "config": {"workload": "optimize512", "molecules": len(mols), "seed": args.seed, "p_tol": p_tol,
+                   "e_tol": e_tol, "gtol": opt.gtol, "history": opt.history,
                    "parallelism": "one batched SCF + gradient launch per L-BFGS round over all running molecules"},
         "converged": int(ok.sum()),
         "mean_iterations": float(np.mean([r.iterations for r in rs])),
