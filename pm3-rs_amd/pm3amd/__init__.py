@@ -305,6 +305,15 @@ class Molecule:
             pos.append([float(v) * ANGSTROM_TO_BOHR for v in parts[1:4]])
         return cls(np.array(z), np.array(pos), charge)
 
+    def to_xyz_str(self, comment="pm3-rs optimized geometry; coordinates in Angstrom"):
+        """XYZ block in Angstrom, the layout of the reference's writer (src/cli.rs:753-790):
+        symbol padded to two columns, coordinates signed with ten decimals."""
+        names = {v: k for k, v in SYMBOLS.items()}
+        lines = [str(len(self.z)), comment]
+        for zz, p in zip(self.z.tolist(), (self.positions / ANGSTROM_TO_BOHR).tolist()):
+            lines.append(f"{names.get(int(zz), 'X'):2} {p[0]:+.10f} {p[1]:+.10f} {p[2]:+.10f}")
+        return "\n".join(lines) + "\n"
+
     def _c(self):
         m = _CMolecule(len(self.z), self.z.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                        self.positions.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(self.charge),
@@ -620,3 +629,24 @@ def optimize(molecule: Molecule, params: Optional[Pm3Parameters] = None, scf_opt
     _raise(native().pm3_optimize(ctypes.byref(cm), params._h, ctypes.byref(scf_options._c()), ctypes.byref(opt._c()),
                                  ctypes.byref(res), _dp(out)))
     return _opt_result(molecule, res, out)
+
+
+def _rust_debug_f64(x):
+    # the shortest round-trip digits (Python's repr), exponent written as Rust's Debug writes it
+    t = repr(float(x))
+    if "e" in t:
+        mant, ex = t.split("e")
+        t = mant + "e" + str(int(ex))
+    return t
+
+
+def energy_report(result: Pm3Result) -> str:
+    """The lines of the reference CLI's `energy` command (src/cli.rs:1358-1371), which its oracle
+    scripts parse (`^Total energy:`, `^MO energies \\(eV\\):\\s*\\[(.*)\\]`)."""
+    mo = [] if result.mo_energies is None else list(result.mo_energies)
+    return (f"Total energy:          {result.total_ev:.12f} eV\n"
+            f"Electronic energy:     {result.electronic_ev:.12f} eV\n"
+            f"Core-core energy:      {result.core_ev:.12f} eV\n"
+            f"Heat of formation:     {result.heat_of_formation_kcal:.8f} kcal/mol\n"
+            f"SCF iterations:        {result.iterations}\n"
+            f"MO energies (eV):       [{', '.join(_rust_debug_f64(v) for v in mo)}]\n")

@@ -59,3 +59,33 @@ def test_mixed_batch_and_conformers():
     for p in o:
         dd = np.linalg.norm(hv - p, axis=1)
         assert np.sum(dd < 1e-9) == 1 and np.sort(dd)[1] >= 2.6
+
+
+def test_xyz_writer_round_trips_and_keeps_the_reference_layout():
+    import pm3amd as G
+    from pm3amd import workloads as W
+    z, x = W.alkane(4)
+    m = G.Molecule(np.asarray(z, np.int32), W.to_bohr(x))
+    text = m.to_xyz_str()
+    lines = text.splitlines()
+    assert lines[0] == str(len(z)) and lines[1] == "pm3-rs optimized geometry; coordinates in Angstrom"
+    assert lines[2].startswith("C  ") and all(len(f.split(".")[1]) == 10 for f in lines[2].split()[1:])
+    back = G.Molecule.from_xyz_str(text)
+    assert np.array_equal(back.z, m.z)
+    assert np.abs(back.positions - m.positions).max() < 1e-9  # ten decimals of Angstrom
+
+
+def test_energy_report_reads_back_with_the_reference_scripts_patterns():
+    import re
+    import pm3amd as G
+    mo = np.array([-40.123456789012345, -1.5, 0.0, 3.25e-07, 12.0])
+    r = G.Pm3Result(None, mo, None, 2, -350.25, 123.5, -226.75, -57.123456789, np.zeros(3), np.zeros(3), 0.0,
+                    -1.5, 0.0, 11, True, 1, 0.0)
+    text = G.energy_report(r)
+    # tools/oracle/internal_consistency.py:43-45 and all_element_validation.py's MO pattern
+    e = float(re.search(r"^Total energy:\s*([-+0-9.eE]+)", text, re.M).group(1))
+    assert e == -226.75
+    body = re.search(r"^MO energies \(eV\):\s*\[(.*)\]\s*$", text, re.M).group(1)
+    assert [float(v) for v in body.split(",")] == mo.tolist()
+    assert "3.25e-7" in body  # Rust's Debug exponent form
+    assert re.search(r"^Heat of formation:\s*(-57\.12345679) kcal/mol$", text, re.M)
