@@ -91,6 +91,10 @@ def lib():
                                     ip]
         L.orc_gradient_batch.argtypes = [ctypes.c_int, ip, ip, dp, dp, ctypes.POINTER(Options), ctypes.c_int, dp,
                                          dp, ip, ip, dp]
+        L.orc_run_full.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.POINTER(Options),
+                                   ctypes.POINTER(Result), dp, dp]
+        L.orc_run_uhf.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(Options),
+                                  ctypes.POINTER(Result), dp, dp, dp]
         L.orc_num_threads.restype = ctypes.c_int
         fp = ctypes.POINTER(ctypes.c_float)
         up = ctypes.POINTER(ctypes.c_uint16)
@@ -199,6 +203,38 @@ def run(z, xyz_bohr, charge=0.0, opts=None, want_arrays=True):
     out["dipole"] = np.array(res.dipole[:])
     if want_arrays:
         out.update(density=P, mo_energies=eps, mo_coeff=C, charges=q)
+    return st, out
+
+
+def run_full(z, xyz_bohr, charge=0.0, opts=None):
+    """run with the whole stability ladder, its two unrestricted rungs included."""
+    z, xyz = _mol(z, xyz_bohr)
+    opts = opts or options()
+    n = nao(z)
+    res = Result()
+    P = np.zeros((n, n))
+    q = np.zeros(len(z))
+    st = lib().orc_run_full(len(z), _i(z), _d(xyz), float(charge), ctypes.byref(opts), ctypes.byref(res), _d(P), _d(q))
+    out = {f: getattr(res, f) for f, _ in Result._fields_}
+    out.update(density=P, charges=q)
+    return st, out
+
+
+def run_uhf(z, xyz_bohr, charge=0.0, multiplicity=1, opts=None):
+    """run_pm3 with a multiplicity (UHF for unequal spin counts); returns (status, dict with the
+    total density, spin density and charges)."""
+    z, xyz = _mol(z, xyz_bohr)
+    opts = opts or options()
+    n = nao(z)
+    res = Result()
+    P = np.zeros((n, n))
+    S = np.zeros((n, n))
+    q = np.zeros(len(z))
+    st = lib().orc_run_uhf(len(z), _i(z), _d(xyz), float(charge), int(multiplicity), ctypes.byref(opts),
+                           ctypes.byref(res), _d(P), _d(S), _d(q))
+    out = {f: getattr(res, f) for f, _ in Result._fields_}
+    out["dipole"] = np.array(res.dipole[:])
+    out.update(density=P, spin_density=S, charges=q)
     return st, out
 
 

@@ -18,8 +18,10 @@
 // water HoF and dipole (src/scf.rs:1899-1904), the O-H overlap (src/overlap.rs:688-705), the
 // closed-form (ss|ss) (src/integrals.rs:585-594), the packed-vs-four-index Fock identity
 // (src/fock.rs:510-602), and finite differences of this file's own SCF energy for the
-// gradient (src/gradient.rs:137-176).  Parity unpinned: exact SCF iteration counts and DIIS
-// trajectories, and the unrestricted stability rungs (see orc_run's header).
+// gradient (src/gradient.rs:137-176).  The UHF restatement (uhf_loop, orc_run_uhf) is pinned by
+// the 17 open-shell MOPAC rows of tests/data/mopac_oracle.tsv, and the unrestricted stability
+// rungs (orc_run_full) by tetrafluoroethene, whose ground state only they reach.  Parity
+// unpinned: exact SCF iteration counts and DIIS trajectories.
 
 #include <algorithm>
 #include <array>
@@ -798,7 +800,9 @@ static double oc_two_electron(int a, int b, int c, int d, const Elem& e) {  // s
   return 0.0;
 }
 
-static void build_fock(const Mol& m, const Core& c, const double* P, double* F) {
+// F = H_core + J(P) - K(Ps): build_fock_spin (src/fock.rs:285-332), Coulomb from the total
+// density P, exchange from the same-spin density Ps.
+static void build_fock_spin(const Mol& m, const Core& c, const double* P, const double* Ps, double* F) {
   int n = m.nao;
   std::memcpy(F, c.h.data(), sizeof(double) * (size_t)n * n);
   for (int ia = 0; ia < m.nat; ++ia) {
@@ -810,7 +814,7 @@ static void build_fock(const Mol& m, const Core& c, const double* P, double* F) 
         for (int la = 0; la < nn; ++la)
           for (int si = 0; si < nn; ++si) {
             acc += P[(size_t)(o + la) * n + o + si] * oc_two_electron(mu, nu, la, si, e);
-            acc -= (0.5 * P[(size_t)(o + la) * n + o + si]) * oc_two_electron(mu, la, nu, si, e);
+            acc -= Ps[(size_t)(o + la) * n + o + si] * oc_two_electron(mu, la, nu, si, e);
           }
         F[(size_t)(o + mu) * n + o + nu] += acc;
       }
@@ -838,7 +842,7 @@ static void build_fock(const Mol& m, const Core& c, const double* P, double* F) 
         const double* row = &t.w[pack(mu, nu) * pb];
         for (int la = 0; la < nb; ++la) {
           double acc = 0.0;
-          for (int si = 0; si < nb; ++si) acc += (0.5 * P[(size_t)(oa + nu) * n + ob + si]) * row[pack(la, si)];
+          for (int si = 0; si < nb; ++si) acc += Ps[(size_t)(oa + nu) * n + ob + si] * row[pack(la, si)];
           kab[mu][la] -= acc;
         }
       }
@@ -853,6 +857,13 @@ static void build_fock(const Mol& m, const Core& c, const double* P, double* F) 
         F[(size_t)(ob + la) * n + oa + mu] = v;
       }
   }
+}
+
+// RHF Fock, build_fock (src/fock.rs:488-500): the spin build with Ps = P / 2.
+static void build_fock(const Mol& m, const Core& c, const double* P, double* F) {
+  std::vector<double> half((size_t)m.nao * m.nao);
+  for (size_t k = 0; k < half.size(); ++k) half[k] = 0.5 * P[k];
+  build_fock_spin(m, c, P, half.data(), F);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1293,6 +1304,160 @@ static void rhf_loop(const Mol& m, const Core& core, int nocc, const Options& o,
   st.iterations += pre;
 }
 
+struct UState {
+  std::vector<double> Pa, Pb, ea, eb, Ca;
+  double e_elec = 0, last_change = INFINITY;
+  int iterations = 0;
+  bool converged = false;
+};
+
+// The starting (P_a, P_b) pair, uhf_guess (src/scf.rs:937-980): core-Hamiltonian orbitals filled
+// to the two depths, or the superposition (or symmetry-broken, src/scf.rs:894-916) density scaled
+// by n_a / n and n_b / n.
+static void uhf_guess(const Mol& m, const Core& core, int na, int nb, int guess, std::vector<double>& Pa,
+                      std::vector<double>& Pb) {
+  int n = m.nao;
+  Pa.assign((size_t)n * n, 0.0);
+  Pb.assign((size_t)n * n, 0.0);
+  if (guess == 2) {
+    std::vector<double> w(n), C((size_t)n * n);
+    sym_eigen(n, core.h.data(), w.data(), C.data());
+    density_from_coeff(n, C.data(), na, 1.0, Pa.data());
+    density_from_coeff(n, C.data(), nb, 1.0, Pb.data());
+    return;
+  }
+  std::vector<double> P;
+  sad_density(m, P);
+  if (guess == 3) {
+    int target = -1;
+    double best = 0.0;
+    for (int i = 0; i < m.nat; ++i)
+      if (m.norb[i] >= 4 && m.el[i]->iop > best) { best = m.el[i]->iop; target = i; }
+    if (target >= 0) {
+      int off = m.off[target];
+      double tilt = std::min(0.2, P[(size_t)(off + 3) * n + off + 3]);
+      P[(size_t)(off + 3) * n + off + 3] -= tilt;
+      P[(size_t)(off + 1) * n + off + 1] += tilt;
+    }
+  }
+  const double ne = (double)(na + nb);
+  const double fa = ne > 0.0 ? na / ne : 0.5, fb = ne > 0.0 ? nb / ne : 0.5;
+  for (size_t k = 0; k < P.size(); ++k) {
+    Pa[k] = P[k] * fa;
+    Pb[k] = P[k] * fb;
+  }
+}
+
+// E = F P - P F for symmetric F and P (src/scf.rs:1039-1054), into E[off..off + n^2).
+static void commutator_into(int n, const double* F, const double* P, double* E) {
+  std::vector<double> FP((size_t)n * n);
+  matmul(n, F, P, FP.data());
+  for (int i = 0; i < n; ++i) {
+    E[(size_t)i * n + i] = 0.0;
+    for (int j = 0; j < i; ++j) {
+      double up = FP[(size_t)j * n + i], lo = FP[(size_t)i * n + j];
+      E[(size_t)i * n + j] = lo - up;
+      E[(size_t)j * n + i] = up - lo;
+    }
+  }
+}
+
+// uhf_loop (src/scf.rs:1319-1496) for s/p elements, so without the d-block penalty: spin Fock
+// matrices from (P_a + P_b, P_s), paired CDIIS on the stacked [E_a; E_b] error only when CDIIS is
+// the accelerator, level shift on each spin's virtual space, aufbau densities of weight 1, the
+// sum of the two RMS density changes, then damping; the energy is
+// (P.H + P_a.F_a + P_b.F_b) / 2.
+static void uhf_loop(const Mol& m, const Core& core, int na, int nb, const Options& o, int guess, int accel,
+                     double shift, double damping, UState& st) {
+  const int n = m.nao;
+  const size_t nn = (size_t)n * n;
+  std::vector<double> Pa, Pb;
+  uhf_guess(m, core, na, nb, guess, Pa, Pb);
+  const bool use_cdiis = o.use_diis && accel == 1;
+  const size_t max_diis = (size_t)diis_depth(n, 4, o.scf_memory_mb);
+  std::vector<std::vector<double>> hfa, hfb, herr;
+  std::vector<double> Pt(nn), Fa(nn), Fb(nn), ea(n), eb(n), Ca(nn), Cb(nn), Pan(nn), Pbn(nn);
+  double e_old = 0.0;
+  st.converged = false;
+  st.last_change = INFINITY;
+  for (int it = 0; it < o.max_scf; ++it) {
+    st.iterations = it + 1;
+    for (size_t k = 0; k < nn; ++k) Pt[k] = Pa[k] + Pb[k];
+    build_fock_spin(m, core, Pt.data(), Pa.data(), Fa.data());
+    build_fock_spin(m, core, Pt.data(), Pb.data(), Fb.data());
+    const double e_elec = 0.5 * (dot(Pt, core.h) + dot(Pa, Fa) + dot(Pb, Fb));
+    std::vector<double> fa = Fa, fb = Fb;
+    if (use_cdiis) {
+      std::vector<double> err(2 * nn);
+      commutator_into(n, Fa.data(), Pa.data(), err.data());
+      commutator_into(n, Fb.data(), Pb.data(), err.data() + nn);
+      hfa.push_back(Fa);
+      hfb.push_back(Fb);
+      herr.push_back(std::move(err));
+      if (hfa.size() > max_diis) {
+        hfa.erase(hfa.begin());
+        hfb.erase(hfb.begin());
+        herr.erase(herr.begin());
+      }
+      std::vector<std::vector<double>> gram(herr.size(), std::vector<double>(herr.size()));
+      for (size_t i = 0; i < herr.size(); ++i)
+        for (size_t j = 0; j < herr.size(); ++j) gram[i][j] = dot(herr[i], herr[j]);
+      std::vector<double> c;
+      if (diis_coeffs(gram, c)) {
+        for (size_t k = 0; k < nn; ++k) {
+          double sa = 0.0, sb = 0.0;
+          for (size_t i = 0; i < c.size(); ++i) {
+            sa += c[i] * hfa[i][k];
+            sb += c[i] * hfb[i][k];
+          }
+          fa[k] = sa;
+          fb[k] = sb;
+        }
+      }
+    }
+    if (shift != 0.0)  // level_shift (src/scf.rs:337-350)
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+          const double id = i == j ? 1.0 : 0.0;
+          fa[(size_t)i * n + j] += shift * (id - Pa[(size_t)i * n + j]);
+          fb[(size_t)i * n + j] += shift * (id - Pb[(size_t)i * n + j]);
+        }
+    sym_eigen(n, fa.data(), ea.data(), Ca.data());
+    sym_eigen(n, fb.data(), eb.data(), Cb.data());
+    density_from_coeff(n, Ca.data(), na, 1.0, Pan.data());
+    density_from_coeff(n, Cb.data(), nb, 1.0, Pbn.data());
+    double sa = 0.0, sb = 0.0;
+    for (size_t k = 0; k < nn; ++k) {
+      sa += (Pan[k] - Pa[k]) * (Pan[k] - Pa[k]);
+      sb += (Pbn[k] - Pb[k]) * (Pbn[k] - Pb[k]);
+    }
+    const double dp = std::sqrt(sa / (double)std::max<size_t>(nn, 1)) + std::sqrt(sb / (double)std::max<size_t>(nn, 1));
+    st.last_change = dp;
+    if (damping > 0.0)
+      for (size_t k = 0; k < nn; ++k) {
+        Pan[k] = (1.0 - damping) * Pan[k] + damping * Pa[k];
+        Pbn[k] = (1.0 - damping) * Pbn[k] + damping * Pb[k];
+      }
+    const double de = std::fabs(e_elec - e_old);
+    st.ea = ea;
+    st.eb = eb;
+    st.Ca = Ca;
+    Pa.swap(Pan);
+    Pb.swap(Pbn);
+    e_old = e_elec;
+    if (it > 0 && de < o.e_tol && dp < o.p_tol) {
+      st.converged = true;
+      break;
+    }
+  }
+  for (size_t k = 0; k < nn; ++k) Pt[k] = Pa[k] + Pb[k];
+  build_fock_spin(m, core, Pt.data(), Pa.data(), Fa.data());
+  build_fock_spin(m, core, Pt.data(), Pb.data(), Fb.data());
+  st.e_elec = 0.5 * (dot(Pt, core.h) + dot(Pa, Fa) + dot(Pb, Fb));
+  st.Pa = Pa;
+  st.Pb = Pb;
+}
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------------------
@@ -1439,11 +1604,14 @@ int orc_eigen(int n, const double* A, double* w, double* V) {
 }
 
 // One RHF PM3 single point: src/scf.rs:450-848 (run_pm3).  The stability search runs the five
-// restricted rungs of src/scf.rs:641-673; the two unrestricted rungs (:674-689) are counted in
-// scf_paths_tried but not run (this restatement has no UHF loop).  Open shells return
-// E_UNSUPPORTED.  Optional outputs may be null.
+// restricted rungs of src/scf.rs:641-673.  With urungs it also runs the two unrestricted rungs
+// (:662-673, :694-708): the closed shell through uhf_loop, kept only when it stays spin-pure
+// (max |P_a - P_b| < 1e-8) and then read as the restricted state (P = P_a + P_b, the alpha
+// orbitals); without, they are counted in scf_paths_tried but not run, as in the batched engine.
+// Open shells return E_UNSUPPORTED.  Optional outputs may be null.
 static int run_impl(int nat, const int* z, const double* xyz, double charge, const Options* opt, const double* P0,
-                    Result* res, double* density, double* mo_energies, double* mo_coeff, double* charges) {
+                    Result* res, double* density, double* mo_energies, double* mo_coeff, double* charges,
+                    bool urungs = false) {
   Mol m;
   int st = make_mol(nat, z, xyz, m);
   if (st) return st;
@@ -1479,7 +1647,30 @@ static int run_impl(int nat, const int* z, const double* xyz, double charge, con
       if (!cand.converged) continue;
       if (!best.converged || cand.e_elec < best.e_elec - 1.0e-6) best = cand;
     }
-    paths += 2;
+    if (!urungs) {
+      paths += 2;
+    } else {
+      const struct { int guess, accel; double shift, damp; } ur[2] = {{3, 0, -2.0, 0.0}, {primary, 1, 0.0, 0.85}};
+      for (const auto& rg : ur) {
+        ++paths;
+        UState u;
+        uhf_loop(m, core, nocc, nocc, o, rg.guess, o.use_diis && rg.accel != 0 ? rg.accel : 0, rg.shift, rg.damp, u);
+        if (!u.converged) continue;
+        double spin = 0.0;
+        for (size_t k = 0; k < u.Pa.size(); ++k) spin = std::max(spin, std::fabs(u.Pa[k] - u.Pb[k]));
+        if (!(spin < 1.0e-8)) continue;
+        State cand;
+        cand.P.resize(u.Pa.size());
+        for (size_t k = 0; k < u.Pa.size(); ++k) cand.P[k] = u.Pa[k] + u.Pb[k];
+        cand.eps = u.ea;
+        cand.C = u.Ca;
+        cand.e_elec = u.e_elec;
+        cand.last_change = u.last_change;
+        cand.iterations = u.iterations;
+        cand.converged = true;
+        if (!best.converged || cand.e_elec < best.e_elec - 1.0e-6) best = cand;
+      }
+    }
   }
   double core_ev = orc_core_core(nat, z, xyz);
   double total = best.e_elec + core_ev;
@@ -1536,6 +1727,141 @@ static int run_impl(int nat, const int* z, const double* xyz, double charge, con
       charges[i] = m.el[i]->tore - pop;
     }
   return best.converged ? OK : E_NOT_CONVERGED;
+}
+
+// Dipole about the centre of mass (src/dipole.rs:64-140) and Mulliken charges from a total
+// density P, as the RHF path computes them.
+static void density_properties(const Mol& m, int nat, const double* xyz, const std::vector<double>& P, Result& r,
+                               double* charges) {
+  const int n = m.nao;
+  double tm = 0.0, com[3] = {0, 0, 0};
+  for (int i = 0; i < nat; ++i) {
+    tm += m.el[i]->mass;
+    for (int k = 0; k < 3; ++k) com[k] += xyz[3 * i + k] * m.el[i]->mass;
+  }
+  for (int k = 0; k < 3; ++k) com[k] = tm > 0 ? com[k] / tm : 0.0;
+  double dip[3];
+  for (int ax = 0; ax < 3; ++ax) {
+    double nuc = 0.0, elec = 0.0;
+    for (int i = 0; i < nat; ++i) {
+      double disp = xyz[3 * i + ax] - com[ax];
+      nuc += disp * m.el[i]->tore;
+      int o2 = m.off[i];
+      for (int mu = 0; mu < m.norb[i]; ++mu) elec += P[(size_t)(o2 + mu) * n + o2 + mu] * disp;
+      if (m.norb[i] >= 4) {
+        int p = o2 + ax + 1;
+        elec += (P[(size_t)o2 * n + p] + P[(size_t)p * n + o2]) * m.el[i]->dd;
+      }
+    }
+    dip[ax] = (nuc - elec) * AU_DIPOLE_TO_DEBYE;
+  }
+  for (int k = 0; k < 3; ++k) r.dipole[k] = dip[k];
+  r.dipole_debye = std::sqrt(dip[0] * dip[0] + dip[1] * dip[1] + dip[2] * dip[2]);
+  if (charges)
+    for (int i = 0; i < nat; ++i) {
+      double pop = 0.0;
+      for (int mu = 0; mu < m.norb[i]; ++mu) pop += P[(size_t)(m.off[i] + mu) * n + m.off[i] + mu];
+      charges[i] = m.el[i]->tore - pop;
+    }
+}
+
+// run_pm3 (src/scf.rs:450-848) with a multiplicity: n_a = (N + M - 1) / 2, n_b = (N - M + 1) / 2
+// (:486-506).  Equal counts take the RHF path (orc_run); otherwise uhf_loop, then the stability
+// search when the frontier gap over both spin channels (frontier_gap, :106-124) is below 6 eV:
+// for an open shell all seven rungs of :647-673 run through uhf_loop (the five guess x
+// accelerator rungs, then symmetry-broken with a -2 eV shift, then CDIIS with damping 0.85), and
+// the lowest converged electronic energy is kept (1e-6 eV).  HOMO = max over the channels, LUMO =
+// min (:797-820); density and charges from P_a + P_b; spin density P_a - P_b.
+int orc_run_uhf(int nat, const int* z, const double* xyz, double charge, int multiplicity, const Options* opt,
+                Result* res, double* density, double* spin_density, double* charges) {
+  if (multiplicity < 1) return E_INVALID;
+  Mol m;
+  int st = make_mol(nat, z, xyz, m);
+  if (st) return st;
+  double n_elec = 0.0;
+  for (int i = 0; i < nat; ++i) n_elec += m.el[i]->tore;
+  n_elec -= charge;
+  const long ne = std::lround(n_elec);
+  if (std::fabs(n_elec - (double)ne) > 1.0e-6 || ne < 0) return E_INVALID;
+  const long nu = multiplicity - 1;
+  if (ne - nu < 0 || (ne - nu) % 2 != 0) return E_INVALID;
+  const int na = (int)((ne + nu) / 2), nb = (int)((ne - nu) / 2);
+  const int n = m.nao;
+  if (na == nb) {
+    const int s2 = run_impl(nat, z, xyz, charge, opt, nullptr, res, density, nullptr, nullptr, charges);
+    if (spin_density) std::fill(spin_density, spin_density + (size_t)n * n, 0.0);
+    return s2;
+  }
+  Core core;
+  if ((st = build_core(m, core))) return st;
+  const Options o = *opt;
+  auto acc_of = [&](int a) { return o.use_diis && a != 0 ? a : 0; };
+  UState best;
+  uhf_loop(m, core, na, nb, o, o.guess, acc_of(o.accelerator), 0.0, 0.0, best);
+  auto frontier = [&](const UState& u, double& h, double& l, bool& hh, bool& hl) {
+    hh = na >= 1 || nb >= 1;
+    hl = na < n || nb < n;
+    h = -INFINITY;
+    l = INFINITY;
+    if (na >= 1) h = std::max(h, u.ea[na - 1]);
+    if (nb >= 1) h = std::max(h, u.eb[nb - 1]);
+    if (na < n) l = std::min(l, u.ea[na]);
+    if (nb < n) l = std::min(l, u.eb[nb]);
+  };
+  double h, l;
+  bool hh, hl;
+  frontier(best, h, l, hh, hl);
+  const double gap = hh && hl ? l - h : INFINITY;
+  int paths = 1;
+  const double first_ev = best.e_elec;
+  if (o.stability == 2 || (o.stability == 1 && gap < 6.0)) {
+    const int primary = o.guess == 2 ? 2 : 1, other = primary == 2 ? 1 : 2;
+    const struct { int guess, accel; double shift, damp; } rungs[7] = {
+        {primary, 0, 0.0, 0.0}, {other, o.accelerator, 0.0, 0.0}, {other, 0, 0.0, 0.0}, {3, o.accelerator, 0.0, 0.0},
+        {3, 0, 0.0, 0.0},       {3, 0, -2.0, 0.0},                  {primary, 1, 0.0, 0.85}};
+    for (const auto& rg : rungs) {
+      ++paths;
+      UState cand;
+      uhf_loop(m, core, na, nb, o, rg.guess, acc_of(rg.accel), rg.shift, rg.damp, cand);
+      if (!cand.converged) continue;
+      if (!best.converged || cand.e_elec < best.e_elec - 1.0e-6) best = cand;
+    }
+  }
+  std::vector<double> Pt((size_t)n * n);
+  for (size_t k = 0; k < Pt.size(); ++k) Pt[k] = best.Pa[k] + best.Pb[k];
+  const double core_ev = orc_core_core(nat, z, xyz);
+  const double total = best.e_elec + core_ev;
+  double eis = 0.0, ehs = 0.0;
+  for (int i = 0; i < nat; ++i) { eis += m.el[i]->e_isol; ehs += m.el[i]->eheat_ev; }
+  Result r{};
+  r.total_ev = total;
+  r.electronic_ev = best.e_elec;
+  r.core_ev = core_ev;
+  r.hof_kcal = (total - eis + ehs) * EV_TO_KCAL;
+  r.iterations = best.iterations;
+  r.converged = best.converged;
+  r.last_change = best.last_change;
+  r.scf_paths_tried = paths;
+  r.scf_improvement_ev = std::max(first_ev - best.e_elec, 0.0);
+  r.n_occ = na;
+  r.nao = n;
+  frontier(best, h, l, hh, hl);
+  r.has_homo = hh;
+  r.has_lumo = hl;
+  r.homo_ev = hh ? h : 0.0;
+  r.lumo_ev = hl ? l : 0.0;
+  density_properties(m, nat, xyz, Pt, r, charges);
+  if (res) *res = r;
+  if (density) std::memcpy(density, Pt.data(), sizeof(double) * Pt.size());
+  if (spin_density)
+    for (size_t k = 0; k < Pt.size(); ++k) spin_density[k] = best.Pa[k] - best.Pb[k];
+  return best.converged ? OK : E_NOT_CONVERGED;
+}
+
+// orc_run with the whole stability ladder, unrestricted rungs included.
+int orc_run_full(int nat, const int* z, const double* xyz, double charge, const Options* opt, Result* res,
+                 double* density, double* charges) {
+  return run_impl(nat, z, xyz, charge, opt, nullptr, res, density, nullptr, nullptr, charges, true);
 }
 
 int orc_run(int nat, const int* z, const double* xyz, double charge, const Options* opt, Result* res, double* density,

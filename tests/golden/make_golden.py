@@ -58,21 +58,28 @@ def main():
         atoms, charge = hydride(sym, z, tore[z])
         hyd.append({"name": name, "atoms": atoms, "charge": charge, "energy_ev": ic[name]["energy_ev"],
                     "hof_kcal": ae[name]["energy_pm3_rs_kcal"] if name in ae else None})
-    mopac = []
+    mopac, open_shell = [], []
     for line in open(os.path.join(REF, "tests/data/mopac_oracle.tsv")):
         if line.startswith("#") or line.startswith("name\t") or not line.strip():
             continue
         f = line.rstrip("\n").split("\t")
         atoms = [a.split() for a in f[9].split(";")]
-        if int(f[3]) != 1 or any(a[0] not in SUPPORTED for a in atoms):
+        if any(a[0] not in SUPPORTED for a in atoms):
+            continue
+        row = {"name": f[0], "formula": f[1], "charge": float(f[2]), "multiplicity": int(f[3]),
+               "hof_kcal": float(f[4]), "dipole_debye": float(f[5]) if f[5] else None,
+               "charges": [float(c) for c in f[7].split(";")],
+               "atoms": [[a[0]] + [float(v) for v in a[1:4]] for a in atoms]}
+        if int(f[3]) != 1:
+            open_shell.append(row)
             continue
         mopac.append({"name": f[0], "formula": f[1], "charge": float(f[2]), "hof_kcal": float(f[4]),
                       "dipole_debye": float(f[5]) if f[5] else None, "ip_ev": float(f[6]) if f[6] else None,
                       "charges": [float(c) for c in f[7].split(";")],
                       "atoms": [[a[0]] + [float(v) for v in a[1:4]] for a in atoms]})
     json.dump({"source": "reference tools/oracle/*.json and tests/data/mopac_oracle.tsv",
-               "hydrides": hyd, "mopac_closed_shell": mopac}, open(OUT, "w"), indent=1)
-    print(f"wrote {OUT}: {len(hyd)} hydrides, {len(mopac)} MOPAC rows")
+               "hydrides": hyd, "mopac_closed_shell": mopac, "mopac_open_shell": open_shell}, open(OUT, "w"), indent=1)
+    print(f"wrote {OUT}: {len(hyd)} hydrides, {len(mopac)} MOPAC rows, {len(open_shell)} open-shell rows")
 
 
 if __name__ == "__main__":
