@@ -120,7 +120,8 @@ pm3_status pm3_run_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* 
 /* Device-resident batch: pm3_batch_create uploads the molecules and allocates the workgroup
  * scratch once; pm3_batch_solve is the persistent launch alone (inputs already in HBM);
  * pm3_batch_results copies back per-molecule status, scalars and optionally the Mulliken charges
- * (charges[atom_ptr[k] + i]) and densities (concatenated nao_k^2 blocks).  No reference
+ * (charges[atom_ptr[k] + i]) and densities (concatenated nao_k^2 blocks; only for a batch
+ * created with PM3_BATCH_WANT_DENSITY, else PM3_INVALID_INPUT).  No reference
  * counterpart: this is the batched entry the reference lacks; pm3_run_batch is
  * create + solve + results + destroy. */
 typedef struct pm3_batch pm3_batch;

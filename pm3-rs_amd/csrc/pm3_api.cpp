@@ -37,6 +37,7 @@ struct pm3_batch {
   const pm3_params* params = nullptr;
   std::vector<MolResult> results;
   bool fetched = false;
+  bool want_density = false;  // created with PM3_BATCH_WANT_DENSITY
 };
 
 namespace {
@@ -218,6 +219,7 @@ pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_
   req.want_density = (flags & PM3_BATCH_WANT_DENSITY) != 0;
   req.want_gradient = (flags & PM3_BATCH_WANT_GRADIENT) != 0;
   req.force_large = (flags & PM3_BATCH_FORCE_LARGE) != 0;
+  h->want_density = req.want_density;
   std::string err;
   if (!batch_create(params->p, h->mols, eo, req, h->b, err)) {
     batch_destroy(h->b);
@@ -240,6 +242,7 @@ pm3_status pm3_batch_solve(pm3_batch* h) {
 pm3_status pm3_batch_results(pm3_batch* h, pm3_status* status, pm3_result* results, double* charges, double* density) {
   g_last_error.clear();
   if (!h) return fail(PM3_INVALID_INPUT, "null batch");
+  if (density && !h->want_density) return fail(PM3_INVALID_INPUT, "densities need PM3_BATCH_WANT_DENSITY at creation");
   std::string err;
   if (!h->fetched) {
     if (!batch_fetch(h->b, h->params->p, h->mols, h->results, err)) return fail(PM3_DEVICE_ERROR, err);

@@ -3495,6 +3495,7 @@ struct Batch {
   std::vector<std::vector<int>> ao_map, atom_map;  // sorted -> caller indices
   std::vector<int> pre_status;
   bool want_gradient = false;
+  bool keep_density = false;  // the caller asked for densities: copy them back at each fetch
   bool purify = false;  // densities by purification; orbitals are then not produced
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
@@ -4524,6 +4525,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
     HIP_OK(hipMemcpy(d_gpart, gpart.data(), sizeof(int) * gpart.size(), hipMemcpyHostToDevice));
   }
   b.want_gradient = req.want_gradient;
+  b.keep_density = req.want_density;
   HIP_OK(hipMemset(d_store, 0, sizeof(double) * std::max<long long>(store_total, 1)));
   HIP_OK(hipMemcpy(d_pbase, pair_base.data(), sizeof(long long) * pair_base.size(), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_desc, b.desc.data(), sizeof(MolDesc) * nmol, hipMemcpyHostToDevice));
@@ -4753,8 +4755,11 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
   HIP_OK(hipMemcpy(hq.data(), b.args.out_q, sizeof(double) * b.atoms_total, hipMemcpyDeviceToHost));
   std::vector<double> hP, hC, hE;
   if (b.mat_total > 0) {
-    hP.resize(b.mat_total);
-    HIP_OK(hipMemcpy(hP.data(), b.args.out_P, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
+    // densities stay on the device unless asked for (a gradient batch keeps its own there)
+    if (b.keep_density) {
+      hP.resize(b.mat_total);
+      HIP_OK(hipMemcpy(hP.data(), b.args.out_P, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
+    }
     if (!b.purify) {
       hC.resize(b.mat_total);
       hE.resize(b.vec_total);
@@ -4797,10 +4802,12 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
     if (d.out_mat >= 0) {
       const int n = d.nao;
       const std::vector<int>& mp = b.ao_map[k];
-      r.density.assign((size_t)n * n, 0.0);
-      const double* Ps = hP.data() + d.out_mat;
-      for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) r.density[(size_t)mp[i] * n + mp[j]] = Ps[(size_t)i * n + j];
+      if (b.keep_density) {
+        r.density.assign((size_t)n * n, 0.0);
+        const double* Ps = hP.data() + d.out_mat;
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j < n; ++j) r.density[(size_t)mp[i] * n + mp[j]] = Ps[(size_t)i * n + j];
+      }
       if (!b.purify) {
         r.mo_coeff.assign((size_t)n * n, 0.0);
         r.mo_energies.assign(hE.begin() + d.out_vec, hE.begin() + d.out_vec + n);
