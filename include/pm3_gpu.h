@@ -128,7 +128,12 @@ typedef struct pm3_batch pm3_batch;
 /* PM3_BATCH_FORCE_LARGE solves every molecule on the multi-kernel path that molecules too large
  * for one workgroup (more than about 190 AOs) always take: a host-driven SCF whose steps are
  * grid-wide kernels, rocBLAS products and a rocSOLVER eigensolve. */
-enum { PM3_BATCH_WANT_DENSITY = 1, PM3_BATCH_WANT_GRADIENT = 2, PM3_BATCH_FORCE_LARGE = 4 };
+/* PM3_BATCH_WARM_START: every solve after the first starts each molecule's first SCF path from
+ * that molecule's converged density of the previous solve, kept in device memory, in place of
+ * the options' guess (the stability rungs keep theirs).  Meant for a batch whose geometries move
+ * a little between solves (pm3_batch_set_positions); the fixed point is the same, the SCF path
+ * shorter.  Molecules on the multi-kernel path keep the options' guess. */
+enum { PM3_BATCH_WANT_DENSITY = 1, PM3_BATCH_WANT_GRADIENT = 2, PM3_BATCH_FORCE_LARGE = 4, PM3_BATCH_WARM_START = 8 };
 pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
                             const double* charge, const pm3_params* params, const pm3_options* opts, int32_t flags,
                             pm3_batch** out);
@@ -215,12 +220,15 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
 const char* pm3_last_error_message(void);
 
 /* OptOptions (src/optimizer.rs:15-35).  grad_step is the reference's finite-difference step; the
- * gradient here is analytic, so it is accepted and not used. */
+ * gradient here is analytic, so it is accepted and not used.  warm_start has no reference
+ * counterpart: 1 starts each round's SCF from the molecule's density of its previous point
+ * (PM3_BATCH_WARM_START) instead of the options' guess, as the reference does every time. */
 typedef struct {
-  int32_t max_iter; /* default 200 */
-  double gtol;      /* default 1e-3 eV/Bohr, on the largest gradient component */
-  double grad_step; /* default 5e-4 Bohr */
-  int32_t history;  /* default 8 L-BFGS pairs */
+  int32_t max_iter;   /* default 200 */
+  double gtol;        /* default 1e-3 eV/Bohr, on the largest gradient component */
+  double grad_step;   /* default 5e-4 Bohr */
+  int32_t history;    /* default 8 L-BFGS pairs */
+  int32_t warm_start; /* default 0 */
 } pm3_opt_options;
 void pm3_opt_options_default(pm3_opt_options* out);
 

@@ -219,6 +219,7 @@ pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_
   req.want_density = (flags & PM3_BATCH_WANT_DENSITY) != 0;
   req.want_gradient = (flags & PM3_BATCH_WANT_GRADIENT) != 0;
   req.force_large = (flags & PM3_BATCH_FORCE_LARGE) != 0;
+  req.warm_start = (flags & PM3_BATCH_WARM_START) != 0;
   h->want_density = req.want_density;
   std::string err;
   if (!batch_create(params->p, h->mols, eo, req, h->b, err)) {

@@ -108,6 +108,7 @@ struct KArgs {
   double eig_rel;
   double eig_skip;
   int purify;                  // 1: densities by purification, frontier levels by Lanczos (no orbitals)
+  int warm;                    // 1: each molecule's first SCF path starts from its density in out_P
   int sp2_lds;                 // purification operand placements allowed: 0 HBM only, 1 + LDS pair, 2 + LDS X,
                                // 3 + LDS X overwritten in place, 4 + leading rows of X in LDS
   int sp2_tight_n;             // smallest molecule (AOs) whose purification uses Lanczos spectral bounds
@@ -558,6 +559,7 @@ struct Mol {
   int sp2_tight_n;  // KArgs::sp2_tight_n
   int sp2_tile16_n;  // KArgs::sp2_tile16_n
   double sp2_margin;  // KArgs::sp2_margin
+  const double* prev;  // this molecule's density of the batch's last solve (n x n, engine order), or null
 };
 
 // ------------------------------------------------------------------------------------------
@@ -2322,10 +2324,17 @@ __device__ double commutator2(const KArgs& A, const Mol& M, const Bufs& B, doubl
 // SCF, so that only idempotent densities enter the DIIS history.  Returns the guess used:
 // 0 superposition (or not learned), 1 learned (one pre-iteration done), 2 learned rejected.
 __device__ __noinline__ int initial_guess(const KArgs& A, const Mol& M, const Bufs& B, int guess, double* lds,
-                                          double* red, bool tight) {
+                                          double* red, bool tight, bool warm = false) {
   const int np = M.np, tid = threadIdx.x;
   for (int e = tid; e < np * np; e += NT) B.P[e] = 0.0;
   __syncthreads();
+  if (warm && M.prev) {
+    // the molecule's converged density of the previous solve (previous geometry)
+    const int n = M.n;
+    for (int e = tid; e < n * n; e += NT) B.P[(e / n) * np + e % n] = M.prev[e];
+    __syncthreads();
+    return 0;
+  }
   if (guess == 2) {
     if (A.purify && sp2_density(M, B, M.H, B.P, red, tight) >= 0) return 0;
     eigensolve(M, B, M.H, false, lds, red);
@@ -2394,14 +2403,14 @@ struct PathResult {
 // last extrapolated Fock) in B; with A.purify each density is the purified aufbau projector of
 // the extrapolated Fock matrix and only eps[nocc - 1], eps[nocc] (its frontier levels) are set.
 __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int guess, int accel, double* lds,
-                               double* red, Diis& D, bool tight = true) {
+                               double* red, Diis& D, bool tight = true, bool warm = false) {
   const int n = M.n, np = M.np, tid = threadIdx.x;
   if (tid == 0) {
     B.bnd[2] = -1.0;  // no spectral-bound estimate for this path yet
     B.bnd[3] = 0.0;   // nor a frontier seed
   }
   __syncthreads();
-  const int gused = initial_guess(A, M, B, guess, lds, red, tight);
+  const int gused = initial_guess(A, M, B, guess, lds, red, tight, warm);
   // DIIS depth from the memory budget (src/scf.rs:309-327, :1205-1228)
   const long long n2 = (long long)n * n * 8;
   const long long budget = (long long)A.scf_memory_mb * 1024 * 1024;
@@ -2679,6 +2688,7 @@ __device__ inline Bufs slab(const KArgs& A, double*& Wbase) {
 __host__ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double* Wbase) {
   Mol M;
   M.atom0 = md.atom0;
+  M.prev = A.warm && md.out_mat >= 0 ? A.out_P + md.out_mat : nullptr;
   M.nat = md.nat;
   M.nh = md.nheavy;
   M.nH = md.nat - md.nheavy;
@@ -2965,8 +2975,8 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
     zero_mats(M, B);
     const double ecore = finish_core(A, M, red);
     stamp(B, 0);
-    PathResult best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D);
-    if (aufbau_broken(A, M, B)) best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D, false);
+    PathResult best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D, true, true);
+    if (aufbau_broken(A, M, B)) best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D, false, true);
     const int guess_used = best.guess_used;
     const int nocc = M.nocc, n = M.n, np = M.np;
     const double first_e = best.e_elec;
@@ -3496,6 +3506,7 @@ struct Batch {
   std::vector<int> pre_status;
   bool want_gradient = false;
   bool keep_density = false;  // the caller asked for densities: copy them back at each fetch
+  bool warm_start = false;    // solves after the first start from each molecule's last density
   bool purify = false;  // densities by purification; orbitals are then not produced
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
@@ -4409,7 +4420,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
       const long long npairs = (long long)d.nat * (d.nat - 1) / 2;
       w_cap = std::max(w_cap, 20LL * npairs);  // Fock-build pair slots per workgroup
     }
-    if (req.want_density || req.want_orbitals || req.want_gradient) {
+    if (req.want_density || req.want_orbitals || req.want_gradient || req.warm_start) {
       d.out_mat = b.mat_total;
       d.out_vec = b.vec_total;
       b.mat_total += (long long)nao * nao;
@@ -4526,6 +4537,7 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   }
   b.want_gradient = req.want_gradient;
   b.keep_density = req.want_density;
+  b.warm_start = req.warm_start;
   HIP_OK(hipMemset(d_store, 0, sizeof(double) * std::max<long long>(store_total, 1)));
   HIP_OK(hipMemcpy(d_pbase, pair_base.data(), sizeof(long long) * pair_base.size(), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(d_desc, b.desc.data(), sizeof(MolDesc) * nmol, hipMemcpyHostToDevice));
@@ -4653,6 +4665,7 @@ bool batch_solve(Batch* bp, std::string& err) {
   if (b.learned && !guess_infer(b.guess_model, infer_view(b), err, b.evm)) return false;
   if (!b.learned) HIP_OK(hipEventRecord(b.evm, 0));
   HIP_OK(hipEventRecord(b.ev1, 0));
+  b.args.warm = b.warm_start && b.solved_once ? 1 : 0;
   if (b.nsolve > 0) hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
   HIP_OK(hipGetLastError());
   for (int k : b.large)

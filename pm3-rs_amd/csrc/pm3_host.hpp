@@ -60,6 +60,7 @@ struct RunRequest {
   bool want_orbitals = false;
   bool want_gradient = false;
   bool force_large = false;  // solve every molecule on the host-driven multi-kernel path
+  bool warm_start = false;   // solves after the first start from each molecule's last density
 };
 
 // Launch shapes and tile choices.  They change speed only: results agree to rounding whatever

@@ -10,7 +10,9 @@
 // trial is accepted, evaluates the same point again for the gradient; both are the same SCF,
 // so here the trial's solve carries the gradient and an accepted point costs no second solve.
 // When fewer than half of a batch's molecules are still running, the batch is rebuilt from the
-// running ones, so the last few slow optimisations do not pay for the finished ones.
+// running ones, so the last few slow optimisations do not pay for the finished ones.  With
+// warm_start each round's SCF starts from the molecule's density at its previous point (the
+// first round of a rebuilt batch from the options' guess).
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -124,6 +126,7 @@ void pm3_opt_options_default(pm3_opt_options* out) {
   out->gtol = 1.0e-3;
   out->grad_step = 5.0e-4;
   out->history = 8;
+  out->warm_start = 0;
 }
 
 pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
@@ -190,7 +193,9 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
         bq.push_back(charge ? charge[k] : 0.0);
       }
       const pm3_status s = pm3_batch_create((int32_t)members.size(), bptr.data(), bz.data(), bx.data(), bq.data(),
-                                            params, opts, PM3_BATCH_WANT_GRADIENT, &batch);
+                                            params, opts,
+                                            PM3_BATCH_WANT_GRADIENT | (opt.warm_start ? PM3_BATCH_WARM_START : 0),
+                                            &batch);
       if (s != PM3_OK) {
         fatal = s;
         break;

@@ -96,7 +96,7 @@ class _CResult(ctypes.Structure):
 
 class _COptOptions(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int32), ("gtol", ctypes.c_double), ("grad_step", ctypes.c_double),
-                ("history", ctypes.c_int32)]
+                ("history", ctypes.c_int32), ("warm_start", ctypes.c_int32)]
 
 
 class _COptResult(ctypes.Structure):
@@ -445,7 +445,8 @@ class Batch:
     """Device-resident batch of independent molecules (pm3_batch_* in include/pm3_gpu.h)."""
 
     def __init__(self, molecules: List[Molecule], params: Optional[Pm3Parameters] = None,
-                 options: Optional[Pm3Options] = None, want_density=False, want_gradient=False, force_large=False):
+                 options: Optional[Pm3Options] = None, want_density=False, want_gradient=False, force_large=False,
+                 warm_start=False):
         self.params = params or Pm3Parameters.standard()
         self.options = options or Pm3Options()
         self.molecules = molecules
@@ -461,7 +462,7 @@ class Batch:
         st = native().pm3_batch_create(len(molecules), ptr.ctypes.data_as(ip), self._z.ctypes.data_as(ip),
                                        _dp(self._xyz), _dp(self._q), self.params._h, ctypes.byref(self.options._c()),
                                        (1 if want_density else 0) | (2 if want_gradient else 0) |
-                                       (4 if force_large else 0), ctypes.byref(h))
+                                       (4 if force_large else 0) | (8 if warm_start else 0), ctypes.byref(h))
         _raise(st)
         self._h = h
         self.want_density = want_density
@@ -563,14 +564,17 @@ class Batch:
 
 @dataclasses.dataclass
 class OptOptions:
-    """OptOptions (src/optimizer.rs:15-35); grad_step is accepted and unused (analytic gradient)."""
+    """OptOptions (src/optimizer.rs:15-35); grad_step is accepted and unused (analytic gradient).
+    warm_start (no reference counterpart) starts each round's SCF from the previous density."""
     max_iter: int = 200
     gtol: float = 1.0e-3
     grad_step: float = 5.0e-4
     history: int = 8
+    warm_start: bool = False
 
     def _c(self):
-        return _COptOptions(int(self.max_iter), float(self.gtol), float(self.grad_step), int(self.history))
+        return _COptOptions(int(self.max_iter), float(self.gtol), float(self.grad_step), int(self.history),
+                            int(bool(self.warm_start)))
 
 
 @dataclasses.dataclass

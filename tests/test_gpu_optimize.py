@@ -127,3 +127,42 @@ def test_new_positions_equal_a_fresh_batch(G):
     s3, r3 = b.results()
     assert all(a.heat_of_formation_kcal == c.heat_of_formation_kcal for a, c in zip(r3, r2))
     b.close()
+
+
+def test_warm_start_keeps_the_answers_in_fewer_iterations(G):
+    """PM3_BATCH_WARM_START: after a small move, each molecule's SCF starts from its own last
+    density; the converged answers are a cold batch's, the SCF paths shorter."""
+    rng = np.random.default_rng(12)
+    base = [(np.asarray(z, np.int32), W.to_bohr(x)) for z, x in W.mixed_batch(16, 21)]
+    moved = [x + rng.normal(0.0, 0.02, x.shape) for _, x in base]
+    opts = G.Pm3Options(**TIGHT)
+    b = G.Batch([G.Molecule(z, x) for z, x in base], options=opts, want_gradient=True, warm_start=True)
+    b.solve()
+    b.set_positions(moved)
+    b.solve()
+    s1, r1 = b.results()
+    g1 = b.gradients()
+    b.close()
+    f = G.Batch([G.Molecule(z, x) for (z, _), x in zip(base, moved)], options=opts, want_gradient=True)
+    f.solve()
+    s2, r2 = f.results()
+    g2 = f.gradients()
+    f.close()
+    assert np.all(s1 == 0) and np.all(s2 == 0)
+    warm, cold = sum(r.iterations for r in r1), sum(r.iterations for r in r2)
+    print("iterations warm", warm, "cold", cold, flush=True)
+    assert warm < cold
+    for a, c, ga, gc in zip(r1, r2, g1, g2):
+        assert abs(a.heat_of_formation_kcal - c.heat_of_formation_kcal) <= 1e-7
+        assert np.abs(ga - gc).max() <= 1e-6
+
+
+def test_warm_optimisations_reach_the_oracles_minima(G, oracle):
+    opt = G.OptOptions(warm_start=True)
+    for name, z, x in _cases():
+        r = G.optimize(G.Molecule(z, x), scf_options=G.Pm3Options(**TIGHT), opt=opt)
+        o = oracle.optimize(z, x, 0.0, oracle.options(**TIGHT))
+        print(f"warm {name} iterations {r.iterations} / {o['iterations']} dHf {r.heat_of_formation_kcal - o['hof_kcal']:.1e}",
+              flush=True)
+        assert r.converged and r.max_gradient < 1e-3
+        assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= 1e-4, name

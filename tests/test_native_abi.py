@@ -138,7 +138,7 @@ def test_optimizer_defaults_and_host_checks_without_device():
     L = G.native()
     o = G._COptOptions()
     L.pm3_opt_options_default(ctypes.byref(o))
-    assert (o.max_iter, o.gtol, o.grad_step, o.history) == (200, 1.0e-3, 5.0e-4, 8)
+    assert (o.max_iter, o.gtol, o.grad_step, o.history, o.warm_start) == (200, 1.0e-3, 5.0e-4, 8, 0)
     ip, dp = ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)
     z = np.array([8, 1, 1], np.int32)
     xyz = np.zeros((3, 3))
@@ -155,7 +155,7 @@ def test_optimizer_defaults_and_host_checks_without_device():
     assert call(0, np.array([0], np.int32)) == 0  # nothing to do, no device work
     assert call(-1, np.array([0], np.int32)) == 1
     assert call(1, np.array([1, 3], np.int32)) == 1  # atom_ptr[0] != 0
-    bad = G._COptOptions(-1, 1e-3, 5e-4, 8)
+    bad = G._COptOptions(-1, 1e-3, 5e-4, 8, 0)
     assert call(1, np.array([0, 3], np.int32), opt=bad) == 1
     learned = G._COptions()
     L.pm3_options_default(ctypes.byref(learned))
