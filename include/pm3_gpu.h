@@ -130,7 +130,8 @@ typedef struct pm3_batch pm3_batch;
  * grid-wide kernels, rocBLAS products and a rocSOLVER eigensolve. */
 /* PM3_BATCH_WARM_START: every solve after the first starts each molecule's first SCF path from
  * that molecule's converged density of the previous solve, kept in device memory, in place of
- * the options' guess (the stability rungs keep theirs).  Meant for a batch whose geometries move
+ * the options' guess (the stability rungs keep theirs; a molecule whose previous solve did not
+ * converge starts from the guess).  Meant for a batch whose geometries move
  * a little between solves (pm3_batch_set_positions); the fixed point is the same, the SCF path
  * shorter.  Molecules on the multi-kernel path keep the options' guess. */
 enum { PM3_BATCH_WANT_DENSITY = 1, PM3_BATCH_WANT_GRADIENT = 2, PM3_BATCH_FORCE_LARGE = 4, PM3_BATCH_WARM_START = 8 };

@@ -559,7 +559,7 @@ struct Mol {
   int sp2_tight_n;  // KArgs::sp2_tight_n
   int sp2_tile16_n;  // KArgs::sp2_tile16_n
   double sp2_margin;  // KArgs::sp2_margin
-  const double* prev;  // this molecule's density of the batch's last solve (n x n, engine order), or null
+  const double* prev;  // this molecule's converged density of the batch's last solve (n x n, engine order), or null
 };
 
 // ------------------------------------------------------------------------------------------
@@ -2969,7 +2969,9 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
     if (qi >= A.nmol) break;
     const int mi = A.order[qi];
     const MolDesc md = A.mols[mi];
-    const Mol M = mol_of(A, md, Wbase);
+    Mol M = mol_of(A, md, Wbase);
+    // a warm start only from a density whose solve converged (A.out still holds that solve's)
+    if (M.prev && A.out[mi].converged == 0) M.prev = nullptr;
     // Matrices use the molecule's own leading dimension np inside the workgroup's slab; the
     // padding beyond nao stays zero for the whole solve.
     zero_mats(M, B);

@@ -166,3 +166,22 @@ def test_warm_optimisations_reach_the_oracles_minima(G, oracle):
               flush=True)
         assert r.converged and r.max_gradient < 1e-3
         assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= 1e-4, name
+
+
+def test_warm_start_skips_densities_of_unconverged_solves(G):
+    """A molecule whose last solve did not converge starts again from the options' guess: with
+    max_scf too small to converge, a warm batch at the same geometry gives a cold batch's answer
+    bit for bit, solve after solve."""
+    mols = [G.Molecule(np.asarray(z, np.int32), W.to_bohr(x)) for z, x in W.mixed_batch(6, 8)]
+    opts = G.Pm3Options(max_scf=3, p_tol=1e-10, e_tol=1e-12)
+    w = G.Batch(mols, options=opts, want_gradient=True, warm_start=True)
+    c = G.Batch(mols, options=opts, want_gradient=True)
+    for _ in range(2):
+        w.solve()
+        c.solve()
+        sw, rw = w.results()
+        sc, rc = c.results()
+        assert np.all(sw == 3) and np.all(sc == 3)  # PM3_SCF_NOT_CONVERGED
+        assert all(a.electronic_ev == b.electronic_ev for a, b in zip(rw, rc))
+    w.close()
+    c.close()
