@@ -149,6 +149,12 @@ void pm3_batch_destroy(pm3_batch* batch);
  * creation); the batch is unchanged on failure.  No reference counterpart: a geometry change in
  * the reference is a new run_pm3 call (src/optimizer.rs:344-347). */
 pm3_status pm3_batch_set_positions(pm3_batch* batch, const double* xyz_bohr);
+/* For a PM3_BATCH_WARM_START batch dst: take each molecule's last density and result record
+ * from the solved batch src, where src_index[k] is dst molecule k's index in src (the same atoms
+ * in the same order), so that dst's first solve is warm-started too.  PM3_INVALID_INPUT when a
+ * molecule is not its source's.  No reference counterpart: it lets a caller rebuild a smaller
+ * batch of moving molecules without losing their densities. */
+pm3_status pm3_batch_carry_densities(pm3_batch* dst, const pm3_batch* src, const int32_t* src_index);
 /* Analytic gradients of the last solve (batch created with PM3_BATCH_WANT_GRADIENT), eV/Bohr,
  * gradient[3 * (atom_ptr[k] + i) + c]; the same per-molecule gradient pm3_gradient returns. */
 pm3_status pm3_batch_gradient(pm3_batch* batch, double* gradient);

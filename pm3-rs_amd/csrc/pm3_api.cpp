@@ -340,6 +340,21 @@ pm3_status pm3_batch_set_positions(pm3_batch* h, const double* xyz_bohr) {
   return PM3_OK;
 }
 
+pm3_status pm3_batch_carry_densities(pm3_batch* dst, const pm3_batch* src, const int32_t* src_index) {
+  g_last_error.clear();
+  if (!dst || !dst->b || !src || !src->b || (!src_index && !dst->mols.empty())) return fail(PM3_INVALID_INPUT, "null argument");
+  std::vector<int> idx(src_index, src_index + dst->mols.size());
+  for (size_t k = 0; k < idx.size(); ++k) {
+    const int j = idx[k];
+    if (j < 0 || j >= (int)src->mols.size() || src->mols[j].n_atoms != dst->mols[k].n_atoms ||
+        !std::equal(dst->mols[k].z, dst->mols[k].z + dst->mols[k].n_atoms, src->mols[j].z))
+      return fail(PM3_INVALID_INPUT, "molecule " + std::to_string(k) + " is not its source's molecule");
+  }
+  std::string why;
+  const int s = batch_carry_densities(dst->b, src->b, idx, why);
+  return s == 0 ? PM3_OK : fail((pm3_status)s, why);
+}
+
 void pm3_batch_destroy(pm3_batch* h) {
   if (!h) return;
   batch_destroy(h->b);

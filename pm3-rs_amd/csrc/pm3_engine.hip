@@ -3509,6 +3509,7 @@ struct Batch {
   bool want_gradient = false;
   bool keep_density = false;  // the caller asked for densities: copy them back at each fetch
   bool warm_start = false;    // solves after the first start from each molecule's last density
+  bool have_prev = false;     // out_P / out hold densities carried in from another batch
   bool purify = false;  // densities by purification; orbitals are then not produced
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
@@ -4667,7 +4668,7 @@ bool batch_solve(Batch* bp, std::string& err) {
   if (b.learned && !guess_infer(b.guess_model, infer_view(b), err, b.evm)) return false;
   if (!b.learned) HIP_OK(hipEventRecord(b.evm, 0));
   HIP_OK(hipEventRecord(b.ev1, 0));
-  b.args.warm = b.warm_start && b.solved_once ? 1 : 0;
+  b.args.warm = b.warm_start && (b.solved_once || b.have_prev) ? 1 : 0;
   if (b.nsolve > 0) hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
   HIP_OK(hipGetLastError());
   for (int k : b.large)
@@ -4836,6 +4837,41 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
 }
 
 void batch_destroy(Batch* b) { delete b; }
+
+int batch_carry_densities(Batch* dp, const Batch* sp, const std::vector<int>& src_of, std::string& err) {
+  Batch& d = *dp;
+  const Batch& s = *sp;
+  if ((int)src_of.size() != d.nmol) {
+    err = "one source index per molecule of the batch";
+    return 1;
+  }
+  if (!s.solved_once) return 0;  // nothing to carry yet
+  for (int k = 0; k < d.nmol; ++k) {
+    const int j = src_of[k];
+    if (j < 0 || j >= s.nmol) {
+      err = "source index out of range";
+      return 1;
+    }
+    if (d.desc[k].nao != s.desc[j].nao) {
+      err = "molecule " + std::to_string(k) + " differs from its source";
+      return 1;
+    }
+  }
+  auto copy = [&]() -> bool {
+    for (int k = 0; k < d.nmol; ++k) {
+      const MolDesc& dk = d.desc[k];
+      const MolDesc& sj = s.desc[src_of[k]];
+      if (dk.out_mat < 0 || sj.out_mat < 0 || dk.nat == 0) continue;
+      HIP_OK(hipMemcpy(d.args.out_P + dk.out_mat, s.args.out_P + sj.out_mat, sizeof(double) * dk.nao * dk.nao,
+                       hipMemcpyDeviceToDevice));
+      HIP_OK(hipMemcpy(d.args.out + k, s.args.out + src_of[k], sizeof(MolOut), hipMemcpyDeviceToDevice));
+    }
+    return true;
+  };
+  if (!copy()) return 6;
+  d.have_prev = true;
+  return 0;
+}
 
 int batch_set_positions(Batch* bp, const std::vector<MolIn>& mols, std::string& err) {
   Batch& b = *bp;

@@ -100,6 +100,9 @@ void batch_destroy(Batch* b);
 // 1 bad positions (not finite, coincident atoms, wrong counts), 4 a learned-guess batch, 6 a
 // device failure.
 int batch_set_positions(Batch* b, const std::vector<MolIn>& mols, std::string& err);
+// Each molecule's last density and result record of src (molecule src_of[k] for dst's k, the
+// same atoms) into dst, for dst's next warm-started solve.  Returns 0 or a pm3_status code.
+int batch_carry_densities(Batch* dst, const Batch* src, const std::vector<int>& src_of, std::string& err);
 double batch_kernel_ms(const Batch* b);  // device time of the last solve's SCF launch (HIP events)
 double batch_pair_ms(const Batch* b);    // device time of the last solve's pair-integral launch
 double batch_guess_ms(const Batch* b);   // device time of the last solve's learned-guess launches

@@ -11,8 +11,8 @@
 // so here the trial's solve carries the gradient and an accepted point costs no second solve.
 // When fewer than half of a batch's molecules are still running, the batch is rebuilt from the
 // running ones, so the last few slow optimisations do not pay for the finished ones.  With
-// warm_start each round's SCF starts from the molecule's density at its previous point (the
-// first round of a rebuilt batch from the options' guess).
+// warm_start each round's SCF starts from the molecule's density at its previous point, carried
+// into a rebuilt batch too (pm3_batch_carry_densities).
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -178,7 +178,8 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
     }
     if (again) continue;
     if (!batch || 2 * run.size() <= members.size()) {
-      if (batch) pm3_batch_destroy(batch);
+      pm3_batch* old = batch;
+      const std::vector<int> old_members = members;
       batch = nullptr;
       members = run;
       bptr.assign(1, 0);
@@ -196,8 +197,17 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
                                             params, opts,
                                             PM3_BATCH_WANT_GRADIENT | (opt.warm_start ? PM3_BATCH_WARM_START : 0),
                                             &batch);
-      if (s != PM3_OK) {
-        fatal = s;
+      pm3_status carried = PM3_OK;
+      if (s == PM3_OK && old && opt.warm_start) {
+        // the running molecules keep their densities across the rebuild
+        std::vector<int32_t> pos(n_mol, -1), src(members.size());
+        for (size_t p = 0; p < old_members.size(); ++p) pos[old_members[p]] = (int32_t)p;
+        for (size_t p = 0; p < members.size(); ++p) src[p] = pos[members[p]];
+        carried = pm3_batch_carry_densities(batch, old, src.data());
+      }
+      if (old) pm3_batch_destroy(old);
+      if (s != PM3_OK || carried != PM3_OK) {
+        fatal = s != PM3_OK ? s : carried;
         break;
       }
       bgrad.assign(bx.size(), 0.0);

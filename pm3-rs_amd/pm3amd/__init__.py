@@ -117,7 +117,7 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_guess_train", "pm3_guess_eval", "pm3_guess_grad", "pm3_guess_mlp", "pm3_guess_features", "pm3_guess_density",
             "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune",
             "pm3_batch_large_timing", "pm3_batch_set_positions", "pm3_opt_options_default", "pm3_optimize_batch",
-            "pm3_optimize"]
+            "pm3_optimize", "pm3_batch_carry_densities"]
 
 _lib = None
 
@@ -186,6 +186,7 @@ def native():
                                    ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_CTuning),
                                    ctypes.POINTER(_CTuneReport)]
         L.pm3_batch_set_positions.argtypes = [ctypes.c_void_p, dp]
+        L.pm3_batch_carry_densities.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ip]
         L.pm3_opt_options_default.argtypes = [ctypes.POINTER(_COptOptions)]
         L.pm3_opt_options_default.restype = None
         L.pm3_optimize_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ctypes.c_void_p, ctypes.POINTER(_COptions),

@@ -185,3 +185,23 @@ def test_warm_start_skips_densities_of_unconverged_solves(G):
         assert all(a.electronic_ev == b.electronic_ev for a, b in zip(rw, rc))
     w.close()
     c.close()
+
+
+def test_warm_batch_gives_each_molecule_its_warm_single_run(G):
+    """With warm starts the densities follow the running molecules into every rebuilt batch
+    (pm3_batch_carry_densities), so each result is still the molecule's own warm run."""
+    rng = np.random.default_rng(6)
+    mols = [G.Molecule(z, x) for _, z, x in _cases()]
+    for z, x in W.mixed_batch(6, 45, sigma=0.08):
+        mols.append(G.Molecule(np.asarray(z, np.int32), W.to_bohr(W.jitter(np.asarray(x), 0.03, rng))))
+    opts = G.Pm3Options(**TIGHT)
+    opt = G.OptOptions(warm_start=True)
+    st, rs = G.optimize_batch(mols, options=opts, opt=opt)
+    assert np.all(st == 0)
+    for m, r in zip(mols, rs):
+        one = G.optimize(m, scf_options=opts, opt=opt)
+        assert r.converged == one.converged and r.iterations == one.iterations
+        assert r.evaluations == one.evaluations
+        assert abs(r.energy_ev - one.energy_ev) <= 1e-10
+        assert np.abs(r.molecule.positions - one.molecule.positions).max() <= 1e-10
+    print("warm iterations", [r.iterations for r in rs], flush=True)

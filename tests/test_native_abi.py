@@ -166,3 +166,4 @@ def test_optimizer_defaults_and_host_checks_without_device():
     assert L.pm3_optimize(ctypes.byref(m._c()), params, ctypes.byref(so), ctypes.byref(o), ctypes.byref(r),
                           out.ctypes.data_as(dp)) == 4  # open shell: not in this engine
     assert L.pm3_batch_set_positions(None, xyz.ctypes.data_as(dp)) == 1
+    assert L.pm3_batch_carry_densities(None, None, None) == 1
