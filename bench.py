@@ -276,8 +276,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this engine runs on one GPU; sharding a batch over several GPUs is "
+                                   "designed but not launched", "n_gpus": args.gpus}))
         sys.exit(2)
 
     if args.workload == "water512":
