@@ -1922,7 +1922,7 @@ __device__ inline double lanczos_start(int n, double* v, const double* s = nullp
 // multisection (the previous value is a lower bound: by interlacing it never decreases) and the
 // run stops when the Ritz residual is below LZ_TOL or the value has stalled over 8 and 16 steps (a
 // converged value gains ghost copies, whose near-degeneracy spoils the residual estimate).
-// A seeded start (side != 0, lanczos_start) is checked from the eighth step on, every fourth.
+// A seeded start (side != 0, lanczos_start) is checked from the twelfth step on, every fourth.
 __device__ __forceinline__ double lanczos_top(const double* La, bool sq, const double* Ag, int n, int np, double scale,
                                               double* v, const double* s = nullptr, const double* t = nullptr,
                                               int side = 0, unsigned long long* steps = nullptr) {
@@ -1932,7 +1932,7 @@ __device__ __forceinline__ double lanczos_top(const double* La, bool sq, const d
   double* be = al + (cap + 2);
   double* sh = be + (cap + 2);  // [0, 16) reductions, [16] value, [17] stop
   double wn = lanczos_start(n, v, s, t, side);
-  const int first = side != 0 ? 8 : 16, every = side != 0 ? 3 : 7;
+  const int first = side != 0 ? 12 : 16, every = side != 0 ? 3 : 7;
   double bsub = 0.0, th1 = -1.0e300, th2 = -1.0e300, top = 0.0;
   for (int k = 0; k < cap; ++k) {
     double alpha, beta;
@@ -2018,7 +2018,7 @@ __device__ __noinline__ double lanczos_top_wave(const double* L, int n, double s
     w = !live ? 0.0 : ca != 0.0 ? ca * x + cg * g : g;
   }
   double wn = sqrt(wave_sum(w * w));
-  const int first = side != 0 ? 8 : 16, every = side != 0 ? 3 : 7;
+  const int first = side != 0 ? 12 : 16, every = side != 0 ? 3 : 7;
   double bsub = 0.0, th1 = -1.0e300, th2 = -1.0e300, top = 0.0;
   for (int k = 0; k < cap; ++k) {
     double s0 = 0.0, s1 = 0.0;
