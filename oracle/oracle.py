@@ -15,7 +15,8 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 ANGSTROM_TO_BOHR = 1.0 / 0.529177210903
 EV_TO_KCAL = 23.06054783061903
 
-STATUS = {0: "ok", 1: "invalid input", 2: "missing element", 3: "SCF not converged", 4: "unsupported"}
+STATUS = {0: "ok", 1: "invalid input", 2: "missing element", 3: "SCF not converged", 4: "unsupported",
+          5: "resource limit"}
 
 
 class Options(ctypes.Structure):
@@ -29,6 +30,12 @@ class Options(ctypes.Structure):
         ("guess", ctypes.c_int),
         ("stability", ctypes.c_int),
         ("scf_memory_mb", ctypes.c_int),
+        ("level_shift_ev", ctypes.c_double),
+        ("damping", ctypes.c_double),
+        ("reference", ctypes.c_int),
+        ("has_field", ctypes.c_int),
+        ("field", ctypes.c_double * 3),
+        ("integral_memory_mb", ctypes.c_int),
     ]
 
 
@@ -51,13 +58,27 @@ class Result(ctypes.Structure):
         ("nao", ctypes.c_int),
         ("has_homo", ctypes.c_int),
         ("has_lumo", ctypes.c_int),
+        ("unrestricted", ctypes.c_int),
+        ("n_beta", ctypes.c_int),
+        ("required_mb", ctypes.c_double),
+        ("limit_mb", ctypes.c_double),
     ]
 
 
+REFERENCE = {"auto": 0, "rhf": 1, "uhf": 2}
+
+
 def options(max_scf=200, e_tol=1e-8, p_tol=1e-7, use_diis=True, accelerator=2, adiis_switch=0.1, guess=0,
-            stability=1, scf_memory_mb=512):
-    """Mirror of the reference's Pm3Options defaults (src/scf.rs:259-287)."""
-    return Options(max_scf, e_tol, p_tol, int(use_diis), accelerator, adiis_switch, guess, stability, scf_memory_mb)
+            stability=1, scf_memory_mb=512, level_shift_ev=0.0, damping=0.0, reference=0, field=None,
+            integral_memory_mb=0):
+    """Mirror of the reference's Pm3Options defaults (src/scf.rs:259-287).  reference: 0 auto,
+    1 RHF, 2 UHF (or the names); field: None or three components (eV / Bohr / e)."""
+    if isinstance(reference, str):
+        reference = REFERENCE[reference]
+    f = (ctypes.c_double * 3)(*(field if field is not None else (0.0, 0.0, 0.0)))
+    return Options(max_scf, e_tol, p_tol, int(use_diis), accelerator, adiis_switch, guess, stability, scf_memory_mb,
+                   float(level_shift_ev), float(damping), int(reference), int(field is not None), f,
+                   int(integral_memory_mb))
 
 
 _lib = None
@@ -91,8 +112,11 @@ def lib():
                                     ip]
         L.orc_gradient_batch.argtypes = [ctypes.c_int, ip, ip, dp, dp, ctypes.POINTER(Options), ctypes.c_int, dp,
                                          dp, ip, ip, dp]
-        L.orc_run_full.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.POINTER(Options),
-                                   ctypes.POINTER(Result), dp, dp]
+        L.orc_run_pm3.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(Options),
+                                  ctypes.POINTER(Result), dp, dp, dp, dp, dp, dp, dp]
+        L.orc_fixed_density_gradient_spin.argtypes = [ctypes.c_int, ip, dp, dp, dp, dp]
+        L.orc_gradient_pm3.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(Options),
+                                       ctypes.POINTER(Result), dp, dp]
         L.orc_run_uhf.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(Options),
                                   ctypes.POINTER(Result), dp, dp, dp]
         L.orc_num_threads.restype = ctypes.c_int
@@ -206,17 +230,22 @@ def run(z, xyz_bohr, charge=0.0, opts=None, want_arrays=True):
     return st, out
 
 
-def run_full(z, xyz_bohr, charge=0.0, opts=None):
-    """run with the whole stability ladder, its two unrestricted rungs included."""
+def run_pm3(z, xyz_bohr, charge=0.0, multiplicity=1, opts=None):
+    """run_pm3 with every output of Pm3Result: (status, dict with density, spin_density,
+    mo_energies, mo_coeff, mo_energies_beta, mo_coeff_beta, charges)."""
     z, xyz = _mol(z, xyz_bohr)
     opts = opts or options()
     n = nao(z)
     res = Result()
-    P = np.zeros((n, n))
-    q = np.zeros(len(z))
-    st = lib().orc_run_full(len(z), _i(z), _d(xyz), float(charge), ctypes.byref(opts), ctypes.byref(res), _d(P), _d(q))
+    arr = {k: np.zeros((n, n)) for k in ("density", "spin_density", "mo_coeff", "mo_coeff_beta")}
+    arr.update(mo_energies=np.zeros(n), mo_energies_beta=np.zeros(n), charges=np.zeros(len(z)))
+    st = lib().orc_run_pm3(len(z), _i(z), _d(xyz), float(charge), int(multiplicity), ctypes.byref(opts),
+                           ctypes.byref(res), _d(arr["density"]), _d(arr["spin_density"]), _d(arr["mo_energies"]),
+                           _d(arr["mo_coeff"]), _d(arr["mo_energies_beta"]), _d(arr["mo_coeff_beta"]),
+                           _d(arr["charges"]))
     out = {f: getattr(res, f) for f, _ in Result._fields_}
-    out.update(density=P, charges=q)
+    out["dipole"] = np.array(res.dipole[:])
+    out.update(arr)
     return st, out
 
 
@@ -264,14 +293,26 @@ def fixed_density_gradient(z, xyz_bohr, P):
     return g
 
 
-def gradient(z, xyz_bohr, charge=0.0, opts=None):
+def fixed_density_gradient_spin(z, xyz_bohr, P, S):
+    z, xyz = _mol(z, xyz_bohr)
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    g = np.zeros((len(z), 3))
+    st = lib().orc_fixed_density_gradient_spin(len(z), _i(z), _d(xyz), _d(P), _d(S), _d(g))
+    if st:
+        raise RuntimeError(STATUS[st])
+    return g
+
+
+def gradient(z, xyz_bohr, charge=0.0, opts=None, multiplicity=1):
     z, xyz = _mol(z, xyz_bohr)
     opts = opts or options()
     n = nao(z)
     res = Result()
     g = np.zeros((len(z), 3))
     P = np.zeros((n, n))
-    st = lib().orc_gradient(len(z), _i(z), _d(xyz), float(charge), ctypes.byref(opts), ctypes.byref(res), _d(g), _d(P))
+    st = lib().orc_gradient_pm3(len(z), _i(z), _d(xyz), float(charge), int(multiplicity), ctypes.byref(opts),
+                                ctypes.byref(res), _d(g), _d(P))
     out = {f: getattr(res, f) for f, _ in Result._fields_}
     out.update(gradient=g, density=P)
     return st, out

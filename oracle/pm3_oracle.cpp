@@ -18,10 +18,11 @@
 // water HoF and dipole (src/scf.rs:1899-1904), the O-H overlap (src/overlap.rs:688-705), the
 // closed-form (ss|ss) (src/integrals.rs:585-594), the packed-vs-four-index Fock identity
 // (src/fock.rs:510-602), and finite differences of this file's own SCF energy for the
-// gradient (src/gradient.rs:137-176).  The UHF restatement (uhf_loop, orc_run_uhf) is pinned by
-// the 17 open-shell MOPAC rows of tests/data/mopac_oracle.tsv, and the unrestricted stability
-// rungs (orc_run_full) by tetrafluoroethene, whose ground state only they reach.  Parity
-// unpinned: exact SCF iteration counts and DIIS trajectories.
+// gradient (src/gradient.rs:137-176).  The UHF restatement (uhf_loop, orc_run_pm3) is pinned by
+// the 17 open-shell MOPAC rows of tests/data/mopac_oracle.tsv, and the closed-shell stability
+// ladder (two unrestricted rungs, spin-purity collapse) by the molecules the reference's own test
+// names as rescued by it (tests/mopac_oracle.rs:268-308): butyne, ketene, tetrafluoroethene,
+// tetrachloroethene.  Parity unpinned: exact SCF iteration counts and DIIS trajectories.
 
 #include <algorithm>
 #include <array>
@@ -704,7 +705,7 @@ struct Mol {
   int nao = 0;
 };
 
-enum Status { OK = 0, E_INVALID = 1, E_MISSING_ELEMENT = 2, E_NOT_CONVERGED = 3, E_UNSUPPORTED = 4 };
+enum Status { OK = 0, E_INVALID = 1, E_MISSING_ELEMENT = 2, E_NOT_CONVERGED = 3, E_UNSUPPORTED = 4, E_RESOURCE = 5 };
 
 static int make_mol(int nat, const int* z, const double* xyz, Mol& m) {
   const Table& T = table();
@@ -738,7 +739,47 @@ struct Core {
 
 static double beta_of(const Elem& e, int orb) { return orb == 0 ? e.betas : e.betap; }
 
-static int build_core(const Mol& m, Core& c) {
+// Bytes of the reference's resident pair cache (pair_cache_bytes, src/hamiltonian.rs:51-71):
+// sum_{i<j} p_i p_j doubles of packed W plus one PairIntegral record per atom pair.  The record is
+// two usize indices and a PackedTwoElec of four usize dimensions and a Vec<f64> (pointer,
+// capacity, length): 72 bytes on a 64-bit target.
+static double pair_cache_bytes(const Mol& m) {
+  double total = 0.0, squares = 0.0;
+  for (int i = 0; i < m.nat; ++i) {
+    const double p = std::max(m.norb[i] * (m.norb[i] + 1) / 2, 1);
+    total += p;
+    squares += p * p;
+  }
+  const double elements = (total * total - squares) / 2.0;
+  const double pairs = (double)m.nat * (double)(m.nat - 1) / 2.0;
+  return elements * 8.0 + pairs * 72.0;
+}
+
+// The external field's one-electron operator +M.f (field_terms, src/dipole.rs:156-181, with
+// dipole_matrix :93-125 at the coordinate origin): every AO of atom A gets f.R_A on the diagonal,
+// the s-p_alpha element gets f_alpha dd_A; added after U and before the pair terms
+// (src/hamiltonian.rs:224-233).  Returns the nuclear half -sum_A Z_A R_A.f.
+static double add_field(const Mol& m, const double* f, std::vector<double>& h) {
+  const int n = m.nao;
+  double nuclear = 0.0;
+  for (int i = 0; i < m.nat; ++i) {
+    const Elem& e = *m.el[i];
+    const int o = m.off[i];
+    const double* R = &m.xyz[3 * i];
+    for (int ax = 0; ax < 3; ++ax) {
+      if (f[ax] == 0.0) continue;
+      for (int mu = 0; mu < m.norb[i]; ++mu) h[(size_t)(o + mu) * n + o + mu] += f[ax] * R[ax];
+      if (m.norb[i] >= 4) {
+        h[(size_t)o * n + o + ax + 1] += f[ax] * e.dd;
+        h[(size_t)(o + ax + 1) * n + o] += f[ax] * e.dd;
+      }
+    }
+    nuclear -= e.tore * (R[0] * f[0] + R[1] * f[1] + R[2] * f[2]);
+  }
+  return nuclear;
+}
+
+static int build_core(const Mol& m, Core& c, const double* field = nullptr, double* field_nuclear = nullptr) {
   int n = m.nao;
   c.h.assign((size_t)n * n, 0.0);
   c.pairs.clear();
@@ -746,6 +787,12 @@ static int build_core(const Mol& m, Core& c) {
   for (int i = 0; i < m.nat; ++i) {
     const Elem& e = *m.el[i];
     for (int k = 0; k < m.norb[i]; ++k) c.h[(size_t)(m.off[i] + k) * n + m.off[i] + k] = k == 0 ? e.uss : e.upp;
+  }
+  if (field) {
+    const double nuc = add_field(m, field, c.h);
+    if (field_nuclear) *field_nuclear = nuc;
+  } else if (field_nuclear) {
+    *field_nuclear = 0.0;
   }
   for (int u = 0; u < m.nat; ++u)
     for (int v = u + 1; v < m.nat; ++v) {
@@ -1016,6 +1063,12 @@ struct Options {
   int guess;        // 0 auto, 1 SAD, 2 core, 3 symmetry-broken
   int stability;    // 0 off, 1 auto, 2 always
   int scf_memory_mb;
+  double level_shift_ev;   // src/scf.rs:193-198: read by uhf_loop; non-zero (or damping) steers the first path
+  double damping;          // src/scf.rs:199-202
+  int reference;           // 0 auto, 1 RHF, 2 UHF (src/scf.rs:508-520)
+  int has_field;           // field is Some (src/scf.rs:248-256)
+  double field[3];         // eV per Bohr per elementary charge, referenced to the coordinate origin
+  int integral_memory_mb;  // pair-cache ceiling in MiB; 0 takes the process default (src/scf.rs:289-297)
 };
 
 struct Result {
@@ -1024,6 +1077,8 @@ struct Result {
   double homo_ev, lumo_ev;
   double last_change, scf_improvement_ev;
   int iterations, converged, scf_paths_tried, n_occ, nao, has_homo, has_lumo;
+  int unrestricted, n_beta;
+  double required_mb, limit_mb;  // ResourceLimit details (src/error.rs:12-47)
 };
 
 static int diis_depth(int nao, int mats, int budget_mb) {
@@ -1305,7 +1360,7 @@ static void rhf_loop(const Mol& m, const Core& core, int nocc, const Options& o,
 }
 
 struct UState {
-  std::vector<double> Pa, Pb, ea, eb, Ca;
+  std::vector<double> Pa, Pb, ea, eb, Ca, Cb;
   double e_elec = 0, last_change = INFINITY;
   int iterations = 0;
   bool converged = false;
@@ -1442,6 +1497,7 @@ static void uhf_loop(const Mol& m, const Core& core, int na, int nb, const Optio
     st.ea = ea;
     st.eb = eb;
     st.Ca = Ca;
+    st.Cb = Cb;
     Pa.swap(Pan);
     Pb.swap(Pbn);
     e_old = e_elec;
@@ -1456,6 +1512,19 @@ static void uhf_loop(const Mol& m, const Core& core, int na, int nb, const Optio
   st.e_elec = 0.5 * (dot(Pt, core.h) + dot(Pa, Fa) + dot(Pb, Fb));
   st.Pa = Pa;
   st.Pb = Pb;
+}
+
+// Core-core repulsion summed over all atom pairs in index order (core_core_energy,
+// src/repulsion.rs:13-27).
+static double orc_core_core_impl(const Mol& m) {
+  double e = 0.0;
+  const double* xyz = m.xyz.data();
+  for (int i = 0; i < m.nat; ++i)
+    for (int j = i + 1; j < m.nat; ++j) {
+      double d[3] = {xyz[3 * j] - xyz[3 * i], xyz[3 * j + 1] - xyz[3 * i + 1], xyz[3 * j + 2] - xyz[3 * i + 2]};
+      e += pair_core_energy<double>(*m.el[i], *m.el[j], std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+    }
+  return e;
 }
 
 }  // namespace orc
@@ -1589,13 +1658,7 @@ int orc_fock_fourindex(int nat, const int* z, const double* xyz, const double* P
 double orc_core_core(int nat, const int* z, const double* xyz) {
   Mol m;
   if (make_mol(nat, z, xyz, m)) return NAN;
-  double e = 0.0;
-  for (int i = 0; i < nat; ++i)
-    for (int j = i + 1; j < nat; ++j) {
-      double d[3] = {xyz[3 * j] - xyz[3 * i], xyz[3 * j + 1] - xyz[3 * i + 1], xyz[3 * j + 2] - xyz[3 * i + 2]};
-      e += pair_core_energy<double>(*m.el[i], *m.el[j], std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
-    }
-  return e;
+  return orc_core_core_impl(m);
 }
 
 int orc_eigen(int n, const double* A, double* w, double* V) {
@@ -1603,80 +1666,194 @@ int orc_eigen(int n, const double* A, double* w, double* V) {
   return OK;
 }
 
-// One RHF PM3 single point: src/scf.rs:450-848 (run_pm3).  The stability search runs the five
-// restricted rungs of src/scf.rs:641-673.  With urungs it also runs the two unrestricted rungs
-// (:662-673, :694-708): the closed shell through uhf_loop, kept only when it stays spin-pure
-// (max |P_a - P_b| < 1e-8) and then read as the restricted state (P = P_a + P_b, the alpha
-// orbitals); without, they are counted in scf_paths_tried but not run, as in the batched engine.
-// Open shells return E_UNSUPPORTED.  Optional outputs may be null.
-static int run_impl(int nat, const int* z, const double* xyz, double charge, const Options* opt, const double* P0,
-                    Result* res, double* density, double* mo_energies, double* mo_coeff, double* charges,
-                    bool urungs = false) {
-  Mol m;
+// The state a solve hands back (ScfState, src/scf.rs:1300-1315 and :1482-1495): the total
+// density, and for an unrestricted state the spin density and the beta spectrum too.
+struct Scf {
+  std::vector<double> P, S, ea, eb, Ca, Cb;  // S, eb, Cb empty for a restricted state
+  double e_elec = 0.0, last_change = INFINITY;
+  int iterations = 0, n_occ = 0, n_beta = 0;
+  bool converged = false, unrestricted = false;
+};
+
+static Scf scf_of_rhf(const State& s, int nocc) {
+  Scf r;
+  r.P = s.P;
+  r.ea = s.eps;
+  r.Ca = s.C;
+  r.e_elec = s.e_elec;
+  r.last_change = s.last_change;
+  r.iterations = s.iterations;
+  r.converged = s.converged;
+  r.n_occ = r.n_beta = nocc;
+  return r;
+}
+
+static Scf scf_of_uhf(const UState& u, int na, int nb) {
+  Scf r;
+  r.P.resize(u.Pa.size());
+  r.S.resize(u.Pa.size());
+  for (size_t k = 0; k < u.Pa.size(); ++k) {
+    r.P[k] = u.Pa[k] + u.Pb[k];
+    r.S[k] = u.Pa[k] - u.Pb[k];
+  }
+  r.ea = u.ea;
+  r.eb = u.eb;
+  r.Ca = u.Ca;
+  r.Cb = u.Cb;
+  r.e_elec = u.e_elec;
+  r.last_change = u.last_change;
+  r.iterations = u.iterations;
+  r.converged = u.converged;
+  r.n_occ = na;
+  r.n_beta = nb;
+  r.unrestricted = true;
+  return r;
+}
+
+// Frontier levels over both spin channels (frontier_gap, src/scf.rs:106-124; the reported
+// HOMO / LUMO, :797-820): HOMO the higher occupied level, LUMO the lower virtual one.
+static void frontier_of(const Scf& s, int n, bool& hh, double& h, bool& hl, double& l) {
+  hh = hl = false;
+  h = -INFINITY;
+  l = INFINITY;
+  auto take = [&](const std::vector<double>& e, int occ) {
+    if (e.empty()) return;
+    if (occ >= 1) { hh = true; h = std::max(h, e[occ - 1]); }
+    if (occ < n) { hl = true; l = std::min(l, e[occ]); }
+  };
+  take(s.ea, s.n_occ);
+  if (s.unrestricted) take(s.eb, s.n_beta);
+}
+
+static double frontier_gap(const Scf& s, int n) {
+  bool hh, hl;
+  double h, l;
+  frontier_of(s, n, hh, h, hl, l);
+  return hh && hl ? l - h : INFINITY;
+}
+
+// Largest element of P_a - P_b (spin_purity, src/scf.rs:80-86).
+static double spin_purity(const Scf& s) {
+  double m = 0.0;
+  for (double v : s.S) m = std::max(m, std::fabs(v));
+  return m;
+}
+
+// A spin-pure unrestricted state read as the restricted state it is (collapse_to_restricted,
+// src/scf.rs:88-95): the alpha orbitals stand for both spins.
+static void collapse(Scf& s) {
+  s.S.clear();
+  s.eb.clear();
+  s.Cb.clear();
+  s.n_beta = s.n_occ;
+  s.unrestricted = false;
+}
+
+constexpr double STABILITY_GAP_EV = 6.0;   // src/scf.rs:74
+constexpr double STABILITY_TOL_EV = 1.0e-6;  // src/scf.rs:79
+constexpr double SPIN_PURE_TOL = 1.0e-8;   // src/scf.rs:100
+
+// run_pm3 (src/scf.rs:450-848) for s/p elements.
+//  - electron count and spin counts n_a = (N + M - 1) / 2, n_b = (N - M + 1) / 2 (:486-506);
+//  - RHF or UHF by options.reference (:508-520): Auto picks UHF for n_a != n_b, RHF refuses an
+//    open shell;
+//  - the pair-cache ceiling (src/hamiltonian.rs:202-211, default 4096 MiB or
+//    PM3_MAX_PAIR_CACHE_MB, src/hamiltonian.rs:44-49) -> E_RESOURCE;
+//  - the first path; a level shift or damping "steers" it (:563-590): rhf_loop reads neither, so
+//    for a restricted solve steering only switches the stability search off;
+//  - the stability search (:599-715) when the frontier gap is below 6 eV (Auto) or always (Always),
+//    never when steered.  Rungs (:647-673), guess x accelerator x shift x damping:
+//      1 (primary, none)  2 (other, options)  3 (other, none)  4 (symmetry-broken, options)
+//      5 (symmetry-broken, none)  6 (symmetry-broken, none, -2 eV, unrestricted)
+//      7 (primary, CDIIS, damping 0.85, unrestricted)
+//    with primary = Core if options.guess is Core else SAD.  A rung runs when its "unrestricted"
+//    flag equals closed_shell (:676): a closed shell on the restricted path runs rungs 6 and 7
+//    only, through uhf_loop, and keeps a converged one only when it stayed spin-pure (max|P_a -
+//    P_b| < 1e-8), collapsed to the restricted state; an unrestricted solve runs all seven rungs
+//    as ordinary unrestricted paths (rungs 6 and 7 with their flag false).  The lowest converged
+//    electronic energy wins (1e-6 eV);
+//  - core-core plus the field's nuclear term, HoF, Mulliken charges, dipole about the centre of
+//    mass, HOMO / LUMO over both spin channels.
+// P0 (restricted first path only) replaces the guess rule: see orc_run_from_density.
+static int solve(int nat, const int* z, const double* xyz, double charge, int multiplicity, const Options& o,
+                 const double* P0, Mol& m, Scf& best, Result& r) {
+  r = Result{};
+  if (multiplicity < 1) return E_INVALID;
   int st = make_mol(nat, z, xyz, m);
   if (st) return st;
   double n_elec = 0.0;
   for (int i = 0; i < nat; ++i) n_elec += m.el[i]->tore;
   n_elec -= charge;
-  long ne = std::lround(n_elec);
+  const long ne = std::lround(n_elec);
   if (std::fabs(n_elec - (double)ne) > 1.0e-6 || ne < 0) return E_INVALID;
-  if (ne % 2 != 0) return E_UNSUPPORTED;
-  int nocc = (int)(ne / 2);
-  Core core;
-  if ((st = build_core(m, core))) return st;
-  int n = m.nao;
-  bool has_he = false;
-  for (int i = 0; i < nat; ++i) has_he |= z[i] == 2;
-  Options o = *opt;
-  State best;
-  int accel0 = (o.use_diis && !has_he) ? o.accelerator : 0;
-  rhf_loop(m, core, nocc, o, o.guess, accel0, best, P0);
-  int paths = 1;
-  auto gap = [&](const State& s) { return (nocc > 0 && nocc < n) ? s.eps[nocc] - s.eps[nocc - 1] : INFINITY; };
-  bool search = o.stability == 2 || (o.stability == 1 && gap(best) < 6.0);
-  double first_ev = best.e_elec;
-  if (search) {
-    int primary = o.guess == 2 ? 2 : 1;
-    int other = primary == 2 ? 1 : 2;
-    int rungs[5][2] = {{primary, 0}, {other, o.accelerator}, {other, 0}, {3, o.accelerator}, {3, 0}};
-    for (auto& rg : rungs) {
-      ++paths;
-      State cand;
-      int acc = (o.use_diis && rg[1] != 0 && !has_he) ? rg[1] : 0;
-      rhf_loop(m, core, nocc, o, rg[0], acc, cand);
-      if (!cand.converged) continue;
-      if (!best.converged || cand.e_elec < best.e_elec - 1.0e-6) best = cand;
-    }
-    if (!urungs) {
-      paths += 2;
-    } else {
-      const struct { int guess, accel; double shift, damp; } ur[2] = {{3, 0, -2.0, 0.0}, {primary, 1, 0.0, 0.85}};
-      for (const auto& rg : ur) {
-        ++paths;
-        UState u;
-        uhf_loop(m, core, nocc, nocc, o, rg.guess, o.use_diis && rg.accel != 0 ? rg.accel : 0, rg.shift, rg.damp, u);
-        if (!u.converged) continue;
-        double spin = 0.0;
-        for (size_t k = 0; k < u.Pa.size(); ++k) spin = std::max(spin, std::fabs(u.Pa[k] - u.Pb[k]));
-        if (!(spin < 1.0e-8)) continue;
-        State cand;
-        cand.P.resize(u.Pa.size());
-        for (size_t k = 0; k < u.Pa.size(); ++k) cand.P[k] = u.Pa[k] + u.Pb[k];
-        cand.eps = u.ea;
-        cand.C = u.Ca;
-        cand.e_elec = u.e_elec;
-        cand.last_change = u.last_change;
-        cand.iterations = u.iterations;
-        cand.converged = true;
-        if (!best.converged || cand.e_elec < best.e_elec - 1.0e-6) best = cand;
-      }
+  const long nu = multiplicity - 1;
+  if (ne - nu < 0 || (ne - nu) % 2 != 0) return E_INVALID;
+  const int na = (int)((ne + nu) / 2), nb = (int)((ne - nu) / 2);
+  bool use_uhf = na != nb;
+  if (o.reference == 2) use_uhf = true;
+  if (o.reference == 1 && na != nb) return E_INVALID;
+  const int n = m.nao;
+  r.nao = n;
+  long limit = o.integral_memory_mb;
+  if (limit <= 0) {
+    limit = 4096;
+    if (const char* env = std::getenv("PM3_MAX_PAIR_CACHE_MB")) limit = std::strtol(env, nullptr, 10);
+  }
+  if (limit > 0) {
+    const double need = pair_cache_bytes(m);
+    if (need > (double)limit * 1048576.0) {
+      r.required_mb = std::ceil(need / 1048576.0);
+      r.limit_mb = (double)limit;
+      return E_RESOURCE;
     }
   }
-  double core_ev = orc_core_core(nat, z, xyz);
-  double total = best.e_elec + core_ev;
+  Core core;
+  double field_nuclear = 0.0;
+  if ((st = build_core(m, core, o.has_field ? o.field : nullptr, &field_nuclear))) return st;
+  bool has_he = false;
+  for (int i = 0; i < nat; ++i) has_he |= z[i] == 2;
+  auto run_scf = [&](double shift, double damp, int guess, int accel, bool force_u, const double* start) {
+    Options oo = o;
+    oo.use_diis = o.use_diis && accel != 0;
+    if (use_uhf || force_u) {
+      UState u;
+      uhf_loop(m, core, na, nb, oo, guess, oo.use_diis ? accel : 0, shift, damp, u);
+      return scf_of_uhf(u, na, nb);
+    }
+    State s;
+    rhf_loop(m, core, na, oo, guess, (oo.use_diis && !has_he) ? accel : 0, s, start);
+    return scf_of_rhf(s, na);
+  };
+  const bool steered = o.level_shift_ev != 0.0 || o.damping != 0.0;
+  best = steered ? run_scf(o.level_shift_ev, o.damping, o.guess, o.accelerator, false, P0)
+                 : run_scf(0.0, 0.0, o.guess, o.accelerator, false, P0);
+  int paths = 1;
+  const bool search = o.stability == 2 ? !steered : o.stability == 1 ? (!steered && frontier_gap(best, n) < STABILITY_GAP_EV)
+                                                                     : false;
+  const double first_ev = best.e_elec;
+  if (search) {
+    const int primary = o.guess == 2 ? 2 : 1, other = primary == 2 ? 1 : 2;
+    const bool closed_shell = !use_uhf;
+    const struct { int guess, accel; double shift, damp; bool unrestricted; } rungs[7] = {
+        {primary, 0, 0.0, 0.0, false},        {other, o.accelerator, 0.0, 0.0, false}, {other, 0, 0.0, 0.0, false},
+        {3, o.accelerator, 0.0, 0.0, false},  {3, 0, 0.0, 0.0, false},                 {3, 0, -2.0, 0.0, closed_shell},
+        {primary, 1, 0.0, 0.85, closed_shell}};
+    for (const auto& rg : rungs) {
+      if (rg.unrestricted != closed_shell) continue;
+      ++paths;
+      Scf cand = run_scf(rg.shift, rg.damp, rg.guess, rg.accel, rg.unrestricted, nullptr);
+      if (!cand.converged) continue;
+      if (rg.unrestricted) {
+        if (!(spin_purity(cand) < SPIN_PURE_TOL)) continue;
+        collapse(cand);
+      }
+      if (!best.converged || cand.e_elec < best.e_elec - STABILITY_TOL_EV) best = cand;
+    }
+  }
+  const double core_ev = orc_core_core_impl(m) + field_nuclear;
+  const double total = best.e_elec + core_ev;
   double eis = 0.0, ehs = 0.0;
   for (int i = 0; i < nat; ++i) { eis += m.el[i]->e_isol; ehs += m.el[i]->eheat_ev; }
-  Result r{};
   r.total_ev = total;
   r.electronic_ev = best.e_elec;
   r.core_ev = core_ev;
@@ -1686,51 +1863,21 @@ static int run_impl(int nat, const int* z, const double* xyz, double charge, con
   r.last_change = best.last_change;
   r.scf_paths_tried = paths;
   r.scf_improvement_ev = std::max(first_ev - best.e_elec, 0.0);
-  r.n_occ = nocc;
-  r.nao = n;
-  r.has_homo = nocc >= 1;
-  r.has_lumo = nocc < n;
-  r.homo_ev = r.has_homo ? best.eps[nocc - 1] : 0.0;
-  r.lumo_ev = r.has_lumo ? best.eps[nocc] : 0.0;
-  // Dipole about the centre of mass, src/dipole.rs:64-140.
-  double tm = 0.0, com[3] = {0, 0, 0};
-  for (int i = 0; i < nat; ++i) {
-    tm += m.el[i]->mass;
-    for (int k = 0; k < 3; ++k) com[k] += xyz[3 * i + k] * m.el[i]->mass;
-  }
-  for (int k = 0; k < 3; ++k) com[k] = tm > 0 ? com[k] / tm : 0.0;
-  double dip[3];
-  for (int ax = 0; ax < 3; ++ax) {
-    double nuc = 0.0, elec = 0.0;
-    for (int i = 0; i < nat; ++i) {
-      double disp = xyz[3 * i + ax] - com[ax];
-      nuc += disp * m.el[i]->tore;
-      int o2 = m.off[i];
-      for (int mu = 0; mu < m.norb[i]; ++mu) elec += best.P[(size_t)(o2 + mu) * n + o2 + mu] * disp;
-      if (m.norb[i] >= 4) {
-        int p = o2 + ax + 1;
-        elec += (best.P[(size_t)o2 * n + p] + best.P[(size_t)p * n + o2]) * m.el[i]->dd;
-      }
-    }
-    dip[ax] = (nuc - elec) * AU_DIPOLE_TO_DEBYE;
-  }
-  for (int k = 0; k < 3; ++k) r.dipole[k] = dip[k];
-  r.dipole_debye = std::sqrt(dip[0] * dip[0] + dip[1] * dip[1] + dip[2] * dip[2]);
-  if (res) *res = r;
-  if (density) std::memcpy(density, best.P.data(), sizeof(double) * (size_t)n * n);
-  if (mo_energies) std::memcpy(mo_energies, best.eps.data(), sizeof(double) * n);
-  if (mo_coeff) std::memcpy(mo_coeff, best.C.data(), sizeof(double) * (size_t)n * n);
-  if (charges)
-    for (int i = 0; i < nat; ++i) {
-      double pop = 0.0;
-      for (int mu = 0; mu < m.norb[i]; ++mu) pop += best.P[(size_t)(m.off[i] + mu) * n + m.off[i] + mu];
-      charges[i] = m.el[i]->tore - pop;
-    }
+  r.n_occ = best.n_occ;
+  r.n_beta = best.n_beta;
+  r.unrestricted = best.unrestricted;
+  bool hh, hl;
+  double h, l;
+  frontier_of(best, n, hh, h, hl, l);
+  r.has_homo = hh;
+  r.has_lumo = hl;
+  r.homo_ev = hh ? h : 0.0;
+  r.lumo_ev = hl ? l : 0.0;
   return best.converged ? OK : E_NOT_CONVERGED;
 }
 
-// Dipole about the centre of mass (src/dipole.rs:64-140) and Mulliken charges from a total
-// density P, as the RHF path computes them.
+// Dipole about the centre of mass (src/dipole.rs:64-140, src/scf.rs:782-791) and Mulliken charges
+// from the total density P (src/scf.rs:762-772).
 static void density_properties(const Mol& m, int nat, const double* xyz, const std::vector<double>& P, Result& r,
                                double* charges) {
   const int n = m.nao;
@@ -1765,108 +1912,59 @@ static void density_properties(const Mol& m, int nat, const double* xyz, const s
     }
 }
 
-// run_pm3 (src/scf.rs:450-848) with a multiplicity: n_a = (N + M - 1) / 2, n_b = (N - M + 1) / 2
-// (:486-506).  Equal counts take the RHF path (orc_run); otherwise uhf_loop, then the stability
-// search when the frontier gap over both spin channels (frontier_gap, :106-124) is below 6 eV:
-// for an open shell all seven rungs of :647-673 run through uhf_loop (the five guess x
-// accelerator rungs, then symmetry-broken with a -2 eV shift, then CDIIS with damping 0.85), and
-// the lowest converged electronic energy is kept (1e-6 eV).  HOMO = max over the channels, LUMO =
-// min (:797-820); density and charges from P_a + P_b; spin density P_a - P_b.
-int orc_run_uhf(int nat, const int* z, const double* xyz, double charge, int multiplicity, const Options* opt,
-                Result* res, double* density, double* spin_density, double* charges) {
-  if (multiplicity < 1) return E_INVALID;
+// solve plus the outputs.  Every output pointer may be null.  spin_density, mo_energies_beta and
+// mo_coeff_beta are zero-filled for a restricted result.
+static int run_out(int nat, const int* z, const double* xyz, double charge, int multiplicity, const Options* opt,
+                   const double* P0, Result* res, double* density, double* spin_density, double* mo_energies,
+                   double* mo_coeff, double* mo_energies_beta, double* mo_coeff_beta, double* charges) {
   Mol m;
-  int st = make_mol(nat, z, xyz, m);
-  if (st) return st;
-  double n_elec = 0.0;
-  for (int i = 0; i < nat; ++i) n_elec += m.el[i]->tore;
-  n_elec -= charge;
-  const long ne = std::lround(n_elec);
-  if (std::fabs(n_elec - (double)ne) > 1.0e-6 || ne < 0) return E_INVALID;
-  const long nu = multiplicity - 1;
-  if (ne - nu < 0 || (ne - nu) % 2 != 0) return E_INVALID;
-  const int na = (int)((ne + nu) / 2), nb = (int)((ne - nu) / 2);
-  const int n = m.nao;
-  if (na == nb) {
-    const int s2 = run_impl(nat, z, xyz, charge, opt, nullptr, res, density, nullptr, nullptr, charges);
-    if (spin_density) std::fill(spin_density, spin_density + (size_t)n * n, 0.0);
-    return s2;
-  }
-  Core core;
-  if ((st = build_core(m, core))) return st;
-  const Options o = *opt;
-  auto acc_of = [&](int a) { return o.use_diis && a != 0 ? a : 0; };
-  UState best;
-  uhf_loop(m, core, na, nb, o, o.guess, acc_of(o.accelerator), 0.0, 0.0, best);
-  auto frontier = [&](const UState& u, double& h, double& l, bool& hh, bool& hl) {
-    hh = na >= 1 || nb >= 1;
-    hl = na < n || nb < n;
-    h = -INFINITY;
-    l = INFINITY;
-    if (na >= 1) h = std::max(h, u.ea[na - 1]);
-    if (nb >= 1) h = std::max(h, u.eb[nb - 1]);
-    if (na < n) l = std::min(l, u.ea[na]);
-    if (nb < n) l = std::min(l, u.eb[nb]);
-  };
-  double h, l;
-  bool hh, hl;
-  frontier(best, h, l, hh, hl);
-  const double gap = hh && hl ? l - h : INFINITY;
-  int paths = 1;
-  const double first_ev = best.e_elec;
-  if (o.stability == 2 || (o.stability == 1 && gap < 6.0)) {
-    const int primary = o.guess == 2 ? 2 : 1, other = primary == 2 ? 1 : 2;
-    const struct { int guess, accel; double shift, damp; } rungs[7] = {
-        {primary, 0, 0.0, 0.0}, {other, o.accelerator, 0.0, 0.0}, {other, 0, 0.0, 0.0}, {3, o.accelerator, 0.0, 0.0},
-        {3, 0, 0.0, 0.0},       {3, 0, -2.0, 0.0},                  {primary, 1, 0.0, 0.85}};
-    for (const auto& rg : rungs) {
-      ++paths;
-      UState cand;
-      uhf_loop(m, core, na, nb, o, rg.guess, acc_of(rg.accel), rg.shift, rg.damp, cand);
-      if (!cand.converged) continue;
-      if (!best.converged || cand.e_elec < best.e_elec - 1.0e-6) best = cand;
+  Scf best;
+  Result r;
+  const int st = solve(nat, z, xyz, charge, multiplicity, *opt, P0, m, best, r);
+  if (st == OK || st == E_NOT_CONVERGED) {
+    const int n = m.nao;
+    const size_t nn = (size_t)n * n;
+    density_properties(m, nat, xyz, best.P, r, charges);
+    if (density) std::memcpy(density, best.P.data(), sizeof(double) * nn);
+    if (spin_density) {
+      if (best.unrestricted) std::memcpy(spin_density, best.S.data(), sizeof(double) * nn);
+      else std::fill(spin_density, spin_density + nn, 0.0);
+    }
+    if (mo_energies) std::memcpy(mo_energies, best.ea.data(), sizeof(double) * n);
+    if (mo_coeff) std::memcpy(mo_coeff, best.Ca.data(), sizeof(double) * nn);
+    if (mo_energies_beta) {
+      if (best.unrestricted) std::memcpy(mo_energies_beta, best.eb.data(), sizeof(double) * n);
+      else std::fill(mo_energies_beta, mo_energies_beta + n, 0.0);
+    }
+    if (mo_coeff_beta) {
+      if (best.unrestricted) std::memcpy(mo_coeff_beta, best.Cb.data(), sizeof(double) * nn);
+      else std::fill(mo_coeff_beta, mo_coeff_beta + nn, 0.0);
     }
   }
-  std::vector<double> Pt((size_t)n * n);
-  for (size_t k = 0; k < Pt.size(); ++k) Pt[k] = best.Pa[k] + best.Pb[k];
-  const double core_ev = orc_core_core(nat, z, xyz);
-  const double total = best.e_elec + core_ev;
-  double eis = 0.0, ehs = 0.0;
-  for (int i = 0; i < nat; ++i) { eis += m.el[i]->e_isol; ehs += m.el[i]->eheat_ev; }
-  Result r{};
-  r.total_ev = total;
-  r.electronic_ev = best.e_elec;
-  r.core_ev = core_ev;
-  r.hof_kcal = (total - eis + ehs) * EV_TO_KCAL;
-  r.iterations = best.iterations;
-  r.converged = best.converged;
-  r.last_change = best.last_change;
-  r.scf_paths_tried = paths;
-  r.scf_improvement_ev = std::max(first_ev - best.e_elec, 0.0);
-  r.n_occ = na;
-  r.nao = n;
-  frontier(best, h, l, hh, hl);
-  r.has_homo = hh;
-  r.has_lumo = hl;
-  r.homo_ev = hh ? h : 0.0;
-  r.lumo_ev = hl ? l : 0.0;
-  density_properties(m, nat, xyz, Pt, r, charges);
   if (res) *res = r;
-  if (density) std::memcpy(density, Pt.data(), sizeof(double) * Pt.size());
-  if (spin_density)
-    for (size_t k = 0; k < Pt.size(); ++k) spin_density[k] = best.Pa[k] - best.Pb[k];
-  return best.converged ? OK : E_NOT_CONVERGED;
+  return st;
 }
 
-// orc_run with the whole stability ladder, unrestricted rungs included.
-int orc_run_full(int nat, const int* z, const double* xyz, double charge, const Options* opt, Result* res,
-                 double* density, double* charges) {
-  return run_impl(nat, z, xyz, charge, opt, nullptr, res, density, nullptr, nullptr, charges, true);
+// run_pm3 with a multiplicity and every output of Pm3Result (src/scf.rs:352-396).
+int orc_run_pm3(int nat, const int* z, const double* xyz, double charge, int multiplicity, const Options* opt,
+                Result* res, double* density, double* spin_density, double* mo_energies, double* mo_coeff,
+                double* mo_energies_beta, double* mo_coeff_beta, double* charges) {
+  return run_out(nat, z, xyz, charge, multiplicity, opt, nullptr, res, density, spin_density, mo_energies, mo_coeff,
+                 mo_energies_beta, mo_coeff_beta, charges);
 }
 
+// run_pm3 of a singlet (multiplicity 1).
 int orc_run(int nat, const int* z, const double* xyz, double charge, const Options* opt, Result* res, double* density,
             double* mo_energies, double* mo_coeff, double* charges) {
-  return run_impl(nat, z, xyz, charge, opt, nullptr, res, density, mo_energies, mo_coeff, charges);
+  return run_out(nat, z, xyz, charge, 1, opt, nullptr, res, density, nullptr, mo_energies, mo_coeff, nullptr, nullptr,
+                 charges);
+}
+
+// run_pm3 with a multiplicity: total density, spin density and charges.
+int orc_run_uhf(int nat, const int* z, const double* xyz, double charge, int multiplicity, const Options* opt,
+                Result* res, double* density, double* spin_density, double* charges) {
+  return run_out(nat, z, xyz, charge, multiplicity, opt, nullptr, res, density, spin_density, nullptr, nullptr,
+                 nullptr, nullptr, charges);
 }
 
 // The same single point started from a caller-provided density P0 (n x n): P0 builds the first
@@ -1878,12 +1976,22 @@ int orc_run_from_density(int nat, const int* z, const double* xyz, double charge
   if (!P0) return E_INVALID;
   Options o = *opt;
   if (o.guess == 2) o.guess = 1;
-  return run_impl(nat, z, xyz, charge, &o, P0, res, density, nullptr, nullptr, nullptr);
+  return run_out(nat, z, xyz, charge, 1, &o, P0, res, density, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 // Hellmann-Feynman gradient at a fixed density P (eV/Bohr): src/gradient.rs:306-318 and
-// :423-489 (electronic part, dual numbers) plus src/repulsion.rs:30-49 (core-core).
+// :423-489 (electronic part, dual numbers) plus src/repulsion.rs:30-49 (core-core).  With a spin
+// density S (may be null) the exchange term is the unrestricted one of
+// electronic_gradient_fixed_density_spin (src/gradient.rs:523-599): P_a = (P + S) / 2,
+// P_b = (P - S) / 2 and -(P_a P_a + P_b P_b) in place of -P P / 2.
+int orc_fixed_density_gradient_spin(int nat, const int* z, const double* xyz, const double* P, const double* S,
+                                    double* g);
 int orc_fixed_density_gradient(int nat, const int* z, const double* xyz, const double* P, double* g) {
+  return orc_fixed_density_gradient_spin(nat, z, xyz, P, nullptr, g);
+}
+
+int orc_fixed_density_gradient_spin(int nat, const int* z, const double* xyz, const double* P, const double* S,
+                                    double* g) {
   Mol m;
   int st = make_mol(nat, z, xyz, m);
   if (st) return st;
@@ -1940,7 +2048,15 @@ int orc_fixed_density_gradient(int nat, const int* z, const double* xyz, const d
             for (int si = 0; si < nb; ++si) {
               const Dual& dw = t.w[pack(mu, nu) * t.pb + pack(la, si)];
               double coul = P[(size_t)(oa + mu) * n + oa + nu] * P[(size_t)(ob + la) * n + ob + si];
-              double exch = -0.5 * P[(size_t)(oa + mu) * n + ob + la] * P[(size_t)(oa + nu) * n + ob + si];
+              double exch;
+              if (S) {
+                const size_t i1 = (size_t)(oa + mu) * n + ob + la, i2 = (size_t)(oa + nu) * n + ob + si;
+                const double a1 = 0.5 * (P[i1] + S[i1]), a2 = 0.5 * (P[i2] + S[i2]);
+                const double b1 = 0.5 * (P[i1] - S[i1]), b2 = 0.5 * (P[i2] - S[i2]);
+                exch = -(a1 * a2 + b1 * b2);
+              } else {
+                exch = -0.5 * P[(size_t)(oa + mu) * n + ob + la] * P[(size_t)(oa + nu) * n + ob + si];
+              }
               double coef = coul + exch;
               for (int ax = 0; ax < 3; ++ax) f[ax] += coef * dw.d[ax];
             }
@@ -1952,18 +2068,33 @@ int orc_fixed_density_gradient(int nat, const int* z, const double* xyz, const d
   return OK;
 }
 
-// closed_form_gradient (src/gradient.rs:201-276) for H/C/O-type systems: SCF, then the
-// fixed-density gradient at the converged density.
-int orc_gradient(int nat, const int* z, const double* xyz, double charge, const Options* opt, Result* res, double* grad,
-                 double* density) {
+// closed_form_gradient (src/gradient.rs:201-276): run_pm3, then the fixed-density gradient at the
+// converged density (the spin-resolved one for an unrestricted result, :242-256), then the field's
+// -q_A f (add_field_gradient, :296-301).
+int orc_gradient_pm3(int nat, const int* z, const double* xyz, double charge, int multiplicity, const Options* opt,
+                     Result* res, double* grad, double* density) {
   Mol m;
   int st = make_mol(nat, z, xyz, m);
   if (st) return st;
-  std::vector<double> P((size_t)m.nao * m.nao);
-  st = orc_run(nat, z, xyz, charge, opt, res, P.data(), nullptr, nullptr, nullptr);
+  const size_t nn = (size_t)m.nao * m.nao;
+  std::vector<double> P(nn), S(nn), q(nat);
+  Result r;
+  st = run_out(nat, z, xyz, charge, multiplicity, opt, nullptr, &r, P.data(), S.data(), nullptr, nullptr, nullptr,
+               nullptr, q.data());
+  if (res) *res = r;
   if (st) return st;
-  if (density) std::memcpy(density, P.data(), sizeof(double) * P.size());
-  return orc_fixed_density_gradient(nat, z, xyz, P.data(), grad);
+  if (density) std::memcpy(density, P.data(), sizeof(double) * nn);
+  st = orc_fixed_density_gradient_spin(nat, z, xyz, P.data(), r.unrestricted ? S.data() : nullptr, grad);
+  if (st) return st;
+  if (opt->has_field)
+    for (int i = 0; i < nat; ++i)
+      for (int ax = 0; ax < 3; ++ax) grad[3 * i + ax] -= opt->field[ax] * q[i];
+  return OK;
+}
+
+int orc_gradient(int nat, const int* z, const double* xyz, double charge, const Options* opt, Result* res, double* grad,
+                 double* density) {
+  return orc_gradient_pm3(nat, z, xyz, charge, 1, opt, res, grad, density);
 }
 
 // A batch of independent single points spread over host threads (one solve per thread),
