@@ -37,10 +37,15 @@ typedef enum {
   PM3_INVALID_INPUT = 1,       /* Pm3Error::InvalidInput */
   PM3_MISSING_ELEMENT = 2,     /* Pm3Error::MissingElement(z) */
   PM3_SCF_NOT_CONVERGED = 3,   /* Pm3Error::ScfNotConverged{iterations, error} */
-  PM3_UNSUPPORTED = 4,         /* open shells, d elements, n >= 4 overlaps: not in this engine */
-  PM3_RESOURCE_LIMIT = 5,      /* Pm3Error::ResourceLimit */
+  PM3_UNSUPPORTED = 4,         /* d elements, He/Ne, n >= 4 overlaps: Pm3Error::UnsupportedDOrbitals(z) and
+                                  the elements this engine has no closed-form overlap for */
+  PM3_RESOURCE_LIMIT = 5,      /* Pm3Error::ResourceLimit{operation, required_mb, limit_mb} */
   PM3_DEVICE_ERROR = 6         /* HIP runtime failure (no reference counterpart) */
 } pm3_status;
+
+/* Reference (src/scf.rs:120-129): AUTO picks RHF for n_alpha == n_beta and UHF otherwise, RHF refuses an
+ * open shell (PM3_INVALID_INPUT), UHF takes the unrestricted path even for a singlet. */
+typedef enum { PM3_REFERENCE_AUTO = 0, PM3_REFERENCE_RHF = 1, PM3_REFERENCE_UHF = 2 } pm3_reference;
 
 typedef enum { PM3_ACCEL_NONE = 0, PM3_ACCEL_CDIIS = 1, PM3_ACCEL_ADIIS_CDIIS = 2 } pm3_accelerator;
 /* ScfGuess (src/scf.rs:133-158) plus the learned initial guess of this engine. */
@@ -59,8 +64,10 @@ typedef struct {
 
 struct pm3_guess_model; /* trained network of the learned guess, see pm3_guess.h */
 
-/* The fields of Pm3Options (src/scf.rs:160-287) that the molecular RHF path reads, plus the
- * network that PM3_GUESS_LEARNED uses. */
+/* The fields of Pm3Options (src/scf.rs:160-287) that the molecular single point reads, plus the
+ * network that PM3_GUESS_LEARNED uses.  The fields that only the reference's Hessian, periodic,
+ * correction-variant and d-element paths read (variant, mmok, d_penalty_*, hessian_*, cphf_max_iter)
+ * have no counterpart here; charge and multiplicity live in pm3_molecule. */
 typedef struct {
   int32_t max_scf;        /* default 200 */
   double e_tol;           /* default 1e-8 eV */
@@ -72,18 +79,31 @@ typedef struct {
   int32_t stability;      /* pm3_stability, default AUTO */
   int32_t scf_memory_mb;  /* default 512: sets the DIIS depth (src/scf.rs:309-327) */
   const struct pm3_guess_model* guess_model; /* required by PM3_GUESS_LEARNED, default NULL */
+  double level_shift_ev;  /* default 0.  With damping non-zero it "steers" the first path and switches the
+                             stability search off (src/scf.rs:563-607); the unrestricted loop applies
+                             F + shift (I - P_spin) and damping, the restricted loop reads neither */
+  double damping;         /* default 0, in [0, 1) */
+  int32_t reference;      /* pm3_reference, default AUTO */
+  int32_t has_field;      /* default 0; 1: the uniform field below is on (Pm3Options::field = Some) */
+  double field[3];        /* eV per Bohr per elementary charge, referenced to the coordinate origin */
+  int32_t integral_memory_mb; /* pair-cache ceiling in MiB (src/hamiltonian.rs:202-211); default 0 = the
+                             process default, PM3_MAX_PAIR_CACHE_MB or 4096 (a value 0 there disables it) */
 } pm3_options;
 
-/* Scalars of Pm3Result (src/scf.rs:352-396). */
+/* Scalars of Pm3Result (src/scf.rs:352-396), plus the detail of a failure. */
 typedef struct {
   double total_ev, electronic_ev, core_ev, heat_of_formation_kcal;
   double dipole_debye[3], dipole_magnitude;
-  double homo_ev, lumo_ev;
-  double last_change;          /* RMS density change of the last iteration */
+  double homo_ev, lumo_ev;     /* over both spin channels for an unrestricted result (src/scf.rs:797-820) */
+  double last_change;          /* RMS density change of the last iteration (the sum over spins for UHF) */
   double scf_improvement_ev;
   int32_t iterations, converged, scf_paths_tried, n_occ, nao, has_homo, has_lumo;
   int32_t guess_used;          /* 0 superposition / other rule, 1 learned, 2 learned rejected
                                   (first-iteration commutator norm above the superposition's) */
+  int32_t unrestricted;        /* Pm3Result::unrestricted */
+  int32_t n_beta;              /* Pm3Result::n_beta (= n_occ for a restricted result) */
+  int32_t missing_z;           /* PM3_MISSING_ELEMENT / PM3_UNSUPPORTED: the atomic number refused */
+  double required_mb, limit_mb; /* PM3_RESOURCE_LIMIT: the pair cache's size and the ceiling, MiB */
 } pm3_result;
 
 typedef struct pm3_params pm3_params; /* opaque, immutable once built */
@@ -98,24 +118,28 @@ void pm3_options_default(pm3_options* out);
 /* Number of AOs of the molecule (Basis::build, src/basis.rs:39-62); -1 for an unknown element. */
 int32_t pm3_query_nao(const pm3_molecule* mol, const pm3_params* params);
 
-/* run_pm3 (src/scf.rs:450-848).  Optional outputs may be NULL: density[n*n], mo_energies[n],
- * mo_coeff[n*n], charges[n_atoms]. */
+/* run_pm3 (src/scf.rs:450-848).  Optional outputs may be NULL: density[n*n] (total), mo_energies[n],
+ * mo_coeff[n*n] (alpha for an unrestricted result), charges[n_atoms], and for an unrestricted result
+ * spin_density[n*n] (P_alpha - P_beta), mo_energies_beta[n], mo_coeff_beta[n*n] (zero-filled for a
+ * restricted one). */
 pm3_status pm3_run(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts, pm3_result* result,
-                   double* density, double* mo_energies, double* mo_coeff, double* charges);
+                   double* density, double* mo_energies, double* mo_coeff, double* charges, double* spin_density,
+                   double* mo_energies_beta, double* mo_coeff_beta);
 
 /* closed_form_gradient (src/gradient.rs:201-276): SCF plus the Hellmann-Feynman gradient at the
  * converged density.  gradient[3*n_atoms] in eV/Bohr; density may be NULL. */
 pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
                         pm3_result* result, double* gradient, double* density);
 
-/* Batched solve of many independent closed-shell molecules in one persistent launch.  Atoms of
- * molecule k are [atom_ptr[k], atom_ptr[k+1]) of z / xyz_bohr; charge may be NULL (all neutral).
- * Per molecule: status[k] (pm3_status), results[k].  want_gradient != 0 also fills
- * gradient[3 * atom_ptr[k] ...] (may be NULL otherwise).  Returns PM3_OK when the launch
+/* Batched solve of many independent molecules in one persistent launch.  Atoms of molecule k are
+ * [atom_ptr[k], atom_ptr[k+1]) of z / xyz_bohr; charge may be NULL (all neutral), multiplicity may
+ * be NULL (all singlets).  Per molecule: status[k] (pm3_status), results[k].  want_gradient != 0
+ * also fills gradient[3 * atom_ptr[k] ...] (may be NULL otherwise).  Returns PM3_OK when the launch
  * itself succeeded; per-molecule failures are reported in status[]. */
 pm3_status pm3_run_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
-                         const double* charge, const pm3_params* params, const pm3_options* opts,
-                         int32_t want_gradient, pm3_status* status, pm3_result* results, double* gradient);
+                         const double* charge, const int32_t* multiplicity, const pm3_params* params,
+                         const pm3_options* opts, int32_t want_gradient, pm3_status* status, pm3_result* results,
+                         double* gradient);
 
 /* Device-resident batch: pm3_batch_create uploads the molecules and allocates the workgroup
  * scratch once; pm3_batch_solve is the persistent launch alone (inputs already in HBM);
@@ -136,11 +160,14 @@ typedef struct pm3_batch pm3_batch;
  * shorter.  Molecules on the multi-kernel path keep the options' guess. */
 enum { PM3_BATCH_WANT_DENSITY = 1, PM3_BATCH_WANT_GRADIENT = 2, PM3_BATCH_FORCE_LARGE = 4, PM3_BATCH_WARM_START = 8 };
 pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
-                            const double* charge, const pm3_params* params, const pm3_options* opts, int32_t flags,
-                            pm3_batch** out);
+                            const double* charge, const int32_t* multiplicity, const pm3_params* params,
+                            const pm3_options* opts, int32_t flags, pm3_batch** out);
 pm3_status pm3_batch_solve(pm3_batch* batch);
 pm3_status pm3_batch_results(pm3_batch* batch, pm3_status* status, pm3_result* results, double* charges,
                              double* density);
+/* Spin densities P_alpha - P_beta of the last solve (a PM3_BATCH_WANT_DENSITY batch), in the layout of
+ * pm3_batch_results' densities; zeros for restricted results. */
+pm3_status pm3_batch_spin_densities(pm3_batch* batch, double* spin_density);
 void pm3_batch_destroy(pm3_batch* batch);
 /* New positions (Bohr, the creation layout xyz_bohr[3 * (atom_ptr[k] + i) + c], the same atoms)
  * for every molecule of the batch; the next pm3_batch_solve computes integrals, SCF and gradients

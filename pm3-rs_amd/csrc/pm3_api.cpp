@@ -70,6 +70,14 @@ bool engine_options(const pm3_options* o, EngineOptions& e, std::string& why) {
   if (o->accelerator < 0 || o->accelerator > 2) { why = "unknown accelerator"; return false; }
   if (o->guess < 0 || o->guess > 4) { why = "unknown guess"; return false; }
   if (o->stability < 0 || o->stability > 2) { why = "unknown stability mode"; return false; }
+  if (o->reference < 0 || o->reference > 2) { why = "unknown reference"; return false; }
+  if (!(o->damping >= 0.0 && o->damping < 1.0)) { why = "damping must be in [0, 1)"; return false; }
+  if (!std::isfinite(o->level_shift_ev)) { why = "level shift must be finite"; return false; }
+  if (o->has_field && !(std::isfinite(o->field[0]) && std::isfinite(o->field[1]) && std::isfinite(o->field[2]))) {
+    why = "field must be finite";
+    return false;
+  }
+  if (o->integral_memory_mb < 0) { why = "integral_memory_mb must be >= 0"; return false; }
   e.max_scf = o->max_scf;
   e.e_tol = o->e_tol;
   e.p_tol = o->p_tol;
@@ -83,6 +91,13 @@ bool engine_options(const pm3_options* o, EngineOptions& e, std::string& why) {
   }
   e.stability = o->stability;
   e.scf_memory_mb = o->scf_memory_mb;
+  e.use_diis = o->use_diis ? 1 : 0;
+  e.level_shift = o->level_shift_ev;
+  e.damping = o->damping;
+  e.reference = o->reference;
+  e.has_field = o->has_field ? 1 : 0;
+  for (int x = 0; x < 3; ++x) e.field[x] = o->has_field ? o->field[x] : 0.0;
+  e.integral_memory_mb = o->integral_memory_mb;
   return true;
 }
 
@@ -106,23 +121,54 @@ void fill_result(const MolResult& r, pm3_result* out) {
   out->n_occ = r.nocc;
   out->nao = r.nao;
   out->guess_used = r.guess_used;
+  out->unrestricted = r.unrestricted;
+  out->n_beta = r.nbeta;
+  out->missing_z = r.missing_z;
+  out->required_mb = r.required_mb;
+  out->limit_mb = r.limit_mb;
+  if (r.unrestricted) {
+    out->has_homo = r.nocc >= 1 || r.nbeta >= 1;
+    out->has_lumo = r.nocc < r.nao || r.nbeta < r.nao;
+  }
 }
 
 pm3_status check_molecule(const pm3_molecule* mol) {
   if (!mol || mol->n_atoms < 1 || !mol->z || !mol->xyz_bohr) return fail(PM3_INVALID_INPUT, "empty molecule");
   if (mol->multiplicity < 1) return fail(PM3_INVALID_INPUT, "multiplicity must be >= 1");
-  if (mol->multiplicity != 1) return fail(PM3_UNSUPPORTED, "open-shell (UHF) molecules are not handled by this engine");
   for (int i = 0; i < 3 * mol->n_atoms; ++i)
     if (!std::isfinite(mol->xyz_bohr[i])) return fail(PM3_INVALID_INPUT, "non-finite coordinate");
   return PM3_OK;
 }
 
+// The error text of a molecule's status, with the details Pm3Error carries (src/error.rs:12-47).
 pm3_status molecule_status(int st, const MolResult& r) {
   if (st == PM3_OK) return PM3_OK;
   if (st == PM3_SCF_NOT_CONVERGED)
     return fail(PM3_SCF_NOT_CONVERGED, "SCF did not converge in " + std::to_string(r.iterations) +
                                            " iterations (last RMS density change " + std::to_string(r.last_change) + ")");
+  if (st == PM3_MISSING_ELEMENT)
+    return fail(PM3_MISSING_ELEMENT, "no PM3 parameters for element " + std::to_string(r.missing_z));
+  if (st == PM3_RESOURCE_LIMIT && r.limit_mb > 0.0)
+    return fail(PM3_RESOURCE_LIMIT, "PM3 two-electron pair cache needs " + std::to_string((long long)r.required_mb) +
+                                        " MiB, above the limit of " + std::to_string((long long)r.limit_mb) + " MiB");
   return fail((pm3_status)st, status_text(st));
+}
+
+// Every atom's electron count and the multiplicity's parity, and RHF on an open shell, are input
+// errors the engine also reports per molecule; this gives run_pm3's message for the RHF case.
+pm3_status reference_check(const pm3_molecule* mol, const pm3_params* params, const EngineOptions& eo) {
+  if (eo.reference != 1 || mol->multiplicity == 1) return PM3_OK;
+  double ne = -mol->charge;
+  for (int i = 0; i < mol->n_atoms; ++i) {
+    const int z = mol->z[i];
+    if (z < 0 || z >= 128 || params->p.slot_of_z[z] < 0) return PM3_OK;  // reported as a missing element
+    ne += params->p.table.el[params->p.slot_of_z[z]].tore;
+  }
+  const long n = std::lround(ne), nu = mol->multiplicity - 1;
+  if (n - nu >= 0 && (n - nu) % 2 == 0)
+    return fail(PM3_INVALID_INPUT, "RHF requested for an open-shell system (n_alpha=" + std::to_string((n + nu) / 2) +
+                                       " != n_beta=" + std::to_string((n - nu) / 2) + "); use UHF or Auto");
+  return PM3_OK;
 }
 
 }  // namespace
@@ -152,6 +198,12 @@ void pm3_options_default(pm3_options* out) {
   out->stability = PM3_STABILITY_AUTO;
   out->scf_memory_mb = 512;
   out->guess_model = nullptr;
+  out->level_shift_ev = 0.0;
+  out->damping = 0.0;
+  out->reference = PM3_REFERENCE_AUTO;
+  out->has_field = 0;
+  out->field[0] = out->field[1] = out->field[2] = 0.0;
+  out->integral_memory_mb = 0;
 }
 
 int32_t pm3_query_nao(const pm3_molecule* mol, const pm3_params* params) {
@@ -166,7 +218,8 @@ int32_t pm3_query_nao(const pm3_molecule* mol, const pm3_params* params) {
 }
 
 pm3_status pm3_run(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts, pm3_result* result,
-                   double* density, double* mo_energies, double* mo_coeff, double* charges) {
+                   double* density, double* mo_energies, double* mo_coeff, double* charges, double* spin_density,
+                   double* mo_energies_beta, double* mo_coeff_beta) {
   g_last_error.clear();
   if (!params) return fail(PM3_INVALID_INPUT, "null parameter set");
   pm3_status s = check_molecule(mol);
@@ -174,28 +227,38 @@ pm3_status pm3_run(const pm3_molecule* mol, const pm3_params* params, const pm3_
   EngineOptions eo;
   std::string why;
   if (!engine_options(opts, eo, why)) return fail(PM3_INVALID_INPUT, why);
-  std::vector<MolIn> mols{MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge}};
+  if ((s = reference_check(mol, params, eo)) != PM3_OK) return s;
+  std::vector<MolIn> mols{MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge, mol->multiplicity}};
   RunRequest req;
-  req.want_density = density != nullptr;
-  req.want_orbitals = mo_energies != nullptr || mo_coeff != nullptr;
+  req.want_density = density != nullptr || spin_density != nullptr;
+  req.want_orbitals = mo_energies != nullptr || mo_coeff != nullptr || mo_energies_beta != nullptr ||
+                      mo_coeff_beta != nullptr;
   std::vector<MolResult> out;
   std::string err;
   if (!engine_run(params->p, mols, eo, req, out, err)) return fail(PM3_DEVICE_ERROR, err);
   const MolResult& r = out[0];
   if (result) fill_result(r, result);
   if (r.status == PM3_OK || r.status == PM3_SCF_NOT_CONVERGED) {
-    const size_t n2 = (size_t)r.nao * r.nao;
-    if (density && r.density.size() == n2) std::memcpy(density, r.density.data(), n2 * sizeof(double));
-    if (mo_coeff && r.mo_coeff.size() == n2) std::memcpy(mo_coeff, r.mo_coeff.data(), n2 * sizeof(double));
-    if (mo_energies && (int)r.mo_energies.size() == r.nao) std::memcpy(mo_energies, r.mo_energies.data(), r.nao * sizeof(double));
-    if (charges && (int)r.charges.size() == mol->n_atoms) std::memcpy(charges, r.charges.data(), mol->n_atoms * sizeof(double));
+    const size_t n2 = (size_t)r.nao * r.nao, n = (size_t)r.nao;
+    auto put = [](double* dst, const std::vector<double>& v, size_t count) {
+      if (!dst) return;
+      if (v.size() == count) std::memcpy(dst, v.data(), count * sizeof(double));
+      else std::memset(dst, 0, count * sizeof(double));
+    };
+    put(density, r.density, n2);
+    put(mo_coeff, r.mo_coeff, n2);
+    put(mo_energies, r.mo_energies, n);
+    put(charges, r.charges, (size_t)mol->n_atoms);
+    put(spin_density, r.spin_density, n2);
+    put(mo_energies_beta, r.mo_energies_beta, n);
+    put(mo_coeff_beta, r.mo_coeff_beta, n2);
   }
   return molecule_status(r.status, r);
 }
 
 pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
-                            const double* charge, const pm3_params* params, const pm3_options* opts, int32_t flags,
-                            pm3_batch** out) {
+                            const double* charge, const int32_t* multiplicity, const pm3_params* params,
+                            const pm3_options* opts, int32_t flags, pm3_batch** out) {
   g_last_error.clear();
   if (!out) return fail(PM3_INVALID_INPUT, "null batch handle");
   *out = nullptr;
@@ -213,7 +276,8 @@ pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_
   h->params = params;
   for (int k = 0; k < n_mol; ++k) {
     const int a0 = atom_ptr[k];
-    h->mols.push_back(MolIn{atom_ptr[k + 1] - a0, h->z.data() + a0, h->xyz.data() + 3 * (size_t)a0, charge ? charge[k] : 0.0});
+    h->mols.push_back(MolIn{atom_ptr[k + 1] - a0, h->z.data() + a0, h->xyz.data() + 3 * (size_t)a0,
+                            charge ? charge[k] : 0.0, multiplicity ? multiplicity[k] : 1});
   }
   RunRequest req;
   req.want_density = (flags & PM3_BATCH_WANT_DENSITY) != 0;
@@ -259,6 +323,23 @@ pm3_status pm3_batch_results(pm3_batch* h, pm3_status* status, pm3_result* resul
     const size_t n2 = (size_t)r.nao * r.nao;
     if (density && r.density.size() == n2 && n2 > 0) std::memcpy(density + moff, r.density.data(), n2 * sizeof(double));
     aoff += h->mols[k].n_atoms;
+    moff += n2;
+  }
+  return PM3_OK;
+}
+
+pm3_status pm3_batch_spin_densities(pm3_batch* h, double* spin_density) {
+  g_last_error.clear();
+  if (!h || !spin_density) return fail(PM3_INVALID_INPUT, "null argument");
+  if (!h->want_density) return fail(PM3_INVALID_INPUT, "spin densities need PM3_BATCH_WANT_DENSITY at creation");
+  pm3_status s = pm3_batch_results(h, nullptr, nullptr, nullptr, nullptr);
+  if (s != PM3_OK) return s;
+  size_t moff = 0;
+  for (size_t k = 0; k < h->mols.size(); ++k) {
+    const MolResult& r = h->results[k];
+    const size_t n2 = (size_t)r.nao * r.nao;
+    if (r.spin_density.size() == n2 && n2 > 0) std::memcpy(spin_density + moff, r.spin_density.data(), n2 * sizeof(double));
+    else if (n2 > 0) std::memset(spin_density + moff, 0, n2 * sizeof(double));
     moff += n2;
   }
   return PM3_OK;
@@ -362,11 +443,12 @@ void pm3_batch_destroy(pm3_batch* h) {
 }
 
 pm3_status pm3_run_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
-                         const double* charge, const pm3_params* params, const pm3_options* opts, int32_t want_gradient,
-                         pm3_status* status, pm3_result* results, double* gradient) {
+                         const double* charge, const int32_t* multiplicity, const pm3_params* params,
+                         const pm3_options* opts, int32_t want_gradient, pm3_status* status, pm3_result* results,
+                         double* gradient) {
   if (want_gradient && !gradient) return fail(PM3_INVALID_INPUT, "gradient buffer required");
   pm3_batch* h = nullptr;
-  pm3_status s = pm3_batch_create(n_mol, atom_ptr, z, xyz_bohr, charge, params, opts,
+  pm3_status s = pm3_batch_create(n_mol, atom_ptr, z, xyz_bohr, charge, multiplicity, params, opts,
                                   want_gradient ? PM3_BATCH_WANT_GRADIENT : 0, &h);
   if (s != PM3_OK) return s;
   s = pm3_batch_solve(h);
@@ -385,7 +467,8 @@ pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const
   EngineOptions eo;
   std::string why;
   if (!engine_options(opts, eo, why)) return fail(PM3_INVALID_INPUT, why);
-  std::vector<MolIn> mols{MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge}};
+  if ((s = reference_check(mol, params, eo)) != PM3_OK) return s;
+  std::vector<MolIn> mols{MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge, mol->multiplicity}};
   RunRequest req;
   req.want_density = true;
   req.want_gradient = true;

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
+#include <mutex>
 #include <numeric>
 
 #include <rocblas/rocblas.h>
@@ -128,6 +129,10 @@ struct KArgs {
   int accel;
   double adiis_switch;
   int guess, stability, scf_memory_mb;
+  int use_diis;           // Pm3Options::use_diis (the unrestricted rung's CDIIS needs it apart from accel)
+  double shift, damp;     // Pm3Options::level_shift_ev, ::damping: non-zero steers the first path
+  int has_field;          // a uniform external field f (eV / Bohr / e) in H_core, core energy and gradient
+  double field[3];
 };
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
@@ -625,6 +630,28 @@ __device__ inline void pair_integrals(const KArgs& A, const Mol& M, int p) {
   }
 }
 
+// The external field's one-electron operator +M.f on atom a's block (field_terms, src/dipole.rs:
+// 156-181, dipole_matrix :93-125 at the coordinate origin): f.R_a on every diagonal element, f_x dd
+// on the s-p_x pair of elements; it enters H_core after U and before the pair terms
+// (src/hamiltonian.rs:224-233).
+__device__ inline double field_term(const KArgs& A, const ElemConst& e, int ga, int mu, int nu) {
+  if (mu == nu) return A.field[0] * A.ax[ga] + A.field[1] * A.ay[ga] + A.field[2] * A.az[ga];
+  if (mu == 0) return A.field[nu - 1] * e.dd;
+  if (nu == 0) return A.field[mu - 1] * e.dd;
+  return 0.0;
+}
+
+// The field's nuclear term -sum_a Z_a R_a.f (src/dipole.rs:176-179), summed by work-items t0, t0 + st.
+__device__ inline double field_nuclear(const KArgs& A, const Mol& M, int t0, int st) {
+  double v = 0.0;
+  if (!A.has_field) return v;
+  for (int a = t0; a < M.nat; a += st) {
+    const int ga = M.atom0 + a;
+    v -= c_tab.el[A.aslot[ga]].tore * (A.field[0] * A.ax[ga] + A.field[1] * A.ay[ga] + A.field[2] * A.az[ga]);
+  }
+  return v;
+}
+
 // H_core diagonal blocks (U plus the attraction to every other core, summed in atom order) and the
 // total core-core energy, from the per-pair slots.  One workgroup per molecule.
 // Diagonal atom blocks of H_core: U plus the electron-core attractions summed over partners in a
@@ -644,7 +671,8 @@ __device__ inline void core_diag(const KArgs& A, const Mol& M, int t0, int st) {
         if (b == a) continue;
         s += b > a ? M.S1[q * npairs + tri_index(a, b, nat)] : M.S2[q * npairs + tri_index(b, a, nat)];
       }
-      const double u = mu != nu ? 0.0 : (mu == 0 ? ea.uss : ea.upp);
+      double u = mu != nu ? 0.0 : (mu == 0 ? ea.uss : ea.upp);
+      if (A.has_field) u += field_term(A, ea, ga, mu, nu);
       M.H[(oa + mu) * np + oa + nu] = u + s;
     }
   }
@@ -654,7 +682,7 @@ __device__ double finish_core(const KArgs& A, const Mol& M, double* red) {
   const int tid = threadIdx.x;
   const long long npairs = (long long)M.nat * (M.nat - 1) / 2;
   core_diag(A, M, tid, NT);
-  double v[1] = {0.0};
+  double v[1] = {field_nuclear(A, M, tid, NT)};
   for (long long p = tid; p < npairs; p += NT) v[0] += M.ecc[p];
   block_sum<1>(v, red);
   __syncthreads();
@@ -2402,8 +2430,21 @@ struct PathResult {
 // One restricted SCF path (rhf_loop, src/scf.rs:1179-1316).  Leaves P, eps, Cn (orbitals of the
 // last extrapolated Fock) in B; with A.purify each density is the purified aufbau projector of
 // the extrapolated Fock matrix and only eps[nocc - 1], eps[nocc] (its frontier levels) are set.
+//
+// urung: the unrestricted iteration of a closed shell (uhf_loop, src/scf.rs:1319-1496, with
+// n_alpha = n_beta and a spin-symmetric guess), which the stability search runs as its last two
+// rungs (src/scf.rs:662-673).  Both spin densities start equal (the scaled guess is P/2 each) and
+// every step treats them alike, so they stay equal bit for bit and the path is carried here on the
+// total density P = 2 P_a.  What differs from the restricted loop: CDIIS only (accel 0 or 1, no
+// A-DIIS; the paired Gram of [E_a; E_b] is half the restricted one, and the scale-normalised
+// coefficients are the same), a history depth for four matrices per slot, the level shift
+// F + shift (I - P/2) on the extrapolated Fock matrix (src/scf.rs:337-350, :1413-1418), the sum of
+// the two spin RMS changes (= the RMS change of P), and damping P <- (1 - damp) P_new + damp P
+// (:1439-1448).  The undamped aufbau projector of the last step stays in B.Pn for the frontier
+// levels; their eigenvalues are those of the shifted matrix, as the reference reports them.
 __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int guess, int accel, double* lds,
-                               double* red, Diis& D, bool tight = true, bool warm = false) {
+                               double* red, Diis& D, bool tight = true, bool warm = false, double shift = 0.0,
+                               double damp = 0.0, bool urung = false) {
   const int n = M.n, np = M.np, tid = threadIdx.x;
   if (tid == 0) {
     B.bnd[2] = -1.0;  // no spectral-bound estimate for this path yet
@@ -2414,10 +2455,11 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
   // DIIS depth from the memory budget (src/scf.rs:309-327, :1205-1228)
   const long long n2 = (long long)n * n * 8;
   const long long budget = (long long)A.scf_memory_mb * 1024 * 1024;
+  if (urung && accel == 2) accel = 1;
   int keep_d = accel == 2;
   if (keep_d && !(A.scf_memory_mb == 0 || 2 * 3 * n2 <= budget)) keep_d = 0;
   if (accel == 2 && !keep_d) accel = 1;
-  const long long per = (keep_d ? 3 : 2) * n2;
+  const long long per = (urung ? 4 : keep_d ? 3 : 2) * n2;
   int depth = MAXD;
   if (A.scf_memory_mb != 0 && per != 0) depth = (int)max(2LL, min((long long)MAXD, budget / per));
   if (tid == 0) {
@@ -2590,6 +2632,14 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       stamp(B, 9);
     }
     stamp(B, 2);
+    if (shift != 0.0) {  // level shift on the virtual space, F + shift (I - P / 2)
+      for (int e = tid; e < n * n; e += NT) {
+        const int i = e / n, j = e % n;
+        B.Fu[i * np + j] = Fuse[i * np + j] + shift * ((i == j ? 1.0 : 0.0) - 0.5 * B.P[i * np + j]);
+      }
+      __syncthreads();
+      Fuse = B.Fu;
+    }
     Flast = Fuse;
     if (A.purify) {
       // the frontier seed is only needed from the path's last purification: take it once the SCF is
@@ -2620,7 +2670,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
     const double de = fabs(e_elec - e_old);
     R.last_dp = dp;
     for (int e = tid; e < np * np; e += NT) {
-      B.P[e] = B.Pn[e];
+      B.P[e] = damp > 0.0 ? (1.0 - damp) * B.Pn[e] + damp * B.P[e] : B.Pn[e];
       if (!A.purify) B.C[e] = B.Cn[e];
     }
     __syncthreads();
@@ -2633,7 +2683,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
     }
   }
   if (A.purify) {
-    frontier_levels(M, B, Flast, B.P, lds, red);
+    frontier_levels(M, B, Flast, damp > 0.0 ? B.Pn : B.P, lds, red);
     stamp(B, 5);
   }
   build_fock(A, M, M.H, B.P, B.F, lds);
@@ -2873,6 +2923,8 @@ __global__ __launch_bounds__(256) void grad_gather_kernel(KArgs A) {
       if (y == x) continue;
       s += y > x ? -M.S1[c * npairs + tri_index(x, y, nat)] : M.S1[c * npairs + tri_index(y, x, nat)];
     }
+    // the field pulls the net charge: -q_x f (add_field_gradient, src/gradient.rs:296-301)
+    if (A.has_field) s -= A.field[c] * A.out_q[md.atom0 + x];
     A.out_g[3LL * (md.atom0 + x) + c] = s;
   }
 }
@@ -2901,9 +2953,10 @@ __global__ __launch_bounds__(256) void grad_gather_wide_kernel(KArgs A, int firs
       s2 += __shfl_down(s2, off, 64);
     }
     if (lane == 0) {
-      A.out_g[3LL * (md.atom0 + x)] = s0;
-      A.out_g[3LL * (md.atom0 + x) + 1] = s1;
-      A.out_g[3LL * (md.atom0 + x) + 2] = s2;
+      const double q = A.has_field ? A.out_q[md.atom0 + x] : 0.0;
+      A.out_g[3LL * (md.atom0 + x)] = s0 - A.field[0] * q;
+      A.out_g[3LL * (md.atom0 + x) + 1] = s1 - A.field[1] * q;
+      A.out_g[3LL * (md.atom0 + x) + 2] = s2 - A.field[2] * q;
     }
   }
 }
@@ -2984,7 +3037,12 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
     const double first_e = best.e_elec;
     int paths = 1;
     auto gap = [&]() { return (nocc > 0 && nocc < n) ? B.eps[nocc] - B.eps[nocc - 1] : 1.0e300; };
-    const bool search = A.stability == 2 || (A.stability == 1 && gap() < 6.0);
+    // the stability search (src/scf.rs:599-715): never on a steered path; for a closed shell on the
+    // restricted path only the two unrestricted rungs run (a rung runs when its unrestricted flag
+    // equals closed_shell, :676), each spin-pure by construction here and so read as the
+    // restricted state (:694-708)
+    const bool steered = A.shift != 0.0 || A.damp != 0.0;
+    const bool search = !steered && (A.stability == 2 || (A.stability == 1 && gap() < 6.0));
     if (search) {
       // keep the incumbent
       for (int e = tid; e < np * np; e += NT) {
@@ -2994,12 +3052,13 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
       for (int j = tid; j < n; j += NT) B.epsb[j] = B.eps[j];
       __syncthreads();
       const int primary = A.guess == 2 ? 2 : 1;
-      const int other = primary == 2 ? 1 : 2;
-      const int rg[5][2] = {{primary, 0}, {other, A.accel}, {other, 0}, {3, A.accel}, {3, 0}};
-      for (int k = 0; k < 5; ++k) {
+      const int rg[2][2] = {{3, 0}, {primary, A.use_diis ? 1 : 0}};
+      const double rs[2][2] = {{-2.0, 0.0}, {0.0, 0.85}};
+      for (int k = 0; k < 2; ++k) {
         ++paths;
-        PathResult c = rhf_loop(A, M, B, rg[k][0], rg[k][1], lds, red, D);
-        if (aufbau_broken(A, M, B)) c = rhf_loop(A, M, B, rg[k][0], rg[k][1], lds, red, D, false);
+        PathResult c = rhf_loop(A, M, B, rg[k][0], rg[k][1], lds, red, D, true, false, rs[k][0], rs[k][1], true);
+        if (aufbau_broken(A, M, B))
+          c = rhf_loop(A, M, B, rg[k][0], rg[k][1], lds, red, D, false, false, rs[k][0], rs[k][1], true);
         if (!c.converged) continue;
         if (!best.converged || c.e_elec < best.e_elec - 1.0e-6) {
           best = c;
@@ -3011,7 +3070,6 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
           __syncthreads();
         }
       }
-      paths += 2;  // the two unrestricted rungs of the reference ladder are not run here
       for (int e = tid; e < np * np; e += NT) {
         B.P[e] = B.Pb[e];
         B.C[e] = B.Cb[e];
@@ -3322,7 +3380,7 @@ __global__ __launch_bounds__(LG_THREADS) void lg_core_kernel(KArgs A, MolDesc md
   const Mol M = mol_of(A, md, nullptr);
   core_diag(A, M, gtid(), gstride());
   const long long npairs = (long long)M.nat * (M.nat - 1) / 2;
-  double v = 0.0;
+  double v = field_nuclear(A, M, gtid(), gstride());
   for (long long p = gtid(); p < npairs; p += gstride()) v += M.ecc[p];
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
@@ -3332,6 +3390,21 @@ __global__ __launch_bounds__(LG_THREADS) void lg_core_kernel(KArgs A, MolDesc md
     for (int q = 0; q < LG_THREADS / 64; ++q) t += part[q];
     red[blockIdx.x] = t;
   }
+}
+
+// Level shift on the virtual space, out = F + shift (I - P / 2) on the n x n block (in place when
+// out == F); the padding is left alone.
+__global__ __launch_bounds__(LG_THREADS) void lg_shift_kernel(const double* F, const double* P, double* out, int n,
+                                                              int np, double shift) {
+  for (long long e = gtid(); e < (long long)n * n; e += gstride()) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    out[(long long)i * np + j] = F[(long long)i * np + j] + shift * ((i == j ? 1.0 : 0.0) - 0.5 * P[(long long)i * np + j]);
+  }
+}
+
+// Density damping P <- (1 - damp) Pn + damp P over all m elements (zero padding stays zero).
+__global__ __launch_bounds__(LG_THREADS) void lg_damp_kernel(const double* Pn, double* P, long long m, double damp) {
+  for (long long e = gtid(); e < m; e += gstride()) P[e] = (1.0 - damp) * Pn[e] + damp * P[e];
 }
 
 // Superposition density on a zeroed P; `tilt_atom` >= 0 moves min(0.2, P_zz) from p_z to p_x on
@@ -3469,10 +3542,25 @@ __global__ __launch_bounds__(NT) void lg_epilogue_kernel(KArgs A, MolDesc md, co
 
 namespace pm3host {
 
+// Synchronous copy and fill on one stream: the stream's earlier work is ordered before them and
+// they are complete on return (the null stream is never used, so batches on other host threads
+// proceed concurrently).
+static hipError_t scopy(hipStream_t s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+  return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+static hipError_t smemset(hipStream_t s, void* dst, int value, size_t bytes) {
+  hipError_t e = hipMemsetAsync(dst, value, bytes, s);
+  return e == hipSuccess ? hipStreamSynchronize(s) : e;
+}
+
 // Device-resident batch: inputs uploaded and scratch allocated once; solve() is the launch only.
+// Every device operation of a batch goes to its own stream (created non-blocking at the default
+// priority), so calls from several host threads do not serialise on the null stream.
 struct LargeBufs {
   double *F = nullptr, *P = nullptr, *Pn = nullptr, *Fu = nullptr, *T = nullptr, *V = nullptr, *w = nullptr;
   double *ework = nullptr, *Pb = nullptr, *Vb = nullptr, *wb = nullptr, *Fh = nullptr, *Eh = nullptr, *Dh = nullptr;
+  double* Pd = nullptr;  // a damped path's final density while its undamped projector gives the frontier levels
   double *J = nullptr, *slot = nullptr, *red = nullptr, *dip = nullptr, *fin = nullptr;
   double* lz = nullptr;                          // Lanczos vectors (3 np)
   double* fs = nullptr;                          // frontier seed s and X s (np each), then ...
@@ -3506,6 +3594,8 @@ struct Batch {
   std::vector<double> h_red;
   std::vector<std::vector<int>> ao_map, atom_map;  // sorted -> caller indices
   std::vector<int> pre_status;
+  std::vector<int> pre_z;              // the refused atomic number of a status-2 / 4 molecule
+  std::vector<double> pre_req, pre_lim;  // pair-cache size and ceiling (MiB) of a status-5 molecule
   bool want_gradient = false;
   bool keep_density = false;  // the caller asked for densities: copy them back at each fetch
   bool warm_start = false;    // solves after the first start from each molecule's last density
@@ -3529,7 +3619,9 @@ struct Batch {
   float* d_Y = nullptr;
   hipEvent_t evg = nullptr, evm = nullptr;
   double last_feature_ms = 0.0, last_network_ms = 0.0;
+  hipStream_t stream = nullptr;
   ~Batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
     for (int i = 0; i < nbufs; ++i)
       if (bufs[i]) (void)hipFree(bufs[i]);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -3540,6 +3632,7 @@ struct Batch {
     if (blas) (void)rocblas_destroy_handle(blas);
     for (hipEvent_t e : {lg.evp0, lg.evp1, lg.evf0, lg.evf1})
       if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
   }
   bool ensure_guess_buffers(std::string& err) {
     if (d_X || ngp_total == 0) return true;
@@ -3565,13 +3658,16 @@ struct Batch {
   }
 };
 
+// The element table in constant memory, once per device (the table is the same for every call).
 static bool upload_table(const Params& P, std::string& err) {
-  static int uploaded_device = -1;
+  static std::mutex mu;
+  static bool uploaded[64] = {false};
   int dev = 0;
   HIP_OK(hipGetDevice(&dev));
-  if (uploaded_device != dev) {
+  std::lock_guard<std::mutex> lock(mu);
+  if (dev < 0 || dev >= 64 || !uploaded[dev]) {
     HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &P.table, sizeof(TableConst)));
-    uploaded_device = dev;
+    if (dev >= 0 && dev < 64) uploaded[dev] = true;
   }
   return true;
 }
@@ -3589,6 +3685,13 @@ static bool upload_table(const Params& P, std::string& err) {
       return false;                                                   \
     }                                                                 \
   } while (0)
+
+static int diis_depth_mats(int n, int mats, int scf_memory_mb) {
+  const long long n2 = (long long)n * n * 8, per = mats * n2;
+  const long long budget = (long long)scf_memory_mb * 1024 * 1024;
+  if (scf_memory_mb == 0 || per == 0) return MAXD;
+  return (int)std::max(2LL, std::min((long long)MAXD, budget / per));
+}
 
 static int diis_depth_for(int n, int keep_d, int scf_memory_mb) {
   const long long n2 = (long long)n * n * 8, per = (keep_d ? 3 : 2) * n2;
@@ -3615,7 +3718,7 @@ static bool large_setup(Batch& b, std::string& err) {
   L.depth = depth;
   if (!b.alloc(L.F, L.m2, err) || !b.alloc(L.P, L.m2, err) || !b.alloc(L.Pn, L.m2, err) || !b.alloc(L.Fu, L.m2, err) ||
       !b.alloc(L.T, L.m2, err) || !b.alloc(L.V, L.m2, err) || !b.alloc(L.w, np, err) || !b.alloc(L.ework, np, err) ||
-      !b.alloc(L.Pb, L.m2, err) || !b.alloc(L.Vb, L.m2, err) || !b.alloc(L.wb, np, err) ||
+      !b.alloc(L.Pb, L.m2, err) || !b.alloc(L.Vb, L.m2, err) || !b.alloc(L.wb, np, err) || !b.alloc(L.Pd, L.m2, err) ||
       !b.alloc(L.Fh, L.m2 * depth, err) || !b.alloc(L.Eh, L.m2 * depth, err) || !b.alloc(L.Dh, L.m2 * depth, err) ||
       !b.alloc(L.J, 10LL * nat, err) || !b.alloc(L.slot, slot, err) ||
       !b.alloc(L.red, (size_t)LG_BLOCKS * LG_MAXDOT, err) || !b.alloc(L.dip, 3, err) || !b.alloc(L.info, 1, err) ||
@@ -3626,11 +3729,11 @@ static bool large_setup(Batch& b, std::string& err) {
     std::vector<double> r(np);
     for (int j = 0; j < np; ++j) r[j] = seed_r(j);
     L.fr = L.fs + 2 * np;
-    HIP_OK(hipMemcpy(L.fr, r.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+    HIP_OK(scopy(b.stream, L.fr, r.data(), sizeof(double) * np, hipMemcpyHostToDevice));
   }
   b.h_red.resize((size_t)LG_BLOCKS * LG_MAXDOT);
   BLAS_OK(rocblas_create_handle(&b.blas));
-  BLAS_OK(rocblas_set_stream(b.blas, 0));
+  BLAS_OK(rocblas_set_stream(b.blas, b.stream));
   HIP_OK(hipEventCreate(&L.evp0));
   HIP_OK(hipEventCreate(&L.evp1));
   HIP_OK(hipEventCreate(&L.evf0));
@@ -3640,7 +3743,7 @@ static bool large_setup(Batch& b, std::string& err) {
 
 // Sum of the first k columns of the per-block partials, blocks in order.
 static bool large_reduce(Batch& b, int k, double* out, std::string& err) {
-  HIP_OK(hipMemcpy(b.h_red.data(), b.lg.red, sizeof(double) * (size_t)LG_BLOCKS * LG_MAXDOT, hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b.stream, b.h_red.data(), b.lg.red, sizeof(double) * (size_t)LG_BLOCKS * LG_MAXDOT, hipMemcpyDeviceToHost));
   for (int q = 0; q < k; ++q) out[q] = 0.0;
   for (int blk = 0; blk < LG_BLOCKS; ++blk)
     for (int q = 0; q < k; ++q) out[q] += b.h_red[(size_t)blk * LG_MAXDOT + q];
@@ -3648,7 +3751,7 @@ static bool large_reduce(Batch& b, int k, double* out, std::string& err) {
 }
 
 static bool large_dots(Batch& b, const Dots& D, long long m, double* out, std::string& err) {
-  hipLaunchKernelGGL(lg_dots_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, D, m, b.lg.red);
+  hipLaunchKernelGGL(lg_dots_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, D, m, b.lg.red);
   HIP_OK(hipGetLastError());
   return large_reduce(b, D.k, out, err);
 }
@@ -3656,7 +3759,7 @@ static bool large_dots(Batch& b, const Dots& D, long long m, double* out, std::s
 // Block partials written as red[block] (one column): sum them.
 static bool large_sum1(Batch& b, double& out, std::string& err) {
   std::vector<double> h(LG_BLOCKS);
-  HIP_OK(hipMemcpy(h.data(), b.lg.red, sizeof(double) * LG_BLOCKS, hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b.stream, h.data(), b.lg.red, sizeof(double) * LG_BLOCKS, hipMemcpyDeviceToHost));
   out = 0.0;
   for (double v : h) out += v;
   return true;
@@ -3664,14 +3767,14 @@ static bool large_sum1(Batch& b, double& out, std::string& err) {
 
 static bool large_fock(Batch& b, const MolDesc& d, const double* P, double* F, std::string& err) {
   LargeBufs& L = b.lg;
-  HIP_OK(hipEventRecord(L.evf0, 0));
-  hipLaunchKernelGGL(lg_fock_pairs_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, F, b.lg.slot);
-  hipLaunchKernelGGL(lg_fock_jsum_wide_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, b.lg.J,
+  HIP_OK(hipEventRecord(L.evf0, b.stream));
+  hipLaunchKernelGGL(lg_fock_pairs_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, b.args, d, P, F, b.lg.slot);
+  hipLaunchKernelGGL(lg_fock_jsum_wide_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, b.args, d, P, b.lg.J,
                      b.lg.slot);
-  hipLaunchKernelGGL(lg_fock_diag_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, P, F,
+  hipLaunchKernelGGL(lg_fock_diag_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, b.args, d, P, F,
                      (const double*)b.lg.J);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(L.evf1, 0));
+  HIP_OK(hipEventRecord(L.evf1, b.stream));
   HIP_OK(hipEventSynchronize(L.evf1));
   float ms = 0.0f;
   HIP_OK(hipEventElapsedTime(&ms, L.evf0, L.evf1));
@@ -3697,10 +3800,10 @@ static bool large_energy(Batch& b, const MolDesc& d, const double* P, const doub
 static bool large_eigen_density(Batch& b, const MolDesc& d, const double* A, double* Pdst, std::string& err) {
   LargeBufs& L = b.lg;
   const int n = d.nao, np = d.np;
-  HIP_OK(hipMemcpyAsync(L.V, A, sizeof(double) * (size_t)np * np, hipMemcpyDeviceToDevice, 0));
+  HIP_OK(hipMemcpyAsync(L.V, A, sizeof(double) * (size_t)np * np, hipMemcpyDeviceToDevice, b.stream));
   BLAS_OK(rocsolver_dsyevd(b.blas, rocblas_evect_original, rocblas_fill_upper, n, L.V, np, L.w, L.ework, L.info));
   int info = 0;
-  HIP_OK(hipMemcpy(&info, L.info, sizeof(int), hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b.stream, &info, L.info, sizeof(int), hipMemcpyDeviceToHost));
   if (info != 0) {
     err = "rocsolver_dsyevd did not converge (info " + std::to_string(info) + ")";
     return false;
@@ -3747,8 +3850,8 @@ static bool large_extremes(Batch& b, const MolDesc& d, const double* A, double& 
   for (double x : h) nn += x * x;
   nn = std::sqrt(nn);
   for (double& x : h) x /= nn;
-  HIP_OK(hipMemcpy(q, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  HIP_OK(hipMemsetAsync(qp, 0, sizeof(double) * n, 0));
+  HIP_OK(scopy(b.stream, q, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HIP_OK(hipMemsetAsync(qp, 0, sizeof(double) * n, b.stream));
   std::vector<double> al, be, nal;
   double beta = 0.0;
   const double one = 1.0, zero = 0.0;
@@ -3767,10 +3870,10 @@ static bool large_extremes(Batch& b, const MolDesc& d, const double* A, double& 
     nal.push_back(-alpha);
     be.push_back(beta);
     if (beta <= 1.0e-13 * (std::fabs(alpha) + beta)) break;
-    HIP_OK(hipMemcpyAsync(qp, q, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+    HIP_OK(hipMemcpyAsync(qp, q, sizeof(double) * n, hipMemcpyDeviceToDevice, b.stream));
     const double ib = 1.0 / beta;
     BLAS_OK(rocblas_dscal(b.blas, n, &ib, w, 1));
-    HIP_OK(hipMemcpyAsync(q, w, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+    HIP_OK(hipMemcpyAsync(q, w, sizeof(double) * n, hipMemcpyDeviceToDevice, b.stream));
   }
   const int m = (int)al.size();
   hi_est = host_tridiag_max(al, be, m);
@@ -3791,9 +3894,9 @@ static bool large_purify(Batch& b, const MolDesc& d, const double* A, double* Pd
   const int n = d.nao, np = d.np, nocc = d.nocc;
   ok = false;
   if (nocc <= 0 || nocc >= n) return true;
-  hipLaunchKernelGGL(lg_gershgorin_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, A, n, np, L.red);
+  hipLaunchKernelGGL(lg_gershgorin_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, A, n, np, L.red);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpy(b.h_red.data(), L.red, sizeof(double) * (size_t)LG_BLOCKS * LG_MAXDOT, hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b.stream, b.h_red.data(), L.red, sizeof(double) * (size_t)LG_BLOCKS * LG_MAXDOT, hipMemcpyDeviceToHost));
   double glo = 1.0e300, ghi = -1.0e300;
   for (int blk = 0; blk < LG_BLOCKS; ++blk) {
     glo = std::min(glo, b.h_red[(size_t)blk * LG_MAXDOT]);
@@ -3844,9 +3947,9 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
   const int n = d.nao, np = d.np, nocc = d.nocc;
   ok = false;
   auto sums = [&](double& tr, double& t2) -> bool {
-    hipLaunchKernelGGL(lg_finish_kernel, dim3(1), dim3(LG_BLOCKS), 0, 0, (const double*)L.red, 2, L.fin);
+    hipLaunchKernelGGL(lg_finish_kernel, dim3(1), dim3(LG_BLOCKS), 0, b.stream, (const double*)L.red, 2, L.fin);
     double h[2];
-    HIP_OK(hipMemcpy(h, L.fin, sizeof(h), hipMemcpyDeviceToHost));
+    HIP_OK(scopy(b.stream, h, L.fin, sizeof(h), hipMemcpyDeviceToHost));
     tr = h[0];
     t2 = h[1];
     return true;
@@ -3869,10 +3972,10 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
           h[((size_t)K * 3 + 2) * nb + e] = L.T + (size_t)I * bs * np + (size_t)J * bs;
         }
     }
-    HIP_OK(hipMemcpy(L.bp, h.data(), sizeof(double*) * h.size(), hipMemcpyHostToDevice));
+    HIP_OK(scopy(b.stream, L.bp, h.data(), sizeof(double*) * h.size(), hipMemcpyHostToDevice));
   }
   double tr, t2;
-  hipLaunchKernelGGL(lg_sp2_init_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, A, L.V, n, np, lo, hi, L.red);
+  hipLaunchKernelGGL(lg_sp2_init_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, A, L.V, n, np, lo, hi, L.red);
   HIP_OK(hipGetLastError());
   if (!sums(tr, t2)) return false;
   const double occ = (double)nocc, one = 1.0, zero = 0.0;
@@ -3885,7 +3988,7 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     }
     if (tail == 0) break;
     const int sq = std::fabs(t2 - occ) < std::fabs(2.0 * tr - t2 - occ) ? 1 : 0;
-    HIP_OK(hipEventRecord(L.evp0, 0));
+    HIP_OK(hipEventRecord(L.evp0, b.stream));
     // the block-lower products, or the whole square by dgemm, or one triangle by dsyrk (upper in
     // column-major order = the row-major lower triangle): the update reads only the row-major lower
     // triangle, so X stays exactly symmetric
@@ -3902,8 +4005,8 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     else
       BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, L.V, np, L.V, np,
                             &zero, L.T, np));
-    HIP_OK(hipEventRecord(L.evp1, 0));
-    hipLaunchKernelGGL(lg_sp2_update_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.T, L.V, n, np, sq,
+    HIP_OK(hipEventRecord(L.evp1, b.stream));
+    hipLaunchKernelGGL(lg_sp2_update_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)L.T, L.V, n, np, sq,
                        L.red);
     HIP_OK(hipGetLastError());
     if (!sums(tr, t2)) return false;  // (synchronises, so the product's events have completed)
@@ -3915,7 +4018,7 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     if (tail > 0) --tail;
   }
   if (tail != 0 || std::fabs(tr - occ) > 1.0e-6) return true;
-  hipLaunchKernelGGL(lg_twice_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.V, Pdst, n, np);
+  hipLaunchKernelGGL(lg_twice_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)L.V, Pdst, n, np);
   HIP_OK(hipGetLastError());
   ok = true;
   return true;
@@ -3926,10 +4029,10 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
 static bool large_levels(Batch& b, const MolDesc& d, const double* A, std::string& err) {
   LargeBufs& L = b.lg;
   const int n = d.nao, np = d.np;
-  HIP_OK(hipMemcpyAsync(L.V, A, sizeof(double) * (size_t)np * np, hipMemcpyDeviceToDevice, 0));
+  HIP_OK(hipMemcpyAsync(L.V, A, sizeof(double) * (size_t)np * np, hipMemcpyDeviceToDevice, b.stream));
   BLAS_OK(rocsolver_dsyevd(b.blas, rocblas_evect_none, rocblas_fill_upper, n, L.V, np, L.w, L.ework, L.info));
   int info = 0;
-  HIP_OK(hipMemcpy(&info, L.info, sizeof(int), hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b.stream, &info, L.info, sizeof(int), hipMemcpyDeviceToHost));
   if (info != 0) {
     err = "rocsolver_dsyevd did not converge (info " + std::to_string(info) + ")";
     return false;
@@ -3962,8 +4065,8 @@ static bool large_lanczos_top(Batch& b, const MolDesc& d, const double* A, const
   double* q = L.lz;
   double* qp = L.lz + np;
   double* w = L.lz + 2 * np;
-  HIP_OK(hipMemcpyAsync(q, st, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
-  HIP_OK(hipMemsetAsync(qp, 0, sizeof(double) * n, 0));
+  HIP_OK(hipMemcpyAsync(q, st, sizeof(double) * n, hipMemcpyDeviceToDevice, b.stream));
+  HIP_OK(hipMemsetAsync(qp, 0, sizeof(double) * n, b.stream));
   std::vector<double> al, be;
   double beta = 0.0, th1 = -1.0e300, th2 = -1.0e300;
   const double one = 1.0, zero = 0.0;
@@ -4037,9 +4140,9 @@ static bool large_levels_lanczos(Batch& b, const MolDesc& d, const double* Fu, b
   LargeBufs& L = b.lg;
   const int n = d.nao, np = d.np, nocc = d.nocc;
   aufbau = true;
-  hipLaunchKernelGGL(lg_gershgorin_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, Fu, n, np, L.red);
+  hipLaunchKernelGGL(lg_gershgorin_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, Fu, n, np, L.red);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpy(b.h_red.data(), L.red, sizeof(double) * (size_t)LG_BLOCKS * LG_MAXDOT, hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b.stream, b.h_red.data(), L.red, sizeof(double) * (size_t)LG_BLOCKS * LG_MAXDOT, hipMemcpyDeviceToHost));
   double glo = 1.0e300, ghi = -1.0e300;
   for (int blk = 0; blk < LG_BLOCKS; ++blk) {
     glo = std::min(glo, b.h_red[(size_t)blk * LG_MAXDOT]);
@@ -4055,7 +4158,7 @@ static bool large_levels_lanczos(Batch& b, const MolDesc& d, const double* Fu, b
                         L.V, np));  // X F X
   double* Ao = L.Pn;
   double* Av = L.Eh;
-  hipLaunchKernelGGL(lg_frontier_ops_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, Fu, (const double*)L.P,
+  hipLaunchKernelGGL(lg_frontier_ops_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, Fu, (const double*)L.P,
                      (const double*)L.T, (const double*)L.V, n, np, c_lo, c_hi, Ao, Av);
   HIP_OK(hipGetLastError());
   // starts: X s for the HOMO, s - X s for the LUMO (s the seed), or the generic vector
@@ -4074,7 +4177,7 @@ static bool large_levels_lanczos(Batch& b, const MolDesc& d, const double* Fu, b
   lumo = -lumo;
   aufbau = homo <= lumo;
   const double lv[2] = {homo, lumo};
-  HIP_OK(hipMemcpy(L.w + nocc - 1, lv, sizeof(lv), hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, L.w + nocc - 1, lv, sizeof(lv), hipMemcpyHostToDevice));
   return true;
 }
 
@@ -4084,14 +4187,17 @@ struct LargePath {
   bool aufbau = true;  // purified path: the final density is the aufbau projector of its Fock matrix
 };
 
-static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePath& R, std::string& err) {
+// One path of the host-driven SCF; shift, damp and urung as in the batched kernel's rhf_loop (the
+// unrestricted iteration of a closed shell, carried on the total density).
+static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePath& R, std::string& err,
+                      double shift = 0.0, double damp = 0.0, bool urung = false) {
   LargeBufs& L = b.lg;
   const int n = d.nao, np = d.np;
   const long long ms = (long long)np * np;
   const Mol M = mol_of(b.args, d, nullptr);
   const EngineOptions& o = b.opt;
-  HIP_OK(hipMemsetAsync(L.P, 0, sizeof(double) * ms, 0));
-  HIP_OK(hipMemsetAsync(L.Pn, 0, sizeof(double) * ms, 0));
+  HIP_OK(hipMemsetAsync(L.P, 0, sizeof(double) * ms, b.stream));
+  HIP_OK(hipMemsetAsync(L.Pn, 0, sizeof(double) * ms, b.stream));
   L.bnd_age = -1;  // no spectral-bound estimate for this path yet
   L.seeded = false;  // nor a frontier seed
   if (guess == 2) {
@@ -4102,20 +4208,22 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
       double best = 0.0;
       for (int a = 0; a < d.nat; ++a) {
         int slot = 0;
-        HIP_OK(hipMemcpy(&slot, b.args.aslot + d.atom0 + a, sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(scopy(b.stream, &slot, b.args.aslot + d.atom0 + a, sizeof(int), hipMemcpyDeviceToHost));
         const ElemConst& e = b.table.el[slot];
         if (e.norb == 4 && e.iop > best) { best = e.iop; tilt = a; }
       }
     }
-    hipLaunchKernelGGL(lg_sad_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, L.P, tilt);
+    hipLaunchKernelGGL(lg_sad_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, b.args, d, L.P, tilt);
     HIP_OK(hipGetLastError());
   }
+  if (urung && accel == 2) accel = 1;
   int keep_d = accel == 2;
   if (keep_d && !(o.scf_memory_mb == 0 || 2LL * 3 * n * n * 8 <= (long long)o.scf_memory_mb * 1024 * 1024)) keep_d = 0;
   if (accel == 2 && !keep_d) accel = 1;
   Diis D{};
   D.len = 0;
-  D.depth = std::min(diis_depth_for(n, keep_d, o.scf_memory_mb), L.depth);
+  D.depth = std::min(urung ? diis_depth_mats(n, 4, o.scf_memory_mb) : diis_depth_for(n, keep_d, o.scf_memory_mb),
+                     L.depth);
   D.keep_d = keep_d;
   R = LargePath();
   const double* Flast = nullptr;  // the matrix of the last aufbau step
@@ -4151,7 +4259,7 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
       // T = F P (row-major; rocBLAS is column-major and both factors are symmetric)
       BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, P, np, L.F, np, &zero,
                             L.T, np));
-      hipLaunchKernelGGL(lg_commutator_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.T,
+      hipLaunchKernelGGL(lg_commutator_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)L.T,
                          (const double*)L.F, (const double*)P, En, Fn, Dn, n, np, keep_d);
       HIP_OK(hipGetLastError());
       const int len = D.len;
@@ -4198,23 +4306,34 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
           C.c[i] = D.coef[i];
           C.slot[i] = D.slot[i];
         }
-        hipLaunchKernelGGL(lg_combine_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.Fh, ms, C, L.Fu);
+        hipLaunchKernelGGL(lg_combine_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)L.Fh, ms, C, L.Fu);
         HIP_OK(hipGetLastError());
         Fuse = L.Fu;
       } else {
         Fuse = Fn;
       }
     }
+    if (shift != 0.0) {
+      hipLaunchKernelGGL(lg_shift_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, Fuse, (const double*)P, L.Fu,
+                         n, np, shift);
+      HIP_OK(hipGetLastError());
+      Fuse = L.Fu;
+    }
     if (!large_density(b, d, Fuse, Pn, err, R.last_dp < FRONTIER_SEED_DP)) return false;
     Flast = Fuse;
-    hipLaunchKernelGGL(lg_diff2_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)Pn, (const double*)P, ms,
+    hipLaunchKernelGGL(lg_diff2_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)Pn, (const double*)P, ms,
                        L.red);
     double s2 = 0.0;
     if (!large_sum1(b, s2, err)) return false;
     const double dp = std::sqrt(s2 / (double)std::max(n * n, 1));
     const double de = std::fabs(e_elec - e_old);
     R.last_dp = dp;
-    std::swap(P, Pn);
+    if (damp > 0.0) {  // P stays L.P (damped), Pn stays L.Pn (the last aufbau projector)
+      hipLaunchKernelGGL(lg_damp_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)Pn, P, ms, damp);
+      HIP_OK(hipGetLastError());
+    } else {
+      std::swap(P, Pn);
+    }
     e_old = e_elec;
     if (it > 0 && de < o.e_tol && dp < o.p_tol) {
       R.converged = 1;
@@ -4222,18 +4341,23 @@ static bool large_rhf(Batch& b, const MolDesc& d, int guess, int accel, LargePat
     }
   }
   if (P != L.P) {  // keep the final density in L.P
-    HIP_OK(hipMemcpyAsync(L.P, P, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
+    HIP_OK(hipMemcpyAsync(L.P, P, sizeof(double) * ms, hipMemcpyDeviceToDevice, b.stream));
   }
   if (b.purify && Flast && !L.last_eig) {
     // frontier levels of the last step by Lanczos, and the aufbau test HOMO <= LUMO (a last step
     // taken by the eigensolver left every eigenvalue in L.w already); with an empty side, the
     // eigenvalues of the last Fock matrix
     bool ok = true;
+    if (damp > 0.0) {  // the levels from the undamped projector, then the damped density back
+      HIP_OK(hipMemcpyAsync(L.Pd, L.P, sizeof(double) * ms, hipMemcpyDeviceToDevice, b.stream));
+      HIP_OK(hipMemcpyAsync(L.P, L.Pn, sizeof(double) * ms, hipMemcpyDeviceToDevice, b.stream));
+    }
     if (d.nocc > 0 && d.nocc < n) {
       if (!large_levels_lanczos(b, d, Flast, ok, err)) return false;
     } else if (!large_levels(b, d, Flast, err)) {
       return false;
     }
+    if (damp > 0.0) HIP_OK(hipMemcpyAsync(L.P, L.Pd, sizeof(double) * ms, hipMemcpyDeviceToDevice, b.stream));
     R.aufbau = ok;
   }
   if (!large_fock(b, d, L.P, L.F, err)) return false;
@@ -4245,8 +4369,8 @@ static bool large_solve(Batch& b, int k, std::string& err) {
   LargeBufs& L = b.lg;
   const int n = d.nao, np = d.np, nocc = d.nocc;
   const long long ms = (long long)np * np;
-  HIP_OK(hipMemsetAsync(L.F, 0, sizeof(double) * ms, 0));
-  hipLaunchKernelGGL(lg_core_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, b.args, d, L.red);
+  HIP_OK(hipMemsetAsync(L.F, 0, sizeof(double) * ms, b.stream));
+  hipLaunchKernelGGL(lg_core_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, b.args, d, L.red);
   HIP_OK(hipGetLastError());
   double ecore = 0.0;
   if (!large_sum1(b, ecore, err)) return false;
@@ -4260,45 +4384,47 @@ static bool large_solve(Batch& b, int k, std::string& err) {
     L.no_tight = false;
   }
   std::vector<double> w(n);
-  HIP_OK(hipMemcpy(w.data(), L.w, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b.stream, w.data(), L.w, sizeof(double) * n, hipMemcpyDeviceToHost));
   const double first_e = best.e_elec;
   int paths = 1;
   const double gap = (nocc > 0 && nocc < n) ? w[nocc] - w[nocc - 1] : 1.0e300;
-  const bool search = b.opt.stability == 2 || (b.opt.stability == 1 && gap < 6.0);
+  // the stability search of the batched kernel: not when steered; a closed shell runs the two
+  // unrestricted rungs (src/scf.rs:647-708)
+  const bool steered = b.opt.level_shift != 0.0 || b.opt.damping != 0.0;
+  const bool search = !steered && (b.opt.stability == 2 || (b.opt.stability == 1 && gap < 6.0));
   if (search) {
-    HIP_OK(hipMemcpyAsync(L.Pb, L.P, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
-    HIP_OK(hipMemcpyAsync(L.Vb, L.V, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
-    HIP_OK(hipMemcpyAsync(L.wb, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+    HIP_OK(hipMemcpyAsync(L.Pb, L.P, sizeof(double) * ms, hipMemcpyDeviceToDevice, b.stream));
+    HIP_OK(hipMemcpyAsync(L.Vb, L.V, sizeof(double) * ms, hipMemcpyDeviceToDevice, b.stream));
+    HIP_OK(hipMemcpyAsync(L.wb, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, b.stream));
     const int primary = guess == 2 ? 2 : 1;
-    const int other = primary == 2 ? 1 : 2;
-    const int rg[5][2] = {{primary, 0}, {other, b.opt.accel}, {other, 0}, {3, b.opt.accel}, {3, 0}};
-    for (int r = 0; r < 5; ++r) {
+    const int rg[2][2] = {{3, 0}, {primary, b.opt.use_diis ? 1 : 0}};
+    const double rs[2][2] = {{-2.0, 0.0}, {0.0, 0.85}};
+    for (int r = 0; r < 2; ++r) {
       ++paths;
       LargePath c;
-      if (!large_rhf(b, d, rg[r][0], rg[r][1], c, err)) return false;
+      if (!large_rhf(b, d, rg[r][0], rg[r][1], c, err, rs[r][0], rs[r][1], true)) return false;
       if (!c.aufbau) {
         L.no_tight = true;
-        if (!large_rhf(b, d, rg[r][0], rg[r][1], c, err)) return false;
+        if (!large_rhf(b, d, rg[r][0], rg[r][1], c, err, rs[r][0], rs[r][1], true)) return false;
         L.no_tight = false;
       }
       if (!c.converged) continue;
       if (!best.converged || c.e_elec < best.e_elec - 1.0e-6) {
         best = c;
-        HIP_OK(hipMemcpyAsync(L.Pb, L.P, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
-        HIP_OK(hipMemcpyAsync(L.Vb, L.V, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
-        HIP_OK(hipMemcpyAsync(L.wb, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+        HIP_OK(hipMemcpyAsync(L.Pb, L.P, sizeof(double) * ms, hipMemcpyDeviceToDevice, b.stream));
+        HIP_OK(hipMemcpyAsync(L.Vb, L.V, sizeof(double) * ms, hipMemcpyDeviceToDevice, b.stream));
+        HIP_OK(hipMemcpyAsync(L.wb, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, b.stream));
       }
     }
-    paths += 2;  // the two unrestricted rungs of the reference ladder are not run here
-    HIP_OK(hipMemcpyAsync(L.P, L.Pb, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
-    HIP_OK(hipMemcpyAsync(L.V, L.Vb, sizeof(double) * ms, hipMemcpyDeviceToDevice, 0));
-    HIP_OK(hipMemcpyAsync(L.w, L.wb, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
-    HIP_OK(hipMemcpy(w.data(), L.w, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpyAsync(L.P, L.Pb, sizeof(double) * ms, hipMemcpyDeviceToDevice, b.stream));
+    HIP_OK(hipMemcpyAsync(L.V, L.Vb, sizeof(double) * ms, hipMemcpyDeviceToDevice, b.stream));
+    HIP_OK(hipMemcpyAsync(L.w, L.wb, sizeof(double) * n, hipMemcpyDeviceToDevice, b.stream));
+    HIP_OK(scopy(b.stream, w.data(), L.w, sizeof(double) * n, hipMemcpyDeviceToHost));
   }
-  hipLaunchKernelGGL(lg_epilogue_kernel, dim3(1), dim3(NT), 0, 0, b.args, d, (const double*)L.P, L.dip);
+  hipLaunchKernelGGL(lg_epilogue_kernel, dim3(1), dim3(NT), 0, b.stream, b.args, d, (const double*)L.P, L.dip);
   HIP_OK(hipGetLastError());
   double dip[3];
-  HIP_OK(hipMemcpy(dip, L.dip, sizeof(dip), hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b.stream, dip, L.dip, sizeof(dip), hipMemcpyDeviceToHost));
   MolOut mo{};
   mo.e_elec = best.e_elec;
   mo.e_core = ecore;
@@ -4312,14 +4438,14 @@ static bool large_solve(Batch& b, int k, std::string& err) {
   mo.paths = paths;
   mo.status = best.converged ? 0 : 3;
   mo.guess_used = 0;
-  HIP_OK(hipMemcpy(b.args.out + k, &mo, sizeof(MolOut), hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, b.args.out + k, &mo, sizeof(MolOut), hipMemcpyHostToDevice));
   if (d.out_mat >= 0) {
-    hipLaunchKernelGGL(lg_pack_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.P,
+    hipLaunchKernelGGL(lg_pack_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)L.P,
                        b.args.out_P + d.out_mat, n, np);
     if (!b.purify) {  // orbitals only on the eigenvector route
-      hipLaunchKernelGGL(lg_vectors_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, 0, (const double*)L.V,
+      hipLaunchKernelGGL(lg_vectors_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)L.V,
                          b.args.out_C + d.out_mat, n, np);
-      HIP_OK(hipMemcpyAsync(b.args.out_eps + d.out_vec, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, 0));
+      HIP_OK(hipMemcpyAsync(b.args.out_eps + d.out_vec, L.w, sizeof(double) * n, hipMemcpyDeviceToDevice, b.stream));
     }
     HIP_OK(hipGetLastError());
   }
@@ -4332,12 +4458,26 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   if (!upload_table(P, err)) return false;
   Batch* bp = new Batch();
   Batch& b = *bp;
+  if (hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking) != hipSuccess) {
+    err = "hipStreamCreateWithFlags failed";
+    delete bp;
+    return false;
+  }
   const int nmol = (int)mols.size();
   b.nmol = nmol;
   b.desc.resize(nmol);
   b.ao_map.resize(nmol);
   b.atom_map.resize(nmol);
   b.pre_status.assign(nmol, 0);
+  b.pre_z.assign(nmol, 0);
+  b.pre_req.assign(nmol, 0.0);
+  b.pre_lim.assign(nmol, 0.0);
+  // the pair-cache ceiling of build_core_limited (src/hamiltonian.rs:44-49, 202-211)
+  long long cache_limit_mb = opt.integral_memory_mb;
+  if (cache_limit_mb <= 0) {
+    cache_limit_mb = 4096;
+    if (const char* ev = std::getenv("PM3_MAX_PAIR_CACHE_MB")) cache_limit_mb = std::strtoll(ev, nullptr, 10);
+  }
   // Without orbitals to return, each SCF density is purified from the Fock matrix instead of
   // built from its eigenvectors (PM3_PURIFY=0 keeps the eigensolver).
   b.purify = !req.want_orbitals;
@@ -4357,14 +4497,42 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
     double nel = 0.0;
     for (int i = 0; i < m.n_atoms && !st; ++i) {
       const int z = m.z[i];
-      if (z < 0 || z >= 128 || P.slot_of_z[z] < 0) st = 2;
-      else nel += P.table.el[P.slot_of_z[z]].tore;
+      if (z < 0 || z >= 128 || P.slot_of_z[z] < 0) {
+        st = 2;
+        b.pre_z[k] = z;
+      } else {
+        nel += P.table.el[P.slot_of_z[z]].tore;
+      }
     }
     if (st) continue;
+    // electron and spin counts, restricted or unrestricted (src/scf.rs:486-520)
     nel -= m.charge;
     const long ne = std::lround(nel);
-    if (std::fabs(nel - (double)ne) > 1.0e-6 || ne < 0) { st = 1; continue; }
-    if (ne % 2 != 0) { st = 4; continue; }
+    if (std::fabs(nel - (double)ne) > 1.0e-6 || ne < 0 || m.multiplicity < 1) { st = 1; continue; }
+    const long nu = m.multiplicity - 1;
+    if (ne - nu < 0 || (ne - nu) % 2 != 0) { st = 1; continue; }
+    const int n_alpha = (int)((ne + nu) / 2), n_beta = (int)((ne - nu) / 2);
+    if (opt.reference == 1 && n_alpha != n_beta) { st = 1; continue; }
+    const bool use_uhf = opt.reference == 2 || n_alpha != n_beta;
+    if (use_uhf) { st = 4; continue; }
+    {
+      // exact size of the reference's pair cache: sum_{i<j} p_i p_j packed doubles and a 72-byte
+      // PairIntegral record per pair (pair_cache_bytes, src/hamiltonian.rs:51-71)
+      double tot = 0.0, sq = 0.0;
+      for (int i = 0; i < m.n_atoms; ++i) {
+        const int no = P.table.el[P.slot_of_z[m.z[i]]].norb;
+        const double pk_ = std::max(no * (no + 1) / 2, 1);
+        tot += pk_;
+        sq += pk_ * pk_;
+      }
+      const double need = (tot * tot - sq) / 2.0 * 8.0 + (double)m.n_atoms * (m.n_atoms - 1) / 2.0 * 72.0;
+      if (cache_limit_mb > 0 && need > (double)cache_limit_mb * 1048576.0) {
+        st = 5;
+        b.pre_req[k] = std::ceil(need / 1048576.0);
+        b.pre_lim[k] = (double)cache_limit_mb;
+        continue;
+      }
+    }
     bool coincident = false;
     for (int i = 0; i < m.n_atoms && !coincident; ++i)
       for (int j = 0; j < i; ++j) {
@@ -4530,26 +4698,28 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
       !b.alloc(d_gcnt, b.atoms_total, err) || !b.alloc(d_gsrc, gsrc.size(), err) || !b.alloc(d_gdst, gdst.size(), err) ||
       !b.alloc(d_gpart, gpart.size(), err))
     return false;
-  HIP_OK(hipMemcpy(d_elem8, elem8.data(), elem8.size(), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_gbeg, gbeg.data(), sizeof(int) * gbeg.size(), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_gcnt, gcnt.data(), sizeof(int) * gcnt.size(), hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, d_elem8, elem8.data(), elem8.size(), hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, d_gbeg, gbeg.data(), sizeof(int) * gbeg.size(), hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, d_gcnt, gcnt.data(), sizeof(int) * gcnt.size(), hipMemcpyHostToDevice));
   if (b.ngp_total > 0) {
-    HIP_OK(hipMemcpy(d_gsrc, gsrc.data(), sizeof(int) * gsrc.size(), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(d_gdst, gdst.data(), sizeof(int) * gdst.size(), hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(d_gpart, gpart.data(), sizeof(int) * gpart.size(), hipMemcpyHostToDevice));
+    HIP_OK(scopy(b.stream, d_gsrc, gsrc.data(), sizeof(int) * gsrc.size(), hipMemcpyHostToDevice));
+    HIP_OK(scopy(b.stream, d_gdst, gdst.data(), sizeof(int) * gdst.size(), hipMemcpyHostToDevice));
+    HIP_OK(scopy(b.stream, d_gpart, gpart.data(), sizeof(int) * gpart.size(), hipMemcpyHostToDevice));
   }
   b.want_gradient = req.want_gradient;
   b.keep_density = req.want_density;
   b.warm_start = req.warm_start;
-  HIP_OK(hipMemset(d_store, 0, sizeof(double) * std::max<long long>(store_total, 1)));
-  HIP_OK(hipMemcpy(d_pbase, pair_base.data(), sizeof(long long) * pair_base.size(), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_desc, b.desc.data(), sizeof(MolDesc) * nmol, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_order, b.order_all.data(), sizeof(int) * nq, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_aslot, aslot.data(), sizeof(int) * aslot.size(), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_aoff, aoff.data(), sizeof(int) * aoff.size(), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_ax, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_ay, ay.data(), sizeof(double) * ay.size(), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(d_az, az.data(), sizeof(double) * az.size(), hipMemcpyHostToDevice));
+  HIP_OK(smemset(b.stream, d_store, 0, sizeof(double) * std::max<long long>(store_total, 1)));
+  // result records start zeroed (converged = 0): a warm start never reads a record no solve wrote
+  HIP_OK(smemset(b.stream, d_out, 0, sizeof(MolOut) * std::max(nmol, 1)));
+  HIP_OK(scopy(b.stream, d_pbase, pair_base.data(), sizeof(long long) * pair_base.size(), hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, d_desc, b.desc.data(), sizeof(MolDesc) * nmol, hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, d_order, b.order_all.data(), sizeof(int) * nq, hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, d_aslot, aslot.data(), sizeof(int) * aslot.size(), hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, d_aoff, aoff.data(), sizeof(int) * aoff.size(), hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, d_ax, ax.data(), sizeof(double) * ax.size(), hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, d_ay, ay.data(), sizeof(double) * ay.size(), hipMemcpyHostToDevice));
+  HIP_OK(scopy(b.stream, d_az, az.data(), sizeof(double) * az.size(), hipMemcpyHostToDevice));
   a.mols = d_desc;
   a.order = d_order;
   a.nmol = b.nsolve;
@@ -4613,6 +4783,11 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   a.guess = opt.guess;
   a.stability = opt.stability;
   a.scf_memory_mb = opt.scf_memory_mb;
+  a.use_diis = opt.use_diis;
+  a.shift = opt.level_shift;
+  a.damp = opt.damping;
+  a.has_field = opt.has_field;
+  for (int x = 0; x < 3; ++x) a.field[x] = opt.has_field ? opt.field[x] : 0.0;
   a.eig_tol_factor = EIG_TOL_FACTOR;
   if (const char* ev = std::getenv("PM3_EIG_TOL_FACTOR")) a.eig_tol_factor = std::atof(ev);
   a.eig_rel = EIG_REL;
@@ -4659,32 +4834,32 @@ bool batch_solve(Batch* bp, std::string& err) {
     HIP_OK(hipEventCreate(&b.evg));
     HIP_OK(hipEventCreate(&b.evm));
   }
-  HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), 0));
-  if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, PH_SLOTS * sizeof(unsigned long long), 0));
-  HIP_OK(hipEventRecord(b.ev0, 0));
-  hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, 0, b.args);
+  HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), b.stream));
+  if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, PH_SLOTS * sizeof(unsigned long long), b.stream));
+  HIP_OK(hipEventRecord(b.ev0, b.stream));
+  hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, b.stream, b.args);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(b.evg, 0));
-  if (b.learned && !guess_infer(b.guess_model, infer_view(b), err, b.evm)) return false;
-  if (!b.learned) HIP_OK(hipEventRecord(b.evm, 0));
-  HIP_OK(hipEventRecord(b.ev1, 0));
+  HIP_OK(hipEventRecord(b.evg, b.stream));
+  if (b.learned && !guess_infer(b.guess_model, infer_view(b), err, b.stream, b.evm)) return false;
+  if (!b.learned) HIP_OK(hipEventRecord(b.evm, b.stream));
+  HIP_OK(hipEventRecord(b.ev1, b.stream));
   b.args.warm = b.warm_start && (b.solved_once || b.have_prev) ? 1 : 0;
-  if (b.nsolve > 0) hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, 0, b.args);
+  if (b.nsolve > 0) hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, b.stream, b.args);
   HIP_OK(hipGetLastError());
   for (int k : b.large)
     if (!large_solve(b, k, err)) return false;
   if (b.want_gradient) {
-    hipLaunchKernelGGL(grad_pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, 0, b.args);
-    if (b.nsolve > 0) hipLaunchKernelGGL(grad_gather_kernel, dim3((unsigned)b.nsolve), dim3(256), 0, 0, b.args);
+    hipLaunchKernelGGL(grad_pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, b.stream, b.args);
+    if (b.nsolve > 0) hipLaunchKernelGGL(grad_gather_kernel, dim3((unsigned)b.nsolve), dim3(256), 0, b.stream, b.args);
     if (!b.large.empty()) {
       int nat = 1;
       for (int k : b.large) nat = std::max(nat, b.desc[k].nat);
       hipLaunchKernelGGL(grad_gather_wide_kernel, dim3((unsigned)b.large.size(), (unsigned)std::min(256, (nat + 3) / 4)),
-                         dim3(256), 0, 0, b.args, b.nsolve);
+                         dim3(256), 0, b.stream, b.args, b.nsolve);
     }
     HIP_OK(hipGetLastError());
   }
-  HIP_OK(hipEventRecord(b.ev2, 0));
+  HIP_OK(hipEventRecord(b.ev2, b.stream));
   HIP_OK(hipEventSynchronize(b.ev2));
   float ms = 0.0f, pms = 0.0f, gms = 0.0f;
   HIP_OK(hipEventElapsedTime(&pms, b.ev0, b.evg));
@@ -4711,7 +4886,7 @@ bool batch_guess_view(Batch* bp, BatchGuessView& v, std::string& err) {
   std::vector<MolOut> ho;
   if (b.solved_once && !b.order_all.empty()) {
     ho.resize(b.nmol);
-    HIP_OK(hipMemcpy(ho.data(), b.args.out, sizeof(MolOut) * b.nmol, hipMemcpyDeviceToHost));
+    HIP_OK(scopy(b.stream, ho.data(), b.args.out, sizeof(MolOut) * b.nmol, hipMemcpyDeviceToHost));
   }
   v.caller_atom.assign(b.atoms_total, 0);
   for (int k : b.order_all) {
@@ -4752,35 +4927,40 @@ double batch_pair_ms(const Batch* b) { return b ? b->last_pair_ms : 0.0; }
 bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err) {
   for (int k = 0; k < PH_SLOTS; ++k) out[k] = 0;
   if (!b || !b->args.prof) return true;
-  HIP_OK(hipMemcpy(out, b->args.prof, PH_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b->stream, out, b->args.prof, PH_SLOTS * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   return true;
 }
 
 bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std::vector<MolResult>& out, std::string& err) {
   Batch& b = *bp;
   out.assign(b.nmol, MolResult());
-  for (int k = 0; k < b.nmol; ++k) out[k].status = b.pre_status[k];
+  for (int k = 0; k < b.nmol; ++k) {
+    out[k].status = b.pre_status[k];
+    out[k].missing_z = b.pre_z[k];
+    out[k].required_mb = b.pre_req[k];
+    out[k].limit_mb = b.pre_lim[k];
+  }
   if (b.order_all.empty()) return true;
   std::vector<MolOut> ho(b.nmol);
   std::vector<double> hq(b.atoms_total), hg;
   if (b.want_gradient) {
     hg.resize(3 * b.atoms_total);
-    HIP_OK(hipMemcpy(hg.data(), b.args.out_g, sizeof(double) * hg.size(), hipMemcpyDeviceToHost));
+    HIP_OK(scopy(b.stream, hg.data(), b.args.out_g, sizeof(double) * hg.size(), hipMemcpyDeviceToHost));
   }
-  HIP_OK(hipMemcpy(ho.data(), b.args.out, sizeof(MolOut) * b.nmol, hipMemcpyDeviceToHost));
-  HIP_OK(hipMemcpy(hq.data(), b.args.out_q, sizeof(double) * b.atoms_total, hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b.stream, ho.data(), b.args.out, sizeof(MolOut) * b.nmol, hipMemcpyDeviceToHost));
+  HIP_OK(scopy(b.stream, hq.data(), b.args.out_q, sizeof(double) * b.atoms_total, hipMemcpyDeviceToHost));
   std::vector<double> hP, hC, hE;
   if (b.mat_total > 0) {
     // densities stay on the device unless asked for (a gradient batch keeps its own there)
     if (b.keep_density) {
       hP.resize(b.mat_total);
-      HIP_OK(hipMemcpy(hP.data(), b.args.out_P, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
+      HIP_OK(scopy(b.stream, hP.data(), b.args.out_P, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
     }
     if (!b.purify) {
       hC.resize(b.mat_total);
       hE.resize(b.vec_total);
-      HIP_OK(hipMemcpy(hC.data(), b.args.out_C, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
-      HIP_OK(hipMemcpy(hE.data(), b.args.out_eps, sizeof(double) * b.vec_total, hipMemcpyDeviceToHost));
+      HIP_OK(scopy(b.stream, hC.data(), b.args.out_C, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
+      HIP_OK(scopy(b.stream, hE.data(), b.args.out_eps, sizeof(double) * b.vec_total, hipMemcpyDeviceToHost));
     }
   }
   for (int k : b.order_all) {
@@ -4800,6 +4980,7 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
     r.guess_used = o.guess_used;
     r.nao = d.nao;
     r.nocc = d.nocc;
+    r.nbeta = d.nocc;
     for (int x = 0; x < 3; ++x) r.dipole[x] = o.dip[x] * AU_DIPOLE_TO_DEBYE;
     r.charges.assign(d.nat, 0.0);
     for (int t = 0; t < d.nat; ++t) r.charges[b.atom_map[k][t]] = hq[d.atom0 + t];
@@ -4862,9 +5043,9 @@ int batch_carry_densities(Batch* dp, const Batch* sp, const std::vector<int>& sr
       const MolDesc& dk = d.desc[k];
       const MolDesc& sj = s.desc[src_of[k]];
       if (dk.out_mat < 0 || sj.out_mat < 0 || dk.nat == 0) continue;
-      HIP_OK(hipMemcpy(d.args.out_P + dk.out_mat, s.args.out_P + sj.out_mat, sizeof(double) * dk.nao * dk.nao,
+      HIP_OK(scopy(d.stream, d.args.out_P + dk.out_mat, s.args.out_P + sj.out_mat, sizeof(double) * dk.nao * dk.nao,
                        hipMemcpyDeviceToDevice));
-      HIP_OK(hipMemcpy(d.args.out + k, s.args.out + src_of[k], sizeof(MolOut), hipMemcpyDeviceToDevice));
+      HIP_OK(scopy(d.stream, d.args.out + k, s.args.out + src_of[k], sizeof(MolOut), hipMemcpyDeviceToDevice));
     }
     return true;
   };
@@ -4919,9 +5100,9 @@ int batch_set_positions(Batch* bp, const std::vector<MolIn>& mols, std::string& 
   }
   const size_t bytes = sizeof(double) * (size_t)b.atoms_total;
   auto upload = [&]() -> bool {
-    HIP_OK(hipMemcpy(const_cast<double*>(b.args.ax), ax.data(), bytes, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(const_cast<double*>(b.args.ay), ay.data(), bytes, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(const_cast<double*>(b.args.az), az.data(), bytes, hipMemcpyHostToDevice));
+    HIP_OK(scopy(b.stream, const_cast<double*>(b.args.ax), ax.data(), bytes, hipMemcpyHostToDevice));
+    HIP_OK(scopy(b.stream, const_cast<double*>(b.args.ay), ay.data(), bytes, hipMemcpyHostToDevice));
+    HIP_OK(scopy(b.stream, const_cast<double*>(b.args.az), az.data(), bytes, hipMemcpyHostToDevice));
     return true;
   };
   return upload() ? 0 : 6;
@@ -4970,19 +5151,19 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
   double *d_in = nullptr, *d_out = nullptr;
   bool ok = b->alloc(d_in, std::max<size_t>(sin.size(), 1), err) && b->alloc(d_out, nout, err);
   if (ok && !sin.empty()) {
-    hipError_t e = hipMemcpy(d_in, sin.data(), sizeof(double) * sin.size(), hipMemcpyHostToDevice);
+    hipError_t e = scopy(b->stream, d_in, sin.data(), sizeof(double) * sin.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) { err = hipGetErrorString(e); ok = false; }
   }
   if (ok) {
-    hipLaunchKernelGGL(pair_kernel, dim3(b->pair_blocks), dim3(b->pair_block), 0, 0, b->args);
-    if (mode == 5 && !guess_infer(model, infer_view(*b), err)) ok = false;
+    hipLaunchKernelGGL(pair_kernel, dim3(b->pair_blocks), dim3(b->pair_block), 0, b->stream, b->args);
+    if (mode == 5 && !guess_infer(model, infer_view(*b), err, b->stream)) ok = false;
   }
   if (ok) {
-    hipLaunchKernelGGL(stage_kernel, dim3(1), dim3(NT), b->lds_need, 0, b->args, mode, d_in, d_out);
+    hipLaunchKernelGGL(stage_kernel, dim3(1), dim3(NT), b->lds_need, b->stream, b->args, mode, d_in, d_out);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     std::vector<double> so(nout);
-    if (e == hipSuccess) e = hipMemcpy(so.data(), d_out, sizeof(double) * nout, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = scopy(b->stream, so.data(), d_out, sizeof(double) * nout, hipMemcpyDeviceToHost);
     if (e != hipSuccess) { err = hipGetErrorString(e); ok = false; }
     if (ok) {
       if (mode == 0 || mode == 1 || mode == 5 || mode == 6) {

@@ -713,17 +713,18 @@ void guess_model_destroy(GuessModel* M) { delete M; }
 
 static int fwd_blocks(int ntile) { return std::max(1, std::min((ntile + 3) / 4, 2048)); }
 
-bool guess_infer(const GuessModel* M, const BatchGuessView& v, std::string& err, hipEvent_t between) {
+bool guess_infer(const GuessModel* M, const BatchGuessView& v, std::string& err, hipStream_t stream,
+                 hipEvent_t between) {
   if (v.ngp_total == 0) {
-    if (between) HIP_OK(hipEventRecord(between, 0));
+    if (between) HIP_OK(hipEventRecord(between, stream));
     return true;
   }
   PairGeom G{v.ax, v.ay, v.az, v.elem, v.gbeg, v.gcnt, v.gdst};
   const int fb = (int)std::min<long long>((v.ngp_total + 255) / 256, 16384);
-  hipLaunchKernelGGL(feature_kernel, dim3(fb), dim3(256), 0, 0, G, v.gsrc, v.ngp_total, v.X);
-  if (between) HIP_OK(hipEventRecord(between, 0));
+  hipLaunchKernelGGL(feature_kernel, dim3(fb), dim3(256), 0, stream, G, v.gsrc, v.ngp_total, v.X);
+  if (between) HIP_OK(hipEventRecord(between, stream));
   const int n = (int)v.ngp_total, ntile = (n + 31) / 32;
-  hipLaunchKernelGGL(mlp_fwd_kernel, dim3(fwd_blocks(ntile)), dim3(256), 0, 0, v.X, (const int*)nullptr, n, ntile,
+  hipLaunchKernelGGL(mlp_fwd_kernel, dim3(fwd_blocks(ntile)), dim3(256), 0, stream, v.X, (const int*)nullptr, n, ntile,
                      (const u32x4*)M->frag, v.Y);
   HIP_OK(hipGetLastError());
   return true;

@@ -43,7 +43,8 @@ void guess_model_destroy(GuessModel* m);
 
 // Features and network outputs of every directed pair of a batch (into v.X / v.Y); `between`, when
 // given, is recorded on the stream after the feature kernel.
-bool guess_infer(const GuessModel* m, const BatchGuessView& v, std::string& err, hipEvent_t between = nullptr);
+bool guess_infer(const GuessModel* m, const BatchGuessView& v, std::string& err, hipStream_t stream,
+                 hipEvent_t between = nullptr);
 
 bool guess_data_create(GuessData*& out, std::string& err);
 bool guess_data_add(GuessData* d, Batch* solved, long long& added, std::string& err);

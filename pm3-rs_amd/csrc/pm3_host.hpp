@@ -34,6 +34,13 @@ struct EngineOptions {
   int stability;    // 0 off, 1 auto, 2 always
   int scf_memory_mb;
   const struct GuessModel* guess_model = nullptr;  // network of the learned guess (guess 4)
+  int use_diis = 1;          // Pm3Options::use_diis (accel already folds it in; the CDIIS rung reads it)
+  double level_shift = 0.0;  // Pm3Options::level_shift_ev (eV)
+  double damping = 0.0;      // Pm3Options::damping
+  int reference = 0;         // 0 auto, 1 RHF, 2 UHF (Pm3Options::reference)
+  int has_field = 0;         // Pm3Options::field is Some
+  double field[3] = {0.0, 0.0, 0.0};  // eV per Bohr per elementary charge
+  int integral_memory_mb = 0;         // pair-cache ceiling (MiB); 0 the process default
 };
 
 // One molecule as handed to the engine, atoms in caller order.
@@ -42,6 +49,7 @@ struct MolIn {
   const int32_t* z;
   const double* xyz;
   double charge;
+  int multiplicity = 1;
 };
 
 // Per-molecule engine output.  Matrices and vectors are in the caller's AO / atom order.
@@ -52,7 +60,11 @@ struct MolResult {
   double dipole[3] = {0, 0, 0};  // Debye
   int iterations = 0, converged = 0, paths = 0, nocc = 0, nao = 0;
   int guess_used = 0;  // 0 superposition, 1 learned, 2 learned rejected by the commutator test
+  int unrestricted = 0, nbeta = 0;
+  int missing_z = 0;                          // status 2 / 4: the atomic number refused
+  double required_mb = 0.0, limit_mb = 0.0;   // status 5: pair-cache size and ceiling (MiB)
   std::vector<double> charges, density, mo_energies, mo_coeff, gradient;
+  std::vector<double> spin_density, mo_energies_beta, mo_coeff_beta;  // unrestricted results only
 };
 
 struct RunRequest {

@@ -40,7 +40,12 @@ class InvalidInput(Pm3Error):
 
 
 class MissingElement(Pm3Error):
+    """Pm3Error::MissingElement(z): ``z`` is the atomic number without parameters."""
     status = 2
+
+    def __init__(self, msg, z=0):
+        super().__init__(msg)
+        self.z = z
 
 
 class ScfNotConverged(Pm3Error):
@@ -57,7 +62,14 @@ class Unsupported(Pm3Error):
 
 
 class ResourceLimit(Pm3Error):
+    """Pm3Error::ResourceLimit{operation, required_mb, limit_mb}."""
     status = 5
+
+    def __init__(self, msg, required_mb=0.0, limit_mb=0.0):
+        super().__init__(msg)
+        self.operation = "PM3 two-electron pair cache"
+        self.required_mb = required_mb
+        self.limit_mb = limit_mb
 
 
 class DeviceError(Pm3Error):
@@ -69,13 +81,16 @@ _ERRORS = {1: InvalidInput, 2: MissingElement, 3: ScfNotConverged, 4: Unsupporte
 ACCEL = {"none": 0, "cdiis": 1, "adiis_cdiis": 2}
 GUESS = {"auto": 0, "sad": 1, "core": 2, "symmetry_broken": 3, "learned": 4}
 STABILITY = {"off": 0, "auto": 1, "always": 2}
+REFERENCE = {"auto": 0, "rhf": 1, "uhf": 2}
 
 
 class _COptions(ctypes.Structure):
     _fields_ = [("max_scf", ctypes.c_int32), ("e_tol", ctypes.c_double), ("p_tol", ctypes.c_double),
                 ("use_diis", ctypes.c_int32), ("accelerator", ctypes.c_int32), ("adiis_switch", ctypes.c_double),
                 ("guess", ctypes.c_int32), ("stability", ctypes.c_int32), ("scf_memory_mb", ctypes.c_int32),
-                ("guess_model", ctypes.c_void_p)]
+                ("guess_model", ctypes.c_void_p), ("level_shift_ev", ctypes.c_double), ("damping", ctypes.c_double),
+                ("reference", ctypes.c_int32), ("has_field", ctypes.c_int32), ("field", ctypes.c_double * 3),
+                ("integral_memory_mb", ctypes.c_int32)]
 
 
 class _CMolecule(ctypes.Structure):
@@ -91,7 +106,9 @@ class _CResult(ctypes.Structure):
                 ("last_change", ctypes.c_double), ("scf_improvement_ev", ctypes.c_double),
                 ("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("scf_paths_tried", ctypes.c_int32),
                 ("n_occ", ctypes.c_int32), ("nao", ctypes.c_int32), ("has_homo", ctypes.c_int32),
-                ("has_lumo", ctypes.c_int32), ("guess_used", ctypes.c_int32)]
+                ("has_lumo", ctypes.c_int32), ("guess_used", ctypes.c_int32), ("unrestricted", ctypes.c_int32),
+                ("n_beta", ctypes.c_int32), ("missing_z", ctypes.c_int32), ("required_mb", ctypes.c_double),
+                ("limit_mb", ctypes.c_double)]
 
 
 class _COptOptions(ctypes.Structure):
@@ -117,7 +134,7 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_guess_train", "pm3_guess_eval", "pm3_guess_grad", "pm3_guess_mlp", "pm3_guess_features", "pm3_guess_density",
             "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune",
             "pm3_batch_large_timing", "pm3_batch_set_positions", "pm3_opt_options_default", "pm3_optimize_batch",
-            "pm3_optimize", "pm3_batch_carry_densities"]
+            "pm3_optimize", "pm3_batch_carry_densities", "pm3_batch_spin_densities"]
 
 _lib = None
 
@@ -134,13 +151,14 @@ def native():
         L.pm3_options_default.argtypes = [ctypes.POINTER(_COptions)]
         L.pm3_query_nao.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p]
         L.pm3_run.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
-                              ctypes.POINTER(_CResult), dp, dp, dp, dp]
+                              ctypes.POINTER(_CResult), dp, dp, dp, dp, dp, dp, dp]
         L.pm3_gradient.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
                                    ctypes.POINTER(_CResult), dp, dp]
-        L.pm3_run_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ctypes.c_void_p, ctypes.POINTER(_COptions),
+        L.pm3_run_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p, ctypes.POINTER(_COptions),
                                     ctypes.c_int32, ip, ctypes.POINTER(_CResult), dp]
-        L.pm3_batch_create.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ctypes.c_void_p, ctypes.POINTER(_COptions),
-                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+        L.pm3_batch_create.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p,
+                                       ctypes.POINTER(_COptions), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+        L.pm3_batch_spin_densities.argtypes = [ctypes.c_void_p, dp]
         L.pm3_batch_solve.argtypes = [ctypes.c_void_p]
         L.pm3_batch_results.argtypes = [ctypes.c_void_p, ip, ctypes.POINTER(_CResult), dp, dp]
         L.pm3_batch_destroy.argtypes = [ctypes.c_void_p]
@@ -270,6 +288,10 @@ def _raise(status, res=None):
     cls = _ERRORS.get(status, Pm3Error)
     if cls is ScfNotConverged and res is not None:
         raise ScfNotConverged(msg, res.iterations, res.last_change)
+    if cls is MissingElement and res is not None:
+        raise MissingElement(msg, res.missing_z)
+    if cls is ResourceLimit and res is not None:
+        raise ResourceLimit(msg, res.required_mb, res.limit_mb)
     raise cls(msg)
 
 
@@ -338,7 +360,9 @@ class Pm3Parameters:
 
 @dataclasses.dataclass
 class Pm3Options:
-    """Pm3Options::default() (src/scf.rs:259-287) for the fields the molecular RHF path reads."""
+    """Pm3Options::default() (src/scf.rs:259-287) for the fields the molecular single point reads.
+    field: None or three components in eV / Bohr / e (Pm3Options::field); reference: "auto",
+    "rhf" or "uhf"; integral_memory_mb 0 takes the process default ceiling (4096 MiB)."""
     max_scf: int = 200
     e_tol: float = 1.0e-8
     p_tol: float = 1.0e-7
@@ -349,11 +373,19 @@ class Pm3Options:
     stability: str = "auto"
     scf_memory_mb: int = 512
     guess_model: Optional[object] = None  # pm3amd.guess.GuessModel, required by guess="learned"
+    level_shift_ev: float = 0.0
+    damping: float = 0.0
+    reference: str = "auto"
+    field: Optional[tuple] = None
+    integral_memory_mb: int = 0
 
     def _c(self):
+        f = (ctypes.c_double * 3)(*(self.field if self.field is not None else (0.0, 0.0, 0.0)))
         return _COptions(self.max_scf, self.e_tol, self.p_tol, int(self.use_diis), ACCEL[self.accelerator],
                          self.adiis_switch, GUESS[self.guess], STABILITY[self.stability], self.scf_memory_mb,
-                         self.guess_model.handle if self.guess_model is not None else None)
+                         self.guess_model.handle if self.guess_model is not None else None,
+                         float(self.level_shift_ev), float(self.damping), REFERENCE[self.reference],
+                         int(self.field is not None), f, int(self.integral_memory_mb))
 
 
 @dataclasses.dataclass
@@ -378,6 +410,10 @@ class Pm3Result:
     scf_improvement_ev: float
     unrestricted: bool = False
     guess_used: int = 0  # 0 superposition / other rule, 1 learned, 2 learned rejected
+    n_beta: int = 0
+    spin_density: Optional[np.ndarray] = None       # P_alpha - P_beta, unrestricted results
+    mo_energies_beta: Optional[np.ndarray] = None
+    mo_coeff_beta: Optional[np.ndarray] = None
 
 
 @dataclasses.dataclass
@@ -390,11 +426,13 @@ class GradientResult:
     max_gradient: float
 
 
-def _result(r, P, eps, C, q):
+def _result(r, P, eps, C, q, S=None, eps_b=None, C_b=None):
+    u = bool(r.unrestricted)
     return Pm3Result(P, eps, C, r.n_occ, r.electronic_ev, r.core_ev, r.total_ev, r.heat_of_formation_kcal, q,
                      np.array(r.dipole_debye[:]), r.dipole_magnitude, r.homo_ev if r.has_homo else None,
                      r.lumo_ev if r.has_lumo else None, r.iterations, bool(r.converged), r.scf_paths_tried,
-                     r.scf_improvement_ev, False, r.guess_used)
+                     r.scf_improvement_ev, u, r.guess_used, r.n_beta, S if u else None, eps_b if u else None,
+                     C_b if u else None)
 
 
 def _dp(a):
@@ -410,17 +448,21 @@ def run_pm3(molecule: Molecule, params: Optional[Pm3Parameters] = None, options:
     nao = native().pm3_query_nao(ctypes.byref(cm), params._h)
     if nao < 0:
         bad = [int(z) for z in molecule.z if int(z) not in SYMBOLS.values() or int(z) in (2, 10, 18, 19, 20)]
-        raise MissingElement(f"no PM3 parameters in this engine for Z={bad[0] if bad else '?'}")
+        raise MissingElement(f"no PM3 parameters for element {bad[0] if bad else '?'}", bad[0] if bad else 0)
     res = _CResult()
     P = np.zeros((nao, nao))
+    S = np.zeros((nao, nao))
     q = np.zeros(len(molecule.z))
     eps = np.zeros(nao) if want_orbitals else None
     C = np.zeros((nao, nao)) if want_orbitals else None
+    eps_b = np.zeros(nao) if want_orbitals else None
+    C_b = np.zeros((nao, nao)) if want_orbitals else None
     null = ctypes.POINTER(ctypes.c_double)()
-    st = native().pm3_run(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(res), _dp(P),
-                          _dp(eps) if want_orbitals else null, _dp(C) if want_orbitals else null, _dp(q))
+    o = lambda a: _dp(a) if a is not None else null  # noqa: E731
+    st = native().pm3_run(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(res), _dp(P), o(eps),
+                          o(C), _dp(q), _dp(S), o(eps_b), o(C_b))
     _raise(st, res)
-    return _result(res, P, eps, C, q)
+    return _result(res, P, eps, C, q, S, eps_b, C_b)
 
 
 def closed_form_gradient(molecule: Molecule, params: Optional[Pm3Parameters] = None,
@@ -442,6 +484,12 @@ def closed_form_gradient(molecule: Molecule, params: Optional[Pm3Parameters] = N
     return GradientResult(scf, res.total_ev, g, -g, float(np.abs(g).max()) if g.size else 0.0)
 
 
+def single_point(molecule: Molecule, params: Optional[Pm3Parameters] = None, options: Optional[Pm3Options] = None,
+                 want_orbitals=True) -> Pm3Result:
+    """run_pm3 under the name of the reference's Python layer (src/python.rs:253)."""
+    return run_pm3(molecule, params, options, want_orbitals)
+
+
 class Batch:
     """Device-resident batch of independent molecules (pm3_batch_* in include/pm3_gpu.h)."""
 
@@ -458,10 +506,12 @@ class Batch:
         self._z = np.ascontiguousarray(np.concatenate([m.z for m in molecules]).astype(np.int32))
         self._xyz = np.ascontiguousarray(np.concatenate([m.positions for m in molecules]))
         self._q = np.ascontiguousarray([float(m.charge) for m in molecules], dtype=np.float64)
+        self._mult = np.ascontiguousarray([int(m.multiplicity) for m in molecules], dtype=np.int32)
         h = ctypes.c_void_p()
         ip = ctypes.POINTER(ctypes.c_int32)
         st = native().pm3_batch_create(len(molecules), ptr.ctypes.data_as(ip), self._z.ctypes.data_as(ip),
-                                       _dp(self._xyz), _dp(self._q), self.params._h, ctypes.byref(self.options._c()),
+                                       _dp(self._xyz), _dp(self._q), self._mult.ctypes.data_as(ip), self.params._h,
+                                       ctypes.byref(self.options._c()),
                                        (1 if want_density else 0) | (2 if want_gradient else 0) |
                                        (4 if force_large else 0) | (8 if warm_start else 0), ctypes.byref(h))
         _raise(st)
@@ -525,13 +575,18 @@ class Batch:
         null = ctypes.POINTER(ctypes.c_double)()
         _raise(native().pm3_batch_results(self._h, status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), res, _dp(q),
                                           _dp(P) if self.want_density else null))
+        S = None
+        if self.want_density:
+            S = np.zeros(max(total, 1))
+            _raise(native().pm3_batch_spin_densities(self._h, _dp(S)))
         out, moff = [], 0
         for i in range(k):
             a0, a1 = self.atom_ptr[i], self.atom_ptr[i + 1]
             n = max(nao[i], 0)
             Pi = P[moff:moff + n * n].reshape(n, n) if self.want_density else None
+            Si = S[moff:moff + n * n].reshape(n, n) if self.want_density else None
             moff += n * n
-            out.append(_result(res[i], Pi, None, None, q[a0:a1].copy()))
+            out.append(_result(res[i], Pi, None, None, q[a0:a1].copy(), Si))
         return status, out
 
     def set_positions(self, positions):

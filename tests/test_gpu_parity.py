@@ -28,7 +28,8 @@ def _rows(pins, names):
 
 PARITY_ROWS = ["water", "methane", "ethane", "benzene", "formaldehyde", "acetic_acid", "methylamine", "nitromethane",
                "hydroxide", "ammonium", "chloroform", "sulfur_dioxide", "lithium_fluoride", "silicon_tetrafluoride",
-               "aluminium_trichloride", "dimethyl_sulfoxide", "ketene", "butyne"]
+               "aluminium_trichloride", "dimethyl_sulfoxide", "ketene", "butyne", "tetrafluoroethene",
+               "tetrachloroethene"]
 
 
 def test_single_point_parity(G, pins, oracle, exact_eigensolves):
@@ -62,7 +63,12 @@ def test_default_engine_settings_reach_the_same_answers(G, pins, oracle):
         assert st == 0 and r.converged
         assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL, row["name"]
         assert np.abs(r.density - o["density"]).max() <= P_TOL, row["name"]
-        assert abs(r.homo_ev - o["homo_ev"]) <= 1e-7 and abs(r.lumo_ev - o["lumo_ev"]) <= 1e-7
+        assert abs(r.homo_ev - o["homo_ev"]) <= 1e-7, row["name"]
+        # the reported LUMO is the winning rung's: the -2 eV shifted unrestricted rung reports its
+        # shifted spectrum (src/scf.rs:1416-1423), and which of two rungs that reach the same state
+        # wins is path-dependent when they tie within 1e-6 eV (src/scf.rs:710-713)
+        if r.scf_paths_tried == 1:
+            assert abs(r.lumo_ev - o["lumo_ev"]) <= 1e-7, row["name"]
 
 
 def test_default_options_follow_the_oracle_path(G, oracle, exact_eigensolves):
@@ -243,14 +249,93 @@ def test_error_surface(G):
     with pytest.raises(G.ScfNotConverged) as e:
         G.run_pm3(G.Molecule(z, x), options=G.Pm3Options(max_scf=1))
     assert e.value.iterations == 1
-    with pytest.raises(G.MissingElement):
+    with pytest.raises(G.MissingElement) as e:
         G.run_pm3(G.Molecule(np.array([18, 1]), np.array([[0, 0, 0], [2.5, 0, 0]])))
+    assert e.value.z == 18 and "18" in str(e.value)  # MissingElement(z) carries the element
     with pytest.raises(G.InvalidInput):
         G.run_pm3(G.Molecule(np.array([1, 1]), np.zeros((2, 3))))
-    with pytest.raises(G.Unsupported):  # odd electron count needs the unrestricted path
+    with pytest.raises(G.InvalidInput):  # odd electron count with a singlet (src/scf.rs:497-503)
         G.run_pm3(G.Molecule(np.array([6, 1, 1, 1]), np.array([[0, 0, 0], [2, 0, 0], [-1, 1.7, 0], [-1, -1.7, 0]])))
     # a batch reports per-molecule failures without failing the launch
     b = G.Batch([G.Molecule(z, x), G.Molecule(np.array([1, 1]), np.zeros((2, 3))), G.Molecule(z, x, -1.0)])
     b.solve()
     st, res = b.results()
-    assert list(st) == [0, 1, 4]
+    assert list(st) == [0, 1, 1]
+
+
+def test_pair_cache_ceiling(G, oracle):
+    # ResourceLimit before any SCF work (src/hamiltonian.rs:202-211), with the reference's sizes
+    atoms = [[e, 3.0 * i + dx, 3.0 * j + dy, 3.0 * k] for i in range(4) for j in range(4) for k in range(4)
+             for e, dx, dy in (("O", 0, 0), ("H", 0.96, 0), ("H", -0.24, 0.93))]
+    z, x = atoms_to_zx(atoms)
+    with pytest.raises(G.ResourceLimit) as e:
+        G.run_pm3(G.Molecule(z, x), options=G.Pm3Options(integral_memory_mb=1))
+    st, o = oracle.run_pm3(z, x, 0.0, 1, oracle.options(integral_memory_mb=1))
+    assert st == 5 and e.value.required_mb == o["required_mb"] and e.value.limit_mb == 1
+    assert "pair cache" in str(e.value)
+
+
+OPTION_ROWS = ["water", "formaldehyde", "acetic_acid", "ketene", "nitromethane"]
+OPTION_MATRIX = [
+    # src/scf.rs:1938-1963: every accelerator reaches the same fixed point
+    dict(accelerator="none"), dict(accelerator="cdiis"), dict(accelerator="adiis_cdiis"),
+    dict(use_diis=False),
+    # src/scf.rs:1776-1801: a 1 MiB history budget gives depth-2 CDIIS and the same energy
+    dict(scf_memory_mb=1),
+    # ScfStability (src/scf.rs:52-70): off, auto, always
+    dict(stability="off"), dict(stability="always"),
+    # ScfGuess (src/scf.rs:133-158): superposition, core Hamiltonian, symmetry-broken first path
+    dict(guess="sad"), dict(guess="core"), dict(guess="symmetry_broken"),
+    # a steered first path: the restricted loop reads neither knob, and the search is off
+    # (src/scf.rs:563-607; tests/api_surface.rs:241-249)
+    dict(level_shift_ev=0.5, damping=0.2),
+]
+
+
+@pytest.mark.parametrize("kw", OPTION_MATRIX, ids=lambda kw: ",".join(f"{k}={v}" for k, v in kw.items()))
+def test_option_matrix_parity(G, pins, oracle, exact_eigensolves, kw):
+    okw = dict(kw)
+    for key, table in (("accelerator", {"none": 0, "cdiis": 1, "adiis_cdiis": 2}),
+                       ("stability", {"off": 0, "auto": 1, "always": 2}),
+                       ("guess", {"sad": 1, "core": 2, "symmetry_broken": 3})):
+        if key in okw:
+            okw[key] = table[okw[key]]
+    for row in _rows(pins, OPTION_ROWS):
+        z, x = atoms_to_zx(row["atoms"])
+        print("case", row["name"], kw, flush=True)
+        r = G.run_pm3(G.Molecule(z, x, row["charge"]), options=G.Pm3Options(max_scf=2000, **TIGHT, **kw))
+        st, o = oracle.run(z, x, row["charge"], oracle.options(max_scf=2000, **TIGHT, **okw))
+        assert st == 0 and r.converged
+        assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL, row["name"]
+        assert np.abs(r.density - o["density"]).max() <= P_TOL, row["name"]
+        assert r.scf_paths_tried == o["scf_paths_tried"], row["name"]
+        # iteration counts follow the restated path up to rounding: the two eigensolvers differ in
+        # the last bits, and a slowly converging path (unaccelerated, or an excited state left by a
+        # rung) can cross its stopping threshold an iteration or two apart (parity unpinned)
+        assert abs(r.iterations - o["iterations"]) <= max(2, o["iterations"] // 20), (row["name"], r.iterations,
+                                                                                      o["iterations"])
+
+
+def test_field_parity(G, oracle):
+    # a uniform field in H_core, the core energy and the gradient (src/hamiltonian.rs:224-233,
+    # src/dipole.rs:156-181, src/gradient.rs:296-301), single and batched, against the oracle
+    f = (0.02, -0.013, 0.031)
+    cases = [atoms_to_zx([["O", 0, 0, 0.1173], ["H", 0, 0.7572, -0.4692], ["H", 0, -0.7572, -0.4692]]),
+             atoms_to_zx([["C", 0, 0, 0], ["O", 0, 0, 1.21], ["H", 0.94, 0, -0.54], ["H", -0.94, 0, -0.54]])]
+    for z, x in cases:
+        g = G.closed_form_gradient(G.Molecule(z, x), options=G.Pm3Options(field=f, **TIGHT))
+        st, o = oracle.gradient(z, x, 0.0, oracle.options(field=f, **TIGHT))
+        assert st == 0
+        assert abs(g.scf.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL
+        assert abs(g.scf.core_ev - o["core_ev"]) < 1e-9
+        assert np.abs(g.scf.density - o["density"]).max() <= P_TOL
+        assert np.abs(g.gradient - o["gradient"]).max() < 2.295e-7
+        _, o0 = oracle.run(z, x, 0.0, oracle.options(**TIGHT))
+        assert abs(o["total_ev"] - o0["total_ev"]) > 1e-4  # the field did something
+    b = G.Batch([G.Molecule(z, x) for z, x in cases], options=G.Pm3Options(field=f, **TIGHT), want_gradient=True)
+    b.solve()
+    st, res = b.results()
+    for (z, x), r, gb in zip(cases, res, b.gradients()):
+        so, o = oracle.gradient(z, x, 0.0, oracle.options(field=f, **TIGHT))
+        assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL
+        assert np.abs(gb - o["gradient"]).max() < 2.295e-7

@@ -45,7 +45,7 @@ def _stage(G):
 
 def _cases(pins):
     names = ["water", "methane", "benzene", "ethane", "lithium_fluoride", "chloroform", "hydroxide",
-             "silicon_tetrafluoride", "ammonium"]
+             "silicon_tetrafluoride", "ammonium", "ketene", "tetrafluoroethene"]
     rows = [r for r in pins["mopac_closed_shell"] if r["name"] in names]
     out = [(r["name"], *atoms_to_zx(r["atoms"]), r["charge"]) for r in rows]
     for name, (z, x) in (("C24H50", W.alkane(24)), ("water16", W.water_cluster(16, 7.0, 5)),

@@ -45,10 +45,15 @@ def test_params_and_defaults_without_device():
 
 def test_host_side_errors_need_no_device():
     import pm3amd as G
-    # multiplicity > 1 is the reference's UHF path, not served by this engine
+    # RHF on an open shell is refused before any device work (src/scf.rs:512-519), with the
+    # reference's message; so is a multiplicity below 1
     m = G.Molecule(np.array([6, 1, 1, 1]), np.eye(4, 3) * 2.0, multiplicity=2)
-    with pytest.raises(G.Unsupported):
-        G.run_pm3(m)
+    with pytest.raises(G.InvalidInput, match="RHF requested"):
+        G.run_pm3(m, options=G.Pm3Options(reference="rhf"))
+    with pytest.raises(G.InvalidInput):
+        G.run_pm3(G.Molecule(m.z, m.positions, multiplicity=0))
+    with pytest.raises(G.InvalidInput, match="damping"):
+        G.run_pm3(m, options=G.Pm3Options(damping=1.0))
     with pytest.raises(G.InvalidInput):
         G.Molecule.from_xyz_str("2\nx\nH 0 0 0\n")
     with pytest.raises(G.InvalidInput):
