@@ -66,8 +66,9 @@ def test_default_engine_settings_reach_the_same_answers(G, pins, oracle):
         assert abs(r.homo_ev - o["homo_ev"]) <= 1e-7, row["name"]
         # the reported LUMO is the winning rung's: the -2 eV shifted unrestricted rung reports its
         # shifted spectrum (src/scf.rs:1416-1423), and which of two rungs that reach the same state
-        # wins is path-dependent when they tie within 1e-6 eV (src/scf.rs:710-713)
-        if r.scf_paths_tried == 1:
+        # wins is path-dependent when they tie within 1e-6 eV (src/scf.rs:710-713); with loosened
+        # eigensolves the first path may also reach the ground state directly (no search at all)
+        if r.scf_paths_tried == 1 and o["scf_paths_tried"] == 1:
             assert abs(r.lumo_ev - o["lumo_ev"]) <= 1e-7, row["name"]
 
 
