@@ -126,6 +126,19 @@ pm3_status pm3_run(const pm3_molecule* mol, const pm3_params* params, const pm3_
                    double* density, double* mo_energies, double* mo_coeff, double* charges, double* spin_density,
                    double* mo_energies_beta, double* mo_coeff_beta);
 
+/* pm3_run without any device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops that forbid hipMalloc).
+ * Every device buffer of the call -- staged inputs, integrals, SCF workspace, outputs -- is carved
+ * from one static block of PM3_NOALLOC_SCRATCH_MB MiB per GPU, and the work runs on one stream per
+ * GPU, both set up at the first call on that device; no hipMalloc, hipFree or allocating library
+ * routine is called.  Molecules up to PM3_NOALLOC_MAX_ATOMS atoms and PM3_NOALLOC_MAX_NAO AOs;
+ * larger ones get PM3_RESOURCE_LIMIT before any device work.  Outputs as pm3_run (density[n*n] and
+ * charges[n_atoms] may be NULL; densities are purified, so no orbitals).  Calls on one GPU share the
+ * block and the stream and must not overlap: concurrent calls on the same GPU race. */
+enum { PM3_NOALLOC_MAX_ATOMS = 128, PM3_NOALLOC_MAX_NAO = 128, PM3_NOALLOC_SCRATCH_MB = 32 };
+pm3_status pm3_run_noalloc(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                           pm3_result* result, double* density, double* charges);
+
 /* closed_form_gradient (src/gradient.rs:201-276): SCF plus the Hellmann-Feynman gradient at the
  * converged density.  gradient[3*n_atoms] in eV/Bohr; density may be NULL. */
 pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,

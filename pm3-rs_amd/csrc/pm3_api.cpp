@@ -256,6 +256,37 @@ pm3_status pm3_run(const pm3_molecule* mol, const pm3_params* params, const pm3_
   return molecule_status(r.status, r);
 }
 
+pm3_status pm3_run_noalloc(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                           pm3_result* result, double* density, double* charges) {
+  static_assert(PM3_NOALLOC_MAX_ATOMS == NOALLOC_MAX_ATOMS && PM3_NOALLOC_MAX_NAO == NOALLOC_MAX_NAO,
+                "header and engine limits of the allocation-free entry point");
+  g_last_error.clear();
+  if (!params) return fail(PM3_INVALID_INPUT, "null parameter set");
+  pm3_status s = check_molecule(mol);
+  if (s != PM3_OK) return s;
+  EngineOptions eo;
+  std::string why;
+  if (!engine_options(opts, eo, why)) return fail(PM3_INVALID_INPUT, why);
+  if ((s = reference_check(mol, params, eo)) != PM3_OK) return s;
+  const int nao = pm3_query_nao(mol, params);
+  if (nao >= 0 && (mol->n_atoms > NOALLOC_MAX_ATOMS || nao > NOALLOC_MAX_NAO))
+    return fail(PM3_RESOURCE_LIMIT, "the allocation-free entry point takes at most " + std::to_string(NOALLOC_MAX_ATOMS) +
+                                        " atoms and " + std::to_string(NOALLOC_MAX_NAO) + " AOs (this molecule: " +
+                                        std::to_string(mol->n_atoms) + " atoms, " + std::to_string(nao) + " AOs)");
+  MolResult r;
+  std::string err;
+  const int code = engine_run_noalloc(params->p, MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge,
+                                                       mol->multiplicity}, eo, density != nullptr, r, err);
+  if (code != 0) return fail((pm3_status)code, err);
+  if (result) fill_result(r, result);
+  if (r.status == PM3_OK || r.status == PM3_SCF_NOT_CONVERGED) {
+    const size_t n2 = (size_t)r.nao * r.nao;
+    if (density && r.density.size() == n2) std::memcpy(density, r.density.data(), n2 * sizeof(double));
+    if (charges && (int)r.charges.size() == mol->n_atoms) std::memcpy(charges, r.charges.data(), mol->n_atoms * sizeof(double));
+  }
+  return molecule_status(r.status, r);
+}
+
 pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
                             const double* charge, const int32_t* multiplicity, const pm3_params* params,
                             const pm3_options* opts, int32_t flags, pm3_batch** out) {

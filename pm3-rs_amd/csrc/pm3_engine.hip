@@ -3620,6 +3620,11 @@ struct Batch {
   hipEvent_t evg = nullptr, evm = nullptr;
   double last_feature_ms = 0.0, last_network_ms = 0.0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;
+  // allocation-free mode: every buffer is carved from this device block, the stream is borrowed,
+  // and no events are created (no timing)
+  unsigned char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
   ~Batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (int i = 0; i < nbufs; ++i)
@@ -3632,7 +3637,7 @@ struct Batch {
     if (blas) (void)rocblas_destroy_handle(blas);
     for (hipEvent_t e : {lg.evp0, lg.evp1, lg.evf0, lg.evf1})
       if (e) (void)hipEventDestroy(e);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
   bool ensure_guess_buffers(std::string& err) {
     if (d_X || ngp_total == 0) return true;
@@ -3644,6 +3649,17 @@ struct Batch {
     return true;
   }
   template <class T> bool alloc(T*& p, size_t count, std::string& err) {
+    if (arena) {
+      const size_t bytes = (sizeof(T) * std::max<size_t>(count, 1) + 255) & ~(size_t)255;
+      if (arena_used + bytes > arena_cap) {
+        err = "the molecule's buffers need more than the " + std::to_string(arena_cap >> 20) +
+              " MiB of the allocation-free scratch block";
+        return false;
+      }
+      p = reinterpret_cast<T*>(arena + arena_used);
+      arena_used += bytes;
+      return true;
+    }
     if (nbufs >= (int)(sizeof(bufs) / sizeof(bufs[0]))) {
       err = "internal: too many batch buffers";
       return false;
@@ -4452,13 +4468,28 @@ static bool large_solve(Batch& b, int k, std::string& err) {
   return true;
 }
 
+static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt,
+                            const RunRequest& req, Batch*& out_batch, std::string& err, unsigned char* arena,
+                            size_t arena_bytes, hipStream_t arena_stream);
+
 bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt, const RunRequest& req,
                   Batch*& out_batch, std::string& err) {
+  return batch_create_in(P, mols, opt, req, out_batch, err, nullptr, 0, nullptr);
+}
+
+static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt,
+                            const RunRequest& req, Batch*& out_batch, std::string& err, unsigned char* arena,
+                            size_t arena_bytes, hipStream_t arena_stream) {
   out_batch = nullptr;
   if (!upload_table(P, err)) return false;
   Batch* bp = new Batch();
   Batch& b = *bp;
-  if (hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking) != hipSuccess) {
+  if (arena) {
+    b.arena = arena;
+    b.arena_cap = arena_bytes;
+    b.stream = arena_stream;
+    b.own_stream = false;
+  } else if (hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking) != hipSuccess) {
     err = "hipStreamCreateWithFlags failed";
     delete bp;
     return false;
@@ -4803,6 +4834,10 @@ bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineO
   if (const char* ev = std::getenv("PM3_SP2_LDS")) a.sp2_lds = std::atoi(ev);
   if (const char* ev = std::getenv("PM3_EIG_SKIP")) a.eig_skip = std::atof(ev);
   HIP_OK(hipFuncSetAttribute((const void*)scf_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_need));
+  if (!b.large.empty() && b.arena) {
+    err = "the allocation-free entry point serves molecules of the single-workgroup path only";
+    return false;
+  }
   if (!b.large.empty() && !large_setup(b, err)) return false;
   return true;
 }
@@ -4827,22 +4862,24 @@ bool batch_solve(Batch* bp, std::string& err) {
   b.lg.products = b.lg.fock_builds = b.lg.product_flop = 0;
   b.lg.lanczos_steps = b.lg.lanczos_runs = 0;
   if (b.order_all.empty()) return true;
-  if (!b.ev0) {
+  const bool timed = b.arena == nullptr;  // the allocation-free mode creates no events
+  if (timed && !b.ev0) {
     HIP_OK(hipEventCreate(&b.ev0));
     HIP_OK(hipEventCreate(&b.ev1));
     HIP_OK(hipEventCreate(&b.ev2));
     HIP_OK(hipEventCreate(&b.evg));
     HIP_OK(hipEventCreate(&b.evm));
   }
+  auto rec = [&](hipEvent_t e) { return timed ? hipEventRecord(e, b.stream) : hipSuccess; };
   HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), b.stream));
   if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, PH_SLOTS * sizeof(unsigned long long), b.stream));
-  HIP_OK(hipEventRecord(b.ev0, b.stream));
+  HIP_OK(rec(b.ev0));
   hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, b.stream, b.args);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(b.evg, b.stream));
-  if (b.learned && !guess_infer(b.guess_model, infer_view(b), err, b.stream, b.evm)) return false;
-  if (!b.learned) HIP_OK(hipEventRecord(b.evm, b.stream));
-  HIP_OK(hipEventRecord(b.ev1, b.stream));
+  HIP_OK(rec(b.evg));
+  if (b.learned && !guess_infer(b.guess_model, infer_view(b), err, b.stream, timed ? b.evm : nullptr)) return false;
+  if (!b.learned) HIP_OK(rec(b.evm));
+  HIP_OK(rec(b.ev1));
   b.args.warm = b.warm_start && (b.solved_once || b.have_prev) ? 1 : 0;
   if (b.nsolve > 0) hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, b.stream, b.args);
   HIP_OK(hipGetLastError());
@@ -4858,6 +4895,11 @@ bool batch_solve(Batch* bp, std::string& err) {
                          dim3(256), 0, b.stream, b.args, b.nsolve);
     }
     HIP_OK(hipGetLastError());
+  }
+  if (!timed) {
+    HIP_OK(hipStreamSynchronize(b.stream));
+    b.solved_once = true;
+    return true;
   }
   HIP_OK(hipEventRecord(b.ev2, b.stream));
   HIP_OK(hipEventSynchronize(b.ev2));
@@ -5179,6 +5221,64 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
   }
   batch_destroy(b);
   return ok;
+}
+
+// ------------------------------------------------------------------------------------------
+// Allocation-free single points.  The device block below is the scratch of every buffer such a
+// call needs (inputs staged from the host, molecule store, workgroup slab, outputs).  Host code
+// never passes `scratch` itself to a kernel or a copy: the symbol is a host-side stand-in, and
+// the block's device address on each GPU comes from hipGetSymbolAddress, taken once per device
+// with that device current.
+constexpr size_t NOALLOC_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+struct NoallocDevice {
+  bool ready = false;
+  unsigned char* ptr = nullptr;  // device address of this GPU's copy of `scratch`
+  hipStream_t stream = nullptr;  // the stream of this GPU's allocation-free calls
+};
+
+static bool noalloc_device(NoallocDevice& out, std::string& err) {
+  static std::mutex mu;
+  static NoallocDevice dev_tab[64];
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    err = "device index out of range";
+    return false;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  NoallocDevice& d = dev_tab[dev];
+  if (!d.ready) {
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    HIP_OK(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
+    d.ptr = static_cast<unsigned char*>(p);
+    d.ready = true;
+  }
+  out = d;
+  return true;
+}
+
+int engine_run_noalloc(const Params& P, const MolIn& mol, const EngineOptions& opt, bool want_density, MolResult& out,
+                       std::string& err) {
+  NoallocDevice d;
+  if (!noalloc_device(d, err)) return 6;
+  RunRequest req;
+  req.want_density = want_density;
+  Batch* b = nullptr;
+  const std::vector<MolIn> mols{mol};
+  if (!batch_create_in(P, mols, opt, req, b, err, d.ptr, NOALLOC_BYTES, d.stream)) {
+    const bool short_block = b && b->arena && err.find("scratch block") != std::string::npos;
+    delete b;
+    return short_block ? 5 : 6;
+  }
+  std::vector<MolResult> res;
+  const bool ok = batch_solve(b, err) && batch_fetch(b, P, mols, res, err);
+  delete b;
+  if (!ok) return 6;
+  out = res[0];
+  return 0;
 }
 
 bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt, const RunRequest& req,

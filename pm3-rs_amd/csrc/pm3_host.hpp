@@ -143,4 +143,14 @@ bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<do
 bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt, const RunRequest& req,
                 std::vector<MolResult>& out, std::string& err);
 
+// The same single point with no device allocation: every buffer is carved from the library's
+// static per-GPU scratch block (NOALLOC_BYTES) and the work runs on that GPU's own stream, both
+// set up once per device.  Returns 0, or a pm3_status code with err set (5 when the molecule's
+// buffers do not fit the block, 6 for a device failure).  Calls on one GPU share the block: they
+// must not overlap.
+constexpr int NOALLOC_MAX_ATOMS = 128;
+constexpr int NOALLOC_MAX_NAO = 128;
+int engine_run_noalloc(const Params& P, const MolIn& mol, const EngineOptions& opt, bool want_density, MolResult& out,
+                       std::string& err);
+
 }  // namespace pm3host

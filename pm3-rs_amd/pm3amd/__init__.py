@@ -134,7 +134,7 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_guess_train", "pm3_guess_eval", "pm3_guess_grad", "pm3_guess_mlp", "pm3_guess_features", "pm3_guess_density",
             "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune",
             "pm3_batch_large_timing", "pm3_batch_set_positions", "pm3_opt_options_default", "pm3_optimize_batch",
-            "pm3_optimize", "pm3_batch_carry_densities", "pm3_batch_spin_densities"]
+            "pm3_optimize", "pm3_batch_carry_densities", "pm3_batch_spin_densities", "pm3_run_noalloc"]
 
 _lib = None
 
@@ -159,6 +159,8 @@ def native():
         L.pm3_batch_create.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p,
                                        ctypes.POINTER(_COptions), ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
         L.pm3_batch_spin_densities.argtypes = [ctypes.c_void_p, dp]
+        L.pm3_run_noalloc.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
+                                      ctypes.POINTER(_CResult), dp, dp]
         L.pm3_batch_solve.argtypes = [ctypes.c_void_p]
         L.pm3_batch_results.argtypes = [ctypes.c_void_p, ip, ctypes.POINTER(_CResult), dp, dp]
         L.pm3_batch_destroy.argtypes = [ctypes.c_void_p]
@@ -463,6 +465,25 @@ def run_pm3(molecule: Molecule, params: Optional[Pm3Parameters] = None, options:
                           o(C), _dp(q), _dp(S), o(eps_b), o(C_b))
     _raise(st, res)
     return _result(res, P, eps, C, q, S, eps_b, C_b)
+
+
+def run_pm3_noalloc(molecule: Molecule, params: Optional[Pm3Parameters] = None,
+                    options: Optional[Pm3Options] = None) -> Pm3Result:
+    """run_pm3 through pm3_run_noalloc: no device allocation (the library's static per-GPU scratch
+    block), at most 128 atoms and 128 AOs, purified densities (no orbitals)."""
+    params = params or Pm3Parameters.standard()
+    options = options or Pm3Options()
+    cm = molecule._c()
+    nao = native().pm3_query_nao(ctypes.byref(cm), params._h)
+    if nao < 0:
+        raise MissingElement("no PM3 parameters for an element of the molecule")
+    res = _CResult()
+    P = np.zeros((nao, nao))
+    q = np.zeros(len(molecule.z))
+    st = native().pm3_run_noalloc(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(res), _dp(P),
+                                  _dp(q))
+    _raise(st, res)
+    return _result(res, P, None, None, q)
 
 
 def closed_form_gradient(molecule: Molecule, params: Optional[Pm3Parameters] = None,
