@@ -289,15 +289,16 @@ typedef struct {
   double first_energy_ev;                                 /* at the starting geometry */
 } pm3_opt_result;
 
-/* optimize (src/optimizer.rs:233-262) for many closed-shell molecules at once: L-BFGS with the
+/* optimize (src/optimizer.rs:233-262) for many molecules at once: L-BFGS with the
  * backtracking Armijo search of drive (:77-221) per molecule, every round one batched SCF +
- * gradient solve over all running molecules.  Molecules in the atom_ptr / z / xyz_bohr / charge
- * layout of pm3_run_batch; results[k] per molecule, final positions xyz_out[3 * atom_ptr[n_mol]]
+ * gradient solve over all running molecules.  Molecules in the atom_ptr / z / xyz_bohr / charge /
+ * multiplicity layout of pm3_run_batch; results[k] per molecule, final positions xyz_out[3 * atom_ptr[n_mol]]
  * (Bohr).  Returns the first device or launch failure, else PM3_OK (per-molecule failures are in
  * results[k].status).  PM3_UNSUPPORTED with PM3_GUESS_LEARNED. */
 pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
-                              const double* charge, const pm3_params* params, const pm3_options* opts,
-                              const pm3_opt_options* opt, pm3_opt_result* results, double* xyz_out);
+                              const double* charge, const int32_t* multiplicity, const pm3_params* params,
+                              const pm3_options* opts, const pm3_opt_options* opt, pm3_opt_result* results,
+                              double* xyz_out);
 /* optimize for one molecule (pm3_optimize_batch of one); xyz_out[3 * n_atoms]. */
 pm3_status pm3_optimize(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
                         const pm3_opt_options* opt, pm3_opt_result* result, double* xyz_out);

@@ -66,6 +66,7 @@ __constant__ TableConst c_tab;
 
 struct MolDesc {
   int atom0, nat, nheavy, nao, nocc, np;
+  int nbeta, uhf;              // beta electrons; uhf = 1: the unrestricted path (nocc is then n_alpha)
   int ngp, learn;              // directed pairs of the learned guess; learn = all atoms H, C or O
   long long out_mat, out_vec;  // offsets into the density/coefficient and vector outputs, -1 if none
   long long h_off, w_off;      // per-molecule H_core (np x np) and pair tables in the molecule store
@@ -76,6 +77,7 @@ struct MolOut {
   double e_elec, e_core, last_dp, first_e, homo, lumo;
   double dip[3];
   int iters, converged, paths, status;
+  int unrestricted;  // the returned state is unrestricted (spin density in out_S)
   int guess_used;  // 0 superposition (learned not requested or not applicable), 1 learned, 2 learned rejected
 };
 
@@ -99,6 +101,10 @@ struct KArgs {
   double* out_eps;
   double* out_q;  // Mulliken charge per atom (sorted atom order)
   double* out_g;  // gradient per atom (sorted atom order, eV/Bohr), when requested
+  double* out_S;     // spin density P_a - P_b per unrestricted molecule (out_P layout), or null
+  double* out_Cb;    // beta orbitals (out_C layout), eigenvector route only, or null
+  double* out_epsb;  // beta orbital energies (out_eps layout), or null
+  long long ustride; // doubles of each workgroup's slab that hold the beta channel (0: none)
   double* store;  // per-molecule H_core and pair tables written by pair_kernel
   const long long* pair_base;  // first global pair of each queue position (nmol + 1 entries)
   long long npairs_total;
@@ -546,6 +552,8 @@ __device__ inline double oc_int(int a, int b, int c, int d, const ElemConst& e) 
 
 struct Mol {
   int atom0, nat, nh, nH, n, np, nocc;
+  int nbeta;       // beta electrons (= nocc for a restricted molecule)
+  double wt, xs;   // occupation of an aufbau orbital (2 restricted, 1 per spin) and its inverse
   int ngp, learn;
   long long gp0;
   int nxx, nxh, nhh;
@@ -697,8 +705,8 @@ __device__ double finish_core(const KArgs& A, const Mol& M, double* red) {
 // launch per phase) runs the same code.
 // Phase 1 of the Fock build: off-diagonal atom blocks (exchange K, owned by their pair) and the
 // per-pair Coulomb partial sums in M.slot.  Work-items t0, t0 + st, ... (a workgroup or a grid).
-__device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H, const double* P, double* F, int t0,
-                                  int st) {
+__device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H, const double* P, const double* X,
+                                  double xf, double* F, int t0, int st) {
   const int np = M.np;
   double* JX = M.slot;                          // [20][nxx]: J_a (0..9), J_b (10..19) per XX pair
   double* JH = M.slot + 20LL * M.nxx;           // [11][nxh]: J_a (0..9), J_b (10) per XH pair
@@ -717,7 +725,7 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
         db[pk(mu, nu)] = f * P[(ob + mu) * np + ob + nu];
       }
 #pragma unroll
-      for (int la = 0; la < 4; ++la) pab[mu][la] = P[(oa + mu) * np + ob + la];
+      for (int la = 0; la < 4; ++la) pab[mu][la] = X[(oa + mu) * np + ob + la];
     }
     double ja[10], jb[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, K[4][4];
 #pragma unroll
@@ -741,7 +749,7 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
           jb[q] += da[p] * w[q];
         }
         ja[p] = acc;
-        // exchange: K[mu][la] -= 1/2 sum_si P(nu, si) (mu nu|la si), and the (nu, mu) term
+        // exchange: K[mu][la] -= xf sum_si X(nu, si) (mu nu|la si), and the (nu, mu) term
 #pragma unroll
         for (int la = 0; la < 4; ++la) {
           double s1 = 0.0, s2 = 0.0;
@@ -750,8 +758,8 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
             s1 += pab[nu][si] * w[pk(la, si)];
             s2 += pab[mu][si] * w[pk(la, si)];
           }
-          K[mu][la] -= 0.5 * s1;
-          if (mu != nu) K[nu][la] -= 0.5 * s2;
+          K[mu][la] -= xf * s1;
+          if (mu != nu) K[nu][la] -= xf * s2;
         }
       }
 #pragma unroll
@@ -775,7 +783,7 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
     const double pbb = P[ob * np + ob];
     double jbv = 0.0, pa[4];
 #pragma unroll
-    for (int nu = 0; nu < 4; ++nu) pa[nu] = P[(oa + nu) * np + ob];
+    for (int nu = 0; nu < 4; ++nu) pa[nu] = X[(oa + nu) * np + ob];
     double w[10];
 #pragma unroll
     for (int q = 0; q < 10; ++q) w[q] = M.Wxh[(long long)q * M.nxh + kx];
@@ -791,7 +799,7 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
       double k = 0.0;
 #pragma unroll
       for (int nu = 0; nu < 4; ++nu) k += pa[nu] * w[pk(mu, nu)];
-      const double v = H[(oa + mu) * np + ob] - 0.5 * k;
+      const double v = H[(oa + mu) * np + ob] - xf * k;
       F[(oa + mu) * np + ob] = v;
       F[ob * np + oa + mu] = v;
     }
@@ -807,7 +815,7 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
     b += M.nh;
     const int oa = A.aoff[M.atom0 + a], ob = A.aoff[M.atom0 + b];
     const double w = M.Whh[kx];
-    const double v = H[oa * np + ob] - 0.5 * P[oa * np + ob] * w;
+    const double v = H[oa * np + ob] - xf * X[oa * np + ob] * w;
     F[oa * np + ob] = v;
     F[ob * np + oa] = v;
   }
@@ -850,8 +858,8 @@ __device__ inline void fock_jsum(const KArgs& A, const Mol& M, const double* P, 
 // Phase 3: diagonal blocks, H + one-centre terms + J.  Each element with mu <= nu is computed once
 // and stored on both sides, so that F is exactly symmetric (the (nu, mu) sum visits the same terms
 // in another order).
-__device__ inline void fock_diag(const KArgs& A, const Mol& M, const double* H, const double* P, double* F,
-                                 const double* J, int t0, int st) {
+__device__ inline void fock_diag(const KArgs& A, const Mol& M, const double* H, const double* P, const double* X,
+                                 double xf, double* F, const double* J, int t0, int st) {
   const int np = M.np;
   for (int e = t0; e < M.nat * 16; e += st) {
     const int a = e >> 4, mu = (e >> 2) & 3, nu = e & 3;
@@ -865,7 +873,7 @@ __device__ inline void fock_diag(const KArgs& A, const Mol& M, const double* H, 
         for (int si = 0; si < nn; ++si) {
           const double pls = P[(o + la) * np + o + si];
           acc += pls * oc_int(mu, nu, la, si, ea);
-          acc -= (0.5 * pls) * oc_int(mu, la, nu, si, ea);
+          acc -= (xf * X[(o + la) * np + o + si]) * oc_int(mu, la, nu, si, ea);
         }
       const double v = H[(o + mu) * np + o + nu] + acc + J[a * 10 + pk(mu, nu)];
       F[(o + mu) * np + o + nu] = v;
@@ -874,15 +882,22 @@ __device__ inline void fock_diag(const KArgs& A, const Mol& M, const double* H, 
   }
 }
 
-// F = H + G(P): RHF Fock build (src/fock.rs:285-332, :488-500) by one workgroup, J in LDS.
-__device__ __noinline__ void build_fock(const KArgs& A, const Mol& M, const double* H, const double* P, double* F, double* lds) {
+// F = H + J(P) - xf K(X) by one workgroup, J in LDS: the spin Fock build (build_fock_spin,
+// src/fock.rs:285-332) with Coulomb from the total density P and exchange from the same-spin
+// density X (xf = 1), or the RHF build (src/fock.rs:488-500) with X = P and xf = 1/2.
+__device__ __noinline__ void build_fock_x(const KArgs& A, const Mol& M, const double* H, const double* P,
+                                          const double* X, double xf, double* F, double* lds) {
   const int tid = threadIdx.x;
-  fock_pairs(A, M, H, P, F, tid, NT);
+  fock_pairs(A, M, H, P, X, xf, F, tid, NT);
   __syncthreads();
   fock_jsum(A, M, P, lds, tid, NT);
   __syncthreads();
-  fock_diag(A, M, H, P, F, lds, tid, NT);
+  fock_diag(A, M, H, P, X, xf, F, lds, tid, NT);
   __syncthreads();
+}
+
+__device__ inline void build_fock(const KArgs& A, const Mol& M, const double* H, const double* P, double* F, double* lds) {
+  build_fock_x(A, M, H, P, P, 0.5, F, lds);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1506,11 +1521,12 @@ __device__ __noinline__ void eigensolve(const Mol& M, const Bufs& B, const doubl
   stamp(B, 5);
 }
 
-// P = 2 C_occ C_occ^T from Cn into Pdst.
+// P = wt C_occ C_occ^T from Cn into Pdst (wt = 2 restricted, 1 for one spin).
 __device__ void density(const Mol& M, const double* Cn, double* Pdst) {
   const int n = M.n, np = M.np;
+  const double wt = M.wt;
   wg_gemm<true>(n, n, M.nocc, [&](int i, int k) { return Cn[i * np + k]; }, [&](int k, int j) { return Cn[j * np + k]; },
-                [&](int i, int j, double v) { Pdst[i * np + j] = 2.0 * v; });
+                [&](int i, int j, double v) { Pdst[i * np + j] = wt * v; });
   __syncthreads();
 }
 
@@ -1687,7 +1703,7 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
   if (fabs(tr - occ) > 1.0e-6) return -2;
   for (int e = tid; e < n * n; e += NT) {
     const int i = e / n, j = e % n;
-    Pdst[i * np + j] = 2.0 * X[i * ldx + j];
+    Pdst[i * np + j] = M.wt * X[i * ldx + j];
   }
   __syncthreads();
   return k;
@@ -1733,7 +1749,7 @@ __device__ __noinline__ int sp2_density(const Mol& M, const Bufs& B, const doubl
   if (nocc == 0 || nocc == n) {
     for (int e = tid; e < n * n; e += NT) {
       const int i = e / n, j = e % n;
-      Pdst[i * np + j] = (nocc == n && i == j) ? 2.0 : 0.0;
+      Pdst[i * np + j] = (nocc == n && i == j) ? M.wt : 0.0;
     }
     __syncthreads();
     return 0;
@@ -2199,11 +2215,12 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
   double* G = B.T;
   double* Ao = B.Cn;
   double* Av = B.V;
-  wg_gemm_pad(n, np, Fu, P, [&](int i, int j, double v) { G[i * np + j] = 0.5 * v; });
+  const double xs = M.xs;  // X = P / wt
+  wg_gemm_pad(n, np, Fu, P, [&](int i, int j, double v) { G[i * np + j] = xs * v; });
   __syncthreads();
   wg_gemm_pad<true>(n, np, P, G, [&](int i, int j, double sv2) {
-                  const double sv = 0.5 * sv2;
-                  const double x = 0.5 * P[i * np + j];
+                  const double sv = xs * sv2;
+                  const double x = xs * P[i * np + j];
                   const double id = i == j ? 1.0 : 0.0;
                   const double ao = sv + c_lo * (id - x);
                   if (square) {
@@ -2223,7 +2240,7 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
   if (seeded) {
     if (tid < n) {
       double t = 0.0;
-      for (int j = 0; j < n; ++j) t += 0.5 * P[j * np + tid] * B.fs[j];
+      for (int j = 0; j < n; ++j) t += xs * P[j * np + tid] * B.fs[j];
       B.ft[tid] = t;
     }
     __syncthreads();
@@ -2345,6 +2362,24 @@ __device__ double commutator2(const KArgs& A, const Mol& M, const Bufs& B, doubl
   return v[0];
 }
 
+// The symmetry-broken guess's tilt (src/scf.rs:894-916), thread 0: a fifth of an electron from
+// p_z to p_x on the first atom with the largest p population (heavy atoms keep their caller order).
+__device__ inline void sb_tilt(const KArgs& A, const Mol& M, double* P) {
+  const int np = M.np;
+  int target = -1;
+  double best = 0.0;
+  for (int a = 0; a < M.nat; ++a) {
+    const ElemConst& e = c_tab.el[A.aslot[M.atom0 + a]];
+    if (e.norb == 4 && e.iop > best) { best = e.iop; target = a; }
+  }
+  if (target >= 0) {
+    const int o = A.aoff[M.atom0 + target];
+    const double tilt = fmin(0.2, P[(o + 3) * np + o + 3]);
+    P[(o + 3) * np + o + 3] -= tilt;
+    P[(o + 1) * np + o + 1] += tilt;
+  }
+}
+
 // Initial density for a guess mode (src/scf.rs:855-935): 1 SAD, 2 core, 3 symmetry-broken,
 // 4 learned with the superposition fallback.  The learned density is kept only when its
 // first-iteration commutator norm is not larger than the superposition's; it then only builds
@@ -2396,21 +2431,7 @@ __device__ __noinline__ int initial_guess(const KArgs& A, const Mol& M, const Bu
     return 2;
   }
   sad_into(A, M, B.P);
-  if (guess == 3 && tid == 0) {
-    // tilt a fifth of an electron from p_z to p_x on the atom with the largest p population
-    int target = -1;
-    double best = 0.0;
-    for (int a = 0; a < M.nat; ++a) {
-      const ElemConst& e = c_tab.el[A.aslot[M.atom0 + a]];
-      if (e.norb == 4 && e.iop > best) { best = e.iop; target = a; }
-    }
-    if (target >= 0) {
-      const int o = A.aoff[M.atom0 + target];
-      const double tilt = fmin(0.2, B.P[(o + 3) * np + o + 3]);
-      B.P[(o + 3) * np + o + 3] -= tilt;
-      B.P[(o + 1) * np + o + 1] += tilt;
-    }
-  }
+  if (guess == 3 && tid == 0) sb_tilt(A, M, B.P);
   __syncthreads();
   return 0;
 }
@@ -2695,6 +2716,289 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
   return R;
 }
 
+// ------------------------------------------------------------------------------------------
+// The unrestricted path (uhf_loop, src/scf.rs:1319-1496) for s/p elements.  Each spin channel is a
+// Bufs view (alpha: the workgroup slab; beta: the slab's beta part) with its own density, Fock
+// matrices, history and orbitals; scratch (T, V, eigen work) is shared, the channels run one after
+// the other.  A channel's Mol has n_occ = its electron count and weight-1 densities.
+__device__ inline Mol spin_mol(const Mol& M, int nocc) {
+  Mol S = M;
+  S.nocc = nocc;
+  S.wt = 1.0;
+  S.xs = 1.0;
+  return S;
+}
+
+// Aufbau density of the channel's (extrapolated, shifted) Fock matrix Fu into Pdst: purified, or
+// from eigenvectors (then eps and Cn hold the channel's orbitals).
+__device__ inline int channel_aufbau(const KArgs& A, const Mol& Mc, const Bufs& Bc, const double* Fu, double* Pdst,
+                                     double* lds, double* red, bool tight, bool seed, bool warm) {
+  if (A.purify) {
+    const int ns = sp2_density(Mc, Bc, Fu, Pdst, red, tight, seed);
+    if (ns < 0) {
+      eigensolve(Mc, Bc, Fu, false, lds, red);
+      density(Mc, Bc.Cn, Pdst);
+    }
+    return ns;
+  }
+  eigensolve(Mc, Bc, Fu, warm, lds, red);
+  density(Mc, Bc.Cn, Pdst);
+  return 0;
+}
+
+// One history entry of a channel: E = F P - P F (through T = F P) and F, packed triangles.
+__device__ inline void uhf_entry(const Mol& M, const Bufs& B, const double* F, const double* P, double* Fn,
+                                 double* En) {
+  const int n = M.n, np = M.np, tid = threadIdx.x;
+  wg_gemm_pad(n, np, F, P, [&](int i, int j, double v) { B.T[i * np + j] = v; });
+  __syncthreads();
+  for (int e = tid; e < n * (n + 1) / 2; e += NT) {
+    int i, j;
+    tri_unpack(e, n, i, j);
+    En[e] = (i == j) ? 0.0 : B.T[i * np + j] - B.T[j * np + i];
+    Fn[e] = F[j * np + i];
+  }
+  __syncthreads();
+}
+
+// Fu = the combination sum_i c_i F_i of a channel's history (unpacked, symmetric).
+__device__ inline void uhf_combine(const Mol& M, const Bufs& B, const Diis& D) {
+  const int n = M.n, np = M.np;
+  const long long ms = (long long)np * np;
+  for (int e = threadIdx.x; e < n * (n + 1) / 2; e += NT) {
+    double v = 0.0;
+    for (int i = 0; i < D.len; ++i) v += D.coef[i] * B.Fh[(long long)D.slot[i] * ms + e];
+    int i, j;
+    tri_unpack(e, n, i, j);
+    B.Fu[i * np + j] = v;
+    B.Fu[j * np + i] = v;
+  }
+  __syncthreads();
+}
+
+// out = F + shift (I - P) for the channel (level_shift, src/scf.rs:337-350).
+__device__ inline void uhf_shift(const Mol& M, const double* F, const double* P, double* out, double shift) {
+  const int n = M.n, np = M.np;
+  for (int e = threadIdx.x; e < n * n; e += NT) {
+    const int i = e / n, j = e % n;
+    out[i * np + j] = F[i * np + j] + shift * ((i == j ? 1.0 : 0.0) - P[i * np + j]);
+  }
+  __syncthreads();
+}
+
+// (P_t . H + P_a . F_a + P_b . F_b) / 2 over the padded matrices.
+__device__ inline double uhf_energy(const Mol& M, const double* Pt, const double* Pa, const double* Fa,
+                                    const double* Pb, const double* Fb, double* red) {
+  double v[1] = {0.0};
+  for (int e = threadIdx.x; e < M.np * M.np; e += NT) v[0] += Pt[e] * M.H[e] + Pa[e] * Fa[e] + Pb[e] * Fb[e];
+  block_sum<1>(v, red);
+  return 0.5 * v[0];
+}
+
+// One unrestricted SCF path, uhf_loop (src/scf.rs:1319-1496): the guess of uhf_guess
+// (:937-980: core-Hamiltonian orbitals filled to each depth, or the superposition / symmetry-broken
+// density scaled by n_a / N and n_b / N); per iteration the two spin Fock matrices from
+// (P_a + P_b, P_sigma), the energy (P_t . H + P_a . F_a + P_b . F_b) / 2, paired CDIIS on the stacked
+// error [E_a; E_b] only when accel = 1 (CDIIS requested and use_diis), depth for four matrices per
+// slot; the level shift F + shift (I - P_sigma); aufbau densities of weight 1; the sum of the two
+// RMS density changes; damping.  Leaves P_a, P_b in Ba.P, Bb.P, each channel's frontier levels (or
+// orbitals) in its eps, and the total density in Pt.
+__device__ PathResult uhf_loop(const KArgs& A, const Mol& M, const Bufs& Ba, const Bufs& Bb, double* Pt, int guess,
+                               int accel, double shift, double damp, double* lds, double* red, Diis& D, bool tight) {
+  const int n = M.n, np = M.np, tid = threadIdx.x;
+  const int na = M.nocc, nb = M.nbeta;
+  const Mol Ma = spin_mol(M, na), Mb = spin_mol(M, nb);
+  if (tid == 0) {
+    Ba.bnd[2] = -1.0;
+    Ba.bnd[3] = 0.0;
+    Bb.bnd[2] = -1.0;
+    Bb.bnd[3] = 0.0;
+  }
+  for (int e = tid; e < np * np; e += NT) {
+    Ba.P[e] = 0.0;
+    Bb.P[e] = 0.0;
+    Pt[e] = 0.0;
+  }
+  __syncthreads();
+  if (guess == 2) {
+    channel_aufbau(A, Ma, Ba, M.H, Ba.P, lds, red, tight, false, false);
+    channel_aufbau(A, Mb, Bb, M.H, Bb.P, lds, red, tight, false, false);
+  } else {
+    sad_into(A, M, Pt);
+    if (guess == 3 && tid == 0) sb_tilt(A, M, Pt);
+    __syncthreads();
+    const double ne = (double)(na + nb);
+    const double fa = ne > 0.0 ? na / ne : 0.5, fb = ne > 0.0 ? nb / ne : 0.5;
+    for (int e = tid; e < np * np; e += NT) {
+      Ba.P[e] = Pt[e] * fa;
+      Bb.P[e] = Pt[e] * fb;
+    }
+    __syncthreads();
+  }
+  const bool cdiis = accel == 1;
+  {
+    const long long per = 4LL * n * n * 8, budget = (long long)A.scf_memory_mb * 1024 * 1024;
+    int depth = MAXD;
+    if (A.scf_memory_mb != 0 && per != 0) depth = (int)max(2LL, min((long long)MAXD, budget / per));
+    if (tid == 0) {
+      D.len = 0;
+      D.depth = depth;
+      D.keep_d = 0;
+      D.mm_stale = 0;
+      D.en = 1.0e300;
+      D.ok = 0;
+    }
+    __syncthreads();
+  }
+  PathResult R{0.0, 1.0e300, 0, 0, 0};
+  double e_old = 0.0;
+  bool have_c = false;
+  const double* Fla = Ba.F;  // the matrices of the last aufbau step
+  const double* Flb = Bb.F;
+  for (int it = 0; it < A.max_scf; ++it) {
+    R.iters = it + 1;
+    for (int e = tid; e < np * np; e += NT) Pt[e] = Ba.P[e] + Bb.P[e];
+    __syncthreads();
+    build_fock_x(A, M, M.H, Pt, Ba.P, 1.0, Ba.F, lds);
+    build_fock_x(A, M, M.H, Pt, Bb.P, 1.0, Bb.F, lds);
+    const double e_elec = uhf_energy(M, Pt, Ba.P, Ba.F, Bb.P, Bb.F, red);
+    const double* Fa = Ba.F;
+    const double* Fb = Bb.F;
+    if (cdiis) {
+      if (tid == 0) {
+        if (D.len == D.depth) {
+          const int s0 = D.slot[0];
+          for (int i = 0; i + 1 < D.len; ++i) {
+            D.slot[i] = D.slot[i + 1];
+            for (int j = 0; j + 1 < D.len; ++j) D.b[i][j] = D.b[i + 1][j + 1];
+          }
+          D.len -= 1;
+          D.newslot = s0;
+        } else {
+          D.newslot = D.len;
+        }
+      }
+      __syncthreads();
+      const int ns = D.newslot, len = D.len;
+      const long long ms = (long long)np * np;
+      uhf_entry(M, Ba, Ba.F, Ba.P, Ba.Fh + ns * ms, Ba.Eh + ns * ms);
+      uhf_entry(M, Bb, Bb.F, Bb.P, Bb.Fh + ns * ms, Bb.Eh + ns * ms);
+      // Gram row of the stacked error: <E_a,i, E_a,new> + <E_b,i, E_b,new> (off-diagonal weight 2)
+      double v[MAXD + 1];
+#pragma unroll
+      for (int q = 0; q <= MAXD; ++q) v[q] = 0.0;
+      const double* Ea = Ba.Eh + ns * ms;
+      const double* Eb = Bb.Eh + ns * ms;
+      for (int e = tid; e < n * (n + 1) / 2; e += NT) {
+        const double w = e < n ? 1.0 : 2.0;
+        const double ea = Ea[e], eb = Eb[e];
+        v[MAXD] += w * (ea * ea + eb * eb);
+#pragma unroll
+        for (int q = 0; q < MAXD; ++q)
+          if (q < len) {
+            const long long so = (long long)D.slot[q] * ms;
+            v[q] += w * (Ba.Eh[so + e] * ea + Bb.Eh[so + e] * eb);
+          }
+      }
+      block_sum<MAXD + 1>(v, red);
+      if (tid == 0) {
+        for (int i = 0; i < len; ++i) {
+          D.b[i][len] = v[i];
+          D.b[len][i] = v[i];
+        }
+        D.b[len][len] = v[MAXD];
+        D.slot[len] = ns;
+        D.len = len + 1;
+        D.en = sqrt(v[MAXD]);
+        D.mode = 1;
+      }
+      __syncthreads();
+      if (tid < 64) {
+        const bool ok = cdiis_wave(D, D.coef);
+        if (tid == 0) D.ok = ok;
+      }
+      __syncthreads();
+      if (D.ok) {
+        uhf_combine(M, Ba, D);
+        uhf_combine(M, Bb, D);
+        Fa = Ba.Fu;
+        Fb = Bb.Fu;
+      }
+    }
+    if (shift != 0.0) {
+      uhf_shift(M, Fa, Ba.P, Ba.Fu, shift);
+      uhf_shift(M, Fb, Bb.P, Bb.Fu, shift);
+      Fa = Ba.Fu;
+      Fb = Bb.Fu;
+    }
+    Fla = Fa;
+    Flb = Fb;
+    const bool seed = R.last_dp < FRONTIER_SEED_DP;
+    channel_aufbau(A, Ma, Ba, Fa, Ba.Pn, lds, red, tight, seed, have_c);
+    channel_aufbau(A, Mb, Bb, Fb, Bb.Pn, lds, red, tight, seed, have_c);
+    double dv[2] = {0.0, 0.0};
+    for (int e = tid; e < np * np; e += NT) {
+      const double da = Ba.Pn[e] - Ba.P[e], db = Bb.Pn[e] - Bb.P[e];
+      dv[0] += da * da;
+      dv[1] += db * db;
+    }
+    block_sum<2>(dv, red);
+    const double nn = (double)max(n * n, 1);
+    const double dp = sqrt(dv[0] / nn) + sqrt(dv[1] / nn);
+    const double de = fabs(e_elec - e_old);
+    R.last_dp = dp;
+    for (int e = tid; e < np * np; e += NT) {
+      Ba.P[e] = damp > 0.0 ? (1.0 - damp) * Ba.Pn[e] + damp * Ba.P[e] : Ba.Pn[e];
+      Bb.P[e] = damp > 0.0 ? (1.0 - damp) * Bb.Pn[e] + damp * Bb.P[e] : Bb.Pn[e];
+      if (!A.purify) {
+        Ba.C[e] = Ba.Cn[e];
+        Bb.C[e] = Bb.Cn[e];
+      }
+    }
+    __syncthreads();
+    have_c = !A.purify;
+    e_old = e_elec;
+    if (it > 0 && de < A.e_tol && dp < A.p_tol) {
+      R.converged = 1;
+      break;
+    }
+  }
+  if (A.purify) {
+    frontier_levels(Ma, Ba, Fla, damp > 0.0 ? Ba.Pn : Ba.P, lds, red);
+    frontier_levels(Mb, Bb, Flb, damp > 0.0 ? Bb.Pn : Bb.P, lds, red);
+  }
+  for (int e = tid; e < np * np; e += NT) Pt[e] = Ba.P[e] + Bb.P[e];
+  __syncthreads();
+  build_fock_x(A, M, M.H, Pt, Ba.P, 1.0, Ba.F, lds);
+  build_fock_x(A, M, M.H, Pt, Bb.P, 1.0, Bb.F, lds);
+  R.e_elec = uhf_energy(M, Pt, Ba.P, Ba.F, Bb.P, Bb.F, red);
+  return R;
+}
+
+// HOMO (max over channels) and LUMO (min) of an unrestricted state (src/scf.rs:106-124, 797-820);
+// +-1e300 for an empty side.
+__device__ inline void uhf_frontier(const Mol& M, const Bufs& Ba, const Bufs& Bb, double& homo, double& lumo) {
+  const int n = M.n, na = M.nocc, nb = M.nbeta;
+  homo = -1.0e300;
+  lumo = 1.0e300;
+  if (na > 0) homo = fmax(homo, Ba.eps[na - 1]);
+  if (nb > 0) homo = fmax(homo, Bb.eps[nb - 1]);
+  if (na < n) lumo = fmin(lumo, Ba.eps[na]);
+  if (nb < n) lumo = fmin(lumo, Bb.eps[nb]);
+}
+
+// A purified unrestricted path whose channel ended with its HOMO above its LUMO (a missed spectral
+// bound): run again with Gershgorin bounds.
+__device__ inline bool uhf_aufbau_broken(const KArgs& A, const Mol& M, const Bufs& Ba, const Bufs& Bb) {
+  if (!A.purify) return false;
+  auto bad = [&](const Bufs& B, int occ) {
+    if (occ <= 0 || occ >= M.n) return false;
+    const double h = B.eps[occ - 1], l = B.eps[occ];
+    return h > l + 1.0e-9 * fmax(1.0, fabs(l));
+  };
+  return bad(Ba, M.nocc) || bad(Bb, M.nbeta);
+}
+
 // With purified densities: a final HOMO above the LUMO means some eigenvalue ended on the wrong
 // side of the purification (a spectral bound that missed it); the path is then run again with
 // Gershgorin bounds throughout.
@@ -2735,6 +3039,32 @@ __device__ inline Bufs slab(const KArgs& A, double*& Wbase) {
   return B;
 }
 
+// The beta channel's part of the workgroup slab (its last A.ustride doubles): P_total, F, P, Pn,
+// Fu, C, Cn, the incumbent P and C of the stability search, the history F[8] and E[8], then eps,
+// the incumbent's eps and the frontier seeds.  Scratch matrices (T, V) and the eigensolver's work
+// vectors are the alpha channel's.
+__device__ inline Bufs beta_slab(const KArgs& A, const Bufs& Ba, double*& Pt) {
+  double* x = A.scratch + (long long)blockIdx.x * A.stride + (A.stride - A.ustride);
+  const long long m2 = (long long)A.np_max * A.np_max;
+  Bufs B = Ba;
+  Pt = x;
+  B.F = x + m2;
+  B.P = B.F + m2;
+  B.Pn = B.P + m2;
+  B.Fu = B.Pn + m2;
+  B.C = B.Fu + m2;
+  B.Cn = B.C + m2;
+  B.Pb = B.Cn + m2;
+  B.Cb = B.Pb + m2;
+  B.Fh = B.Cb + m2;
+  B.Eh = B.Fh + MAXD * m2;
+  B.eps = B.Eh + MAXD * m2;
+  B.epsb = B.eps + A.np_max;
+  B.fs = B.epsb + A.np_max;
+  B.ft = B.fs + A.np_max;
+  return B;
+}
+
 __host__ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double* Wbase) {
   Mol M;
   M.atom0 = md.atom0;
@@ -2745,6 +3075,9 @@ __host__ __device__ inline Mol mol_of(const KArgs& A, const MolDesc& md, double*
   M.n = md.nao;
   M.np = md.np;
   M.nocc = md.nocc;
+  M.nbeta = md.nbeta;
+  M.wt = 2.0;
+  M.xs = 0.5;
   M.ngp = md.ngp;
   M.learn = md.learn;
   M.gp0 = md.gp0;
@@ -2801,7 +3134,11 @@ __global__ __launch_bounds__(256) void pair_kernel(KArgs A) {
 //   dE/dR_b = sum P_ab (beta_a + beta_b) dS + sum P_aa de1b + sum P_bb de2a
 //           + sum (P_aa P_bb - 1/2 P_ab P_ab) dW + dE_core
 // The per-pair force goes to a slot; grad_gather sums them per atom in a fixed order.
-__device__ inline void pair_gradient(const KArgs& A, const Mol& M, const double* P, int n, int p, double* out) {
+// With a spin density S (unrestricted result) the exchange term is the spin-resolved one of
+// electronic_gradient_fixed_density_spin (src/gradient.rs:523-599): -(P_a P_a + P_b P_b) on the
+// off-diagonal block with P_a = (P + S) / 2, P_b = (P - S) / 2.
+__device__ inline void pair_gradient(const KArgs& A, const Mol& M, const double* P, const double* S, int n, int p,
+                                     double* out) {
   const int nat = M.nat;
   int a, b;
   tri_decode(p, nat, a, b);
@@ -2821,12 +3158,13 @@ __device__ inline void pair_gradient(const KArgs& A, const Mol& M, const double*
   // core-core
   acc(1.0, core_core_pair<D3>(c_tab, sa, sb, r));
   // density blocks (P_aa, P_bb, P_ab) of the pair
-  double paa[4][4], pbb[4][4], pab[4][4];
+  double paa[4][4], pbb[4][4], pab[4][4], sab[4][4];
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) {
       paa[i][j] = (i < na && j < na) ? P[(oa + i) * n + oa + j] : 0.0;
       pbb[i][j] = (i < nb && j < nb) ? P[(ob + i) * n + ob + j] : 0.0;
       pab[i][j] = (i < na && j < nb) ? P[(oa + i) * n + ob + j] : 0.0;
+      sab[i][j] = (S && i < na && j < nb) ? S[(oa + i) * n + ob + j] : 0.0;
     }
   // electron-core attraction blocks
   {
@@ -2857,7 +3195,13 @@ __device__ inline void pair_gradient(const KArgs& A, const Mol& M, const double*
     for (int x = 0; x < m2; ++x)
       for (int y = 0; y < l2; ++y) {
         const int u = x ? nu : mu, v = x ? mu : nu, s = y ? si : la, t = y ? la : si;
-        c += paa[u][v] * pbb[s][t] - 0.5 * pab[u][s] * pab[v][t];
+        if (S) {
+          const double a1 = 0.5 * (pab[u][s] + sab[u][s]), a2 = 0.5 * (pab[v][t] + sab[v][t]);
+          const double b1 = 0.5 * (pab[u][s] - sab[u][s]), b2 = 0.5 * (pab[v][t] - sab[v][t]);
+          c += paa[u][v] * pbb[s][t] - (a1 * a2 + b1 * b2);
+        } else {
+          c += paa[u][v] * pbb[s][t] - 0.5 * pab[u][s] * pab[v][t];
+        }
       }
     return c;
   };
@@ -2902,7 +3246,8 @@ __global__ __launch_bounds__(256) void grad_pair_kernel(KArgs A) {
     const Mol M = mol_of(A, md, nullptr);
     const int p = (int)(g - A.pair_base[lo]);
     double f[3];
-    pair_gradient(A, M, A.out_P + md.out_mat, md.nao, p, f);
+    const double* S = (md.uhf && A.out_S) ? A.out_S + md.out_mat : nullptr;
+    pair_gradient(A, M, A.out_P + md.out_mat, S, md.nao, p, f);
     const long long npairs = (long long)md.nat * (md.nat - 1) / 2;
     M.S1[p] = f[0];  // the electron-core slots are free once H_core is assembled
     M.S1[npairs + p] = f[1];
@@ -2992,6 +3337,115 @@ __device__ inline void charges_dipole(const KArgs& A, const Mol& M, const double
   block_sum<3>(dip, red);
 }
 
+// An unrestricted molecule of the batch (run_pm3's UHF branch, src/scf.rs:521-715): the first
+// path (steered by the level shift and damping when either is set), then, unless steered, the
+// stability search when the frontier gap over both channels is below 6 eV (or always): all seven
+// rungs of src/scf.rs:647-673 as unrestricted paths, the lowest converged energy kept (1e-6 eV).
+// Writes the result record, the total and spin densities, and the orbitals on the eigenvector route.
+__device__ __noinline__ void uhf_molecule(const KArgs& A, const Mol& M, const MolDesc& md, int mi, const Bufs& B,
+                                          double ecore, double* lds, double* red, Diis& D, double* bnd_b) {
+  const int tid = threadIdx.x, n = M.n, np = M.np;
+  double* Pt;
+  Bufs Bb = beta_slab(A, B, Pt);
+  Bb.bnd = bnd_b;
+  for (long long e = tid; e < (long long)np * np; e += NT) {
+    Bb.F[e] = 0.0; Bb.P[e] = 0.0; Bb.Pn[e] = 0.0; Bb.Fu[e] = 0.0; Bb.C[e] = 0.0; Bb.Cn[e] = 0.0; Pt[e] = 0.0;
+  }
+  __syncthreads();
+  const bool steered = A.shift != 0.0 || A.damp != 0.0;
+  const int g0 = A.guess == 4 ? 1 : A.guess;  // the learned guess covers restricted molecules
+  const int cd = A.accel == 1 ? 1 : 0;        // uhf_loop extrapolates only when CDIIS is asked for
+  const double s0 = steered ? A.shift : 0.0, d0 = steered ? A.damp : 0.0;
+  PathResult best = uhf_loop(A, M, B, Bb, Pt, g0, cd, s0, d0, lds, red, D, true);
+  if (uhf_aufbau_broken(A, M, B, Bb)) best = uhf_loop(A, M, B, Bb, Pt, g0, cd, s0, d0, lds, red, D, false);
+  const double first_e = best.e_elec;
+  int paths = 1;
+  double homo, lumo;
+  uhf_frontier(M, B, Bb, homo, lumo);
+  const double gap = (homo > -1.0e299 && lumo < 1.0e299) ? lumo - homo : 1.0e300;
+  const bool search = !steered && (A.stability == 2 || (A.stability == 1 && gap < 6.0));
+  auto keep = [&]() {  // the current state becomes the incumbent
+    for (int e = tid; e < np * np; e += NT) {
+      B.Pb[e] = B.P[e];
+      B.Cb[e] = B.C[e];
+      Bb.Pb[e] = Bb.P[e];
+      Bb.Cb[e] = Bb.C[e];
+    }
+    for (int j = tid; j < n; j += NT) {
+      B.epsb[j] = B.eps[j];
+      Bb.epsb[j] = Bb.eps[j];
+    }
+    __syncthreads();
+  };
+  if (search) {
+    keep();
+    const int primary = A.guess == 2 ? 2 : 1, other = primary == 2 ? 1 : 2;
+    const int rg[7][2] = {{primary, 0}, {other, cd}, {other, 0}, {3, cd}, {3, 0}, {3, 0}, {primary, A.use_diis ? 1 : 0}};
+    const double rs[7][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {-2.0, 0.0}, {0.0, 0.85}};
+    for (int k = 0; k < 7; ++k) {
+      ++paths;
+      PathResult c = uhf_loop(A, M, B, Bb, Pt, rg[k][0], rg[k][1], rs[k][0], rs[k][1], lds, red, D, true);
+      if (uhf_aufbau_broken(A, M, B, Bb))
+        c = uhf_loop(A, M, B, Bb, Pt, rg[k][0], rg[k][1], rs[k][0], rs[k][1], lds, red, D, false);
+      if (!c.converged) continue;
+      if (!best.converged || c.e_elec < best.e_elec - 1.0e-6) {
+        best = c;
+        keep();
+      }
+    }
+    for (int e = tid; e < np * np; e += NT) {
+      B.P[e] = B.Pb[e];
+      B.C[e] = B.Cb[e];
+      Bb.P[e] = Bb.Pb[e];
+      Bb.C[e] = Bb.Cb[e];
+    }
+    for (int j = tid; j < n; j += NT) {
+      B.eps[j] = B.epsb[j];
+      Bb.eps[j] = Bb.epsb[j];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < np * np; e += NT) Pt[e] = B.P[e] + Bb.P[e];
+  __syncthreads();
+  double dip[3];
+  charges_dipole(A, M, Pt, red, dip);
+  uhf_frontier(M, B, Bb, homo, lumo);
+  if (tid == 0) {
+    MolOut o;
+    o.dip[0] = dip[0]; o.dip[1] = dip[1]; o.dip[2] = dip[2];
+    o.e_elec = best.e_elec;
+    o.e_core = ecore;
+    o.last_dp = best.last_dp;
+    o.first_e = first_e;
+    o.homo = homo > -1.0e299 ? homo : 0.0;
+    o.lumo = lumo < 1.0e299 ? lumo : 0.0;
+    o.iters = best.iters;
+    o.converged = best.converged;
+    o.paths = paths;
+    o.status = best.converged ? 0 : 3;
+    o.unrestricted = 1;
+    o.guess_used = 0;
+    A.out[mi] = o;
+  }
+  if (md.out_mat >= 0) {
+    for (int e = tid; e < n * n; e += NT) {
+      const int i = e / n, j = e % n;
+      A.out_P[md.out_mat + e] = Pt[i * np + j];
+      if (A.out_S) A.out_S[md.out_mat + e] = B.P[i * np + j] - Bb.P[i * np + j];
+      if (!A.purify) {
+        A.out_C[md.out_mat + e] = B.C[i * np + j];
+        if (A.out_Cb) A.out_Cb[md.out_mat + e] = Bb.C[i * np + j];
+      }
+    }
+    if (!A.purify)
+      for (int j = tid; j < n; j += NT) {
+        A.out_eps[md.out_vec + j] = B.eps[j];
+        if (A.out_epsb) A.out_epsb[md.out_vec + j] = Bb.eps[j];
+      }
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
   double* lds = g_lds;
   __shared__ double red[NW * RED_K];
@@ -2999,6 +3453,7 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
   __shared__ int s_mol;
   __shared__ unsigned long long s_ph[PH_SLOTS];
   __shared__ double s_bnd[4];
+  __shared__ double s_bnd_b[4];
   const int tid = threadIdx.x;
   double* Wbase;
   Bufs B = slab(A, Wbase);
@@ -3030,6 +3485,10 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
     zero_mats(M, B);
     const double ecore = finish_core(A, M, red);
     stamp(B, 0);
+    if (md.uhf) {
+      uhf_molecule(A, M, md, mi, B, ecore, lds, red, D, s_bnd_b);
+      continue;
+    }
     PathResult best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D, true, true);
     if (aufbau_broken(A, M, B)) best = rhf_loop(A, M, B, A.guess, A.accel, lds, red, D, false, true);
     const int guess_used = best.guess_used;
@@ -3092,6 +3551,7 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
       o.converged = best.converged;
       o.paths = paths;
       o.status = best.converged ? 0 : 3;
+      o.unrestricted = 0;
       o.guess_used = guess_used;
       A.out[mi] = o;
     }
@@ -3431,7 +3891,7 @@ __global__ __launch_bounds__(LG_THREADS) void lg_sad_kernel(KArgs A, MolDesc md,
 __global__ __launch_bounds__(LG_THREADS) void lg_fock_pairs_kernel(KArgs A, MolDesc md, const double* P, double* F,
                                                                    double* slot) {
   const Mol M = mol_of(A, md, slot);
-  fock_pairs(A, M, M.H, P, F, gtid(), gstride());
+  fock_pairs(A, M, M.H, P, P, 0.5, F, gtid(), gstride());
 }
 
 // The sums of fock_jsum with one wave per output (heavy atom and component, or hydrogen): lanes
@@ -3474,7 +3934,7 @@ __global__ __launch_bounds__(LG_THREADS) void lg_fock_jsum_wide_kernel(KArgs A, 
 __global__ __launch_bounds__(LG_THREADS) void lg_fock_diag_kernel(KArgs A, MolDesc md, const double* P, double* F,
                                                                   const double* J) {
   const Mol M = mol_of(A, md, nullptr);
-  fock_diag(A, M, M.H, P, F, J, gtid(), gstride());
+  fock_diag(A, M, M.H, P, P, 0.5, F, J, gtid(), gstride());
 }
 
 // DIIS entry of one iteration: E = T - T^T with T = F P (commutator, src/scf.rs:1039-1054), and
@@ -3601,6 +4061,7 @@ struct Batch {
   bool warm_start = false;    // solves after the first start from each molecule's last density
   bool have_prev = false;     // out_P / out hold densities carried in from another batch
   bool purify = false;  // densities by purification; orbitals are then not produced
+  bool any_uhf = false; // some molecule takes the unrestricted path (the slab then holds a beta part)
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
   void* bufs[64] = {nullptr};
@@ -4545,7 +5006,6 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
     const int n_alpha = (int)((ne + nu) / 2), n_beta = (int)((ne - nu) / 2);
     if (opt.reference == 1 && n_alpha != n_beta) { st = 1; continue; }
     const bool use_uhf = opt.reference == 2 || n_alpha != n_beta;
-    if (use_uhf) { st = 4; continue; }
     {
       // exact size of the reference's pair cache: sum_{i<j} p_i p_j packed doubles and a 72-byte
       // PairIntegral record per pair (pair_cache_bytes, src/hamiltonian.rs:51-71)
@@ -4595,7 +5055,9 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
     d.nat = m.n_atoms;
     d.nheavy = nh;
     d.nao = nao;
-    d.nocc = (int)(ne / 2);
+    d.nocc = n_alpha;
+    d.nbeta = n_beta;
+    d.uhf = use_uhf ? 1 : 0;
     d.np = (nao + 15) & ~15;
     // eigensolver LDS: packed matrix + rotation schedule, plus the eigenvectors when they fit
     // (with purification the eigensolver only runs when a gap closes: its vectors stay in HBM)
@@ -4610,13 +5072,15 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
     const long long lneed = (long long)std::max(jl, m.n_atoms * 16) * 8;
     // molecules that do not fit one workgroup's LDS take the host-driven multi-kernel path
     const bool large = lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || req.force_large;
-    if (d.nocc > nao || (large && nao > LARGE_MAX_NAO)) {
-      st = d.nocc > nao ? 1 : 5;
+    // unrestricted molecules take the single-workgroup path only
+    if (d.nocc > nao || (large && nao > LARGE_MAX_NAO) || (large && use_uhf)) {
+      st = d.nocc > nao ? 1 : large && use_uhf ? 4 : 5;
       aslot.resize(d.atom0); aoff.resize(d.atom0); ax.resize(d.atom0); ay.resize(d.atom0); az.resize(d.atom0);
       d.nat = 0;
       continue;
     }
     if (!large) {
+      b.any_uhf = b.any_uhf || use_uhf;
       b.lds_need = std::max(b.lds_need, (int)lneed);
       np_max = std::max(np_max, d.np);
       const long long npairs = (long long)d.nat * (d.nat - 1) / 2;
@@ -4644,7 +5108,7 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
   for (long long t = 0; t < b.atoms_total; ++t) elem8[t] = (int8_t)pm3guess::elem_index(P.table.el[aslot[t]].z);
   for (int k : b.solve) {
     MolDesc& d = b.desc[k];
-    bool ok = true;
+    bool ok = d.uhf == 0;  // the learned guess covers restricted molecules
     for (int t = 0; t < d.nat; ++t) ok = ok && elem8[d.atom0 + t] >= 0;
     d.learn = ok ? 1 : 0;
     d.gp0 = (long long)gsrc.size();
@@ -4709,7 +5173,8 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
   HIP_OK(hipGetDeviceProperties(&prop, dev));
   b.nwg = std::max(1, std::min(std::max(b.nsolve, 1), prop.multiProcessorCount));
   const long long m2 = (long long)np_max * np_max;
-  const long long stride = ((11 + 3 * MAXD) * m2 + w_cap + 6LL * np_max + 64 + 31) & ~31LL;
+  const long long ustride = b.any_uhf ? ((9 + 2 * MAXD) * m2 + 4LL * np_max + 31) & ~31LL : 0;
+  const long long stride = (((11 + 3 * MAXD) * m2 + w_cap + 6LL * np_max + 64 + 31) & ~31LL) + ustride;
   KArgs& a = b.args;
   MolDesc* d_desc; int *d_order, *d_queue, *d_aslot, *d_aoff;
   double *d_ax, *d_ay, *d_az, *d_scr, *d_P, *d_C, *d_eps, *d_q, *d_store, *d_g;
@@ -4771,6 +5236,14 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
   a.out_eps = d_eps;
   a.out_q = d_q;
   a.out_g = d_g;
+  a.ustride = ustride;
+  a.out_S = nullptr;
+  a.out_Cb = nullptr;
+  a.out_epsb = nullptr;
+  if (b.any_uhf) {
+    if (!b.alloc(a.out_S, b.mat_total, err)) return false;
+    if (!b.purify && (!b.alloc(a.out_Cb, b.mat_total, err) || !b.alloc(a.out_epsb, b.vec_total, err))) return false;
+  }
   a.store = d_store;
   a.pair_base = d_pbase;
   a.npairs_total = pair_base.back();
@@ -4991,18 +5464,28 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
   }
   HIP_OK(scopy(b.stream, ho.data(), b.args.out, sizeof(MolOut) * b.nmol, hipMemcpyDeviceToHost));
   HIP_OK(scopy(b.stream, hq.data(), b.args.out_q, sizeof(double) * b.atoms_total, hipMemcpyDeviceToHost));
-  std::vector<double> hP, hC, hE;
+  std::vector<double> hP, hC, hE, hS, hCb, hEb;
   if (b.mat_total > 0) {
     // densities stay on the device unless asked for (a gradient batch keeps its own there)
     if (b.keep_density) {
       hP.resize(b.mat_total);
       HIP_OK(scopy(b.stream, hP.data(), b.args.out_P, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
+      if (b.args.out_S) {
+        hS.resize(b.mat_total);
+        HIP_OK(scopy(b.stream, hS.data(), b.args.out_S, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
+      }
     }
     if (!b.purify) {
       hC.resize(b.mat_total);
       hE.resize(b.vec_total);
       HIP_OK(scopy(b.stream, hC.data(), b.args.out_C, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
       HIP_OK(scopy(b.stream, hE.data(), b.args.out_eps, sizeof(double) * b.vec_total, hipMemcpyDeviceToHost));
+      if (b.args.out_Cb) {
+        hCb.resize(b.mat_total);
+        hEb.resize(b.vec_total);
+        HIP_OK(scopy(b.stream, hCb.data(), b.args.out_Cb, sizeof(double) * b.mat_total, hipMemcpyDeviceToHost));
+        HIP_OK(scopy(b.stream, hEb.data(), b.args.out_epsb, sizeof(double) * b.vec_total, hipMemcpyDeviceToHost));
+      }
     }
   }
   for (int k : b.order_all) {
@@ -5022,7 +5505,8 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
     r.guess_used = o.guess_used;
     r.nao = d.nao;
     r.nocc = d.nocc;
-    r.nbeta = d.nocc;
+    r.nbeta = d.nbeta;
+    r.unrestricted = o.unrestricted;
     for (int x = 0; x < 3; ++x) r.dipole[x] = o.dip[x] * AU_DIPOLE_TO_DEBYE;
     r.charges.assign(d.nat, 0.0);
     for (int t = 0; t < d.nat; ++t) r.charges[b.atom_map[k][t]] = hq[d.atom0 + t];
@@ -5046,6 +5530,12 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
         const double* Ps = hP.data() + d.out_mat;
         for (int i = 0; i < n; ++i)
           for (int j = 0; j < n; ++j) r.density[(size_t)mp[i] * n + mp[j]] = Ps[(size_t)i * n + j];
+        if (r.unrestricted && !hS.empty()) {
+          r.spin_density.assign((size_t)n * n, 0.0);
+          const double* Ss = hS.data() + d.out_mat;
+          for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) r.spin_density[(size_t)mp[i] * n + mp[j]] = Ss[(size_t)i * n + j];
+        }
       }
       if (!b.purify) {
         r.mo_coeff.assign((size_t)n * n, 0.0);
@@ -5053,6 +5543,13 @@ bool batch_fetch(Batch* bp, const Params& P, const std::vector<MolIn>& mols, std
         const double* Cs = hC.data() + d.out_mat;
         for (int i = 0; i < n; ++i)
           for (int j = 0; j < n; ++j) r.mo_coeff[(size_t)mp[i] * n + j] = Cs[(size_t)i * n + j];
+        if (r.unrestricted && !hCb.empty()) {
+          r.mo_coeff_beta.assign((size_t)n * n, 0.0);
+          r.mo_energies_beta.assign(hEb.begin() + d.out_vec, hEb.begin() + d.out_vec + n);
+          const double* Cb = hCb.data() + d.out_mat;
+          for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) r.mo_coeff_beta[(size_t)mp[i] * n + j] = Cb[(size_t)i * n + j];
+        }
       }
     }
   }

@@ -130,8 +130,9 @@ void pm3_opt_options_default(pm3_opt_options* out) {
 }
 
 pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
-                              const double* charge, const pm3_params* params, const pm3_options* opts,
-                              const pm3_opt_options* opt_in, pm3_opt_result* results, double* xyz_out) {
+                              const double* charge, const int32_t* multiplicity, const pm3_params* params,
+                              const pm3_options* opts, const pm3_opt_options* opt_in, pm3_opt_result* results,
+                              double* xyz_out) {
   if (n_mol < 0 || !params || (n_mol > 0 && (!atom_ptr || !z || !xyz_bohr || !results || !xyz_out)))
     return PM3_INVALID_INPUT;
   if (n_mol > 0 && atom_ptr[0] != 0) return PM3_INVALID_INPUT;
@@ -157,7 +158,7 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
 
   pm3_batch* batch = nullptr;
   std::vector<int> members;  // caller indices of the batch's molecules, in batch order
-  std::vector<int32_t> bptr, bz;
+  std::vector<int32_t> bptr, bz, bm;
   std::vector<double> bx, bq, bgrad;
   std::vector<pm3_status> bst;
   std::vector<pm3_result> bres;
@@ -186,15 +187,17 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
       bz.clear();
       bx.clear();
       bq.clear();
+      bm.clear();
       for (int k : members) {
         const OptState& o = st[k];
         bptr.push_back(bptr.back() + o.nat);
         bz.insert(bz.end(), z + o.a0, z + o.a0 + o.nat);
         bx.insert(bx.end(), o.xt.begin(), o.xt.end());
         bq.push_back(charge ? charge[k] : 0.0);
+        bm.push_back(multiplicity ? multiplicity[k] : 1);
       }
       const pm3_status s = pm3_batch_create((int32_t)members.size(), bptr.data(), bz.data(), bx.data(), bq.data(),
-                                            nullptr, params, opts,
+                                            bm.data(), params, opts,
                                             PM3_BATCH_WANT_GRADIENT | (opt.warm_start ? PM3_BATCH_WARM_START : 0),
                                             &batch);
       pm3_status carried = PM3_OK;
@@ -308,10 +311,10 @@ pm3_status pm3_optimize(const pm3_molecule* mol, const pm3_params* params, const
   if (!mol || !params || !result || !xyz_out || mol->n_atoms < 1 || !mol->z || !mol->xyz_bohr)
     return PM3_INVALID_INPUT;
   if (mol->multiplicity < 1) return PM3_INVALID_INPUT;
-  if (mol->multiplicity != 1) return PM3_UNSUPPORTED;
   const int32_t ptr[2] = {0, mol->n_atoms};
   const double q = mol->charge;
-  const pm3_status s = pm3_optimize_batch(1, ptr, mol->z, mol->xyz_bohr, &q, params, opts, opt, result, xyz_out);
+  const int32_t mult = mol->multiplicity;
+  const pm3_status s = pm3_optimize_batch(1, ptr, mol->z, mol->xyz_bohr, &q, &mult, params, opts, opt, result, xyz_out);
   return s != PM3_OK ? s : (pm3_status)result->status;
 }
 

@@ -209,8 +209,9 @@ def native():
         L.pm3_batch_carry_densities.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ip]
         L.pm3_opt_options_default.argtypes = [ctypes.POINTER(_COptOptions)]
         L.pm3_opt_options_default.restype = None
-        L.pm3_optimize_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ctypes.c_void_p, ctypes.POINTER(_COptions),
-                                         ctypes.POINTER(_COptOptions), ctypes.POINTER(_COptResult), dp]
+        L.pm3_optimize_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p,
+                                         ctypes.POINTER(_COptions), ctypes.POINTER(_COptOptions),
+                                         ctypes.POINTER(_COptResult), dp]
         L.pm3_optimize.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
                                    ctypes.POINTER(_COptOptions), ctypes.POINTER(_COptResult), dp]
         _lib = L
@@ -688,11 +689,13 @@ def optimize_batch(molecules: List[Molecule], params: Optional[Pm3Parameters] = 
     z = np.ascontiguousarray(np.concatenate([m.z for m in molecules]).astype(np.int32)) if k else np.zeros(0, np.int32)
     xyz = np.ascontiguousarray(np.concatenate([m.positions for m in molecules])) if k else np.zeros((0, 3))
     q = np.ascontiguousarray([float(m.charge) for m in molecules], dtype=np.float64)
+    mult = np.ascontiguousarray([int(m.multiplicity) for m in molecules], dtype=np.int32)
     out = np.zeros_like(xyz)
     res = (_COptResult * max(k, 1))()
     ip = ctypes.POINTER(ctypes.c_int32)
-    _raise(native().pm3_optimize_batch(k, ptr.ctypes.data_as(ip), z.ctypes.data_as(ip), _dp(xyz), _dp(q), params._h,
-                                       ctypes.byref(options._c()), ctypes.byref(opt._c()), res, _dp(out)))
+    _raise(native().pm3_optimize_batch(k, ptr.ctypes.data_as(ip), z.ctypes.data_as(ip), _dp(xyz), _dp(q),
+                                       mult.ctypes.data_as(ip), params._h, ctypes.byref(options._c()),
+                                       ctypes.byref(opt._c()), res, _dp(out)))
     status = np.array([res[i].status for i in range(k)], dtype=np.int32)
     return status, [_opt_result(m, res[i], out[ptr[i]:ptr[i + 1]].copy()) for i, m in enumerate(molecules)]
 
