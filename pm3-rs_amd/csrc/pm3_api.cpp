@@ -147,7 +147,12 @@ pm3_status molecule_status(int st, const MolResult& r) {
     return fail(PM3_SCF_NOT_CONVERGED, "SCF did not converge in " + std::to_string(r.iterations) +
                                            " iterations (last RMS density change " + std::to_string(r.last_change) + ")");
   if (st == PM3_MISSING_ELEMENT)
-    return fail(PM3_MISSING_ELEMENT, "no PM3 parameters for element " + std::to_string(r.missing_z));
+    return fail(PM3_MISSING_ELEMENT, "missing PM3 parameter block for Z=" + std::to_string(r.missing_z));
+  if (st == PM3_UNSUPPORTED && r.missing_z > 0)
+    return fail(PM3_UNSUPPORTED, "this engine covers the s/p elements H, Li-F, Na-Cl; Z=" + std::to_string(r.missing_z) +
+                                     " is not among them");
+  if (st == PM3_UNSUPPORTED)
+    return fail(PM3_UNSUPPORTED, "unrestricted molecules above the single-workgroup size (about 190 AOs) are not handled");
   if (st == PM3_RESOURCE_LIMIT && r.limit_mb > 0.0)
     return fail(PM3_RESOURCE_LIMIT, "PM3 two-electron pair cache needs " + std::to_string((long long)r.required_mb) +
                                         " MiB, above the limit of " + std::to_string((long long)r.limit_mb) + " MiB");

@@ -4933,6 +4933,14 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
                             const RunRequest& req, Batch*& out_batch, std::string& err, unsigned char* arena,
                             size_t arena_bytes, hipStream_t arena_stream);
 
+// Atomic numbers with a PM3 parameter block in the reference (the rows of its
+// src/data/pm3_parameters.csv, plus the lanthanide sparkles 57-71 and MOPAC's capped bond 102).
+// An element there that this engine lacks (d shells, principal quantum number 4 and above, the
+// noble gases) is refused as unsupported; any other is a missing element (src/error.rs:19-24).
+static bool reference_parameterises(int z) {
+  return (z >= 1 && z <= 20) || (z >= 30 && z <= 38) || (z >= 48 && z <= 71) || (z >= 80 && z <= 83) || z == 102;
+}
+
 bool batch_create(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt, const RunRequest& req,
                   Batch*& out_batch, std::string& err) {
   return batch_create_in(P, mols, opt, req, out_batch, err, nullptr, 0, nullptr);
@@ -4990,7 +4998,7 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
     for (int i = 0; i < m.n_atoms && !st; ++i) {
       const int z = m.z[i];
       if (z < 0 || z >= 128 || P.slot_of_z[z] < 0) {
-        st = 2;
+        st = reference_parameterises(z) ? 4 : 2;
         b.pre_z[k] = z;
       } else {
         nel += P.table.el[P.slot_of_z[z]].tore;

@@ -58,7 +58,13 @@ class ScfNotConverged(Pm3Error):
 
 
 class Unsupported(Pm3Error):
+    """An input the reference handles and this engine does not (Pm3Error::UnsupportedDOrbitals(z) for
+    an element outside its s/p set; ``z`` is 0 otherwise)."""
     status = 4
+
+    def __init__(self, msg, z=0):
+        super().__init__(msg)
+        self.z = z
 
 
 class ResourceLimit(Pm3Error):
@@ -291,8 +297,8 @@ def _raise(status, res=None):
     cls = _ERRORS.get(status, Pm3Error)
     if cls is ScfNotConverged and res is not None:
         raise ScfNotConverged(msg, res.iterations, res.last_change)
-    if cls is MissingElement and res is not None:
-        raise MissingElement(msg, res.missing_z)
+    if cls in (MissingElement, Unsupported) and res is not None:
+        raise cls(msg, res.missing_z)
     if cls is ResourceLimit and res is not None:
         raise ResourceLimit(msg, res.required_mb, res.limit_mb)
     raise cls(msg)
@@ -449,9 +455,12 @@ def run_pm3(molecule: Molecule, params: Optional[Pm3Parameters] = None, options:
     options = options or Pm3Options()
     cm = molecule._c()
     nao = native().pm3_query_nao(ctypes.byref(cm), params._h)
-    if nao < 0:
-        bad = [int(z) for z in molecule.z if int(z) not in SYMBOLS.values() or int(z) in (2, 10, 18, 19, 20)]
-        raise MissingElement(f"no PM3 parameters for element {bad[0] if bad else '?'}", bad[0] if bad else 0)
+    if nao < 0:  # an element without parameters here: the library reports which and why
+        res = _CResult()
+        null = ctypes.POINTER(ctypes.c_double)()
+        st = native().pm3_run(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(res), null, null,
+                              null, null, null, null, null)
+        _raise(st if st else 2, res)
     res = _CResult()
     P = np.zeros((nao, nao))
     S = np.zeros((nao, nao))

@@ -250,9 +250,14 @@ def test_error_surface(G):
     with pytest.raises(G.ScfNotConverged) as e:
         G.run_pm3(G.Molecule(z, x), options=G.Pm3Options(max_scf=1))
     assert e.value.iterations == 1
+    # francium has no PM3 block anywhere (tests/mopac_oracle.rs:249-262): MissingElement(87), named
     with pytest.raises(G.MissingElement) as e:
+        G.run_pm3(G.Molecule(np.array([87, 17]), np.array([[0, 0, 0], [5.6, 0, 0]])))
+    assert e.value.z == 87 and "87" in str(e.value)
+    # argon has one in the reference, but no closed-form s/p path here: refused as unsupported
+    with pytest.raises(G.Unsupported) as e:
         G.run_pm3(G.Molecule(np.array([18, 1]), np.array([[0, 0, 0], [2.5, 0, 0]])))
-    assert e.value.z == 18 and "18" in str(e.value)  # MissingElement(z) carries the element
+    assert e.value.z == 18 and "18" in str(e.value)
     with pytest.raises(G.InvalidInput):
         G.run_pm3(G.Molecule(np.array([1, 1]), np.zeros((2, 3))))
     with pytest.raises(G.InvalidInput):  # odd electron count with a singlet (src/scf.rs:497-503)
