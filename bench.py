@@ -10,6 +10,11 @@ guess, timed the same way.  A learned solve counts only if it converged and its 
 formation is within 1e-5 kcal/mol of the plain GPU solve of the same molecule (and the CPU sample
 within 1e-5 of the CPU restatement).
 
+With no ``--workload`` the line's headline is mixed4096 (learned vs plain) and ``configs`` holds the
+other configurations, each measured after it with the built-in tuning table: water32 (one 32-water
+cluster), water512 (SCF + gradient of one 512-water cluster) and conformers2048 (SCF + gradient,
+learned guess), each with its roofline and CPU baseline.
+
 Workloads (``--workload``):
   mixed4096      4096 molecules: alkanes CnH2n+2 (n in [1,24]) and water clusters (2-16 waters),
                  every atom jittered by sigma 0.05 A (default; the batched distribution the
@@ -44,9 +49,11 @@ FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
 NET_FLOP_PER_PAIR = 2.0 * (96 * 128 + 128 * 128 + 128 * 16)  # forward pass per directed pair
 # HBM bytes per scf_batch_kernel launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a
-# 512 MB copy: x2.0 / x1.0) for this build on the default workload, per leg
-# (profiles/scf_batch_kernel_traffic_mixed4096.md); not measured for other workloads.
+# 512 MB copy: x2.0 / x1.0) on the default workload, per leg.  A recorded value, not measured by this
+# run: the JSON line carries it with its source (the profile summary and the build it was taken on).
 MEASURED_TRAFFIC = {("mixed4096", "learned"): 1.295e11, ("mixed4096", "plain"): 1.771e11}
+TRAFFIC_SOURCE = ("recorded, not measured in this run: profiles/scf_batch_kernel_traffic_mixed4096.md "
+                  "(rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of bench.py --steps 1)")
 HBM_PEAK_GBS = 8000.0
 HOF_TOL = 1.0e-5
 GRAD_TOL = 2.295e-7  # eV/Bohr, max |dg| against the CPU path
@@ -54,6 +61,20 @@ GRAD_TOL = 2.295e-7  # eV/Bohr, max |dg| against the CPU path
 # eV/Bohr, and on soft modes (methyl torsions, intermolecular modes) that leaves up to about 1e-3
 # kcal/mol between two stopping points once the trajectories have parted.
 OPT_HOF_TOL = 1.0e-3
+
+
+def cpu_info(threads):
+    """The host the CPU baseline ran on: CPU model, logical cores, threads used."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "threads": threads}
 
 
 def build_workload(name, seed):
@@ -78,18 +99,49 @@ def algorithmic_flops(naos, results):
     return tot
 
 
-def bench_large(args):
+def large_cpu_baseline(z, x, iterations, threads):
+    """The CPU path of one water512 solve, priced per SCF iteration: the restatement's Fock build at
+    full size (orc_fock minus its H_core, which the reference builds once) plus its dense eigensolve,
+    timed at n = 768 and scaled by (3072 / 768)^3 (a full-size eigensolve takes minutes on one
+    thread), times the GPU run's iteration count."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O
+    n = O.nao(z)
+    P = np.zeros((n, n))
+    t0 = time.perf_counter()
+    O.hcore(z, x)
+    t1 = time.perf_counter()
+    O.fock(z, x, P)
+    t2 = time.perf_counter()
+    m = 768
+    A = np.random.default_rng(0).standard_normal((m, m))
+    A = A + A.T
+    t3 = time.perf_counter()
+    O.eigen(A)
+    t4 = time.perf_counter()
+    fock_s = max((t2 - t1) - (t1 - t0), 0.0)
+    eig_s = (t4 - t3) * (n / m) ** 3
+    per_iter = fock_s + eig_s
+    return {"value": 1.0 / (per_iter * iterations), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"one SCF iteration priced from the C++ restatement: Fock build at {n} AOs "
+                      f"({fock_s:.2f} s) + dense eigensolve timed at {m} and scaled by ({n}/{m})^3 "
+                      f"({eig_s:.1f} s), times {iterations} iterations; one thread (the restatement "
+                      f"does not split a solve across threads); gradient not included",
+            **cpu_info(1)}
+
+
+def run_large(args, steps, warmup, tune, cpu=True):
     """One large molecule (SCF + analytic gradient per step) on the host-driven multi-kernel path."""
     z, x = W.water_cluster(512, 25.0, args.seed)
     mol = G.Molecule(np.asarray(z, np.int32), W.to_bohr(x))
     opts = G.Pm3Options(p_tol=1.0e-8, e_tol=1.0e-8)
-    tuning = apply_tuning(args, [mol], opts, True, reps=1)
+    tuning = apply_tuning(args, [mol], opts, True, reps=1, tune=tune)
     batch = G.Batch([mol], options=opts, want_gradient=True, want_density=True)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         batch.solve()
     kms, lts = [], []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         batch.solve()
         kms.append(batch.last_kernel_ms())
         lts.append(batch.large_timing())
@@ -97,7 +149,7 @@ def bench_large(args):
     st, res = batch.results()
     r = res[0]
     g = batch.gradients()[0]
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    ms_per_step = (t1 - t0) * 1e3 / steps
     P = r.density
     n = P.shape[0]
     idem = float(np.abs((P / 2) @ (P / 2) - P / 2).max())
@@ -116,7 +168,7 @@ def bench_large(args):
     out = {
         "metric": "converged PM3 SCF + analytic gradient solves/sec (one 512-water cluster, 3072 AOs)",
         "value": (1.0 if st[0] == 0 else 0.0) / (ms_per_step * 1e-3),
-        "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "unit": "solves/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded geometry)",
         "config": {"workload": "water512", "molecules": 1, "atoms": len(z), "nao": n, "seed": args.seed,
@@ -132,18 +184,16 @@ def bench_large(args):
                         "block-lower part" if G.tuning_get()["large_split"] > 2 else "whole", n),
                      "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "launch_ms": prod_ms / max(nprod, 1), "launches_per_step": nprod / args.steps,
+                     "launch_ms": prod_ms / max(nprod, 1), "launches_per_step": nprod / steps,
                      "flop_per_launch": prod_flop / max(nprod, 1)},
         "fock_roofline": {"bound": "hbm", "kernel": "lg_fock_pairs + lg_fock_jsum + lg_fock_diag",
                           "achieved": fock_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fock_gbs / HBM_PEAK_GBS,
                           "bytes_per_build": fock_bytes, "build_ms": fock_ms / max(nfock, 1),
-                          "builds_per_step": nfock / args.steps},
-        "cpu_baseline": None,
-        "cpu_baseline_note": "the CPU restatement's dense eigensolver is not bounded at 3072 AOs "
-                             "(one eigensolve takes minutes); the CPU leg is measured on the batched workloads",
+                          "builds_per_step": nfock / steps},
+        "cpu_baseline": large_cpu_baseline(np.asarray(z, np.int32), W.to_bohr(x), r.iterations, 1) if cpu else None,
         "tuning": tuning,
     }
-    print(json.dumps(out))
+    return out
 
 
 def bench_optimize(args):
@@ -240,70 +290,32 @@ def bench_optimize(args):
     print(json.dumps(out))
 
 
-def apply_tuning(args, molecules, options, want_gradient, reps=2):
-    """Set the process-wide tuning table as --tune asks; returns its record for the JSON line."""
-    if args.tune == "off":
+def apply_tuning(args, molecules, options, want_gradient, reps=2, tune=None):
+    """Set the process-wide tuning table as --tune asks (or `tune`); returns its record."""
+    tune = tune or args.tune
+    if tune == "off":
         G.tuning_set(G.tuning_default())
-        return {"source": "built-in", "table": G.tuning_get()}
-    if args.tune != "auto":
-        G.tuning_load(args.tune)
-        return {"source": args.tune, "table": G.tuning_get()}
+        return {"source": "built-in (the table pm3_autotune keeps on this workload, profiles/autotune_tables.md)",
+                "table": G.tuning_get()}
+    if tune != "auto":
+        G.tuning_load(tune)
+        return {"source": tune, "table": G.tuning_get()}
     t0 = time.perf_counter()
     table, rep = G.autotune(molecules, options=options, want_gradient=want_gradient, reps=reps)
     return {"source": "pm3_autotune on this workload (untimed, before the legs)", "table": table, "report": rep,
             "seconds": time.perf_counter() - t0}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed4096")
-    ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--cpu-sample", type=int, default=512, help="molecules in the CPU-baseline sample (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--gradient", action="store_true", help="also compute analytic gradients in every step")
-    ap.add_argument("--p-tol", type=float, default=0.0,
-                    help="density convergence (default 1e-8; 1e-10 with --gradient, where the gradient check needs it)")
-    ap.add_argument("--copy-calibration-mb", type=float, default=0.0,
-                    help="profiling helper: run only a plain device copy of this size (for counter scaling)")
-    ap.add_argument("--guess-model", default="", help="learned-guess network file (default: the packaged one)")
-    ap.add_argument("--plain-only", action="store_true", help="time only the superposition-guess leg")
-    ap.add_argument("--tune", default="auto",
-                    help="launch-shape / tile table: 'auto' runs pm3_autotune on the workload first (untimed), "
-                         "'off' keeps the built-in table, anything else is a table file to load")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; sharding a batch over several GPUs is "
-                                   "designed but not launched", "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    if args.workload == "water512":
-        return bench_large(args)
-    if args.workload == "optimize512":
-        return bench_optimize(args)
-
-    if args.copy_calibration_mb > 0:
-        import torch
-        n = int(args.copy_calibration_mb * 1e6 / 8)
-        x = torch.ones(n, dtype=torch.float64, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(5):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": 8 * n}))
-        return
-
+def run_batched(args, workload, gradient, steps, warmup, cpu_sample, tune, learned=True):
+    """The batched workloads: plain and learned legs over the same device-resident batch."""
     # The gradient at the converged density inherits the density's convergence error, so the
     # gradient runs converge to 1e-10 (energy 1e-12), the tolerance of the gradient parity tests.
-    p_tol = args.p_tol or (1.0e-10 if args.gradient else 1.0e-8)
+    p_tol = args.p_tol or (1.0e-10 if gradient else 1.0e-8)
     e_tol = 1.0e-12 if p_tol < 1.0e-8 else 1.0e-8
-    mols = build_workload(args.workload, args.seed)
+    mols = build_workload(workload, args.seed)
     molecules = [G.Molecule(z, x) for z, x in mols]
     model = None
-    if not args.plain_only:
+    if not args.plain_only and learned:
         import pm3amd.guess as GS
         path = args.guess_model or GS.DEFAULT_MODEL
         if os.path.exists(path):
@@ -311,15 +323,15 @@ def main():
 
     tune_opts = (G.Pm3Options(p_tol=p_tol, e_tol=e_tol, guess="learned", guess_model=model) if model is not None
                  else G.Pm3Options(p_tol=p_tol, e_tol=e_tol))
-    tuning = apply_tuning(args, molecules, tune_opts, args.gradient)
+    tuning = apply_tuning(args, molecules, tune_opts, gradient, tune=tune)
 
     def leg(opts):
-        batch = G.Batch(molecules, options=opts, want_gradient=args.gradient)
-        for _ in range(args.warmup):
+        batch = G.Batch(molecules, options=opts, want_gradient=gradient)
+        for _ in range(warmup):
             batch.solve()
         kms, pms, gms, nms = [], [], [], []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             batch.solve()
             kms.append(batch.last_kernel_ms())
             pms.append(batch.last_pair_ms())
@@ -329,7 +341,7 @@ def main():
         st, rs = batch.results()
         leg.network_ms = float(np.mean(nms))
         leg.pairs = batch.guess_timing()[2]
-        return batch, st, rs, (t1 - t0) * 1e3 / args.steps, float(np.mean(kms)), float(np.mean(pms)), float(np.mean(gms))
+        return batch, st, rs, (t1 - t0) * 1e3 / steps, float(np.mean(kms)), float(np.mean(pms)), float(np.mean(gms))
 
     pbatch, pstatus, pres, p_ms, p_kms, p_pms, _ = leg(G.Pm3Options(p_tol=p_tol, e_tol=e_tol))
     if model is not None:
@@ -352,22 +364,22 @@ def main():
     max_dhf = None
     checked = 0
     hof_ok = True
-    if args.cpu_sample > 0:
+    if cpu_sample > 0:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
         import oracle as O
-        stride = max(1, len(mols) // args.cpu_sample)
-        idx = list(range(0, len(mols), stride))[: args.cpu_sample]
+        stride = max(1, len(mols) // cpu_sample)
+        idx = list(range(0, len(mols), stride))[:cpu_sample]
         sample = [mols[i] for i in idx]
         threads = min(args.cpu_threads, os.cpu_count() or 1)
         c0 = time.perf_counter()
-        if args.gradient:
+        if gradient:
             # the CPU path does the step's whole work: SCF, then closed_form_gradient
             _, hof_cpu, _, st_cpu, g_cpu = O.gradient_batch(sample, O.options(p_tol=p_tol, e_tol=e_tol),
                                                             nthreads=threads)
         else:
             _, hof_cpu, _, st_cpu = O.run_batch(sample, O.options(p_tol=p_tol, e_tol=e_tol), nthreads=threads)
         c1 = time.perf_counter()
-        if args.gradient:
+        if gradient:
             gpu_g = batch.gradients()
             both = [k for k, i in enumerate(idx) if st_cpu[k] == 0 and status[i] == 0]
             gmax = max((float(np.abs(gpu_g[idx[k]] - g_cpu[k]).max()) for k in both), default=None)
@@ -382,10 +394,24 @@ def main():
         if grad_check is not None and grad_check["max_abs_dgrad_ev_per_bohr"] is not None:
             grad_check["within_tolerance"] = grad_check["max_abs_dgrad_ev_per_bohr"] <= GRAD_TOL
             hof_ok = hof_ok and grad_check["within_tolerance"]
-        cpu = {"value": len(sample) / (c1 - c0), "unit": "solves/s", "cores": threads, "kind": "port",
-               "sample": f"every {stride}th molecule of the same workload ({len(sample)} molecules), "
-                         f"C++ restatement of the reference CPU path, one "
-                         f"{'SCF + gradient' if args.gradient else 'solve'} per thread"}
+        # the other variant: one solve at a time (the restatement runs one solve on one thread, so
+        # this is the single-solve rate; the faster of the two is the baseline)
+        nseq = min(len(sample), max(1, threads // 2))
+        s0 = time.perf_counter()
+        if gradient:
+            O.gradient_batch(sample[:nseq], O.options(p_tol=p_tol, e_tol=e_tol), nthreads=1)
+        else:
+            O.run_batch(sample[:nseq], O.options(p_tol=p_tol, e_tol=e_tol), nthreads=1)
+        s1 = time.perf_counter()
+        per_thread = len(sample) / (c1 - c0)
+        one_at_a_time = nseq / (s1 - s0)
+        what = "SCF + gradient" if gradient else "solve"
+        cpu = {"value": max(per_thread, one_at_a_time), "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": f"every {stride}th molecule of the same workload ({len(sample)} molecules), C++ "
+                         f"restatement of the reference CPU path, one {what} per thread on {threads} threads "
+                         f"({per_thread:.3g}/s); one {what} at a time on one thread over the first {nseq} "
+                         f"({one_at_a_time:.3g}/s); value is the faster",
+               "one_per_thread": per_thread, "one_at_a_time": one_at_a_time, **cpu_info(threads)}
 
     valid = int(np.sum(ok_vs_plain)) if hof_ok else 0
     value = valid / (ms_per_step * 1e-3)
@@ -393,19 +419,21 @@ def main():
     plain_value = plain_valid / (p_ms * 1e-3)
     used = np.array([r.guess_used for r in res])
     out = {
-        "metric": "converged PM3 SCF solves/sec, learned-guess vs plain, at |dHf|<=1e-5 kcal/mol",
+        "metric": ("converged PM3 SCF + analytic gradient solves/sec (conformer-gradients/sec), learned-guess vs "
+                   "plain, at |dHf|<=1e-5 kcal/mol and max|dg|<=2.295e-7 eV/Bohr" if gradient else
+                   "converged PM3 SCF solves/sec, learned-guess vs plain, at |dHf|<=1e-5 kcal/mol"),
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded geometries)",
-        "config": {"workload": args.workload, "molecules": len(mols), "seed": args.seed, "p_tol": p_tol, "e_tol": e_tol,
+        "config": {"workload": workload, "molecules": len(mols), "seed": args.seed, "p_tol": p_tol, "e_tol": e_tol,
                    "guess": "learned" if model is not None else "sad",
                    "parallelism": "one workgroup per molecule, persistent launch"},
         "converged": converged,
@@ -427,14 +455,16 @@ def main():
             "model_steps": model.steps},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": MEASURED_TRAFFIC.get((args.workload, "learned" if model is not None else "plain"))
+                     "traffic": MEASURED_TRAFFIC.get((workload, "learned" if model is not None else "plain"))
                      if args.seed == 2026 else None,
+                     "traffic_source": TRAFFIC_SOURCE if (workload, "learned") in MEASURED_TRAFFIC and args.seed == 2026
+                     else "not measured for this workload",
                      "kernel": "scf_batch_kernel", "kernel_ms": k_ms, "algorithmic_flop": flops,
                      "pair_kernel_ms": pair_ms},
         "cpu_baseline": cpu,
         "tuning": tuning,
     }
-    if args.gradient:
+    if gradient:
         out["gradient_check"] = grad_check
         out["config"]["gradient"] = True
     if os.environ.get("PM3_PHASE_TIMERS") == "1":
@@ -443,6 +473,69 @@ def main():
         tot = sum(ticks.values()) or 1
         out["phase_share"] = {k: round(v / tot, 4) for k, v in ticks.items()}
         out["phase_counts"] = counts
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="mixed4096")
+    ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--cpu-sample", type=int, default=512, help="molecules in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--gradient", action="store_true", help="also compute analytic gradients in every step")
+    ap.add_argument("--p-tol", type=float, default=0.0,
+                    help="density convergence (default 1e-8; 1e-10 with --gradient, where the gradient check needs it)")
+    ap.add_argument("--copy-calibration-mb", type=float, default=0.0,
+                    help="profiling helper: run only a plain device copy of this size (for counter scaling)")
+    ap.add_argument("--guess-model", default="", help="learned-guess network file (default: the packaged one)")
+    ap.add_argument("--plain-only", action="store_true", help="time only the superposition-guess leg")
+    ap.add_argument("--tune", default="auto",
+                    help="launch-shape / tile table: 'auto' runs pm3_autotune on the workload first (untimed), "
+                         "'off' keeps the built-in table, anything else is a table file to load")
+    ap.add_argument("--configs", default="all",
+                    help="with the default workload: the other configurations measured after the headline, as "
+                         "sub-records of the same line ('all', 'none', or a comma list of water32, water512, "
+                         "conformers2048)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; sharding a batch over several GPUs is "
+                                   "designed but not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    if args.workload == "water512":
+        print(json.dumps(run_large(args, args.steps, args.warmup, args.tune)))
+        return
+    if args.workload == "optimize512":
+        return bench_optimize(args)
+
+    if args.copy_calibration_mb > 0:
+        import torch
+        n = int(args.copy_calibration_mb * 1e6 / 8)
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 8 * n}))
+        return
+
+    out = run_batched(args, args.workload, args.gradient, args.steps, args.warmup, args.cpu_sample, args.tune)
+    if args.workload == "mixed4096" and args.configs != "none" and not args.gradient:
+        # The other configurations, each timed on its own after the headline with the built-in tuning
+        # table (what the tuner keeps on them); sub-records, not part of the headline value.
+        want = {"water32", "water512", "conformers2048"} if args.configs == "all" else set(args.configs.split(","))
+        sub = {"C2_C3_mixed4096": "this line's headline (value: learned leg; plain: the plain leg)"}
+        if "water32" in want:
+            sub["C0_water32"] = run_batched(args, "water32", False, 10, 2, 1, "off")
+        if "water512" in want:
+            sub["C1_water512"] = run_large(args, 1, 1, "off")
+        if "conformers2048" in want:
+            sub["C4_conformers2048"] = run_batched(args, "conformers2048", True, 2, 1, 64, "off")
+        out["configs"] = sub
     print(json.dumps(out))
 
 
