@@ -215,18 +215,18 @@ def bench_optimize(args):
     opt = G.OptOptions()
     tuning = apply_tuning(args, mols, opts, True, reps=1)
     for _ in range(args.warmup):
-        G.optimize_batch(mols, options=opts, opt=opt)
+        G.optimize_batch(mols, options=opts, opt=opt, trajectory=False)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        st, rs = G.optimize_batch(mols, options=opts, opt=opt)
+        st, rs = G.optimize_batch(mols, options=opts, opt=opt, trajectory=False)
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     # the same batch with every round's SCF started from the molecule's previous density
     wopt = G.OptOptions(warm_start=True)
-    G.optimize_batch(mols, options=opts, opt=wopt)
+    G.optimize_batch(mols, options=opts, opt=wopt, trajectory=False)
     w0 = time.perf_counter()
     for _ in range(args.steps):
-        wst, wrs = G.optimize_batch(mols, options=opts, opt=wopt)
+        wst, wrs = G.optimize_batch(mols, options=opts, opt=wopt, trajectory=False)
     w1 = time.perf_counter()
     w_ms = (w1 - w0) * 1e3 / args.steps
     wok = np.array([s == 0 and r.converged for s, r in zip(wst, wrs)])

@@ -140,9 +140,10 @@ pm3_status pm3_run_noalloc(const pm3_molecule* mol, const pm3_params* params, co
                            pm3_result* result, double* density, double* charges);
 
 /* closed_form_gradient (src/gradient.rs:201-276): SCF plus the Hellmann-Feynman gradient at the
- * converged density.  gradient[3*n_atoms] in eV/Bohr; density may be NULL. */
+ * converged density.  gradient[3*n_atoms] in eV/Bohr; density[nao*nao] and the Mulliken charges
+ * of GradientResult::scf, charges[n_atoms], may be NULL. */
 pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
-                        pm3_result* result, double* gradient, double* density);
+                        pm3_result* result, double* gradient, double* density, double* charges);
 
 /* Batched solve of many independent molecules in one persistent launch.  Atoms of molecule k are
  * [atom_ptr[k], atom_ptr[k+1]) of z / xyz_bohr; charge may be NULL (all neutral), multiplicity may
@@ -285,23 +286,37 @@ void pm3_opt_options_default(pm3_opt_options* out);
 typedef struct {
   int32_t status; /* PM3_OK, or the failure of the first evaluation */
   int32_t converged, iterations, evaluations;
+  int32_t trajectory_len; /* accepted points, the starting one included (<= max_iter + 1) */
   double energy_ev, heat_of_formation_kcal, max_gradient; /* at the final geometry */
   double first_energy_ev;                                 /* at the starting geometry */
 } pm3_opt_result;
+
+/* OptStep (src/optimizer.rs:37-43) without its positions, which go to a separate array. */
+typedef struct {
+  double energy_ev, heat_of_formation_kcal, max_gradient;
+} pm3_opt_step;
 
 /* optimize (src/optimizer.rs:233-262) for many molecules at once: L-BFGS with the
  * backtracking Armijo search of drive (:77-221) per molecule, every round one batched SCF +
  * gradient solve over all running molecules.  Molecules in the atom_ptr / z / xyz_bohr / charge /
  * multiplicity layout of pm3_run_batch; results[k] per molecule, final positions xyz_out[3 * atom_ptr[n_mol]]
  * (Bohr).  Returns the first device or launch failure, else PM3_OK (per-molecule failures are in
- * results[k].status).  PM3_UNSUPPORTED with PM3_GUESS_LEARNED. */
+ * results[k].status).  PM3_UNSUPPORTED with PM3_GUESS_LEARNED.
+ *
+ * The trajectory (OptResult::trajectory, drive :121-126,207-212: the starting point and every
+ * accepted point) is written when the optional arrays are given, with room for T = max_iter + 1
+ * steps per molecule: steps[k * T + j] and the positions of step j of molecule k at
+ * traj_xyz[3 * (T * atom_ptr[k] + j * n_k)] (Bohr, n_k atoms).  results[k].trajectory_len is
+ * the number of steps written. */
 pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
                               const double* charge, const int32_t* multiplicity, const pm3_params* params,
                               const pm3_options* opts, const pm3_opt_options* opt, pm3_opt_result* results,
-                              double* xyz_out);
-/* optimize for one molecule (pm3_optimize_batch of one); xyz_out[3 * n_atoms]. */
+                              double* xyz_out, pm3_opt_step* steps, double* traj_xyz);
+/* optimize for one molecule (pm3_optimize_batch of one); xyz_out[3 * n_atoms], and optionally
+ * steps[max_iter + 1] and traj_xyz[3 * n_atoms * (max_iter + 1)]. */
 pm3_status pm3_optimize(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
-                        const pm3_opt_options* opt, pm3_opt_result* result, double* xyz_out);
+                        const pm3_opt_options* opt, pm3_opt_result* result, double* xyz_out, pm3_opt_step* steps,
+                        double* traj_xyz);
 
 #ifdef __cplusplus
 }

@@ -495,7 +495,7 @@ pm3_status pm3_run_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* 
 }
 
 pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
-                        pm3_result* result, double* gradient, double* density) {
+                        pm3_result* result, double* gradient, double* density, double* charges) {
   g_last_error.clear();
   if (!params || !gradient) return fail(PM3_INVALID_INPUT, "null argument");
   pm3_status s = check_molecule(mol);
@@ -516,6 +516,7 @@ pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const
   if (r.status == PM3_OK) {
     std::memcpy(gradient, r.gradient.data(), 3 * (size_t)mol->n_atoms * sizeof(double));
     if (density) std::memcpy(density, r.density.data(), (size_t)r.nao * r.nao * sizeof(double));
+    if (charges) std::memcpy(charges, r.charges.data(), (size_t)mol->n_atoms * sizeof(double));
   }
   return molecule_status(r.status, r);
 }

@@ -106,6 +106,15 @@ bool next_trial(OptState& o) {
   return true;
 }
 
+// One OptStep of the trajectory (src/optimizer.rs:121-126,207-212): the point just accepted.
+void record(const OptState& o, pm3_opt_result& r, size_t cap, pm3_opt_step* steps, double* traj_xyz) {
+  const size_t j = (size_t)r.trajectory_len;
+  if (j >= cap) return;
+  if (steps) steps[(size_t)o.k * cap + j] = pm3_opt_step{o.energy, o.heat, o.maxg};
+  if (traj_xyz) std::memcpy(traj_xyz + 3 * (cap * (size_t)o.a0 + j * (size_t)o.nat), o.x.data(), sizeof(double) * o.x.size());
+  r.trajectory_len += 1;
+}
+
 bool coincident(const std::vector<double>& x) {
   const size_t n = x.size() / 3;
   for (size_t i = 0; i < n; ++i)
@@ -132,7 +141,7 @@ void pm3_opt_options_default(pm3_opt_options* out) {
 pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
                               const double* charge, const int32_t* multiplicity, const pm3_params* params,
                               const pm3_options* opts, const pm3_opt_options* opt_in, pm3_opt_result* results,
-                              double* xyz_out) {
+                              double* xyz_out, pm3_opt_step* steps, double* traj_xyz) {
   if (n_mol < 0 || !params || (n_mol > 0 && (!atom_ptr || !z || !xyz_bohr || !results || !xyz_out)))
     return PM3_INVALID_INPUT;
   if (n_mol > 0 && atom_ptr[0] != 0) return PM3_INVALID_INPUT;
@@ -144,6 +153,8 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
   if (opt.max_iter < 0 || opt.history < 0 || !(opt.gtol >= 0.0)) return PM3_INVALID_INPUT;
   if (opts && opts->guess == PM3_GUESS_LEARNED) return PM3_UNSUPPORTED;  // neighbour lists are fixed at creation
 
+  // trajectory slots per molecule; the number of accepted points never exceeds max_iter
+  const size_t cap = (size_t)opt.max_iter + 1;
   std::vector<OptState> st(n_mol);
   for (int k = 0; k < n_mol; ++k) {
     OptState& o = st[k];
@@ -258,6 +269,7 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
         r.first_energy_ev = o.energy;
         o.converged = o.maxg < opt.gtol;
         o.iter = 0;
+        record(o, r, cap, steps, traj_xyz);
         loop_top(o, opt, r);
         continue;
       }
@@ -286,6 +298,7 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
         o.maxg = max_abs(gn);
         o.converged = o.maxg < opt.gtol;
         o.iter += 1;
+        record(o, r, cap, steps, traj_xyz);
         loop_top(o, opt, r);
       } else if (!next_trial(o)) {
         o.done = true;  // no progress possible: stop at the current point
@@ -307,14 +320,16 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
 }
 
 pm3_status pm3_optimize(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
-                        const pm3_opt_options* opt, pm3_opt_result* result, double* xyz_out) {
+                        const pm3_opt_options* opt, pm3_opt_result* result, double* xyz_out, pm3_opt_step* steps,
+                        double* traj_xyz) {
   if (!mol || !params || !result || !xyz_out || mol->n_atoms < 1 || !mol->z || !mol->xyz_bohr)
     return PM3_INVALID_INPUT;
   if (mol->multiplicity < 1) return PM3_INVALID_INPUT;
   const int32_t ptr[2] = {0, mol->n_atoms};
   const double q = mol->charge;
   const int32_t mult = mol->multiplicity;
-  const pm3_status s = pm3_optimize_batch(1, ptr, mol->z, mol->xyz_bohr, &q, &mult, params, opts, opt, result, xyz_out);
+  const pm3_status s =
+      pm3_optimize_batch(1, ptr, mol->z, mol->xyz_bohr, &q, &mult, params, opts, opt, result, xyz_out, steps, traj_xyz);
   return s != PM3_OK ? s : (pm3_status)result->status;
 }
 

@@ -124,9 +124,14 @@ class _COptOptions(ctypes.Structure):
 
 class _COptResult(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("converged", ctypes.c_int32), ("iterations", ctypes.c_int32),
-                ("evaluations", ctypes.c_int32), ("energy_ev", ctypes.c_double),
+                ("evaluations", ctypes.c_int32), ("trajectory_len", ctypes.c_int32), ("energy_ev", ctypes.c_double),
                 ("heat_of_formation_kcal", ctypes.c_double), ("max_gradient", ctypes.c_double),
                 ("first_energy_ev", ctypes.c_double)]
+
+
+class _COptStep(ctypes.Structure):
+    _fields_ = [("energy_ev", ctypes.c_double), ("heat_of_formation_kcal", ctypes.c_double),
+                ("max_gradient", ctypes.c_double)]
 
 
 EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_run", "pm3_gradient",
@@ -159,7 +164,7 @@ def native():
         L.pm3_run.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
                               ctypes.POINTER(_CResult), dp, dp, dp, dp, dp, dp, dp]
         L.pm3_gradient.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
-                                   ctypes.POINTER(_CResult), dp, dp]
+                                   ctypes.POINTER(_CResult), dp, dp, dp]
         L.pm3_run_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p, ctypes.POINTER(_COptions),
                                     ctypes.c_int32, ip, ctypes.POINTER(_CResult), dp]
         L.pm3_batch_create.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p,
@@ -217,9 +222,10 @@ def native():
         L.pm3_opt_options_default.restype = None
         L.pm3_optimize_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p,
                                          ctypes.POINTER(_COptions), ctypes.POINTER(_COptOptions),
-                                         ctypes.POINTER(_COptResult), dp]
+                                         ctypes.POINTER(_COptResult), dp, ctypes.POINTER(_COptStep), dp]
         L.pm3_optimize.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
-                                   ctypes.POINTER(_COptOptions), ctypes.POINTER(_COptResult), dp]
+                                   ctypes.POINTER(_COptOptions), ctypes.POINTER(_COptResult), dp,
+                                   ctypes.POINTER(_COptStep), dp]
         _lib = L
     return _lib
 
@@ -508,10 +514,11 @@ def closed_form_gradient(molecule: Molecule, params: Optional[Pm3Parameters] = N
     res = _CResult()
     g = np.zeros((len(molecule.z), 3))
     P = np.zeros((nao, nao))
+    q = np.zeros(len(molecule.z))
     st = native().pm3_gradient(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(res), _dp(g),
-                               _dp(P))
+                               _dp(P), _dp(q))
     _raise(st, res)
-    scf = _result(res, P, None, None, np.zeros(len(molecule.z)))
+    scf = _result(res, P, None, None, q)
     return GradientResult(scf, res.total_ev, g, -g, float(np.abs(g).max()) if g.size else 0.0)
 
 
@@ -665,9 +672,19 @@ class OptOptions:
 
 
 @dataclasses.dataclass
+class OptStep:
+    """OptStep (src/optimizer.rs:37-43): one accepted point; positions in Bohr."""
+    energy_ev: float
+    heat_of_formation_kcal: float
+    max_gradient: float
+    positions: np.ndarray
+
+
+@dataclasses.dataclass
 class OptResult:
-    """OptResult (src/optimizer.rs:45-52) without the per-step trajectory: the final molecule,
-    the flags and counts, and the energies of the first and last steps."""
+    """OptResult (src/optimizer.rs:45-52): the final molecule, the flags and counts, the energies
+    of the first and last steps, and the trajectory (the starting point and every accepted point;
+    empty when it was not asked for)."""
     molecule: Molecule
     converged: bool
     iterations: int
@@ -676,18 +693,26 @@ class OptResult:
     heat_of_formation_kcal: float
     max_gradient: float
     first_energy_ev: float
+    trajectory: List[OptStep] = dataclasses.field(default_factory=list)
 
 
-def _opt_result(m, r, xyz):
+def _opt_result(m, r, xyz, steps=None, traj=None):
+    path = []
+    if steps is not None:
+        n = len(m.z)
+        path = [OptStep(steps[j].energy_ev, steps[j].heat_of_formation_kcal, steps[j].max_gradient,
+                        traj[3 * n * j:3 * n * (j + 1)].reshape(n, 3).copy()) for j in range(int(r.trajectory_len))]
     return OptResult(Molecule(m.z, xyz, m.charge, m.multiplicity), bool(r.converged), int(r.iterations),
-                     int(r.evaluations), r.energy_ev, r.heat_of_formation_kcal, r.max_gradient, r.first_energy_ev)
+                     int(r.evaluations), r.energy_ev, r.heat_of_formation_kcal, r.max_gradient, r.first_energy_ev,
+                     path)
 
 
 def optimize_batch(molecules: List[Molecule], params: Optional[Pm3Parameters] = None,
-                   options: Optional[Pm3Options] = None, opt: Optional[OptOptions] = None):
+                   options: Optional[Pm3Options] = None, opt: Optional[OptOptions] = None, trajectory: bool = True):
     """L-BFGS geometry optimisation of many molecules at once (pm3_optimize_batch): the optimiser
     of src/optimizer.rs:77-262 per molecule, one batched SCF + gradient solve per round.
-    Returns (status array, list of OptResult)."""
+    Returns (status array, list of OptResult).  trajectory=False skips the per-step record
+    (it holds max_iter + 1 geometries per molecule)."""
     params = params or Pm3Parameters.standard()
     options = options or Pm3Options()
     opt = opt or OptOptions()
@@ -701,12 +726,21 @@ def optimize_batch(molecules: List[Molecule], params: Optional[Pm3Parameters] = 
     mult = np.ascontiguousarray([int(m.multiplicity) for m in molecules], dtype=np.int32)
     out = np.zeros_like(xyz)
     res = (_COptResult * max(k, 1))()
+    cap = max(int(opt.max_iter), 0) + 1
+    steps = (_COptStep * max(k * cap, 1))() if trajectory else None
+    traj = np.zeros(3 * cap * int(ptr[-1])) if trajectory else None
     ip = ctypes.POINTER(ctypes.c_int32)
     _raise(native().pm3_optimize_batch(k, ptr.ctypes.data_as(ip), z.ctypes.data_as(ip), _dp(xyz), _dp(q),
                                        mult.ctypes.data_as(ip), params._h, ctypes.byref(options._c()),
-                                       ctypes.byref(opt._c()), res, _dp(out)))
+                                       ctypes.byref(opt._c()), res, _dp(out), steps,
+                                       None if traj is None else _dp(traj)))
     status = np.array([res[i].status for i in range(k)], dtype=np.int32)
-    return status, [_opt_result(m, res[i], out[ptr[i]:ptr[i + 1]].copy()) for i, m in enumerate(molecules)]
+    results = []
+    for i, m in enumerate(molecules):
+        st_i = (_COptStep * cap).from_buffer(steps, ctypes.sizeof(_COptStep) * cap * i) if trajectory else None
+        tr_i = traj[3 * cap * int(ptr[i]):3 * cap * int(ptr[i + 1])] if trajectory else None
+        results.append(_opt_result(m, res[i], out[ptr[i]:ptr[i + 1]].copy(), st_i, tr_i))
+    return status, results
 
 
 def optimize(molecule: Molecule, params: Optional[Pm3Parameters] = None, scf_options: Optional[Pm3Options] = None,
@@ -719,9 +753,12 @@ def optimize(molecule: Molecule, params: Optional[Pm3Parameters] = None, scf_opt
     cm = molecule._c()
     res = _COptResult()
     out = np.zeros((len(molecule.z), 3))
+    cap = max(int(opt.max_iter), 0) + 1
+    steps = (_COptStep * cap)()
+    traj = np.zeros(3 * cap * len(molecule.z))
     _raise(native().pm3_optimize(ctypes.byref(cm), params._h, ctypes.byref(scf_options._c()), ctypes.byref(opt._c()),
-                                 ctypes.byref(res), _dp(out)))
-    return _opt_result(molecule, res, out)
+                                 ctypes.byref(res), _dp(out), steps, _dp(traj)))
+    return _opt_result(molecule, res, out, steps, traj)
 
 
 def _rust_debug_f64(x):

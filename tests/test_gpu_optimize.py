@@ -53,6 +53,40 @@ def test_optimisations_follow_the_oracle(G, oracle):
         assert abs(r.first_energy_ev - o["trajectory"][0]) <= 1e-6 and r.energy_ev <= r.first_energy_ev
         # one solve per accepted point less than the reference's energy-then-gradient evaluations
         assert r.evaluations >= r.iterations
+        # the trajectory (OptResult::trajectory): the same accepted points, step for step
+        assert len(r.trajectory) == len(o["steps"]), name
+        # (on soft modes -- methanol's methyl rotation -- the two paths part by up to ~1e-5 Bohr
+        # midway and meet again at the minimum, so midway points get the looser bounds)
+        for a, b in zip(r.trajectory, o["steps"]):
+            assert abs(a.energy_ev - b["energy_ev"]) <= 1e-5
+            assert abs(a.heat_of_formation_kcal - b["hof_kcal"]) <= 2.5e-4
+            assert abs(a.max_gradient - b["max_gradient"]) <= 1e-4
+            assert np.abs(a.positions - b["xyz"]).max() <= 1e-4
+        # each step's scalars belong to its own positions: the oracle at the GPU's points
+        for a in (r.trajectory[0], r.trajectory[len(r.trajectory) // 2], r.trajectory[-1]):
+            st, g = oracle.gradient(z, a.positions, 0.0, oracle.options(**TIGHT))
+            assert st == 0 and abs(g["total_ev"] - a.energy_ev) <= 1e-7
+            assert abs(np.abs(g["gradient"]).max() - a.max_gradient) <= 1e-6
+        assert np.array_equal(r.trajectory[0].positions, x) and np.array_equal(r.trajectory[-1].positions,
+                                                                                 r.molecule.positions)
+        assert r.trajectory[-1].energy_ev == r.energy_ev and r.trajectory[0].energy_ev == r.first_energy_ev
+
+
+def test_open_shell_optimisation_follows_the_oracle(G, oracle):
+    """A doublet (ethyl radical, distorted) optimised on the unrestricted surface: the UHF
+    energy and gradient at every point, the trajectory step for step against the oracle."""
+    z = np.array([6, 6, 1, 1, 1, 1, 1], np.int32)
+    x = np.array([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0], [-0.4, 1.0, 0.1], [-0.4, -0.5, 0.9], [-0.35, -0.5, -0.85],
+                  [2.0, 0.95, 0.1], [2.05, -0.9, -0.1]]) * A
+    r = G.optimize(G.Molecule(z, x, multiplicity=2), scf_options=G.Pm3Options(**TIGHT))
+    o = oracle.optimize(z, x, 0.0, oracle.options(**TIGHT), multiplicity=2)
+    print("ethyl", r.iterations, o["iterations"], r.heat_of_formation_kcal - o["hof_kcal"], flush=True)
+    assert r.converged and o["converged"]
+    assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= 1e-4
+    n = min(len(r.trajectory), len(o["steps"]), 10)
+    for a, b in zip(r.trajectory[:n], o["steps"][:n]):
+        assert abs(a.energy_ev - b["energy_ev"]) <= 1e-5
+        assert np.abs(a.positions - b["xyz"]).max() <= 1e-4
 
 
 def test_flat_surface_reaches_the_oracles_basin(G, oracle):
@@ -93,6 +127,8 @@ def test_batch_gives_each_molecule_its_single_run(G):
         assert r.iterations == one.iterations and r.evaluations == one.evaluations
         assert abs(r.energy_ev - one.energy_ev) <= 1e-10
         assert np.abs(r.molecule.positions - one.molecule.positions).max() <= 1e-10
+        assert len(r.trajectory) == len(one.trajectory) >= 1
+        assert np.abs(r.trajectory[-1].positions - one.trajectory[-1].positions).max() <= 1e-10
     print("iterations", [r.iterations for r in rs], "converged", sum(r.converged for r in rs), flush=True)
     assert all(r.converged for r in rs[:5])  # the rigid molecules
     assert len({r.iterations for r in rs}) > 1

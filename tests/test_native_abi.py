@@ -155,7 +155,8 @@ def test_optimizer_defaults_and_host_checks_without_device():
 
     def call(n_mol, ptr, opts=so, opt=o):
         return L.pm3_optimize_batch(n_mol, ptr.ctypes.data_as(ip), z.ctypes.data_as(ip), xyz.ctypes.data_as(dp), None,
-                                    None, params, ctypes.byref(opts), ctypes.byref(opt), res, out.ctypes.data_as(dp))
+                                    None, params, ctypes.byref(opts), ctypes.byref(opt), res, out.ctypes.data_as(dp),
+                                    None, None)
 
     assert call(0, np.array([0], np.int32)) == 0  # nothing to do, no device work
     assert call(-1, np.array([0], np.int32)) == 1
@@ -169,6 +170,6 @@ def test_optimizer_defaults_and_host_checks_without_device():
     m = G.Molecule(z, np.eye(3), 0.0, 0)
     r = G._COptResult()
     assert L.pm3_optimize(ctypes.byref(m._c()), params, ctypes.byref(so), ctypes.byref(o), ctypes.byref(r),
-                          out.ctypes.data_as(dp)) == 1  # multiplicity below 1
+                          out.ctypes.data_as(dp), None, None) == 1  # multiplicity below 1
     assert L.pm3_batch_set_positions(None, xyz.ctypes.data_as(dp)) == 1
     assert L.pm3_batch_carry_densities(None, None, None) == 1

@@ -273,6 +273,38 @@ def test_optimizer_reproduces_the_reference_water_run(oracle):
     assert abs(r["hof_kcal"] - (-53.4330121104622)) < 1e-5
     assert r["trajectory"][-1] < r["trajectory"][0]
     assert all(b <= a for a, b in zip(r["trajectory"], r["trajectory"][1:]))  # Armijo: never uphill
+    # every OptStep holds the energy, heat and gradient of its own positions (a step that paired
+    # a point with its predecessor's scalars fails here); the first is the start, the last the end
+    steps = r["steps"]
+    assert len(steps) == len(r["trajectory"]) and [s["energy_ev"] for s in steps] == r["trajectory"]
+    assert np.array_equal(steps[0]["xyz"], x) and np.array_equal(steps[-1]["xyz"], r["xyz"])
+    for s in (steps[0], steps[len(steps) // 2], steps[-1]):
+        st, g = oracle.gradient(z, s["xyz"])
+        assert st == 0 and g["total_ev"] == s["energy_ev"] and g["hof_kcal"] == s["hof_kcal"]
+        assert np.abs(g["gradient"]).max() == s["max_gradient"]
+
+
+def test_optimizer_on_an_open_shell(oracle):
+    # a distorted ethyl radical on the UHF surface: the Armijo condition holds at every step and
+    # the end point is stationary by a central difference of the energy along each coordinate
+    A = oracle.ANGSTROM_TO_BOHR
+    z = np.array([6, 6, 1, 1, 1, 1, 1], np.int32)
+    x = np.array([[0.0, 0.0, 0.0], [1.5, 0.0, 0.0], [-0.4, 1.0, 0.1], [-0.4, -0.5, 0.9], [-0.35, -0.5, -0.85],
+                  [2.0, 0.95, 0.1], [2.05, -0.9, -0.1]]) * A
+    opts = oracle.options(p_tol=1e-10, e_tol=1e-12)
+    r = oracle.optimize(z, x, 0.0, opts, multiplicity=2)
+    assert r["converged"] and r["max_gradient"] < 1e-3
+    assert all(b <= a for a, b in zip(r["trajectory"], r["trajectory"][1:]))
+    h = 1e-4
+    xe = r["xyz"].copy()
+    for i in range(len(z)):
+        for c in range(3):
+            xp, xm = xe.copy(), xe.copy()
+            xp[i, c] += h
+            xm[i, c] -= h
+            ep = oracle.run_pm3(z, xp, 0.0, 2, opts)[1]["total_ev"]
+            em = oracle.run_pm3(z, xm, 0.0, 2, opts)[1]["total_ev"]
+            assert abs((ep - em) / (2 * h)) < 2e-3
 
 
 def test_optimizer_finds_the_minimum_of_a_bond_scan(oracle):
