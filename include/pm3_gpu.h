@@ -244,6 +244,10 @@ typedef struct {
   int32_t large_syrk;         /* multi-kernel path: purification products by dsyrk (1) or dgemm (0) */
   int32_t large_split;        /* multi-kernel path: q in 2..4 computes only the q x q block-lower part of
                                  each product (q batched dgemm rounds) when q divides the AO count */
+  int32_t wg128_nao;          /* SCF workgroup width: molecules up to this many AOs (and LDS needs within
+                                 36 KiB) on 128-thread workgroups, four per CU */
+  int32_t wg256_nao;          /* then up to this many AOs (within 76 KiB) on 256-thread workgroups, two per
+                                 CU; the rest on 512-thread workgroups, one per CU */
 } pm3_tuning;
 void pm3_tuning_default(pm3_tuning* t);
 pm3_status pm3_tuning_get(pm3_tuning* t);

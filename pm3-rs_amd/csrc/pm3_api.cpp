@@ -532,6 +532,8 @@ static void to_c(const Tuning& t, pm3_tuning* c) {
   c->sp2_tight_n = t.sp2_tight_n;
   c->large_syrk = t.large_syrk;
   c->large_split = t.large_split;
+  c->wg128_nao = t.wg128_nao;
+  c->wg256_nao = t.wg256_nao;
 }
 
 static Tuning from_c(const pm3_tuning* c) {
@@ -543,6 +545,8 @@ static Tuning from_c(const pm3_tuning* c) {
   t.sp2_tight_n = c->sp2_tight_n;
   t.large_syrk = c->large_syrk;
   t.large_split = c->large_split;
+  t.wg128_nao = c->wg128_nao;
+  t.wg256_nao = c->wg256_nao;
   return t;
 }
 
@@ -655,6 +659,8 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
       {&Tuning::sp2_tight_n, {24, 32, 48, 64, 96, 1 << 20}, 1},
       {&Tuning::large_syrk, {0, 1}, 2},
       {&Tuning::large_split, {0, 2, 3, 4}, 2},
+      {&Tuning::wg128_nao, {0, 32, 48, 64, 80, 96}, 1},
+      {&Tuning::wg256_nao, {0, 64, 80, 96, 112, 128}, 1},
   };
   Tuning best = start;
   double tbest = t0;
@@ -690,6 +696,8 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
         else if (ax.m == &Tuning::sp2_tight_n) name = "sp2_tight_n";
         else if (ax.m == &Tuning::large_syrk) name = "large_syrk";
         else if (ax.m == &Tuning::large_split) name = "large_split";
+        else if (ax.m == &Tuning::wg128_nao) name = "wg128_nao";
+        else if (ax.m == &Tuning::wg256_nao) name = "wg256_nao";
         std::fprintf(stderr, "pm3_autotune: %s %d: %.3f ms%s\n", name.c_str(), v, tc, same ? "" : " (answers differ)");
       }
       if (!same) {

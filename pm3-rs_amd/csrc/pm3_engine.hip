@@ -45,7 +45,13 @@ using namespace pm3dev;
 
 namespace {
 
-constexpr int NT = 512;       // threads per workgroup
+// Threads per workgroup of the SCF kernel.  The main build is 512 (one workgroup per CU); the
+// same device code is also compiled for 256 and 128 (pm3_scf_nt256.hip, pm3_scf_nt128.hip: two
+// and four workgroups per CU) for molecules small enough to share a CU.
+#ifndef PM3_NT
+#define PM3_NT 512
+#endif
+constexpr int NT = PM3_NT;
 constexpr int NW = NT / 64;   // waves per workgroup
 constexpr int MAXD = 8;       // maximum DIIS depth (src/scf.rs:309)
 constexpr int RED_K = 3 * MAXD + 3;  // most values one block_sum reduces (the DIIS history pass)
@@ -3111,6 +3117,7 @@ __device__ inline void zero_mats(const Mol& M, const Bufs& B) {
   }
 }
 
+#ifndef PM3_KERNEL_VARIANT  // kernels launched by the main build only
 // All atom pairs of all molecules, one work-item each (grid-stride).
 __global__ __launch_bounds__(256) void pair_kernel(KArgs A) {
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -3306,6 +3313,8 @@ __global__ __launch_bounds__(256) void grad_gather_wide_kernel(KArgs A, int firs
   }
 }
 
+#endif  // PM3_KERNEL_VARIANT
+
 // Mulliken charges (into out_q) and the dipole about the centre of mass, atomic units
 // (src/scf.rs:762-791, src/dipole.rs:64-140).  One workgroup; P has leading dimension M.np.
 __device__ inline void charges_dipole(const KArgs& A, const Mol& M, const double* P, double* red, double (&dip)[3]) {
@@ -3446,7 +3455,12 @@ __device__ __noinline__ void uhf_molecule(const KArgs& A, const Mol& M, const Mo
   __syncthreads();
 }
 
-__global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
+// Launch bounds of 512 threads in every build: the register budget is then 256 VGPRs per lane
+// (two waves per SIMD), so the 256- and 128-thread builds fit two and four workgroups per CU.
+#ifndef PM3_SCF_KERNEL
+#define PM3_SCF_KERNEL scf_batch_kernel
+#endif
+__global__ __launch_bounds__(512) void PM3_SCF_KERNEL(KArgs A) {
   double* lds = g_lds;
   __shared__ double red[NW * RED_K];
   __shared__ Diis D;
@@ -3569,6 +3583,7 @@ __global__ __launch_bounds__(NT) void scf_batch_kernel(KArgs A) {
   }
 }
 
+#ifndef PM3_KERNEL_VARIANT  // kernels launched by the main build only
 // Single stages of the pipeline on molecule 0 of a batch, for kernel-level parity checks:
 // mode 0 H_core (+ core-core energy at out[n*n]); mode 1 F(P) for P given in `in`;
 // mode 2 eigenpairs of `in` (cold Jacobi): out = eps[n] then C[n*n];
@@ -3989,6 +4004,8 @@ __global__ __launch_bounds__(NT) void lg_epilogue_kernel(KArgs A, MolDesc md, co
     for (int k = 0; k < 3; ++k) dip_out[k] = dip[k];
 }
 
+#endif  // PM3_KERNEL_VARIANT
+
 #define HIP_OK(x)                                                  \
   do {                                                             \
     hipError_t e_ = (x);                                           \
@@ -3999,6 +4016,40 @@ __global__ __launch_bounds__(NT) void lg_epilogue_kernel(KArgs A, MolDesc md, co
   } while (0)
 
 }  // namespace
+
+// Entry points of the SCF kernel's narrower builds (one translation unit each; their device code
+// objects, element tables and kernels are their own).  KArgs crosses as bytes: both sides compile
+// the same definition.
+#define PM3_VCAT2(a, b) a##_##b
+#define PM3_VCAT(a, b) PM3_VCAT2(a, b)
+namespace pm3variant {
+#ifdef PM3_KERNEL_VARIANT
+size_t PM3_VCAT(PM3_KERNEL_VARIANT, kargs_bytes)() { return sizeof(KArgs); }
+hipError_t PM3_VCAT(PM3_KERNEL_VARIANT, table)(const void* table, size_t bytes) {
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_tab), table, bytes);
+}
+hipError_t PM3_VCAT(PM3_KERNEL_VARIANT, prepare)(int lds_bytes) {
+  return hipFuncSetAttribute((const void*)PM3_SCF_KERNEL, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
+}
+hipError_t PM3_VCAT(PM3_KERNEL_VARIANT, launch)(const void* args, int nwg, int lds_bytes, hipStream_t s) {
+  KArgs a;
+  std::memcpy(&a, args, sizeof(KArgs));
+  hipLaunchKernelGGL(PM3_SCF_KERNEL, dim3(nwg), dim3(NT), lds_bytes, s, a);
+  return hipGetLastError();
+}
+#else
+size_t nt256_kargs_bytes();
+hipError_t nt256_table(const void* table, size_t bytes);
+hipError_t nt256_prepare(int lds_bytes);
+hipError_t nt256_launch(const void* args, int nwg, int lds_bytes, hipStream_t s);
+size_t nt128_kargs_bytes();
+hipError_t nt128_table(const void* table, size_t bytes);
+hipError_t nt128_prepare(int lds_bytes);
+hipError_t nt128_launch(const void* args, int nwg, int lds_bytes, hipStream_t s);
+#endif
+}  // namespace pm3variant
+
+#ifndef PM3_KERNEL_VARIANT
 
 namespace pm3host {
 
@@ -4041,8 +4092,24 @@ struct LargeBufs {
   long long lanczos_steps = 0, lanczos_runs = 0;  // frontier-level Lanczos of the last solve
 };
 
+// The SCF launch of one workgroup width: its molecules are order_all[off, off + n), pulled from its
+// own queue counter by nwg persistent workgroups of `lds` bytes of dynamic LDS, each with a slab of
+// `stride` doubles of its own scratch region.
+struct ScfClass {
+  int n = 0, off = 0, nwg = 0, lds = 0;
+  int np_max = 16;
+  long long w_cap = 1, stride = 0, ustride = 0, scr_off = 0;
+  bool any_uhf = false;
+};
+constexpr int SCF_WIDTH[3] = {512, 256, 128};  // threads per workgroup of each class
+constexpr int SCF_PER_CU[3] = {1, 2, 4};       // workgroups resident per CU (256 VGPRs per lane each)
+
 struct Batch {
   int nmol = 0, nsolve = 0, nwg = 0, lds_need = 0;
+  ScfClass cls[3];                  // 512-, 256- and 128-thread launches
+  double* scr_base = nullptr;       // the classes' scratch regions start here (offsets scr_off)
+  hipStream_t aux[2] = {nullptr, nullptr};  // the narrower launches run beside the widest one
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
   std::vector<MolDesc> desc;
   std::vector<int> solve;
   std::vector<int> large;      // molecules solved by the host-driven multi-kernel path
@@ -4054,6 +4121,7 @@ struct Batch {
   std::vector<double> h_red;
   std::vector<std::vector<int>> ao_map, atom_map;  // sorted -> caller indices
   std::vector<int> pre_status;
+  std::vector<int8_t> cls_of;          // SCF class (workgroup width) of each molecule
   std::vector<int> pre_z;              // the refused atomic number of a status-2 / 4 molecule
   std::vector<double> pre_req, pre_lim;  // pair-cache size and ceiling (MiB) of a status-5 molecule
   bool want_gradient = false;
@@ -4098,6 +4166,12 @@ struct Batch {
     if (blas) (void)rocblas_destroy_handle(blas);
     for (hipEvent_t e : {lg.evp0, lg.evp1, lg.evf0, lg.evf1})
       if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; ++i) {
+      if (aux[i]) (void)hipStreamSynchronize(aux[i]);
+      if (ev_join[i]) (void)hipEventDestroy(ev_join[i]);
+      if (aux[i]) (void)hipStreamDestroy(aux[i]);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
   bool ensure_guess_buffers(std::string& err) {
@@ -4144,6 +4218,9 @@ static bool upload_table(const Params& P, std::string& err) {
   std::lock_guard<std::mutex> lock(mu);
   if (dev < 0 || dev >= 64 || !uploaded[dev]) {
     HIP_OK(hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &P.table, sizeof(TableConst)));
+    // the narrower SCF builds hold their own copies of the table
+    HIP_OK(pm3variant::nt256_table(&P.table, sizeof(TableConst)));
+    HIP_OK(pm3variant::nt128_table(&P.table, sizeof(TableConst)));
     if (dev >= 0 && dev < 64) uploaded[dev] = true;
   }
   return true;
@@ -4972,6 +5049,8 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
   b.pre_z.assign(nmol, 0);
   b.pre_req.assign(nmol, 0.0);
   b.pre_lim.assign(nmol, 0.0);
+  b.cls_of.assign(nmol, 0);
+  const Tuning tune = tuning_table();
   // the pair-cache ceiling of build_core_limited (src/hamiltonian.rs:44-49, 202-211)
   long long cache_limit_mb = opt.integral_memory_mb;
   if (cache_limit_mb <= 0) {
@@ -5093,6 +5172,19 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
       np_max = std::max(np_max, d.np);
       const long long npairs = (long long)d.nat * (d.nat - 1) / 2;
       w_cap = std::max(w_cap, 20LL * npairs);  // Fock-build pair slots per workgroup
+      // workgroup width: the narrowest whose share of a CU's LDS holds what the molecule cannot
+      // do without (the Jacobi fallback's packed matrix, the Lanczos vectors); every other LDS
+      // placement adapts to the launch's budget on the device
+      const long long lmin =
+          8LL * std::max({jac_lds_base(nao), b.purify ? lanczos_lds(nao) : 0, m.n_atoms * 16});
+      int c = 0;
+      if (nao <= tune.wg128_nao && lmin + STATIC_LDS_BYTES <= LDS_BYTES_MAX / SCF_PER_CU[2]) c = 2;
+      else if (nao <= tune.wg256_nao && lmin + STATIC_LDS_BYTES <= LDS_BYTES_MAX / SCF_PER_CU[1]) c = 1;
+      b.cls_of[k] = (int8_t)c;
+      ScfClass& sc = b.cls[c];
+      sc.any_uhf = sc.any_uhf || use_uhf;
+      sc.np_max = std::max(sc.np_max, d.np);
+      sc.w_cap = std::max(sc.w_cap, 20LL * npairs);
     }
     if (req.want_density || req.want_orbitals || req.want_gradient || req.warm_start) {
       d.out_mat = b.mat_total;
@@ -5153,8 +5245,17 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
     delete bp;
     return false;
   }
-  // largest molecules first so the queue ends on short work
-  std::stable_sort(b.solve.begin(), b.solve.end(), [&](int x, int y) { return b.desc[x].nao > b.desc[y].nao; });
+  // by class (widest workgroups first), and inside a class largest molecules first so that its
+  // queue ends on short work
+  std::stable_sort(b.solve.begin(), b.solve.end(), [&](int x, int y) {
+    return b.cls_of[x] != b.cls_of[y] ? b.cls_of[x] < b.cls_of[y] : b.desc[x].nao > b.desc[y].nao;
+  });
+  for (int c = 0, off = 0; c < 3; ++c) {
+    b.cls[c].off = off;
+    b.cls[c].n = 0;
+    for (int k : b.solve) b.cls[c].n += b.cls_of[k] == c;
+    off += b.cls[c].n;
+  }
   b.order_all = b.solve;
   b.order_all.insert(b.order_all.end(), b.large.begin(), b.large.end());
   const int nq = (int)b.order_all.size();
@@ -5180,17 +5281,38 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
   HIP_OK(hipGetDevice(&dev));
   HIP_OK(hipGetDeviceProperties(&prop, dev));
   b.nwg = std::max(1, std::min(std::max(b.nsolve, 1), prop.multiProcessorCount));
-  const long long m2 = (long long)np_max * np_max;
-  const long long ustride = b.any_uhf ? ((9 + 2 * MAXD) * m2 + 4LL * np_max + 31) & ~31LL : 0;
-  const long long stride = (((11 + 3 * MAXD) * m2 + w_cap + 6LL * np_max + 64 + 31) & ~31LL) + ustride;
+  // each class's workgroups get slabs sized for that class's largest molecule
+  long long scr_total = 0;
+  for (int c = 0; c < 3; ++c) {
+    ScfClass& sc = b.cls[c];
+    const long long cm2 = (long long)sc.np_max * sc.np_max;
+    sc.ustride = sc.any_uhf ? ((9 + 2 * MAXD) * cm2 + 4LL * sc.np_max + 31) & ~31LL : 0;
+    sc.stride = (((11 + 3 * MAXD) * cm2 + sc.w_cap + 6LL * sc.np_max + 64 + 31) & ~31LL) + sc.ustride;
+    sc.nwg = sc.n > 0 ? std::min(sc.n, SCF_PER_CU[c] * prop.multiProcessorCount) : 0;
+    sc.lds = LDS_BYTES_MAX / SCF_PER_CU[c] - STATIC_LDS_BYTES;
+    sc.scr_off = scr_total;
+    scr_total += sc.stride * sc.nwg;
+  }
+  // the stage kernel and the molecule at the head of the queue use the slab layout of its class
+  const int c0 = b.nsolve > 0 ? b.cls_of[b.solve[0]] : 0;
+  if (b.nsolve == 0) {
+    const long long cm2 = (long long)np_max * np_max;
+    b.cls[0].ustride = b.any_uhf ? ((9 + 2 * MAXD) * cm2 + 4LL * np_max + 31) & ~31LL : 0;
+    b.cls[0].stride = (((11 + 3 * MAXD) * cm2 + w_cap + 6LL * np_max + 64 + 31) & ~31LL) + b.cls[0].ustride;
+    b.cls[0].np_max = np_max;
+    b.cls[0].w_cap = w_cap;
+    scr_total = b.cls[0].stride;
+  }
+  const long long ustride = b.cls[c0].ustride;
+  const long long stride = b.cls[c0].stride;
   KArgs& a = b.args;
   MolDesc* d_desc; int *d_order, *d_queue, *d_aslot, *d_aoff;
   double *d_ax, *d_ay, *d_az, *d_scr, *d_P, *d_C, *d_eps, *d_q, *d_store, *d_g;
   long long* d_pbase;
   MolOut* d_out;
-  if (!b.alloc(d_desc, nmol, err) || !b.alloc(d_order, nq, err) || !b.alloc(d_queue, 1, err) ||
+  if (!b.alloc(d_desc, nmol, err) || !b.alloc(d_order, nq, err) || !b.alloc(d_queue, 3, err) ||
       !b.alloc(d_aslot, b.atoms_total, err) || !b.alloc(d_aoff, b.atoms_total, err) || !b.alloc(d_ax, b.atoms_total, err) ||
-      !b.alloc(d_ay, b.atoms_total, err) || !b.alloc(d_az, b.atoms_total, err) || !b.alloc(d_scr, (size_t)(stride * b.nwg), err) ||
+      !b.alloc(d_ay, b.atoms_total, err) || !b.alloc(d_az, b.atoms_total, err) || !b.alloc(d_scr, (size_t)scr_total, err) ||
       !b.alloc(d_out, nmol, err) || !b.alloc(d_P, b.mat_total, err) || !b.alloc(d_C, b.mat_total, err) ||
       !b.alloc(d_eps, b.vec_total, err) || !b.alloc(d_q, b.atoms_total, err) ||
       !b.alloc(d_store, (size_t)store_total, err) || !b.alloc(d_pbase, pair_base.size(), err) ||
@@ -5234,10 +5356,11 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
   a.ay = d_ay;
   a.az = d_az;
   a.aoff = d_aoff;
-  a.scratch = d_scr;
+  b.scr_base = d_scr;
+  a.scratch = d_scr + b.cls[c0].scr_off;
   a.stride = stride;
-  a.np_max = np_max;
-  a.w_cap = w_cap;
+  a.np_max = b.cls[c0].np_max;
+  a.w_cap = b.cls[c0].w_cap;
   a.out = d_out;
   a.out_P = d_P;
   a.out_C = d_C;
@@ -5280,7 +5403,6 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
       a.prof = d_prof;
     }
   }
-  const Tuning tune = tuning_table();
   b.pair_block = tune.pair_block;
   b.pair_blocks = (int)std::max<long long>(
       1, std::min<long long>((a.npairs_total + b.pair_block - 1) / b.pair_block,
@@ -5315,6 +5437,12 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
   if (const char* ev = std::getenv("PM3_SP2_LDS")) a.sp2_lds = std::atoi(ev);
   if (const char* ev = std::getenv("PM3_EIG_SKIP")) a.eig_skip = std::atof(ev);
   HIP_OK(hipFuncSetAttribute((const void*)scf_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_need));
+  if (pm3variant::nt256_kargs_bytes() != sizeof(KArgs) || pm3variant::nt128_kargs_bytes() != sizeof(KArgs)) {
+    err = "the SCF kernel builds disagree on their argument layout";
+    return false;
+  }
+  if (b.cls[1].n > 0) HIP_OK(pm3variant::nt256_prepare(b.cls[1].lds));
+  if (b.cls[2].n > 0) HIP_OK(pm3variant::nt128_prepare(b.cls[2].lds));
   if (!b.large.empty() && b.arena) {
     err = "the allocation-free entry point serves molecules of the single-workgroup path only";
     return false;
@@ -5336,6 +5464,58 @@ static BatchGuessView infer_view(Batch& b) {
   return v;
 }
 
+// The persistent SCF launches, one per workgroup width in use: the widest on the batch's stream,
+// the narrower ones on streams of their own that start after the same point of the batch's stream
+// and that the batch's stream then waits for, so that they fill the CUs the widest launch's tail
+// frees.  Without events (the allocation-free mode) they run one after another on the batch's
+// stream.
+static bool launch_scf(Batch& b, std::string& err) {
+  int used = 0;
+  for (const ScfClass& sc : b.cls) used += sc.n > 0;
+  const bool fork = used > 1 && b.arena == nullptr;
+  if (fork && !b.ev_fork) {
+    HIP_OK(hipEventCreateWithFlags(&b.ev_fork, hipEventDisableTiming));
+    for (int i = 0; i < 2; ++i) {
+      HIP_OK(hipStreamCreateWithFlags(&b.aux[i], hipStreamNonBlocking));
+      HIP_OK(hipEventCreateWithFlags(&b.ev_join[i], hipEventDisableTiming));
+    }
+  }
+  if (fork) HIP_OK(hipEventRecord(b.ev_fork, b.stream));
+  int naux = 0;
+  bool first = true;
+  for (int c = 0; c < 3; ++c) {
+    const ScfClass& sc = b.cls[c];
+    if (sc.n == 0) continue;
+    KArgs a = b.args;
+    a.order = b.args.order + sc.off;
+    a.nmol = sc.n;
+    a.queue = b.args.queue + c;
+    a.scratch = b.scr_base + sc.scr_off;
+    a.stride = sc.stride;
+    a.ustride = sc.ustride;
+    a.np_max = sc.np_max;
+    a.w_cap = sc.w_cap;
+    a.lds_doubles = sc.lds / 8;
+    hipStream_t s = b.stream;
+    if (fork && !first) {
+      s = b.aux[naux];
+      HIP_OK(hipStreamWaitEvent(s, b.ev_fork, 0));
+    }
+    if (c == 0) {
+      hipLaunchKernelGGL(scf_batch_kernel, dim3(sc.nwg), dim3(SCF_WIDTH[0]), sc.lds, s, a);
+      HIP_OK(hipGetLastError());
+    } else if (c == 1) {
+      HIP_OK(pm3variant::nt256_launch(&a, sc.nwg, sc.lds, s));
+    } else {
+      HIP_OK(pm3variant::nt128_launch(&a, sc.nwg, sc.lds, s));
+    }
+    if (s != b.stream) HIP_OK(hipEventRecord(b.ev_join[naux++], s));
+    first = false;
+  }
+  for (int i = 0; i < naux; ++i) HIP_OK(hipStreamWaitEvent(b.stream, b.ev_join[i], 0));
+  return true;
+}
+
 bool batch_solve(Batch* bp, std::string& err) {
   Batch& b = *bp;
   b.last_kernel_ms = 0.0;
@@ -5352,7 +5532,7 @@ bool batch_solve(Batch* bp, std::string& err) {
     HIP_OK(hipEventCreate(&b.evm));
   }
   auto rec = [&](hipEvent_t e) { return timed ? hipEventRecord(e, b.stream) : hipSuccess; };
-  HIP_OK(hipMemsetAsync(b.args.queue, 0, sizeof(int), b.stream));
+  HIP_OK(hipMemsetAsync(b.args.queue, 0, 3 * sizeof(int), b.stream));
   if (b.args.prof) HIP_OK(hipMemsetAsync(b.args.prof, 0, PH_SLOTS * sizeof(unsigned long long), b.stream));
   HIP_OK(rec(b.ev0));
   hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, b.stream, b.args);
@@ -5362,8 +5542,7 @@ bool batch_solve(Batch* bp, std::string& err) {
   if (!b.learned) HIP_OK(rec(b.evm));
   HIP_OK(rec(b.ev1));
   b.args.warm = b.warm_start && (b.solved_once || b.have_prev) ? 1 : 0;
-  if (b.nsolve > 0) hipLaunchKernelGGL(scf_batch_kernel, dim3(b.nwg), dim3(NT), b.lds_need, b.stream, b.args);
-  HIP_OK(hipGetLastError());
+  if (b.nsolve > 0 && !launch_scf(b, err)) return false;
   for (int k : b.large)
     if (!large_solve(b, k, err)) return false;
   if (b.want_gradient) {
@@ -5795,3 +5974,4 @@ bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOpt
 }
 
 }  // namespace pm3host
+#endif  // PM3_KERNEL_VARIANT

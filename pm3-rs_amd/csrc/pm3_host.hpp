@@ -90,6 +90,10 @@ struct Tuning {
   int large_split = 2;          // host-driven path: q >= 2 computes only the q x q block-lower part of each
                                 // product (q batched dgemm rounds) when q divides n; 0 one dgemm
                                 // (2 measured fastest at 3072 AOs: 0.70 ms per product against 0.87)
+  int wg128_nao = 64;           // molecules up to this size (AOs) whose LDS needs fit 36 KiB run on
+                                // 128-thread workgroups, four per CU
+  int wg256_nao = 64;           // then those up to this size fitting 76 KiB on 256-thread workgroups, two
+                                // per CU; the rest on 512-thread workgroups, one per CU
 };
 // The table batch_create reads; on first use it is loaded from the file PM3_TUNING_FILE names,
 // if set.  Set or loaded tables apply to batches created afterwards (not thread-safe against a

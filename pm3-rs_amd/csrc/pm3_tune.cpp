@@ -28,6 +28,8 @@ const Field kFields[] = {
     {"sp2_tight_n", &Tuning::sp2_tight_n, 0, 1 << 20},
     {"large_syrk", &Tuning::large_syrk, 0, 1},
     {"large_split", &Tuning::large_split, 0, 4},
+    {"wg128_nao", &Tuning::wg128_nao, 0, 1 << 20},
+    {"wg256_nao", &Tuning::wg256_nao, 0, 1 << 20},
 };
 
 Tuning initial_table() {
