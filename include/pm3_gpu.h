@@ -117,6 +117,9 @@ void pm3_options_default(pm3_options* out);
 
 /* Number of AOs of the molecule (Basis::build, src/basis.rs:39-62); -1 for an unknown element. */
 int32_t pm3_query_nao(const pm3_molecule* mol, const pm3_params* params);
+/* Atomic mass (amu) of element z in the parameter set (Pm3Element::mass, src/params.rs); NaN for
+ * an element the set lacks. */
+double pm3_element_mass(const pm3_params* params, int32_t z);
 
 /* run_pm3 (src/scf.rs:450-848).  Optional outputs may be NULL: density[n*n] (total), mo_energies[n],
  * mo_coeff[n*n] (alpha for an unrestricted result), charges[n_atoms], and for an unrestricted result
@@ -144,6 +147,24 @@ pm3_status pm3_run_noalloc(const pm3_molecule* mol, const pm3_params* params, co
  * of GradientResult::scf, charges[n_atoms], may be NULL. */
 pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
                         pm3_result* result, double* gradient, double* density, double* charges);
+
+/* response_fock_rhf (src/fock.rs:444-485), the CPHF response operator of the analytic Hessian
+ * (src/hessian.rs:1025-1027), for n_rhs densities at once: out[r] = J(dP_r) - K(dP_r / 2) with no
+ * H_core, for symmetric dp[r] (n_rhs x nao x nao, row-major, the molecule's AO order).  pair_sw
+ * (may be NULL: every pair 1) weights each atom pair's two-centre terms, one finite value per pair
+ * in the reference's order ((u, v), u < v, row by row over the atoms); a zero drops the pair.
+ * The molecule must be one run_pm3 accepts (charge and multiplicity); its electrons are not
+ * otherwise used.  out: n_rhs x nao x nao. */
+pm3_status pm3_response_fock(const pm3_molecule* mol, const pm3_params* params, int32_t n_rhs, const double* dp,
+                             const double* pair_sw, double* out);
+
+/* numerical_hessian (src/hessian.rs:268-322): the 3N x 3N Cartesian Hessian (eV/Bohr^2, row-major)
+ * by central differences of the analytic gradient with displacement `step` (Bohr), symmetrised by
+ * averaging, under the options tightened as tighten_scf_for_hessian does (:169-175: e_tol <= 1e-12,
+ * p_tol <= 1e-11, max_scf >= 500).  All 6N displaced gradients are one batched solve.  Returns the
+ * first displaced solve's failure, if any. */
+pm3_status pm3_numerical_hessian(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                                 double step, double* hessian);
 
 /* Batched solve of many independent molecules in one persistent launch.  Atoms of molecule k are
  * [atom_ptr[k], atom_ptr[k+1]) of z / xyz_bohr; charge may be NULL (all neutral), multiplicity may

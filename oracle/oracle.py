@@ -98,6 +98,7 @@ def lib():
         L.orc_hcore.argtypes = [ctypes.c_int, ip, dp, dp]
         L.orc_fock.argtypes = [ctypes.c_int, ip, dp, dp, dp]
         L.orc_fock_fourindex.argtypes = [ctypes.c_int, ip, dp, dp, dp]
+        L.orc_response_fock.argtypes = [ctypes.c_int, ip, dp, ctypes.c_int, dp, dp, dp]
         L.orc_core_core.argtypes = [ctypes.c_int, ip, dp]
         L.orc_core_core.restype = ctypes.c_double
         L.orc_eigen.argtypes = [ctypes.c_int, dp, dp, dp]
@@ -193,6 +194,22 @@ def fock(z, xyz_bohr, P, four_index=False):
     if st:
         raise RuntimeError(STATUS[st])
     return F
+
+
+def response_fock(z, xyz_bohr, dp, pair_sw=None):
+    """response_fock_rhf (src/fock.rs:444-485) for one density (n, n) or a stack (k, n, n); pair_sw
+    one weight per atom pair in the reference's order (u < v, row by row), or None."""
+    z, xyz = _mol(z, xyz_bohr)
+    dp = np.ascontiguousarray(dp, dtype=np.float64)
+    one = dp.ndim == 2
+    stack = dp.reshape(1, *dp.shape) if one else dp
+    out = np.zeros_like(stack)
+    sw = None if pair_sw is None else np.ascontiguousarray(pair_sw, dtype=np.float64)
+    st = lib().orc_response_fock(len(z), _i(z), _d(xyz), stack.shape[0], _d(stack),
+                                 _d(sw) if sw is not None else ctypes.POINTER(ctypes.c_double)(), _d(out))
+    if st:
+        raise RuntimeError(STATUS[st])
+    return out[0] if one else out
 
 
 def core_core(z, xyz_bohr):
@@ -335,6 +352,31 @@ def run_batch(mols, opts=None, nthreads=0):
     lib().orc_run_batch(k, _i(ptr), _i(zz), _d(xyz), nullp, ctypes.byref(opts), int(nthreads), _d(tot), _d(hof),
                         _i(it), _i(st))
     return tot, hof, it, st
+
+
+def numerical_hessian(z, xyz_bohr, charge=0.0, opts=None, step=1.0e-3, multiplicity=1):
+    """numerical_hessian (src/hessian.rs:268-322): column j = (g(x + h e_j) - g(x - h e_j)) / 2h of the
+    analytic gradient under tighten_scf_for_hessian (:169-175: e_tol <= 1e-12, p_tol <= 1e-11,
+    max_scf >= 500), then H = (H + H^T) / 2 (eV/Bohr^2)."""
+    z = np.asarray(z, dtype=np.int32)
+    x0 = np.array(xyz_bohr, dtype=np.float64).reshape(-1)
+    o = opts or options()
+    o = options(max_scf=max(o.max_scf, 500), e_tol=min(o.e_tol, 1e-12), p_tol=min(o.p_tol, 1e-11),
+                use_diis=bool(o.use_diis), accelerator=o.accelerator, adiis_switch=o.adiis_switch, guess=o.guess,
+                stability=o.stability, scf_memory_mb=o.scf_memory_mb)
+    n = x0.size
+    h = np.zeros((n, n))
+    for j in range(n):
+        cols = []
+        for sgn in (1.0, -1.0):
+            x = x0.copy()
+            x[j] += sgn * step
+            st, g = gradient(z, x.reshape(-1, 3), charge, o, multiplicity)
+            if st != 0:
+                raise RuntimeError(STATUS[st])
+            cols.append(g["gradient"].reshape(-1))
+        h[:, j] = (cols[0] - cols[1]) / (2.0 * step)
+    return 0.5 * (h + h.T)
 
 
 def gradient_batch(mols, opts=None, nthreads=0):

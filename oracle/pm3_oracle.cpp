@@ -1655,6 +1655,84 @@ int orc_fock_fourindex(int nat, const int* z, const double* xyz, const double* P
   return OK;
 }
 
+// response_fock_rhf (src/fock.rs:444-485): G(dP) = J(dP) - K(dP / 2), no H_core; the one-centre
+// blocks as in the Fock build, the two-centre terms of pair_contribution (:106-159) with each pair's
+// switch sw (pair_sw[k] in core.pairs order, src/hamiltonian.rs:237-240; null: 1) on J_a, on d_a in
+// J_b and on the exchange scale, a zero switch skipping the pair, then scatter_pair (:163-193).
+int orc_response_fock(int nat, const int* z, const double* xyz, int nrhs, const double* dp, const double* pair_sw,
+                      double* out) {
+  Mol m;
+  int st = make_mol(nat, z, xyz, m);
+  if (st) return st;
+  Core c;
+  if ((st = build_core(m, c))) return st;
+  const int n = m.nao;
+  const size_t nn = (size_t)n * n;
+  for (int r = 0; r < nrhs; ++r) {
+    const double* P = dp + r * nn;
+    double* F = out + r * nn;
+    std::fill(F, F + nn, 0.0);
+    for (int ia = 0; ia < m.nat; ++ia) {
+      const Elem& e = *m.el[ia];
+      const int o = m.off[ia], nb = m.norb[ia];
+      for (int mu = 0; mu < nb; ++mu)
+        for (int nu = 0; nu < nb; ++nu) {
+          double acc = 0.0;
+          for (int la = 0; la < nb; ++la)
+            for (int si = 0; si < nb; ++si) {
+              acc += P[(size_t)(o + la) * n + o + si] * oc_two_electron(mu, nu, la, si, e);
+              acc -= 0.5 * P[(size_t)(o + la) * n + o + si] * oc_two_electron(mu, la, nu, si, e);
+            }
+          F[(size_t)(o + mu) * n + o + nu] += acc;
+        }
+    }
+    for (size_t k = 0; k < c.pairs.size(); ++k) {
+      const Pair& p = c.pairs[k];
+      const double sw = pair_sw ? pair_sw[k] : 1.0;
+      const auto& t = p.t;
+      const int oa = m.off[p.a], ob = m.off[p.b], na = t.na, nb = t.nb, pa = t.pa, pb = t.pb;
+      double ja[10] = {0}, jb[10] = {0}, kab[4][4] = {{0}};
+      if (sw != 0.0) {
+        double da[10], db[10];
+        for (int mu = 0; mu < na; ++mu)
+          for (int nu = 0; nu <= mu; ++nu)
+            da[pack(mu, nu)] = (mu == nu ? 1.0 : 2.0) * P[(size_t)(oa + mu) * n + oa + nu];
+        for (int mu = 0; mu < nb; ++mu)
+          for (int nu = 0; nu <= mu; ++nu)
+            db[pack(mu, nu)] = (mu == nu ? 1.0 : 2.0) * P[(size_t)(ob + mu) * n + ob + nu];
+        for (int q = 0; q < pa; ++q) {
+          double acc = 0.0;
+          for (int s2 = 0; s2 < pb; ++s2) acc += db[s2] * t.w[q * pb + s2];
+          ja[q] = sw * acc;
+          const double d = sw * da[q];
+          for (int s2 = 0; s2 < pb; ++s2) jb[s2] += d * t.w[q * pb + s2];
+        }
+        const double scale = sw * 0.5;
+        for (int mu = 0; mu < na; ++mu)
+          for (int nu = 0; nu < na; ++nu) {
+            const double* row = &t.w[pack(mu, nu) * pb];
+            for (int la = 0; la < nb; ++la) {
+              double acc = 0.0;
+              for (int si = 0; si < nb; ++si) acc += P[(size_t)(oa + nu) * n + ob + si] * row[pack(la, si)];
+              kab[mu][la] -= scale * acc;
+            }
+          }
+      }
+      for (int mu = 0; mu < na; ++mu)
+        for (int nu = 0; nu < na; ++nu) F[(size_t)(oa + mu) * n + oa + nu] += ja[pack(mu, nu)];
+      for (int la = 0; la < nb; ++la)
+        for (int si = 0; si < nb; ++si) F[(size_t)(ob + la) * n + ob + si] += jb[pack(la, si)];
+      for (int mu = 0; mu < na; ++mu)
+        for (int la = 0; la < nb; ++la) {
+          const double v = F[(size_t)(oa + mu) * n + ob + la] + kab[mu][la];
+          F[(size_t)(oa + mu) * n + ob + la] = v;
+          F[(size_t)(ob + la) * n + oa + mu] = v;
+        }
+    }
+  }
+  return OK;
+}
+
 double orc_core_core(int nat, const int* z, const double* xyz) {
   Mol m;
   if (make_mol(nat, z, xyz, m)) return NAN;

@@ -222,6 +222,11 @@ int32_t pm3_query_nao(const pm3_molecule* mol, const pm3_params* params) {
   return n;
 }
 
+double pm3_element_mass(const pm3_params* params, int32_t z) {
+  if (!params || z < 0 || z >= 128 || params->p.slot_of_z[z] < 0) return std::nan("");
+  return params->p.table.el[params->p.slot_of_z[z]].mass;
+}
+
 pm3_status pm3_run(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts, pm3_result* result,
                    double* density, double* mo_energies, double* mo_coeff, double* charges, double* spin_density,
                    double* mo_energies_beta, double* mo_coeff_beta) {
@@ -519,6 +524,24 @@ pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const
     if (charges) std::memcpy(charges, r.charges.data(), (size_t)mol->n_atoms * sizeof(double));
   }
   return molecule_status(r.status, r);
+}
+
+pm3_status pm3_response_fock(const pm3_molecule* mol, const pm3_params* params, int32_t n_rhs, const double* dp,
+                             const double* pair_sw, double* out) {
+  g_last_error.clear();
+  if (!params || n_rhs < 0 || (n_rhs > 0 && (!dp || !out))) return fail(PM3_INVALID_INPUT, "null argument");
+  pm3_status s = check_molecule(mol);
+  if (s != PM3_OK) return s;
+  if (n_rhs == 0) return PM3_OK;
+  if (pair_sw) {
+    const long long np = (long long)mol->n_atoms * (mol->n_atoms - 1) / 2;
+    for (long long p = 0; p < np; ++p)
+      if (!std::isfinite(pair_sw[p])) return fail(PM3_INVALID_INPUT, "pair_sw must be finite");
+  }
+  std::string err;
+  const int st = response_fock_run(params->p, MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge, mol->multiplicity},
+                                   n_rhs, dp, pair_sw, out, err);
+  return st == 0 ? PM3_OK : fail((pm3_status)st, err);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -711,8 +711,12 @@ __device__ double finish_core(const KArgs& A, const Mol& M, double* red) {
 // launch per phase) runs the same code.
 // Phase 1 of the Fock build: off-diagonal atom blocks (exchange K, owned by their pair) and the
 // per-pair Coulomb partial sums in M.slot.  Work-items t0, t0 + st, ... (a workgroup or a grid).
+// SW: every two-centre term of a pair is weighted by its switch (the CPHF distance cutoff of
+// response_fock_rhf, src/fock.rs:106-159 with sw): sw[kx] for heavy-heavy pairs, sw[nxx + kx]
+// heavy-hydrogen, sw[nxx + nxh + kx] hydrogen-hydrogen, applied where the reference applies it.
+template <bool SW = false>
 __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H, const double* P, const double* X,
-                                  double xf, double* F, int t0, int st) {
+                                  double xf, double* F, int t0, int st, const double* sw = nullptr) {
   const int np = M.np;
   double* JX = M.slot;                          // [20][nxx]: J_a (0..9), J_b (10..19) per XX pair
   double* JH = M.slot + 20LL * M.nxx;           // [11][nxh]: J_a (0..9), J_b (10) per XH pair
@@ -740,6 +744,8 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
       for (int j = 0; j < 4; ++j) K[i][j] = 0.0;
     const double* W = M.Wxx + kx;
     const long long ld = M.nxx;
+    const double swp = SW ? sw[kx] : 1.0;
+    const double xfp = SW ? swp * xf : xf;
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu)
 #pragma unroll
@@ -749,12 +755,13 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
 #pragma unroll
         for (int q = 0; q < 10; ++q) w[q] = W[(long long)(p * 10 + q) * ld];
         double acc = 0.0;
+        const double dap = SW ? swp * da[p] : da[p];
 #pragma unroll
         for (int q = 0; q < 10; ++q) {
           acc += db[q] * w[q];
-          jb[q] += da[p] * w[q];
+          jb[q] += dap * w[q];
         }
-        ja[p] = acc;
+        ja[p] = SW ? swp * acc : acc;
         // exchange: K[mu][la] -= xf sum_si X(nu, si) (mu nu|la si), and the (nu, mu) term
 #pragma unroll
         for (int la = 0; la < 4; ++la) {
@@ -764,8 +771,8 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
             s1 += pab[nu][si] * w[pk(la, si)];
             s2 += pab[mu][si] * w[pk(la, si)];
           }
-          K[mu][la] -= xf * s1;
-          if (mu != nu) K[nu][la] -= xf * s2;
+          K[mu][la] -= xfp * s1;
+          if (mu != nu) K[nu][la] -= xfp * s2;
         }
       }
 #pragma unroll
@@ -787,6 +794,7 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
     const int a = kx / M.nH, b = M.nh + kx % M.nH;
     const int oa = A.aoff[M.atom0 + a], ob = A.aoff[M.atom0 + b];
     const double pbb = P[ob * np + ob];
+    const double swp = SW ? sw[M.nxx + kx] : 1.0;
     double jbv = 0.0, pa[4];
 #pragma unroll
     for (int nu = 0; nu < 4; ++nu) pa[nu] = X[(oa + nu) * np + ob];
@@ -798,19 +806,20 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
 #pragma unroll
       for (int nu = 0; nu <= mu; ++nu) {
         const double f = mu == nu ? 1.0 : 2.0;
-        jbv += f * P[(oa + mu) * np + oa + nu] * w[pk(mu, nu)];
+        if (SW) jbv += swp * (f * P[(oa + mu) * np + oa + nu]) * w[pk(mu, nu)];
+        else jbv += f * P[(oa + mu) * np + oa + nu] * w[pk(mu, nu)];
       }
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu) {
       double k = 0.0;
 #pragma unroll
       for (int nu = 0; nu < 4; ++nu) k += pa[nu] * w[pk(mu, nu)];
-      const double v = H[(oa + mu) * np + ob] - xf * k;
+      const double v = H[(oa + mu) * np + ob] - (SW ? swp * xf : xf) * k;
       F[(oa + mu) * np + ob] = v;
       F[ob * np + oa + mu] = v;
     }
 #pragma unroll
-    for (int q = 0; q < 10; ++q) JH[(long long)q * M.nxh + kx] = w[q] * pbb;
+    for (int q = 0; q < 10; ++q) JH[(long long)q * M.nxh + kx] = SW ? swp * (w[q] * pbb) : w[q] * pbb;
     JH[10LL * M.nxh + kx] = jbv;
   }
   // hydrogen-hydrogen pairs
@@ -821,7 +830,7 @@ __device__ inline void fock_pairs(const KArgs& A, const Mol& M, const double* H,
     b += M.nh;
     const int oa = A.aoff[M.atom0 + a], ob = A.aoff[M.atom0 + b];
     const double w = M.Whh[kx];
-    const double v = H[oa * np + ob] - xf * X[oa * np + ob] * w;
+    const double v = H[oa * np + ob] - (SW ? sw[M.nxx + M.nxh + kx] * xf : xf) * X[oa * np + ob] * w;
     F[oa * np + ob] = v;
     F[ob * np + oa] = v;
   }
@@ -3912,15 +3921,17 @@ __global__ __launch_bounds__(LG_THREADS) void lg_fock_pairs_kernel(KArgs A, MolD
 // The sums of fock_jsum with one wave per output (heavy atom and component, or hydrogen): lanes
 // over the partner atoms, then a fixed shuffle tree, so the whole grid works on a single molecule's
 // few thousand outputs and the sums stay reproducible.
-__global__ __launch_bounds__(LG_THREADS) void lg_fock_jsum_wide_kernel(KArgs A, MolDesc md, const double* P, double* J,
-                                                                       double* slot) {
-  const Mol M = mol_of(A, md, slot);
+// Phase 2 of the Fock build over a grid, one wave per output (J[a][q] of a heavy atom, J[h] of a
+// hydrogen), partners summed lane-strided then by a fixed shuffle tree.  SW weights the
+// hydrogen-hydrogen Coulomb terms by their pair switch (the other pairs' partials carry theirs).
+template <bool SW>
+__device__ inline void fock_jsum_wide(const KArgs& A, const Mol& M, const double* P, double* J, const double* sw,
+                                      int wave0, int nwaves) {
   const int np = M.np, lane = threadIdx.x & 63;
   const double* JX = M.slot;
   const double* JH = M.slot + 20LL * M.nxx;
   const int nitems = M.nh * 10 + M.nH;
-  const int nwaves = (gridDim.x * blockDim.x) >> 6;
-  for (int e = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); e < nitems; e += nwaves) {
+  for (int e = wave0; e < nitems; e += nwaves) {
     double s = 0.0;
     if (e < M.nh * 10) {
       const int a = e / 10, q = e % 10;
@@ -3938,7 +3949,9 @@ __global__ __launch_bounds__(LG_THREADS) void lg_fock_jsum_wide_kernel(KArgs A, 
         if (g == h) continue;
         const int lo = g < h ? g : h, hi = g < h ? h : g;
         const int og = A.aoff[M.atom0 + M.nh + g];
-        s += M.Whh[tri_index(lo, hi, M.nH)] * P[(long long)og * np + og];
+        const int t = tri_index(lo, hi, M.nH);
+        if (SW) s += sw[M.nxx + M.nxh + t] * (M.Whh[t] * P[(long long)og * np + og]);
+        else s += M.Whh[t] * P[(long long)og * np + og];
       }
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
@@ -3946,10 +3959,49 @@ __global__ __launch_bounds__(LG_THREADS) void lg_fock_jsum_wide_kernel(KArgs A, 
   }
 }
 
+__global__ __launch_bounds__(LG_THREADS) void lg_fock_jsum_wide_kernel(KArgs A, MolDesc md, const double* P, double* J,
+                                                                       double* slot) {
+  const Mol M = mol_of(A, md, slot);
+  fock_jsum_wide<false>(A, M, P, J, nullptr, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6),
+                        (int)((gridDim.x * blockDim.x) >> 6));
+}
+
 __global__ __launch_bounds__(LG_THREADS) void lg_fock_diag_kernel(KArgs A, MolDesc md, const double* P, double* F,
                                                                   const double* J) {
   const Mol M = mol_of(A, md, nullptr);
   fock_diag(A, M, M.H, P, P, 0.5, F, J, gtid(), gstride());
+}
+
+// The CPHF response operator G(dP) = J(dP) - K(dP / 2) with no H_core (response_fock_rhf,
+// src/fock.rs:444-485) for many right-hand sides at once: blockIdx.y is the right-hand side r,
+// its density dP_r and its output at r * m2 (leading dimension np), its Coulomb partials at
+// r * w_cap of `slots` and its per-atom sums at r * 10 nat of J.  Hz is a zero np x np matrix (the
+// Fock phases add H).  The same three phases as the Fock build, each a grid over the pairs or
+// atoms of every right-hand side; SW applies the per-pair switch (null: every pair at 1).
+template <bool SW>
+__global__ __launch_bounds__(LG_THREADS) void rf_pairs_kernel(KArgs A, MolDesc md, const double* dP, double* G,
+                                                              const double* Hz, const double* sw, double* slots,
+                                                              long long m2, long long wcap) {
+  const long long r = blockIdx.y;
+  const Mol M = mol_of(A, md, slots + r * wcap);
+  fock_pairs<SW>(A, M, Hz, dP + r * m2, dP + r * m2, 0.5, G + r * m2, (int)(blockIdx.x * blockDim.x + threadIdx.x),
+                 (int)(gridDim.x * blockDim.x), sw);
+}
+template <bool SW>
+__global__ __launch_bounds__(LG_THREADS) void rf_jsum_kernel(KArgs A, MolDesc md, const double* dP, double* J,
+                                                             const double* sw, double* slots, long long m2,
+                                                             long long wcap) {
+  const long long r = blockIdx.y;
+  const Mol M = mol_of(A, md, slots + r * wcap);
+  fock_jsum_wide<SW>(A, M, dP + r * m2, J + r * 10LL * md.nat, sw, (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6),
+                     (int)((gridDim.x * blockDim.x) >> 6));
+}
+__global__ __launch_bounds__(LG_THREADS) void rf_diag_kernel(KArgs A, MolDesc md, const double* dP, double* G,
+                                                             const double* Hz, const double* J, long long m2) {
+  const long long r = blockIdx.y;
+  const Mol M = mol_of(A, md, nullptr);
+  fock_diag(A, M, Hz, dP + r * m2, dP + r * m2, 0.5, G + r * m2, J + r * 10LL * md.nat,
+            (int)(blockIdx.x * blockDim.x + threadIdx.x), (int)(gridDim.x * blockDim.x));
 }
 
 // DIIS entry of one iteration: E = T - T^T with T = F P (commutator, src/scf.rs:1039-1054), and
@@ -5832,6 +5884,104 @@ int batch_set_positions(Batch* bp, const std::vector<MolIn>& mols, std::string& 
     return true;
   };
   return upload() ? 0 : 6;
+}
+
+int response_fock_run(const Params& P, const MolIn& mol, int nrhs, const double* dp, const double* pair_sw,
+                      double* out, std::string& err) {
+  std::vector<MolIn> mols{mol};
+  EngineOptions opt{1, 0.0, 0.0, 0, 0.1, 1, 0, 512};
+  Batch* b = nullptr;
+  if (!batch_create(P, mols, opt, RunRequest(), b, err)) {
+    batch_destroy(b);
+    return 6;
+  }
+  if (b->pre_status[0]) {
+    const int st = b->pre_status[0];
+    err = st == 2 || st == 4 ? "no PM3 parameters in this engine for element Z = " + std::to_string(b->pre_z[0])
+                             : "the molecule was refused (status " + std::to_string(st) + ")";
+    batch_destroy(b);
+    return st;
+  }
+  const MolDesc d = b->desc[0];
+  const int n = d.nao, np = d.np, nat = d.nat, nh = d.nheavy, nH = nat - nh;
+  const long long m2 = (long long)np * np, nn = (long long)n * n;
+  const long long npairs = (long long)nat * (nat - 1) / 2;
+  const long long nxx = (long long)nh * (nh - 1) / 2, nxh = (long long)nh * nH;
+  const long long wcap = std::max(20 * npairs, 1LL);
+  const std::vector<int>& mp = b->ao_map[0];
+  const std::vector<int>& am = b->atom_map[0];
+  // the per-pair switch from the reference's pair order -- (u, v), u < v over the caller's atoms,
+  // row by row (src/hamiltonian.rs:237-240) -- to the engine's heavy-heavy, heavy-hydrogen and
+  // hydrogen-hydrogen lists
+  std::vector<double> hsw;
+  if (pair_sw) {
+    auto ref = [&](int u, int v) {
+      if (u > v) std::swap(u, v);
+      return (long long)u * (2LL * nat - u - 1) / 2 + (v - u - 1);
+    };
+    hsw.reserve((size_t)std::max(npairs, 1LL));
+    for (int a = 0; a < nh; ++a)
+      for (int c = a + 1; c < nh; ++c) hsw.push_back(pair_sw[ref(am[a], am[c])]);
+    for (int a = 0; a < nh; ++a)
+      for (int h = 0; h < nH; ++h) hsw.push_back(pair_sw[ref(am[a], am[nh + h])]);
+    for (int g = 0; g < nH; ++g)
+      for (int h = g + 1; h < nH; ++h) hsw.push_back(pair_sw[ref(am[nh + g], am[nh + h])]);
+    if (hsw.empty()) hsw.push_back(0.0);
+  }
+  // right-hand sides per launch: gridDim.y, and the Coulomb partials kept within 1 GiB
+  const int chunk = (int)std::max(1LL, std::min<long long>({(long long)nrhs, 65535LL, (1LL << 27) / wcap}));
+  double *d_in = nullptr, *d_out = nullptr, *d_hz = nullptr, *d_slots = nullptr, *d_J = nullptr, *d_sw = nullptr;
+  bool ok = b->alloc(d_in, (size_t)(chunk * m2), err) && b->alloc(d_out, (size_t)(chunk * m2), err) &&
+            b->alloc(d_hz, (size_t)m2, err) && b->alloc(d_slots, (size_t)(chunk * wcap), err) &&
+            b->alloc(d_J, (size_t)chunk * 10 * std::max(nat, 1), err) &&
+            (!pair_sw || b->alloc(d_sw, hsw.size(), err));
+  auto dev = [&](hipError_t e) {
+    if (e != hipSuccess && ok) {
+      err = hipGetErrorString(e);
+      ok = false;
+    }
+    return ok;
+  };
+  if (ok) dev(smemset(b->stream, d_hz, 0, sizeof(double) * (size_t)m2));
+  if (ok && pair_sw) dev(scopy(b->stream, d_sw, hsw.data(), sizeof(double) * hsw.size(), hipMemcpyHostToDevice));
+  if (ok) {
+    hipLaunchKernelGGL(pair_kernel, dim3(b->pair_blocks), dim3(b->pair_block), 0, b->stream, b->args);
+    dev(hipGetLastError());
+  }
+  const int gp = (int)std::max(1LL, std::min(64LL, (std::max({nxx, nxh, npairs}) + LG_THREADS - 1) / LG_THREADS));
+  const int gj = std::max(1, std::min(64, (nh * 10 + nH + 3) / 4));
+  const int gd = std::max(1, (nat * 16 + LG_THREADS - 1) / LG_THREADS);
+  std::vector<double> hin, hout;
+  for (int r0 = 0; ok && r0 < nrhs; r0 += chunk) {
+    const int nr = std::min(chunk, nrhs - r0);
+    hin.assign((size_t)(nr * m2), 0.0);
+    for (int r = 0; r < nr; ++r)
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) hin[r * m2 + (long long)i * np + j] = dp[(r0 + r) * nn + (long long)mp[i] * n + mp[j]];
+    if (!dev(scopy(b->stream, d_in, hin.data(), sizeof(double) * hin.size(), hipMemcpyHostToDevice))) break;
+    if (!dev(smemset(b->stream, d_out, 0, sizeof(double) * hin.size()))) break;
+    if (pair_sw) {
+      hipLaunchKernelGGL(rf_pairs_kernel<true>, dim3(gp, nr), dim3(LG_THREADS), 0, b->stream, b->args, d, d_in, d_out,
+                         d_hz, d_sw, d_slots, m2, wcap);
+      hipLaunchKernelGGL(rf_jsum_kernel<true>, dim3(gj, nr), dim3(LG_THREADS), 0, b->stream, b->args, d, d_in, d_J, d_sw,
+                         d_slots, m2, wcap);
+    } else {
+      hipLaunchKernelGGL(rf_pairs_kernel<false>, dim3(gp, nr), dim3(LG_THREADS), 0, b->stream, b->args, d, d_in, d_out,
+                         d_hz, d_sw, d_slots, m2, wcap);
+      hipLaunchKernelGGL(rf_jsum_kernel<false>, dim3(gj, nr), dim3(LG_THREADS), 0, b->stream, b->args, d, d_in, d_J,
+                         d_sw, d_slots, m2, wcap);
+    }
+    hipLaunchKernelGGL(rf_diag_kernel, dim3(gd, nr), dim3(LG_THREADS), 0, b->stream, b->args, d, d_in, d_out, d_hz, d_J,
+                       m2);
+    if (!dev(hipGetLastError())) break;
+    hout.resize(hin.size());
+    if (!dev(scopy(b->stream, hout.data(), d_out, sizeof(double) * hout.size(), hipMemcpyDeviceToHost))) break;
+    for (int r = 0; r < nr; ++r)
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) out[(r0 + r) * nn + (long long)mp[i] * n + mp[j]] = hout[r * m2 + (long long)i * np + j];
+  }
+  batch_destroy(b);
+  return ok ? 0 : 6;
 }
 
 bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,

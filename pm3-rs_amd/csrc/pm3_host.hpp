@@ -142,6 +142,13 @@ bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err
 bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,
                int& status, std::string& err, const struct GuessModel* model = nullptr);
 
+// response_fock_rhf (src/fock.rs:444-485) of one molecule for nrhs densities dp[nrhs][n][n]
+// (caller's AO order): out[r] = J(dp_r) - K(dp_r / 2), no H_core, two-centre terms weighted by
+// pair_sw (one weight per atom pair in the reference's order, u < v row by row; null: all 1).
+// Returns 0, or a pm3_status code with err set.
+int response_fock_run(const Params& P, const MolIn& mol, int nrhs, const double* dp, const double* pair_sw,
+                      double* out, std::string& err);
+
 // create + solve + fetch.  Returns false (and sets err) only for a device/launch failure;
 // per-molecule failures are in out[k].status.
 bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt, const RunRequest& req,

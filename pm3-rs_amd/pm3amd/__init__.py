@@ -145,7 +145,8 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_guess_train", "pm3_guess_eval", "pm3_guess_grad", "pm3_guess_mlp", "pm3_guess_features", "pm3_guess_density",
             "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune",
             "pm3_batch_large_timing", "pm3_batch_set_positions", "pm3_opt_options_default", "pm3_optimize_batch",
-            "pm3_optimize", "pm3_batch_carry_densities", "pm3_batch_spin_densities", "pm3_run_noalloc"]
+            "pm3_optimize", "pm3_batch_carry_densities", "pm3_batch_spin_densities", "pm3_run_noalloc",
+            "pm3_response_fock", "pm3_numerical_hessian", "pm3_element_mass"]
 
 _lib = None
 
@@ -165,6 +166,7 @@ def native():
                               ctypes.POINTER(_CResult), dp, dp, dp, dp, dp, dp, dp]
         L.pm3_gradient.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
                                    ctypes.POINTER(_CResult), dp, dp, dp]
+        L.pm3_response_fock.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.c_int32, dp, dp, dp]
         L.pm3_run_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p, ctypes.POINTER(_COptions),
                                     ctypes.c_int32, ip, ctypes.POINTER(_CResult), dp]
         L.pm3_batch_create.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p,
@@ -520,6 +522,32 @@ def closed_form_gradient(molecule: Molecule, params: Optional[Pm3Parameters] = N
     _raise(st, res)
     scf = _result(res, P, None, None, q)
     return GradientResult(scf, res.total_ev, g, -g, float(np.abs(g).max()) if g.size else 0.0)
+
+
+def response_fock(molecule: Molecule, dp, pair_sw=None, params: Optional[Pm3Parameters] = None) -> np.ndarray:
+    """response_fock_rhf (src/fock.rs:444-485) on the GPU: G(dP) = J(dP) - K(dP / 2), no H_core,
+    for one symmetric density (nao, nao) or a stack (k, nao, nao) in one call (pm3_response_fock).
+    pair_sw: None, or one weight per atom pair in the reference's order (u < v, row by row)."""
+    params = params or Pm3Parameters.standard()
+    cm = molecule._c()
+    nao = native().pm3_query_nao(ctypes.byref(cm), params._h)
+    if nao < 0:
+        raise MissingElement("no PM3 parameters in this engine for an element of the molecule")
+    dp = np.ascontiguousarray(dp, dtype=np.float64)
+    one = dp.ndim == 2
+    stack = dp.reshape(1, *dp.shape) if one else dp
+    if stack.ndim != 3 or stack.shape[1:] != (nao, nao):
+        raise InvalidInput(f"dp must be ({nao}, {nao}) or (k, {nao}, {nao})")
+    nat = len(molecule.z)
+    sw = None
+    if pair_sw is not None:
+        sw = np.ascontiguousarray(pair_sw, dtype=np.float64).reshape(-1)
+        if sw.size != nat * (nat - 1) // 2:
+            raise InvalidInput(f"pair_sw needs {nat * (nat - 1) // 2} values, one per atom pair")
+    out = np.zeros_like(stack)
+    _raise(native().pm3_response_fock(ctypes.byref(cm), params._h, stack.shape[0], _dp(stack),
+                                      None if sw is None else _dp(sw), _dp(out)))
+    return out[0] if one else out
 
 
 def single_point(molecule: Molecule, params: Optional[Pm3Parameters] = None, options: Optional[Pm3Options] = None,
