@@ -193,7 +193,11 @@ typedef struct pm3_batch pm3_batch;
  * converge starts from the guess).  Meant for a batch whose geometries move
  * a little between solves (pm3_batch_set_positions); the fixed point is the same, the SCF path
  * shorter.  Molecules on the multi-kernel path keep the options' guess. */
-enum { PM3_BATCH_WANT_DENSITY = 1, PM3_BATCH_WANT_GRADIENT = 2, PM3_BATCH_FORCE_LARGE = 4, PM3_BATCH_WARM_START = 8 };
+/* A batch of one restricted molecule above the tuning entry solo_large_nao AOs (and so a pm3_run or
+ * pm3_gradient call of one) takes the multi-kernel path, unless it is a warm-start batch or asks
+ * for PM3_BATCH_KEEP_BATCHED: one workgroup would leave the rest of the GPU idle. */
+enum { PM3_BATCH_WANT_DENSITY = 1, PM3_BATCH_WANT_GRADIENT = 2, PM3_BATCH_FORCE_LARGE = 4, PM3_BATCH_WARM_START = 8,
+       PM3_BATCH_KEEP_BATCHED = 16 };
 pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z, const double* xyz_bohr,
                             const double* charge, const int32_t* multiplicity, const pm3_params* params,
                             const pm3_options* opts, int32_t flags, pm3_batch** out);
@@ -269,6 +273,8 @@ typedef struct {
                                  36 KiB) on 128-thread workgroups, four per CU */
   int32_t wg256_nao;          /* then up to this many AOs (within 76 KiB) on 256-thread workgroups, two per
                                  CU; the rest on 512-thread workgroups, one per CU */
+  int32_t solo_large_nao;     /* a batch (or pm3_run / pm3_gradient call) of one restricted molecule above
+                                 this many AOs takes the multi-kernel path */
 } pm3_tuning;
 void pm3_tuning_default(pm3_tuning* t);
 pm3_status pm3_tuning_get(pm3_tuning* t);

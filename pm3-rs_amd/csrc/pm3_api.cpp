@@ -240,6 +240,7 @@ pm3_status pm3_run(const pm3_molecule* mol, const pm3_params* params, const pm3_
   if ((s = reference_check(mol, params, eo)) != PM3_OK) return s;
   std::vector<MolIn> mols{MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge, mol->multiplicity}};
   RunRequest req;
+  req.solo_large = true;
   req.want_density = density != nullptr || spin_density != nullptr;
   req.want_orbitals = mo_energies != nullptr || mo_coeff != nullptr || mo_energies_beta != nullptr ||
                       mo_coeff_beta != nullptr;
@@ -321,6 +322,7 @@ pm3_status pm3_batch_create(int32_t n_mol, const int32_t* atom_ptr, const int32_
                             charge ? charge[k] : 0.0, multiplicity ? multiplicity[k] : 1});
   }
   RunRequest req;
+  req.solo_large = (flags & (PM3_BATCH_KEEP_BATCHED | PM3_BATCH_WARM_START)) == 0;
   req.want_density = (flags & PM3_BATCH_WANT_DENSITY) != 0;
   req.want_gradient = (flags & PM3_BATCH_WANT_GRADIENT) != 0;
   req.force_large = (flags & PM3_BATCH_FORCE_LARGE) != 0;
@@ -511,6 +513,7 @@ pm3_status pm3_gradient(const pm3_molecule* mol, const pm3_params* params, const
   if ((s = reference_check(mol, params, eo)) != PM3_OK) return s;
   std::vector<MolIn> mols{MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge, mol->multiplicity}};
   RunRequest req;
+  req.solo_large = true;
   req.want_density = true;
   req.want_gradient = true;
   std::vector<MolResult> out;
@@ -557,6 +560,7 @@ static void to_c(const Tuning& t, pm3_tuning* c) {
   c->large_split = t.large_split;
   c->wg128_nao = t.wg128_nao;
   c->wg256_nao = t.wg256_nao;
+  c->solo_large_nao = t.solo_large_nao;
 }
 
 static Tuning from_c(const pm3_tuning* c) {
@@ -570,6 +574,7 @@ static Tuning from_c(const pm3_tuning* c) {
   t.large_split = c->large_split;
   t.wg128_nao = c->wg128_nao;
   t.wg256_nao = c->wg256_nao;
+  t.solo_large_nao = c->solo_large_nao;
   return t;
 }
 
@@ -634,6 +639,7 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
     mols.push_back(MolIn{atom_ptr[k + 1] - atom_ptr[k], z + atom_ptr[k], xyz_bohr + 3 * (size_t)atom_ptr[k],
                          charge ? charge[k] : 0.0});
   RunRequest req;
+  req.solo_large = (flags & (PM3_BATCH_KEEP_BATCHED | PM3_BATCH_WARM_START)) == 0;
   req.want_density = (flags & PM3_BATCH_WANT_DENSITY) != 0;
   req.want_gradient = (flags & PM3_BATCH_WANT_GRADIENT) != 0;
   req.force_large = (flags & PM3_BATCH_FORCE_LARGE) != 0;
@@ -684,6 +690,7 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
       {&Tuning::large_split, {0, 2, 3, 4}, 2},
       {&Tuning::wg128_nao, {0, 32, 48, 64, 80, 96}, 1},
       {&Tuning::wg256_nao, {0, 64, 80, 96, 112, 128}, 1},
+      {&Tuning::solo_large_nao, {96, 128, 160, 192, 1 << 20}, 0},
   };
   Tuning best = start;
   double tbest = t0;
@@ -721,6 +728,7 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
         else if (ax.m == &Tuning::large_split) name = "large_split";
         else if (ax.m == &Tuning::wg128_nao) name = "wg128_nao";
         else if (ax.m == &Tuning::wg256_nao) name = "wg256_nao";
+        else if (ax.m == &Tuning::solo_large_nao) name = "solo_large_nao";
         std::fprintf(stderr, "pm3_autotune: %s %d: %.3f ms%s\n", name.c_str(), v, tc, same ? "" : " (answers differ)");
       }
       if (!same) {

@@ -3683,6 +3683,11 @@ constexpr int LG_THREADS = 256;
 constexpr int LG_MAXDOT = 3 * MAXD + 2;
 constexpr double LARGE_SP2_MARGIN = 0.15;  // least relative margin of the large path's Lanczos bounds
 constexpr int LG_SPLIT_MAX = 4;            // largest q of the q x q block-lower purification products
+constexpr int LARGE_STEPS_N = 1024;        // up to this size the host enqueues several purification steps ...
+constexpr int LARGE_STEPS_PER_LOOK = 8;    // ... this many, between looks at the device state, each
+                                           // step a single fused product and update kernel
+constexpr int LARGE_LZ_FUSED_N = 512;      // up to this size a Lanczos step is one one-block kernel
+constexpr int LZ_VEC_THREADS = 1024;       // threads of the one-block Lanczos kernels
 
 struct Dots {
   const double* x[LG_MAXDOT];
@@ -3814,6 +3819,266 @@ __global__ __launch_bounds__(LG_THREADS) void lg_sp2_update_kernel(const double*
     if (i == j) tr += x;
   }
   lg_trace_partials(tr, fr, red);
+}
+
+// Device-side control of the purification loop (large_purify_run): the state sp2_density keeps in
+// registers -- tr X and tr X^2 of the current iterate, the tail counter, the steps done, the next
+// step's choice and the outcome -- in a small device array that one block updates after every
+// step, so that the host looks at it only every few steps.  Status: 0 running, 1 converged (X is
+// the projector), 2 a bound was violated (tr X^2 > tr X), 3 no convergence or a wrong trace.
+constexpr int SP2S_TR = 0, SP2S_T2 = 1, SP2S_TAIL = 2, SP2S_STATUS = 3, SP2S_SQ = 4, SP2S_K = 5, SP2S_SEED = 6,
+              SP2S_SEEDED = 7, SP2S_N = 8;
+
+__device__ inline void sp2_decide(double* st, double tr, double t2, double occ, int init, int want_seed);
+
+// After X0 (init) or after a step: tr and tr X^2 from the block partials (the halving tree of
+// lg_finish_kernel), the bookkeeping of the step just taken, then the decisions at the top of the
+// next step of sp2_density's loop (bound check, tail start with the frontier seed, stop, X^2 or
+// 2X - X^2).  A finished state is left as it is.
+__global__ __launch_bounds__(LG_BLOCKS) void lg_sp2_state_kernel(const double* red, double* st, double occ, int init,
+                                                                 int want_seed, int nrows) {
+  __shared__ double t[LG_BLOCKS];
+  if (!init && st[SP2S_STATUS] != 0.0) return;  // the same value for every thread of the block
+  double v[2];
+  for (int q = 0; q < 2; ++q) {
+    t[threadIdx.x] = (int)threadIdx.x < nrows ? red[(size_t)threadIdx.x * LG_MAXDOT + q] : 0.0;
+    __syncthreads();
+    for (int h = LG_BLOCKS / 2; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h) t[threadIdx.x] += t[threadIdx.x + h];
+      __syncthreads();
+    }
+    v[q] = t[0];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  sp2_decide(st, v[0], v[1], occ, init, want_seed);
+}
+
+// lg_sp2_state_kernel's bookkeeping and decisions, by one thread.
+__device__ inline void sp2_decide(double* st, double tr, double t2, double occ, int init, int want_seed) {
+  int tail = init ? -1 : (int)st[SP2S_TAIL];
+  const int k = init ? 0 : (int)st[SP2S_K] + 1;
+  if (!init && tail > 0) --tail;
+  int status = 0, seed = 0;
+  if (init) st[SP2S_SEEDED] = 0.0;
+  if (k >= SP2_MAX) {
+    status = (tail == 0 && fabs(tr - occ) <= 1.0e-6) ? 1 : 3;
+  } else if (t2 > tr + 1.0e-9 * fmax(1.0, tr)) {
+    status = 2;
+  } else {
+    if (tail < 0 && tr - t2 < SP2_TAIL) {
+      tail = SP2_EXTRA;
+      seed = want_seed;
+    }
+    if (tail == 0) status = fabs(tr - occ) > 1.0e-6 ? 3 : 1;
+  }
+  st[SP2S_TR] = tr;
+  st[SP2S_T2] = t2;
+  st[SP2S_TAIL] = (double)tail;
+  st[SP2S_K] = (double)k;
+  st[SP2S_STATUS] = (double)status;
+  st[SP2S_SEED] = (double)seed;
+  st[SP2S_SQ] = fabs(t2 - occ) < fabs(2.0 * tr - t2 - occ) ? 1.0 : 0.0;
+  if (seed) st[SP2S_SEEDED] = 1.0;
+}
+
+// lg_sp2_update_kernel with the step's choice read from the state; nothing once it has finished.
+__global__ __launch_bounds__(LG_THREADS) void lg_sp2_step_kernel(const double* Y, double* X, int n, int np,
+                                                                 const double* st, double* red) {
+  if (st[SP2S_STATUS] != 0.0) return;
+  const bool sq = st[SP2S_SQ] != 0.0;
+  double tr = 0.0, fr = 0.0;
+  for (long long e = gtid(); e < (long long)n * n; e += gstride()) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double y = i >= j ? Y[(size_t)i * np + j] : Y[(size_t)j * np + i];
+    const double x = sq ? y : 2.0 * X[(size_t)i * np + j] - y;
+    X[(size_t)i * np + j] = x;
+    fr += x * x;
+    if (i == j) tr += x;
+  }
+  lg_trace_partials(tr, fr, red);
+}
+
+// One purification step on the matrix cores for matrices up to LARGE_STEPS_N AOs, product and
+// update in one launch: Xn = X X, or 2 X - X X, as the state chose, from X into the other buffer.
+// Each wave owns 16 x 16 tiles of the upper triangle (blocks stride over them) and stores each
+// element and its mirror; loads are bounds-checked, so no padding is read.  Block b leaves the
+// trace and norm partials of its tiles in red[b] (the state kernel sums the grid's rows).
+__global__ __launch_bounds__(64) void lg_sp2_fused_kernel(const double* X, double* Xn, int n, int np, const double* st,
+                                                          double* red) {
+  if (st[SP2S_STATUS] != 0.0) return;
+  const bool sq = st[SP2S_SQ] != 0.0;
+  const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+  const int bm = (n + 15) >> 4, ntile = bm * (bm + 1) / 2;
+  double tr = 0.0, fr = 0.0;
+  for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+    int bi, bj;
+    pad_block<true>(t, bm, bi, bj);
+    const int ia = bi * 16 + li, ja = bj * 16 + li;
+    double4_t c = {0.0, 0.0, 0.0, 0.0};
+    // 64 k values per round: all 32 loads of a round are issued before its 16 MFMAs
+    constexpr int FK = 16;
+    for (int k0 = 0; k0 < n; k0 += 4 * FK) {
+      double a[FK], bv[FK];
+#pragma unroll
+      for (int u = 0; u < FK; ++u) {
+        const int k = k0 + 4 * u + lk;
+        // A[i][k] = X[k][i] (X symmetric) and B[k][j]: lane l holds A[l % 16][l / 16], B[l / 16][l % 16]
+        a[u] = (k < n && ia < n) ? X[(size_t)k * np + ia] : 0.0;
+        bv[u] = (k < n && ja < n) ? X[(size_t)k * np + ja] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < FK; ++u) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], bv[u], c, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {  // item r: row 4 r + l / 16, column l % 16 of the tile
+      const int i = bi * 16 + 4 * r + lk, j = ja;
+      if (i < n && j < n && (bi < bj || i <= j)) {
+        const double y = sq ? c[r] : 2.0 * X[(size_t)i * np + j] - c[r];
+        Xn[(size_t)i * np + j] = y;
+        if (i != j) Xn[(size_t)j * np + i] = y;
+        fr += (i == j ? 1.0 : 2.0) * y * y;
+        if (i == j) tr += y;
+      }
+    }
+  }
+  tr = wave_sum(tr);
+  fr = wave_sum(fr);
+  if (lane == 0) {
+    red[(size_t)blockIdx.x * LG_MAXDOT] = tr;
+    red[(size_t)blockIdx.x * LG_MAXDOT + 1] = fr;
+  }
+}
+
+// A whole Lanczos step for matrices up to LARGE_LZ_FUSED_N AOs in one block: w = A q (A symmetric,
+// np-strided, rows read along their length), then the vector work of lg_lanczos_vec_kernel.
+__global__ __launch_bounds__(LZ_VEC_THREADS) void lg_lanczos_fused_kernel(const double* A, int np, double* q, double* qp,
+                                                                          double* w, int n, double* al, double* be,
+                                                                          int k) {
+  __shared__ double part[LZ_VEC_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int i = wv; i < n; i += LZ_VEC_THREADS / 64) {
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) s += A[(size_t)i * np + j] * q[j];
+    s = wave_sum(s);
+    if (lane == 0) w[i] = s;
+  }
+  __syncthreads();
+  auto block_total = [&](double x) {
+    x = wave_sum(x);
+    __syncthreads();
+    if (lane == 0) part[wv] = x;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < LZ_VEC_THREADS / 64; ++i) s += part[i];
+    return s;
+  };
+  const double bprev = k > 0 ? be[k - 1] : 0.0;
+  double a = 0.0;
+  for (int i = tid; i < n; i += LZ_VEC_THREADS) {
+    const double x = w[i] - bprev * qp[i];
+    w[i] = x;
+    a += q[i] * x;
+  }
+  const double alpha = block_total(a);
+  double b2 = 0.0;
+  for (int i = tid; i < n; i += LZ_VEC_THREADS) {
+    const double x = w[i] - alpha * q[i];
+    w[i] = x;
+    b2 += x * x;
+  }
+  const double beta = sqrt(block_total(b2));
+  if (tid == 0) {
+    al[k] = alpha;
+    be[k] = beta;
+  }
+  const double ib = beta > 0.0 ? 1.0 / beta : 0.0;
+  __syncthreads();  // every thread has read q in the products above before it is overwritten
+  for (int i = tid; i < n; i += LZ_VEC_THREADS) {
+    qp[i] = q[i];
+    q[i] = w[i] * ib;
+  }
+}
+
+// The frontier seed's products when the state asks for one (the tail has just started): out = X v
+// for the symmetric np-strided X, one wave per row with lanes along it; diff: s = t - u.
+__global__ __launch_bounds__(LG_THREADS) void lg_seed_gemv_kernel(const double* X, const double* v, double* out, int n,
+                                                                  int np, const double* st) {
+  if (st[SP2S_SEED] == 0.0 || st[SP2S_STATUS] != 0.0) return;
+  const int lane = threadIdx.x & 63;
+  for (int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); i < n; i += (int)((gridDim.x * blockDim.x) >> 6)) {
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) s += X[(size_t)i * np + j] * v[j];
+    s = wave_sum(s);
+    if (lane == 0) out[i] = s;
+  }
+}
+__global__ __launch_bounds__(LG_THREADS) void lg_seed_diff_kernel(const double* t, const double* u, double* sv, int n,
+                                                                  const double* st) {
+  if (st[SP2S_SEED] == 0.0 || st[SP2S_STATUS] != 0.0) return;
+  for (long long i = gtid(); i < n; i += gstride()) sv[i] = t[i] - u[i];
+}
+
+// Start of the bound-estimating Lanczos run: q_i = 1 + sin(1.7 i + 0.3) / 2, normalised, and qp = 0.
+__global__ __launch_bounds__(LZ_VEC_THREADS) void lg_extremes_start_kernel(double* q, double* qp, int n) {
+  __shared__ double part[LZ_VEC_THREADS / 64];
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  for (int i = tid; i < n; i += LZ_VEC_THREADS) {
+    const double x = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);
+    q[i] = x;
+    qp[i] = 0.0;
+    s += x * x;
+  }
+  s = wave_sum(s);
+  if ((tid & 63) == 0) part[tid >> 6] = s;
+  __syncthreads();
+  double t = 0.0;
+  for (int i = 0; i < LZ_VEC_THREADS / 64; ++i) t += part[i];
+  const double inv = 1.0 / sqrt(t);
+  for (int i = tid; i < n; i += LZ_VEC_THREADS) q[i] *= inv;
+}
+
+// The vector work of one Lanczos step after w = A q (lanczos_step): w -= beta_{k-1} qp,
+// alpha = q.w, w -= alpha q, beta = |w|, then qp = q and q = w / beta (q = 0 after a breakdown to
+// zero); alpha and beta go to al[k], be[k] on the device.  One block, fixed reduction order.
+__global__ __launch_bounds__(LZ_VEC_THREADS) void lg_lanczos_vec_kernel(double* q, double* qp, double* w, int n,
+                                                                        double* al, double* be, int k) {
+  __shared__ double part[LZ_VEC_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  auto block_total = [&](double x) {
+    x = wave_sum(x);
+    __syncthreads();
+    if (lane == 0) part[wv] = x;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < LZ_VEC_THREADS / 64; ++i) s += part[i];
+    return s;
+  };
+  const double bprev = k > 0 ? be[k - 1] : 0.0;
+  double a = 0.0;
+  for (int i = tid; i < n; i += LZ_VEC_THREADS) {
+    const double x = w[i] - bprev * qp[i];
+    w[i] = x;
+    a += q[i] * x;
+  }
+  const double alpha = block_total(a);
+  double b2 = 0.0;
+  for (int i = tid; i < n; i += LZ_VEC_THREADS) {
+    const double x = w[i] - alpha * q[i];
+    w[i] = x;
+    b2 += x * x;
+  }
+  const double beta = sqrt(block_total(b2));
+  if (tid == 0) {
+    al[k] = alpha;
+    be[k] = beta;
+  }
+  const double ib = beta > 0.0 ? 1.0 / beta : 0.0;
+  for (int i = tid; i < n; i += LZ_VEC_THREADS) {
+    qp[i] = q[i];
+    q[i] = w[i] * ib;
+  }
 }
 
 // Pdst = 2 X on the n x n block.
@@ -4137,8 +4402,12 @@ struct LargeBufs {
   int* info = nullptr;
   long long m2 = 0;  // elements per matrix (np_max^2)
   int depth = 0;     // DIIS history slots allocated
-  // device time of the purification products and of the Fock builds in the last solve
-  hipEvent_t evp0 = nullptr, evp1 = nullptr, evf0 = nullptr, evf1 = nullptr;
+  double* sp2s = nullptr;  // device state of the purification loop (SP2S_* slots)
+  double* lzab = nullptr;  // Lanczos coefficients alpha (first half) and beta of the current run
+  // device time of the purification products and of the Fock builds in the last solve: (start,
+  // end) event pairs recorded as they are launched and read once the solve has synchronised
+  std::vector<hipEvent_t> pev, fev;
+  int npev = 0, nfev = 0;
   double product_ms = 0.0, fock_ms = 0.0;
   long long products = 0, fock_builds = 0, product_flop = 0;
   long long lanczos_steps = 0, lanczos_runs = 0;  // frontier-level Lanczos of the last solve
@@ -4216,8 +4485,8 @@ struct Batch {
     if (evg) (void)hipEventDestroy(evg);
     if (evm) (void)hipEventDestroy(evm);
     if (blas) (void)rocblas_destroy_handle(blas);
-    for (hipEvent_t e : {lg.evp0, lg.evp1, lg.evf0, lg.evf1})
-      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : lg.pev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : lg.fev) (void)hipEventDestroy(e);
     for (int i = 0; i < 2; ++i) {
       if (aux[i]) (void)hipStreamSynchronize(aux[i]);
       if (ev_join[i]) (void)hipEventDestroy(ev_join[i]);
@@ -4329,7 +4598,8 @@ static bool large_setup(Batch& b, std::string& err) {
       !b.alloc(L.J, 10LL * nat, err) || !b.alloc(L.slot, slot, err) ||
       !b.alloc(L.red, (size_t)LG_BLOCKS * LG_MAXDOT, err) || !b.alloc(L.dip, 3, err) || !b.alloc(L.info, 1, err) ||
       !b.alloc(L.fin, LG_MAXDOT, err) || !b.alloc(L.lz, 3LL * np, err) || !b.alloc(L.fs, 3LL * np, err) ||
-      !b.alloc(L.bp, (size_t)LG_SPLIT_MAX * 3 * (LG_SPLIT_MAX * (LG_SPLIT_MAX + 1) / 2), err))
+      !b.alloc(L.bp, (size_t)LG_SPLIT_MAX * 3 * (LG_SPLIT_MAX * (LG_SPLIT_MAX + 1) / 2), err) ||
+      !b.alloc(L.sp2s, SP2S_N, err) || !b.alloc(L.lzab, 2LL * (2 * np + 16), err))
     return false;
   {
     std::vector<double> r(np);
@@ -4340,10 +4610,31 @@ static bool large_setup(Batch& b, std::string& err) {
   b.h_red.resize((size_t)LG_BLOCKS * LG_MAXDOT);
   BLAS_OK(rocblas_create_handle(&b.blas));
   BLAS_OK(rocblas_set_stream(b.blas, b.stream));
-  HIP_OK(hipEventCreate(&L.evp0));
-  HIP_OK(hipEventCreate(&L.evp1));
-  HIP_OK(hipEventCreate(&L.evf0));
-  HIP_OK(hipEventCreate(&L.evf1));
+  return true;
+}
+
+// The next (start, end) pair of an event pool, created on first use.
+static hipError_t pool_pair(std::vector<hipEvent_t>& pool, int& used, hipEvent_t& e0, hipEvent_t& e1) {
+  while ((int)pool.size() < 2 * (used + 1)) {
+    hipEvent_t e = nullptr;
+    const hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return r;
+    pool.push_back(e);
+  }
+  e0 = pool[2 * (size_t)used];
+  e1 = pool[2 * (size_t)used + 1];
+  ++used;
+  return hipSuccess;
+}
+
+// Summed device time (ms) of the first `used` pairs of a pool; the events must have completed.
+static bool pool_ms(const std::vector<hipEvent_t>& pool, int used, double& ms, std::string& err) {
+  ms = 0.0;
+  for (int i = 0; i < used; ++i) {
+    float t = 0.0f;
+    HIP_OK(hipEventElapsedTime(&t, pool[2 * (size_t)i], pool[2 * (size_t)i + 1]));
+    ms += t;
+  }
   return true;
 }
 
@@ -4373,18 +4664,16 @@ static bool large_sum1(Batch& b, double& out, std::string& err) {
 
 static bool large_fock(Batch& b, const MolDesc& d, const double* P, double* F, std::string& err) {
   LargeBufs& L = b.lg;
-  HIP_OK(hipEventRecord(L.evf0, b.stream));
+  hipEvent_t f0, f1;
+  HIP_OK(pool_pair(L.fev, L.nfev, f0, f1));
+  HIP_OK(hipEventRecord(f0, b.stream));
   hipLaunchKernelGGL(lg_fock_pairs_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, b.args, d, P, F, b.lg.slot);
   hipLaunchKernelGGL(lg_fock_jsum_wide_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, b.args, d, P, b.lg.J,
                      b.lg.slot);
   hipLaunchKernelGGL(lg_fock_diag_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, b.args, d, P, F,
                      (const double*)b.lg.J);
   HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(L.evf1, b.stream));
-  HIP_OK(hipEventSynchronize(L.evf1));
-  float ms = 0.0f;
-  HIP_OK(hipEventElapsedTime(&ms, L.evf0, L.evf1));
-  L.fock_ms += ms;
+  HIP_OK(hipEventRecord(f1, b.stream));
   ++L.fock_builds;
   return true;
 }
@@ -4450,36 +4739,33 @@ static bool large_extremes(Batch& b, const MolDesc& d, const double* A, double& 
   double* q = L.lz;
   double* qp = L.lz + np;
   double* w = L.lz + 2 * np;
-  std::vector<double> h(n);
-  for (int i = 0; i < n; ++i) h[i] = 1.0 + 0.5 * std::sin(1.7 * (double)i + 0.3);
-  double nn = 0.0;
-  for (double x : h) nn += x * x;
-  nn = std::sqrt(nn);
-  for (double& x : h) x /= nn;
-  HIP_OK(scopy(b.stream, q, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-  HIP_OK(hipMemsetAsync(qp, 0, sizeof(double) * n, b.stream));
-  std::vector<double> al, be, nal;
-  double beta = 0.0;
+  hipLaunchKernelGGL(lg_extremes_start_kernel, dim3(1), dim3(LZ_VEC_THREADS), 0, b.stream, q, qp, n);
   const double one = 1.0, zero = 0.0;
   const int steps = std::min(n, SP2_LZ_STEPS);
+  // all steps enqueued at once (alpha, beta stay on the device), read back together
+  double* dal = L.lzab;
+  double* dbe = L.lzab + (2 * np + 16);
   for (int k = 0; k < steps; ++k) {
+    if (n <= LARGE_LZ_FUSED_N) {
+      hipLaunchKernelGGL(lg_lanczos_fused_kernel, dim3(1), dim3(LZ_VEC_THREADS), 0, b.stream, A, np, q, qp, w, n, dal,
+                         dbe, k);
+      continue;
+    }
     // A symmetric: A^T q, whose kernel reads the column-major columns along rows of A (coalesced)
     BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_transpose, n, n, &one, A, np, q, 1, &zero, w, 1));
-    const double mb = -beta;
-    BLAS_OK(rocblas_daxpy(b.blas, n, &mb, qp, 1, w, 1));
-    double alpha = 0.0;
-    BLAS_OK(rocblas_ddot(b.blas, n, q, 1, w, 1, &alpha));
-    const double ma = -alpha;
-    BLAS_OK(rocblas_daxpy(b.blas, n, &ma, q, 1, w, 1));
-    BLAS_OK(rocblas_dnrm2(b.blas, n, w, 1, &beta));
-    al.push_back(alpha);
-    nal.push_back(-alpha);
-    be.push_back(beta);
-    if (beta <= 1.0e-13 * (std::fabs(alpha) + beta)) break;
-    HIP_OK(hipMemcpyAsync(qp, q, sizeof(double) * n, hipMemcpyDeviceToDevice, b.stream));
-    const double ib = 1.0 / beta;
-    BLAS_OK(rocblas_dscal(b.blas, n, &ib, w, 1));
-    HIP_OK(hipMemcpyAsync(q, w, sizeof(double) * n, hipMemcpyDeviceToDevice, b.stream));
+    hipLaunchKernelGGL(lg_lanczos_vec_kernel, dim3(1), dim3(LZ_VEC_THREADS), 0, b.stream, q, qp, w, n, dal, dbe, k);
+  }
+  HIP_OK(hipGetLastError());
+  std::vector<double> ha(steps), hb(steps);
+  HIP_OK(hipMemcpyAsync(ha.data(), dal, sizeof(double) * steps, hipMemcpyDeviceToHost, b.stream));
+  HIP_OK(hipMemcpyAsync(hb.data(), dbe, sizeof(double) * steps, hipMemcpyDeviceToHost, b.stream));
+  HIP_OK(hipStreamSynchronize(b.stream));
+  std::vector<double> al, be, nal;
+  for (int k = 0; k < steps; ++k) {  // up to and including a breakdown, as the one-step loop stopped
+    al.push_back(ha[k]);
+    nal.push_back(-ha[k]);
+    be.push_back(hb[k]);
+    if (hb[k] <= 1.0e-13 * (std::fabs(ha[k]) + hb[k])) break;
   }
   const int m = (int)al.size();
   hi_est = host_tridiag_max(al, be, m);
@@ -4530,41 +4816,17 @@ static bool large_purify(Batch& b, const MolDesc& d, const double* A, double* Pd
   return large_purify_run(b, d, A, Pdst, glo, ghi, seed, ok, err);
 }
 
-// Frontier seed s = (X - X^2) r of the iterate X in L.V (as frontier_seed in the batched kernel).
-static bool large_seed(Batch& b, const MolDesc& d, std::string& err) {
-  LargeBufs& L = b.lg;
-  const int n = d.nao, np = d.np;
-  double* sv = L.fs;
-  double* t = L.fs + np;
-  const double* r = L.fr;
-  double* u = L.lz;
-  const double one = 1.0, zero = 0.0, mone = -1.0;
-  BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_transpose, n, n, &one, L.V, np, r, 1, &zero, t, 1));
-  BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_transpose, n, n, &one, L.V, np, t, 1, &zero, u, 1));
-  BLAS_OK(rocblas_dcopy(b.blas, n, t, 1, sv, 1));
-  BLAS_OK(rocblas_daxpy(b.blas, n, &mone, u, 1, sv, 1));
-  L.seeded = true;
-  return true;
-}
-
 static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double* Pdst, double lo, double hi, bool seed,
                              bool& ok, std::string& err) {
   LargeBufs& L = b.lg;
   const int n = d.nao, np = d.np, nocc = d.nocc;
   ok = false;
-  auto sums = [&](double& tr, double& t2) -> bool {
-    hipLaunchKernelGGL(lg_finish_kernel, dim3(1), dim3(LG_BLOCKS), 0, b.stream, (const double*)L.red, 2, L.fin);
-    double h[2];
-    HIP_OK(scopy(b.stream, h, L.fin, sizeof(h), hipMemcpyDeviceToHost));
-    tr = h[0];
-    t2 = h[1];
-    return true;
-  };
   // q x q block-lower products: X X restricted to the blocks (I, J), I >= J, of size bs = n / q, as
   // q rounds of one batched dgemm each (round K: Y_IJ += X_IK X_KJ); the update reads only the lower
   // triangle, which these blocks cover.  (q + 1) / (2 q) of the full product's flops.
-  const int q = (!b.large_syrk && b.large_split >= 2 && b.large_split <= LG_SPLIT_MAX && n % b.large_split == 0 &&
-                 n / b.large_split >= 64) ? b.large_split : 0;
+  const bool fused = n <= LARGE_STEPS_N;  // one kernel per step, X ping-ponging between L.V and L.T
+  const int q = (!fused && !b.large_syrk && b.large_split >= 2 && b.large_split <= LG_SPLIT_MAX &&
+                 n % b.large_split == 0 && n / b.large_split >= 64) ? b.large_split : 0;
   const int bs = q ? n / q : 0, nb = q * (q + 1) / 2;
   if (q) {
     std::vector<double*> h((size_t)q * 3 * nb);
@@ -4580,51 +4842,88 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     }
     HIP_OK(scopy(b.stream, L.bp, h.data(), sizeof(double*) * h.size(), hipMemcpyHostToDevice));
   }
-  double tr, t2;
-  hipLaunchKernelGGL(lg_sp2_init_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, A, L.V, n, np, lo, hi, L.red);
-  HIP_OK(hipGetLastError());
-  if (!sums(tr, t2)) return false;
+  // The loop of sp2_density with its decisions on the device (lg_sp2_state_kernel): the host
+  // enqueues `per` steps at a time -- product, update, state -- and then reads the state.  Steps
+  // enqueued after the loop has stopped leave X alone; only their products run.  Small matrices
+  // take several steps per look (their products cost less than the host's round trip), large
+  // ones one.
+  const int per = fused ? LARGE_STEPS_PER_LOOK : 1;
   const double occ = (double)nocc, one = 1.0, zero = 0.0;
-  int tail = -1, k = 0;
-  for (; k < SP2_MAX; ++k) {
-    if (t2 > tr + 1.0e-9 * std::max(1.0, tr)) return true;
-    if (tail < 0 && tr - t2 < SP2_TAIL) {
-      tail = SP2_EXTRA;
-      if (seed && !large_seed(b, d, err)) return false;
-    }
-    if (tail == 0) break;
-    const int sq = std::fabs(t2 - occ) < std::fabs(2.0 * tr - t2 - occ) ? 1 : 0;
-    HIP_OK(hipEventRecord(L.evp0, b.stream));
-    // the block-lower products, or the whole square by dgemm, or one triangle by dsyrk (upper in
-    // column-major order = the row-major lower triangle): the update reads only the row-major lower
-    // triangle, so X stays exactly symmetric
-    if (q) {
-      for (int K = 0; K < q; ++K) {
-        const double beta = K == 0 ? 0.0 : 1.0;
-        BLAS_OK(rocblas_dgemm_batched(b.blas, rocblas_operation_none, rocblas_operation_none, bs, bs, bs, &one,
-                                      (const double* const*)(L.bp + ((size_t)K * 3 + 0) * nb), np,
-                                      (const double* const*)(L.bp + ((size_t)K * 3 + 1) * nb), np, &beta,
-                                      L.bp + ((size_t)K * 3 + 2) * nb, np, nb));
-      }
-    } else if (b.large_syrk)
-      BLAS_OK(rocblas_dsyrk(b.blas, rocblas_fill_upper, rocblas_operation_none, n, n, &one, L.V, np, &zero, L.T, np));
-    else
-      BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, L.V, np, L.V, np,
-                            &zero, L.T, np));
-    HIP_OK(hipEventRecord(L.evp1, b.stream));
-    hipLaunchKernelGGL(lg_sp2_update_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)L.T, L.V, n, np, sq,
-                       L.red);
-    HIP_OK(hipGetLastError());
-    if (!sums(tr, t2)) return false;  // (synchronises, so the product's events have completed)
-    float pms = 0.0f;
-    HIP_OK(hipEventElapsedTime(&pms, L.evp0, L.evp1));
-    L.product_ms += pms;
-    ++L.products;
-    L.product_flop += q ? 2LL * bs * (long long)bs * bs * nb * q : (b.large_syrk ? 1LL : 2LL) * n * (long long)n * n;
-    if (tail > 0) --tail;
+  hipLaunchKernelGGL(lg_sp2_init_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, A, L.V, n, np, lo, hi, L.red);
+  hipLaunchKernelGGL(lg_sp2_state_kernel, dim3(1), dim3(LG_BLOCKS), 0, b.stream, (const double*)L.red, L.sp2s, occ, 1,
+                     seed ? 1 : 0, LG_BLOCKS);
+  HIP_OK(hipGetLastError());
+  double st[SP2S_N] = {0.0};  // read after each group of steps (a state finished at X0 makes them no-ops)
+  const int gs = std::max(1, std::min(LG_BLOCKS, (n + 3) / 4));
+  hipEvent_t f0 = nullptr, f1 = nullptr;  // the fused path times its whole loop (products and updates)
+  const int fbm = (n + 15) / 16, fgrid = std::min(LG_BLOCKS, fbm * (fbm + 1) / 2);
+  if (fused) {
+    HIP_OK(pool_pair(L.pev, L.npev, f0, f1));
+    HIP_OK(hipEventRecord(f0, b.stream));
   }
-  if (tail != 0 || std::fabs(tr - occ) > 1.0e-6) return true;
-  hipLaunchKernelGGL(lg_twice_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)L.V, Pdst, n, np);
+  for (int enq = 0; st[SP2S_STATUS] == 0.0 && enq < SP2_MAX;) {
+    for (int s = 0; s < per && enq < SP2_MAX; ++s, ++enq) {
+      // the iterate of this step: the fused path alternates buffers (if the loop has already
+      // stopped, the step does nothing and this pointer is not read)
+      const double* Xc = fused && (enq & 1) ? L.T : L.V;
+      if (seed) {  // s = (X - X^2) r when the tail has just started (frontier_seed)
+        double* sv = L.fs;
+        double* t = L.fs + np;
+        double* u = L.lz;
+        hipLaunchKernelGGL(lg_seed_gemv_kernel, dim3(gs), dim3(LG_THREADS), 0, b.stream, Xc,
+                           (const double*)L.fr, t, n, np, (const double*)L.sp2s);
+        hipLaunchKernelGGL(lg_seed_gemv_kernel, dim3(gs), dim3(LG_THREADS), 0, b.stream, Xc,
+                           (const double*)t, u, n, np, (const double*)L.sp2s);
+        hipLaunchKernelGGL(lg_seed_diff_kernel, dim3(std::max(1, std::min(LG_BLOCKS, (n + LG_THREADS - 1) / LG_THREADS))),
+                           dim3(LG_THREADS), 0, b.stream, (const double*)t, (const double*)u, sv, n,
+                           (const double*)L.sp2s);
+      }
+      if (fused) {
+        hipLaunchKernelGGL(lg_sp2_fused_kernel, dim3(fgrid), dim3(64), 0, b.stream, Xc, (enq & 1) ? L.V : L.T, n, np,
+                           (const double*)L.sp2s, L.red);
+        hipLaunchKernelGGL(lg_sp2_state_kernel, dim3(1), dim3(LG_BLOCKS), 0, b.stream, (const double*)L.red, L.sp2s,
+                           occ, 0, seed ? 1 : 0, fgrid);
+        HIP_OK(hipGetLastError());
+        ++L.products;
+        const long long bm = (n + 15) / 16;
+        L.product_flop += bm * (bm + 1) / 2 * 2LL * 256 * ((n + 63) / 64 * 64);
+        continue;
+      }
+      hipEvent_t p0, p1;
+      HIP_OK(pool_pair(L.pev, L.npev, p0, p1));
+      HIP_OK(hipEventRecord(p0, b.stream));
+      // the block-lower products, or the whole square by dgemm, or one triangle by dsyrk (upper in
+      // column-major order = the row-major lower triangle): the update reads only the row-major
+      // lower triangle, so X stays exactly symmetric
+      if (q) {
+        for (int K = 0; K < q; ++K) {
+          const double beta = K == 0 ? 0.0 : 1.0;
+          BLAS_OK(rocblas_dgemm_batched(b.blas, rocblas_operation_none, rocblas_operation_none, bs, bs, bs, &one,
+                                        (const double* const*)(L.bp + ((size_t)K * 3 + 0) * nb), np,
+                                        (const double* const*)(L.bp + ((size_t)K * 3 + 1) * nb), np, &beta,
+                                        L.bp + ((size_t)K * 3 + 2) * nb, np, nb));
+        }
+      } else if (b.large_syrk)
+        BLAS_OK(rocblas_dsyrk(b.blas, rocblas_fill_upper, rocblas_operation_none, n, n, &one, L.V, np, &zero, L.T, np));
+      else
+        BLAS_OK(rocblas_dgemm(b.blas, rocblas_operation_none, rocblas_operation_none, n, n, n, &one, L.V, np, L.V, np,
+                              &zero, L.T, np));
+      HIP_OK(hipEventRecord(p1, b.stream));
+      hipLaunchKernelGGL(lg_sp2_step_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)L.T, L.V, n,
+                         np, (const double*)L.sp2s, L.red);
+      hipLaunchKernelGGL(lg_sp2_state_kernel, dim3(1), dim3(LG_BLOCKS), 0, b.stream, (const double*)L.red, L.sp2s, occ,
+                         0, seed ? 1 : 0, LG_BLOCKS);
+      HIP_OK(hipGetLastError());
+      ++L.products;
+      L.product_flop += q ? 2LL * bs * (long long)bs * bs * nb * q : (b.large_syrk ? 1LL : 2LL) * n * (long long)n * n;
+    }
+    HIP_OK(scopy(b.stream, st, L.sp2s, sizeof(st), hipMemcpyDeviceToHost));
+  }
+  if (fused) HIP_OK(hipEventRecord(f1, b.stream));
+  if (st[SP2S_SEEDED] != 0.0) L.seeded = true;
+  if (st[SP2S_STATUS] != 1.0) return true;  // a bound violated, or no convergence: ok stays false
+  const double* Xf = fused && ((int)st[SP2S_K] & 1) ? L.T : L.V;  // after K steps
+  hipLaunchKernelGGL(lg_twice_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, Xf, Pdst, n, np);
   HIP_OK(hipGetLastError());
   ok = true;
   return true;
@@ -4673,40 +4972,48 @@ static bool large_lanczos_top(Batch& b, const MolDesc& d, const double* A, const
   double* w = L.lz + 2 * np;
   HIP_OK(hipMemcpyAsync(q, st, sizeof(double) * n, hipMemcpyDeviceToDevice, b.stream));
   HIP_OK(hipMemsetAsync(qp, 0, sizeof(double) * n, b.stream));
-  std::vector<double> al, be;
-  double beta = 0.0, th1 = -1.0e300, th2 = -1.0e300;
+  double th1 = -1.0e300, th2 = -1.0e300;
   const double one = 1.0, zero = 0.0;
   const int cap = 2 * n + 16, first = seeded ? 8 : 16, every = seeded ? 4 : 8;
   top = 0.0;
-  for (int k = 0; k < cap; ++k) {
-    BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_transpose, n, n, &one, A, np, q, 1, &zero, w, 1));
-    const double mb = -beta;
-    BLAS_OK(rocblas_daxpy(b.blas, n, &mb, qp, 1, w, 1));
-    double alpha = 0.0;
-    BLAS_OK(rocblas_ddot(b.blas, n, q, 1, w, 1, &alpha));
-    const double ma = -alpha;
-    BLAS_OK(rocblas_daxpy(b.blas, n, &ma, q, 1, w, 1));
-    BLAS_OK(rocblas_dnrm2(b.blas, n, w, 1, &beta));
-    al.push_back(alpha);
-    be.push_back(beta);
-    const int mk = k + 1;
-    const bool breakdown = beta <= 1.0e-13 * scale;
-    if (mk == cap || breakdown || (mk >= first && mk % every == 0)) {
-      const double th = host_tridiag_max(al, be, mk);
-      const bool stall = th - th1 <= 1.0e-13 * scale && th - th2 <= 1.0e-12 * scale;
-      top = th;
-      if (mk == cap || breakdown || stall || ritz_residual(al.data(), be.data(), mk, th) <= LZ_TOL) {
-        L.lanczos_steps += mk;
-        ++L.lanczos_runs;
-        return true;
+  // steps are enqueued up to the next check (alpha and beta stay on the device) and then read
+  // back; the checks see exactly the coefficients of the one-step loop, so they stop at the same
+  // step (steps enqueued past a breakdown are not used)
+  double* dal = L.lzab;
+  double* dbe = L.lzab + (2 * np + 16);
+  std::vector<double> al(cap), be(cap);
+  int done = 0, seen = 0;
+  while (done < cap) {
+    const int next = std::min(cap, done < first ? first : done + every);
+    for (; done < next; ++done) {
+      if (n <= LARGE_LZ_FUSED_N) {
+        hipLaunchKernelGGL(lg_lanczos_fused_kernel, dim3(1), dim3(LZ_VEC_THREADS), 0, b.stream, A, np, q, qp, w, n, dal,
+                           dbe, done);
+        continue;
       }
-      th2 = th1;
-      th1 = th;
+      BLAS_OK(rocblas_dgemv(b.blas, rocblas_operation_transpose, n, n, &one, A, np, q, 1, &zero, w, 1));
+      hipLaunchKernelGGL(lg_lanczos_vec_kernel, dim3(1), dim3(LZ_VEC_THREADS), 0, b.stream, q, qp, w, n, dal, dbe, done);
     }
-    BLAS_OK(rocblas_dcopy(b.blas, n, q, 1, qp, 1));
-    const double ib = 1.0 / beta;
-    BLAS_OK(rocblas_dscal(b.blas, n, &ib, w, 1));
-    BLAS_OK(rocblas_dcopy(b.blas, n, w, 1, q, 1));
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(al.data() + seen, dal + seen, sizeof(double) * (done - seen), hipMemcpyDeviceToHost, b.stream));
+    HIP_OK(hipMemcpyAsync(be.data() + seen, dbe + seen, sizeof(double) * (done - seen), hipMemcpyDeviceToHost, b.stream));
+    HIP_OK(hipStreamSynchronize(b.stream));
+    for (int mk = seen + 1; mk <= done; ++mk) {
+      const bool breakdown = be[mk - 1] <= 1.0e-13 * scale;
+      if (mk == cap || breakdown || (mk >= first && mk % every == 0)) {
+        const double th = host_tridiag_max(al, be, mk);
+        const bool stall = th - th1 <= 1.0e-13 * scale && th - th2 <= 1.0e-12 * scale;
+        top = th;
+        if (mk == cap || breakdown || stall || ritz_residual(al.data(), be.data(), mk, th) <= LZ_TOL) {
+          L.lanczos_steps += mk;
+          ++L.lanczos_runs;
+          return true;
+        }
+        th2 = th1;
+        th1 = th;
+      }
+    }
+    seen = done;
   }
   return true;
 }
@@ -5210,7 +5517,12 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
                                                         : lanczos_lds(nao));
     const long long lneed = (long long)std::max(jl, m.n_atoms * 16) * 8;
     // molecules that do not fit one workgroup's LDS take the host-driven multi-kernel path
-    const bool large = lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || req.force_large;
+    // and so does a lone restricted molecule above solo_large_nao AOs: on one workgroup it would
+    // leave the rest of the GPU idle, on the multi-kernel path each step spreads over every CU
+    // (the multi-kernel path starts from the superposition guess: the learned guess covers the
+    // molecules of the persistent kernel)
+    const bool solo = req.solo_large && !arena && nmol == 1 && !use_uhf && nao > tune.solo_large_nao;
+    const bool large = lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || req.force_large || solo;
     // unrestricted molecules take the single-workgroup path only
     if (d.nocc > nao || (large && nao > LARGE_MAX_NAO) || (large && use_uhf)) {
       st = d.nocc > nao ? 1 : large && use_uhf ? 4 : 5;
@@ -5573,6 +5885,7 @@ bool batch_solve(Batch* bp, std::string& err) {
   b.last_kernel_ms = 0.0;
   b.lg.product_ms = b.lg.fock_ms = 0.0;
   b.lg.products = b.lg.fock_builds = b.lg.product_flop = 0;
+  b.lg.npev = b.lg.nfev = 0;
   b.lg.lanczos_steps = b.lg.lanczos_runs = 0;
   if (b.order_all.empty()) return true;
   const bool timed = b.arena == nullptr;  // the allocation-free mode creates no events
@@ -5615,6 +5928,7 @@ bool batch_solve(Batch* bp, std::string& err) {
   }
   HIP_OK(hipEventRecord(b.ev2, b.stream));
   HIP_OK(hipEventSynchronize(b.ev2));
+  if (!pool_ms(b.lg.pev, b.lg.npev, b.lg.product_ms, err) || !pool_ms(b.lg.fev, b.lg.nfev, b.lg.fock_ms, err)) return false;
   float ms = 0.0f, pms = 0.0f, gms = 0.0f;
   HIP_OK(hipEventElapsedTime(&pms, b.ev0, b.evg));
   HIP_OK(hipEventElapsedTime(&gms, b.evg, b.ev1));

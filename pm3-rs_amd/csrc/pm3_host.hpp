@@ -73,6 +73,8 @@ struct RunRequest {
   bool want_gradient = false;
   bool force_large = false;  // solve every molecule on the host-driven multi-kernel path
   bool warm_start = false;   // solves after the first start from each molecule's last density
+  bool solo_large = false;   // a batch of one molecule above Tuning::solo_large_nao AOs takes the
+                             // multi-kernel path (the caller-facing entry points set it)
 };
 
 // Launch shapes and tile choices.  They change speed only: results agree to rounding whatever
@@ -94,6 +96,8 @@ struct Tuning {
                                 // 128-thread workgroups, four per CU
   int wg256_nao = 64;           // then those up to this size fitting 76 KiB on 256-thread workgroups, two
                                 // per CU; the rest on 512-thread workgroups, one per CU
+  int solo_large_nao = 128;     // a batch of a single restricted molecule above this size (AOs) runs on
+                                // the multi-kernel path, which spreads each step over the whole GPU
 };
 // The table batch_create reads; on first use it is loaded from the file PM3_TUNING_FILE names,
 // if set.  Set or loaded tables apply to batches created afterwards (not thread-safe against a

@@ -209,7 +209,8 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
       }
       const pm3_status s = pm3_batch_create((int32_t)members.size(), bptr.data(), bz.data(), bx.data(), bq.data(),
                                             bm.data(), params, opts,
-                                            PM3_BATCH_WANT_GRADIENT | (opt.warm_start ? PM3_BATCH_WARM_START : 0),
+                                            PM3_BATCH_WANT_GRADIENT | PM3_BATCH_KEEP_BATCHED |
+                                                (opt.warm_start ? PM3_BATCH_WARM_START : 0),
                                             &batch);
       pm3_status carried = PM3_OK;
       if (s == PM3_OK && old && opt.warm_start) {

@@ -30,6 +30,7 @@ const Field kFields[] = {
     {"large_split", &Tuning::large_split, 0, 4},
     {"wg128_nao", &Tuning::wg128_nao, 0, 1 << 20},
     {"wg256_nao", &Tuning::wg256_nao, 0, 1 << 20},
+    {"solo_large_nao", &Tuning::solo_large_nao, 0, 1 << 20},
 };
 
 Tuning initial_table() {

@@ -234,7 +234,8 @@ def native():
 
 class _CTuning(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int32) for k in ("pair_block", "pair_blocks_per_cu", "sp2_tile16_n", "sp2_lds",
-                                              "sp2_tight_n", "large_syrk", "large_split", "wg128_nao", "wg256_nao")]
+                                              "sp2_tight_n", "large_syrk", "large_split", "wg128_nao", "wg256_nao",
+                                              "solo_large_nao")]
 
 
 class _CTuneReport(ctypes.Structure):
