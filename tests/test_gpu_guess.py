@@ -227,3 +227,30 @@ def test_trained_guess_cuts_iterations(G):
     print(f"iterations SAD {it_s:.2f} learned {it_l:.2f}; learned kept {used:.3f}; max|dHf| {dh:.2e}", flush=True)
     assert dh <= HOF_TOL
     assert it_l <= (0.88 if trained else 0.97) * it_s
+
+
+def test_conformer_batch_with_learned_guess_and_gradients(G, oracle):
+    """The C4 configuration at reduced count: solvated C12H26 conformers (98 atoms, 194 AOs) in one
+    batch with the packaged learned guess and analytic gradients, at the gradient bench's
+    tolerances (p_tol 1e-10, e_tol 1e-12), against the oracle's SCF + gradient of every molecule:
+    heats of formation within 1e-5 kcal/mol and gradients within 2.295e-7 eV/Bohr."""
+    import pm3amd.guess as GS
+    from pm3amd import workloads as W
+    model = GS.GuessModel.load(GS.DEFAULT_MODEL)
+    mols = [(np.asarray(z, np.int32), W.to_bohr(x)) for z, x in W.solvated_conformers(12, 2027)]
+    tight = dict(p_tol=1e-10, e_tol=1e-12)
+    b = G.Batch([G.Molecule(z, x) for z, x in mols], options=G.Pm3Options(guess="learned", guess_model=model, **tight),
+                want_gradient=True)
+    b.solve()
+    st, res = b.results()
+    grads = b.gradients()
+    b.close()
+    tot, hof, it, ost, og = oracle.gradient_batch(mols, oracle.options(**tight), nthreads=8)
+    assert np.all(st == 0) and np.all(ost == 0)
+    dh = max(abs(r.heat_of_formation_kcal - h) for r, h in zip(res, hof))
+    dg = max(float(np.abs(g - o).max()) for g, o in zip(grads, og))
+    kept = np.mean([r.guess_used == 1 for r in res])
+    print(f"conformers: max|dHf| {dh:.2e} max|dg| {dg:.2e} learned kept {kept:.2f} "
+          f"iterations {np.mean([r.iterations for r in res]):.2f} (oracle {np.mean(it):.2f})", flush=True)
+    assert dh <= HOF_TOL and dg <= 2.295e-7
+    assert kept > 0.5  # the learned density is kept for most conformers (H, C and O only)
