@@ -210,10 +210,12 @@ pm3_status pm3_batch_spin_densities(pm3_batch* batch, double* spin_density);
 void pm3_batch_destroy(pm3_batch* batch);
 /* New positions (Bohr, the creation layout xyz_bohr[3 * (atom_ptr[k] + i) + c], the same atoms)
  * for every molecule of the batch; the next pm3_batch_solve computes integrals, SCF and gradients
- * there.  PM3_INVALID_INPUT for positions that are not finite or put two atoms of a molecule
- * together, PM3_UNSUPPORTED for a batch using PM3_GUESS_LEARNED (its neighbour lists are fixed at
- * creation); the batch is unchanged on failure.  No reference counterpart: a geometry change in
- * the reference is a new run_pm3 call (src/optimizer.rs:344-347). */
+ * there.  A batch using PM3_GUESS_LEARNED rebuilds its 4 Angstrom neighbour lists for the new
+ * positions (growing their device buffers when the pair count outgrows them), so the next solve's
+ * learned guess is the one a batch created at these positions would make.  PM3_INVALID_INPUT for
+ * positions that are not finite or put two atoms of a molecule together; the batch is unchanged
+ * on such a failure.  No reference counterpart: a geometry change in the reference is a new
+ * run_pm3 call (src/optimizer.rs:344-347). */
 pm3_status pm3_batch_set_positions(pm3_batch* batch, const double* xyz_bohr);
 /* For a PM3_BATCH_WARM_START batch dst: take each molecule's last density and result record
  * from the solved batch src, where src_index[k] is dst molecule k's index in src (the same atoms
@@ -332,7 +334,8 @@ typedef struct {
  * gradient solve over all running molecules.  Molecules in the atom_ptr / z / xyz_bohr / charge /
  * multiplicity layout of pm3_run_batch; results[k] per molecule, final positions xyz_out[3 * atom_ptr[n_mol]]
  * (Bohr).  Returns the first device or launch failure, else PM3_OK (per-molecule failures are in
- * results[k].status).  PM3_UNSUPPORTED with PM3_GUESS_LEARNED.
+ * results[k].status).  With PM3_GUESS_LEARNED a solve starts from the learned guess at the
+ * molecule's current positions, unless warm_start gives it the molecule's last converged density.
  *
  * The trajectory (OptResult::trajectory, drive :121-126,207-212: the starting point and every
  * accepted point) is written when the optional arrays are given, with room for T = max_iter + 1

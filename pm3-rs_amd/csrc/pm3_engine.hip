@@ -4464,7 +4464,9 @@ struct Batch {
   const GuessModel* guess_model = nullptr;
   bool learned = false, solved_once = false;
   long long ngp_total = 0;
+  long long gp_cap = 0;          // directed pairs the device pair lists and network buffers hold
   std::vector<int> eligible;
+  std::vector<int8_t> h_elem8;   // guess element index of every atom (engine order), -1 outside H, C, O
   uint16_t* d_X = nullptr;
   float* d_Y = nullptr;
   hipEvent_t evg = nullptr, evm = nullptr;
@@ -4496,9 +4498,32 @@ struct Batch {
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
   bool ensure_guess_buffers(std::string& err) {
-    if (d_X || ngp_total == 0) return true;
-    return alloc(d_X, (size_t)ngp_total * pm3guess::NF, err) && alloc(d_Y, (size_t)ngp_total * pm3guess::NO, err) &&
+    if (d_X || gp_cap == 0) return true;
+    return alloc(d_X, (size_t)gp_cap * pm3guess::NF, err) && alloc(d_Y, (size_t)gp_cap * pm3guess::NO, err) &&
            set_y();
+  }
+  // frees one buffer of this batch (not in the allocation-free mode) after the stream drains
+  template <class T> bool release(T*& p, std::string& err) {
+    if (!p) return true;
+    if (arena) {
+      err = "internal: release in the allocation-free mode";
+      return false;
+    }
+    for (int i = 0; i < nbufs; ++i)
+      if (bufs[i] == (void*)p) {
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e == hipSuccess) e = hipFree(bufs[i]);
+        if (e != hipSuccess) {
+          err = std::string("hipFree: ") + hipGetErrorString(e);
+          return false;
+        }
+        bufs[i] = bufs[--nbufs];
+        bufs[nbufs] = nullptr;
+        p = nullptr;
+        return true;
+      }
+    err = "internal: release of a buffer the batch does not own";
+    return false;
   }
   bool set_y() {
     args.gY = d_Y;
@@ -5365,6 +5390,57 @@ static bool large_solve(Batch& b, int k, std::string& err) {
   return true;
 }
 
+// Directed pairs of the learned guess (molecules made of H, C and O only): for every source atom
+// in the engine's order, its neighbours inside 4 Angstrom in the caller's atom order, the pairs
+// of one molecule contiguous and the molecules in caller order.  Sets each single-workgroup
+// molecule's learn flag and pair range; positions are the engine-order atom coordinates (Bohr).
+struct GuessPairs {
+  std::vector<int> gbeg, gcnt, gsrc, gdst, gpart;
+};
+static void build_guess_pairs(Batch& b, const double* ax, const double* ay, const double* az, GuessPairs& g) {
+  g.gbeg.assign(b.atoms_total, 0);
+  g.gcnt.assign(b.atoms_total, 0);
+  g.gsrc.clear();
+  g.gdst.clear();
+  g.gpart.clear();
+  b.eligible.assign(b.nmol, 0);
+  std::vector<int> mols = b.solve;
+  std::sort(mols.begin(), mols.end());
+  for (int k : mols) {
+    MolDesc& d = b.desc[k];
+    bool ok = d.uhf == 0;  // the learned guess covers restricted molecules
+    for (int t = 0; t < d.nat; ++t) ok = ok && b.h_elem8[d.atom0 + t] >= 0;
+    d.learn = ok ? 1 : 0;
+    d.gp0 = (long long)g.gsrc.size();
+    d.ngp = 0;
+    b.eligible[k] = d.learn;
+    if (!ok) continue;
+    const std::vector<int>& ord = b.atom_map[k];
+    std::vector<int> inv(d.nat);
+    for (int t = 0; t < d.nat; ++t) inv[ord[t]] = t;
+    std::vector<int> pidx((size_t)d.nat * d.nat, -1);
+    for (int sa = 0; sa < d.nat; ++sa) {
+      const int ga = d.atom0 + sa;
+      g.gbeg[ga] = (int)g.gsrc.size();
+      for (int j = 0; j < d.nat; ++j) {
+        const int sb = inv[j];
+        if (sb == sa) continue;
+        const int gb = d.atom0 + sb;
+        const double dx = ax[gb] - ax[ga], dy = ay[gb] - ay[ga], dz = az[gb] - az[ga];
+        if (!(std::sqrt(dx * dx + dy * dy + dz * dz) * pm3guess::BOHR_A < pm3guess::CUT_A)) continue;
+        pidx[(size_t)sa * d.nat + sb] = (int)g.gsrc.size();
+        g.gsrc.push_back(ga);
+        g.gdst.push_back(gb);
+      }
+      g.gcnt[ga] = (int)g.gsrc.size() - g.gbeg[ga];
+    }
+    d.ngp = (int)((long long)g.gsrc.size() - d.gp0);
+    g.gpart.resize(g.gsrc.size());
+    for (long long j = d.gp0; j < (long long)g.gsrc.size(); ++j)
+      g.gpart[j] = pidx[(size_t)(g.gdst[j] - d.atom0) * d.nat + (g.gsrc[j] - d.atom0)];
+  }
+}
+
 static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, const EngineOptions& opt,
                             const RunRequest& req, Batch*& out_batch, std::string& err, unsigned char* arena,
                             size_t arena_bytes, hipStream_t arena_stream);
@@ -5564,46 +5640,14 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
   // launch takes all of it: eigenvectors, purification operands and Lanczos matrices then stay in
   // LDS for as many molecules as they fit.
   if (b.nsolve > 0) b.lds_need = LDS_BYTES_MAX - STATIC_LDS_BYTES;
-  // Directed pairs of the learned guess (molecules made of H, C and O only): for every source
-  // atom in the engine's order, its neighbours inside 4 Angstrom in the caller's atom order.
   std::vector<int8_t> elem8(b.atoms_total, (int8_t)-1);
-  std::vector<int> gbeg(b.atoms_total, 0), gcnt(b.atoms_total, 0), gsrc, gdst, gpart;
-  b.eligible.assign(nmol, 0);
   for (long long t = 0; t < b.atoms_total; ++t) elem8[t] = (int8_t)pm3guess::elem_index(P.table.el[aslot[t]].z);
-  for (int k : b.solve) {
-    MolDesc& d = b.desc[k];
-    bool ok = d.uhf == 0;  // the learned guess covers restricted molecules
-    for (int t = 0; t < d.nat; ++t) ok = ok && elem8[d.atom0 + t] >= 0;
-    d.learn = ok ? 1 : 0;
-    d.gp0 = (long long)gsrc.size();
-    d.ngp = 0;
-    b.eligible[k] = d.learn;
-    if (!ok) continue;
-    const std::vector<int>& ord = b.atom_map[k];
-    std::vector<int> inv(d.nat);
-    for (int t = 0; t < d.nat; ++t) inv[ord[t]] = t;
-    std::vector<int> pidx((size_t)d.nat * d.nat, -1);
-    for (int sa = 0; sa < d.nat; ++sa) {
-      const int ga = d.atom0 + sa;
-      gbeg[ga] = (int)gsrc.size();
-      for (int j = 0; j < d.nat; ++j) {
-        const int sb = inv[j];
-        if (sb == sa) continue;
-        const int gb = d.atom0 + sb;
-        const double dx = ax[gb] - ax[ga], dy = ay[gb] - ay[ga], dz = az[gb] - az[ga];
-        if (!(std::sqrt(dx * dx + dy * dy + dz * dz) * pm3guess::BOHR_A < pm3guess::CUT_A)) continue;
-        pidx[(size_t)sa * d.nat + sb] = (int)gsrc.size();
-        gsrc.push_back(ga);
-        gdst.push_back(gb);
-      }
-      gcnt[ga] = (int)gsrc.size() - gbeg[ga];
-    }
-    d.ngp = (int)((long long)gsrc.size() - d.gp0);
-    gpart.resize(gsrc.size());
-    for (long long j = d.gp0; j < (long long)gsrc.size(); ++j)
-      gpart[j] = pidx[(size_t)(gdst[j] - d.atom0) * d.nat + (gsrc[j] - d.atom0)];
-  }
+  b.h_elem8 = elem8;
+  GuessPairs gp;
+  build_guess_pairs(b, ax.data(), ay.data(), az.data(), gp);
+  std::vector<int>&gbeg = gp.gbeg, &gcnt = gp.gcnt, &gsrc = gp.gsrc, &gdst = gp.gdst, &gpart = gp.gpart;
   b.ngp_total = (long long)gsrc.size();
+  b.gp_cap = b.ngp_total;
   if (b.ngp_total >= (1LL << 31)) {
     err = "too many directed atom pairs in one batch";
     delete bp;
@@ -6148,10 +6192,6 @@ int batch_carry_densities(Batch* dp, const Batch* sp, const std::vector<int>& sr
 
 int batch_set_positions(Batch* bp, const std::vector<MolIn>& mols, std::string& err) {
   Batch& b = *bp;
-  if (b.learned) {
-    err = "new positions for a batch with the learned guess: its neighbour lists come from the creation geometry";
-    return 4;
-  }
   if ((int)mols.size() != b.nmol) {
     err = "molecule count differs from the batch's";
     return 1;
@@ -6190,11 +6230,53 @@ int batch_set_positions(Batch* bp, const std::vector<MolIn>& mols, std::string& 
       az[d.atom0 + t] = m.xyz[3 * i + 2];
     }
   }
+  // the learned guess's neighbour lists follow the new positions: rebuilt on the host, the device
+  // lists and the network buffers regrown (by a quarter at least) when the pair count outgrows them
+  GuessPairs gp;
+  if (b.learned) {
+    const std::vector<MolDesc> saved = b.desc;
+    build_guess_pairs(b, ax.data(), ay.data(), az.data(), gp);
+    const long long ngp = (long long)gp.gsrc.size();
+    if (ngp >= (1LL << 31) || (ngp > b.gp_cap && b.arena)) {
+      b.desc = saved;
+      err = ngp >= (1LL << 31) ? "too many directed atom pairs in one batch"
+                               : "the new positions need more directed atom pairs than the allocation-free scratch holds";
+      return ngp >= (1LL << 31) ? 1 : 4;
+    }
+    if (ngp > b.gp_cap) {
+      int* src = const_cast<int*>(b.args.gsrc);
+      int* dst = const_cast<int*>(b.args.gdst);
+      int* part = const_cast<int*>(b.args.gpart);
+      const long long cap = std::max(ngp, b.gp_cap + b.gp_cap / 4);
+      b.args.gY = nullptr;
+      if (!b.release(src, err) || !b.release(dst, err) || !b.release(part, err) || !b.release(b.d_X, err) ||
+          !b.release(b.d_Y, err) || !b.alloc(src, (size_t)cap, err) || !b.alloc(dst, (size_t)cap, err) ||
+          !b.alloc(part, (size_t)cap, err))
+        return 6;
+      b.args.gsrc = src;
+      b.args.gdst = dst;
+      b.args.gpart = part;
+      b.gp_cap = cap;
+      if (!b.ensure_guess_buffers(err)) return 6;
+    }
+    b.ngp_total = ngp;
+  }
   const size_t bytes = sizeof(double) * (size_t)b.atoms_total;
   auto upload = [&]() -> bool {
     HIP_OK(scopy(b.stream, const_cast<double*>(b.args.ax), ax.data(), bytes, hipMemcpyHostToDevice));
     HIP_OK(scopy(b.stream, const_cast<double*>(b.args.ay), ay.data(), bytes, hipMemcpyHostToDevice));
     HIP_OK(scopy(b.stream, const_cast<double*>(b.args.az), az.data(), bytes, hipMemcpyHostToDevice));
+    if (!b.learned) return true;
+    const size_t ab = sizeof(int) * (size_t)b.atoms_total, pb = sizeof(int) * gp.gsrc.size();
+    HIP_OK(scopy(b.stream, const_cast<int*>(b.args.gbeg), gp.gbeg.data(), ab, hipMemcpyHostToDevice));
+    HIP_OK(scopy(b.stream, const_cast<int*>(b.args.gcnt), gp.gcnt.data(), ab, hipMemcpyHostToDevice));
+    if (pb > 0) {
+      HIP_OK(scopy(b.stream, const_cast<int*>(b.args.gsrc), gp.gsrc.data(), pb, hipMemcpyHostToDevice));
+      HIP_OK(scopy(b.stream, const_cast<int*>(b.args.gdst), gp.gdst.data(), pb, hipMemcpyHostToDevice));
+      HIP_OK(scopy(b.stream, const_cast<int*>(b.args.gpart), gp.gpart.data(), pb, hipMemcpyHostToDevice));
+    }
+    HIP_OK(scopy(b.stream, const_cast<MolDesc*>(b.args.mols), b.desc.data(), sizeof(MolDesc) * b.nmol,
+                 hipMemcpyHostToDevice));
     return true;
   };
   return upload() ? 0 : 6;

@@ -151,7 +151,6 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
   pm3_opt_options_default(&opt);
   if (opt_in) opt = *opt_in;
   if (opt.max_iter < 0 || opt.history < 0 || !(opt.gtol >= 0.0)) return PM3_INVALID_INPUT;
-  if (opts && opts->guess == PM3_GUESS_LEARNED) return PM3_UNSUPPORTED;  // neighbour lists are fixed at creation
 
   // trajectory slots per molecule; the number of accepted points never exceeds max_iter
   const size_t cap = (size_t)opt.max_iter + 1;

@@ -254,3 +254,63 @@ def test_conformer_batch_with_learned_guess_and_gradients(G, oracle):
           f"iterations {np.mean([r.iterations for r in res]):.2f} (oracle {np.mean(it):.2f})", flush=True)
     assert dh <= HOF_TOL and dg <= 2.295e-7
     assert kept > 0.5  # the learned density is kept for most conformers (H, C and O only)
+
+
+def test_moved_learned_batch_matches_a_fresh_one(G):
+    """pm3_batch_set_positions on a learned-guess batch rebuilds its 4 Angstrom neighbour lists:
+    after compressing the molecules (more pairs: the device lists and network buffers regrow),
+    stretching them and moving them back, every solve equals that of a learned batch created at
+    the same positions bit for bit, with the same directed-pair count."""
+    model, _ = _model(G)
+    base = _mols(24, 606)
+    opts = G.Pm3Options(guess="learned", guess_model=model, p_tol=1e-10, e_tol=1e-12)
+
+    def scaled(f):
+        return [x.mean(axis=0) + f * (x - x.mean(axis=0)) for _, x in base]
+
+    b = G.Batch([G.Molecule(z, x) for z, x in base], options=opts, want_density=True, want_gradient=True)
+    b.solve()
+    counts = [b.guess_timing()[2]]
+    for f in (0.9, 1.12, 1.0):
+        moved = scaled(f)
+        b.set_positions(moved)
+        b.solve()
+        s1, r1 = b.results()
+        g1 = b.gradients()
+        n1 = b.guess_timing()[2]
+        fresh = G.Batch([G.Molecule(z, x) for (z, _), x in zip(base, moved)], options=opts, want_density=True,
+                        want_gradient=True)
+        fresh.solve()
+        s2, r2 = fresh.results()
+        g2 = fresh.gradients()
+        n2 = fresh.guess_timing()[2]
+        fresh.close()
+        print(f"scale {f}: directed pairs {n1} (fresh {n2}), learned kept "
+              f"{np.mean([r.guess_used == 1 for r in r1]):.2f}, statuses {np.bincount(s1)}", flush=True)
+        counts.append(n1)
+        assert n1 == n2 and np.array_equal(s1, s2)
+        for a, c, ga, gc in zip(r1, r2, g1, g2):
+            assert a.heat_of_formation_kcal == c.heat_of_formation_kcal
+            assert a.iterations == c.iterations and a.guess_used == c.guess_used
+            assert np.array_equal(a.density, c.density) and np.array_equal(ga, gc)
+    b.close()
+    assert counts[1] > counts[0] > counts[2] and counts[3] == counts[0]
+
+
+def test_learned_guess_optimisations_reach_the_plain_minima(G):
+    """pm3_optimize_batch with the learned guess (each round's batch moves its atoms and rebuilds
+    the guess's neighbour lists) reaches the minima of the superposition-guess optimiser."""
+    model, _ = _model(G)
+    rng = np.random.default_rng(55)
+    mols = [G.Molecule(np.asarray(z, np.int32), W.to_bohr(W.jitter(np.asarray(x), 0.06, rng)))
+            for z, x in (W.alkane(n) for n in (2, 3, 4, 5, 6))]
+    tight = dict(p_tol=1e-10, e_tol=1e-12)
+    st_p, rp = G.optimize_batch(mols, options=G.Pm3Options(**tight), trajectory=False)
+    st_l, rl = G.optimize_batch(mols, options=G.Pm3Options(guess="learned", guess_model=model, **tight),
+                                trajectory=False)
+    assert np.all(st_p == 0) and np.all(st_l == 0)
+    for a, c in zip(rp, rl):
+        print(f"plain {a.heat_of_formation_kcal:.6f} ({a.iterations} it) learned "
+              f"{c.heat_of_formation_kcal:.6f} ({c.iterations} it)", flush=True)
+        assert a.converged and c.converged and c.max_gradient < 1e-3
+        assert abs(a.heat_of_formation_kcal - c.heat_of_formation_kcal) <= 1e-4

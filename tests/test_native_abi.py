@@ -165,8 +165,8 @@ def test_optimizer_defaults_and_host_checks_without_device():
     assert call(1, np.array([0, 3], np.int32), opt=bad) == 1
     learned = G._COptions()
     L.pm3_options_default(ctypes.byref(learned))
-    learned.guess = 4  # PM3_GUESS_LEARNED: neighbour lists are fixed at batch creation
-    assert call(1, np.array([0, 3], np.int32), opts=learned) == 4
+    learned.guess = 4  # PM3_GUESS_LEARNED without a guess model: refused before any device work
+    assert call(1, np.array([0, 3], np.int32), opts=learned) == 1
     m = G.Molecule(z, np.eye(3), 0.0, 0)
     r = G._COptResult()
     assert L.pm3_optimize(ctypes.byref(m._c()), params, ctypes.byref(so), ctypes.byref(o), ctypes.byref(r),
