@@ -121,6 +121,10 @@ def lib():
         L.orc_run_uhf.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(Options),
                                   ctypes.POINTER(Result), dp, dp, dp]
         L.orc_num_threads.restype = ctypes.c_int
+        L.orc_run_dc.argtypes = [ctypes.c_int, ip, dp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(Options), dp,
+                                 ctypes.POINTER(DcResult), dp, dp, dp]
+        L.orc_dc_partition.argtypes = [ctypes.c_int, ip, dp, dp, ip, ip, dp, ctypes.c_int, ip, ip, ip]
+        L.orc_density_diis.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp, ip]
         fp = ctypes.POINTER(ctypes.c_float)
         up = ctypes.POINTER(ctypes.c_uint16)
         L.orc_guess_pairs.argtypes = [ctypes.c_int, ip, dp, ip, ip]
@@ -245,6 +249,89 @@ def run(z, xyz_bohr, charge=0.0, opts=None, want_arrays=True):
     if want_arrays:
         out.update(density=P, mo_energies=eps, mo_coeff=C, charges=q)
     return st, out
+
+
+class DcResult(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_double) for f in ("total_ev", "electronic_ev", "core_ev", "hof_kcal", "fermi_ev",
+                                                "fermi_beta_ev")] + \
+               [(f, ctypes.c_int) for f in ("iterations", "converged", "unrestricted", "dropped_pairs",
+                                             "largest_subsystem", "n_subsystems")]
+
+
+def dc_options(core_radius=6.0, buffer_radius=9.0, min_core_atoms=1, electron_tol=1.0e-9, smearing_ev=0.1):
+    """DcOptions (src/dc/partition.rs:37-107) as the oracle's five numbers; defaults :96-107."""
+    return np.array([core_radius, buffer_radius, float(min_core_atoms), electron_tol, smearing_ev])
+
+
+def run_dc(z, xyz_bohr, charge=0.0, multiplicity=1, opts=None, dc=None):
+    """run_dc (src/dc/scf.rs:76-160): (status, dict of DcResult fields with density, spin_density
+    and charges)."""
+    z, xyz = _mol(z, xyz_bohr)
+    opts = opts or options()
+    dc = dc_options() if dc is None else np.ascontiguousarray(dc, dtype=np.float64)
+    n = nao(z)
+    res = DcResult()
+    P = np.zeros((n, n))
+    S = np.zeros((n, n))
+    q = np.zeros(len(z))
+    st = lib().orc_run_dc(len(z), _i(z), _d(xyz), float(charge), int(multiplicity), ctypes.byref(opts), _d(dc),
+                          ctypes.byref(res), _d(P), _d(S), _d(q))
+    out = {f: getattr(res, f) for f, _ in DcResult._fields_}
+    out.update(density=P, spin_density=S, charges=q)
+    return st, out
+
+
+def dc_gradient(z, xyz_bohr, charge=0.0, multiplicity=1, opts=None, dc=None):
+    """dc_gradient (src/dc/derivatives.rs:55-121): the fixed-density gradient at the partitioned
+    density (the unrestricted form for an unrestricted result), minus f q_A in a field."""
+    opts = opts or options()
+    st, out = run_dc(z, xyz_bohr, charge, multiplicity, opts, dc)
+    if st:
+        return st, out
+    if out["unrestricted"]:
+        g = fixed_density_gradient_spin(z, xyz_bohr, out["density"], out["spin_density"])
+    else:
+        g = fixed_density_gradient(z, xyz_bohr, out["density"])
+    if opts.has_field:
+        g = g - np.outer(out["charges"], np.array(opts.field[:]))
+    out["gradient"] = g
+    return st, out
+
+
+def dc_partition(z, xyz_bohr, dc=None, want_weights=False):
+    """partition (src/dc/partition.rs:184-275): core_of, subsystem count, largest subsystem,
+    dropped pairs, pattern entries, the orbital lists with their core flags and, on request, the
+    summed weights (Partition::weight_sum)."""
+    z, xyz = _mol(z, xyz_bohr)
+    dc = dc_options() if dc is None else np.ascontiguousarray(dc, dtype=np.float64)
+    n = nao(z)
+    core_of = np.zeros(len(z), np.int32)
+    stats = np.zeros(4, np.int32)
+    ws = np.zeros((n, n)) if want_weights else None
+    cap = n * len(z) + 1
+    ptr = np.zeros(cap + 1, np.int32)
+    orb = np.zeros(cap, np.int32)
+    inc = np.zeros(cap, np.int32)
+    st = lib().orc_dc_partition(len(z), _i(z), _d(xyz), _d(dc), _i(core_of), _i(stats),
+                                _d(ws) if ws is not None else None, cap, _i(ptr), _i(orb), _i(inc))
+    if st:
+        raise RuntimeError(STATUS[st])
+    ns = int(stats[0])
+    subs = [(orb[ptr[s]:ptr[s + 1]].copy(), inc[ptr[s]:ptr[s + 1]].astype(bool)) for s in range(ns)]
+    return dict(core_of=core_of, n_subsystems=ns, largest_subsystem=int(stats[1]), dropped_pairs=int(stats[2]),
+                nnz=int(stats[3]), subsystems=subs, weight_sum=ws)
+
+
+def density_diis(dens, res, memory_mb=0):
+    """DensityDiis (src/densitydiis.rs:34-119): push every (density, residual) row, then
+    extrapolate; returns (extrapolation or None, depth)."""
+    dens = np.ascontiguousarray(dens, dtype=np.float64)
+    res = np.ascontiguousarray(res, dtype=np.float64)
+    nvec, ln = dens.shape
+    out = np.zeros(ln)
+    depth = np.zeros(1, np.int32)
+    ok = lib().orc_density_diis(ln, int(memory_mb), nvec, _d(dens), _d(res), _d(out), _i(depth))
+    return (out if ok else None), int(depth[0])
 
 
 def run_pm3(z, xyz_bohr, charge=0.0, multiplicity=1, opts=None):

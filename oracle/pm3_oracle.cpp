@@ -2225,4 +2225,508 @@ int orc_num_threads() {
 #endif
 }
 
+// ---------------------------------------------------------------------------------------
+// Divide-and-conquer SCF (Yang-Lee) of a molecule: the partition (src/dc/partition.rs:184-275),
+// the density pattern (src/dc/pattern.rs:62-171), the Pulay DIIS on the density
+// (src/densitydiis.rs:34-119) and the SCF loop with one global chemical potential
+// (src/dc/scf.rs:76-160, :352-690).  Matrices are dense nao x nao in the caller's AO order; every
+// quantity the reference keeps on the pattern is held as a compact vector of the pattern's entries
+// in its row-major order, so sums run in the reference's order.
+
+struct DcOpts {  // DcOptions (src/dc/partition.rs:37-107), without long_range_cutoff
+  double core_radius, buffer_radius;
+  int min_core_atoms;
+  double electron_tol, smearing_ev;
+};
+
+struct DcPart {
+  std::vector<std::vector<int>> core, atoms, orbitals;
+  std::vector<std::vector<char>> in_core;
+  std::vector<int> core_of;
+};
+
+static inline double v3_norm2(const double* a, const double* b) {
+  const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Cores grown greedily from the outside in; buffers are every other atom within core + buffer
+// radius of some core atom (src/dc/partition.rs:184-275).  Ties of the furthest-atom search go to
+// the last index (Iterator::max_by), of the nearest-core search to the first (min_by).
+static int dc_partition(const Mol& m, const DcOpts& o, DcPart& p) {
+  const int nat = m.nat;
+  if (nat == 0) return E_INVALID;
+  if (o.core_radius <= 0.0 || o.buffer_radius < 0.0) return E_INVALID;
+  const double* x = m.xyz.data();
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int i = 0; i < nat; ++i)
+    for (int k = 0; k < 3; ++k) c[k] += x[3 * i + k];
+  for (int k = 0; k < 3; ++k) c[k] *= 1.0 / nat;
+  const int NONE = -1;
+  p.core_of.assign(nat, NONE);
+  p.core.clear();
+  int remaining = nat;
+  while (remaining > 0) {
+    int seed = -1;
+    double best = 0.0;
+    for (int a = 0; a < nat; ++a) {
+      if (p.core_of[a] != NONE) continue;
+      const double d = v3_norm2(x + 3 * a, c);
+      if (seed < 0 || !(d < best)) { seed = a; best = d; }  // max_by keeps the later of equal elements
+    }
+    const int index = (int)p.core.size();
+    std::vector<int> core;
+    for (int a = 0; a < nat; ++a) {
+      if (p.core_of[a] != NONE) continue;
+      if (std::sqrt(v3_norm2(x + 3 * a, x + 3 * seed)) <= o.core_radius) {
+        p.core_of[a] = index;
+        core.push_back(a);
+        --remaining;
+      }
+    }
+    if ((int)core.size() < o.min_core_atoms && index > 0) {
+      int nearest = 0;
+      double dn = 0.0;
+      for (int q = 0; q < index; ++q) {
+        double dq = INFINITY;
+        for (int i : p.core[q])
+          for (int j : core) dq = std::min(dq, std::sqrt(v3_norm2(x + 3 * i, x + 3 * j)));
+        if (q == 0 || dq < dn) { nearest = q; dn = dq; }
+      }
+      for (int a : core) p.core_of[a] = nearest;
+      p.core[nearest].insert(p.core[nearest].end(), core.begin(), core.end());
+      continue;
+    }
+    p.core.push_back(core);
+  }
+  const double reach = o.core_radius + o.buffer_radius;
+  const int ns = (int)p.core.size();
+  p.atoms.assign(ns, {});
+  p.orbitals.assign(ns, {});
+  p.in_core.assign(ns, {});
+  for (int s = 0; s < ns; ++s) {
+    const std::vector<int>& core = p.core[s];
+    std::vector<int>& atoms = p.atoms[s];
+    atoms = core;
+    const int mine = p.core_of[core[0]];
+    for (int a = 0; a < nat; ++a) {
+      if (p.core_of[a] == mine) continue;
+      bool close = false;
+      for (int q : core) close = close || std::sqrt(v3_norm2(x + 3 * a, x + 3 * q)) <= reach;
+      if (close) atoms.push_back(a);
+    }
+    for (int a : atoms) {
+      const bool is_core = p.core_of[a] == mine;
+      for (int k = 0; k < m.norb[a]; ++k) {
+        p.orbitals[s].push_back(m.off[a] + k);
+        p.in_core[s].push_back(is_core);
+      }
+    }
+  }
+  return OK;
+}
+
+static inline double dc_weight(bool mu_core, bool nu_core) {  // src/dc/partition.rs:171-178
+  return mu_core && nu_core ? 1.0 : (mu_core || nu_core) ? 0.5 : 0.0;
+}
+
+// sum over subsystems of the Yang-Lee weights of every orbital pair (Partition::weight_sum)
+static std::vector<double> dc_weight_sum(const DcPart& p, int nao) {
+  std::vector<double> out((size_t)nao * nao, 0.0);
+  for (size_t s = 0; s < p.orbitals.size(); ++s) {
+    const std::vector<int>& o = p.orbitals[s];
+    for (size_t i = 0; i < o.size(); ++i)
+      for (size_t j = 0; j < o.size(); ++j)
+        out[(size_t)o[i] * nao + o[j]] += dc_weight(p.in_core[s][i], p.in_core[s][j]);
+  }
+  return out;
+}
+
+// Pulay DIIS on a density against its residual P_out - P_in (src/densitydiis.rs:34-119).
+struct DensityDiis {
+  int depth = 8;
+  std::vector<std::vector<double>> dens, res, gram;
+  DensityDiis(size_t len, int memory_mb) {
+    if (memory_mb == 0 || len == 0) {
+      depth = 8;
+    } else {
+      const size_t per = 2 * len * sizeof(double);
+      const size_t d = (size_t)memory_mb * 1024 * 1024 / std::max<size_t>(per, 1);
+      depth = (int)std::min<size_t>(std::max<size_t>(d, 2), 8);
+    }
+  }
+  static double vdot(const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+  }
+  void push(std::vector<double> d, std::vector<double> r) {
+    std::vector<double> row;
+    for (const auto& o : res) row.push_back(vdot(o, r));
+    row.push_back(vdot(r, r));
+    for (size_t i = 0; i < gram.size(); ++i) gram[i].push_back(row[i]);
+    gram.push_back(row);
+    dens.push_back(std::move(d));
+    res.push_back(std::move(r));
+    while ((int)dens.size() > depth) {
+      dens.erase(dens.begin());
+      res.erase(res.begin());
+      gram.erase(gram.begin());
+      for (auto& g : gram) g.erase(g.begin());
+    }
+  }
+  // sum c_i P_i over the longest suffix of the history that solves with sum |c_i| <= 20
+  bool extrapolate(std::vector<double>& out) const {
+    const int n = (int)dens.size();
+    for (int first = 0; first + 1 < n; ++first) {
+      std::vector<std::vector<double>> block;
+      for (int i = first; i < n; ++i) block.emplace_back(gram[i].begin() + first, gram[i].end());
+      std::vector<double> c;
+      if (!diis_coeffs(block, c)) continue;
+      const int kept = n - first;
+      double w = 0.0;
+      for (int i = 0; i < kept; ++i) w += std::fabs(c[i]);
+      if (!std::isfinite(w) || w > 20.0) continue;
+      out.assign(dens[0].size(), 0.0);
+      for (int i = 0; i < kept; ++i)
+        for (size_t j = 0; j < out.size(); ++j) out[j] += c[i] * dens[first + i][j];
+      return true;
+    }
+    return false;
+  }
+};
+
+static inline double fermi_dirac(double e, double mu, double s) {  // src/dc/scf.rs:726-738
+  if (s <= 0.0) return e <= mu ? 1.0 : 0.0;
+  const double x = (e - mu) / s;
+  if (x > 40.0) return 0.0;
+  if (x < -40.0) return 1.0;
+  return 1.0 / (1.0 + std::exp(x));
+}
+
+// The live entries of a partitioned density, row-major (DensityPattern::from_partition).
+struct DcPattern {
+  int nao = 0;
+  std::vector<long long> at;  // dense index mu * nao + nu of every entry, ascending
+  std::vector<int> slot;      // dense index -> entry, -1 outside
+  void build(const DcPart& p, int n) {
+    nao = n;
+    slot.assign((size_t)n * n, -1);
+    std::vector<char> live((size_t)n * n, 0);
+    for (const auto& o : p.orbitals)
+      for (int mu : o)
+        for (int nu : o) live[(size_t)mu * n + nu] = 1;
+    at.clear();
+    for (long long e = 0; e < (long long)n * n; ++e)
+      if (live[e]) {
+        slot[e] = (int)at.size();
+        at.push_back(e);
+      }
+  }
+  size_t nnz() const { return at.size(); }
+  void gather(const std::vector<double>& dense, std::vector<double>& v) const {
+    v.resize(at.size());
+    for (size_t i = 0; i < at.size(); ++i) v[i] = dense[at[i]];
+  }
+  void scatter(const std::vector<double>& v, std::vector<double>& dense) const {
+    dense.assign((size_t)nao * nao, 0.0);
+    for (size_t i = 0; i < at.size(); ++i) dense[at[i]] = v[i];
+  }
+};
+
+// One spin channel (src/dc/scf.rs:604-690): every subsystem's Fock block diagonalised, the
+// chemical potential bisected to `target` electrons, the density assembled with the Yang-Lee
+// weights.  Returns mu; `out` is the assembled density on the pattern.
+static double dc_solve_channel(const std::vector<double>& F, int nao, const DcPart& part, double target,
+                               const DcOpts& dc, const DcPattern& pat, std::vector<double>& out) {
+  const int ns = (int)part.orbitals.size();
+  std::vector<std::vector<double>> w(ns), V(ns);
+  for (int s = 0; s < ns; ++s) {
+    const std::vector<int>& o = part.orbitals[s];
+    const int n = (int)o.size();
+    std::vector<double> loc((size_t)n * n);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) loc[(size_t)i * n + j] = F[(size_t)o[i] * nao + o[j]];
+    w[s].resize(n);
+    V[s].resize((size_t)n * n);
+    sym_eigen(n, loc.data(), w[s].data(), V[s].data());
+  }
+  auto electrons_at = [&](double mu) {
+    double total = 0.0;
+    for (int s = 0; s < ns; ++s) {
+      const int n = (int)w[s].size();
+      for (int k = 0; k < n; ++k) {
+        const double f = fermi_dirac(w[s][k], mu, dc.smearing_ev);
+        if (f <= 0.0) continue;
+        for (int i = 0; i < n; ++i)
+          if (part.in_core[s][i]) {
+            const double c = V[s][(size_t)i * n + k];
+            total += f * c * c;
+          }
+      }
+    }
+    return total;
+  };
+  // bracket (src/dc/scf.rs:693-724)
+  double lowest = INFINITY, highest = -INFINITY;
+  for (int s = 0; s < ns; ++s)
+    for (double e : w[s]) { lowest = std::min(lowest, e); highest = std::max(highest, e); }
+  const double pad = 40.0 * std::max(dc.smearing_ev, 1.0);
+  double low = lowest - pad, high = highest + pad;
+  for (int wid = 0; electrons_at(low) > target && wid < 40; ++wid) low -= pad;
+  for (int wid = 0; electrons_at(high) < target && wid < 40; ++wid) high += pad;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (low + high);
+    if (electrons_at(mid) > target) high = mid;
+    else low = mid;
+    if (std::fabs(high - low) < dc.electron_tol) break;
+  }
+  const double mu = 0.5 * (low + high);
+  out.assign(pat.nnz(), 0.0);
+  for (int s = 0; s < ns; ++s) {
+    const std::vector<int>& o = part.orbitals[s];
+    const int n = (int)o.size();
+    for (int k = 0; k < n; ++k) {
+      const double f = fermi_dirac(w[s][k], mu, dc.smearing_ev);
+      if (f <= 1.0e-14) continue;
+      for (int i = 0; i < n; ++i) {
+        const double ci = V[s][(size_t)i * n + k];
+        if (ci == 0.0) continue;
+        for (int j = 0; j < n; ++j) {
+          const double wt = dc_weight(part.in_core[s][i], part.in_core[s][j]);
+          if (wt == 0.0) continue;
+          out[pat.slot[(size_t)o[i] * nao + o[j]]] += wt * f * ci * V[s][(size_t)j * n + k];
+        }
+      }
+    }
+  }
+  return mu;
+}
+
+struct DcResultC {  // DcResult (src/dc/scf.rs:25-46) without the dense matrices
+  double total_ev, electronic_ev, core_ev, hof_kcal, fermi_ev, fermi_beta_ev;
+  int iterations, converged, unrestricted, dropped_pairs, largest_subsystem, n_subsystems;
+};
+
+// run_dc (src/dc/scf.rs:76-160) with the loop of :376-548.  dcopt = {core_radius, buffer_radius,
+// min_core_atoms, electron_tol, smearing_ev}.  density / spin_density nao x nao, charges nat (any
+// may be null).  E_NOT_CONVERGED leaves the outputs unwritten, as the reference returns an error.
+int orc_run_dc(int nat, const int* z, const double* xyz, double charge, int multiplicity, const Options* opt,
+               const double* dcopt, DcResultC* res, double* density, double* spin_density, double* charges) {
+  const Options& o = *opt;
+  DcOpts dc{dcopt[0], dcopt[1], (int)dcopt[2], dcopt[3], dcopt[4]};
+  Mol m;
+  int st = make_mol(nat, z, xyz, m);
+  if (st) return st;
+  // electron_counts (src/dc/scf.rs:357-398)
+  const int mult = std::max(multiplicity, 1);
+  double n_elec = 0.0;
+  for (int i = 0; i < nat; ++i) n_elec += m.el[i]->tore;
+  n_elec -= charge;
+  const double n_unp = mult - 1;
+  if (n_elec < n_unp || n_elec < 0.0) return E_INVALID;
+  const double na = 0.5 * (n_elec + n_unp), nb = 0.5 * (n_elec - n_unp);
+  bool uhf = na != nb;
+  if (o.reference == 2) uhf = true;
+  if (o.reference == 1) {
+    if (na != nb) return E_INVALID;
+    uhf = false;
+  }
+  long limit = o.integral_memory_mb;
+  if (limit <= 0) {
+    limit = 4096;
+    if (const char* env = std::getenv("PM3_MAX_PAIR_CACHE_MB")) limit = std::strtol(env, nullptr, 10);
+  }
+  if (limit > 0 && pair_cache_bytes(m) > (double)limit * 1048576.0) return E_RESOURCE;
+  Core core;
+  double field_nuclear = 0.0;
+  if ((st = build_core(m, core, o.has_field ? o.field : nullptr, &field_nuclear))) return st;
+  DcPart part;
+  if ((st = dc_partition(m, dc, part))) return st;
+  const int n = m.nao;
+  DcPattern pat;
+  pat.build(part, n);
+  const size_t nnz = pat.nnz();
+  // loop_scf (src/dc/scf.rs:412-548)
+  const double total_e = na + nb;
+  auto share = [&](double e) { return total_e > 0.0 ? e / total_e : 0.5; };
+  std::vector<double> atomic;
+  sad_density(m, atomic);
+  std::vector<double> a(nnz), b(nnz), tmp((size_t)n * n);
+  for (size_t i = 0; i < nnz; ++i) {
+    a[i] = atomic[pat.at[i]] * share(na);
+    b[i] = atomic[pat.at[i]] * share(nb);
+  }
+  const double damping = o.damping > 0.0 ? std::min(std::max(o.damping, 0.0), 0.95) : 0.3;
+  double last = INFINITY;
+  bool converged = false;
+  int iterations = 0;
+  double mu_a = 0.0, mu_b = 0.0;
+  DensityDiis hist(2 * nnz, o.scf_memory_mb);
+  std::vector<double> ptot(nnz), fa, fb, newtot(nnz), Fa((size_t)n * n), Fb((size_t)n * n);
+  std::vector<double> Pd, Pa, Pb, Fva, Fvb;
+  for (int it = 1; it <= o.max_scf; ++it) {
+    iterations = it;
+    for (size_t i = 0; i < nnz; ++i) ptot[i] = a[i] + b[i];
+    pat.scatter(ptot, Pd);
+    pat.scatter(a, Pa);
+    build_fock_spin(m, core, Pd.data(), Pa.data(), Fa.data());
+    if (uhf) {
+      pat.scatter(b, Pb);
+      build_fock_spin(m, core, Pd.data(), Pb.data(), Fb.data());
+    }
+    mu_a = dc_solve_channel(Fa, n, part, na, dc, pat, fa);
+    if (uhf) mu_b = dc_solve_channel(Fb, n, part, nb, dc, pat, fb);
+    else mu_b = mu_a;
+    const std::vector<double>& bfock = uhf ? Fb : Fa;
+    const std::vector<double>& bfresh = uhf ? fb : fa;
+    for (size_t i = 0; i < nnz; ++i) newtot[i] = fa[i] + bfresh[i];
+    pat.gather(Fa, Fva);
+    pat.gather(bfock, Fvb);
+    double d1 = 0.0, d2 = 0.0, d3 = 0.0, sq = 0.0;
+    for (size_t i = 0; i < nnz; ++i) d1 += newtot[i] * Fva[i];
+    for (size_t i = 0; i < nnz; ++i) d2 += fa[i] * Fva[i];
+    for (size_t i = 0; i < nnz; ++i) d3 += bfresh[i] * Fvb[i];
+    for (size_t i = 0; i < nnz; ++i) sq += (newtot[i] - ptot[i]) * (newtot[i] - ptot[i]);
+    const double electronic = 0.5 * (d1 + d2 + d3);
+    const double change = std::sqrt(sq / (double)std::max((long long)n * n, 1LL));
+    for (size_t i = 0; i < nnz; ++i) a[i] = damping * a[i] + (1.0 - damping) * fa[i];
+    for (size_t i = 0; i < nnz; ++i) b[i] = damping * b[i] + (1.0 - damping) * bfresh[i];
+    std::vector<double> cur(2 * nnz), res(2 * nnz);
+    for (size_t i = 0; i < nnz; ++i) {
+      cur[i] = a[i];
+      cur[nnz + i] = b[i];
+      res[i] = fa[i] - a[i];
+      res[nnz + i] = bfresh[i] - b[i];
+    }
+    hist.push(cur, res);
+    std::vector<double> blended;
+    if (hist.extrapolate(blended)) {
+      for (size_t i = 0; i < nnz; ++i) {
+        a[i] = blended[i];
+        b[i] = blended[nnz + i];
+      }
+    }
+    if (std::fabs(electronic - last) < o.e_tol && change < o.p_tol) {
+      converged = true;
+      break;
+    }
+    last = electronic;
+  }
+  if (!converged) return E_NOT_CONVERGED;
+  // the Fock matrices rebuilt at the converged density, the energy there (src/dc/scf.rs:554-572,
+  // :110-122)
+  std::vector<double> dens(nnz);
+  for (size_t i = 0; i < nnz; ++i) dens[i] = a[i] + b[i];
+  pat.scatter(dens, Pd);
+  pat.scatter(a, Pa);
+  build_fock_spin(m, core, Pd.data(), Pa.data(), Fa.data());
+  if (uhf) {
+    pat.scatter(b, Pb);
+    build_fock_spin(m, core, Pd.data(), Pb.data(), Fb.data());
+  } else {
+    Fb = Fa;
+  }
+  std::vector<double> Hv;
+  pat.gather(core.h, Hv);
+  pat.gather(Fa, Fva);
+  pat.gather(Fb, Fvb);
+  double e1 = 0.0, e2 = 0.0, e3 = 0.0;
+  for (size_t i = 0; i < nnz; ++i) e1 += dens[i] * Hv[i];
+  for (size_t i = 0; i < nnz; ++i) e2 += a[i] * Fva[i];
+  for (size_t i = 0; i < nnz; ++i) e3 += b[i] * Fvb[i];
+  const double electronic = 0.5 * (e1 + e2 + e3);
+  const double core_ev = orc_core_core_impl(m) + field_nuclear;
+  double eis = 0.0, ehs = 0.0;
+  for (int i = 0; i < nat; ++i) { eis += m.el[i]->e_isol; ehs += m.el[i]->eheat_ev; }
+  std::vector<double> wsum = dc_weight_sum(part, n);
+  int dropped = 0, largest = 0;
+  for (double v : wsum) dropped += v < 0.5;
+  for (const auto& ob : part.orbitals) largest = std::max(largest, (int)ob.size());
+  if (res) {
+    DcResultC r{};
+    r.electronic_ev = electronic;
+    r.core_ev = core_ev;
+    r.total_ev = electronic + core_ev;
+    r.hof_kcal = (r.total_ev - eis + ehs) * EV_TO_KCAL;
+    r.fermi_ev = mu_a;
+    r.fermi_beta_ev = mu_b;
+    r.iterations = iterations;
+    r.converged = 1;
+    r.unrestricted = uhf;
+    r.dropped_pairs = dropped;
+    r.largest_subsystem = largest;
+    r.n_subsystems = (int)part.orbitals.size();
+    *res = r;
+  }
+  if (density) std::memcpy(density, Pd.data(), sizeof(double) * (size_t)n * n);
+  if (spin_density) {
+    std::vector<double> sd(nnz);
+    for (size_t i = 0; i < nnz; ++i) sd[i] = a[i] - b[i];
+    pat.scatter(sd, tmp);
+    if (uhf) std::memcpy(spin_density, tmp.data(), sizeof(double) * (size_t)n * n);
+    else std::fill(spin_density, spin_density + (size_t)n * n, 0.0);
+  }
+  if (charges)
+    for (int i = 0; i < nat; ++i) {
+      double pop = 0.0;
+      for (int mu = 0; mu < m.norb[i]; ++mu) pop += Pd[(size_t)(m.off[i] + mu) * n + m.off[i] + mu];
+      charges[i] = m.el[i]->tore - pop;
+    }
+  return OK;
+}
+
+// The partition alone: core_of[nat], and {subsystems, largest subsystem, dropped pairs}; with
+// weight_sum (nao x nao, may be null) the summed Yang-Lee weights; with orbitals / in_core /
+// sub_ptr (capacity cap entries, may be null) every subsystem's orbital list.
+int orc_dc_partition(int nat, const int* z, const double* xyz, const double* dcopt, int* core_of, int* stats,
+                     double* weight_sum, int cap, int* sub_ptr, int* orbitals, int* in_core) {
+  DcOpts dc{dcopt[0], dcopt[1], (int)dcopt[2], dcopt[3], dcopt[4]};
+  Mol m;
+  int st = make_mol(nat, z, xyz, m);
+  if (st) return st;
+  DcPart part;
+  if ((st = dc_partition(m, dc, part))) return st;
+  for (int i = 0; i < nat; ++i) core_of[i] = part.core_of[i];
+  std::vector<double> ws = dc_weight_sum(part, m.nao);
+  int dropped = 0, largest = 0;
+  for (double v : ws) dropped += v < 0.5;
+  for (const auto& ob : part.orbitals) largest = std::max(largest, (int)ob.size());
+  DcPattern pat;
+  pat.build(part, m.nao);
+  stats[0] = (int)part.orbitals.size();
+  stats[1] = largest;
+  stats[2] = dropped;
+  stats[3] = (int)pat.nnz();
+  if (weight_sum) std::memcpy(weight_sum, ws.data(), sizeof(double) * ws.size());
+  if (sub_ptr) {
+    int k = 0;
+    sub_ptr[0] = 0;
+    for (size_t s = 0; s < part.orbitals.size(); ++s) {
+      for (size_t i = 0; i < part.orbitals[s].size(); ++i) {
+        if (k >= cap) return E_INVALID;
+        orbitals[k] = part.orbitals[s][i];
+        in_core[k] = part.in_core[s][i];
+        ++k;
+      }
+      sub_ptr[s + 1] = k;
+    }
+  }
+  return OK;
+}
+
+// DensityDiis on nvec pushed (density, residual) pairs of length len: out = the extrapolation
+// (returns 1) or untouched (returns 0); depth gets the history depth.
+int orc_density_diis(int len, int memory_mb, int nvec, const double* dens, const double* res, double* out, int* depth) {
+  DensityDiis d((size_t)len, memory_mb);
+  for (int v = 0; v < nvec; ++v)
+    d.push(std::vector<double>(dens + (size_t)v * len, dens + (size_t)(v + 1) * len),
+           std::vector<double>(res + (size_t)v * len, res + (size_t)(v + 1) * len));
+  if (depth) *depth = d.depth;
+  std::vector<double> o;
+  if (!d.extrapolate(o)) return 0;
+  std::memcpy(out, o.data(), sizeof(double) * o.size());
+  return 1;
+}
+
 }  // extern "C"
