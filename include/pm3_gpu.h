@@ -266,7 +266,8 @@ typedef struct {
   int32_t sp2_tile16_n;       /* purification products on 16x16 wave tiles up to this many AOs, 32x32 above */
   int32_t sp2_lds;            /* purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS,
                                  3 + X in LDS overwritten in place by each product,
-                                 4 + the leading rows of X in LDS where the whole of X does not fit */
+                                 4 + the leading rows of X in LDS where the whole of X does not fit,
+                                 5 + X's upper 16x16 blocks (tiled, swizzled) in LDS, updated in place */
   int32_t sp2_tight_n;        /* purification spectral bounds from Lanczos from this many AOs on */
   int32_t large_syrk;         /* multi-kernel path: purification products by dsyrk (1) or dgemm (0) */
   int32_t large_split;        /* multi-kernel path: q in 2..4 computes only the q x q block-lower part of

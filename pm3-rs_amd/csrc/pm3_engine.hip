@@ -151,10 +151,23 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------------------------------
 // Workgroup reductions.
+// DPP move of a double: both halves through the same data-parallel-primitive lane pattern.
+template <int CTRL> __device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_value(double v, int j);
+// Sum over the 64 lanes, the same value on every lane (full waves only): four DPP steps inside
+// each 16-lane row (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror), then
+// the four row sums read by v_readlane.  VALU latency only, where a butterfly of __shfl_xor would
+// chain six round trips through the LDS crossbar.
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 // Lane j's value of v, given to every lane of the wave (j wave-uniform): v_readlane, a register
 // read, where __shfl would go through the LDS crossbar (ds_bpermute).
@@ -176,6 +189,25 @@ template <int K> __device__ inline void block_sum(double (&v)[K], double* red) {
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
   __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[wave * K + k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = 0.0;
+    for (int w = 0; w < NW; ++w) s += red[w * K + k];
+    v[k] = s;
+  }
+}
+
+// block_sum for callers that have passed a workgroup barrier since every wave last read red (so
+// red may be rewritten at once): one barrier instead of two.
+template <int K> __device__ inline void block_sum_fenced(double (&v)[K], double* red) {
+  static_assert(K <= RED_K, "reduction buffer holds RED_K values per wave");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
   if (lane == 0)
 #pragma unroll
     for (int k = 0; k < K; ++k) red[wave * K + k] = v[k];
@@ -1724,6 +1756,179 @@ __device__ int sp2_run(const Mol& M, const Bufs& B, const double* Fu, double* Pd
   return k;
 }
 
+// ---- Packed tiled image of a symmetric matrix (the purification's LM = 5 placement) -----------
+// Only the upper-triangle 16 x 16 blocks are kept, row-major over the block rows (bm = n / 16
+// rounded up): block (br, bc), br <= bc, starts at tb_index(bm, br, bc) * 256 doubles, and its
+// element (r, c) sits at r * 16 + (c ^ r).  The XOR swizzle puts the 16 elements of a block row
+// and the 16 elements of a block column each in 16 different bank pairs, so a lane group reads
+// either direction, and writes its product tile, without bank conflicts.  Diagonal blocks hold
+// both triangles; X(r, c) with r's block below c's is read from the (c, r) copy.  Half the LDS of
+// a square image and no mirrored stores.
+__host__ __device__ constexpr int tb_blocks(int n) { return ((n + 15) >> 4) * (((n + 15) >> 4) + 1) / 2; }
+__device__ __forceinline__ int tb_index(int bm, int br, int bc) { return br * bm - (br * (br - 1)) / 2 + (bc - br); }
+__device__ __forceinline__ int tb_at(int bm, int r, int c) {
+  int br = r >> 4, bc = c >> 4, lr = r & 15, lc = c & 15;
+  if (br > bc) {
+    const int t = br; br = bc; bc = t;
+    const int u = lr; lr = lc; lc = u;
+  }
+  return (tb_index(bm, br, bc) << 8) + (lr << 4) + (lc ^ lr);
+}
+// Product tiles a wave holds at most in tb_sym_step (registers: 4 doubles each).
+constexpr int TB_HOLD = NT == 512 ? 10 : 9;
+__host__ __device__ constexpr bool tb_fits(int n, int lds_doubles) {
+  return tb_blocks(n) * 256 <= lds_doubles && tb_blocks(n) <= TB_HOLD * NW;
+}
+
+// One 16 x 16 tile (bi, bj) of X X on the FP64 matrix cores: out(i, j) = sum_k X(k, i) X(k, j),
+// each 16-row block of k as four v_mfma_f64_16x16x4_f64 steps, the next block's operands loaded
+// while the current one's MFMAs run.  Lane l: A value X(k, 16 bi + l % 16), B value
+// X(k, 16 bj + l % 16) with k = 16 kb + 4 u + l / 16.
+__device__ inline double4_t tb_tile(int bm, const double* X, int bi, int bj, int li, int lk) {
+  int offd[4], offt[4];  // this lane's element of a directly stored / transposed operand block
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int lr = 4 * u + lk;
+    offd[u] = (lr << 4) + (li ^ lr);
+    offt[u] = (li << 4) + (lr ^ li);
+  }
+  auto load = [&](int kb, int ob, double (&v)[4]) {
+    const bool direct = kb <= ob;
+    const double* base = X + (direct ? (tb_index(bm, kb, ob) << 8) : (tb_index(bm, ob, kb) << 8));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = base[direct ? offd[u] : offt[u]];
+  };
+  double4_t c = {0.0, 0.0, 0.0, 0.0};
+  double a[4], b[4];
+  load(0, bi, a);
+  load(0, bj, b);
+  for (int kb = 0; kb < bm; ++kb) {
+    const int kn = kb + 1 < bm ? kb + 1 : kb;  // after the last block: a harmless reload
+    double na[4], nb[4];
+    load(kn, bi, na);
+    load(kn, bj, nb);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], c, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = na[u];
+      b[u] = nb[u];
+    }
+  }
+  return c;
+}
+
+// X <- X^2 (sq) or 2 X - X^2 in place on the packed tiled image: every wave computes its upper
+// tiles into registers, all waves meet once X has been read, then each tile is stored over its own
+// block (one writer per element, no mirror).  w0 / w1 receive this thread's share of tr X and
+// |X|_F^2 of the new X (off-diagonal blocks count twice).  Requires tb_fits.
+__device__ inline void tb_sym_step(int n, int bm, double* X, bool sq, double& w0, double& w1) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nblk = bm * (bm + 1) / 2;
+  double4_t c[TB_HOLD];
+  int bi_[TB_HOLD], bj_[TB_HOLD];
+#pragma unroll
+  for (int h = 0; h < TB_HOLD; ++h) {
+    const int t = wave + h * NW;
+    bi_[h] = bj_[h] = 0;
+    if (t < nblk) {
+      pad_block<true>(t, bm, bi_[h], bj_[h]);
+      c[h] = tb_tile(bm, X, bi_[h], bj_[h], li, lk);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int h = 0; h < TB_HOLD; ++h) {
+    const int t = wave + h * NW;
+    if (t < nblk) {
+      const int bi = bi_[h], bj = bj_[h];
+      double* blk = X + (tb_index(bm, bi, bj) << 8);
+      const double wt = bi < bj ? 2.0 : 1.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int lr = 4 * r + lk;
+        double* p = blk + (lr << 4) + (li ^ lr);
+        const double x2 = c[h][r];
+        const double y = sq ? x2 : 2.0 * *p - x2;
+        *p = y;
+        w1 += wt * y * y;
+        if (bi == bj && lr == li) w0 += y;
+      }
+    }
+  }
+}
+
+// frontier_seed on the packed tiled image.
+__device__ void frontier_seed_tb(const double* X, int bm, int n, const Bufs& B) {
+  const int tid = threadIdx.x;
+  double t = 0.0;
+  if (tid < n)
+    for (int j = 0; j < n; ++j) t += X[tb_at(bm, j, tid)] * seed_r(j);
+  if (tid < n) B.ft[tid] = t;
+  __syncthreads();
+  double u = 0.0;
+  if (tid < n)
+    for (int j = 0; j < n; ++j) u += X[tb_at(bm, j, tid)] * B.ft[j];
+  __syncthreads();
+  if (tid < n) B.fs[tid] = t - u;
+  if (tid == 0 && B.bnd) B.bnd[3] = 1.0;
+  __syncthreads();
+}
+
+// The purification loop of sp2_run on the packed tiled image (the same stopping rules; the trace
+// sums run in another order, so a step choice can differ only where two of them tie to rounding).
+__device__ int sp2_run_tb(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo,
+                          double hi, bool seed) {
+  const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
+  const int bm = (n + 15) >> 4, nimg = tb_blocks(n) * 256;
+  double* X = g_lds;
+  const double s = 1.0 / (hi - lo);
+  double v[2] = {0.0, 0.0};
+  for (int e = tid; e < nimg; e += NT) {
+    const int blk = e >> 8;
+    int br = 0;
+    while (tb_index(bm, br + 1, br + 1) <= blk && br + 1 < bm) ++br;
+    const int bc = br + (blk - tb_index(bm, br, br));
+    const int lr = (e >> 4) & 15, lc = (e & 15) ^ lr;
+    const int r = (br << 4) + lr, c = (bc << 4) + lc;
+    double x = 0.0;
+    if (r < n && c < n) {
+      x = ((r == c ? hi : 0.0) - Fu[r * np + c]) * s;
+      v[1] += (br < bc ? 2.0 : 1.0) * x * x;
+      if (r == c) v[0] += x;
+    }
+    X[e] = x;
+  }
+  block_sum<2>(v, red);
+  double tr = v[0], t2 = v[1];
+  const double occ = (double)nocc;
+  int tail = -1, k = 0;
+  for (; k < SP2_MAX; ++k) {
+    if (t2 > tr + 1.0e-9 * fmax(1.0, tr)) return -2;
+    if (tail < 0 && tr - t2 < SP2_TAIL) {
+      tail = SP2_EXTRA;
+      if (seed) frontier_seed_tb(X, bm, n, B);
+    }
+    if (tail == 0) break;
+    const bool sq = fabs(t2 - occ) < fabs(2.0 * tr - t2 - occ);
+    double w[2] = {0.0, 0.0};
+    tb_sym_step(n, bm, X, sq, w[0], w[1]);
+    block_sum_fenced<2>(w, red);  // tb_sym_step's barrier follows every wave's last read of red
+    tr = w[0];
+    t2 = w[1];
+    if (tail > 0) --tail;
+  }
+  if (tail != 0) return -1;
+  if (fabs(tr - occ) > 1.0e-6) return -2;
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e % n;
+    Pdst[i * np + j] = M.wt * X[tb_at(bm, i, j)];
+  }
+  __syncthreads();
+  return k;
+}
+
 __host__ __device__ constexpr int lanczos_lds(int n);
 __device__ __noinline__ void lanczos_extremes(const double* Ag, int n, int np, double* v, double& lo_est, double& hi_est);
 constexpr double SP2_MARGIN_AGED = 0.03;  // extra margin when the estimate is from an earlier SCF step
@@ -1733,6 +1938,7 @@ constexpr double FRONTIER_SEED_DP = 1.0e-4;  // RMS density change below which p
 __device__ inline int sp2_dispatch(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo,
                                    double hi, bool seed) {
   const int img = sp2_img(M.n);
+  if (M.sp2_lds >= 5 && tb_fits(M.n, M.lds_doubles)) return sp2_run_tb(M, B, Fu, Pdst, red, lo, hi, seed);
   if (M.sp2_lds >= 1 && 2 * img <= M.lds_doubles) return sp2_run<2>(M, B, Fu, Pdst, red, lo, hi, seed);
   if (M.sp2_lds >= 3 && img <= M.lds_doubles && hold_fits(M.n, M.n <= M.sp2_tile16_n ? 16 : 32))
     return sp2_run<3>(M, B, Fu, Pdst, red, lo, hi, seed);
@@ -2048,52 +2254,96 @@ __device__ __forceinline__ double lanczos_top(const double* La, bool sq, const d
   return top;
 }
 
-// lanczos_top run by one wave for n <= 64, with no workgroup barrier: lane i holds element i of
-// the basis vector q and the residual w in registers; the product reads the square LDS matrix L
-// (ld n) down its columns, w_j given to the wave by lane j.  The tridiagonal al / be (lanczos_cap(n)
-// + 2 doubles each) is the wave's own LDS; lane 0 writes it and the whole wave reads it at the
-// checks.  Start vector, recurrence, checks and stopping rules are lanczos_top's; only the
-// product's summation order differs (two running sums over the columns instead of per-slice sums).
-__device__ __noinline__ double lanczos_top_wave(const double* L, int n, double scale, double* al, double* be,
+// lanczos_top run by one wave, with no workgroup barrier: lane i holds elements i and i + 64 (R = 2,
+// n <= 128) of the basis vector q and the residual w in registers; the product reads the operator
+// L (leading dimension ld: LDS or an HBM slab matrix) down its columns, w_j given to the wave by the
+// lane that holds it.  The tridiagonal al / be (lanczos_cap(n) + 2 doubles each) is the wave's own
+// LDS; lane 0 writes it and the whole wave reads it at the checks.  Start vector, recurrence,
+// checks and stopping rules are lanczos_top's; only the product's summation order differs (two
+// running sums over the columns instead of per-slice sums).
+template <int R>
+__device__ __noinline__ double lanczos_top_wave(const double* L, int ld, int n, double scale, double* al, double* be,
                                                 const double* s, const double* t, int side,
                                                 unsigned long long* steps) {
   const int lane = threadIdx.x & 63;
   const int cap = lanczos_cap(n);
-  const bool live = lane < n;
-  const int r = live ? lane : 0;  // lanes past n read row 0 and keep zeros
-  double w, q = 0.0;
+  bool live[R];
+  int row[R];
+  double w[R], q[R];
+#pragma unroll
+  for (int h = 0; h < R; ++h) {
+    const int i = lane + 64 * h;
+    live[h] = i < n;
+    row[h] = live[h] ? i : 0;  // lanes past n read row 0 and keep zeros
+    q[h] = 0.0;
+  }
   {
-    const double g = 1.0 + 0.5 * sin(1.7 * (double)lane + 0.3);
-    const double x = side == 0 || !live ? 0.0 : side == 1 ? t[lane] : s[lane] - t[lane];
+    double x[R], g[R], xx = 0.0, gg = 0.0;
+#pragma unroll
+    for (int h = 0; h < R; ++h) {
+      const int i = lane + 64 * h;
+      g[h] = 1.0 + 0.5 * sin(1.7 * (double)i + 0.3);
+      x[h] = side == 0 || !live[h] ? 0.0 : side == 1 ? t[i] : s[i] - t[i];
+      xx += x[h] * x[h];
+      gg += live[h] ? g[h] * g[h] : 0.0;
+    }
     double ca = 0.0, cg = 1.0;
     if (side != 0) {
-      const double a2 = wave_sum(x * x);
-      const double g2 = wave_sum(live ? g * g : 0.0);
+      const double a2 = wave_sum(xx);
+      const double g2 = wave_sum(gg);
       if (a2 > 1.0e-280 && a2 < 1.0e300) {
         ca = 1.0 / sqrt(a2);
         cg = 1.0e-3 / sqrt(g2);
       }
     }
-    w = !live ? 0.0 : ca != 0.0 ? ca * x + cg * g : g;
+#pragma unroll
+    for (int h = 0; h < R; ++h) w[h] = !live[h] ? 0.0 : ca != 0.0 ? ca * x[h] + cg * g[h] : g[h];
   }
-  double wn = sqrt(wave_sum(w * w));
+  double ww = 0.0;
+#pragma unroll
+  for (int h = 0; h < R; ++h) ww += w[h] * w[h];
+  double wn = sqrt(wave_sum(ww));
   const int first = side != 0 ? 12 : 16, every = side != 0 ? 3 : 7;
   double bsub = 0.0, th1 = -1.0e300, th2 = -1.0e300, top = 0.0;
   for (int k = 0; k < cap; ++k) {
-    double s0 = 0.0, s1 = 0.0;
-    int j = 0;
-    for (; j + 1 < n; j += 2) {
-      s0 += L[j * n + r] * lane_value(w, j);
-      s1 += L[(j + 1) * n + r] * lane_value(w, j + 1);
+    double s0[R], s1[R];
+#pragma unroll
+    for (int h = 0; h < R; ++h) s0[h] = s1[h] = 0.0;
+#pragma unroll
+    for (int hj = 0; hj < R; ++hj) {  // columns j held by w[hj]
+      const int j0 = 64 * hj, j1 = min(n, j0 + 64);
+      int j = j0;
+      for (; j + 1 < j1; j += 2) {
+        const double wa = lane_value(w[hj], j - j0), wb = lane_value(w[hj], j + 1 - j0);
+#pragma unroll
+        for (int h = 0; h < R; ++h) {
+          s0[h] += L[(size_t)j * ld + row[h]] * wa;
+          s1[h] += L[(size_t)(j + 1) * ld + row[h]] * wb;
+        }
+      }
+      if (j < j1) {
+        const double wa = lane_value(w[hj], j - j0);
+#pragma unroll
+        for (int h = 0; h < R; ++h) s0[h] += L[(size_t)j * ld + row[h]] * wa;
+      }
     }
-    if (j < n) s0 += L[j * n + r] * lane_value(w, j);
     // q_k = w / wn; w = A q_k - beta_{k-1} q_{k-1}; alpha_k = q_k . w; w -= alpha_k q_k
-    const double qn = w / wn;
-    const double x = live ? (s0 + s1) / wn - bsub * q : 0.0;
-    q = qn;
-    const double alpha = wave_sum(qn * x);
-    w = x - alpha * q;
-    const double beta = sqrt(wave_sum(w * w));
+    double qx = 0.0, x[R], qn[R];
+#pragma unroll
+    for (int h = 0; h < R; ++h) {
+      qn[h] = w[h] / wn;
+      x[h] = live[h] ? (s0[h] + s1[h]) / wn - bsub * q[h] : 0.0;
+      q[h] = qn[h];
+      qx += qn[h] * x[h];
+    }
+    const double alpha = wave_sum(qx);
+    double bb = 0.0;
+#pragma unroll
+    for (int h = 0; h < R; ++h) {
+      w[h] = x[h] - alpha * q[h];
+      bb += w[h] * w[h];
+    }
+    const double beta = sqrt(wave_sum(bb));
     const int mk = k + 1;
     const bool breakdown = beta <= 1.0e-13 * scale;
     wn = beta;
@@ -2138,10 +2388,6 @@ __device__ __noinline__ double lanczos_top_wave(const double* L, int n, double s
   }
   return top;
 }
-// LDS doubles of frontier_levels' one-wave runs: both operators as full squares and each run's
-// tridiagonal.
-__host__ __device__ constexpr int lanczos_wave_lds(int n) { return 2 * n * n + 4 * (lanczos_cap(n) + 2); }
-
 // Estimates of the lowest and highest eigenvalue of the symmetric np-strided matrix Ag from a
 // fixed number of Lanczos steps (Ritz values: lo_est >= lambda_min, hi_est <= lambda_max); LDS
 // v as lanczos_lds(n).
@@ -2215,10 +2461,16 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
     __syncthreads();
     return;
   }
-  // up to 64 AOs both runs go at once, one wave each, both operators square in LDS
-  const bool waves = n <= 64 && lanczos_wave_lds(n) <= M.lds_doubles;
-  const bool square = waves || lanczos_lds_square(n) <= M.lds_doubles;
-  const bool packed = square || lanczos_lds_packed(n) <= M.lds_doubles;
+  // up to 128 AOs both runs go at once, one wave each (one or two vector elements per lane), with
+  // their operators square in LDS as far as they fit and in the slab (HBM) otherwise; above, one
+  // run after the other over the whole workgroup
+  const int cap2 = lanczos_cap(n) + 2;
+  const bool waves = n <= 128;
+  const bool wv_both = waves && 2 * n * n + 4 * cap2 <= M.lds_doubles;
+  const bool wv_one = waves && !wv_both && n * n + 4 * cap2 <= M.lds_doubles;
+  const bool square = wv_both || wv_one || (!waves && lanczos_lds_square(n) <= M.lds_doubles);
+  const bool packed = square || (!waves && lanczos_lds_packed(n) <= M.lds_doubles);
+  const bool av_lds = wv_both;
   double* Lp = packed ? g_lds : nullptr;
   double* Lv = g_lds + n * n;  // -A_v of the one-wave runs
   double* vec = square ? g_lds + n * n : packed ? g_lds + n * (n + 1) / 2 : g_lds;
@@ -2246,7 +2498,7 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
                     Ao[i * np + j] = ao;
                   }
                   const double av = -(Fu[i * np + j] - G[i * np + j] - G[j * np + i] + sv + c_hi * x);
-                  if (waves) Lv[i * n + j] = av;
+                  if (av_lds) Lv[i * n + j] = av;
                   else Av[i * np + j] = av;
                 });
   __syncthreads();
@@ -2263,13 +2515,19 @@ __device__ __noinline__ void frontier_levels(const Mol& M, const Bufs& B, const 
   unsigned long long* lsteps = B.ph ? &B.ph[10] : nullptr;
   if (B.ph && tid == 0) B.ph[11] += (unsigned long long)((nocc > 0) + (nocc < n));
   if (waves) {
-    const int wave = tid >> 6, cap2 = lanczos_cap(n) + 2;
-    double* tri = g_lds + 2 * n * n + 2 * cap2 * wave;
+    const int wave = tid >> 6;
+    double* tri = g_lds + (wv_both ? 2 * n * n : wv_one ? n * n : 0) + 2 * cap2 * wave;
+    const double* Lo = square ? Lp : Ao;
+    const double* Lw = av_lds ? Lv : Av;
+    const int ldo = square ? n : np, ldw = av_lds ? n : np;
     if (wave == 0 && nocc > 0) {
-      const double homo = lanczos_top_wave(Lp, n, scale, tri, tri + cap2, B.fs, B.ft, seeded, lsteps);
+      const double homo = n <= 64 ? lanczos_top_wave<1>(Lo, ldo, n, scale, tri, tri + cap2, B.fs, B.ft, seeded, lsteps)
+                                  : lanczos_top_wave<2>(Lo, ldo, n, scale, tri, tri + cap2, B.fs, B.ft, seeded, lsteps);
       if ((tid & 63) == 0) B.eps[nocc - 1] = homo;
     } else if (wave == 1 && nocc < n) {
-      const double lumo = -lanczos_top_wave(Lv, n, scale, tri, tri + cap2, B.fs, B.ft, seeded ? 2 : 0, lsteps);
+      const int side = seeded ? 2 : 0;
+      const double lumo = -(n <= 64 ? lanczos_top_wave<1>(Lw, ldw, n, scale, tri, tri + cap2, B.fs, B.ft, side, lsteps)
+                                    : lanczos_top_wave<2>(Lw, ldw, n, scale, tri, tri + cap2, B.fs, B.ft, side, lsteps));
       if ((tid & 63) == 0) B.eps[nocc] = lumo;
     }
     __syncthreads();
@@ -2570,20 +2828,27 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
         const double en = (i == j) ? 0.0 : B.T[i * np + j] - B.T[j * np + i];
         const double fn = B.F[j * np + i];
         const double pn = B.P[j * np + i];
+        const double hn = M.H[j * np + i];
+        // the older entries' values are loaded before this element's stores, so that they do not
+        // wait behind them (the history slots are never the new slot)
+        double eh[MAXD];
+#pragma unroll
+        for (int q = 0; q < MAXD; ++q)
+          if (q < len) eh[q] = B.Eh[(long long)D.slot[q] * ms + e];
         En[e] = en;
         Fn[e] = fn;
         if (keep_d) Dn[e] = pn;
         const double wgt = e < n ? 1.0 : 2.0;
         const double dn = keep_d ? pn : 0.0;
         const double enw = wgt * en, fnw = wgt * fn, dnw = wgt * dn;
-        v[3 * MAXD + 2] += wgt * pn * (M.H[j * np + i] + fn);
+        v[3 * MAXD + 2] += wgt * pn * (hn + fn);
         v[3 * MAXD] += enw * en;
         if (want_mm) v[3 * MAXD + 1] += dnw * fn;
 #pragma unroll
         for (int q = 0; q < MAXD; ++q) {
           if (q < len) {
             const long long so = (long long)D.slot[q] * ms;
-            v[q] += B.Eh[so + e] * enw;
+            v[q] += eh[q] * enw;
             if (want_mm) {
               v[MAXD + q] += dnw * B.Fh[so + e];
               v[2 * MAXD + q] += B.Dh[so + e] * fnw;
@@ -2696,20 +2961,20 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       eigensolve(M, B, Fuse, have_c, lds, red, eig_tol);
       density(M, B.Cn, B.Pn);
     }
+    // the density change and the new density in one pass (block_sum's barriers order the stores
+    // before any later read of P)
     double dv[1] = {0.0};
     for (int e = tid; e < np * np; e += NT) {
-      const double d = B.Pn[e] - B.P[e];
+      const double pn = B.Pn[e], po = B.P[e];
+      const double d = pn - po;
       dv[0] += d * d;
+      B.P[e] = damp > 0.0 ? (1.0 - damp) * pn + damp * po : pn;
+      if (!A.purify) B.C[e] = B.Cn[e];
     }
     block_sum<1>(dv, red);
     const double dp = sqrt(dv[0] / (double)max(n * n, 1));
     const double de = fabs(e_elec - e_old);
     R.last_dp = dp;
-    for (int e = tid; e < np * np; e += NT) {
-      B.P[e] = damp > 0.0 ? (1.0 - damp) * B.Pn[e] + damp * B.P[e] : B.Pn[e];
-      if (!A.purify) B.C[e] = B.Cn[e];
-    }
-    __syncthreads();
     have_c = !A.purify;
     stamp(B, 6);
     e_old = e_elec;

@@ -84,17 +84,18 @@ struct Tuning {
   int pair_block = 256;         // threads per block of the pair and gradient-pair kernels: 64, 128 or 256
   int pair_blocks_per_cu = 8;   // grid of those grid-stride kernels, blocks per CU: 1 to 64
   int sp2_tile16_n = 128;       // purification products on 16 x 16 wave tiles up to this size (AOs)
-  int sp2_lds = 3;              // purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS,
+  int sp2_lds = 5;              // purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS,
                                 // 3 + X in LDS overwritten in place by each product,
-                                // 4 + the leading rows of X in LDS where the whole of X does not fit
+                                // 4 + the leading rows of X in LDS where the whole of X does not fit,
+                                // 5 + X's upper 16 x 16 blocks (tiled, swizzled) in LDS, updated in place
   int sp2_tight_n = 48;         // purification bounds from Lanczos from this size (AOs) on
   int large_syrk = 0;           // host-driven path: purification products by dsyrk (1) or dgemm (0)
   int large_split = 2;          // host-driven path: q >= 2 computes only the q x q block-lower part of each
                                 // product (q batched dgemm rounds) when q divides n; 0 one dgemm
                                 // (2 measured fastest at 3072 AOs: 0.70 ms per product against 0.87)
-  int wg128_nao = 64;           // molecules up to this size (AOs) whose LDS needs fit 36 KiB run on
+  int wg128_nao = 80;           // molecules up to this size (AOs) whose LDS needs fit 36 KiB run on
                                 // 128-thread workgroups, four per CU
-  int wg256_nao = 64;           // then those up to this size fitting 76 KiB on 256-thread workgroups, two
+  int wg256_nao = 96;           // then those up to this size fitting 76 KiB on 256-thread workgroups, two
                                 // per CU; the rest on 512-thread workgroups, one per CU
   int solo_large_nao = 128;     // a batch of a single restricted molecule above this size (AOs) runs on
                                 // the multi-kernel path, which spreads each step over the whole GPU
