@@ -166,6 +166,39 @@ pm3_status pm3_response_fock(const pm3_molecule* mol, const pm3_params* params, 
 pm3_status pm3_numerical_hessian(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
                                  double step, double* hessian);
 
+/* DcOptions (src/dc/partition.rs:40-97): how divide-and-conquer cuts the molecule.  The screened far
+ * field (long_range_cutoff) has no counterpart here: every pair keeps its exact two-electron table. */
+typedef struct {
+  double core_radius;      /* Bohr, default 6.0: target radius of a core region */
+  double buffer_radius;    /* Bohr, default 9.0: how far beyond a core its buffer reaches */
+  int32_t min_core_atoms;  /* default 1: a smaller core joins the nearest existing one */
+  double electron_tol;     /* default 1e-9: bracket width at which the chemical potential is taken */
+  double smearing_ev;      /* default 0.1: Fermi-Dirac broadening of the subsystem occupations (eV) */
+} pm3_dc_options;
+
+/* Scalars of DcResult (src/dc/scf.rs:25-46). */
+typedef struct {
+  double total_ev, electronic_ev, core_ev, heat_of_formation_kcal;
+  double fermi_ev, fermi_beta_ev;  /* global chemical potential of each spin channel (eV) */
+  int32_t iterations, converged, unrestricted;
+  int32_t dropped_pairs, largest_subsystem, n_subsystems, nao;
+} pm3_dc_result;
+
+/* Fills *out with DcOptions::default() (src/dc/partition.rs:85-97). */
+void pm3_dc_options_default(pm3_dc_options* out);
+
+/* run_dc (src/dc/scf.rs:74-160): divide-and-conquer SCF of one molecule on the device -- every
+ * subsystem's Fock block diagonalised in one batched eigensolve per iteration, one global chemical
+ * potential, the Yang-Lee assembly and Pulay DIIS on the density (src/densitydiis.rs:34-119).  The
+ * options read are max_scf, e_tol, p_tol, scf_memory_mb (the density DIIS depth), damping (0: 0.3),
+ * reference, has_field / field and integral_memory_mb.  Optional outputs: density[n*n] (total),
+ * spin_density[n*n] (P_alpha - P_beta, zero-filled for a restricted run), charges[n_atoms].
+ * PM3_SCF_NOT_CONVERGED fills *result and leaves the arrays unwritten, as the reference returns an
+ * error there. */
+pm3_status pm3_run_dc(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                      const pm3_dc_options* dc, pm3_dc_result* result, double* density, double* spin_density,
+                      double* charges);
+
 /* Batched solve of many independent molecules in one persistent launch.  Atoms of molecule k are
  * [atom_ptr[k], atom_ptr[k+1]) of z / xyz_bohr; charge may be NULL (all neutral), multiplicity may
  * be NULL (all singlets).  Per molecule: status[k] (pm3_status), results[k].  want_gradient != 0

@@ -547,6 +547,74 @@ pm3_status pm3_response_fock(const pm3_molecule* mol, const pm3_params* params, 
   return st == 0 ? PM3_OK : fail((pm3_status)st, err);
 }
 
+void pm3_dc_options_default(pm3_dc_options* out) {
+  const DcOptionsHost d;
+  out->core_radius = d.core_radius;
+  out->buffer_radius = d.buffer_radius;
+  out->min_core_atoms = d.min_core_atoms;
+  out->electron_tol = d.electron_tol;
+  out->smearing_ev = d.smearing_ev;
+}
+
+pm3_status pm3_run_dc(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                      const pm3_dc_options* dc, pm3_dc_result* result, double* density, double* spin_density,
+                      double* charges) {
+  g_last_error.clear();
+  if (!params) return fail(PM3_INVALID_INPUT, "null parameter set");
+  pm3_status s = check_molecule(mol);
+  if (s != PM3_OK) return s;
+  EngineOptions eo;
+  std::string why;
+  if (!engine_options(opts, eo, why)) return fail(PM3_INVALID_INPUT, why);
+  DcOptionsHost d;
+  if (dc) {
+    d.core_radius = dc->core_radius;
+    d.buffer_radius = dc->buffer_radius;
+    d.min_core_atoms = dc->min_core_atoms;
+    d.electron_tol = dc->electron_tol;
+    d.smearing_ev = dc->smearing_ev;
+  }
+  if (!std::isfinite(d.core_radius) || !std::isfinite(d.buffer_radius) || !std::isfinite(d.electron_tol) ||
+      !std::isfinite(d.smearing_ev) || d.electron_tol <= 0.0 || d.smearing_ev < 0.0)
+    return fail(PM3_INVALID_INPUT, "divide-and-conquer options must be finite, electron_tol > 0, smearing_ev >= 0");
+  DcResultHost r;
+  std::string err;
+  const int st = dc_run(params->p, MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge, mol->multiplicity}, eo, d,
+                        r, err);
+  if (result && (st == 0 || st == PM3_SCF_NOT_CONVERGED)) {
+    result->total_ev = r.total_ev;
+    result->electronic_ev = r.electronic_ev;
+    result->core_ev = r.core_ev;
+    result->heat_of_formation_kcal = r.hof_kcal;
+    result->fermi_ev = r.fermi_ev;
+    result->fermi_beta_ev = r.fermi_beta_ev;
+    result->iterations = r.iterations;
+    result->converged = r.converged;
+    result->unrestricted = r.unrestricted;
+    result->dropped_pairs = r.dropped_pairs;
+    result->largest_subsystem = r.largest_subsystem;
+    result->n_subsystems = r.n_subsystems;
+    result->nao = r.nao;
+  }
+  if (st == PM3_SCF_NOT_CONVERGED)
+    return fail(PM3_SCF_NOT_CONVERGED, "divide-and-conquer SCF did not converge in " + std::to_string(r.iterations) +
+                                           " iterations");
+  if (st == PM3_MISSING_ELEMENT)
+    return fail(PM3_MISSING_ELEMENT, "missing PM3 parameter block for Z=" + std::to_string(r.missing_z));
+  if (st == PM3_UNSUPPORTED)
+    return fail(PM3_UNSUPPORTED, "this engine covers the s/p elements H, Li-F, Na-Cl; Z=" + std::to_string(r.missing_z) +
+                                     " is not among them");
+  if (st == PM3_RESOURCE_LIMIT)
+    return fail(PM3_RESOURCE_LIMIT, "PM3 two-electron pair cache needs " + std::to_string((long long)r.required_mb) +
+                                        " MiB, above the limit of " + std::to_string((long long)r.limit_mb) + " MiB");
+  if (st != 0) return fail((pm3_status)st, err.empty() ? std::string("divide-and-conquer failed") : err);
+  const size_t n2 = (size_t)r.nao * r.nao;
+  if (density) std::memcpy(density, r.density.data(), n2 * sizeof(double));
+  if (spin_density) std::memcpy(spin_density, r.spin_density.data(), n2 * sizeof(double));
+  if (charges) std::memcpy(charges, r.charges.data(), (size_t)mol->n_atoms * sizeof(double));
+  return PM3_OK;
+}
+
 // ------------------------------------------------------------------------------------------
 // Tuning table and autotuner (no reference counterpart: the reference has no device kernels).
 

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
+#include <memory>
 #include <mutex>
 #include <numeric>
 
@@ -4718,7 +4719,7 @@ struct Batch {
   bool any_uhf = false; // some molecule takes the unrestricted path (the slab then holds a beta part)
   long long mat_total = 0, vec_total = 0, atoms_total = 0;
   KArgs args{};
-  void* bufs[64] = {nullptr};
+  void* bufs[128] = {nullptr};
   int nbufs = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   double last_kernel_ms = 0.0, last_pair_ms = 0.0, last_guess_ms = 0.0;
@@ -5865,7 +5866,7 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
     const bool solo = req.solo_large && !arena && nmol == 1 && !use_uhf && nao > tune.solo_large_nao;
     const bool large = lneed + STATIC_LDS_BYTES > LDS_BYTES_MAX || req.force_large || solo;
     // unrestricted molecules take the single-workgroup path only
-    if (d.nocc > nao || (large && nao > LARGE_MAX_NAO) || (large && use_uhf)) {
+    if (d.nocc > nao || (large && nao > LARGE_MAX_NAO) || (large && use_uhf && !req.dc)) {
       st = d.nocc > nao ? 1 : large && use_uhf ? 4 : 5;
       aslot.resize(d.atom0); aoff.resize(d.atom0); ax.resize(d.atom0); ay.resize(d.atom0); az.resize(d.atom0);
       d.nat = 0;
@@ -6783,6 +6784,9 @@ bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOpt
   batch_destroy(b);
   return ok;
 }
+
+// divide-and-conquer SCF (run_dc) over the host-driven path
+#include "pm3_dc.hpp"
 
 }  // namespace pm3host
 #endif  // PM3_KERNEL_VARIANT

@@ -75,6 +75,24 @@ struct RunRequest {
   bool warm_start = false;   // solves after the first start from each molecule's last density
   bool solo_large = false;   // a batch of one molecule above Tuning::solo_large_nao AOs takes the
                              // multi-kernel path (the caller-facing entry points set it)
+  bool dc = false;           // the divide-and-conquer driver's batch: open shells on the multi-kernel
+                             // path too (it builds spin Fock matrices there)
+};
+
+// DcOptions (src/dc/partition.rs:40-97) without long_range_cutoff.
+struct DcOptionsHost {
+  double core_radius = 6.0, buffer_radius = 9.0;  // Bohr
+  int min_core_atoms = 1;
+  double electron_tol = 1.0e-9, smearing_ev = 0.1;
+};
+
+// DcResult (src/dc/scf.rs:25-46); matrices in the caller's AO order.
+struct DcResultHost {
+  double total_ev = 0, electronic_ev = 0, core_ev = 0, hof_kcal = 0, fermi_ev = 0, fermi_beta_ev = 0;
+  int iterations = 0, converged = 0, unrestricted = 0, dropped_pairs = 0, largest_subsystem = 0, n_subsystems = 0;
+  int nao = 0, missing_z = 0;
+  double required_mb = 0.0, limit_mb = 0.0;
+  std::vector<double> density, spin_density, charges;
 };
 
 // Launch shapes and tile choices.  They change speed only: results agree to rounding whatever
@@ -164,6 +182,12 @@ bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOpt
 // set up once per device.  Returns 0, or a pm3_status code with err set (5 when the molecule's
 // buffers do not fit the block, 6 for a device failure).  Calls on one GPU share the block: they
 // must not overlap.
+// run_dc (src/dc/scf.rs:74-160) of one molecule on the device.  Returns 0, or a pm3_status code
+// with err set (1 invalid input, 2 missing element, 3 not converged: the scalars are then set and
+// the matrices empty, 4 unsupported element, 5 resource limit, 6 device failure).
+int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const DcOptionsHost& dc, DcResultHost& out,
+           std::string& err);
+
 constexpr int NOALLOC_MAX_ATOMS = 128;
 constexpr int NOALLOC_MAX_NAO = 128;
 int engine_run_noalloc(const Params& P, const MolIn& mol, const EngineOptions& opt, bool want_density, MolResult& out,

@@ -117,6 +117,19 @@ class _CResult(ctypes.Structure):
                 ("limit_mb", ctypes.c_double)]
 
 
+class _CDcOptions(ctypes.Structure):
+    _fields_ = [("core_radius", ctypes.c_double), ("buffer_radius", ctypes.c_double),
+                ("min_core_atoms", ctypes.c_int32), ("electron_tol", ctypes.c_double), ("smearing_ev", ctypes.c_double)]
+
+
+class _CDcResult(ctypes.Structure):
+    _fields_ = [("total_ev", ctypes.c_double), ("electronic_ev", ctypes.c_double), ("core_ev", ctypes.c_double),
+                ("heat_of_formation_kcal", ctypes.c_double), ("fermi_ev", ctypes.c_double),
+                ("fermi_beta_ev", ctypes.c_double), ("iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("unrestricted", ctypes.c_int32), ("dropped_pairs", ctypes.c_int32),
+                ("largest_subsystem", ctypes.c_int32), ("n_subsystems", ctypes.c_int32), ("nao", ctypes.c_int32)]
+
+
 class _COptOptions(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int32), ("gtol", ctypes.c_double), ("grad_step", ctypes.c_double),
                 ("history", ctypes.c_int32), ("warm_start", ctypes.c_int32)]
@@ -146,7 +159,7 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune",
             "pm3_batch_large_timing", "pm3_batch_set_positions", "pm3_opt_options_default", "pm3_optimize_batch",
             "pm3_optimize", "pm3_batch_carry_densities", "pm3_batch_spin_densities", "pm3_run_noalloc",
-            "pm3_response_fock", "pm3_numerical_hessian", "pm3_element_mass"]
+            "pm3_response_fock", "pm3_numerical_hessian", "pm3_element_mass", "pm3_dc_options_default", "pm3_run_dc"]
 
 _lib = None
 
@@ -167,6 +180,9 @@ def native():
         L.pm3_gradient.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
                                    ctypes.POINTER(_CResult), dp, dp, dp]
         L.pm3_response_fock.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.c_int32, dp, dp, dp]
+        L.pm3_dc_options_default.argtypes = [ctypes.POINTER(_CDcOptions)]
+        L.pm3_run_dc.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
+                                 ctypes.POINTER(_CDcOptions), ctypes.POINTER(_CDcResult), dp, dp, dp]
         L.pm3_run_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p, ctypes.POINTER(_COptions),
                                     ctypes.c_int32, ip, ctypes.POINTER(_CResult), dp]
         L.pm3_batch_create.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p,
@@ -484,6 +500,69 @@ def run_pm3(molecule: Molecule, params: Optional[Pm3Parameters] = None, options:
                           o(C), _dp(q), _dp(S), o(eps_b), o(C_b))
     _raise(st, res)
     return _result(res, P, eps, C, q, S, eps_b, C_b)
+
+
+@dataclasses.dataclass
+class DcOptions:
+    """DcOptions (src/dc/partition.rs:40-97), lengths in Bohr; long_range_cutoff is not offered."""
+    core_radius: float = 6.0
+    buffer_radius: float = 9.0
+    min_core_atoms: int = 1
+    electron_tol: float = 1.0e-9
+    smearing_ev: float = 0.1
+
+    def _c(self):
+        return _CDcOptions(self.core_radius, self.buffer_radius, self.min_core_atoms, self.electron_tol,
+                           self.smearing_ev)
+
+
+@dataclasses.dataclass
+class DcResult:
+    """DcResult (src/dc/scf.rs:25-46)."""
+    density: np.ndarray
+    spin_density: Optional[np.ndarray]
+    unrestricted: bool
+    electronic_ev: float
+    core_ev: float
+    total_ev: float
+    heat_of_formation_kcal: float
+    charges: np.ndarray
+    fermi_ev: float
+    fermi_beta_ev: float
+    iterations: int
+    converged: bool
+    dropped_pairs: int
+    largest_subsystem: int
+    n_subsystems: int
+
+
+def run_dc(molecule: Molecule, params: Optional[Pm3Parameters] = None, options: Optional[Pm3Options] = None,
+           dc: Optional[DcOptions] = None) -> DcResult:
+    """Divide-and-conquer SCF on the GPU (run_dc, src/dc/scf.rs:74): batched subsystem eigensolves,
+    one global chemical potential, Pulay DIIS on the density.  Raises ScfNotConverged like the
+    reference."""
+    params = params or Pm3Parameters.standard()
+    options = options or Pm3Options()
+    dc = dc or DcOptions()
+    cm = molecule._c()
+    nao = native().pm3_query_nao(ctypes.byref(cm), params._h)
+    res = _CDcResult()
+    null = ctypes.POINTER(ctypes.c_double)()
+    if nao < 0:
+        st = native().pm3_run_dc(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(dc._c()),
+                                 ctypes.byref(res), null, null, null)
+        _raise(st if st else 2)
+    P = np.zeros((nao, nao))
+    S = np.zeros((nao, nao))
+    q = np.zeros(len(molecule.z))
+    st = native().pm3_run_dc(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(dc._c()),
+                             ctypes.byref(res), _dp(P), _dp(S), _dp(q))
+    if st == 3:
+        raise ScfNotConverged(native().pm3_last_error_message().decode(), res.iterations, float("nan"))
+    _raise(st)
+    return DcResult(P, S if res.unrestricted else None, bool(res.unrestricted), res.electronic_ev, res.core_ev,
+                    res.total_ev, res.heat_of_formation_kcal, q, res.fermi_ev, res.fermi_beta_ev, res.iterations,
+                    bool(res.converged), res.dropped_pairs, res.largest_subsystem, res.n_subsystems)
 
 
 def run_pm3_noalloc(molecule: Molecule, params: Optional[Pm3Parameters] = None,
