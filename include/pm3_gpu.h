@@ -199,6 +199,13 @@ pm3_status pm3_run_dc(const pm3_molecule* mol, const pm3_params* params, const p
                       const pm3_dc_options* dc, pm3_dc_result* result, double* density, double* spin_density,
                       double* charges);
 
+/* dc_gradient (src/dc/derivatives.rs:55-121): run_dc, then the fixed-density analytic gradient at the
+ * partitioned density (eV/Bohr, gradient[3*n_atoms]; the spin form for an unrestricted result),
+ * minus f q_A in a field.  The other outputs as pm3_run_dc. */
+pm3_status pm3_dc_gradient(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                           const pm3_dc_options* dc, pm3_dc_result* result, double* gradient, double* density,
+                           double* spin_density, double* charges);
+
 /* Batched solve of many independent molecules in one persistent launch.  Atoms of molecule k are
  * [atom_ptr[k], atom_ptr[k+1]) of z / xyz_bohr; charge may be NULL (all neutral), multiplicity may
  * be NULL (all singlets).  Per molecule: status[k] (pm3_status), results[k].  want_gradient != 0

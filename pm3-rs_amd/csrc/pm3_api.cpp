@@ -556,9 +556,29 @@ void pm3_dc_options_default(pm3_dc_options* out) {
   out->smearing_ev = d.smearing_ev;
 }
 
+static pm3_status run_dc_impl(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                              const pm3_dc_options* dc, pm3_dc_result* result, double* gradient, double* density,
+                              double* spin_density, double* charges);
+
 pm3_status pm3_run_dc(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
                       const pm3_dc_options* dc, pm3_dc_result* result, double* density, double* spin_density,
                       double* charges) {
+  return run_dc_impl(mol, params, opts, dc, result, nullptr, density, spin_density, charges);
+}
+
+pm3_status pm3_dc_gradient(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                           const pm3_dc_options* dc, pm3_dc_result* result, double* gradient, double* density,
+                           double* spin_density, double* charges) {
+  if (!gradient) {
+    g_last_error = "null gradient array";
+    return PM3_INVALID_INPUT;
+  }
+  return run_dc_impl(mol, params, opts, dc, result, gradient, density, spin_density, charges);
+}
+
+static pm3_status run_dc_impl(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                              const pm3_dc_options* dc, pm3_dc_result* result, double* gradient, double* density,
+                              double* spin_density, double* charges) {
   g_last_error.clear();
   if (!params) return fail(PM3_INVALID_INPUT, "null parameter set");
   pm3_status s = check_molecule(mol);
@@ -580,7 +600,7 @@ pm3_status pm3_run_dc(const pm3_molecule* mol, const pm3_params* params, const p
   DcResultHost r;
   std::string err;
   const int st = dc_run(params->p, MolIn{mol->n_atoms, mol->z, mol->xyz_bohr, mol->charge, mol->multiplicity}, eo, d,
-                        r, err);
+                        r, err, gradient != nullptr);
   if (result && (st == 0 || st == PM3_SCF_NOT_CONVERGED)) {
     result->total_ev = r.total_ev;
     result->electronic_ev = r.electronic_ev;
@@ -612,6 +632,7 @@ pm3_status pm3_run_dc(const pm3_molecule* mol, const pm3_params* params, const p
   if (density) std::memcpy(density, r.density.data(), n2 * sizeof(double));
   if (spin_density) std::memcpy(spin_density, r.spin_density.data(), n2 * sizeof(double));
   if (charges) std::memcpy(charges, r.charges.data(), (size_t)mol->n_atoms * sizeof(double));
+  if (gradient) std::memcpy(gradient, r.gradient.data(), 3 * (size_t)mol->n_atoms * sizeof(double));
   return PM3_OK;
 }
 

@@ -131,6 +131,9 @@ __device__ inline int gs() { return gridDim.x * blockDim.x; }
 __global__ __launch_bounds__(THREADS) void add_kernel(const double* a, const double* b, double* out, long long m) {
   for (long long e = gt(); e < m; e += gs()) out[e] = a[e] + b[e];
 }
+__global__ __launch_bounds__(THREADS) void sub_kernel(const double* a, const double* b, double* out, long long m) {
+  for (long long e = gt(); e < m; e += gs()) out[e] = a[e] - b[e];
+}
 // c <- d c + (1 - d) f (damp_sparse, src/dc/pattern.rs:131-136)
 __global__ __launch_bounds__(THREADS) void damp_kernel(double* c, const double* f, long long m, double d) {
   for (long long e = gt(); e < m; e += gs()) c[e] = d * c[e] + (1.0 - d) * f[e];
@@ -323,7 +326,7 @@ static bool dc_fock(Batch& b, const MolDesc& d, const double* P, const double* X
 }
 
 int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const DcOptionsHost& dco, DcResultHost& out,
-           std::string& err) {
+           std::string& err, bool want_gradient) {
   out = DcResultHost();
   // electron counts (src/dc/scf.rs:357-398)
   double ne = -mol.charge;
@@ -361,6 +364,7 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
   RunRequest req;
   req.force_large = true;
   req.dc = true;
+  req.want_gradient = want_gradient;
   Batch* bp = nullptr;
   if (!batch_create(P, {mol}, opt, req, bp, err)) return 6;
   std::unique_ptr<Batch, void (*)(Batch*)> hold(bp, batch_destroy);
@@ -655,6 +659,29 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
     double pop = 0.0;
     for (int k = 0; k < norb[i]; ++k) pop += out.density[(size_t)(off[i] + k) * n + off[i] + k];
     out.charges[i] = P.table.el[P.slot_of_z[mol.z[i]]].tore - pop;
+  }
+  if (want_gradient) {
+    // dc_gradient (src/dc/derivatives.rs:55-121): the fixed-density gradient at the partitioned
+    // density (the spin form for an unrestricted result) by the batched path's pair-gradient
+    // kernels, which read the density (and spin density) from the output slots, minus f q_A in a
+    // field
+    hipLaunchKernelGGL(lg_pack_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)Pt,
+                       b.args.out_P + d.out_mat, n, np);
+    if (uhf) {
+      hipLaunchKernelGGL(dc::sub_kernel, G, T, 0, b.stream, (const double*)Pa, (const double*)Pb, nt, m2);
+      hipLaunchKernelGGL(lg_pack_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)nt,
+                         b.args.out_S + d.out_mat, n, np);
+    }
+    hipLaunchKernelGGL(lg_epilogue_kernel, dim3(1), dim3(NT), 0, b.stream, b.args, d, (const double*)Pt, b.lg.dip);
+    hipLaunchKernelGGL(grad_pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, b.stream, b.args);
+    hipLaunchKernelGGL(grad_gather_wide_kernel, dim3(1, (unsigned)std::min(256, (d.nat + 3) / 4)), dim3(256), 0,
+                       b.stream, b.args, 0);
+    HIP_OK(hipGetLastError());
+    std::vector<double> hg(3 * (size_t)d.nat);
+    HIP_OK(scopy(b.stream, hg.data(), b.args.out_g + 3LL * d.atom0, sizeof(double) * hg.size(), hipMemcpyDeviceToHost));
+    out.gradient.assign(3 * (size_t)mol.n_atoms, 0.0);
+    for (int t = 0; t < d.nat; ++t)
+      for (int c = 0; c < 3; ++c) out.gradient[3 * (size_t)b.atom_map[0][t] + c] = hg[3 * (size_t)t + c];
   }
   return 0;
 }

@@ -5898,6 +5898,7 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
       b.mat_total += (long long)nao * nao;
       b.vec_total += nao;
     }
+    if (large && req.dc) b.any_uhf = b.any_uhf || use_uhf;  // the spin density slot of its gradient
     (large ? b.large : b.solve).push_back(k);
   }
   b.atoms_total = (long long)aslot.size();

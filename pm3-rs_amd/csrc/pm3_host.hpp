@@ -93,6 +93,7 @@ struct DcResultHost {
   int nao = 0, missing_z = 0;
   double required_mb = 0.0, limit_mb = 0.0;
   std::vector<double> density, spin_density, charges;
+  std::vector<double> gradient;  // dc_gradient, eV/Bohr, n_atoms x 3 (when asked for)
 };
 
 // Launch shapes and tile choices.  They change speed only: results agree to rounding whatever
@@ -182,11 +183,12 @@ bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOpt
 // set up once per device.  Returns 0, or a pm3_status code with err set (5 when the molecule's
 // buffers do not fit the block, 6 for a device failure).  Calls on one GPU share the block: they
 // must not overlap.
-// run_dc (src/dc/scf.rs:74-160) of one molecule on the device.  Returns 0, or a pm3_status code
-// with err set (1 invalid input, 2 missing element, 3 not converged: the scalars are then set and
-// the matrices empty, 4 unsupported element, 5 resource limit, 6 device failure).
+// run_dc (src/dc/scf.rs:74-160) of one molecule on the device, with want_gradient also dc_gradient
+// (src/dc/derivatives.rs:55-121).  Returns 0, or a pm3_status code with err set (1 invalid input,
+// 2 missing element, 3 not converged: the scalars are then set and the matrices empty, 4
+// unsupported element, 5 resource limit, 6 device failure).
 int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const DcOptionsHost& dc, DcResultHost& out,
-           std::string& err);
+           std::string& err, bool want_gradient = false);
 
 constexpr int NOALLOC_MAX_ATOMS = 128;
 constexpr int NOALLOC_MAX_NAO = 128;

@@ -159,7 +159,7 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune",
             "pm3_batch_large_timing", "pm3_batch_set_positions", "pm3_opt_options_default", "pm3_optimize_batch",
             "pm3_optimize", "pm3_batch_carry_densities", "pm3_batch_spin_densities", "pm3_run_noalloc",
-            "pm3_response_fock", "pm3_numerical_hessian", "pm3_element_mass", "pm3_dc_options_default", "pm3_run_dc"]
+            "pm3_response_fock", "pm3_numerical_hessian", "pm3_element_mass", "pm3_dc_options_default", "pm3_run_dc", "pm3_dc_gradient"]
 
 _lib = None
 
@@ -183,6 +183,8 @@ def native():
         L.pm3_dc_options_default.argtypes = [ctypes.POINTER(_CDcOptions)]
         L.pm3_run_dc.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
                                  ctypes.POINTER(_CDcOptions), ctypes.POINTER(_CDcResult), dp, dp, dp]
+        L.pm3_dc_gradient.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
+                                      ctypes.POINTER(_CDcOptions), ctypes.POINTER(_CDcResult), dp, dp, dp, dp]
         L.pm3_run_batch.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p, ctypes.POINTER(_COptions),
                                     ctypes.c_int32, ip, ctypes.POINTER(_CResult), dp]
         L.pm3_batch_create.argtypes = [ctypes.c_int32, ip, ip, dp, dp, ip, ctypes.c_void_p,
@@ -563,6 +565,46 @@ def run_dc(molecule: Molecule, params: Optional[Pm3Parameters] = None, options: 
     return DcResult(P, S if res.unrestricted else None, bool(res.unrestricted), res.electronic_ev, res.core_ev,
                     res.total_ev, res.heat_of_formation_kcal, q, res.fermi_ev, res.fermi_beta_ev, res.iterations,
                     bool(res.converged), res.dropped_pairs, res.largest_subsystem, res.n_subsystems)
+
+
+@dataclasses.dataclass
+class DcGradient:
+    """DcGradient (src/dc/derivatives.rs:34-40): the divide-and-conquer result and its gradient."""
+    scf: DcResult
+    energy_ev: float
+    gradient: np.ndarray  # eV/Bohr, n_atoms x 3
+    forces: np.ndarray
+    max_gradient: float
+
+
+def dc_gradient(molecule: Molecule, params: Optional[Pm3Parameters] = None, options: Optional[Pm3Options] = None,
+                dc: Optional[DcOptions] = None) -> DcGradient:
+    """dc_gradient (src/dc/derivatives.rs:55-121) on the GPU: run_dc, then the fixed-density
+    analytic gradient at the partitioned density, minus f q_A in a field."""
+    params = params or Pm3Parameters.standard()
+    options = options or Pm3Options()
+    dc = dc or DcOptions()
+    cm = molecule._c()
+    nao = native().pm3_query_nao(ctypes.byref(cm), params._h)
+    res = _CDcResult()
+    g = np.zeros((len(molecule.z), 3))
+    if nao < 0:
+        null = ctypes.POINTER(ctypes.c_double)()
+        st = native().pm3_dc_gradient(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(dc._c()),
+                                      ctypes.byref(res), _dp(g), null, null, null)
+        _raise(st if st else 2)
+    P = np.zeros((nao, nao))
+    S = np.zeros((nao, nao))
+    q = np.zeros(len(molecule.z))
+    st = native().pm3_dc_gradient(ctypes.byref(cm), params._h, ctypes.byref(options._c()), ctypes.byref(dc._c()),
+                                  ctypes.byref(res), _dp(g), _dp(P), _dp(S), _dp(q))
+    if st == 3:
+        raise ScfNotConverged(native().pm3_last_error_message().decode(), res.iterations, float("nan"))
+    _raise(st)
+    scf = DcResult(P, S if res.unrestricted else None, bool(res.unrestricted), res.electronic_ev, res.core_ev,
+                   res.total_ev, res.heat_of_formation_kcal, q, res.fermi_ev, res.fermi_beta_ev, res.iterations,
+                   bool(res.converged), res.dropped_pairs, res.largest_subsystem, res.n_subsystems)
+    return DcGradient(scf, res.total_ev, g, -g, float(np.abs(g).max()) if g.size else 0.0)
 
 
 def run_pm3_noalloc(molecule: Molecule, params: Optional[Pm3Parameters] = None,

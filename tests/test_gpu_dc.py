@@ -111,6 +111,27 @@ def test_full_coverage_reproduces_run_pm3(G):
     assert np.abs(g.density - full.density).max() < 1e-5
 
 
+@pytest.mark.parametrize("charge,mult,field", [(0.0, 1, None), (1.0, 2, None), (0.0, 1, (0.01, -0.02, 0.005))])
+def test_dc_gradient_matches_the_oracle(G, oracle, charge, mult, field):
+    """dc_gradient (src/dc/derivatives.rs:55-121): the fixed-density gradient at the partitioned
+    density, the spin form for an open shell, minus f q_A in a field, to 2.295e-7 eV/Bohr."""
+    z, x = water_chain(5)
+    dc = dict(core_radius=3.0, buffer_radius=6.0)
+    o = dict(TIGHT, field=field) if field else dict(TIGHT)
+    g = G.dc_gradient(G.Molecule(z, x, charge, mult), options=G.Pm3Options(**o), dc=G.DcOptions(**dc))
+    oo = oracle.options(**TIGHT)
+    if field:
+        oo = oracle.options(**TIGHT, field=field)
+    st, r = oracle.dc_gradient(z, x, charge, mult, opts=oo, dc=oracle.dc_options(**dc))
+    assert st == 0
+    dg = float(np.abs(g.gradient - r["gradient"]).max())
+    print(f"dc_gradient q={charge} field={field}: max|dg| {dg:.2e} eV/Bohr, |dHf| "
+          f"{abs(g.scf.heat_of_formation_kcal - r['hof_kcal']):.2e}", flush=True)
+    assert dg <= 2.295e-7
+    assert abs(g.scf.heat_of_formation_kcal - r["hof_kcal"]) <= HOF_TOL
+    assert abs(g.gradient.sum(axis=0)).max() < 1e-6 or field
+
+
 def test_refusals(G):
     z, x = water_chain(2)
     m = G.Molecule(z, x)
