@@ -48,11 +48,11 @@ from pm3amd import workloads as W  # noqa: E402
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
 NET_FLOP_PER_PAIR = 2.0 * (96 * 128 + 128 * 128 + 128 * 16)  # forward pass per directed pair
-# HBM bytes per SCF launch (the 512- and 128-thread kernels of one solve, summed) measured with
-# rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a 512 MB copy: x2.0 / x1.0) on the default workload,
-# per leg.  A recorded value, not measured by this run: the JSON line carries it with its source
-# (the profile summary and the build it was taken on).
-MEASURED_TRAFFIC = {("mixed4096", "learned"): 1.288e11, ("mixed4096", "plain"): 1.769e11}
+# HBM bytes per SCF launch (the 512-, 256- and 128-thread kernels of one solve, summed) measured
+# with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a 512 MB copy: x2.0 / x1.0) on the default
+# workload, per leg.  A recorded value, not measured by this run: the JSON line carries it with its
+# source (the profile summary and the build it was taken on).
+MEASURED_TRAFFIC = {("mixed4096", "learned"): 9.70e10, ("mixed4096", "plain"): 1.349e11}
 TRAFFIC_SOURCE = ("recorded, not measured in this run: profiles/scf_batch_kernel_traffic_mixed4096.md "
                   "(rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of bench.py --steps 1)")
 HBM_PEAK_GBS = 8000.0
