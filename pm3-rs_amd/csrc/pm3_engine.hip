@@ -2717,6 +2717,191 @@ __device__ inline double dot_nn(const double* X, const double* Y, int np, double
   return v[0];
 }
 
+// The DIIS history pass of rhf_loop: the new entry in slot D.newslot, its energy, and the Gram
+// and A-DIIS entries it adds, with D's bookkeeping (length, mode, error norm) on thread 0; returns
+// the electronic energy P.(H + F) / 2 on every thread.  A function of its own so that its sums
+// and loads do not compete for registers with the SCF loop's state.
+__device__ __noinline__ double diis_history_pass(const KArgs& A, const Mol& M, const Bufs& B, Diis& D, int accel,
+                                                  bool keep_d, double* red) {
+  const int n = M.n, np = M.np, tid = threadIdx.x;
+  const int ns = D.newslot;
+  const long long ms = (long long)np * np;
+  double* Fn = B.Fh + ns * ms;
+  double* En = B.Eh + ns * ms;
+  double* Dn = B.Dh + ns * ms;
+  // One pass over the packed triangle: the new history entry (F and D symmetric, E
+  // antisymmetric with zero diagonal; diagonal weight 1, off-diagonal weight 2 in the inner
+  // products), the energy P.(H + F) / 2, and the Gram updates <E_i,E_new>, <D_new,F_j>,
+  // <D_i,F_new> for the existing entries plus <E_new,E_new>, <D_new,F_new> (index len).  The
+  // <D,F> entries feed only A-DIIS: they are skipped while the previous error is well below
+  // the A-DIIS switch, and recomputed in full should A-DIIS be needed later.
+  const int npk = n * (n + 1) / 2;
+  const int len = D.len;
+  const bool want_mm = keep_d && accel == 2 && !D.mm_stale && D.en > 0.25 * A.adiis_switch;
+  double v[3 * MAXD + 3];
+#pragma unroll
+  for (int k = 0; k < 3 * MAXD + 3; ++k) v[k] = 0.0;
+  // one packed element: the new entry's values and the older entries' E values are loaded
+  // before this element's stores (the history slots are never the new slot)
+  struct El {
+    double en, fn, pn, hn;
+  };
+  auto elem = [&](int e, El& x) {
+    int i, j;
+    tri_unpack(e, n, i, j);  // i <= j; consecutive e run down column j: read (j, i) of symmetric F, P, H
+    x.en = (i == j) ? 0.0 : B.T[i * np + j] - B.T[j * np + i];
+    x.fn = B.F[j * np + i];
+    x.pn = B.P[j * np + i];
+    x.hn = M.H[j * np + i];
+  };
+  auto put = [&](int e, const El& x) {
+    En[e] = x.en;
+    Fn[e] = x.fn;
+    if (keep_d) Dn[e] = x.pn;
+  };
+  if (want_mm) {  // A-DIIS may be needed: the <D, F> entries as well
+    for (int e = tid; e < npk; e += NT) {
+      El x;
+      elem(e, x);
+      double eh[MAXD];
+#pragma unroll
+      for (int q = 0; q < MAXD; ++q)
+        if (q < len) eh[q] = B.Eh[(long long)D.slot[q] * ms + e];
+      put(e, x);
+      const double wgt = e < n ? 1.0 : 2.0;
+      const double dn = keep_d ? x.pn : 0.0;
+      const double enw = wgt * x.en, fnw = wgt * x.fn, dnw = wgt * dn;
+      v[3 * MAXD + 2] += wgt * x.pn * (x.hn + x.fn);
+      v[3 * MAXD] += enw * x.en;
+      v[3 * MAXD + 1] += dnw * x.fn;
+#pragma unroll
+      for (int q = 0; q < MAXD; ++q) {
+        if (q < len) {
+          const long long so = (long long)D.slot[q] * ms;
+          v[q] += eh[q] * enw;
+          v[MAXD + q] += dnw * B.Fh[so + e];
+          v[2 * MAXD + q] += B.Dh[so + e] * fnw;
+        }
+      }
+    }
+    block_sum<3 * MAXD + 3>(v, red);
+  } else {
+    // the common case, E history only: ten sums (E dots, E.E, energy), which leaves the
+    // registers for two elements' loads in flight at once; each sum takes the same terms in the
+    // same order as above, so the results are the same bits
+    double u[MAXD + 2];
+#pragma unroll
+    for (int k = 0; k < MAXD + 2; ++k) u[k] = 0.0;
+    auto acc = [&](int e, const El& x, const double (&eh)[MAXD]) {
+      const double wgt = e < n ? 1.0 : 2.0;
+      const double enw = wgt * x.en;
+      u[MAXD + 1] += wgt * x.pn * (x.hn + x.fn);
+      u[MAXD] += enw * x.en;
+#pragma unroll
+      for (int q = 0; q < MAXD; ++q)
+        if (q < len) u[q] += eh[q] * enw;
+    };
+    int e = tid;
+    for (; e + NT < npk; e += 2 * NT) {
+      El x0, x1;
+      elem(e, x0);
+      elem(e + NT, x1);
+      double eh0[MAXD], eh1[MAXD];
+#pragma unroll
+      for (int q = 0; q < MAXD; ++q)
+        if (q < len) {
+          const long long so = (long long)D.slot[q] * ms;
+          eh0[q] = B.Eh[so + e];
+          eh1[q] = B.Eh[so + e + NT];
+        }
+      put(e, x0);
+      put(e + NT, x1);
+      acc(e, x0, eh0);
+      acc(e + NT, x1, eh1);
+    }
+    if (e < npk) {
+      El x0;
+      elem(e, x0);
+      double eh0[MAXD];
+#pragma unroll
+      for (int q = 0; q < MAXD; ++q)
+        if (q < len) eh0[q] = B.Eh[(long long)D.slot[q] * ms + e];
+      put(e, x0);
+      acc(e, x0, eh0);
+    }
+    block_sum<MAXD + 2>(u, red);
+#pragma unroll
+    for (int q = 0; q < MAXD; ++q) v[q] = u[q];
+    v[3 * MAXD] = u[MAXD];
+    v[3 * MAXD + 2] = u[MAXD + 1];
+  }
+  const double e_elec = 0.5 * v[3 * MAXD + 2];
+  stamp(B, 2);
+  if (tid == 0) {
+    for (int i = 0; i < len; ++i) {
+      D.b[i][len] = v[i];
+      D.b[len][i] = v[i];
+      D.mm[len][i] = v[MAXD + i];
+      D.mm[i][len] = v[2 * MAXD + i];
+    }
+    D.b[len][len] = v[3 * MAXD];
+    D.mm[len][len] = v[3 * MAXD + 1];
+    D.slot[len] = ns;
+    D.len = len + 1;
+    if (keep_d && accel == 2 && !want_mm) D.mm_stale = 1;
+    const double en = sqrt(v[3 * MAXD]);
+    D.en = en;
+    if (accel == 2 && en > A.adiis_switch) {
+      D.mode = (D.len >= 2 && D.keep_d) ? 2 : 0;
+      D.ok = D.mode == 2;
+    } else {
+      D.mode = 1;
+    }
+  }
+  return e_elec;
+}
+
+// sum_i c_i F_i over the kept history into both triangles of Fu (rhf_loop's extrapolation).
+__device__ __noinline__ void diis_extrapolate(const Mol& M, const Bufs& B, const Diis& D) {
+  const int n = M.n, np = M.np, tid = threadIdx.x;
+  const long long ms = (long long)np * np;
+  const int npk = n * (n + 1) / 2;
+  const int L = D.len;
+  for (int e = tid; e < npk; e += NT) {
+    double s = 0.0;
+    for (int i = 0; i < L; ++i) s += D.coef[i] * B.Fh[(long long)D.slot[i] * ms + e];
+    int i, j;
+    tri_unpack(e, n, i, j);
+    B.Fu[i * np + j] = s;
+    B.Fu[j * np + i] = s;
+  }
+  __syncthreads();
+}
+
+// T = F P on the matrix cores (the DIIS error's product), then a barrier.
+__device__ __noinline__ void fp_product(const Mol& M, const Bufs& B) {
+  const int n = M.n, np = M.np;
+  wg_gemm_pad(n, np, B.F, B.P, [&](int i, int j, double v) { B.T[i * np + j] = v; });
+  __syncthreads();
+}
+
+// The density change of an SCF step and the new density in one pass: returns the RMS change of
+// Pn against P, then P <- Pn (damped: (1 - damp) Pn + damp P), and C <- Cn on the eigenvector
+// route; block_sum's barriers order the stores before any later read of P.
+__device__ __noinline__ double density_step(const KArgs& A, const Mol& M, const Bufs& B, double damp, double* red) {
+  const int n = M.n, np = M.np, tid = threadIdx.x;
+  double dv[1] = {0.0};
+  for (int e = tid; e < np * np; e += NT) {
+    const double pn = B.Pn[e], po = B.P[e];
+    const double d = pn - po;
+    dv[0] += d * d;
+    B.P[e] = damp > 0.0 ? (1.0 - damp) * pn + damp * po : pn;
+    if (!A.purify) B.C[e] = B.Cn[e];
+  }
+  block_sum<1>(dv, red);
+  return sqrt(dv[0] / (double)max(n * n, 1));
+}
+
 struct PathResult {
   double e_elec, last_dp;
   int iters, converged, guess_used;
@@ -2737,7 +2922,7 @@ struct PathResult {
 // the two spin RMS changes (= the RMS change of P), and damping P <- (1 - damp) P_new + damp P
 // (:1439-1448).  The undamped aufbau projector of the last step stays in B.Pn for the frontier
 // levels; their eigenvalues are those of the shifted matrix, as the reference reports them.
-__device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int guess, int accel, double* lds,
+__device__ __noinline__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int guess, int accel, double* lds,
                                double* red, Diis& D, bool tight = true, bool warm = false, double shift = 0.0,
                                double damp = 0.0, bool urung = false) {
   const int n = M.n, np = M.np, tid = threadIdx.x;
@@ -2808,79 +2993,10 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       double* Dn = B.Dh + ns * ms;
       // E = F P - P F (src/scf.rs:1039-1054) via T = F P
       stamp(B, 2);
-      wg_gemm_pad(n, np, B.F, B.P, [&](int i, int j, double v) { B.T[i * np + j] = v; });
-      __syncthreads();
+      fp_product(M, B);
       stamp(B, 3);
-      // One pass over the packed triangle: the new history entry (F and D symmetric, E
-      // antisymmetric with zero diagonal; diagonal weight 1, off-diagonal weight 2 in the inner
-      // products), the energy P.(H + F) / 2, and the Gram updates <E_i,E_new>, <D_new,F_j>,
-      // <D_i,F_new> for the existing entries plus <E_new,E_new>, <D_new,F_new> (index len).  The
-      // <D,F> entries feed only A-DIIS: they are skipped while the previous error is well below
-      // the A-DIIS switch, and recomputed in full should A-DIIS be needed later.
       const int npk = n * (n + 1) / 2;
-      const int len = D.len;
-      const bool want_mm = keep_d && accel == 2 && !D.mm_stale && D.en > 0.25 * A.adiis_switch;
-      double v[3 * MAXD + 3];
-#pragma unroll
-      for (int k = 0; k < 3 * MAXD + 3; ++k) v[k] = 0.0;
-      for (int e = tid; e < npk; e += NT) {
-        int i, j;
-        tri_unpack(e, n, i, j);  // i <= j; consecutive e run down column j: read (j, i) of symmetric F, P, H
-        const double en = (i == j) ? 0.0 : B.T[i * np + j] - B.T[j * np + i];
-        const double fn = B.F[j * np + i];
-        const double pn = B.P[j * np + i];
-        const double hn = M.H[j * np + i];
-        // the older entries' values are loaded before this element's stores, so that they do not
-        // wait behind them (the history slots are never the new slot)
-        double eh[MAXD];
-#pragma unroll
-        for (int q = 0; q < MAXD; ++q)
-          if (q < len) eh[q] = B.Eh[(long long)D.slot[q] * ms + e];
-        En[e] = en;
-        Fn[e] = fn;
-        if (keep_d) Dn[e] = pn;
-        const double wgt = e < n ? 1.0 : 2.0;
-        const double dn = keep_d ? pn : 0.0;
-        const double enw = wgt * en, fnw = wgt * fn, dnw = wgt * dn;
-        v[3 * MAXD + 2] += wgt * pn * (hn + fn);
-        v[3 * MAXD] += enw * en;
-        if (want_mm) v[3 * MAXD + 1] += dnw * fn;
-#pragma unroll
-        for (int q = 0; q < MAXD; ++q) {
-          if (q < len) {
-            const long long so = (long long)D.slot[q] * ms;
-            v[q] += eh[q] * enw;
-            if (want_mm) {
-              v[MAXD + q] += dnw * B.Fh[so + e];
-              v[2 * MAXD + q] += B.Dh[so + e] * fnw;
-            }
-          }
-        }
-      }
-      block_sum<3 * MAXD + 3>(v, red);
-      e_elec = 0.5 * v[3 * MAXD + 2];
-      stamp(B, 2);
-      if (tid == 0) {
-        for (int i = 0; i < len; ++i) {
-          D.b[i][len] = v[i];
-          D.b[len][i] = v[i];
-          D.mm[len][i] = v[MAXD + i];
-          D.mm[i][len] = v[2 * MAXD + i];
-        }
-        D.b[len][len] = v[3 * MAXD];
-        D.mm[len][len] = v[3 * MAXD + 1];
-        D.slot[len] = ns;
-        D.len = len + 1;
-        if (keep_d && accel == 2 && !want_mm) D.mm_stale = 1;
-        const double en = sqrt(v[3 * MAXD]);
-        D.en = en;
-        if (accel == 2 && en > A.adiis_switch) {
-          D.mode = (D.len >= 2 && D.keep_d) ? 2 : 0;
-          D.ok = D.mode == 2;
-        } else {
-          D.mode = 1;
-        }
-      }
+      e_elec = diis_history_pass(A, M, B, D, accel, keep_d, red);
       __syncthreads();
       if (D.mode == 2 && D.mm_stale) {
         // A-DIIS after skipped steps: every <D_i, F_j> again, summed as in the pass above
@@ -2917,16 +3033,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       }
       stamp(B, 8);
       if (D.ok) {
-        const int L = D.len;
-        for (int e = tid; e < npk; e += NT) {
-          double s = 0.0;
-          for (int i = 0; i < L; ++i) s += D.coef[i] * B.Fh[(long long)D.slot[i] * ms + e];
-          int i, j;
-          tri_unpack(e, n, i, j);
-          B.Fu[i * np + j] = s;
-          B.Fu[j * np + i] = s;
-        }
-        __syncthreads();
+        diis_extrapolate(M, B, D);
         Fuse = B.Fu;
       } else {
         Fuse = B.F;  // the newest entry's Fock matrix
@@ -2962,18 +3069,7 @@ __device__ PathResult rhf_loop(const KArgs& A, const Mol& M, const Bufs& B, int 
       eigensolve(M, B, Fuse, have_c, lds, red, eig_tol);
       density(M, B.Cn, B.Pn);
     }
-    // the density change and the new density in one pass (block_sum's barriers order the stores
-    // before any later read of P)
-    double dv[1] = {0.0};
-    for (int e = tid; e < np * np; e += NT) {
-      const double pn = B.Pn[e], po = B.P[e];
-      const double d = pn - po;
-      dv[0] += d * d;
-      B.P[e] = damp > 0.0 ? (1.0 - damp) * pn + damp * po : pn;
-      if (!A.purify) B.C[e] = B.Cn[e];
-    }
-    block_sum<1>(dv, red);
-    const double dp = sqrt(dv[0] / (double)max(n * n, 1));
+    const double dp = density_step(A, M, B, damp, red);
     const double de = fabs(e_elec - e_old);
     R.last_dp = dp;
     have_c = !A.purify;
