@@ -310,8 +310,10 @@ typedef struct {
                                  5 + X's upper 16x16 blocks (tiled, swizzled) in LDS, updated in place */
   int32_t sp2_tight_n;        /* purification spectral bounds from Lanczos from this many AOs on */
   int32_t large_syrk;         /* multi-kernel path: purification products by dsyrk (1) or dgemm (0) */
-  int32_t large_split;        /* multi-kernel path: q in 2..4 computes only the q x q block-lower part of
-                                 each product (q batched dgemm rounds) when q divides the AO count */
+  int32_t large_split;        /* multi-kernel path above 1024 AOs: 1 purification steps by the library's
+                                 lower-triangle tile kernel (product and update in one launch); q in
+                                 2..4 only the q x q block-lower part of each product by q batched
+                                 rocBLAS dgemm rounds when q divides the AO count; 0 one dgemm */
   int32_t wg128_nao;          /* SCF workgroup width: molecules up to this many AOs (and LDS needs within
                                  36 KiB) on 128-thread workgroups, four per CU */
   int32_t wg256_nao;          /* then up to this many AOs (within 76 KiB) on 256-thread workgroups, two per

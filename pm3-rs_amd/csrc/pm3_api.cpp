@@ -776,7 +776,7 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
       {&Tuning::sp2_lds, {0, 1, 2, 3, 4, 5}, 1},
       {&Tuning::sp2_tight_n, {24, 32, 48, 64, 96, 1 << 20}, 1},
       {&Tuning::large_syrk, {0, 1}, 2},
-      {&Tuning::large_split, {0, 2, 3, 4}, 2},
+      {&Tuning::large_split, {0, 1, 2, 3, 4}, 2},
       {&Tuning::wg128_nao, {0, 32, 48, 64, 80, 96}, 1},
       {&Tuning::wg256_nao, {0, 64, 80, 96, 112, 128}, 1},
       {&Tuning::solo_large_nao, {96, 128, 160, 192, 1 << 20}, 0},
@@ -953,6 +953,17 @@ pm3_status pm3_stage_gemm(int32_t n, const double* a, const double* b, double* c
   pm3_status s = generic_stage(n, 3, in, out);
   if (s != PM3_OK) return s;
   std::memcpy(c, out.data(), (size_t)n * n * sizeof(double));
+  return PM3_OK;
+}
+
+pm3_status pm3_stage_sym_step(int32_t n, int32_t square, int32_t slab, const double* x, double* y, double* trace,
+                              double* norm2, int32_t reps, double* ms_per_launch) {
+  g_last_error.clear();
+  if (!x || !y) return fail(PM3_INVALID_INPUT, "null argument");
+  std::string err;
+  const int r = sym_step_stage(n, square, slab, x, y, trace, norm2, reps, ms_per_launch, err);
+  if (r == 2) return fail(PM3_INVALID_INPUT, err);
+  if (r != 0) return fail(PM3_DEVICE_ERROR, err);
   return PM3_OK;
 }
 

@@ -4196,14 +4196,17 @@ __device__ inline void sp2_decide(double* st, double tr, double t2, double occ, 
 // After X0 (init) or after a step: tr and tr X^2 from the block partials (the halving tree of
 // lg_finish_kernel), the bookkeeping of the step just taken, then the decisions at the top of the
 // next step of sp2_density's loop (bound check, tail start with the frontier seed, stop, X^2 or
-// 2X - X^2).  A finished state is left as it is.
+// 2X - X^2).  A finished state is left as it is.  Row r of the partials starts at red[r * rstride];
+// thread t adds rows t, t + LG_BLOCKS, ... in that order before the tree.
 __global__ __launch_bounds__(LG_BLOCKS) void lg_sp2_state_kernel(const double* red, double* st, double occ, int init,
-                                                                 int want_seed, int nrows) {
+                                                                 int want_seed, int nrows, int rstride) {
   __shared__ double t[LG_BLOCKS];
   if (!init && st[SP2S_STATUS] != 0.0) return;  // the same value for every thread of the block
   double v[2];
   for (int q = 0; q < 2; ++q) {
-    t[threadIdx.x] = (int)threadIdx.x < nrows ? red[(size_t)threadIdx.x * LG_MAXDOT + q] : 0.0;
+    double a = 0.0;
+    for (int r = threadIdx.x; r < nrows; r += LG_BLOCKS) a += red[(size_t)r * rstride + q];
+    t[threadIdx.x] = a;
     __syncthreads();
     for (int h = LG_BLOCKS / 2; h > 0; h >>= 1) {
       if ((int)threadIdx.x < h) t[threadIdx.x] += t[threadIdx.x + h];
@@ -4309,6 +4312,159 @@ __global__ __launch_bounds__(64) void lg_sp2_fused_kernel(const double* X, doubl
   if (lane == 0) {
     red[(size_t)blockIdx.x * LG_MAXDOT] = tr;
     red[(size_t)blockIdx.x * LG_MAXDOT + 1] = fr;
+  }
+}
+
+// One purification step of a large matrix on the matrix cores, product and update in one launch:
+// Xn = X X, or 2 X - X X, as the state chose.  X is symmetric, so tile (I, J) of X X is the product
+// of X's row panels I and J (X[I,:] X[J,:]^T), and only the tiles of the lower triangle (I >= J)
+// are computed: about half the flops of the whole square.  A workgroup of four waves computes a
+// 64 x 64 tile, each wave a 32 x 32 quarter as 2 x 2 MFMA blocks.  32-column slabs of both panels
+// pass through LDS, double-buffered: a slab's global loads are issued before the MFMAs of the
+// previous one and stored after them.  In LDS a panel row is 16 slots of 16 bytes, slot c of row r
+// at c ^ (r % 16), so the row-strided fragment reads and the slab stores meet no bank conflict.
+// The k index of the 16 x 16 x 4 MFMA is permuted inside a slab: lane group g = lane / 16 takes
+// columns 8 g .. 8 g + 7, two per 16-byte read, in both operands, so each column enters once.  The
+// store writes each element of the lower triangle and its mirror, so Xn is exactly symmetric.
+// Loads beyond n read as zero, so padding rows and columns are never read.  Workgroup w computes
+// one tile, map[x * chunk + w / 8] with x = w % 8: workgroups go to the eight XCDs in turn, so each
+// XCD takes a contiguous run of the map, which lists the tiles by 8 x 8 super-blocks; the tiles an
+// XCD has in flight then share few panels, and its L2 serves the rest.  Workgroup w leaves the
+// trace and norm partials of its tile in red[2 w], red[2 w + 1] (zeros when it has none).
+constexpr int SY_T = 64, SY_THREADS = 256;
+
+// Slot of 16-byte column chunk c of panel row `row` in a slab of K columns (K / 2 chunks a row):
+// the chunk index is XORed with the row's place among the rows that share a 256-byte span of banks,
+// so 16 consecutive rows at one chunk hit 16 different bank groups.
+template <int K> __device__ __forceinline__ int sy_slot(int row, int c) {
+  constexpr int CH = K / 2, RW = 16 / CH;
+  return row * CH + (c ^ ((row / RW) % CH));
+}
+
+__host__ __device__ inline int sy_tiles(int n) {
+  const int nt = (n + SY_T - 1) / SY_T;
+  return nt * (nt + 1) / 2;
+}
+
+template <int K>
+__global__ __launch_bounds__(SY_THREADS) void lg_sp2_sym_kernel(const double* X, double* Xn, int n, int np,
+                                                               const double* st, double* red, const int* map,
+                                                               int chunk) {
+  static_assert(K == 16 || K == 32, "slab width");
+  constexpr int CH = K / 2, LQ = SY_T * CH / SY_THREADS, LC = CH / 4;  // chunks: a row, a thread's loads, a lane's
+  if (st[SP2S_STATUS] != 0.0) return;
+  __shared__ double2 slab[2][2][SY_T * CH];  // [buffer][panel I, panel J][slot]
+  const bool sq = st[SP2S_SQ] != 0.0;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;  // the wave's quarter of the tile
+  const int ntile = sy_tiles(n), nk = (n + K - 1) / K;
+  const int slot = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+  double tr = 0.0, fr = 0.0;
+  if ((int)(blockIdx.x >> 3) < chunk && slot < ntile) {
+    const int I = map[slot] >> 16, J = map[slot] & 0xffff;
+    const bool diag = I == J;  // one panel serves both operands
+    const int np_ = diag ? 1 : 2;
+    const int r0[2] = {I * SY_T, J * SY_T};
+    // chunk q * 256 + tid of a slab: panel row id / CH, 16-byte column chunk id % CH.  The loads
+    // are unconditional (at a clamped address inside X's n x n part), so that all of a slab's
+    // loads are in flight over the previous slab's MFMAs; values beyond n are zeroed as they are
+    // stored to LDS.
+    double2 g[2][LQ];
+    const int kmax = (n - 1) & ~1;  // the last even column: kmax + 1 < np
+    auto fetch = [&](int k0) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        if (p >= np_) break;
+#pragma unroll
+        for (int q = 0; q < LQ; ++q) {
+          const int id = q * SY_THREADS + tid, row = r0[p] + id / CH, k = k0 + 2 * (id % CH);
+          g[p][q] = *(const double2*)(X + (size_t)min(row, n - 1) * np + min(k, kmax));
+        }
+      }
+    };
+    auto put = [&](int buf, int k0) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        if (p >= np_) break;
+#pragma unroll
+        for (int q = 0; q < LQ; ++q) {
+          const int id = q * SY_THREADS + tid, row = r0[p] + id / CH, k = k0 + 2 * (id % CH);
+          double2 v = g[p][q];
+          const bool in = row < n && k < n;
+          v.x = in ? v.x : 0.0;
+          v.y = in && k + 1 < n ? v.y : 0.0;
+          slab[buf][p][sy_slot<K>(id / CH, id % CH)] = v;
+        }
+      }
+    };
+    double4_t acc[2][2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 2; ++v) acc[u][v] = double4_t{0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    put(0, 0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int buf = kt & 1;
+      if (kt + 1 < nk) fetch((kt + 1) * K);
+      const double2* A = slab[buf][0];
+      const double2* B = slab[buf][np_ - 1];
+#pragma unroll
+      for (int s2 = 0; s2 < LC; ++s2) {
+        const int c = lg * LC + s2;
+        double2 a[2], b[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          a[u] = A[sy_slot<K>(wr + 16 * u + li, c)];
+          b[u] = B[sy_slot<K>(wc + 16 * u + li, c)];
+        }
+        // A[i][k] = X[I row i][k], B[k][j] = X[J row j][k]: lane l holds A[l % 16][l / 16], B[l / 16][l % 16]
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 2; ++v)
+              acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(h ? a[u].y : a[u].x, h ? b[v].y : b[v].x, acc[u][v], 0,
+                                                               0, 0);
+      }
+      if (kt + 1 < nk) put(buf ^ 1, (kt + 1) * K);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int v = 0; v < 2; ++v)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {  // item r: row 4 r + l / 16, column l % 16 of the block
+          const int i = I * SY_T + wr + 16 * u + 4 * r + lg, j = J * SY_T + wc + 16 * v + li;
+          if (i < n && j < n && (!diag || i >= j)) {
+            const double y = sq ? acc[u][v][r] : 2.0 * X[(size_t)i * np + j] - acc[u][v][r];
+            Xn[(size_t)i * np + j] = y;
+            if (i != j) Xn[(size_t)j * np + i] = y;
+            fr += (i == j ? 1.0 : 2.0) * y * y;
+            if (i == j) tr += y;
+          }
+        }
+  }
+  tr = wave_sum(tr);
+  fr = wave_sum(fr);
+  double* part = (double*)&slab[0][0][0];  // every slab read has passed the loop's last barrier
+  if (lane == 0) {
+    part[2 * wv] = tr;
+    part[2 * wv + 1] = fr;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0.0, c = 0.0;
+    for (int q = 0; q < SY_THREADS / 64; ++q) {
+      a += part[2 * q];
+      c += part[2 * q + 1];
+    }
+    red[2 * (size_t)blockIdx.x] = a;
+    red[2 * (size_t)blockIdx.x + 1] = c;
   }
 }
 
@@ -4772,6 +4928,9 @@ struct LargeBufs {
   int npev = 0, nfev = 0;
   double product_ms = 0.0, fock_ms = 0.0;
   long long products = 0, fock_builds = 0, product_flop = 0;
+  int sp2_steps = 8;  // steps of the last converged purification (when the tile kernel first looks)
+  int* sy_map = nullptr;  // the tile kernel's tile list (I << 16 | J) for sy_map_n AOs
+  int sy_map_n = 0, sy_map_cap = 0;
   long long lanczos_steps = 0, lanczos_runs = 0;  // frontier-level Lanczos of the last solve
 };
 
@@ -4822,6 +4981,8 @@ struct Batch {
   int pair_blocks = 1, pair_block = 256;
   bool large_syrk = false;
   int large_split = 0;
+  int large_steps_n = 0;  // the fused purification kernel up to this many AOs
+  int sy_k = 16;          // slab width of the tile kernel (16 or 32 columns)
   // learned guess: directed pairs of the eligible molecules and their network inputs/outputs
   const GuessModel* guess_model = nullptr;
   bool learned = false, solved_once = false;
@@ -5203,6 +5364,43 @@ static bool large_purify(Batch& b, const MolDesc& d, const double* A, double* Pd
   return large_purify_run(b, d, A, Pdst, glo, ghi, seed, ok, err);
 }
 
+// The tile list of lg_sp2_sym_kernel for n AOs: the lower-triangle 64 x 64 tiles (I << 16 | J),
+// grouped by 8 x 8 super-blocks taken row by row, each super-block's tiles row by row.
+static std::vector<int> sy_map_host(int n) {
+  const int nt = (n + SY_T - 1) / SY_T, ns = (nt + 7) / 8;
+  std::vector<int> m;
+  m.reserve(sy_tiles(n));
+  for (int P = 0; P < ns; ++P)
+    for (int Q = 0; Q <= P; ++Q)
+      for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) {
+          const int I = 8 * P + i, J = 8 * Q + j;
+          if (I < nt && J <= I) m.push_back(I << 16 | J);
+        }
+  return m;
+}
+
+// The tile kernel's launch for one step from X into Xn (slab width k: 16 or 32 columns).
+static void sy_launch(int k, int grid, hipStream_t s, const double* X, double* Xn, int n, int np, const double* st,
+                      double* red, const int* map, int chunk) {
+  if (k == 16)
+    hipLaunchKernelGGL(lg_sp2_sym_kernel<16>, dim3(grid), dim3(SY_THREADS), 0, s, X, Xn, n, np, st, red, map, chunk);
+  else
+    hipLaunchKernelGGL(lg_sp2_sym_kernel<32>, dim3(grid), dim3(SY_THREADS), 0, s, X, Xn, n, np, st, red, map, chunk);
+}
+
+static bool sy_tile_map(Batch& b, int n, std::string& err) {
+  LargeBufs& L = b.lg;
+  const std::vector<int> m = sy_map_host(n);
+  if ((int)m.size() > L.sy_map_cap) {
+    if (!b.alloc(L.sy_map, (long long)m.size(), err)) return false;
+    L.sy_map_cap = (int)m.size();
+  }
+  HIP_OK(scopy(b.stream, L.sy_map, m.data(), sizeof(int) * m.size(), hipMemcpyHostToDevice));
+  L.sy_map_n = n;
+  return true;
+}
+
 static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double* Pdst, double lo, double hi, bool seed,
                              bool& ok, std::string& err) {
   LargeBufs& L = b.lg;
@@ -5211,8 +5409,11 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
   // q x q block-lower products: X X restricted to the blocks (I, J), I >= J, of size bs = n / q, as
   // q rounds of one batched dgemm each (round K: Y_IJ += X_IK X_KJ); the update reads only the lower
   // triangle, which these blocks cover.  (q + 1) / (2 q) of the full product's flops.
-  const bool fused = n <= LARGE_STEPS_N;  // one kernel per step, X ping-ponging between L.V and L.T
-  const int q = (!fused && !b.large_syrk && b.large_split >= 2 && b.large_split <= LG_SPLIT_MAX &&
+  const bool fused = n <= b.large_steps_n;  // one kernel per step, X ping-ponging between L.V and L.T
+  // the lower-triangle tile kernel (lg_sp2_sym_kernel): also one kernel per step, ping-ponging
+  const bool own = !fused && b.large_split == 1;
+  const bool pp = fused || own;
+  const int q = (!pp && !b.large_syrk && b.large_split >= 2 && b.large_split <= LG_SPLIT_MAX &&
                  n % b.large_split == 0 && n / b.large_split >= 64) ? b.large_split : 0;
   const int bs = q ? n / q : 0, nb = q * (q + 1) / 2;
   if (q) {
@@ -5231,14 +5432,24 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
   }
   // The loop of sp2_density with its decisions on the device (lg_sp2_state_kernel): the host
   // enqueues `per` steps at a time -- product, update, state -- and then reads the state.  Steps
-  // enqueued after the loop has stopped leave X alone; only their products run.  Small matrices
-  // take several steps per look (their products cost less than the host's round trip), large
-  // ones one.
-  const int per = fused ? LARGE_STEPS_PER_LOOK : 1;
+  // enqueued after the loop has stopped leave X alone; only their products run (the fused and
+  // tile kernels return at once).  Small matrices take several steps per look (their products
+  // cost less than the host's round trip); the tile kernel's first look comes two steps before
+  // the step count of the last purification, then one step per look; the rest look every step.
+  const int per0 = fused ? LARGE_STEPS_PER_LOOK : own ? std::max(1, L.sp2_steps - 2) : 1;
+  // one workgroup per tile (the dispatcher balances them over the CUs), two partials each in L.red
+  const int schunk = own ? (sy_tiles(n) + 7) / 8 : 0, sgrid = 8 * schunk;
+  if (own && sgrid > LG_BLOCKS * LG_MAXDOT / 2) {
+    err = "internal: too many product tiles";
+    return false;
+  }
+  if (own && L.sy_map_n != n) {
+    if (!sy_tile_map(b, n, err)) return false;
+  }
   const double occ = (double)nocc, one = 1.0, zero = 0.0;
   hipLaunchKernelGGL(lg_sp2_init_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, A, L.V, n, np, lo, hi, L.red);
   hipLaunchKernelGGL(lg_sp2_state_kernel, dim3(1), dim3(LG_BLOCKS), 0, b.stream, (const double*)L.red, L.sp2s, occ, 1,
-                     seed ? 1 : 0, LG_BLOCKS);
+                     seed ? 1 : 0, LG_BLOCKS, LG_MAXDOT);
   HIP_OK(hipGetLastError());
   double st[SP2S_N] = {0.0};  // read after each group of steps (a state finished at X0 makes them no-ops)
   const int gs = std::max(1, std::min(LG_BLOCKS, (n + 3) / 4));
@@ -5249,10 +5460,11 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
     HIP_OK(hipEventRecord(f0, b.stream));
   }
   for (int enq = 0; st[SP2S_STATUS] == 0.0 && enq < SP2_MAX;) {
+    const int per = enq == 0 || !own ? per0 : 1;
     for (int s = 0; s < per && enq < SP2_MAX; ++s, ++enq) {
-      // the iterate of this step: the fused path alternates buffers (if the loop has already
-      // stopped, the step does nothing and this pointer is not read)
-      const double* Xc = fused && (enq & 1) ? L.T : L.V;
+      // the iterate of this step: the fused and tile paths alternate buffers (if the loop has
+      // already stopped, the step does nothing and this pointer is not read)
+      const double* Xc = pp && (enq & 1) ? L.T : L.V;
       if (seed) {  // s = (X - X^2) r when the tail has just started (frontier_seed)
         double* sv = L.fs;
         double* t = L.fs + np;
@@ -5269,11 +5481,25 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
         hipLaunchKernelGGL(lg_sp2_fused_kernel, dim3(fgrid), dim3(64), 0, b.stream, Xc, (enq & 1) ? L.V : L.T, n, np,
                            (const double*)L.sp2s, L.red);
         hipLaunchKernelGGL(lg_sp2_state_kernel, dim3(1), dim3(LG_BLOCKS), 0, b.stream, (const double*)L.red, L.sp2s,
-                           occ, 0, seed ? 1 : 0, fgrid);
+                           occ, 0, seed ? 1 : 0, fgrid, LG_MAXDOT);
         HIP_OK(hipGetLastError());
         ++L.products;
         const long long bm = (n + 15) / 16;
         L.product_flop += bm * (bm + 1) / 2 * 2LL * 256 * ((n + 63) / 64 * 64);
+        continue;
+      }
+      if (own) {
+        hipEvent_t p0, p1;
+        HIP_OK(pool_pair(L.pev, L.npev, p0, p1));
+        HIP_OK(hipEventRecord(p0, b.stream));
+        sy_launch(b.sy_k, sgrid, b.stream, Xc, (enq & 1) ? L.V : L.T, n, np, (const double*)L.sp2s, L.red,
+                  (const int*)L.sy_map, schunk);
+        HIP_OK(hipEventRecord(p1, b.stream));
+        hipLaunchKernelGGL(lg_sp2_state_kernel, dim3(1), dim3(LG_BLOCKS), 0, b.stream, (const double*)L.red, L.sp2s,
+                           occ, 0, seed ? 1 : 0, sgrid, 2);
+        HIP_OK(hipGetLastError());
+        ++L.products;
+        L.product_flop += 2LL * sy_tiles(n) * SY_T * SY_T * ((n + b.sy_k - 1) / b.sy_k * b.sy_k);
         continue;
       }
       hipEvent_t p0, p1;
@@ -5299,7 +5525,7 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
       hipLaunchKernelGGL(lg_sp2_step_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, (const double*)L.T, L.V, n,
                          np, (const double*)L.sp2s, L.red);
       hipLaunchKernelGGL(lg_sp2_state_kernel, dim3(1), dim3(LG_BLOCKS), 0, b.stream, (const double*)L.red, L.sp2s, occ,
-                         0, seed ? 1 : 0, LG_BLOCKS);
+                         0, seed ? 1 : 0, LG_BLOCKS, LG_MAXDOT);
       HIP_OK(hipGetLastError());
       ++L.products;
       L.product_flop += q ? 2LL * bs * (long long)bs * bs * nb * q : (b.large_syrk ? 1LL : 2LL) * n * (long long)n * n;
@@ -5308,8 +5534,9 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
   }
   if (fused) HIP_OK(hipEventRecord(f1, b.stream));
   if (st[SP2S_SEEDED] != 0.0) L.seeded = true;
+  if (st[SP2S_STATUS] == 1.0) L.sp2_steps = (int)st[SP2S_K];
   if (st[SP2S_STATUS] != 1.0) return true;  // a bound violated, or no convergence: ok stays false
-  const double* Xf = fused && ((int)st[SP2S_K] & 1) ? L.T : L.V;  // after K steps
+  const double* Xf = pp && ((int)st[SP2S_K] & 1) ? L.T : L.V;  // after K steps
   hipLaunchKernelGGL(lg_twice_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, Xf, Pdst, n, np);
   HIP_OK(hipGetLastError());
   ok = true;
@@ -6180,6 +6407,9 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
                              (long long)tune.pair_blocks_per_cu * prop.multiProcessorCount));
   b.large_syrk = tune.large_syrk != 0;
   b.large_split = tune.large_split;
+  b.large_steps_n = LARGE_STEPS_N;  // the environment may lower it, so that tests reach the larger-matrix kernels
+  if (const char* ev = std::getenv("PM3_LARGE_STEPS_N")) b.large_steps_n = std::atoi(ev);
+  if (const char* ev = std::getenv("PM3_SY_K")) b.sy_k = std::atoi(ev) == 32 ? 32 : 16;
   a.max_scf = opt.max_scf;
   a.e_tol = opt.e_tol;
   a.p_tol = opt.p_tol;
@@ -6884,6 +7114,82 @@ bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOpt
 
 // divide-and-conquer SCF (run_dc) over the host-driven path
 #include "pm3_dc.hpp"
+
+// One step of the tile kernel on a host matrix (kernel-level hook): y = x x (square) or 2 x - x x,
+// with tr y and |y|_F^2 summed from the kernel's partials.  The device copy of x has NaN in its
+// padding and y starts as NaN, so a read beyond n or an element left unwritten shows.  With
+// reps > 0 the same step is launched reps more times between two events (ms per launch).
+int sym_step_stage(int n, int square, int slab, const double* x, double* y, double* trace, double* norm2, int reps,
+                   double* ms, std::string& err) {
+  if (n < 1 || n > 16384 || (slab != 16 && slab != 32) || reps < 0) {
+    err = "n must be in [1, 16384], slab 16 or 32, reps >= 0";
+    return 2;
+  }
+  const int np = (n + 15) & ~15, chunk = (sy_tiles(n) + 7) / 8, grid = 8 * chunk;
+  const size_t m2 = (size_t)np * np;
+  double *dx = nullptr, *dy = nullptr, *dred = nullptr, *dst = nullptr;
+  int* dmap = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto done = [&](bool ok) {
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    (void)hipFree(dred);
+    (void)hipFree(dst);
+    (void)hipFree(dmap);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return ok ? 0 : 6;
+  };
+  const std::vector<int> map = sy_map_host(n);
+  double st[SP2S_N] = {0.0};
+  st[SP2S_SQ] = square ? 1.0 : 0.0;
+#define SY_OK(x)                                           \
+  do {                                                     \
+    hipError_t e_ = (x);                                   \
+    if (e_ != hipSuccess) {                                \
+      err = std::string(#x) + ": " + hipGetErrorString(e_); \
+      return done(false);                                  \
+    }                                                      \
+  } while (0)
+  SY_OK(hipMalloc(&dx, m2 * sizeof(double)));
+  SY_OK(hipMalloc(&dy, m2 * sizeof(double)));
+  SY_OK(hipMalloc(&dred, 2 * (size_t)grid * sizeof(double)));
+  SY_OK(hipMalloc(&dst, SP2S_N * sizeof(double)));
+  SY_OK(hipMalloc(&dmap, map.size() * sizeof(int)));
+  SY_OK(hipMemset(dx, 0xff, m2 * sizeof(double)));
+  SY_OK(hipMemset(dy, 0xff, m2 * sizeof(double)));
+  SY_OK(hipMemcpy2D(dx, np * sizeof(double), x, n * sizeof(double), n * sizeof(double), n, hipMemcpyHostToDevice));
+  SY_OK(hipMemcpy(dst, st, sizeof(st), hipMemcpyHostToDevice));
+  SY_OK(hipMemcpy(dmap, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice));
+  sy_launch(slab, grid, nullptr, dx, dy, n, np, dst, dred, dmap, chunk);
+  SY_OK(hipGetLastError());
+  SY_OK(hipDeviceSynchronize());
+  std::vector<double> red(2 * (size_t)grid);
+  SY_OK(hipMemcpy2D(y, n * sizeof(double), dy, np * sizeof(double), n * sizeof(double), n, hipMemcpyDeviceToHost));
+  SY_OK(hipMemcpy(red.data(), dred, red.size() * sizeof(double), hipMemcpyDeviceToHost));
+  double a = 0.0, c = 0.0;
+  for (int w = 0; w < grid; ++w) {
+    a += red[2 * w];
+    c += red[2 * w + 1];
+  }
+  if (trace) *trace = a;
+  if (norm2) *norm2 = c;
+  if (ms) *ms = 0.0;
+  if (reps > 0) {
+    SY_OK(hipEventCreate(&e0));
+    SY_OK(hipEventCreate(&e1));
+    SY_OK(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r) sy_launch(slab, grid, nullptr, dx, dy, n, np, dst, dred, dmap, chunk);
+    SY_OK(hipEventRecord(e1, nullptr));
+    SY_OK(hipGetLastError());
+    SY_OK(hipEventSynchronize(e1));
+    float t = 0.0f;
+    SY_OK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = (double)t / reps;
+  }
+#undef SY_OK
+  return done(true);
+}
 
 }  // namespace pm3host
 #endif  // PM3_KERNEL_VARIANT

@@ -109,9 +109,11 @@ struct Tuning {
                                 // 5 + X's upper 16 x 16 blocks (tiled, swizzled) in LDS, updated in place
   int sp2_tight_n = 48;         // purification bounds from Lanczos from this size (AOs) on
   int large_syrk = 0;           // host-driven path: purification products by dsyrk (1) or dgemm (0)
-  int large_split = 2;          // host-driven path: q >= 2 computes only the q x q block-lower part of each
-                                // product (q batched dgemm rounds) when q divides n; 0 one dgemm
-                                // (2 measured fastest at 3072 AOs: 0.70 ms per product against 0.87)
+  int large_split = 1;          // host-driven path, matrices above 1024 AOs: 1 purification products by the
+                                // lower-triangle 64 x 64 tile kernel (update fused into its store);
+                                // q in 2..4 only the q x q block-lower part of each product by q batched
+                                // rocBLAS dgemm rounds when q divides n; 0 one rocBLAS dgemm (at 3072 AOs:
+                                // 0.59 ms per step against 0.70 + 0.04 for q = 2 and 0.87 + 0.04 for 0)
   int wg128_nao = 80;           // molecules up to this size (AOs) whose LDS needs fit 36 KiB run on
                                 // 128-thread workgroups, four per CU
   int wg256_nao = 96;           // then those up to this size fitting 76 KiB on 256-thread workgroups, two
@@ -161,6 +163,12 @@ void batch_guess_timing(const Batch* b, double& features_ms, double& network_ms,
 // Lanczos runs, [13] Jacobi sweeps or purification products, [14] eigensolves or purifications,
 // [15] Jacobi rounds or purification fallbacks (16 slots).
 bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err);
+
+// One step of the large path's purification tile kernel on a host n x n matrix: y = x x (square)
+// or 2 x - x x, tr y and |y|_F^2; reps more launches timed (ms per launch).  Returns 0, 2 (bad
+// argument) or 6 (device error), with err set.
+int sym_step_stage(int n, int square, int slab, const double* x, double* y, double* trace, double* norm2, int reps,
+                   double* ms, std::string& err);
 
 // One stage of the pipeline on a single molecule (kernel-level parity hooks); see stage_kernel.
 bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,

@@ -159,7 +159,8 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_tuning_default", "pm3_tuning_get", "pm3_tuning_set", "pm3_tuning_load", "pm3_tuning_save", "pm3_autotune",
             "pm3_batch_large_timing", "pm3_batch_set_positions", "pm3_opt_options_default", "pm3_optimize_batch",
             "pm3_optimize", "pm3_batch_carry_densities", "pm3_batch_spin_densities", "pm3_run_noalloc",
-            "pm3_response_fock", "pm3_numerical_hessian", "pm3_element_mass", "pm3_dc_options_default", "pm3_run_dc", "pm3_dc_gradient"]
+            "pm3_response_fock", "pm3_numerical_hessian", "pm3_element_mass", "pm3_dc_options_default", "pm3_run_dc", "pm3_dc_gradient",
+            "pm3_stage_sym_step"]
 
 _lib = None
 

@@ -176,3 +176,79 @@ def test_mixed_batch_of_small_and_large_molecules(G):
         assert abs(r.heat_of_formation_kcal - one.scf.heat_of_formation_kcal) <= HOF_TOL
         assert np.abs(g - one.gradient).max() <= 1e-6
         assert np.abs(g.sum(axis=0)).max() < 1e-8
+
+
+def _sym_step(G):
+    import ctypes
+    L = G.native()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.pm3_stage_sym_step.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp, dp, dp, dp, ctypes.c_int32, dp]
+
+    def run(x, square, slab, reps=0):
+        n = x.shape[0]
+        x = np.ascontiguousarray(x)
+        y = np.zeros((n, n))
+        tr, f2, ms = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        st = L.pm3_stage_sym_step(n, square, slab, x.ctypes.data_as(dp), y.ctypes.data_as(dp), ctypes.byref(tr),
+                                  ctypes.byref(f2), reps, ctypes.byref(ms))
+        assert st == 0, G.native().pm3_last_error_message()
+        return y, tr.value, f2.value, ms.value
+    return run
+
+
+@pytest.mark.parametrize("slab", [16, 32])
+def test_tile_kernel_step_matches_numpy_at_ragged_sizes(G, slab):
+    """One purification step of the lower-triangle tile kernel (64 x 64 tiles, mirrored store) on
+    symmetric matrices of ragged sizes: y = x x and 2 x - x x against numpy to 1e-13 relative,
+    exactly symmetric, every element written (y starts as NaN, x's padding is NaN), trace and
+    Frobenius sums to 1e-12, and the same bits on a second call."""
+    run = _sym_step(G)
+    rng = np.random.default_rng(7 + slab)
+    for n in (1, 2, 15, 16, 17, 31, 63, 64, 65, 127, 129, 200, 420, 1000):
+        a = rng.standard_normal((n, n))
+        x = (a + a.T) / (2.0 * np.sqrt(n))
+        x2 = x @ x
+        scale = np.abs(x2).max() + 1.0
+        for square in (1, 0):
+            ref = x2 if square else 2.0 * x - x2
+            y, tr, f2, _ = run(x, square, slab)
+            assert np.all(np.isfinite(y)), n
+            assert np.array_equal(y, y.T), n
+            err = np.abs(y - ref).max() / scale
+            assert err <= 1e-13, (n, square, err)
+            assert abs(tr - np.trace(ref)) <= 1e-12 * max(1.0, abs(np.trace(ref))), n
+            assert abs(f2 - np.sum(ref * ref)) <= 1e-12 * max(1.0, np.sum(ref * ref)), n
+            y2, tr2, f22, _ = run(x, square, slab)
+            assert np.array_equal(y, y2) and tr == tr2 and f22 == f2, n
+        print("n", n, "slab", slab, "max rel err", err, flush=True)
+
+
+def test_tile_kernel_large_path_matches_oracle(G, oracle, monkeypatch):
+    """The 420-AO cluster through the multi-kernel path with its purification products on the tile
+    kernel (large_split 1, fused-kernel ceiling lowered below the size), both slab widths: the
+    oracle's answers to the same tolerances as the default route."""
+    z, x = W.water_cluster(70, W.water_box_for(70), 70)
+    z = np.asarray(z, np.int32)
+    xb = W.to_bohr(x)
+    st, o = oracle.run(z, xb, 0.0, oracle.options(**TIGHT))
+    assert st == 0
+    d = G.tuning_default()
+    monkeypatch.setenv("PM3_LARGE_STEPS_N", "64")
+    try:
+        for slab in ("16", "32"):
+            monkeypatch.setenv("PM3_SY_K", slab)
+            G.tuning_set(dict(d, large_split=1))
+            b = G.Batch([G.Molecule(z, xb)], options=G.Pm3Options(**TIGHT), want_density=True)
+            b.solve()
+            s, r = b.results()
+            t = b.large_timing()
+            b.close()
+            q = r[0]
+            print("slab", slab, "iterations", q.iterations, "products", t["products"], flush=True)
+            assert s[0] == 0 and abs(q.heat_of_formation_kcal - o["hof_kcal"]) <= HOF_TOL
+            assert np.abs(q.density - o["density"]).max() <= 1e-8
+            assert abs(q.homo_ev - o["homo_ev"]) <= 1e-7 and abs(q.lumo_ev - o["lumo_ev"]) <= 1e-7
+            nt = (420 + 63) // 64
+            assert t["product_flop"] == nt * (nt + 1) // 2 * 2 * 64 * 64 * ((420 + int(slab) - 1) // int(slab) * int(slab)) * t["products"]
+    finally:
+        G.tuning_set(d)

@@ -82,7 +82,7 @@ def test_tuning_table_round_trip_and_validation(tmp_path):
     import pm3amd as G
     d = G.tuning_default()
     assert d == {"pair_block": 256, "pair_blocks_per_cu": 8, "sp2_tile16_n": 128, "sp2_lds": 5, "sp2_tight_n": 48,
-                 "large_syrk": 0, "large_split": 2, "wg128_nao": 80, "wg256_nao": 96, "solo_large_nao": 128}
+                 "large_syrk": 0, "large_split": 1, "wg128_nao": 80, "wg256_nao": 96, "solo_large_nao": 128}
     start = G.tuning_get()
     try:
         G.tuning_set({"pair_block": 128, "sp2_tile16_n": 96})
