@@ -4350,8 +4350,9 @@ template <int K>
 __global__ __launch_bounds__(SY_THREADS) void lg_sp2_sym_kernel(const double* X, double* Xn, int n, int np,
                                                                const double* st, double* red, const int* map,
                                                                int chunk) {
-  static_assert(K == 16 || K == 32, "slab width");
-  constexpr int CH = K / 2, LQ = SY_T * CH / SY_THREADS, LC = CH / 4;  // chunks: a row, a thread's loads, a lane's
+  static_assert(K == 8 || K == 16 || K == 32, "slab width");
+  constexpr int NT = SY_THREADS;
+  constexpr int CH = K / 2, LQ = SY_T * CH / NT, LC = CH / 4;  // chunks: a row, a thread's loads, a lane's
   if (st[SP2S_STATUS] != 0.0) return;
   __shared__ double2 slab[2][2][SY_T * CH];  // [buffer][panel I, panel J][slot]
   const bool sq = st[SP2S_SQ] != 0.0;
@@ -4378,7 +4379,7 @@ __global__ __launch_bounds__(SY_THREADS) void lg_sp2_sym_kernel(const double* X,
         if (p >= np_) break;
 #pragma unroll
         for (int q = 0; q < LQ; ++q) {
-          const int id = q * SY_THREADS + tid, row = r0[p] + id / CH, k = k0 + 2 * (id % CH);
+          const int id = q * NT + tid, row = r0[p] + id / CH, k = k0 + 2 * (id % CH);
           g[p][q] = *(const double2*)(X + (size_t)min(row, n - 1) * np + min(k, kmax));
         }
       }
@@ -4389,7 +4390,7 @@ __global__ __launch_bounds__(SY_THREADS) void lg_sp2_sym_kernel(const double* X,
         if (p >= np_) break;
 #pragma unroll
         for (int q = 0; q < LQ; ++q) {
-          const int id = q * SY_THREADS + tid, row = r0[p] + id / CH, k = k0 + 2 * (id % CH);
+          const int id = q * NT + tid, row = r0[p] + id / CH, k = k0 + 2 * (id % CH);
           double2 v = g[p][q];
           const bool in = row < n && k < n;
           v.x = in ? v.x : 0.0;
@@ -4982,7 +4983,7 @@ struct Batch {
   bool large_syrk = false;
   int large_split = 0;
   int large_steps_n = 0;  // the fused purification kernel up to this many AOs
-  int sy_k = 16;          // slab width of the tile kernel (16 or 32 columns)
+  int sy_k = 16;          // slab width of the tile kernel (8, 16 or 32 columns)
   // learned guess: directed pairs of the eligible molecules and their network inputs/outputs
   const GuessModel* guess_model = nullptr;
   bool learned = false, solved_once = false;
@@ -5383,10 +5384,12 @@ static std::vector<int> sy_map_host(int n) {
 // The tile kernel's launch for one step from X into Xn (slab width k: 16 or 32 columns).
 static void sy_launch(int k, int grid, hipStream_t s, const double* X, double* Xn, int n, int np, const double* st,
                       double* red, const int* map, int chunk) {
-  if (k == 16)
-    hipLaunchKernelGGL(lg_sp2_sym_kernel<16>, dim3(grid), dim3(SY_THREADS), 0, s, X, Xn, n, np, st, red, map, chunk);
-  else
+  if (k == 8)
+    hipLaunchKernelGGL(lg_sp2_sym_kernel<8>, dim3(grid), dim3(SY_THREADS), 0, s, X, Xn, n, np, st, red, map, chunk);
+  else if (k == 32)
     hipLaunchKernelGGL(lg_sp2_sym_kernel<32>, dim3(grid), dim3(SY_THREADS), 0, s, X, Xn, n, np, st, red, map, chunk);
+  else
+    hipLaunchKernelGGL(lg_sp2_sym_kernel<16>, dim3(grid), dim3(SY_THREADS), 0, s, X, Xn, n, np, st, red, map, chunk);
 }
 
 static bool sy_tile_map(Batch& b, int n, std::string& err) {
@@ -6409,7 +6412,10 @@ static bool batch_create_in(const Params& P, const std::vector<MolIn>& mols, con
   b.large_split = tune.large_split;
   b.large_steps_n = LARGE_STEPS_N;  // the environment may lower it, so that tests reach the larger-matrix kernels
   if (const char* ev = std::getenv("PM3_LARGE_STEPS_N")) b.large_steps_n = std::atoi(ev);
-  if (const char* ev = std::getenv("PM3_SY_K")) b.sy_k = std::atoi(ev) == 32 ? 32 : 16;
+  if (const char* ev = std::getenv("PM3_SY_K")) {  // the slab width of the tile kernel: 8, 16 (default) or 32
+    const int k = std::atoi(ev);
+    b.sy_k = k == 8 || k == 32 ? k : 16;
+  }
   a.max_scf = opt.max_scf;
   a.e_tol = opt.e_tol;
   a.p_tol = opt.p_tol;
@@ -7121,8 +7127,8 @@ bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOpt
 // reps > 0 the same step is launched reps more times between two events (ms per launch).
 int sym_step_stage(int n, int square, int slab, const double* x, double* y, double* trace, double* norm2, int reps,
                    double* ms, std::string& err) {
-  if (n < 1 || n > 16384 || (slab != 16 && slab != 32) || reps < 0) {
-    err = "n must be in [1, 16384], slab 16 or 32, reps >= 0";
+  if (n < 1 || n > 16384 || (slab != 8 && slab != 16 && slab != 32) || reps < 0) {
+    err = "n must be in [1, 16384], slab 8, 16 or 32, reps >= 0";
     return 2;
   }
   const int np = (n + 15) & ~15, chunk = (sy_tiles(n) + 7) / 8, grid = 8 * chunk;

@@ -196,7 +196,7 @@ def _sym_step(G):
     return run
 
 
-@pytest.mark.parametrize("slab", [16, 32])
+@pytest.mark.parametrize("slab", [8, 16, 32])
 def test_tile_kernel_step_matches_numpy_at_ragged_sizes(G, slab):
     """One purification step of the lower-triangle tile kernel (64 x 64 tiles, mirrored store) on
     symmetric matrices of ragged sizes: y = x x and 2 x - x x against numpy to 1e-13 relative,
@@ -235,7 +235,7 @@ def test_tile_kernel_large_path_matches_oracle(G, oracle, monkeypatch):
     d = G.tuning_default()
     monkeypatch.setenv("PM3_LARGE_STEPS_N", "64")
     try:
-        for slab in ("16", "32"):
+        for slab in ("8", "16", "32"):
             monkeypatch.setenv("PM3_SY_K", slab)
             G.tuning_set(dict(d, large_split=1))
             b = G.Batch([G.Molecule(z, xb)], options=G.Pm3Options(**TIGHT), want_density=True)
