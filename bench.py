@@ -52,7 +52,8 @@ NET_FLOP_PER_PAIR = 2.0 * (96 * 128 + 128 * 128 + 128 * 16)  # forward pass per 
 # with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a 512 MB copy: x2.0 / x1.0) on the default
 # workload, per leg.  A recorded value, not measured by this run: the JSON line carries it with its
 # source (the profile summary and the build it was taken on).
-MEASURED_TRAFFIC = {("mixed4096", "learned"): 9.70e10, ("mixed4096", "plain"): 1.349e11}
+MEASURED_TRAFFIC = {("mixed4096", "learned"): 8.98e10, ("mixed4096", "plain"): 1.237e11,
+                    ("water512", "tile"): 9.97e8}  # HBM bytes per SCF launch / per purification step
 TRAFFIC_SOURCE = ("recorded, not measured in this run: profiles/scf_batch_kernel_traffic_mixed4096.md "
                   "(rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of bench.py --steps 1)")
 HBM_PEAK_GBS = 8000.0
@@ -189,7 +190,11 @@ def run_large(args, steps, warmup, tune, cpu=True):
                    "gradient_sum_max_abs": float(np.abs(g.sum(axis=0)).max())},
         "device_ms_per_step": float(np.mean(kms)),
         "roofline": {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": MEASURED_TRAFFIC.get(("water512", "tile")) if split == 1 and n == 3072 else None,
+                     "traffic_source": ("recorded, not measured in this run: profiles/large_path_water512.md (rocprofv3 "
+                                        "--pmc FETCH_SIZE and WRITE_SIZE runs of bench.py --workload water512)")
+                     if split == 1 and n == 3072 else "not measured for this kernel",
                      "launch_ms": prod_ms / max(nprod, 1), "launches_per_step": nprod / steps,
                      "flop_per_launch": prod_flop / max(nprod, 1),
                      "flop_note": "achieved counts the FLOP issued (2 x 64 x 64 x the slab-rounded AO count per "
