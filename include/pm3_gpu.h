@@ -11,6 +11,8 @@
  *   pm3_optimize   <- optimize(&Molecule, &Pm3Parameters, &Pm3Options, &OptOptions)
  *                     -> Result<OptResult> (src/optimizer.rs:233-262)
  *   pm3_optimize_batch -- the same L-BFGS for many molecules, one batched solve per round.
+ *   pm3_optimize_dc <- optimize_dc(&Molecule, &Pm3Parameters, &Pm3Options, &DcOptions, &OptOptions)
+ *                     -> Result<DcOptResult> (src/optimizer.rs:277-320)
  *
  * Units follow the reference (src/system.rs:21-26): positions in Bohr, energies in eV, heat
  * of formation in kcal/mol, gradient in eV/Bohr, dipole in Debye.  Matrices are n x n
@@ -394,6 +396,15 @@ pm3_status pm3_optimize_batch(int32_t n_mol, const int32_t* atom_ptr, const int3
 pm3_status pm3_optimize(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
                         const pm3_opt_options* opt, pm3_opt_result* result, double* xyz_out, pm3_opt_step* steps,
                         double* traj_xyz);
+
+/* optimize_dc (src/optimizer.rs:277-310) -> Result<DcOptResult> (:313-320): the optimiser of
+ * pm3_optimize over the divide-and-conquer gradient (pm3_dc_gradient at every point, a failed trial
+ * a rejected step).  *result as pm3_optimize's; *scf (may be NULL) the DcResult scalars of the last
+ * accepted point; xyz_out, steps and traj_xyz as pm3_optimize's.  Returns the first evaluation's
+ * error, else PM3_OK (not converged within max_iter is result->converged = 0, as the reference). */
+pm3_status pm3_optimize_dc(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                           const pm3_dc_options* dc, const pm3_opt_options* opt, pm3_opt_result* result,
+                           pm3_dc_result* scf, double* xyz_out, pm3_opt_step* steps, double* traj_xyz);
 
 #ifdef __cplusplus
 }

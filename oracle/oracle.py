@@ -579,31 +579,10 @@ def _dot(a, b):
     return s
 
 
-def optimize(z, xyz_bohr, charge=0.0, opts=None, max_iter=200, gtol=1.0e-3, history=8, multiplicity=1):
-    """optimize / drive (src/optimizer.rs:77-262): L-BFGS on the PM3 surface with a backtracking
-    Armijo search.  Trials ask for the energy alone (run_pm3's total_ev, src/optimizer.rs:254),
-    accepted points are evaluated again for the gradient (closed_form_gradient, :244-252).
-    Returns a dict: converged, iterations, evaluations (gradient calls), energy_ev, hof_kcal,
-    max_gradient, xyz (final, Bohr), trajectory (energies of the accepted points, first included),
-    steps (OptStep, :37-43, of the same points: energy_ev, hof_kcal, max_gradient, xyz)."""
-    z = np.asarray(z, dtype=np.int32)
-    opts = opts or options()
-    x = np.array(xyz_bohr, dtype=np.float64).reshape(-1)
-
-    def evaluate(xv):
-        st, o = gradient(z, xv.reshape(-1, 3), charge, opts, multiplicity)
-        if st != 0:
-            raise RuntimeError(STATUS[st])
-        g = o["gradient"].reshape(-1).copy()
-        return o["total_ev"], g, o["hof_kcal"], float(np.abs(g).max()) if g.size else 0.0
-
-    def energy_only(xv):
-        if multiplicity == 1:
-            st, o = run(z, xv.reshape(-1, 3), charge, opts, want_arrays=False)
-        else:
-            st, o = run_pm3(z, xv.reshape(-1, 3), charge, multiplicity, opts)
-        return o["total_ev"] if st == 0 else None
-
+def _drive(x, evaluate, energy_only, max_iter, gtol, history):
+    """drive (src/optimizer.rs:77-221): L-BFGS with a backtracking Armijo search over an evaluator
+    of (energy, gradient, heat of formation, max |g|) and a trial-energy function that returns
+    None for a point it cannot evaluate (a rejected step)."""
     energy, g, heat, maxg = evaluate(x)
     evaluations = 1
     s_hist, y_hist, rho_hist = [], [], []
@@ -668,3 +647,52 @@ def optimize(z, xyz_bohr, charge=0.0, opts=None, max_iter=200, gtol=1.0e-3, hist
     return {"converged": converged, "iterations": iterations, "evaluations": evaluations, "energy_ev": energy,
             "hof_kcal": heat, "max_gradient": maxg, "xyz": x.reshape(-1, 3), "trajectory": trajectory,
             "steps": steps}
+
+
+def optimize(z, xyz_bohr, charge=0.0, opts=None, max_iter=200, gtol=1.0e-3, history=8, multiplicity=1):
+    """optimize / drive (src/optimizer.rs:77-262): L-BFGS on the PM3 surface with a backtracking
+    Armijo search.  Trials ask for the energy alone (run_pm3's total_ev, src/optimizer.rs:254),
+    accepted points are evaluated again for the gradient (closed_form_gradient, :244-252).
+    Returns a dict: converged, iterations, evaluations (gradient calls), energy_ev, hof_kcal,
+    max_gradient, xyz (final, Bohr), trajectory (energies of the accepted points, first included),
+    steps (OptStep, :37-43, of the same points: energy_ev, hof_kcal, max_gradient, xyz)."""
+    z = np.asarray(z, dtype=np.int32)
+    opts = opts or options()
+    x = np.array(xyz_bohr, dtype=np.float64).reshape(-1)
+
+    def evaluate(xv):
+        st, o = gradient(z, xv.reshape(-1, 3), charge, opts, multiplicity)
+        if st != 0:
+            raise RuntimeError(STATUS[st])
+        g = o["gradient"].reshape(-1).copy()
+        return o["total_ev"], g, o["hof_kcal"], float(np.abs(g).max()) if g.size else 0.0
+
+    def energy_only(xv):
+        if multiplicity == 1:
+            st, o = run(z, xv.reshape(-1, 3), charge, opts, want_arrays=False)
+        else:
+            st, o = run_pm3(z, xv.reshape(-1, 3), charge, multiplicity, opts)
+        return o["total_ev"] if st == 0 else None
+
+    return _drive(x, evaluate, energy_only, max_iter, gtol, history)
+
+
+def optimize_dc(z, xyz_bohr, charge=0.0, multiplicity=1, opts=None, dc=None, max_iter=200, gtol=1.0e-3, history=8):
+    """optimize_dc (src/optimizer.rs:277-310): the same drive over dc_gradient, trial energies from
+    run_dc's total_ev (any failure a rejected trial).  Returns optimize's dict."""
+    z = np.asarray(z, dtype=np.int32)
+    opts = opts or options()
+    x = np.array(xyz_bohr, dtype=np.float64).reshape(-1)
+
+    def evaluate(xv):
+        st, o = dc_gradient(z, xv.reshape(-1, 3), charge, multiplicity, opts, dc)
+        if st != 0:
+            raise RuntimeError(STATUS[st])
+        g = o["gradient"].reshape(-1).copy()
+        return o["total_ev"], g, o["hof_kcal"], float(np.abs(g).max()) if g.size else 0.0
+
+    def energy_only(xv):
+        st, o = run_dc(z, xv.reshape(-1, 3), charge, multiplicity, opts, dc)
+        return o["total_ev"] if st == 0 else None
+
+    return _drive(x, evaluate, energy_only, max_iter, gtol, history)

@@ -333,4 +333,100 @@ pm3_status pm3_optimize(const pm3_molecule* mol, const pm3_params* params, const
   return s != PM3_OK ? s : (pm3_status)result->status;
 }
 
+// optimize_dc (src/optimizer.rs:277-310): the same optimiser for one molecule over the
+// divide-and-conquer gradient.  Every point is one pm3_dc_gradient call (run_dc, then the gradient
+// at the partitioned density); a trial's call carries its gradient, so an accepted point costs no
+// second solve (the reference asks a trial for run_dc's energy alone and evaluates the accepted
+// point again: the same SCF).  A trial that fails in any way is a rejected step, as the reference's
+// energy_only maps an error to None; a failure of the first evaluation is the call's error.
+pm3_status pm3_optimize_dc(const pm3_molecule* mol, const pm3_params* params, const pm3_options* opts,
+                           const pm3_dc_options* dc, const pm3_opt_options* opt_in, pm3_opt_result* result,
+                           pm3_dc_result* scf, double* xyz_out, pm3_opt_step* steps, double* traj_xyz) {
+  if (!mol || !params || !result || !xyz_out || mol->n_atoms < 1 || !mol->z || !mol->xyz_bohr)
+    return PM3_INVALID_INPUT;
+  if (mol->multiplicity < 1) return PM3_INVALID_INPUT;
+  pm3_opt_options opt;
+  pm3_opt_options_default(&opt);
+  if (opt_in) opt = *opt_in;
+  if (opt.max_iter < 0 || opt.history < 0 || !(opt.gtol >= 0.0)) return PM3_INVALID_INPUT;
+  const size_t cap = (size_t)opt.max_iter + 1;
+  std::memset(result, 0, sizeof(*result));
+  OptState o;
+  o.nat = mol->n_atoms;
+  o.x.assign(mol->xyz_bohr, mol->xyz_bohr + 3 * (size_t)o.nat);
+  o.xt = o.x;
+  auto eval = [&](const std::vector<double>& x, pm3_dc_result& r, std::vector<double>& g) {
+    pm3_molecule m = *mol;
+    m.xyz_bohr = x.data();
+    g.assign(x.size(), 0.0);
+    return pm3_dc_gradient(&m, params, opts, dc, &r, g.data(), nullptr, nullptr, nullptr);
+  };
+  pm3_dc_result cur{};
+  std::vector<double> g;
+  pm3_status s = eval(o.x, cur, g);
+  result->evaluations = 1;
+  if (s != PM3_OK) {
+    result->status = s;
+    if (scf) *scf = cur;
+    std::memcpy(xyz_out, o.x.data(), sizeof(double) * o.x.size());
+    return s;
+  }
+  o.energy = cur.total_ev;
+  o.heat = cur.heat_of_formation_kcal;
+  o.g = g;
+  o.maxg = max_abs(g);
+  result->first_energy_ev = o.energy;
+  o.converged = o.maxg < opt.gtol;
+  record(o, *result, cap, steps, traj_xyz);
+  loop_top(o, opt, *result);
+  while (!o.done) {
+    if (coincident(o.xt)) {  // a trial point with coincident atoms cannot be solved: rejected
+      if (!next_trial(o)) o.done = true;
+      continue;
+    }
+    pm3_dc_result tr{};
+    std::vector<double> gn;
+    s = eval(o.xt, tr, gn);
+    result->evaluations += 1;
+    if (s == PM3_OK && tr.total_ev <= o.energy + 1.0e-4 * o.step * o.gdd) {
+      // accepted: the L-BFGS memory update (src/optimizer.rs:183-197), then the next iteration
+      std::vector<double> sv(o.x.size()), yv(o.x.size());
+      for (size_t i = 0; i < sv.size(); ++i) {
+        sv[i] = o.xt[i] - o.x[i];
+        yv[i] = gn[i] - o.g[i];
+      }
+      const double sy = dot(sv, yv);
+      if (sy > 1.0e-10) {
+        o.s.push_back(std::move(sv));
+        o.y.push_back(std::move(yv));
+        o.rho.push_back(1.0 / sy);
+        if ((int)o.s.size() > opt.history) {
+          o.s.erase(o.s.begin());
+          o.y.erase(o.y.begin());
+          o.rho.erase(o.rho.begin());
+        }
+      }
+      o.x = o.xt;
+      o.g = gn;
+      cur = tr;
+      o.energy = tr.total_ev;
+      o.heat = tr.heat_of_formation_kcal;
+      o.maxg = max_abs(gn);
+      o.converged = o.maxg < opt.gtol;
+      o.iter += 1;
+      record(o, *result, cap, steps, traj_xyz);
+      loop_top(o, opt, *result);
+    } else if (!next_trial(o)) {
+      o.done = true;  // no progress possible: stop at the current point
+    }
+  }
+  result->converged = o.converged ? 1 : 0;
+  result->energy_ev = o.energy;
+  result->heat_of_formation_kcal = o.heat;
+  result->max_gradient = o.maxg;
+  if (scf) *scf = cur;
+  std::memcpy(xyz_out, o.x.data(), sizeof(double) * o.x.size());
+  return PM3_OK;
+}
+
 }  // extern "C"

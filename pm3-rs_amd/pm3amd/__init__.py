@@ -160,7 +160,7 @@ EXPORTED = ["pm3_params_standard", "pm3_options_default", "pm3_query_nao", "pm3_
             "pm3_batch_large_timing", "pm3_batch_set_positions", "pm3_opt_options_default", "pm3_optimize_batch",
             "pm3_optimize", "pm3_batch_carry_densities", "pm3_batch_spin_densities", "pm3_run_noalloc",
             "pm3_response_fock", "pm3_numerical_hessian", "pm3_element_mass", "pm3_dc_options_default", "pm3_run_dc", "pm3_dc_gradient",
-            "pm3_stage_sym_step"]
+            "pm3_stage_sym_step", "pm3_optimize_dc"]
 
 _lib = None
 
@@ -247,6 +247,10 @@ def native():
         L.pm3_optimize.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
                                    ctypes.POINTER(_COptOptions), ctypes.POINTER(_COptResult), dp,
                                    ctypes.POINTER(_COptStep), dp]
+        L.pm3_optimize_dc.argtypes = [ctypes.POINTER(_CMolecule), ctypes.c_void_p, ctypes.POINTER(_COptions),
+                                      ctypes.POINTER(_CDcOptions), ctypes.POINTER(_COptOptions),
+                                      ctypes.POINTER(_COptResult), ctypes.POINTER(_CDcResult), dp,
+                                      ctypes.POINTER(_COptStep), dp]
         _lib = L
     return _lib
 
@@ -910,6 +914,52 @@ def optimize(molecule: Molecule, params: Optional[Pm3Parameters] = None, scf_opt
     _raise(native().pm3_optimize(ctypes.byref(cm), params._h, ctypes.byref(scf_options._c()), ctypes.byref(opt._c()),
                                  ctypes.byref(res), _dp(out), steps, _dp(traj)))
     return _opt_result(molecule, res, out, steps, traj)
+
+
+@dataclasses.dataclass
+class DcOptResult:
+    """DcOptResult (src/optimizer.rs:313-320): OptResult's fields with the divide-and-conquer scalars of
+    the last accepted point (scf: a DcResult without arrays: density, spin_density and charges are
+    None; run_dc at the final geometry gives them)."""
+    molecule: Molecule
+    scf: "DcResult"
+    converged: bool
+    iterations: int
+    evaluations: int
+    energy_ev: float
+    heat_of_formation_kcal: float
+    max_gradient: float
+    first_energy_ev: float
+    trajectory: List[OptStep] = dataclasses.field(default_factory=list)
+
+
+def optimize_dc(molecule: Molecule, params: Optional[Pm3Parameters] = None, scf_options: Optional[Pm3Options] = None,
+                dc: Optional[DcOptions] = None, opt: Optional[OptOptions] = None) -> DcOptResult:
+    """optimize_dc (src/optimizer.rs:277-310) on the GPU: the L-BFGS of optimize over dc_gradient;
+    raises like the reference when the first evaluation fails."""
+    params = params or Pm3Parameters.standard()
+    scf_options = scf_options or Pm3Options()
+    dc = dc or DcOptions()
+    opt = opt or OptOptions()
+    cm = molecule._c()
+    res = _COptResult()
+    sres = _CDcResult()
+    out = np.zeros((len(molecule.z), 3))
+    cap = max(int(opt.max_iter), 0) + 1
+    steps = (_COptStep * cap)()
+    traj = np.zeros(3 * cap * len(molecule.z))
+    st = native().pm3_optimize_dc(ctypes.byref(cm), params._h, ctypes.byref(scf_options._c()), ctypes.byref(dc._c()),
+                                  ctypes.byref(opt._c()), ctypes.byref(res), ctypes.byref(sres), _dp(out), steps,
+                                  _dp(traj))
+    if st == 3:
+        raise ScfNotConverged(native().pm3_last_error_message().decode(), sres.iterations, float("nan"))
+    _raise(st)
+    o = _opt_result(molecule, res, out, steps, traj)
+    scf = DcResult(None, None, bool(sres.unrestricted), sres.electronic_ev, sres.core_ev, sres.total_ev,
+                   sres.heat_of_formation_kcal, None, sres.fermi_ev, sres.fermi_beta_ev, sres.iterations,
+                   bool(sres.converged), sres.dropped_pairs, sres.largest_subsystem, sres.n_subsystems)
+    return DcOptResult(o.molecule, scf, o.converged, o.iterations, o.evaluations, o.energy_ev,
+                       o.heat_of_formation_kcal, o.max_gradient, o.first_energy_ev, o.trajectory)
 
 
 def _rust_debug_f64(x):

@@ -143,3 +143,39 @@ def test_refusals(G):
         G.run_dc(G.Molecule(z, x, 0.0, 2), options=G.Pm3Options(reference="rhf"))
     with pytest.raises(G.ScfNotConverged):
         G.run_dc(m, options=G.Pm3Options(max_scf=2))
+
+
+@pytest.mark.parametrize("charge,mult", [(0.0, 1), (1.0, 2)])
+def test_optimize_dc_follows_the_oracle(G, oracle, charge, mult):
+    """optimize_dc (src/optimizer.rs:277-310): the L-BFGS of optimize over dc_gradient, on a distorted
+    three-water chain split into three subsystems, against the oracle's restatement of the same
+    optimiser over its own dc_gradient, step by step for ten iterations: the same accepted points,
+    energies and final geometry.  (The partitioned density is not variational, so on this floppy
+    chain neither side reaches max |g| < 1e-3; the reference's docstring warns of exactly that.)"""
+    z, x = water_chain(3, spacing=3.0)
+    x = x + np.random.default_rng(5).normal(0.0, 0.05 * A, x.shape)
+    dc = dict(core_radius=3.0, buffer_radius=8.0)
+    r = G.optimize_dc(G.Molecule(z, x, charge, mult), scf_options=G.Pm3Options(**TIGHT), dc=G.DcOptions(**dc),
+                      opt=G.OptOptions(max_iter=10))
+    o = oracle.optimize_dc(z, x, charge, mult, oracle.options(**TIGHT), oracle.dc_options(**dc), max_iter=10)
+    print(f"optimize_dc q={charge}: iterations {r.iterations}/{o['iterations']} evaluations {r.evaluations} "
+          f"HoF {r.heat_of_formation_kcal:.6f}/{o['hof_kcal']:.6f} max|g| {r.max_gradient:.2e} "
+          f"subsystems {r.scf.n_subsystems}", flush=True)
+    assert r.scf.n_subsystems > 1 and r.scf.unrestricted == (mult != 1)
+    assert r.converged == o["converged"] and r.iterations == o["iterations"]
+    assert r.evaluations >= o["evaluations"]  # every solve, trials included, against gradient calls
+    assert abs(r.heat_of_formation_kcal - o["hof_kcal"]) <= 1e-5
+    assert np.abs(r.molecule.positions - o["xyz"]).max() <= 1e-5
+    assert abs(r.first_energy_ev - o["trajectory"][0]) <= 1e-6 and r.energy_ev < r.first_energy_ev
+    assert len(r.trajectory) == len(o["steps"]) == 11
+    for a, b in zip(r.trajectory, o["steps"]):
+        assert abs(a.energy_ev - b["energy_ev"]) <= 1e-6
+        assert abs(a.max_gradient - b["max_gradient"]) <= 1e-5
+        assert np.abs(a.positions - b["xyz"]).max() <= 1e-5
+    assert r.scf.total_ev == r.energy_ev
+
+
+def test_optimize_dc_first_evaluation_failure_raises(G):
+    z, x = water_chain(2)
+    with pytest.raises(G.ScfNotConverged):
+        G.optimize_dc(G.Molecule(z, x), scf_options=G.Pm3Options(max_scf=2))

@@ -171,5 +171,15 @@ def test_optimizer_defaults_and_host_checks_without_device():
     r = G._COptResult()
     assert L.pm3_optimize(ctypes.byref(m._c()), params, ctypes.byref(so), ctypes.byref(o), ctypes.byref(r),
                           out.ctypes.data_as(dp), None, None) == 1  # multiplicity below 1
+    dco = G._CDcOptions()
+    L.pm3_dc_options_default(ctypes.byref(dco))
+    sres = G._CDcResult()
+    assert L.pm3_optimize_dc(ctypes.byref(m._c()), params, ctypes.byref(so), ctypes.byref(dco), ctypes.byref(o),
+                             ctypes.byref(r), ctypes.byref(sres), out.ctypes.data_as(dp), None, None) == 1
+    m1 = G.Molecule(z, np.eye(3), 0.0, 1)
+    assert L.pm3_optimize_dc(ctypes.byref(m1._c()), params, ctypes.byref(so), ctypes.byref(dco), ctypes.byref(bad),
+                             ctypes.byref(r), ctypes.byref(sres), out.ctypes.data_as(dp), None, None) == 1
+    assert L.pm3_optimize_dc(ctypes.byref(m1._c()), params, ctypes.byref(so), ctypes.byref(dco), ctypes.byref(o),
+                             None, None, out.ctypes.data_as(dp), None, None) == 1  # no result struct
     assert L.pm3_batch_set_positions(None, xyz.ctypes.data_as(dp)) == 1
     assert L.pm3_batch_carry_densities(None, None, None) == 1
