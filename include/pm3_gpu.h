@@ -309,7 +309,8 @@ typedef struct {
   int32_t sp2_lds;            /* purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS,
                                  3 + X in LDS overwritten in place by each product,
                                  4 + the leading rows of X in LDS where the whole of X does not fit,
-                                 5 + X's upper 16x16 blocks (tiled, swizzled) in LDS, updated in place */
+                                 5 + X's upper 16x16 blocks (tiled, swizzled) in LDS, updated in place,
+                                 6 + that tiled image split between LDS and HBM where it does not fit */
   int32_t sp2_tight_n;        /* purification spectral bounds from Lanczos from this many AOs on */
   int32_t large_syrk;         /* multi-kernel path: purification products by dsyrk (1) or dgemm (0) */
   int32_t large_split;        /* multi-kernel path above 1024 AOs: 1 purification steps by the library's

@@ -773,7 +773,7 @@ pm3_status pm3_autotune(int32_t n_mol, const int32_t* atom_ptr, const int32_t* z
       {&Tuning::pair_block, {64, 128, 256}, 0},
       {&Tuning::pair_blocks_per_cu, {2, 4, 8, 16, 32}, 0},
       {&Tuning::sp2_tile16_n, {0, 32, 48, 64, 96, 128, 160, 200}, 1},
-      {&Tuning::sp2_lds, {0, 1, 2, 3, 4, 5}, 1},
+      {&Tuning::sp2_lds, {0, 1, 2, 3, 4, 5, 6}, 1},
       {&Tuning::sp2_tight_n, {24, 32, 48, 64, 96, 1 << 20}, 1},
       {&Tuning::large_syrk, {0, 1}, 2},
       {&Tuning::large_split, {0, 1, 2, 3, 4}, 2},

@@ -124,7 +124,8 @@ struct KArgs {
   int purify;                  // 1: densities by purification, frontier levels by Lanczos (no orbitals)
   int warm;                    // 1: each molecule's first SCF path starts from its density in out_P
   int sp2_lds;                 // purification operand placements allowed: 0 HBM only, 1 + LDS pair, 2 + LDS X,
-                               // 3 + LDS X overwritten in place, 4 + leading rows of X in LDS
+                               // 3 + LDS X overwritten in place, 4 + leading rows of X in LDS, 5 + the packed
+                               // tiled image in LDS, 6 + that image split between LDS and HBM
   int sp2_tight_n;             // smallest molecule (AOs) whose purification uses Lanczos spectral bounds
   int sp2_tile16_n;            // purification products on 16 x 16 wave tiles up to this size (AOs), 32 x 32 above
   double sp2_margin;           // margin of those bounds relative to the estimated width (fresh estimate)
@@ -1781,11 +1782,33 @@ __host__ __device__ constexpr bool tb_fits(int n, int lds_doubles) {
   return tb_blocks(n) * 256 <= lds_doubles && tb_blocks(n) <= TB_HOLD * NW;
 }
 
+// The packed tiled image as a whole: blocks [0, nl) at L (LDS) and, when the image does not fit,
+// blocks [nl, nblk) at H (the workgroup's HBM slab, where L2 keeps them).  Every access branches on
+// the block index (the same for a whole wave), so an LDS block is read with LDS instructions.
+struct TbImg {
+  double* L;
+  double* H;
+  int nl;
+};
+template <bool SPLIT> __device__ __forceinline__ double tb_get(const TbImg& im, int at) {
+  const int blk = at >> 8;
+  if (!SPLIT || blk < im.nl) return im.L[at];
+  return im.H[at - (im.nl << 8)];
+}
+template <bool SPLIT> __device__ __forceinline__ void tb_put(const TbImg& im, int at, double v) {
+  const int blk = at >> 8;
+  if (!SPLIT || blk < im.nl)
+    im.L[at] = v;
+  else
+    im.H[at - (im.nl << 8)] = v;
+}
+
 // One 16 x 16 tile (bi, bj) of X X on the FP64 matrix cores: out(i, j) = sum_k X(k, i) X(k, j),
 // each 16-row block of k as four v_mfma_f64_16x16x4_f64 steps, the next block's operands loaded
 // while the current one's MFMAs run.  Lane l: A value X(k, 16 bi + l % 16), B value
 // X(k, 16 bj + l % 16) with k = 16 kb + 4 u + l / 16.
-__device__ inline double4_t tb_tile(int bm, const double* X, int bi, int bj, int li, int lk) {
+template <bool SPLIT>
+__device__ inline double4_t tb_tile(int bm, const TbImg& im, int bi, int bj, int li, int lk) {
   int offd[4], offt[4];  // this lane's element of a directly stored / transposed operand block
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -1795,9 +1818,16 @@ __device__ inline double4_t tb_tile(int bm, const double* X, int bi, int bj, int
   }
   auto load = [&](int kb, int ob, double (&v)[4]) {
     const bool direct = kb <= ob;
-    const double* base = X + (direct ? (tb_index(bm, kb, ob) << 8) : (tb_index(bm, ob, kb) << 8));
+    const int blk = direct ? tb_index(bm, kb, ob) : tb_index(bm, ob, kb);
+    if (!SPLIT || blk < im.nl) {
+      const double* base = im.L + (blk << 8);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = base[direct ? offd[u] : offt[u]];
+      for (int u = 0; u < 4; ++u) v[u] = base[direct ? offd[u] : offt[u]];
+    } else {
+      const double* base = im.H + ((blk - im.nl) << 8);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = base[direct ? offd[u] : offt[u]];
+    }
   };
   double4_t c = {0.0, 0.0, 0.0, 0.0};
   double a[4], b[4];
@@ -1822,8 +1852,11 @@ __device__ inline double4_t tb_tile(int bm, const double* X, int bi, int bj, int
 // X <- X^2 (sq) or 2 X - X^2 in place on the packed tiled image: every wave computes its upper
 // tiles into registers, all waves meet once X has been read, then each tile is stored over its own
 // block (one writer per element, no mirror).  w0 / w1 receive this thread's share of tr X and
-// |X|_F^2 of the new X (off-diagonal blocks count twice).  Requires tb_fits.
-__device__ inline void tb_sym_step(int n, int bm, double* X, bool sq, double& w0, double& w1) {
+// |X|_F^2 of the new X (off-diagonal blocks count twice).  Without SPLIT this requires tb_fits;
+// with it, the tiles beyond the TB_HOLD a wave holds go through T (HBM, same layout) before the
+// meeting, and the image may extend into HBM.
+template <bool SPLIT>
+__device__ inline void tb_sym_step(int n, int bm, const TbImg& im, bool sq, double& w0, double& w1, double* T) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int nblk = bm * (bm + 1) / 2;
@@ -1835,55 +1868,86 @@ __device__ inline void tb_sym_step(int n, int bm, double* X, bool sq, double& w0
     bi_[h] = bj_[h] = 0;
     if (t < nblk) {
       pad_block<true>(t, bm, bi_[h], bj_[h]);
-      c[h] = tb_tile(bm, X, bi_[h], bj_[h], li, lk);
+      c[h] = tb_tile<SPLIT>(bm, im, bi_[h], bj_[h], li, lk);
+    }
+  }
+  if (SPLIT) {
+    for (int t = wave + TB_HOLD * NW; t < nblk; t += NW) {
+      int bi, bj;
+      pad_block<true>(t, bm, bi, bj);
+      const double4_t x2 = tb_tile<SPLIT>(bm, im, bi, bj, li, lk);
+      double* o = T + ((t - TB_HOLD * NW) << 8);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int lr = 4 * r + lk;
+        o[(lr << 4) + (li ^ lr)] = x2[r];
+      }
     }
   }
   __syncthreads();
+  auto upd = [&](int bi, int bj, int r, double x2) {
+    const int lr = 4 * r + lk;
+    const int at = (tb_index(bm, bi, bj) << 8) + (lr << 4) + (li ^ lr);
+    const double y = sq ? x2 : 2.0 * tb_get<SPLIT>(im, at) - x2;
+    tb_put<SPLIT>(im, at, y);
+    w1 += (bi < bj ? 2.0 : 1.0) * y * y;
+    if (bi == bj && lr == li) w0 += y;
+  };
 #pragma unroll
   for (int h = 0; h < TB_HOLD; ++h) {
     const int t = wave + h * NW;
     if (t < nblk) {
-      const int bi = bi_[h], bj = bj_[h];
-      double* blk = X + (tb_index(bm, bi, bj) << 8);
-      const double wt = bi < bj ? 2.0 : 1.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) upd(bi_[h], bj_[h], r, c[h][r]);
+    }
+  }
+  if (SPLIT) {
+    for (int t = wave + TB_HOLD * NW; t < nblk; t += NW) {  // this lane's own entries of T
+      int bi, bj;
+      pad_block<true>(t, bm, bi, bj);
+      const double* o = T + ((t - TB_HOLD * NW) << 8);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int lr = 4 * r + lk;
-        double* p = blk + (lr << 4) + (li ^ lr);
-        const double x2 = c[h][r];
-        const double y = sq ? x2 : 2.0 * *p - x2;
-        *p = y;
-        w1 += wt * y * y;
-        if (bi == bj && lr == li) w0 += y;
+        upd(bi, bj, r, o[(lr << 4) + (li ^ lr)]);
       }
     }
   }
 }
 
 // frontier_seed on the packed tiled image.
-__device__ void frontier_seed_tb(const double* X, int bm, int n, const Bufs& B) {
+template <bool SPLIT>
+__device__ void frontier_seed_tb(const TbImg& im, int bm, int n, const Bufs& B) {
   const int tid = threadIdx.x;
   double t = 0.0;
   if (tid < n)
-    for (int j = 0; j < n; ++j) t += X[tb_at(bm, j, tid)] * seed_r(j);
+    for (int j = 0; j < n; ++j) t += tb_get<SPLIT>(im, tb_at(bm, j, tid)) * seed_r(j);
   if (tid < n) B.ft[tid] = t;
   __syncthreads();
   double u = 0.0;
   if (tid < n)
-    for (int j = 0; j < n; ++j) u += X[tb_at(bm, j, tid)] * B.ft[j];
+    for (int j = 0; j < n; ++j) u += tb_get<SPLIT>(im, tb_at(bm, j, tid)) * B.ft[j];
   __syncthreads();
   if (tid < n) B.fs[tid] = t - u;
   if (tid == 0 && B.bnd) B.bnd[3] = 1.0;
   __syncthreads();
 }
 
+// Blocks of the packed tiled image that LDS holds for the split placement.
+__host__ __device__ constexpr int tb_lds_blocks(int n, int lds_doubles) {
+  return tb_blocks(n) < lds_doubles / 256 ? tb_blocks(n) : lds_doubles / 256;
+}
+
 // The purification loop of sp2_run on the packed tiled image (the same stopping rules; the trace
 // sums run in another order, so a step choice can differ only where two of them tie to rounding).
+// SPLIT: the blocks beyond what LDS holds live in B.Cn, and the product tiles beyond a wave's hold
+// pass through B.T (the image of a 194-AO molecule is 91 blocks, 182 KiB).
+template <bool SPLIT>
 __device__ int sp2_run_tb(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo,
                           double hi, bool seed) {
   const int n = M.n, np = M.np, nocc = M.nocc, tid = threadIdx.x;
   const int bm = (n + 15) >> 4, nimg = tb_blocks(n) * 256;
-  double* X = g_lds;
+  const TbImg im{g_lds, B.Cn, SPLIT ? tb_lds_blocks(n, M.lds_doubles) : tb_blocks(n)};
   const double s = 1.0 / (hi - lo);
   double v[2] = {0.0, 0.0};
   for (int e = tid; e < nimg; e += NT) {
@@ -1899,7 +1963,7 @@ __device__ int sp2_run_tb(const Mol& M, const Bufs& B, const double* Fu, double*
       v[1] += (br < bc ? 2.0 : 1.0) * x * x;
       if (r == c) v[0] += x;
     }
-    X[e] = x;
+    tb_put<SPLIT>(im, e, x);
   }
   block_sum<2>(v, red);
   double tr = v[0], t2 = v[1];
@@ -1909,12 +1973,12 @@ __device__ int sp2_run_tb(const Mol& M, const Bufs& B, const double* Fu, double*
     if (t2 > tr + 1.0e-9 * fmax(1.0, tr)) return -2;
     if (tail < 0 && tr - t2 < SP2_TAIL) {
       tail = SP2_EXTRA;
-      if (seed) frontier_seed_tb(X, bm, n, B);
+      if (seed) frontier_seed_tb<SPLIT>(im, bm, n, B);
     }
     if (tail == 0) break;
     const bool sq = fabs(t2 - occ) < fabs(2.0 * tr - t2 - occ);
     double w[2] = {0.0, 0.0};
-    tb_sym_step(n, bm, X, sq, w[0], w[1]);
+    tb_sym_step<SPLIT>(n, bm, im, sq, w[0], w[1], B.T);
     block_sum_fenced<2>(w, red);  // tb_sym_step's barrier follows every wave's last read of red
     tr = w[0];
     t2 = w[1];
@@ -1924,7 +1988,7 @@ __device__ int sp2_run_tb(const Mol& M, const Bufs& B, const double* Fu, double*
   if (fabs(tr - occ) > 1.0e-6) return -2;
   for (int e = tid; e < n * n; e += NT) {
     const int i = e / n, j = e % n;
-    Pdst[i * np + j] = M.wt * X[tb_at(bm, i, j)];
+    Pdst[i * np + j] = M.wt * tb_get<SPLIT>(im, tb_at(bm, i, j));
   }
   __syncthreads();
   return k;
@@ -1939,7 +2003,8 @@ constexpr double FRONTIER_SEED_DP = 1.0e-4;  // RMS density change below which p
 __device__ inline int sp2_dispatch(const Mol& M, const Bufs& B, const double* Fu, double* Pdst, double* red, double lo,
                                    double hi, bool seed) {
   const int img = sp2_img(M.n);
-  if (M.sp2_lds >= 5 && tb_fits(M.n, M.lds_doubles)) return sp2_run_tb(M, B, Fu, Pdst, red, lo, hi, seed);
+  if (M.sp2_lds >= 5 && tb_fits(M.n, M.lds_doubles)) return sp2_run_tb<false>(M, B, Fu, Pdst, red, lo, hi, seed);
+  if (M.sp2_lds >= 6) return sp2_run_tb<true>(M, B, Fu, Pdst, red, lo, hi, seed);
   if (M.sp2_lds >= 1 && 2 * img <= M.lds_doubles) return sp2_run<2>(M, B, Fu, Pdst, red, lo, hi, seed);
   if (M.sp2_lds >= 3 && img <= M.lds_doubles && hold_fits(M.n, M.n <= M.sp2_tile16_n ? 16 : 32))
     return sp2_run<3>(M, B, Fu, Pdst, red, lo, hi, seed);

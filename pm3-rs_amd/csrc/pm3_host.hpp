@@ -103,10 +103,11 @@ struct Tuning {
   int pair_block = 256;         // threads per block of the pair and gradient-pair kernels: 64, 128 or 256
   int pair_blocks_per_cu = 8;   // grid of those grid-stride kernels, blocks per CU: 1 to 64
   int sp2_tile16_n = 128;       // purification products on 16 x 16 wave tiles up to this size (AOs)
-  int sp2_lds = 5;              // purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS,
+  int sp2_lds = 6;              // purification operand placements: 0 HBM, 1 + both in LDS, 2 + X in LDS,
                                 // 3 + X in LDS overwritten in place by each product,
                                 // 4 + the leading rows of X in LDS where the whole of X does not fit,
-                                // 5 + X's upper 16 x 16 blocks (tiled, swizzled) in LDS, updated in place
+                                // 5 + X's upper 16 x 16 blocks (tiled, swizzled) in LDS, updated in place,
+                                // 6 + that tiled image split between LDS and HBM where it does not fit
   int sp2_tight_n = 48;         // purification bounds from Lanczos from this size (AOs) on
   int large_syrk = 0;           // host-driven path: purification products by dsyrk (1) or dgemm (0)
   int large_split = 1;          // host-driven path, matrices above 1024 AOs: 1 purification products by the

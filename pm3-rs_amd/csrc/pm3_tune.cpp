@@ -24,7 +24,7 @@ const Field kFields[] = {
     {"pair_block", &Tuning::pair_block, 64, 256},
     {"pair_blocks_per_cu", &Tuning::pair_blocks_per_cu, 1, 64},
     {"sp2_tile16_n", &Tuning::sp2_tile16_n, 0, 1 << 20},
-    {"sp2_lds", &Tuning::sp2_lds, 0, 5},
+    {"sp2_lds", &Tuning::sp2_lds, 0, 6},
     {"sp2_tight_n", &Tuning::sp2_tight_n, 0, 1 << 20},
     {"large_syrk", &Tuning::large_syrk, 0, 1},
     {"large_split", &Tuning::large_split, 0, 4},

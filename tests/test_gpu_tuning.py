@@ -90,6 +90,33 @@ def test_partial_lds_rows_give_the_same_bits(G, oracle):
     G.tuning_set(d)
 
 
+def test_split_tiled_image_matches_the_hbm_route(G, oracle):
+    """Molecules whose packed tiled image does not fit LDS (194 AOs: 91 blocks): the split placement
+    (leading blocks in LDS, the rest in the workgroup's HBM slab, tiles beyond a wave's hold through
+    HBM) against the all-HBM route: the same number of products and densities within 1e-13 (the
+    tiles sum in another order), and both within 1e-12 of the oracle's converged density."""
+    from test_gpu_purify import _stage
+    run = _stage(G)
+    d = G.tuning_default()
+    for seed in (9, 10, 11):
+        z, x = W.solvated_conformers(1, seed)[0]
+        z = np.asarray(z, np.int32)
+        xb = W.to_bohr(x)
+        st, o = oracle.run(z, xb, 0.0, oracle.options(p_tol=1e-10, e_tol=1e-12))
+        F = oracle.fock(z, xb, o["density"])
+        m = G.Molecule(z, xb)
+        G.tuning_set(dict(d, sp2_lds=6))
+        P6, k6, h6, l6 = run(m, F)
+        G.tuning_set(dict(d, sp2_lds=0))
+        P0, k0, h0, l0 = run(m, F)
+        print("nao", F.shape[0], "products", k6, k0, "max|dP|", np.abs(P6 - P0).max(), flush=True)
+        assert F.shape[0] == 194 and k6 == k0 > 0
+        assert np.abs(P6 - P0).max() <= 1e-13
+        assert np.abs(P6 - o["density"]).max() <= 1e-12
+        assert abs(h6 - h0) <= 1e-10 and abs(l6 - l0) <= 1e-10
+    G.tuning_set(d)
+
+
 def test_every_entry_changes_speed_only(G):
     mols = _mols(G, 64, 88)
     d = G.tuning_default()
@@ -97,7 +124,7 @@ def test_every_entry_changes_speed_only(G):
     assert np.all(s0 == 0)
     tables = [dict(d, sp2_tile16_n=1 << 20), dict(d, sp2_tile16_n=0), dict(d, pair_block=64, pair_blocks_per_cu=1),
               dict(d, pair_block=128, pair_blocks_per_cu=64), dict(d, sp2_lds=0), dict(d, sp2_lds=1),
-              dict(d, sp2_lds=3), dict(d, sp2_lds=3, sp2_tile16_n=0), dict(d, sp2_lds=4),
+              dict(d, sp2_lds=3), dict(d, sp2_lds=3, sp2_tile16_n=0), dict(d, sp2_lds=4), dict(d, sp2_lds=5),
               dict(d, sp2_tight_n=1 << 20), dict(d, sp2_tight_n=8)]
     for t in tables:
         s, r, ms = _solve(G, mols, t)

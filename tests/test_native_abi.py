@@ -81,7 +81,7 @@ def test_tuning_table_round_trip_and_validation(tmp_path):
     """The tuning table is host state: set/get/save/load and range checks need no device."""
     import pm3amd as G
     d = G.tuning_default()
-    assert d == {"pair_block": 256, "pair_blocks_per_cu": 8, "sp2_tile16_n": 128, "sp2_lds": 5, "sp2_tight_n": 48,
+    assert d == {"pair_block": 256, "pair_blocks_per_cu": 8, "sp2_tile16_n": 128, "sp2_lds": 6, "sp2_tight_n": 48,
                  "large_syrk": 0, "large_split": 1, "wg128_nao": 80, "wg256_nao": 96, "solo_large_nao": 128}
     start = G.tuning_get()
     try:
@@ -92,7 +92,7 @@ def test_tuning_table_round_trip_and_validation(tmp_path):
         G.tuning_set(d)
         G.tuning_load(path)
         assert G.tuning_get() == dict(d, pair_block=128, sp2_tile16_n=96)
-        for bad in ({"pair_block": 96}, {"pair_block": 512}, {"sp2_lds": 6}, {"pair_blocks_per_cu": 0}):
+        for bad in ({"pair_block": 96}, {"pair_block": 512}, {"sp2_lds": 7}, {"pair_blocks_per_cu": 0}):
             with pytest.raises(G.InvalidInput):
                 G.tuning_set(bad)
         assert G.tuning_get() == dict(d, pair_block=128, sp2_tile16_n=96)  # a rejected table changes nothing
