@@ -42,11 +42,13 @@ pm3_status pm3_stage_purify(const pm3_molecule* mol, const pm3_params* params, c
  * n x n matrix x (1 <= n <= 16384): y = x x (square != 0) or 2 x - x x, computed from the 64 x 64
  * tiles of the lower triangle and mirrored, with trace = tr y and norm2 = |y|_F^2 = tr y^2 summed
  * from the kernel's partials; slab = 8, 16 or 32 columns of the operand panels per LDS stage.  With
- * reps > 0 the same step runs reps more times and ms_per_launch is their mean device time.  One
+ * reps > 0 the same step runs reps more times and ms_per_launch is their mean device time.
+ * workgroups = 0 launches one workgroup per tile; a positive count caps the grid (a multiple of 8,
+ * at least 8) so that each workgroup takes several tiles, as beyond about 10 000 AOs.  One
  * product of sp2_density's loop in the large path (the reference obtains the density from
  * symmetric_eigen + density_from_coeff instead, src/scf.rs:1264, src/scf.rs:982-994). */
 pm3_status pm3_stage_sym_step(int32_t n, int32_t square, int32_t slab, const double* x, double* y, double* trace,
-                              double* norm2, int32_t reps, double* ms_per_launch);
+                              double* norm2, int32_t reps, int32_t workgroups, double* ms_per_launch);
 
 #ifdef __cplusplus
 }

@@ -957,11 +957,11 @@ pm3_status pm3_stage_gemm(int32_t n, const double* a, const double* b, double* c
 }
 
 pm3_status pm3_stage_sym_step(int32_t n, int32_t square, int32_t slab, const double* x, double* y, double* trace,
-                              double* norm2, int32_t reps, double* ms_per_launch) {
+                              double* norm2, int32_t reps, int32_t workgroups, double* ms_per_launch) {
   g_last_error.clear();
   if (!x || !y) return fail(PM3_INVALID_INPUT, "null argument");
   std::string err;
-  const int r = sym_step_stage(n, square, slab, x, y, trace, norm2, reps, ms_per_launch, err);
+  const int r = sym_step_stage(n, square, slab, x, y, trace, norm2, reps, workgroups, ms_per_launch, err);
   if (r == 2) return fail(PM3_INVALID_INPUT, err);
   if (r != 0) return fail(PM3_DEVICE_ERROR, err);
   return PM3_OK;

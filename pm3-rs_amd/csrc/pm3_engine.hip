@@ -4392,10 +4392,12 @@ __global__ __launch_bounds__(64) void lg_sp2_fused_kernel(const double* X, doubl
 // columns 8 g .. 8 g + 7, two per 16-byte read, in both operands, so each column enters once.  The
 // store writes each element of the lower triangle and its mirror, so Xn is exactly symmetric.
 // Loads beyond n read as zero, so padding rows and columns are never read.  Workgroup w computes
-// one tile, map[x * chunk + w / 8] with x = w % 8: workgroups go to the eight XCDs in turn, so each
+// the tiles map[x * chunk + p] with x = w % 8 and p = w / 8, w / 8 + g / 8, ... (g the grid, a
+// multiple of 8; one tile each when g = 8 chunk): workgroups go to the eight XCDs in turn, so each
 // XCD takes a contiguous run of the map, which lists the tiles by 8 x 8 super-blocks; the tiles an
 // XCD has in flight then share few panels, and its L2 serves the rest.  Workgroup w leaves the
-// trace and norm partials of its tile in red[2 w], red[2 w + 1] (zeros when it has none).
+// trace and norm partials of its tiles, summed in that order, in red[2 w], red[2 w + 1] (zeros when
+// it has none).
 constexpr int SY_T = 64, SY_THREADS = 256;
 
 // Slot of 16-byte column chunk c of panel row `row` in a slab of K columns (K / 2 chunks a row):
@@ -4425,9 +4427,11 @@ __global__ __launch_bounds__(SY_THREADS) void lg_sp2_sym_kernel(const double* X,
   const int li = lane & 15, lg = lane >> 4;
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;  // the wave's quarter of the tile
   const int ntile = sy_tiles(n), nk = (n + K - 1) / K;
-  const int slot = (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+  const int per = (int)(gridDim.x >> 3);  // workgroups of each XCD
   double tr = 0.0, fr = 0.0;
-  if ((int)(blockIdx.x >> 3) < chunk && slot < ntile) {
+  for (int pos = (int)(blockIdx.x >> 3); pos < chunk; pos += per) {
+    const int slot = (int)(blockIdx.x & 7) * chunk + pos;
+    if (slot >= ntile) break;
     const int I = map[slot] >> 16, J = map[slot] & 0xffff;
     const bool diag = I == J;  // one panel serves both operands
     const int np_ = diag ? 1 : 2;
@@ -5505,12 +5509,10 @@ static bool large_purify_run(Batch& b, const MolDesc& d, const double* A, double
   // cost less than the host's round trip); the tile kernel's first look comes two steps before
   // the step count of the last purification, then one step per look; the rest look every step.
   const int per0 = fused ? LARGE_STEPS_PER_LOOK : own ? std::max(1, L.sp2_steps - 2) : 1;
-  // one workgroup per tile (the dispatcher balances them over the CUs), two partials each in L.red
-  const int schunk = own ? (sy_tiles(n) + 7) / 8 : 0, sgrid = 8 * schunk;
-  if (own && sgrid > LG_BLOCKS * LG_MAXDOT / 2) {
-    err = "internal: too many product tiles";
-    return false;
-  }
+  // one workgroup per tile (the dispatcher balances them over the CUs), two partials each in L.red;
+  // above about 10 000 AOs the grid is capped by L.red and workgroups take several tiles
+  const int schunk = own ? (sy_tiles(n) + 7) / 8 : 0;
+  const int sgrid = std::min(8 * schunk, (LG_BLOCKS * LG_MAXDOT / 2) & ~7);
   if (own && L.sy_map_n != n) {
     if (!sy_tile_map(b, n, err)) return false;
   }
@@ -7190,13 +7192,16 @@ bool engine_run(const Params& P, const std::vector<MolIn>& mols, const EngineOpt
 // with tr y and |y|_F^2 summed from the kernel's partials.  The device copy of x has NaN in its
 // padding and y starts as NaN, so a read beyond n or an element left unwritten shows.  With
 // reps > 0 the same step is launched reps more times between two events (ms per launch).
+// workgroups > 0 caps the grid (rounded down to a multiple of 8, at least 8), so that
+// workgroups take several tiles each as they do beyond about 10 000 AOs.
 int sym_step_stage(int n, int square, int slab, const double* x, double* y, double* trace, double* norm2, int reps,
-                   double* ms, std::string& err) {
-  if (n < 1 || n > 16384 || (slab != 8 && slab != 16 && slab != 32) || reps < 0) {
-    err = "n must be in [1, 16384], slab 8, 16 or 32, reps >= 0";
+                   int workgroups, double* ms, std::string& err) {
+  if (n < 1 || n > 16384 || (slab != 8 && slab != 16 && slab != 32) || reps < 0 || workgroups < 0) {
+    err = "n must be in [1, 16384], slab 8, 16 or 32, reps >= 0, workgroups >= 0";
     return 2;
   }
-  const int np = (n + 15) & ~15, chunk = (sy_tiles(n) + 7) / 8, grid = 8 * chunk;
+  const int np = (n + 15) & ~15, chunk = (sy_tiles(n) + 7) / 8;
+  const int grid = workgroups > 0 ? std::min(8 * chunk, std::max(8, workgroups & ~7)) : 8 * chunk;
   const size_t m2 = (size_t)np * np;
   double *dx = nullptr, *dy = nullptr, *dred = nullptr, *dst = nullptr;
   int* dmap = nullptr;

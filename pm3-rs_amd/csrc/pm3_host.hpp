@@ -166,10 +166,10 @@ void batch_guess_timing(const Batch* b, double& features_ms, double& network_ms,
 bool batch_phase_ticks(const Batch* b, unsigned long long* out, std::string& err);
 
 // One step of the large path's purification tile kernel on a host n x n matrix: y = x x (square)
-// or 2 x - x x, tr y and |y|_F^2; reps more launches timed (ms per launch).  Returns 0, 2 (bad
-// argument) or 6 (device error), with err set.
+// or 2 x - x x, tr y and |y|_F^2; reps more launches timed (ms per launch); workgroups > 0 caps
+// the grid.  Returns 0, 2 (bad argument) or 6 (device error), with err set.
 int sym_step_stage(int n, int square, int slab, const double* x, double* y, double* trace, double* norm2, int reps,
-                   double* ms, std::string& err);
+                   int workgroups, double* ms, std::string& err);
 
 // One stage of the pipeline on a single molecule (kernel-level parity hooks); see stage_kernel.
 bool stage_run(const Params& P, const MolIn& mol, int mode, const std::vector<double>& in, std::vector<double>& out,

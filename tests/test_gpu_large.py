@@ -182,15 +182,16 @@ def _sym_step(G):
     import ctypes
     L = G.native()
     dp = ctypes.POINTER(ctypes.c_double)
-    L.pm3_stage_sym_step.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp, dp, dp, dp, ctypes.c_int32, dp]
+    L.pm3_stage_sym_step.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, dp, dp, dp, dp, ctypes.c_int32,
+                                     ctypes.c_int32, dp]
 
-    def run(x, square, slab, reps=0):
+    def run(x, square, slab, reps=0, workgroups=0):
         n = x.shape[0]
         x = np.ascontiguousarray(x)
         y = np.zeros((n, n))
         tr, f2, ms = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         st = L.pm3_stage_sym_step(n, square, slab, x.ctypes.data_as(dp), y.ctypes.data_as(dp), ctypes.byref(tr),
-                                  ctypes.byref(f2), reps, ctypes.byref(ms))
+                                  ctypes.byref(f2), reps, workgroups, ctypes.byref(ms))
         assert st == 0, G.native().pm3_last_error_message()
         return y, tr.value, f2.value, ms.value
     return run
@@ -220,6 +221,10 @@ def test_tile_kernel_step_matches_numpy_at_ragged_sizes(G, slab):
             assert abs(f2 - np.sum(ref * ref)) <= 1e-12 * max(1.0, np.sum(ref * ref)), n
             y2, tr2, f22, _ = run(x, square, slab)
             assert np.array_equal(y, y2) and tr == tr2 and f22 == f2, n
+            # a capped grid (each workgroup several tiles): the same elements, bit for bit
+            y3, tr3, f23, _ = run(x, square, slab, workgroups=16)
+            assert np.array_equal(y, y3), n
+            assert abs(tr3 - tr) <= 1e-12 * max(1.0, abs(tr)) and abs(f23 - f2) <= 1e-12 * max(1.0, f2), n
         print("n", n, "slab", slab, "max rel err", err, flush=True)
 
 
