@@ -60,9 +60,10 @@ HBM_PEAK_GBS = 8000.0
 HOF_TOL = 1.0e-5
 GRAD_TOL = 2.295e-7  # eV/Bohr, max |dg| against the CPU path
 # Final heats of formation of two optimisations of one molecule: both stop at max |g| < 1e-3
-# eV/Bohr, and on soft modes (methyl torsions, intermolecular modes) that leaves up to about 1e-3
-# kcal/mol between two stopping points once the trajectories have parted.
-OPT_HOF_TOL = 1.0e-3
+# eV/Bohr, and on soft modes (methyl torsions, intermolecular modes) two stopping points can lie
+# a few 1e-3 kcal/mol apart once the trajectories have parted.  The reference's own optimiser test
+# accepts an optimised heat of formation within 0.005 kcal/mol (src/optimizer.rs:384).
+OPT_HOF_TOL = 5.0e-3
 
 
 def cpu_info(threads):
