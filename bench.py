@@ -262,8 +262,8 @@ def bench_dc(args):
                                   "chemical potential by device bisection, Yang-Lee assembly and density DIIS kernels"},
         "sizes": out_sizes,
         "roofline": None,
-        "roofline_note": "the time goes to the batched subsystem eigensolve (rocSOLVER, subsystems padded to the "
-                         "largest); the kernel trace is in profiles/dc_water.md",
+        "roofline_note": "the time goes to the batched subsystem eigensolves (rocSOLVER, one call per size "
+                         "bucket); the kernel trace is in profiles/dc_water.md",
         "cpu_baseline": cpu,
     }
 

@@ -7,14 +7,20 @@
 // ever receives pattern entries, and only pattern entries of a Fock matrix are read), so the
 // dense Fock build equals the reference's pattern-restricted one (build_fock_spin_sparse,
 // src/fock.rs:359-442) on every entry that is used.  Per iteration:
-//   Fock build(s) -> every subsystem's Fock block gathered into one padded batch -> batched
-//   symmetric eigensolve (rocSOLVER, strided batched) -> core weights of each eigenvector ->
-//   the global chemical potential bisected by one workgroup -> occupied blocks V f V^T as one
-//   batched product -> Yang-Lee weighted assembly, subsystem by subsystem in order -> energy,
-//   change, damping and the density DIIS on the pattern entries (src/densitydiis.rs:34-119).
-// Padding: subsystem s of n_s orbitals sits in an nmax x nmax block whose extra diagonal carries
-// a value above every eigenvalue of the Fock matrix (its Gershgorin bound + 1 eV), so the first n_s
-// eigenpairs (ascending) are the subsystem's own and the rest are unit vectors on the padding.
+//   Fock build(s) -> every subsystem's Fock block gathered into its size bucket's padded batch ->
+//   batched symmetric eigensolve per bucket (rocSOLVER, strided batched) -> core weights of each
+//   eigenvector -> the global chemical potential bisected by one workgroup -> occupied blocks
+//   V f V^T as one batched product per bucket -> Yang-Lee weighted assembly, subsystem by
+//   subsystem in the partition's order -> energy, change, damping and the density DIIS on the
+//   pattern entries (src/densitydiis.rs:34-119).
+// Buckets: subsystems are grouped by their size rounded up to BUCKET orbitals (capped at the
+// largest subsystem), and neighbouring groups merged where the latency of another batched call
+// costs more than the padding it saves (LATENCY), so the eigensolves and products, cubic in the
+// padded size, run at a fraction of the volume of one batch padded to the largest subsystem.
+// Padding: subsystem s of n_s orbitals sits in an nb x nb block (nb its bucket's size) whose extra
+// diagonal carries a value above every eigenvalue of the Fock matrix (its Gershgorin bound + 1 eV),
+// so the first n_s eigenpairs (ascending) are the subsystem's own and the rest are unit vectors on
+// the padding.
 
 namespace dc {
 
@@ -124,6 +130,11 @@ __host__ __device__ inline double fermi_dirac(double e, double mu, double s) {  
 constexpr int THREADS = 256;
 constexpr int BLOCKS = 1024;
 constexpr int MU_THREADS = 1024;
+constexpr int BUCKET = 64;  // subsystem sizes are rounded up to a multiple of this for the batches
+// Cost of one batched eigensolve and product in units of one padded block's nb^3: LATENCY nb +
+// count nb^3.  The latency term (the solver's n sequential reduction steps, whatever the batch)
+// is fitted to single-batch solves of 64- and 512-water clusters (51 x 319 and 348 x 587 AOs).
+constexpr double LATENCY = 1.0e7;
 
 __device__ inline int gt() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ inline int gs() { return gridDim.x * blockDim.x; }
@@ -147,34 +158,36 @@ __global__ __launch_bounds__(THREADS) void keep_pattern_kernel(const double* src
   for (long long e = gt(); e < nnz; e += gs()) dst[at[e]] = src[at[e]];
 }
 
-// Every subsystem's Fock block into its padded nmax x nmax slot of the batch (column-major; the
-// block is symmetric).
+// Every subsystem of one bucket into its padded nb x nb slot of the bucket's batch (column-major;
+// the block is symmetric).  orb and nsz point at the bucket's first subsystem; the orbital table
+// has ostride entries per subsystem.
 __global__ __launch_bounds__(THREADS) void gather_kernel(const double* F, int np, const int* orb, const int* nsz,
-                                                         int nsub, int nmax, double pad, double* Fs) {
-  const long long per = (long long)nmax * nmax, tot = per * nsub;
+                                                         int nsub, int nb, int ostride, double pad, double* Fs) {
+  const long long per = (long long)nb * nb, tot = per * nsub;
   for (long long e = gt(); e < tot; e += gs()) {
-    const int s = (int)(e / per), r = (int)(e % per), i = r % nmax, j = r / nmax, n = nsz[s];
+    const int s = (int)(e / per), r = (int)(e % per), i = r % nb, j = r / nb, n = nsz[s];
     double v = 0.0;
-    if (i < n && j < n) v = F[(long long)orb[s * nmax + i] * np + orb[s * nmax + j]];
+    if (i < n && j < n) v = F[(long long)orb[(long long)s * ostride + i] * np + orb[(long long)s * ostride + j]];
     else if (i == j) v = pad;
     Fs[e] = v;
   }
 }
 
 // q[s][k] = sum over the core rows i of V[s](i, k)^2 (the eigenvector's electron count at full
-// occupation), k < n_s; the padding's columns get 0.
+// occupation), k < n_s; the padding's columns get 0.  One bucket: V holds its nb x nb blocks,
+// core and q have ostride entries per subsystem.
 __global__ __launch_bounds__(THREADS) void core_weight_kernel(const double* V, const int* nsz, const char* core,
-                                                              int nsub, int nmax, double* q) {
-  const long long tot = (long long)nsub * nmax;
+                                                              int nsub, int nb, int ostride, double* q) {
+  const long long tot = (long long)nsub * nb;
   for (long long e = gt(); e < tot; e += gs()) {
-    const int s = (int)(e / nmax), k = (int)(e % nmax), n = nsz[s];
+    const int s = (int)(e / nb), k = (int)(e % nb), n = nsz[s];
     double t = 0.0;
     if (k < n) {
-      const double* col = V + (long long)s * nmax * nmax + (long long)k * nmax;
+      const double* col = V + (long long)s * nb * nb + (long long)k * nb;
       for (int i = 0; i < n; ++i)
-        if (core[s * nmax + i]) t += col[i] * col[i];
+        if (core[(long long)s * ostride + i]) t += col[i] * col[i];
     }
-    q[e] = t;
+    q[(long long)s * ostride + k] = t;
   }
 }
 
@@ -236,25 +249,26 @@ __global__ __launch_bounds__(MU_THREADS) void mu_kernel(const double* w, const d
 }
 
 // Y[s](:, k) = f_k V[s](:, k) with f_k the Fermi-Dirac occupation at mu (0 where the reference
-// skips f <= 1e-14, and for the padding).
+// skips f <= 1e-14, and for the padding).  One bucket: V and Y hold its nb x nb blocks, w has
+// ostride entries per subsystem.
 __global__ __launch_bounds__(THREADS) void occupy_kernel(const double* V, const double* w, const int* nsz, int nsub,
-                                                         int nmax, int kcols, const double* mu, double smear, double* Y) {
-  const long long per = (long long)nmax * kcols, tot = per * nsub;
+                                                         int nb, int ostride, const double* mu, double smear, double* Y) {
+  const long long per = (long long)nb * nb, tot = per * nsub;
   const double m = mu[0];
   for (long long e = gt(); e < tot; e += gs()) {
-    const int s = (int)(e / per), r = (int)(e % per), i = r % nmax, k = r / nmax;
+    const int s = (int)(e / per), r = (int)(e % per), k = r / nb;
     double f = 0.0;
     if (k < nsz[s]) {
-      f = fermi_dirac(w[(long long)s * nmax + k], m, smear);
+      f = fermi_dirac(w[(long long)s * ostride + k], m, smear);
       if (f <= 1.0e-14) f = 0.0;
     }
-    Y[e] = f * V[(long long)s * nmax * nmax + (long long)k * nmax + i];
+    Y[e] = f * V[e];
   }
 }
 
 // Subsystem s's block of V f V^T added to the dense density with the Yang-Lee weights (1 core-core,
 // 1/2 core-buffer, 0 buffer-buffer, src/dc/partition.rs:161-169); one launch per subsystem, in order.
-__global__ __launch_bounds__(THREADS) void assemble_kernel(const double* C, int nmax, int n, const int* orb,
+__global__ __launch_bounds__(THREADS) void assemble_kernel(const double* C, int ld, int n, const int* orb,
                                                            const char* core, double* P, int np) {
   const long long tot = (long long)n * n;
   for (long long e = gt(); e < tot; e += gs()) {
@@ -262,7 +276,7 @@ __global__ __launch_bounds__(THREADS) void assemble_kernel(const double* C, int 
     const bool ci = core[i], cj = core[j];
     const double wt = ci && cj ? 1.0 : (ci || cj) ? 0.5 : 0.0;
     if (wt == 0.0) continue;
-    P[(long long)orb[i] * np + orb[j]] += wt * C[(long long)j * nmax + i];
+    P[(long long)orb[i] * np + orb[j]] += wt * C[(long long)j * ld + i];
   }
 }
 
@@ -410,18 +424,76 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
   out.n_subsystems = nsub;
   out.nao = n;
   out.unrestricted = uhf ? 1 : 0;
-  // subsystem tables (engine AOs, padded to nmax)
+  // size buckets: each subsystem's size rounded up to BUCKET (at most nmax), then consecutive
+  // rounded sizes merged where one batch at the larger size costs less than two calls
+  // (dc::LATENCY; least total cost over every grouping, by dynamic programming); nb is the
+  // bucket's padded size.  The batch order sorts the subsystems by nb (stably), so each bucket is
+  // one contiguous run.
+  std::vector<int> bsize(nsub), order(nsub), slot_of(nsub);
+  for (int s = 0; s < nsub; ++s)
+    bsize[s] = std::min(nmax, ((int)part.orb[s].size() + dc::BUCKET - 1) / dc::BUCKET * dc::BUCKET);
+  {
+    std::vector<int> sizes(bsize);
+    std::sort(sizes.begin(), sizes.end());
+    std::vector<int> cand, cnt;  // distinct rounded sizes and their subsystem counts
+    for (int v : sizes) {
+      if (cand.empty() || cand.back() != v) {
+        cand.push_back(v);
+        cnt.push_back(0);
+      }
+      ++cnt.back();
+    }
+    const int m = (int)cand.size();
+    std::vector<double> best(m + 1, 0.0);
+    std::vector<int> from(m + 1, 0);
+    for (int j = 1; j <= m; ++j) {  // best[j]: the first j sizes grouped, the last group ending at j - 1
+      const double nb = cand[j - 1];
+      best[j] = INFINITY;
+      long long c = 0;
+      for (int i = j; i >= 1; --i) {
+        c += cnt[i - 1];
+        const double v = best[i - 1] + dc::LATENCY * nb + (double)c * nb * nb * nb;
+        if (v < best[j]) {
+          best[j] = v;
+          from[j] = i - 1;
+        }
+      }
+    }
+    std::vector<int> to(m);  // rounded size -> its group's padded size
+    for (int j = m; j > 0; j = from[j])
+      for (int i = from[j]; i < j; ++i) to[i] = cand[j - 1];
+    for (int s = 0; s < nsub; ++s)
+      bsize[s] = to[std::lower_bound(cand.begin(), cand.end(), bsize[s]) - cand.begin()];
+  }
+  for (int s = 0; s < nsub; ++s) order[s] = s;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return bsize[a] < bsize[c]; });
+  struct Bucket {
+    int nb, first, count;
+    long long off;  // offset of its first nb x nb block in the batch buffers
+  };
+  std::vector<Bucket> buckets;
+  std::vector<long long> boff(nsub);  // batch offset of each slot's block
+  long long per_total = 0;
+  for (int t = 0; t < nsub; ++t) {
+    const int s = order[t], nb = bsize[s];
+    slot_of[s] = t;
+    if (buckets.empty() || buckets.back().nb != nb) buckets.push_back({nb, t, 0, per_total});
+    ++buckets.back().count;
+    boff[t] = per_total;
+    per_total += (long long)nb * nb;
+  }
+  // subsystem tables in batch order (engine AOs, nmax entries per subsystem)
   std::vector<int> horb((size_t)nsub * nmax, 0), hsz(nsub);
   std::vector<char> hcore((size_t)nsub * nmax, 0);
-  for (int s = 0; s < nsub; ++s) {
-    hsz[s] = (int)part.orb[s].size();
-    for (int i = 0; i < hsz[s]; ++i) {
-      horb[(size_t)s * nmax + i] = eng_of[part.orb[s][i]];
-      hcore[(size_t)s * nmax + i] = part.in_core[s][i];
+  for (int t = 0; t < nsub; ++t) {
+    const int s = order[t];
+    hsz[t] = (int)part.orb[s].size();
+    for (int i = 0; i < hsz[t]; ++i) {
+      horb[(size_t)t * nmax + i] = eng_of[part.orb[s][i]];
+      hcore[(size_t)t * nmax + i] = part.in_core[s][i];
     }
   }
   // device buffers
-  const long long per = (long long)nmax * nmax;
   double *Pa, *Pb, *Pt, *Fa, *Fb, *fa, *fb, *nt, *Fs, *Ws, *Ew, *Qw, *Ys, *Cs, *mu;
   int *dorb, *dsz, *dinfo;
   char* dcore;
@@ -435,9 +507,9 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
   double *Hd, *Hr;
   if (!b.alloc(Pa, m2, err) || !b.alloc(Pb, m2, err) || !b.alloc(Pt, m2, err) || !b.alloc(Fa, m2, err) ||
       !b.alloc(Fb, m2, err) || !b.alloc(fa, m2, err) || !b.alloc(fb, m2, err) || !b.alloc(nt, m2, err) ||
-      !b.alloc(Fs, per * nsub, err) || !b.alloc(Ws, (long long)nmax * nsub, err) ||
+      !b.alloc(Fs, per_total, err) || !b.alloc(Ws, (long long)nmax * nsub, err) ||
       !b.alloc(Ew, (long long)nmax * nsub, err) || !b.alloc(Qw, (long long)nmax * nsub, err) ||
-      !b.alloc(Ys, per * nsub, err) || !b.alloc(Cs, per * nsub, err) || !b.alloc(mu, 2, err) ||
+      !b.alloc(Ys, per_total, err) || !b.alloc(Cs, per_total, err) || !b.alloc(mu, 2, err) ||
       !b.alloc(dorb, (long long)nsub * nmax, err) || !b.alloc(dsz, nsub, err) || !b.alloc(dinfo, nsub, err) ||
       !b.alloc(dcore, (long long)nsub * nmax, err) || !b.alloc(dat, std::max(1LL, nnz), err) ||
       !b.alloc(Hd, 2 * nnz * depth, err) || !b.alloc(Hr, 2 * nnz * depth, err))
@@ -473,7 +545,6 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
   // alpha matrix only (the beta channel reads it)
   auto fock = [&](double* F, const double* Pspin) { return dc_fock(b, d, Pt, Pspin, 1.0, F, err); };
   // one spin channel: subsystem eigenpairs, the chemical potential, the assembled density
-  int kcols = nmax;
   auto solve_channel = [&](const double* F, double target, double* dst, double& mu_out) -> bool {
     hipLaunchKernelGGL(lg_gershgorin_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, F, n, np, b.lg.red);
     HIP_OK(hipGetLastError());
@@ -481,45 +552,54 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
                  hipMemcpyDeviceToHost));
     double ghi = -1.0e300;
     for (int blk = 0; blk < LG_BLOCKS; ++blk) ghi = std::max(ghi, b.h_red[(size_t)blk * LG_MAXDOT + 1]);
-    hipLaunchKernelGGL(dc::gather_kernel, G, T, 0, b.stream, F, np, (const int*)dorb, (const int*)dsz, nsub, nmax,
-                       ghi + 1.0, Fs);
-    HIP_OK(hipGetLastError());
-    BLAS_OK(rocsolver_dsyevd_strided_batched(b.blas, rocblas_evect_original, rocblas_fill_upper, nmax, Fs, nmax, per,
-                                             Ws, nmax, Ew, nmax, dinfo, nsub));
+    for (const Bucket& k : buckets) {
+      const long long kper = (long long)k.nb * k.nb;
+      hipLaunchKernelGGL(dc::gather_kernel, G, T, 0, b.stream, F, np, (const int*)(dorb + (long long)k.first * nmax),
+                         (const int*)(dsz + k.first), k.count, k.nb, nmax, ghi + 1.0, Fs + k.off);
+      HIP_OK(hipGetLastError());
+      BLAS_OK(rocsolver_dsyevd_strided_batched(b.blas, rocblas_evect_original, rocblas_fill_upper, k.nb, Fs + k.off,
+                                               k.nb, kper, Ws + (long long)k.first * nmax, nmax,
+                                               Ew + (long long)k.first * nmax, nmax, dinfo + k.first, k.count));
+    }
     std::vector<int> info(nsub);
     HIP_OK(scopy(b.stream, info.data(), dinfo, sizeof(int) * nsub, hipMemcpyDeviceToHost));
-    for (int s = 0; s < nsub; ++s)
-      if (info[s] != 0) {
-        err = "rocsolver_dsyevd_strided_batched did not converge on subsystem " + std::to_string(s);
+    for (int t = 0; t < nsub; ++t)
+      if (info[t] != 0) {
+        err = "rocsolver_dsyevd_strided_batched did not converge on subsystem " + std::to_string(order[t]);
         return false;
       }
-    hipLaunchKernelGGL(dc::core_weight_kernel, G, T, 0, b.stream, (const double*)Fs, (const int*)dsz,
-                       (const char*)dcore, nsub, nmax, Qw);
+    for (const Bucket& k : buckets)
+      hipLaunchKernelGGL(dc::core_weight_kernel, G, T, 0, b.stream, (const double*)(Fs + k.off),
+                         (const int*)(dsz + k.first), (const char*)(dcore + (long long)k.first * nmax), k.count, k.nb,
+                         nmax, Qw + (long long)k.first * nmax);
     hipLaunchKernelGGL(dc::mu_kernel, dim3(1), dim3(dc::MU_THREADS), 0, b.stream, (const double*)Ws, (const double*)Qw,
                        (const int*)dsz, nsub, nmax, target, dco.smearing_ev, dco.electron_tol, mu);
-    hipLaunchKernelGGL(dc::occupy_kernel, G, T, 0, b.stream, (const double*)Fs, (const double*)Ws, (const int*)dsz,
-                       nsub, nmax, kcols, (const double*)mu, dco.smearing_ev, Ys);
     HIP_OK(hipGetLastError());
-    // C_s = Y_s V_s^T (column-major nmax x nmax, the k sum over the occupied columns)
+    // C_s = Y_s V_s^T (column-major nb x nb per subsystem, the k sum over every column)
     const double one = 1.0, zero = 0.0;
-    BLAS_OK(rocblas_dgemm_strided_batched(b.blas, rocblas_operation_none, rocblas_operation_transpose, nmax, nmax,
-                                          kcols, &one, Ys, nmax, (long long)nmax * kcols, Fs, nmax, per, &zero, Cs,
-                                          nmax, per, nsub));
+    for (const Bucket& k : buckets) {
+      const long long kper = (long long)k.nb * k.nb;
+      hipLaunchKernelGGL(dc::occupy_kernel, G, T, 0, b.stream, (const double*)(Fs + k.off),
+                         (const double*)(Ws + (long long)k.first * nmax), (const int*)(dsz + k.first), k.count, k.nb,
+                         nmax, (const double*)mu, dco.smearing_ev, Ys + k.off);
+      HIP_OK(hipGetLastError());
+      BLAS_OK(rocblas_dgemm_strided_batched(b.blas, rocblas_operation_none, rocblas_operation_transpose, k.nb, k.nb,
+                                            k.nb, &one, Ys + k.off, k.nb, kper, Fs + k.off, k.nb, kper, &zero,
+                                            Cs + k.off, k.nb, kper, k.count));
+    }
     HIP_OK(hipMemsetAsync(dst, 0, sizeof(double) * m2, b.stream));
-    for (int s = 0; s < nsub; ++s) {
-      const long long nn = (long long)hsz[s] * hsz[s];
+    for (int s = 0; s < nsub; ++s) {  // the partition's order
+      const int t = slot_of[s];
+      const long long nn = (long long)hsz[t] * hsz[t];
       const int blocks = (int)std::min<long long>(dc::BLOCKS, (nn + dc::THREADS - 1) / dc::THREADS);
-      hipLaunchKernelGGL(dc::assemble_kernel, dim3(std::max(1, blocks)), T, 0, b.stream, (const double*)(Cs + per * s),
-                         nmax, hsz[s], (const int*)(dorb + (long long)s * nmax), (const char*)(dcore + (long long)s * nmax),
-                         dst, np);
+      hipLaunchKernelGGL(dc::assemble_kernel, dim3(std::max(1, blocks)), T, 0, b.stream, (const double*)(Cs + boff[t]),
+                         bsize[s], hsz[t], (const int*)(dorb + (long long)t * nmax),
+                         (const char*)(dcore + (long long)t * nmax), dst, np);
     }
     HIP_OK(hipGetLastError());
     HIP_OK(scopy(b.stream, &mu_out, mu, sizeof(double), hipMemcpyDeviceToHost));
     return true;
   };
-  // occupied columns: the largest number of eigenpairs of any subsystem that the reference keeps
-  // (f > 1e-14) is not known before the first solve, so the products take every column
-  kcols = nmax;
   auto dots = [&](std::initializer_list<std::pair<const double*, const double*>> xy, double* res) {
     Dots D{};
     D.k = 0;
