@@ -339,6 +339,25 @@ static bool dc_fock(Batch& b, const MolDesc& d, const double* P, const double* X
   return true;
 }
 
+// Status-returning checks for dc_run's own statements (its lambdas return bool and use HIP_OK):
+// a HIP or rocBLAS failure there is status 6 with the message in err.
+#define DC_HIP(x)                                           \
+  do {                                                      \
+    hipError_t e_ = (x);                                    \
+    if (e_ != hipSuccess) {                                 \
+      err = std::string(#x) + ": " + hipGetErrorString(e_); \
+      return 6;                                             \
+    }                                                       \
+  } while (0)
+#define DC_BLAS(x)                                                 \
+  do {                                                             \
+    rocblas_status s_ = (x);                                       \
+    if (s_ != rocblas_status_success) {                            \
+      err = std::string(#x) + ": " + rocblas_status_to_string(s_); \
+      return 6;                                                    \
+    }                                                              \
+  } while (0)
+
 int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const DcOptionsHost& dco, DcResultHost& out,
            std::string& err, bool want_gradient) {
   out = DcResultHost();
@@ -514,16 +533,16 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
       !b.alloc(dcore, (long long)nsub * nmax, err) || !b.alloc(dat, std::max(1LL, nnz), err) ||
       !b.alloc(Hd, 2 * nnz * depth, err) || !b.alloc(Hr, 2 * nnz * depth, err))
     return 6;
-  HIP_OK(scopy(b.stream, dorb, horb.data(), sizeof(int) * horb.size(), hipMemcpyHostToDevice));
-  HIP_OK(scopy(b.stream, dsz, hsz.data(), sizeof(int) * hsz.size(), hipMemcpyHostToDevice));
-  HIP_OK(scopy(b.stream, dcore, hcore.data(), hcore.size(), hipMemcpyHostToDevice));
-  HIP_OK(scopy(b.stream, dat, at.data(), sizeof(long long) * at.size(), hipMemcpyHostToDevice));
+  DC_HIP(scopy(b.stream, dorb, horb.data(), sizeof(int) * horb.size(), hipMemcpyHostToDevice));
+  DC_HIP(scopy(b.stream, dsz, hsz.data(), sizeof(int) * hsz.size(), hipMemcpyHostToDevice));
+  DC_HIP(scopy(b.stream, dcore, hcore.data(), hcore.size(), hipMemcpyHostToDevice));
+  DC_HIP(scopy(b.stream, dat, at.data(), sizeof(long long) * at.size(), hipMemcpyHostToDevice));
   // pair tables and H_core
-  HIP_OK(hipMemsetAsync(b.args.queue, 0, 3 * sizeof(int), b.stream));
+  DC_HIP(hipMemsetAsync(b.args.queue, 0, 3 * sizeof(int), b.stream));
   hipLaunchKernelGGL(pair_kernel, dim3(b.pair_blocks), dim3(b.pair_block), 0, b.stream, b.args);
-  HIP_OK(hipGetLastError());
+  DC_HIP(hipGetLastError());
   hipLaunchKernelGGL(lg_core_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, b.args, d, b.lg.red);
-  HIP_OK(hipGetLastError());
+  DC_HIP(hipGetLastError());
   double ecore = 0.0;
   if (!large_sum1(b, ecore, err)) return 6;
   const dim3 G(dc::BLOCKS), T(dc::THREADS);
@@ -531,19 +550,42 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
   // read onto the pattern (src/dc/scf.rs:431-433)
   const double tot_e = na + nb;
   auto share = [&](double e) { return tot_e > 0.0 ? e / tot_e : 0.5; };
-  HIP_OK(hipMemsetAsync(nt, 0, sizeof(double) * m2, b.stream));
+  DC_HIP(hipMemsetAsync(nt, 0, sizeof(double) * m2, b.stream));
   hipLaunchKernelGGL(lg_sad_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, b.args, d, nt, -1);
-  HIP_OK(hipMemsetAsync(Pa, 0, sizeof(double) * m2, b.stream));
-  HIP_OK(hipMemsetAsync(Pb, 0, sizeof(double) * m2, b.stream));
+  DC_HIP(hipMemsetAsync(Pa, 0, sizeof(double) * m2, b.stream));
+  DC_HIP(hipMemsetAsync(Pb, 0, sizeof(double) * m2, b.stream));
   hipLaunchKernelGGL(dc::scale_kernel, G, T, 0, b.stream, (const double*)nt, Fa, m2, share(na));
   hipLaunchKernelGGL(dc::keep_pattern_kernel, G, T, 0, b.stream, (const double*)Fa, (const long long*)dat, nnz, Pa);
   hipLaunchKernelGGL(dc::scale_kernel, G, T, 0, b.stream, (const double*)nt, Fa, m2, share(nb));
   hipLaunchKernelGGL(dc::keep_pattern_kernel, G, T, 0, b.stream, (const double*)Fa, (const long long*)dat, nnz, Pb);
-  HIP_OK(hipGetLastError());
+  DC_HIP(hipGetLastError());
   const double damping = opt.damping > 0.0 ? std::min(std::max(opt.damping, 0.0), 0.95) : 0.3;
   // Fock matrices of one channel: F = H + J(P_total) - K(P_spin); a restricted run builds the
   // alpha matrix only (the beta channel reads it)
   auto fock = [&](double* F, const double* Pspin) { return dc_fock(b, d, Pt, Pspin, 1.0, F, err); };
+  // one stream and solver handle per bucket after the first (default priority, non-blocking)
+  if (!b.dc_fork) DC_HIP(hipEventCreateWithFlags(&b.dc_fork, hipEventDisableTiming));
+  while (b.dc_lanes.size() + 1 < buckets.size()) {
+    b.dc_lanes.emplace_back();
+    Batch::DcLane& l = b.dc_lanes.back();
+    DC_HIP(hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking));
+    DC_HIP(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+    DC_BLAS(rocblas_create_handle(&l.h));
+    DC_BLAS(rocblas_set_stream(l.h, l.s));
+  }
+  const int nlanes = (int)buckets.size() - 1;
+  auto lane_stream = [&](int k) { return k == 0 ? b.stream : b.dc_lanes[k - 1].s; };
+  auto lane_blas = [&](int k) { return k == 0 ? b.blas : b.dc_lanes[k - 1].h; };
+  auto fork = [&]() -> bool {  // the lanes start after the work queued so far on the batch stream
+    HIP_OK(hipEventRecord(b.dc_fork, b.stream));
+    for (int k = 0; k < nlanes; ++k) HIP_OK(hipStreamWaitEvent(b.dc_lanes[k].s, b.dc_fork, 0));
+    return true;
+  };
+  auto join = [&]() -> bool {  // the batch stream continues after every lane's work
+    for (int k = 0; k < nlanes; ++k) HIP_OK(hipEventRecord(b.dc_lanes[k].done, b.dc_lanes[k].s));
+    for (int k = 0; k < nlanes; ++k) HIP_OK(hipStreamWaitEvent(b.stream, b.dc_lanes[k].done, 0));
+    return true;
+  };
   // one spin channel: subsystem eigenpairs, the chemical potential, the assembled density
   auto solve_channel = [&](const double* F, double target, double* dst, double& mu_out) -> bool {
     hipLaunchKernelGGL(lg_gershgorin_kernel, dim3(LG_BLOCKS), dim3(LG_THREADS), 0, b.stream, F, n, np, b.lg.red);
@@ -552,15 +594,23 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
                  hipMemcpyDeviceToHost));
     double ghi = -1.0e300;
     for (int blk = 0; blk < LG_BLOCKS; ++blk) ghi = std::max(ghi, b.h_red[(size_t)blk * LG_MAXDOT + 1]);
-    for (const Bucket& k : buckets) {
+    if (!fork()) return false;
+    for (int bk = 0; bk < (int)buckets.size(); ++bk) {
+      const Bucket& k = buckets[bk];
       const long long kper = (long long)k.nb * k.nb;
-      hipLaunchKernelGGL(dc::gather_kernel, G, T, 0, b.stream, F, np, (const int*)(dorb + (long long)k.first * nmax),
-                         (const int*)(dsz + k.first), k.count, k.nb, nmax, ghi + 1.0, Fs + k.off);
+      hipLaunchKernelGGL(dc::gather_kernel, G, T, 0, lane_stream(bk), F, np,
+                         (const int*)(dorb + (long long)k.first * nmax), (const int*)(dsz + k.first), k.count, k.nb,
+                         nmax, ghi + 1.0, Fs + k.off);
       HIP_OK(hipGetLastError());
-      BLAS_OK(rocsolver_dsyevd_strided_batched(b.blas, rocblas_evect_original, rocblas_fill_upper, k.nb, Fs + k.off,
-                                               k.nb, kper, Ws + (long long)k.first * nmax, nmax,
+      BLAS_OK(rocsolver_dsyevd_strided_batched(lane_blas(bk), rocblas_evect_original, rocblas_fill_upper, k.nb,
+                                               Fs + k.off, k.nb, kper, Ws + (long long)k.first * nmax, nmax,
                                                Ew + (long long)k.first * nmax, nmax, dinfo + k.first, k.count));
+      hipLaunchKernelGGL(dc::core_weight_kernel, G, T, 0, lane_stream(bk), (const double*)(Fs + k.off),
+                         (const int*)(dsz + k.first), (const char*)(dcore + (long long)k.first * nmax), k.count, k.nb,
+                         nmax, Qw + (long long)k.first * nmax);
+      HIP_OK(hipGetLastError());
     }
+    if (!join()) return false;
     std::vector<int> info(nsub);
     HIP_OK(scopy(b.stream, info.data(), dinfo, sizeof(int) * nsub, hipMemcpyDeviceToHost));
     for (int t = 0; t < nsub; ++t)
@@ -568,25 +618,24 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
         err = "rocsolver_dsyevd_strided_batched did not converge on subsystem " + std::to_string(order[t]);
         return false;
       }
-    for (const Bucket& k : buckets)
-      hipLaunchKernelGGL(dc::core_weight_kernel, G, T, 0, b.stream, (const double*)(Fs + k.off),
-                         (const int*)(dsz + k.first), (const char*)(dcore + (long long)k.first * nmax), k.count, k.nb,
-                         nmax, Qw + (long long)k.first * nmax);
     hipLaunchKernelGGL(dc::mu_kernel, dim3(1), dim3(dc::MU_THREADS), 0, b.stream, (const double*)Ws, (const double*)Qw,
                        (const int*)dsz, nsub, nmax, target, dco.smearing_ev, dco.electron_tol, mu);
     HIP_OK(hipGetLastError());
     // C_s = Y_s V_s^T (column-major nb x nb per subsystem, the k sum over every column)
     const double one = 1.0, zero = 0.0;
-    for (const Bucket& k : buckets) {
+    if (!fork()) return false;
+    for (int bk = 0; bk < (int)buckets.size(); ++bk) {
+      const Bucket& k = buckets[bk];
       const long long kper = (long long)k.nb * k.nb;
-      hipLaunchKernelGGL(dc::occupy_kernel, G, T, 0, b.stream, (const double*)(Fs + k.off),
+      hipLaunchKernelGGL(dc::occupy_kernel, G, T, 0, lane_stream(bk), (const double*)(Fs + k.off),
                          (const double*)(Ws + (long long)k.first * nmax), (const int*)(dsz + k.first), k.count, k.nb,
                          nmax, (const double*)mu, dco.smearing_ev, Ys + k.off);
       HIP_OK(hipGetLastError());
-      BLAS_OK(rocblas_dgemm_strided_batched(b.blas, rocblas_operation_none, rocblas_operation_transpose, k.nb, k.nb,
-                                            k.nb, &one, Ys + k.off, k.nb, kper, Fs + k.off, k.nb, kper, &zero,
+      BLAS_OK(rocblas_dgemm_strided_batched(lane_blas(bk), rocblas_operation_none, rocblas_operation_transpose, k.nb,
+                                            k.nb, k.nb, &one, Ys + k.off, k.nb, kper, Fs + k.off, k.nb, kper, &zero,
                                             Cs + k.off, k.nb, kper, k.count));
     }
+    if (!join()) return false;
     HIP_OK(hipMemsetAsync(dst, 0, sizeof(double) * m2, b.stream));
     for (int s = 0; s < nsub; ++s) {  // the partition's order
       const int t = slot_of[s];
@@ -704,7 +753,7 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
   if (!converged) return 3;
   // the Fock matrices at the converged density and the energy there (src/dc/scf.rs:554-572, :110-122)
   hipLaunchKernelGGL(dc::add_kernel, G, T, 0, b.stream, (const double*)Pa, (const double*)Pb, Pt, m2);
-  HIP_OK(hipGetLastError());
+  DC_HIP(hipGetLastError());
   if (!fock(Fa, Pa)) return 6;
   if (uhf && !fock(Fb, Pb)) return 6;
   const Mol M = mol_of(b.args, d, nullptr);
@@ -724,8 +773,8 @@ int dc_run(const Params& P, const MolIn& mol, const EngineOptions& opt, const Dc
   out.fermi_beta_ev = mu_b;
   // densities back in the caller's AO order; charges from the total diagonal
   std::vector<double> ha(m2), hb(m2);
-  HIP_OK(scopy(b.stream, ha.data(), Pa, sizeof(double) * m2, hipMemcpyDeviceToHost));
-  HIP_OK(scopy(b.stream, hb.data(), Pb, sizeof(double) * m2, hipMemcpyDeviceToHost));
+  DC_HIP(scopy(b.stream, ha.data(), Pa, sizeof(double) * m2, hipMemcpyDeviceToHost));
+  DC_HIP(scopy(b.stream, hb.data(), Pb, sizeof(double) * m2, hipMemcpyDeviceToHost));
   out.density.assign((size_t)n * n, 0.0);
   out.spin_density.assign((size_t)n * n, 0.0);
   for (int mu2 = 0; mu2 < n; ++mu2)

@@ -5028,6 +5028,15 @@ struct Batch {
   std::vector<int> order_all;  // queue order of the pair kernels: solve (largest first), then large
   LargeBufs lg;
   rocblas_handle blas = nullptr;
+  // divide-and-conquer: a stream, solver handle and completion event for every size bucket after
+  // the first (which runs on stream and blas), so the buckets' batched eigensolves overlap
+  struct DcLane {
+    hipStream_t s = nullptr;
+    rocblas_handle h = nullptr;
+    hipEvent_t done = nullptr;
+  };
+  std::vector<DcLane> dc_lanes;
+  hipEvent_t dc_fork = nullptr;
   EngineOptions opt{};
   TableConst table{};
   std::vector<double> h_red;
@@ -5079,6 +5088,13 @@ struct Batch {
     if (ev2) (void)hipEventDestroy(ev2);
     if (evg) (void)hipEventDestroy(evg);
     if (evm) (void)hipEventDestroy(evm);
+    for (DcLane& l : dc_lanes) {
+      if (l.s) (void)hipStreamSynchronize(l.s);
+      if (l.h) (void)rocblas_destroy_handle(l.h);
+      if (l.done) (void)hipEventDestroy(l.done);
+      if (l.s) (void)hipStreamDestroy(l.s);
+    }
+    if (dc_fork) (void)hipEventDestroy(dc_fork);
     if (blas) (void)rocblas_destroy_handle(blas);
     for (hipEvent_t e : lg.pev) (void)hipEventDestroy(e);
     for (hipEvent_t e : lg.fev) (void)hipEventDestroy(e);
